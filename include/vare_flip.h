@@ -1,0 +1,218 @@
+/* vare_flip.h -- C ABI of the MI355X (gfx950) FLIP step on the two-level sparse grid.
+ *
+ * One handle = one Bora::FLIPMethod instance (reference: base/header/FLIPMethod.h:13-126) plus the
+ * Particles it advances.  Plain C types only: no HIP, torch or C++ types cross this boundary.
+ * Pointers passed in are host pointers unless a `mem` argument says VARE_MEM_DEVICE.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (VARE_OK on success).
+ *   - The library owns all device state; the caller owns the buffers it passes.
+ *   - Thread-safe across handles, not within one handle.
+ *   - Inside the step everything is in voxel space (reference FLIPMethod.cu:191-200).
+ */
+#ifndef VARE_FLIP_H
+#define VARE_FLIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference has none: every FLIPMethod call returns void) ---- */
+#define VARE_OK        0
+#define VARE_EINVAL   -1   /* bad argument (null handle, zero-sized grid, bad block size ...) */
+#define VARE_ENOMEM   -2   /* device allocation failed or capacity exceeded */
+#define VARE_EBUDGET  -3   /* serving: request exceeds the HBM budget */
+#define VARE_ENOCONV  -4   /* solver hit its iteration cap without meeting its tolerance */
+#define VARE_EHIP     -5   /* a HIP runtime call failed */
+#define VARE_ENOTIMPL -6   /* feature outside the built path */
+#define VARE_ENONFINITE -7 /* NaN/Inf found in particle input */
+
+#define VARE_MEM_HOST   0
+#define VARE_MEM_DEVICE 1
+
+#define VARE_SPACE_WORLD 0
+#define VARE_SPACE_VOXEL 1
+
+/* solver modes */
+#define VARE_SOLVER_REFERENCE 0   /* CG, stop at |r|_2 < cg_abs_tol or after proj_iteration+2
+                                     iterations (LinearSystemSolver.cpp:13-50) */
+#define VARE_SOLVER_RELATIVE  1   /* PCG, stop at |r|_2 <= cg_rel_tol*|b|_2 or max_iterations */
+
+/* preconditioners for VARE_SOLVER_RELATIVE */
+#define VARE_PRECOND_NONE   0
+#define VARE_PRECOND_JACOBI 1
+
+/* Grid description.  Mirrors Bora::Grid (base/header/Grid.h:97-158): cell counts and the world
+ * bounding box [min, max].  block_size is the sparse-frame block edge: 4 is the reference's
+ * _block_res (SparseFrame.h:19), 8 is the performance layout. */
+typedef struct vare_grid_desc {
+    int64_t nx, ny, nz;
+    float min[3];
+    float max[3];
+    int block_size;
+} vare_grid_desc;
+
+/* Mirrors FLIPMethod::Params (FLIPMethod.h:51-67) plus the constants the reference hard-codes. */
+typedef struct vare_flip_params {
+    float ext_force[3];      /* world units, default (0,-9.80665,0) */
+    int max_substeps;        /* default 1 */
+    float dt;                /* frame dt, default 1/24 */
+    float wall_level;        /* world y; default -1e30 (open domain faces) */
+    int enable_water_tank;   /* default 0 */
+    int enable_curl;         /* default 0; 1 is rejected with VARE_ENOTIMPL */
+    float damping_band;      /* default 0 */
+    float voxel_size;        /* default 0.1 */
+    float col_vel_scale;     /* default 1 */
+    int pts_per_cell;        /* default 8 */
+    float vort_pres_rate;    /* default 0.2 (post-process only) */
+    float adjacency_under;   /* default 2 */
+    int proj_iteration;      /* default 200 (CG cap in reference mode is this + 2) */
+    int redist_iteration;    /* default 80 (post-process only) */
+    /* reference constants, made explicit */
+    float flip_ratio;        /* FLIPMethod.cu:99, 0.95 */
+    int solver_mode;         /* VARE_SOLVER_REFERENCE (default) | VARE_SOLVER_RELATIVE */
+    float cg_abs_tol;        /* LinearSystemSolver.cpp:39, 1e-3 */
+    float cg_rel_tol;        /* relative mode tolerance, default 1e-5 */
+    int max_iterations;      /* relative mode cap, default 2000 */
+    int preconditioner;      /* relative mode: VARE_PRECOND_* */
+    int reserved[8];
+} vare_flip_params;
+
+/* Frame geometry after SparseFrame::initialize (SparseFrame.cu:72-117): padded dims, block
+ * counts, and the voxel<->world transform (Grid.h:431-457). */
+typedef struct vare_frame_info {
+    int64_t nx, ny, nz;
+    int fx, fy, fz;
+    int block_size;
+    float translate[3];
+    float scale[3];
+    float voxel_wall_level;
+    float voxel_ext_force[3];
+} vare_frame_info;
+
+/* Per-step record (the reference returns nothing; FLIPMethod.cu:287 drops the CG count). */
+typedef struct vare_step_stats {
+    int64_t num_particles;       /* after removal */
+    int64_t num_removed;
+    int64_t num_blocks;          /* active blocks */
+    int64_t num_cells;           /* active cells = num_blocks * B^3 */
+    int64_t num_fluid;           /* fluid cells (rows of the Poisson system) */
+    int cg_iterations;           /* the value LinearSystemSolver::cg returns */
+    int converged;
+    double residual_norm;        /* recursive |r|_2 at exit */
+    double b_norm;               /* |b|_2 */
+    double step_ms;              /* device time of the whole step (HIP events) */
+    double solve_ms;             /* device time of the Poisson solve */
+} vare_step_stats;
+
+/* cell fields for vare_flip_get_cell_field (storage order = block-major, x fastest) */
+#define VARE_FIELD_MASS     0   /* float   (_sMassField) */
+#define VARE_FIELD_VELOCITY 1   /* float*3 (_sVeloField, pre-projection) */
+#define VARE_FIELD_COMP     2   /* float*3 (_sCompField, projected) */
+#define VARE_FIELD_MARK     3   /* uint64  (_sMarkField: 0 air, 1 solid, 4+fluid rank) */
+#define VARE_FIELD_PRESSURE 4   /* float   (_sPresField) */
+#define VARE_FIELD_COUNT    5   /* float   (_sSurfField after PointsToCountField) */
+#define VARE_FIELD_DIV      6   /* float   right-hand side b per cell (0 off the fluid set) */
+
+typedef struct vare_flip vare_flip;
+
+/* Defaults of FLIPMethod::Params plus the reference constants. */
+int vare_flip_params_default(vare_flip_params* p);
+
+/* Grid::initialize(voxelSize, min, max) (Grid.h:143-158): nx = (size_t)((max-min)/voxelSize). */
+int vare_grid_from_voxel_size(float voxel_size, const float min[3], const float max[3], int block_size,
+                              vare_grid_desc* out);
+
+/* FLIPMethod::initialize(const Grid&) (FLIPMethod.cu:16-47).  particle_capacity and
+ * block_capacity reserve device memory (0 = choose from the grid); both grow on demand
+ * outside of graph capture. */
+int vare_flip_create(const vare_grid_desc* grid, const vare_flip_params* params, size_t particle_capacity,
+                     size_t block_capacity, vare_flip** out);
+int vare_flip_destroy(vare_flip* h);
+int vare_flip_set_params(vare_flip* h, const vare_flip_params* params);
+int vare_flip_get_frame(const vare_flip* h, vare_frame_info* out);
+
+/* FLIPMethod::newParticle (FLIPMethod.h:75-99), batched: appends n particles with zero
+ * vorticity, curvature and adjacency.  pos/vel are xyz-interleaved float[3n]. */
+int vare_flip_add_particles(vare_flip* h, const float* pos, const float* vel, size_t n, int space, int mem);
+/* Replace the whole particle set (checkpoint resume).  vort/curv/adj may be NULL (zeros). */
+int vare_flip_set_particles(vare_flip* h, const float* pos, const float* vel, const float* vort,
+                            const float* curv, const float* adj, size_t n, int space, int mem);
+/* Read particles back in the reference's order (Array::remove order, Array.h:995-1024).
+ * Any output pointer may be NULL. */
+int vare_flip_get_particles(vare_flip* h, float* pos, float* vel, float* adj, float* vort, float* curv,
+                            size_t cap, size_t* n, int space, int mem);
+int vare_flip_num_particles(const vare_flip* h, size_t* n);
+
+/* FLIPMethod::particlesToVoxel / particlesToWorld (FLIPMethod.cu:480-514). */
+int vare_flip_particles_to_voxel(vare_flip* h);
+int vare_flip_particles_to_world(vare_flip* h);
+
+/* FLIPMethod::advanceOneStep(pts, dt) (FLIPMethod.cu:208-298), voxel space, including
+ * prepareNextStep (:49-52). */
+int vare_flip_step(vare_flip* h, float dt);
+/* FLIPMethod::advanceOneFrame (FLIPMethod.cu:177-206) minus postProcess (surfacing, curl,
+ * vorticity and curvature are outside the built path): toVoxel, step(params.dt), toWorld. */
+int vare_flip_advance_frame(vare_flip* h);
+
+/* Stages of the step, each equal to the named part of FLIPMethod::advanceOneStep:
+ *   0 prepareNextStep / SparseFrame::buildFromPoints   (SparseFrame.cu:127-199)
+ *   1 P2G + count field + cell classification          (Rasterization.cu:121-230, FLIPMethod.cu:242-280)
+ *   2 FluidProjection::buildLinearSystem (divergence)  (FluidProjection.cu:166-274)
+ *   3 FluidProjection::solve (CG + pressure apply)     (FluidProjection.cu:277-381)
+ *   4 FLIPMethod::updateParticles                      (FLIPMethod.cu:54-175)
+ *   5 Array::remove on the five particle arrays         (FLIPMethod.cu:293-297)
+ * Running 0..5 in order with the same dt equals vare_flip_step. */
+int vare_flip_stage(vare_flip* h, int stage, float dt);
+
+int vare_flip_get_stats(const vare_flip* h, vare_step_stats* out);
+
+/* Debug/parity accessors. pointers[fx*fy*fz] receives the active slot start (cell offset) or -1
+ * (SparseFrame::_pointers with NULL_MAX as -1); coords[3*nb] the active block coordinates in
+ * ascending linear order (SparseFrame::_coords). */
+int vare_flip_get_block_table(vare_flip* h, int64_t* pointers, int32_t* coords, size_t cap, size_t* nb);
+/* Copy a cell field (VARE_FIELD_*) in storage order; n = number of cells. */
+int vare_flip_get_cell_field(vare_flip* h, int which, void* out, size_t cap_bytes, size_t* n);
+/* Per-particle removal flags of the last step, in the order the particles had before removal. */
+int vare_flip_get_removed_mask(vare_flip* h, uint8_t* out, size_t cap, size_t* n);
+
+/* Seed a box of water in voxel space (the dam-break input of the benchmark): for every cell
+ * whose centre is inside [0,wx)x[0,wy)x[0,wz), ppc points at cell + Rand(s+3q+a), a=0,1,2,
+ * q = ppc*(i + nx*(j + ny*k)) + n (Rand = Random.h:18-27); only points inside the box are kept.
+ * pos may be NULL to count.  Host-side generator. */
+int vare_seed_box(int64_t nx, int64_t ny, int64_t nz, float wx, float wy, float wz, int ppc, uint32_t seed,
+                  float* pos, size_t cap, size_t* n);
+/* Rand(seed) of Random.h:18-27 (for checking the generator). */
+float vare_rand(uint32_t seed);
+
+/* Device helpers for harnesses. */
+int vare_device_synchronize(void);
+int vare_device_name(char* buf, size_t cap);
+
+/* Kernel timing.  `mask` is a bitmask of kernel classes (1 << VARE_KERNEL_*); each launch of a
+ * selected class is bracketed by HIP events on the handle's stream (the stream the kernel runs
+ * on) and the durations are accumulated until reset.  0 disables. */
+#define VARE_KERNEL_SPMV     0   /* CG: A*p + p.Ap */
+#define VARE_KERNEL_UPDATE   1   /* CG: x,r update + r.r */
+#define VARE_KERNEL_P2G      2
+#define VARE_KERNEL_G2P      3
+#define VARE_KERNEL_BIN      4
+#define VARE_KERNEL_DIV      5
+#define VARE_KERNEL_APPLY    6
+#define VARE_KERNEL_PUPDATE  7   /* CG: p = r + beta p */
+#define VARE_KERNEL_COUNT    8
+int vare_flip_set_kernel_timing(vare_flip* h, int mask);
+int vare_flip_get_kernel_timing(vare_flip* h, double ms_total[VARE_KERNEL_COUNT], int64_t launches[VARE_KERNEL_COUNT],
+                                int reset);
+
+/* Last error message of the calling thread. */
+const char* vare_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VARE_FLIP_H */

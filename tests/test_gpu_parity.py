@@ -1,0 +1,263 @@
+"""GPU parity of the HIP step against the CPU oracle, stage by stage, through the C ABI.
+
+Bars (stated per check):
+  * integer / index / byte outputs -- block table, count field, cell marks and fluid numbering,
+    adjacency, removal mask and permutation: bit-exact vs the FP32 oracle on the same input;
+  * floating outputs: relative L2 vs the FP32 oracle of 1e-6 (P2G), 1e-5 (divergence, particles),
+    and for the pressure solve the GPU's error against the FP64 oracle must stay within
+    max(2 x the FP32 oracle's own error, 1e-5).  Differences come from float-atomic P2G summation
+    order and the dot-product reduction order.
+"""
+import numpy as np
+import pytest
+
+from vare_amd import _lib as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 240.0
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def make_scene(n=32, wx=0.4, wy=0.6, ppc=8, seed=1, vel_scale=20.0):
+    pos = O.seed_box(n, n, n, wx * n, wy * n, n, ppc, seed)
+    rng = np.random.default_rng(seed)
+    vel = (rng.standard_normal(pos.shape) * vel_scale).astype(np.float32)
+    return pos, vel
+
+
+def pair(n, B, pos, vel, vort=None, **kw):
+    base = dict(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n)
+    base.update(kw)
+    g = L.grid_desc(n, n, n, (0, 0, 0), (1, 1, 1), B)
+    h = L.FlipHandle(g, L.default_params(**base))
+    h.set_particles(pos, vel, vort=vort)
+    o32 = O.Oracle((n, n, n), B=B, prm=O.params(**base), fp64=False)
+    o64 = O.Oracle((n, n, n), B=B, prm=O.params(**base), fp64=True)
+    for o in (o32, o64):
+        o.set_particles(pos, vel, vort=vort)
+    return h, o32, o64
+
+
+@pytest.mark.parametrize("B", [4, 8])
+def test_stagewise_parity(B):
+    n = 32
+    pos, vel = make_scene(n)
+    h, o32, o64 = pair(n, B, pos, vel, solver_mode=1, cg_rel_tol=1e-6, max_iterations=3000)
+    print("case stagewise B=%d n=%d np=%d" % (B, n, pos.shape[0]), flush=True)
+
+    # stage 0: binning + block activation -- bit-exact
+    h.stage(0)
+    for o in (o32, o64):
+        o.stage(0)
+    gp, gc = h.block_table()
+    op, oc = o32.block_table()
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gc, oc)
+
+    # stage 1: P2G + count + classification
+    h.stage(1)
+    for o in (o32, o64):
+        o.stage(1)
+    np.testing.assert_array_equal(h.cell_field(L.FIELD_COUNT), o32.field(O.FIELD_COUNT))
+    assert rel(h.cell_field(L.FIELD_MASS), o32.field(O.FIELD_MASS)) < 1e-6
+    assert rel(h.cell_field(L.FIELD_VELOCITY), o32.field(O.FIELD_VELO)) < 1e-6
+    assert np.all((h.cell_field(L.FIELD_MASS) > 0) == (o32.field(O.FIELD_MASS) > 0))
+
+    # stage 2: fluid numbering (bit-exact) + divergence
+    h.stage(2)
+    for o in (o32, o64):
+        o.stage(2)
+    np.testing.assert_array_equal(h.cell_field(L.FIELD_MARK), o32.field(O.FIELD_MARK).astype(np.uint64))
+    fluid = o32.field(O.FIELD_FLUID).astype(np.int64)
+    b_gpu = h.cell_field(L.FIELD_DIV)
+    assert rel(b_gpu[fluid], o32.field(O.FIELD_B)) < 1e-5
+    nonfluid = np.ones(b_gpu.size, bool)
+    nonfluid[fluid] = False
+    assert np.all(b_gpu[nonfluid] == 0)
+    assert h.stats()["num_fluid"] == fluid.size
+
+    # stage 3: solve + pressure apply
+    h.stage(3)
+    for o in (o32, o64):
+        o.stage(3)
+    st = h.stats()
+    e_gpu = rel(h.cell_field(L.FIELD_PRESSURE), o64.field(O.FIELD_PRES))
+    e_o32 = rel(o32.field(O.FIELD_PRES), o64.field(O.FIELD_PRES))
+    print("pressure err gpu=%.3g oracle32=%.3g iters gpu=%d o32=%d o64=%d" % (
+        e_gpu, e_o32, st["cg_iterations"], o32.stats()["cg_iters"], o64.stats()["cg_iters"]), flush=True)
+    assert st["converged"] == 1
+    assert abs(st["cg_iterations"] - o64.stats()["cg_iters"]) <= max(3, 0.05 * o64.stats()["cg_iters"])
+    assert e_gpu < max(2 * e_o32, 1e-5)
+    c_gpu = rel(h.cell_field(L.FIELD_COMP), o64.field(O.FIELD_COMP))
+    c_o32 = rel(o32.field(O.FIELD_COMP), o64.field(O.FIELD_COMP))
+    assert c_gpu < max(2 * c_o32, 1e-5)
+
+    # stage 4: G2P + advection
+    h.stage(4, DT)
+    for o in (o32, o64):
+        o.stage(4, DT)
+    np.testing.assert_array_equal(h.removed_mask(), o32.removed())
+    g = h.get_particles()
+    r = o32.get_particles()
+    r64 = o64.get_particles()
+    np.testing.assert_array_equal(g["adj"], r["adj"])
+    assert rel(g["pos"], r64["pos"]) < 1e-6
+    ev_gpu = rel(g["vel"], r64["vel"])
+    ev_o32 = rel(r["vel"], r64["vel"])
+    print("velocity err gpu=%.3g oracle32=%.3g" % (ev_gpu, ev_o32), flush=True)
+    assert ev_gpu < max(2 * ev_o32, 1e-5)
+
+    # stage 5: removal (none in a closed box)
+    h.stage(5)
+    for o in (o32, o64):
+        o.stage(5)
+    assert h.num_particles() == o32.num_particles() == pos.shape[0]
+
+
+@pytest.mark.parametrize("B", [4, 8])
+def test_reference_mode_step(B):
+    # reference solver: unpreconditioned CG, |r| < 1e-3 absolute or 202 products
+    n = 32
+    pos, vel = make_scene(n, seed=2)
+    h, o32, o64 = pair(n, B, pos, vel)
+    h.step(DT)
+    it32 = o32.step(DT)
+    o64.step(DT)
+    st = h.stats()
+    print("reference mode B=%d iters gpu=%d o32=%d" % (B, st["cg_iterations"], it32), flush=True)
+    assert abs(st["cg_iterations"] - it32) <= 3
+    g = h.get_particles()
+    r = o32.get_particles()
+    r64 = o64.get_particles()
+    np.testing.assert_array_equal(g["adj"], r["adj"])
+    assert rel(g["vel"], r64["vel"]) < max(2 * rel(r["vel"], r64["vel"]), 1e-5)
+
+
+def test_removal_order_bit_exact():
+    # open domain (wallLevel = -1e30: no clamping, FLIPMethod.cu:135) with particles falling out
+    # of the bottom: removal flags and the Array::remove permutation must match exactly.
+    n = 32
+    pos, vel = make_scene(n, wx=0.5, wy=0.3, seed=3, vel_scale=5.0)
+    low = pos[:, 1] < 2.0
+    vel[low, 1] = -900.0                            # leaves through y < 0 within one step
+    vort = np.arange(pos.shape[0], dtype=np.float32)  # rides along: identifies each particle
+    h, o32, _ = pair(n, 4, pos, vel, vort=vort, wall_level=-1e30)
+    h.step(DT)
+    o32.step(DT)
+    mask_g = h.removed_mask()
+    mask_o = o32.removed()
+    np.testing.assert_array_equal(mask_g, mask_o)
+    assert mask_o.sum() > 100
+    g = h.get_particles()
+    r = o32.get_particles()
+    assert g["vort"].shape[0] == pos.shape[0] - int(mask_o.sum())
+    np.testing.assert_array_equal(g["vort"].astype(np.int64), r["vort"].astype(np.int64))
+    np.testing.assert_array_equal(g["vort"].astype(np.int64), O.remove_perm(mask_o))
+
+
+def test_multistep_aggregates():
+    n = 32
+    pos, _ = make_scene(n, vel_scale=0.0)
+    vel = np.zeros_like(pos)
+    h, o32, _ = pair(n, 4, pos, vel)
+    for _ in range(10):
+        h.step(DT)
+        o32.step(DT)
+    g = h.get_particles()
+    r = o32.get_particles()
+    assert g["pos"].shape == r["pos"].shape
+    ke_g = float((g["vel"].astype(np.float64) ** 2).sum())
+    ke_r = float((r["vel"] ** 2).sum())
+    assert abs(ke_g - ke_r) / ke_r < 1e-3
+    np.testing.assert_allclose(g["pos"].mean(0), r["pos"].mean(0), rtol=1e-4)
+    assert abs(g["pos"][:, 0].max() - r["pos"][:, 0].max()) < 1e-2
+
+
+def test_block_size_invariance():
+    # B=4 and B=8 give the same fluid set and physics up to summation order
+    n = 32
+    pos, vel = make_scene(n, seed=4)
+    out = {}
+    for B in (4, 8):
+        g = L.grid_desc(n, n, n, (0, 0, 0), (1, 1, 1), B)
+        h = L.FlipHandle(g, L.default_params(ext_force=(0, -9.8, 0), wall_level=1e30, voxel_size=1 / n,
+                                             solver_mode=1, cg_rel_tol=1e-6, max_iterations=3000))
+        h.set_particles(pos, vel)
+        h.step(DT)
+        out[B] = (h.get_particles(), h.stats())
+    assert out[4][1]["num_fluid"] == out[8][1]["num_fluid"]
+    np.testing.assert_array_equal(out[4][0]["adj"], out[8][0]["adj"])
+    assert rel(out[4][0]["vel"], out[8][0]["vel"]) < 1e-4
+
+
+def test_world_frame_api_matches_oracle():
+    # advanceOneFrame in world space: toVoxel, step(params.dt), toWorld (FLIPMethod.cu:177-206)
+    from vare_amd import FLIPMethod, Grid, Particles
+    n = 24
+    pos_v, vel_v = make_scene(n, seed=5, vel_scale=3.0)
+    fm = FLIPMethod(block_size=4)
+    fm.params.extForce = (0.0, -9.8, 0.0)
+    fm.params.wallLevel = 10.0
+    fm.params.voxelSize = 1.0 / n
+    fm.params.dt = DT
+    fm.initialize(Grid(n, n, n, (0, 0, 0), (1, 1, 1)))
+    pts = Particles(position=(pos_v / n).astype(np.float32), velocity=(vel_v / n).astype(np.float32),
+                    vorticity=np.zeros(pos_v.shape[0], np.float32), curvature=np.zeros(pos_v.shape[0], np.float32),
+                    adjacency=np.zeros(pos_v.shape[0], np.float32))
+    fm.advanceOneFrame(pts)
+    o = O.Oracle((n, n, n), B=4, prm=O.params(ext_force=(0, -9.8, 0), wall_level=10.0, voxel_size=1 / n))
+    # the oracle steps in voxel space: apply Grid::voxelPoint/voxelVector (Grid.h:445-456) in FP32
+    s = np.float32(1.0) / np.float32(n)
+    o.set_particles((pos_v / n).astype(np.float32) / s, (vel_v / n).astype(np.float32) / s)
+    o.step(DT)
+    r = o.get_particles()
+    assert pts.position.shape[0] == r["pos"].shape[0]
+    assert rel(pts.position, r["pos"] / n) < 1e-5
+    assert rel(pts.velocity, r["vel"] / n) < 1e-4
+
+
+def test_empty_and_error_paths():
+    n = 16
+    g = L.grid_desc(n, n, n, block_size=4)
+    h = L.FlipHandle(g, L.default_params(voxel_size=1 / n))
+    h.step(DT)                                                    # zero particles
+    assert h.stats()["num_particles"] == 0
+    bad = np.array([[1.0, np.nan, 2.0]], np.float32)
+    with pytest.raises(L.VareError) as e:
+        h.set_particles(bad, np.zeros_like(bad))
+    assert e.value.code == L.VARE_ENONFINITE
+    pos, vel = make_scene(n, seed=6)
+    h2 = L.FlipHandle(g, L.default_params(voxel_size=1 / n, ext_force=(0, -9.8, 0), wall_level=1e30,
+                                          solver_mode=1, cg_rel_tol=1e-9, max_iterations=2))
+    h2.set_particles(pos, vel)
+    rc = h2.step_rc(DT)
+    assert rc == L.VARE_ENOCONV
+    assert h2.stats()["cg_iterations"] == 2
+
+
+def test_mid_size_properties():
+    # 128^3 dam break (B=8, relative PCG): size-independent invariants of one step
+    n = 128
+    pos = L.seed_box(n, n, n, 0.5 * n, 0.5 * n, n, 8, 1)
+    g = L.grid_desc(n, n, n, block_size=8)
+    h = L.FlipHandle(g, L.default_params(ext_force=(0, -9.8, 0), wall_level=1e30, voxel_size=1 / n,
+                                         solver_mode=1, cg_rel_tol=1e-5, max_iterations=4000))
+    h.set_particles(pos, np.zeros_like(pos))
+    h.step(DT)
+    h.step(DT)
+    st = h.stats()
+    assert st["converged"] == 1 and st["residual_norm"] <= 1e-5 * st["b_norm"] * 1.0001
+    assert st["num_particles"] == pos.shape[0] and st["num_removed"] == 0
+    d = h.get_particles()
+    assert np.all(d["pos"] >= 0) and np.all(d["pos"] <= n)
+    assert np.all(np.isfinite(d["vel"]))
+    cnt = h.cell_field(L.FIELD_COUNT)
+    assert cnt.sum() >= 8 * pos.shape[0] * 0.99 and np.all(cnt == np.round(cnt))

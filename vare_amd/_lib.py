@@ -1,0 +1,323 @@
+"""ctypes bindings of ``libvareflip.so`` (the C ABI declared in ``include/vare_flip.h``).
+
+The library must be built (``python -m vare_amd.build`` or ``__graft_entry__.build()``); loading
+fails loudly otherwise -- there is no CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libvareflip.so")
+
+VARE_OK = 0
+VARE_EINVAL = -1
+VARE_ENOMEM = -2
+VARE_EBUDGET = -3
+VARE_ENOCONV = -4
+VARE_EHIP = -5
+VARE_ENOTIMPL = -6
+VARE_ENONFINITE = -7
+
+MEM_HOST = 0
+MEM_DEVICE = 1
+SPACE_WORLD = 0
+SPACE_VOXEL = 1
+SOLVER_REFERENCE = 0
+SOLVER_RELATIVE = 1
+PRECOND_NONE = 0
+PRECOND_JACOBI = 1
+
+FIELD_MASS = 0
+FIELD_VELOCITY = 1
+FIELD_COMP = 2
+FIELD_MARK = 3
+FIELD_PRESSURE = 4
+FIELD_COUNT = 5
+FIELD_DIV = 6
+
+KERNEL_NAMES = ["spmv", "update", "p2g", "g2p", "bin", "div", "apply", "pupdate"]
+KERNEL_COUNT = 8
+
+ERROR_NAMES = {
+    VARE_EINVAL: "VARE_EINVAL",
+    VARE_ENOMEM: "VARE_ENOMEM",
+    VARE_EBUDGET: "VARE_EBUDGET",
+    VARE_ENOCONV: "VARE_ENOCONV",
+    VARE_EHIP: "VARE_EHIP",
+    VARE_ENOTIMPL: "VARE_ENOTIMPL",
+    VARE_ENONFINITE: "VARE_ENONFINITE",
+}
+
+
+class VareError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERROR_NAMES.get(code, str(code)), msg))
+        self.code = code
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+                ("min", C.c_float * 3), ("max", C.c_float * 3), ("block_size", C.c_int)]
+
+
+class FlipParams(C.Structure):
+    _fields_ = [("ext_force", C.c_float * 3), ("max_substeps", C.c_int), ("dt", C.c_float),
+                ("wall_level", C.c_float), ("enable_water_tank", C.c_int), ("enable_curl", C.c_int),
+                ("damping_band", C.c_float), ("voxel_size", C.c_float), ("col_vel_scale", C.c_float),
+                ("pts_per_cell", C.c_int), ("vort_pres_rate", C.c_float), ("adjacency_under", C.c_float),
+                ("proj_iteration", C.c_int), ("redist_iteration", C.c_int), ("flip_ratio", C.c_float),
+                ("solver_mode", C.c_int), ("cg_abs_tol", C.c_float), ("cg_rel_tol", C.c_float),
+                ("max_iterations", C.c_int), ("preconditioner", C.c_int), ("reserved", C.c_int * 8)]
+
+
+class FrameInfo(C.Structure):
+    _fields_ = [("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+                ("fx", C.c_int), ("fy", C.c_int), ("fz", C.c_int), ("block_size", C.c_int),
+                ("translate", C.c_float * 3), ("scale", C.c_float * 3), ("voxel_wall_level", C.c_float),
+                ("voxel_ext_force", C.c_float * 3)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("num_particles", C.c_int64), ("num_removed", C.c_int64), ("num_blocks", C.c_int64),
+                ("num_cells", C.c_int64), ("num_fluid", C.c_int64), ("cg_iterations", C.c_int),
+                ("converged", C.c_int), ("residual_norm", C.c_double), ("b_norm", C.c_double),
+                ("step_ms", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = C.c_void_p
+_FP = C.POINTER(C.c_float)
+
+# name -> (restype, argtypes); every symbol declared in include/vare_flip.h
+SIGNATURES = {
+    "vare_flip_params_default": (C.c_int, [C.POINTER(FlipParams)]),
+    "vare_grid_from_voxel_size": (C.c_int, [C.c_float, C.c_float * 3, C.c_float * 3, C.c_int, C.POINTER(GridDesc)]),
+    "vare_flip_create": (C.c_int, [C.POINTER(GridDesc), C.POINTER(FlipParams), C.c_size_t, C.c_size_t, C.POINTER(_P)]),
+    "vare_flip_destroy": (C.c_int, [_P]),
+    "vare_flip_set_params": (C.c_int, [_P, C.POINTER(FlipParams)]),
+    "vare_flip_get_frame": (C.c_int, [_P, C.POINTER(FrameInfo)]),
+    "vare_flip_add_particles": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_int, C.c_int]),
+    "vare_flip_set_particles": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_size_t, C.c_int, C.c_int]),
+    "vare_flip_get_particles": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), C.c_int, C.c_int]),
+    "vare_flip_num_particles": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
+    "vare_flip_particles_to_voxel": (C.c_int, [_P]),
+    "vare_flip_particles_to_world": (C.c_int, [_P]),
+    "vare_flip_step": (C.c_int, [_P, C.c_float]),
+    "vare_flip_advance_frame": (C.c_int, [_P]),
+    "vare_flip_stage": (C.c_int, [_P, C.c_int, C.c_float]),
+    "vare_flip_get_stats": (C.c_int, [_P, C.POINTER(StepStats)]),
+    "vare_flip_get_block_table": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vare_flip_get_cell_field": (C.c_int, [_P, C.c_int, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vare_flip_get_removed_mask": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vare_seed_box": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_int,
+                                C.c_uint32, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vare_rand": (C.c_float, [C.c_uint32]),
+    "vare_device_synchronize": (C.c_int, []),
+    "vare_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "vare_flip_set_kernel_timing": (C.c_int, [_P, C.c_int]),
+    "vare_flip_get_kernel_timing": (C.c_int, [_P, C.POINTER(C.c_double * KERNEL_COUNT),
+                                              C.POINTER(C.c_int64 * KERNEL_COUNT), C.c_int]),
+    "vare_last_error": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libvareflip.so not built at %s; run __graft_entry__.build()" % path)
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != VARE_OK:
+        raise VareError(rc, load().vare_last_error().decode())
+    return rc
+
+
+def ptr(a):
+    """ctypes void pointer of a contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_params(**kw) -> FlipParams:
+    p = FlipParams()
+    check(load().vare_flip_params_default(C.byref(p)))
+    for k, v in kw.items():
+        if k == "ext_force":
+            p.ext_force[:] = [float(x) for x in v]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def grid_desc(nx, ny, nz, bmin=(0.0, 0.0, 0.0), bmax=(1.0, 1.0, 1.0), block_size=4) -> GridDesc:
+    g = GridDesc()
+    g.nx, g.ny, g.nz = int(nx), int(ny), int(nz)
+    g.min[:] = [float(x) for x in bmin]
+    g.max[:] = [float(x) for x in bmax]
+    g.block_size = int(block_size)
+    return g
+
+
+def seed_box(nx, ny, nz, wx, wy, wz, ppc, seed=1):
+    """Dam-break particles in voxel space (see vare_seed_box)."""
+    lib = load()
+    n = C.c_size_t()
+    check(lib.vare_seed_box(nx, ny, nz, wx, wy, wz, ppc, seed, None, 0, C.byref(n)))
+    pos = np.empty((n.value, 3), np.float32)
+    check(lib.vare_seed_box(nx, ny, nz, wx, wy, wz, ppc, seed, ptr(pos), n.value, C.byref(n)))
+    return pos
+
+
+class FlipHandle:
+    """Owning wrapper of one ``vare_flip`` handle."""
+
+    def __init__(self, grid: GridDesc, params: FlipParams, particle_capacity=0, block_capacity=0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        check(self.lib.vare_flip_create(C.byref(grid), C.byref(params), particle_capacity, block_capacity,
+                                        C.byref(self.h)))
+        self.grid = grid
+        self.params = params
+
+    def close(self):
+        if self.h:
+            self.lib.vare_flip_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: FlipParams):
+        check(self.lib.vare_flip_set_params(self.h, C.byref(params)))
+        self.params = params
+
+    def frame(self) -> FrameInfo:
+        f = FrameInfo()
+        check(self.lib.vare_flip_get_frame(self.h, C.byref(f)))
+        return f
+
+    def set_particles(self, pos, vel, vort=None, curv=None, adj=None, space=SPACE_VOXEL):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        n = pos.shape[0]
+        conv = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (vort, curv, adj)]
+        check(self.lib.vare_flip_set_particles(self.h, ptr(pos), ptr(vel), ptr(conv[0]), ptr(conv[1]), ptr(conv[2]),
+                                               n, space, MEM_HOST))
+
+    def add_particles(self, pos, vel, space=SPACE_VOXEL):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        check(self.lib.vare_flip_add_particles(self.h, ptr(pos), ptr(vel), pos.shape[0], space, MEM_HOST))
+
+    def num_particles(self) -> int:
+        n = C.c_size_t()
+        check(self.lib.vare_flip_num_particles(self.h, C.byref(n)))
+        return n.value
+
+    def get_particles(self, space=SPACE_VOXEL):
+        n = self.num_particles()
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        adj = np.empty(n, np.float32)
+        vort = np.empty(n, np.float32)
+        curv = np.empty(n, np.float32)
+        got = C.c_size_t()
+        check(self.lib.vare_flip_get_particles(self.h, ptr(pos), ptr(vel), ptr(adj), ptr(vort), ptr(curv), n,
+                                               C.byref(got), space, MEM_HOST))
+        return {"pos": pos, "vel": vel, "adj": adj, "vort": vort, "curv": curv}
+
+    def step(self, dt):
+        check(self.lib.vare_flip_step(self.h, float(dt)))
+
+    def step_rc(self, dt):
+        return self.lib.vare_flip_step(self.h, float(dt))
+
+    def stage(self, s, dt=0.0):
+        check(self.lib.vare_flip_stage(self.h, int(s), float(dt)))
+
+    def advance_frame(self):
+        check(self.lib.vare_flip_advance_frame(self.h))
+
+    def to_voxel(self):
+        check(self.lib.vare_flip_particles_to_voxel(self.h))
+
+    def to_world(self):
+        check(self.lib.vare_flip_particles_to_world(self.h))
+
+    def stats(self) -> dict:
+        s = StepStats()
+        check(self.lib.vare_flip_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def block_table(self):
+        f = self.frame()
+        nb = C.c_size_t()
+        check(self.lib.vare_flip_get_block_table(self.h, None, None, 0, C.byref(nb)))
+        pointers = np.empty(f.fx * f.fy * f.fz, np.int64)
+        coords = np.empty((nb.value, 3), np.int32)
+        check(self.lib.vare_flip_get_block_table(self.h, ptr(pointers), ptr(coords), nb.value, C.byref(nb)))
+        return pointers, coords
+
+    def cell_field(self, which):
+        n = C.c_size_t()
+        check(self.lib.vare_flip_get_cell_field(self.h, which, None, 0, C.byref(n)))
+        nc = n.value
+        if which in (FIELD_VELOCITY, FIELD_COMP):
+            out = np.empty((nc, 3), np.float32)
+        elif which == FIELD_MARK:
+            out = np.empty(nc, np.uint64)
+        else:
+            out = np.empty(nc, np.float32)
+        check(self.lib.vare_flip_get_cell_field(self.h, which, ptr(out), out.nbytes, C.byref(n)))
+        return out
+
+    def removed_mask(self):
+        n = C.c_size_t()
+        check(self.lib.vare_flip_get_removed_mask(self.h, None, 0, C.byref(n)))
+        out = np.empty(n.value, np.uint8)
+        check(self.lib.vare_flip_get_removed_mask(self.h, ptr(out), n.value, C.byref(n)))
+        return out
+
+    def set_kernel_timing(self, kernels=()):
+        mask = 0
+        for k in kernels:
+            mask |= 1 << KERNEL_NAMES.index(k)
+        check(self.lib.vare_flip_set_kernel_timing(self.h, mask))
+
+    def kernel_timing(self, reset=False):
+        ms = (C.c_double * KERNEL_COUNT)()
+        cnt = (C.c_int64 * KERNEL_COUNT)()
+        check(self.lib.vare_flip_get_kernel_timing(self.h, C.byref(ms), C.byref(cnt), 1 if reset else 0))
+        return {KERNEL_NAMES[i]: (ms[i], cnt[i]) for i in range(KERNEL_COUNT)}
+
+
+def device_synchronize():
+    check(load().vare_device_synchronize())
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(load().vare_device_name(buf, 256))
+    return buf.value.decode()

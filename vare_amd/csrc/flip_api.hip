@@ -1,0 +1,927 @@
+// flip_api.hip -- host runtime + C ABI (include/vare_flip.h) of the gfx950 FLIP step.
+//
+// One vare_flip handle owns its device state and one HIP stream (default priority).  The step is
+// the reference's FLIPMethod::advanceOneStep (base/source/FLIPMethod.cu:208-298) split into six
+// stages; each stage enqueues hand-written kernels from flip_kernels.h on the handle's stream.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/vare_flip.h"
+#include "flip_kernels.h"
+
+using namespace vare;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(VARE_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                        __FILE__, __LINE__);                                                 \
+    } while (0)
+
+#define CHK(expr)                       \
+    do {                                \
+        int rc_ = (expr);               \
+        if (rc_ != VARE_OK) return rc_; \
+    } while (0)
+
+inline unsigned nblocks(size_t n) { return (unsigned)((n + WG - 1) / WG); }
+
+template <class T> int dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(VARE_ENOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    }
+    return VARE_OK;
+}
+template <class T> void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct vare_flip {
+    // geometry (SparseFrame::initialize)
+    int B = 4;
+    int nx = 0, ny = 0, nz = 0;
+    int fx = 0, fy = 0, fz = 0;
+    int nblk = 0;
+    float t[3] = {0, 0, 0}, s[3] = {1, 1, 1};
+    vare_flip_params prm;
+    float vox_force[3] = {0, 0, 0};
+    float vox_wall = 0.f;
+    hipStream_t stream = nullptr;
+
+    // particles, SoA in reference order: px,py,pz,vx,vy,vz,adj,vort,curv
+    size_t np = 0, pcap = 0;
+    float* pa[9] = {};
+    uint8_t* removed = nullptr;
+    int* hole = nullptr;
+    int* src = nullptr;
+    size_t last_np_before_remove = 0;
+    int nremoved = 0;
+
+    // frame
+    int* flag = nullptr;
+    int* slot = nullptr;
+    int* coord = nullptr;     // capacity nblk (4 bytes per frame block)
+    int* nbr = nullptr;       // capacity bcap*6
+    size_t bcap = 0;
+    int nb = 0;
+
+    // cells, capacity bcap*B^3
+    float *mass = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *cnt = nullptr;
+    float *cx = nullptr, *cy = nullptr, *cz = nullptr;
+    float *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *b = nullptr;
+    uint16_t* info = nullptr;
+    uint8_t* type = nullptr;
+    int* rank = nullptr;
+    double* partials = nullptr;
+    double* partials2 = nullptr;
+
+    // scan scratch
+    int* tiles = nullptr;
+    size_t tiles_cap = 0;
+    int* dint = nullptr;          // [0] scan total, [1] removed count, [2] second scan total
+    CgState* st = nullptr;
+    CgState* st_host = nullptr;   // pinned mirror
+    int* dint_host = nullptr;     // pinned mirror of dint
+
+    // stats + timing
+    vare_step_stats stats;
+    int stage_next = 0;
+    unsigned timing = 0;          // bitmask of VARE_KERNEL_* classes bracketed by events
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kid;
+    size_t ev_used = 0;
+    double kms[VARE_KERNEL_COUNT] = {};
+    int64_t klaunch[VARE_KERNEL_COUNT] = {};
+    hipEvent_t step_ev[4] = {};
+
+    size_t ncells() const { return (size_t)nb * (size_t)B * B * B; }
+    FrameK frame() const { return FrameK{fx, fy, fz, nx, ny, nz, slot, coord, nbr}; }
+};
+
+namespace {
+
+void free_cells(vare_flip* h) {
+    dfree(h->nbr);
+    dfree(h->mass); dfree(h->vx); dfree(h->vy); dfree(h->vz); dfree(h->cnt);
+    dfree(h->cx); dfree(h->cy); dfree(h->cz);
+    dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b);
+    dfree(h->info); dfree(h->type); dfree(h->rank);
+    dfree(h->partials); dfree(h->partials2);
+    h->bcap = 0;
+}
+
+int ensure_blocks(vare_flip* h, size_t nb) {
+    if (nb <= h->bcap && h->mass) return VARE_OK;
+    size_t cap = std::max(nb, h->bcap + h->bcap / 4);
+    cap = std::min(cap, (size_t)h->nblk);
+    cap = std::max(cap, nb);
+    free_cells(h);
+    const size_t nc = cap * (size_t)h->B * h->B * h->B;
+    const size_t npart = (nc + WG - 1) / WG + 1;
+    CHK(dalloc(&h->nbr, cap * 6));
+    CHK(dalloc(&h->mass, nc)); CHK(dalloc(&h->vx, nc)); CHK(dalloc(&h->vy, nc)); CHK(dalloc(&h->vz, nc));
+    CHK(dalloc(&h->cnt, nc));
+    CHK(dalloc(&h->cx, nc)); CHK(dalloc(&h->cy, nc)); CHK(dalloc(&h->cz, nc));
+    CHK(dalloc(&h->x, nc)); CHK(dalloc(&h->r, nc)); CHK(dalloc(&h->p, nc)); CHK(dalloc(&h->ap, nc));
+    CHK(dalloc(&h->b, nc));
+    CHK(dalloc(&h->info, nc)); CHK(dalloc(&h->type, nc)); CHK(dalloc(&h->rank, nc));
+    CHK(dalloc(&h->partials, npart)); CHK(dalloc(&h->partials2, npart));
+    h->bcap = cap;
+    return VARE_OK;
+}
+
+int ensure_particles(vare_flip* h, size_t n) {
+    if (n <= h->pcap && h->pa[0]) return VARE_OK;
+    size_t cap = std::max(n, h->pcap + h->pcap / 2);
+    cap = std::max<size_t>(cap, 1024);
+    if (cap > (size_t)0x7fffffff) return fail(VARE_ENOMEM, "particle capacity %zu exceeds 2^31-1", cap);
+    float* na[9] = {};
+    for (int q = 0; q < 9; ++q) CHK(dalloc(&na[q], cap));
+    for (int q = 0; q < 9; ++q) {
+        if (h->np) HIPCHK(hipMemcpyAsync(na[q], h->pa[q], h->np * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int q = 0; q < 9; ++q) { dfree(h->pa[q]); h->pa[q] = na[q]; }
+    dfree(h->removed); dfree(h->hole); dfree(h->src);
+    CHK(dalloc(&h->removed, cap)); CHK(dalloc(&h->hole, cap)); CHK(dalloc(&h->src, cap));
+    h->pcap = cap;
+    return VARE_OK;
+}
+
+int ensure_tiles(vare_flip* h, size_t ntiles) {
+    if (ntiles <= h->tiles_cap && h->tiles) return VARE_OK;
+    dfree(h->tiles);
+    CHK(dalloc(&h->tiles, ntiles + 16));
+    h->tiles_cap = ntiles + 16;
+    return VARE_OK;
+}
+
+int fill_u32(vare_flip* h, void* p, uint32_t v, size_t n) {
+    if (n == 0) return VARE_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(n)), dim3(WG), 0, h->stream, (uint32_t*)p, v, n);
+    HIPCHK(hipGetLastError());
+    return VARE_OK;
+}
+
+// Three-phase scan; the total lands in dint[total_slot].
+template <class Op> int run_scan(vare_flip* h, const Op& op, int n, int total_slot) {
+    if (n <= 0) return fill_u32(h, h->dint + total_slot, 0, 1);
+    const size_t ntiles = ((size_t)n + SCAN_TILE - 1) / SCAN_TILE;
+    CHK(ensure_tiles(h, ntiles));
+    hipLaunchKernelGGL(k_scan_reduce<Op>, dim3((unsigned)ntiles), dim3(WG), 0, h->stream, op, n, h->tiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(WG), 0, h->stream, h->tiles, (int)ntiles, h->dint + total_slot);
+    hipLaunchKernelGGL(k_scan_apply<Op>, dim3((unsigned)ntiles), dim3(WG), 0, h->stream, op, n, (const int*)h->tiles);
+    HIPCHK(hipGetLastError());
+    return VARE_OK;
+}
+
+// ---- kernel timing (HIP events around launches of one class on the handle's stream) ----
+void tbegin(vare_flip* h, int kid) {
+    if (!(h->timing & (1u << kid))) return;
+    if (h->ev_used + 2 > h->ev_pool.size()) {
+        for (int q = 0; q < 64; ++q) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            h->ev_pool.push_back(e);
+            h->ev_kid.push_back(-1);
+        }
+    }
+    h->ev_kid[h->ev_used] = kid;
+    (void)hipEventRecord(h->ev_pool[h->ev_used], h->stream);
+    h->ev_used++;
+}
+void tend(vare_flip* h, int kid) {
+    if (!(h->timing & (1u << kid))) return;
+    if (h->ev_used == 0 || h->ev_kid[h->ev_used - 1] != kid) return;
+    h->ev_kid[h->ev_used] = -2;
+    (void)hipEventRecord(h->ev_pool[h->ev_used], h->stream);
+    h->ev_used++;
+}
+int tcollect(vare_flip* h) {
+    if (h->ev_used == 0) return VARE_OK;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (size_t q = 0; q + 1 < h->ev_used; q += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->ev_pool[q], h->ev_pool[q + 1]));
+        const int kid = h->ev_kid[q];
+        if (kid >= 0 && kid < VARE_KERNEL_COUNT) { h->kms[kid] += ms; h->klaunch[kid]++; }
+    }
+    h->ev_used = 0;
+    return VARE_OK;
+}
+
+bool finite3(const float* a, size_t n) {
+    for (size_t i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+
+G2PParams g2p_params(const vare_flip* h, float dt) {
+    G2PParams gp;
+    gp.dt = dt;
+    // FLIPMethod.cu:99 uses the literal pair 0.95f / 0.05f
+    gp.flip = h->prm.flip_ratio;
+    gp.pic = h->prm.flip_ratio == 0.95f ? 0.05f : 1.f - h->prm.flip_ratio;
+    gp.wall = h->vox_wall;
+    gp.gx = h->vox_force[0]; gp.gy = h->vox_force[1]; gp.gz = h->vox_force[2];
+    gp.max_substeps = h->prm.max_substeps;
+    gp.tank = h->prm.enable_water_tank;
+    gp.band = h->prm.damping_band / h->prm.voxel_size;
+    return gp;
+}
+
+__global__ void k_cg_setup(CgState* st, int mode, int max_count, float abs_tol, float rel_tol, int precond) {
+    if (threadIdx.x != 0) return;
+    st->mode = mode;
+    st->max_count = max_count;
+    st->abs_tol = abs_tol;
+    st->rel_tol = rel_tol;
+    st->precond = precond;
+    st->ticket = 0;
+    st->done = 0;
+    st->iter = 0;
+    st->count = 0;
+    st->converged = 0;
+}
+
+// ---- stages ----
+template <int B> int stage_prepare(vare_flip* h) {
+    CHK(fill_u32(h, h->flag, 0, (size_t)h->nblk));
+    tbegin(h, VARE_KERNEL_BIN);
+    if (h->np)
+        hipLaunchKernelGGL(k_bin_mark<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
+                           (int)h->np, h->fx, h->fy, h->fz, h->flag);
+    tend(h, VARE_KERNEL_BIN);
+    if (h->prm.enable_water_tank) {
+        const int tall = std::min(std::max((int)(h->vox_wall) / B, 0), h->fy);   // SparseFrame.cu:165
+        hipLaunchKernelGGL(k_bin_tank, dim3(nblocks(h->nblk)), dim3(WG), 0, h->stream, h->fx, h->fy, h->fz, tall, h->flag);
+    }
+    HIPCHK(hipGetLastError());
+    OpBlocks op{h->flag, h->slot, h->coord};
+    CHK(run_scan(h, op, h->nblk, 0));
+    HIPCHK(hipMemcpyAsync(h->dint_host, h->dint, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int nb = h->dint_host[0];
+    CHK(ensure_blocks(h, (size_t)nb));
+    h->nb = nb;
+    if (nb)
+        hipLaunchKernelGGL(k_block_nbr, dim3(nblocks((size_t)nb * 6)), dim3(WG), 0, h->stream, nb, h->fx, h->fy, h->fz,
+                           (const int*)h->coord, (const int*)h->slot, h->nbr);
+    HIPCHK(hipGetLastError());
+    return VARE_OK;
+}
+
+template <int B> int stage_p2g(vare_flip* h) {
+    const size_t nc = h->ncells();
+    CHK(fill_u32(h, h->mass, 0, nc)); CHK(fill_u32(h, h->vx, 0, nc)); CHK(fill_u32(h, h->vy, 0, nc));
+    CHK(fill_u32(h, h->vz, 0, nc)); CHK(fill_u32(h, h->cnt, 0, nc));
+    CellFields cf{h->mass, h->vx, h->vy, h->vz, h->cnt};
+    tbegin(h, VARE_KERNEL_P2G);
+    if (h->np)
+        hipLaunchKernelGGL(k_p2g_atomic<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
+                           h->pa[3], h->pa[4], h->pa[5], (int)h->np, h->frame(), cf);
+    tend(h, VARE_KERNEL_P2G);
+    if (nc)
+        hipLaunchKernelGGL(k_cell_finalize, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, cf, h->cx, h->cy, h->cz,
+                           h->type);
+    HIPCHK(hipGetLastError());
+    return VARE_OK;
+}
+
+CgVecs vecs(vare_flip* h) { return CgVecs{h->x, h->r, h->p, h->ap, h->b, h->info}; }
+
+template <int B> int stage_system(vare_flip* h) {
+    const size_t nc = h->ncells();
+    const int mode = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? 1 : 0;
+    const int precond = mode == 1 ? h->prm.preconditioner : 0;
+    const int maxc = mode == 0 ? h->prm.proj_iteration : h->prm.max_iterations;
+    hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(64), 0, h->stream, h->st, mode, maxc, h->prm.cg_abs_tol,
+                       h->prm.cg_rel_tol, precond);
+    const unsigned g = nblocks(std::max<size_t>(nc, 1));
+    tbegin(h, VARE_KERNEL_DIV);
+    hipLaunchKernelGGL(k_divergence<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const uint8_t*)h->type,
+                       (const float*)h->cx, (const float*)h->cy, (const float*)h->cz, h->vox_wall, vecs(h), h->partials,
+                       h->partials2, h->st);
+    tend(h, VARE_KERNEL_DIV);
+    if (mode == 1)
+        hipLaunchKernelGGL(k_cg_precond_init, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials, h->st);
+    HIPCHK(hipGetLastError());
+    return VARE_OK;
+}
+
+template <int B> int stage_solve(vare_flip* h) {
+    const size_t nc = h->ncells();
+    const unsigned g = nblocks(std::max<size_t>(nc, 1));
+    const int mode = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? 1 : 0;
+    const int cap = mode == 0 ? h->prm.proj_iteration + 2 : h->prm.max_iterations;
+    HIPCHK(hipEventRecord(h->step_ev[2], h->stream));
+    int launched = 0;
+    int chunk = 8;
+    for (;;) {
+        HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (h->st_host->done) break;
+        if (mode == 0 && launched >= h->st_host->loop_limit + 1) break;   // defensive: cannot be reached
+        if (launched > cap + 1) break;                                     // defensive: cannot be reached
+        for (int q = 0; q < chunk; ++q) {
+            tbegin(h, VARE_KERNEL_SPMV);
+            hipLaunchKernelGGL(k_cg_spmv<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), vecs(h), h->partials,
+                               h->st);
+            tend(h, VARE_KERNEL_SPMV);
+            tbegin(h, VARE_KERNEL_UPDATE);
+            hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials, h->partials2,
+                               h->st);
+            tend(h, VARE_KERNEL_UPDATE);
+            tbegin(h, VARE_KERNEL_PUPDATE);
+            hipLaunchKernelGGL(k_cg_pupdate, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), (const CgState*)h->st);
+            tend(h, VARE_KERNEL_PUPDATE);
+        }
+        HIPCHK(hipGetLastError());
+        launched += chunk;
+        chunk = std::min(chunk * 2, 64);
+    }
+    tbegin(h, VARE_KERNEL_APPLY);
+    if (nc) {
+        hipLaunchKernelGGL(k_press_extrap<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const uint8_t*)h->type,
+                           h->x);
+        hipLaunchKernelGGL(k_press_grad<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const uint8_t*)h->type,
+                           (const float*)h->x, h->vox_wall, h->cx, h->cy, h->cz);
+    }
+    tend(h, VARE_KERNEL_APPLY);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->step_ev[3], h->stream));
+    const CgState& s = *h->st_host;
+    h->stats.cg_iterations = s.count;
+    h->stats.converged = s.converged;
+    h->stats.residual_norm = std::sqrt(std::max(s.rr, 0.0));
+    h->stats.b_norm = std::sqrt(std::max(s.bb, 0.0));
+    h->stats.num_fluid = s.nfluid;
+    return VARE_OK;
+}
+
+template <int B> int stage_g2p(vare_flip* h, float dt) {
+    CHK(fill_u32(h, h->dint + 1, 0, 1));
+    tbegin(h, VARE_KERNEL_G2P);
+    if (h->np)
+        hipLaunchKernelGGL(k_g2p<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2], h->pa[3],
+                           h->pa[4], h->pa[5], h->pa[6], (int)h->np, h->frame(), (const float*)h->cnt, (const float*)h->cx,
+                           (const float*)h->cy, (const float*)h->cz, (const float*)h->vx, (const float*)h->vy,
+                           (const float*)h->vz, g2p_params(h, dt), h->removed, h->dint + 1);
+    tend(h, VARE_KERNEL_G2P);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h->dint_host + 1, h->dint + 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->nremoved = h->dint_host[1];
+    h->last_np_before_remove = h->np;
+    return VARE_OK;
+}
+
+int stage_remove(vare_flip* h) {
+    const int nrem = h->nremoved;
+    h->stats.num_removed = nrem;
+    if (nrem <= 0) return VARE_OK;
+    const int n = (int)h->np;
+    const int newn = n - nrem;
+    OpHoles oh{h->removed, h->hole};
+    CHK(run_scan(h, oh, newn, 0));
+    OpSurvivors os{h->removed, newn, h->src};
+    CHK(run_scan(h, os, n - newn, 2));
+    HIPCHK(hipMemcpyAsync(h->dint_host, h->dint, 3 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int nmove = h->dint_host[0];
+    if (nmove != h->dint_host[2]) return fail(VARE_EINVAL, "removal compaction mismatch (%d holes, %d survivors)", nmove, h->dint_host[2]);
+    PArrays pa;
+    for (int q = 0; q < 9; ++q) pa.a[q] = h->pa[q];
+    if (nmove)
+        hipLaunchKernelGGL(k_remove_move, dim3(nblocks(nmove)), dim3(WG), 0, h->stream, nmove, (const int*)h->hole,
+                           (const int*)h->src, pa);
+    HIPCHK(hipGetLastError());
+    h->np = (size_t)newn;
+    return VARE_OK;
+}
+
+template <int B> int run_stage(vare_flip* h, int stage, float dt) {
+    switch (stage) {
+        case 0: return stage_prepare<B>(h);
+        case 1: return stage_p2g<B>(h);
+        case 2: return stage_system<B>(h);
+        case 3: return stage_solve<B>(h);
+        case 4: return stage_g2p<B>(h, dt);
+        case 5: return stage_remove(h);
+    }
+    return fail(VARE_EINVAL, "bad stage %d", stage);
+}
+
+int dispatch_stage(vare_flip* h, int stage, float dt) {
+    if (stage != h->stage_next)
+        return fail(VARE_EINVAL, "stage %d called out of order (expected %d)", stage, h->stage_next);
+    int rc = h->B == 4 ? run_stage<4>(h, stage, dt) : run_stage<8>(h, stage, dt);
+    if (rc != VARE_OK) { h->stage_next = 0; return rc; }
+    h->stage_next = (stage + 1) % 6;
+    if (stage == 0) {
+        h->stats.num_blocks = h->nb;
+        h->stats.num_cells = (int64_t)h->ncells();
+    }
+    h->stats.num_particles = (int64_t)h->np;
+    return VARE_OK;
+}
+
+int validate_params(const vare_flip_params* p) {
+    if (p->enable_curl) return fail(VARE_ENOTIMPL, "enable_curl: curl noise is outside the built path");
+    if (p->solver_mode != VARE_SOLVER_REFERENCE && p->solver_mode != VARE_SOLVER_RELATIVE)
+        return fail(VARE_EINVAL, "bad solver_mode %d", p->solver_mode);
+    if (p->preconditioner != VARE_PRECOND_NONE && p->preconditioner != VARE_PRECOND_JACOBI)
+        return fail(VARE_EINVAL, "bad preconditioner %d", p->preconditioner);
+    if (p->voxel_size <= 0.f) return fail(VARE_EINVAL, "voxel_size must be positive");
+    if (p->max_iterations < 1 || p->proj_iteration < 0) return fail(VARE_EINVAL, "bad iteration cap");
+    return VARE_OK;
+}
+
+void apply_params(vare_flip* h, const vare_flip_params* p) {
+    h->prm = *p;
+    for (int a = 0; a < 3; ++a) h->vox_force[a] = p->ext_force[a] / h->s[a];   // FLIPMethod.cu:42
+    h->vox_wall = (p->wall_level - h->t[1]) / h->s[1];                        // FLIPMethod.cu:44-46
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vare_last_error(void) { return g_err.c_str(); }
+
+float vare_rand(uint32_t seed) { return rand01(seed); }
+
+int vare_flip_params_default(vare_flip_params* p) {
+    if (!p) return fail(VARE_EINVAL, "null params");
+    std::memset(p, 0, sizeof *p);
+    p->ext_force[0] = 0.f; p->ext_force[1] = -9.80665f; p->ext_force[2] = 0.f;   // FLIPMethod.h:53-66
+    p->max_substeps = 1;
+    p->dt = 1.f / 24.f;
+    p->wall_level = -1e+30f;
+    p->enable_water_tank = 0;
+    p->enable_curl = 0;
+    p->damping_band = 0.f;
+    p->voxel_size = 0.1f;
+    p->col_vel_scale = 1.f;
+    p->pts_per_cell = 8;
+    p->vort_pres_rate = 0.2f;
+    p->adjacency_under = 2.f;
+    p->proj_iteration = 200;
+    p->redist_iteration = 80;
+    p->flip_ratio = 0.95f;
+    p->solver_mode = VARE_SOLVER_REFERENCE;
+    p->cg_abs_tol = 1e-3f;
+    p->cg_rel_tol = 1e-5f;
+    p->max_iterations = 2000;
+    p->preconditioner = VARE_PRECOND_NONE;
+    return VARE_OK;
+}
+
+int vare_grid_from_voxel_size(float voxel, const float mn[3], const float mx[3], int block_size, vare_grid_desc* out) {
+    if (!out || !mn || !mx || !(voxel > 0.f)) return fail(VARE_EINVAL, "bad grid arguments");
+    out->nx = (int64_t)(size_t)((mx[0] - mn[0]) / voxel);     // Grid.h:145-147 (truncation)
+    out->ny = (int64_t)(size_t)((mx[1] - mn[1]) / voxel);
+    out->nz = (int64_t)(size_t)((mx[2] - mn[2]) / voxel);
+    for (int a = 0; a < 3; ++a) { out->min[a] = mn[a]; out->max[a] = mx[a]; }
+    out->block_size = block_size;
+    return VARE_OK;
+}
+
+int vare_flip_create(const vare_grid_desc* g, const vare_flip_params* prm, size_t pcap, size_t bcap, vare_flip** out) {
+    if (!g || !out) return fail(VARE_EINVAL, "null argument");
+    *out = nullptr;
+    if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0) return fail(VARE_EINVAL, "grid has zero cells");   // Grid.h:107
+    if (g->block_size != 4 && g->block_size != 8) return fail(VARE_EINVAL, "block_size must be 4 or 8");
+    vare_flip_params def;
+    vare_flip_params_default(&def);
+    const vare_flip_params* p = prm ? prm : &def;
+    CHK(validate_params(p));
+    vare_flip* h = new vare_flip();
+    std::memset(&h->stats, 0, sizeof h->stats);
+    const int B = g->block_size;
+    h->B = B;
+    // SparseFrame::initialize (SparseFrame.cu:72-117)
+    h->fx = (int)(g->nx / B + ((g->nx % B) ? 1 : 0));
+    h->fy = (int)(g->ny / B + ((g->ny % B) ? 1 : 0));
+    h->fz = (int)(g->nz / B + ((g->nz % B) ? 1 : 0));
+    const int64_t nblk = (int64_t)h->fx * h->fy * h->fz;
+    const int64_t ncell = nblk * B * B * B;
+    if (ncell > (int64_t)0x7fffffff) { delete h; return fail(VARE_EINVAL, "frame of %lld cells exceeds 2^31-1", (long long)ncell); }
+    h->nblk = (int)nblk;
+    h->nx = h->fx * B; h->ny = h->fy * B; h->nz = h->fz * B;
+    float cen[3], d[3];
+    for (int a = 0; a < 3; ++a) cen[a] = (g->min[a] + g->max[a]) * 0.5f;
+    d[0] = (g->max[0] - g->min[0]) / (float)g->nx;
+    d[1] = (g->max[1] - g->min[1]) / (float)g->ny;
+    d[2] = (g->max[2] - g->min[2]) / (float)g->nz;
+    const float hh = std::min(std::min(d[0], d[1]), d[2]);
+    const int n3[3] = {h->nx, h->ny, h->nz};
+    for (int a = 0; a < 3; ++a) {
+        const float l = (float)n3[a] * hh;
+        const float mn = cen[a] - l * 0.5f;
+        const float mx = mn + l;
+        float L = mx - mn;
+        if (a == 2 && n3[2] == 1) L = 1.f;     // Grid.h:131
+        h->t[a] = mn;
+        h->s[a] = L / (float)n3[a];
+    }
+    apply_params(h, p);
+    if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(VARE_EHIP, "hipStreamCreate failed"); }
+    int rc = VARE_OK;
+    if (rc == VARE_OK) rc = dalloc(&h->flag, (size_t)h->nblk);
+    if (rc == VARE_OK) rc = dalloc(&h->slot, (size_t)h->nblk);
+    if (rc == VARE_OK) rc = dalloc(&h->coord, (size_t)h->nblk);
+    if (rc == VARE_OK) rc = dalloc(&h->dint, 8);
+    if (rc == VARE_OK) rc = dalloc(&h->st, 1);
+    if (rc == VARE_OK && hipHostMalloc((void**)&h->st_host, sizeof(CgState), 0) != hipSuccess) rc = fail(VARE_EHIP, "hipHostMalloc failed");
+    if (rc == VARE_OK && hipHostMalloc((void**)&h->dint_host, 8 * sizeof(int), 0) != hipSuccess) rc = fail(VARE_EHIP, "hipHostMalloc failed");
+    for (int q = 0; q < 4 && rc == VARE_OK; ++q)
+        if (hipEventCreate(&h->step_ev[q]) != hipSuccess) rc = fail(VARE_EHIP, "hipEventCreate failed");
+    if (rc == VARE_OK && pcap) rc = ensure_particles(h, pcap);
+    if (rc == VARE_OK) {
+        size_t bc = bcap ? std::min(bcap, (size_t)h->nblk) : std::min((size_t)h->nblk, (size_t)((64u << 20) / (B * B * B)));
+        rc = ensure_blocks(h, std::max<size_t>(bc, 1));
+    }
+    if (rc == VARE_OK) {
+        rc = fill_u32(h, h->slot, 0xffffffffu, (size_t)h->nblk);
+        if (rc == VARE_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(VARE_EHIP, "init sync failed");
+    }
+    if (rc != VARE_OK) { vare_flip_destroy(h); return rc; }
+    *out = h;
+    return VARE_OK;
+}
+
+int vare_flip_destroy(vare_flip* h) {
+    if (!h) return VARE_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (int q = 0; q < 9; ++q) dfree(h->pa[q]);
+    dfree(h->removed); dfree(h->hole); dfree(h->src);
+    dfree(h->flag); dfree(h->slot); dfree(h->coord);
+    free_cells(h);
+    dfree(h->tiles); dfree(h->dint); dfree(h->st);
+    if (h->st_host) (void)hipHostFree(h->st_host);
+    if (h->dint_host) (void)hipHostFree(h->dint_host);
+    for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+    for (int q = 0; q < 4; ++q) if (h->step_ev[q]) (void)hipEventDestroy(h->step_ev[q]);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return VARE_OK;
+}
+
+int vare_flip_set_params(vare_flip* h, const vare_flip_params* p) {
+    if (!h || !p) return fail(VARE_EINVAL, "null argument");
+    CHK(validate_params(p));
+    apply_params(h, p);
+    return VARE_OK;
+}
+
+int vare_flip_get_frame(const vare_flip* h, vare_frame_info* o) {
+    if (!h || !o) return fail(VARE_EINVAL, "null argument");
+    o->nx = h->nx; o->ny = h->ny; o->nz = h->nz;
+    o->fx = h->fx; o->fy = h->fy; o->fz = h->fz;
+    o->block_size = h->B;
+    for (int a = 0; a < 3; ++a) { o->translate[a] = h->t[a]; o->scale[a] = h->s[a]; o->voxel_ext_force[a] = h->vox_force[a]; }
+    o->voxel_wall_level = h->vox_wall;
+    return VARE_OK;
+}
+
+static int upload_particles(vare_flip* h, size_t off, const float* pos, const float* vel, const float* vort,
+                            const float* curv, const float* adj, size_t n, int space, int mem) {
+    if (n == 0) return VARE_OK;
+    if (mem == VARE_MEM_HOST) {
+        if (!finite3(pos, n) || !finite3(vel, n)) return fail(VARE_ENONFINITE, "NaN/Inf in particle input");
+        std::vector<float> tmp(n);
+        for (int q = 0; q < 6; ++q) {
+            const float* in = q < 3 ? pos : vel;
+            const int a = q % 3;
+            for (size_t i = 0; i < n; ++i) tmp[i] = in[3 * i + a];
+            HIPCHK(hipMemcpy(h->pa[q] + off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        }
+        const float* sc[3] = {adj, vort, curv};
+        for (int q = 0; q < 3; ++q) {
+            if (sc[q]) HIPCHK(hipMemcpy(h->pa[6 + q] + off, sc[q], n * sizeof(float), hipMemcpyHostToDevice));
+            else CHK(fill_u32(h, h->pa[6 + q] + off, 0, n));
+        }
+    } else {
+        hipLaunchKernelGGL(k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, pos, h->pa[0] + off,
+                           h->pa[1] + off, h->pa[2] + off);
+        hipLaunchKernelGGL(k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, vel, h->pa[3] + off,
+                           h->pa[4] + off, h->pa[5] + off);
+        const float* sc[3] = {adj, vort, curv};
+        for (int q = 0; q < 3; ++q) {
+            if (sc[q]) HIPCHK(hipMemcpyAsync(h->pa[6 + q] + off, sc[q], n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+            else CHK(fill_u32(h, h->pa[6 + q] + off, 0, n));
+        }
+        HIPCHK(hipGetLastError());
+    }
+    if (space == VARE_SPACE_WORLD) {
+        PArrays pa;
+        for (int q = 0; q < 9; ++q) pa.a[q] = h->pa[q] + off;
+        hipLaunchKernelGGL(k_to_voxel, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, pa, h->t[0], h->t[1], h->t[2],
+                           h->s[0], h->s[1], h->s[2]);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return VARE_OK;
+}
+
+int vare_flip_add_particles(vare_flip* h, const float* pos, const float* vel, size_t n, int space, int mem) {
+    if (!h || (n && (!pos || !vel))) return fail(VARE_EINVAL, "null argument");
+    CHK(ensure_particles(h, h->np + n));
+    CHK(upload_particles(h, h->np, pos, vel, nullptr, nullptr, nullptr, n, space, mem));
+    h->np += n;
+    h->stage_next = 0;
+    return VARE_OK;
+}
+
+int vare_flip_set_particles(vare_flip* h, const float* pos, const float* vel, const float* vort, const float* curv,
+                            const float* adj, size_t n, int space, int mem) {
+    if (!h || (n && (!pos || !vel))) return fail(VARE_EINVAL, "null argument");
+    CHK(ensure_particles(h, n));
+    CHK(upload_particles(h, 0, pos, vel, vort, curv, adj, n, space, mem));
+    h->np = n;
+    h->stage_next = 0;
+    return VARE_OK;
+}
+
+int vare_flip_get_particles(vare_flip* h, float* pos, float* vel, float* adj, float* vort, float* curv, size_t cap,
+                            size_t* n, int space, int mem) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (n) *n = h->np;
+    if (cap < h->np) return fail(VARE_EINVAL, "capacity %zu < %zu particles", cap, h->np);
+    const size_t np = h->np;
+    if (np == 0) return VARE_OK;
+    float* tmp[6] = {};
+    if (space == VARE_SPACE_WORLD) {
+        for (int q = 0; q < 6; ++q) CHK(dalloc(&tmp[q], np));
+        for (int q = 0; q < 6; ++q)
+            HIPCHK(hipMemcpyAsync(tmp[q], h->pa[q], np * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+        PArrays pa;
+        for (int q = 0; q < 9; ++q) pa.a[q] = q < 6 ? tmp[q] : nullptr;
+        hipLaunchKernelGGL(k_to_world, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, pa, h->t[0], h->t[1], h->t[2],
+                           h->s[0], h->s[1], h->s[2]);
+    }
+    float* srcs[6];
+    for (int q = 0; q < 6; ++q) srcs[q] = space == VARE_SPACE_WORLD ? tmp[q] : h->pa[q];
+    int rc = VARE_OK;
+    if (mem == VARE_MEM_HOST) {
+        std::vector<float> buf(np);
+        for (int q = 0; q < 6 && rc == VARE_OK; ++q) {
+            float* outp = q < 3 ? pos : vel;
+            if (!outp) continue;
+            if (hipMemcpyAsync(buf.data(), srcs[q], np * sizeof(float), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess) { rc = fail(VARE_EHIP, "particle download failed"); break; }
+            const int a = q % 3;
+            for (size_t i = 0; i < np; ++i) outp[3 * i + a] = buf[i];
+        }
+        float* sc[3] = {adj, vort, curv};
+        for (int q = 0; q < 3 && rc == VARE_OK; ++q)
+            if (sc[q] && hipMemcpy(sc[q], h->pa[6 + q], np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+                rc = fail(VARE_EHIP, "particle download failed");
+    } else {
+        if (pos) hipLaunchKernelGGL(k_interleave, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, srcs[0], srcs[1], srcs[2], pos);
+        if (vel) hipLaunchKernelGGL(k_interleave, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, srcs[3], srcs[4], srcs[5], vel);
+        float* sc[3] = {adj, vort, curv};
+        for (int q = 0; q < 3; ++q)
+            if (sc[q]) (void)hipMemcpyAsync(sc[q], h->pa[6 + q], np * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
+        if (hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(VARE_EHIP, "particle copy failed");
+    }
+    for (int q = 0; q < 6; ++q) dfree(tmp[q]);
+    return rc;
+}
+
+int vare_flip_num_particles(const vare_flip* h, size_t* n) {
+    if (!h || !n) return fail(VARE_EINVAL, "null argument");
+    *n = h->np;
+    return VARE_OK;
+}
+
+static int convert_all(vare_flip* h, bool to_voxel) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (!h->np) return VARE_OK;
+    PArrays pa;
+    for (int q = 0; q < 9; ++q) pa.a[q] = h->pa[q];
+    if (to_voxel)
+        hipLaunchKernelGGL(k_to_voxel, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, pa, h->t[0], h->t[1],
+                           h->t[2], h->s[0], h->s[1], h->s[2]);
+    else
+        hipLaunchKernelGGL(k_to_world, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, pa, h->t[0], h->t[1],
+                           h->t[2], h->s[0], h->s[1], h->s[2]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return VARE_OK;
+}
+int vare_flip_particles_to_voxel(vare_flip* h) { return convert_all(h, true); }
+int vare_flip_particles_to_world(vare_flip* h) { return convert_all(h, false); }
+
+int vare_flip_stage(vare_flip* h, int stage, float dt) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (stage < 0 || stage > 5) return fail(VARE_EINVAL, "bad stage %d", stage);
+    if (stage == 0) h->stats.num_removed = 0;
+    CHK(dispatch_stage(h, stage, dt));
+    return tcollect(h);
+}
+
+int vare_flip_step(vare_flip* h, float dt) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    h->stage_next = 0;
+    h->stats.num_removed = 0;
+    HIPCHK(hipEventRecord(h->step_ev[0], h->stream));
+    for (int s = 0; s < 6; ++s) CHK(dispatch_stage(h, s, dt));
+    HIPCHK(hipEventRecord(h->step_ev[1], h->stream));
+    HIPCHK(hipEventSynchronize(h->step_ev[1]));
+    float ms = 0.f, sms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->step_ev[0], h->step_ev[1]));
+    HIPCHK(hipEventElapsedTime(&sms, h->step_ev[2], h->step_ev[3]));
+    h->stats.step_ms = ms;
+    h->stats.solve_ms = sms;
+    CHK(tcollect(h));
+    if (h->prm.solver_mode == VARE_SOLVER_RELATIVE && !h->stats.converged)
+        return fail(VARE_ENOCONV, "PCG stopped at %d iterations, |r|=%g |b|=%g", h->stats.cg_iterations,
+                    h->stats.residual_norm, h->stats.b_norm);
+    return VARE_OK;
+}
+
+int vare_flip_advance_frame(vare_flip* h) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (h->np == 0) return VARE_OK;                         // FLIPMethod.cu:179
+    CHK(convert_all(h, true));
+    int rc = vare_flip_step(h, h->prm.dt);
+    CHK(convert_all(h, false));
+    return rc;
+}
+
+int vare_flip_get_stats(const vare_flip* h, vare_step_stats* o) {
+    if (!h || !o) return fail(VARE_EINVAL, "null argument");
+    *o = h->stats;
+    return VARE_OK;
+}
+
+int vare_flip_get_block_table(vare_flip* h, int64_t* pointers, int32_t* coords, size_t cap, size_t* nb) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (nb) *nb = (size_t)h->nb;
+    const int B3 = h->B * h->B * h->B;
+    if (pointers) {
+        std::vector<int> sl((size_t)h->nblk);
+        HIPCHK(hipMemcpy(sl.data(), h->slot, sl.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < sl.size(); ++t) pointers[t] = sl[t] < 0 ? -1 : (int64_t)sl[t] * B3;
+    }
+    if (coords) {
+        if (cap < (size_t)h->nb) return fail(VARE_EINVAL, "coords capacity too small");
+        std::vector<int> co((size_t)h->nb);
+        if (h->nb) HIPCHK(hipMemcpy(co.data(), h->coord, co.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < co.size(); ++q) {
+            const int t = co[q];
+            coords[3 * q] = t % h->fx;
+            coords[3 * q + 1] = (t / h->fx) % h->fy;
+            coords[3 * q + 2] = t / (h->fx * h->fy);
+        }
+    }
+    return VARE_OK;
+}
+
+int vare_flip_get_cell_field(vare_flip* h, int which, void* out, size_t cap_bytes, size_t* n) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    const size_t nc = h->ncells();
+    if (n) *n = nc;
+    if (!out) return VARE_OK;
+    size_t need = nc * 4;
+    if (which == VARE_FIELD_VELOCITY || which == VARE_FIELD_COMP) need = nc * 12;
+    if (which == VARE_FIELD_MARK) need = nc * 8;
+    if (cap_bytes < need) return fail(VARE_EINVAL, "buffer too small (%zu < %zu)", cap_bytes, need);
+    if (nc == 0) return VARE_OK;
+    const float* one = nullptr;
+    switch (which) {
+        case VARE_FIELD_MASS: one = h->mass; break;
+        case VARE_FIELD_PRESSURE: one = h->x; break;
+        case VARE_FIELD_COUNT: one = h->cnt; break;
+        case VARE_FIELD_DIV: one = h->b; break;
+        default: break;
+    }
+    if (one) {
+        HIPCHK(hipMemcpy(out, one, nc * 4, hipMemcpyDeviceToHost));
+        return VARE_OK;
+    }
+    if (which == VARE_FIELD_VELOCITY || which == VARE_FIELD_COMP) {
+        const float* c3[3] = {which == VARE_FIELD_VELOCITY ? h->vx : h->cx, which == VARE_FIELD_VELOCITY ? h->vy : h->cy,
+                              which == VARE_FIELD_VELOCITY ? h->vz : h->cz};
+        std::vector<float> buf(nc);
+        float* o = (float*)out;
+        for (int a = 0; a < 3; ++a) {
+            HIPCHK(hipMemcpy(buf.data(), c3[a], nc * 4, hipMemcpyDeviceToHost));
+            for (size_t c = 0; c < nc; ++c) o[3 * c + a] = buf[c];
+        }
+        return VARE_OK;
+    }
+    if (which == VARE_FIELD_MARK) {
+        OpFluid op{h->type, h->rank};
+        CHK(run_scan(h, op, (int)nc, 2));
+        unsigned long long* tmp = nullptr;
+        CHK(dalloc(&tmp, nc));
+        hipLaunchKernelGGL(k_mark_u64, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, (const uint8_t*)h->type,
+                           (const int*)h->rank, tmp);
+        int rc = VARE_OK;
+        if (hipStreamSynchronize(h->stream) != hipSuccess || hipMemcpy(out, tmp, nc * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(VARE_EHIP, "mark download failed");
+        dfree(tmp);
+        return rc;
+    }
+    return fail(VARE_EINVAL, "bad field id %d", which);
+}
+
+int vare_flip_get_removed_mask(vare_flip* h, uint8_t* out, size_t cap, size_t* n) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    const size_t m = h->last_np_before_remove;
+    if (n) *n = m;
+    if (!out) return VARE_OK;
+    if (cap < m) return fail(VARE_EINVAL, "capacity too small");
+    if (m) HIPCHK(hipMemcpy(out, h->removed, m, hipMemcpyDeviceToHost));
+    return VARE_OK;
+}
+
+int vare_seed_box(int64_t nx, int64_t ny, int64_t nz, float wx, float wy, float wz, int ppc, uint32_t seed, float* pos,
+                  size_t cap, size_t* n) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || ppc <= 0) return fail(VARE_EINVAL, "bad seeding arguments");
+    size_t cnt = 0;
+    for (int64_t k = 0; k < nz; ++k)
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t i = 0; i < nx; ++i) {
+                if (!((float)i + 0.5f < wx && (float)j + 0.5f < wy && (float)k + 0.5f < wz)) continue;
+                const uint64_t cell = (uint64_t)i + (uint64_t)nx * ((uint64_t)j + (uint64_t)ny * (uint64_t)k);
+                for (int q = 0; q < ppc; ++q) {
+                    const uint32_t qq = (uint32_t)((uint64_t)ppc * cell + (uint64_t)q);
+                    const float px = (float)i + rand01(seed + 3u * qq);
+                    const float py = (float)j + rand01(seed + 3u * qq + 1u);
+                    const float pz = (float)k + rand01(seed + 3u * qq + 2u);
+                    if (!(px < wx && py < wy && pz < wz)) continue;
+                    if (pos && cnt < cap) { pos[3 * cnt] = px; pos[3 * cnt + 1] = py; pos[3 * cnt + 2] = pz; }
+                    cnt++;
+                }
+            }
+    if (n) *n = cnt;
+    if (pos && cnt > cap) return fail(VARE_EINVAL, "capacity %zu < %zu points", cap, cnt);
+    return VARE_OK;
+}
+
+int vare_device_synchronize(void) {
+    HIPCHK(hipDeviceSynchronize());
+    return VARE_OK;
+}
+
+int vare_device_name(char* buf, size_t cap) {
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, 0));
+    if (buf && cap) {
+        snprintf(buf, cap, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    }
+    return VARE_OK;
+}
+
+int vare_flip_set_kernel_timing(vare_flip* h, int enable) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    CHK(tcollect(h));
+    h->timing = (unsigned)enable;
+    return VARE_OK;
+}
+
+int vare_flip_get_kernel_timing(vare_flip* h, double ms[VARE_KERNEL_COUNT], int64_t launches[VARE_KERNEL_COUNT], int reset) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    CHK(tcollect(h));
+    for (int q = 0; q < VARE_KERNEL_COUNT; ++q) {
+        if (ms) ms[q] = h->kms[q];
+        if (launches) launches[q] = h->klaunch[q];
+        if (reset) { h->kms[q] = 0; h->klaunch[q] = 0; }
+    }
+    return VARE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+// flip_device.h -- device-side building blocks of the gfx950 FLIP step.
+//
+// Layout in HBM (one handle):
+//   particles  : structure of arrays px,py,pz,vx,vy,vz,adj,vort,curv (float), in the reference's
+//                particle order (so Array::remove's order is kept without a permutation).
+//   frame      : slot[fx*fy*fz] (int32: active slot or -1), coord[slot] (linear block index),
+//                nbr[slot*6] (slot of the -x,+x,-y,+y,-z,+z neighbour block, -1 inactive,
+//                -2 outside the domain).
+//   cells      : Nb*B^3 cells, block-major, x fastest inside a block (SparseField.h:57-61), one
+//                array per scalar component (mass, vel.xyz, comp.xyz, count, p, b, r, p, Ap) plus
+//                a 1-byte cell type and a 2-byte stencil descriptor.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vare {
+
+constexpr int WG = 256;        // threads per workgroup for every kernel (4 waves of 64)
+
+// cell types (FluidProjection.h:32-39); fluid cells carry kFluid here, the reference's
+// 4+rank numbering is produced on demand by a scan.
+enum : uint8_t { kAir = 0, kSolid = 1, kWall = 2, kFluid = 4 };
+
+// stencil descriptor (uint16) of a fluid cell: bit q (q=0..5 for -x,+x,-y,+y,-z,+z) = neighbour
+// is fluid, bits 8..11 = diagonal 6 + nAir - nSolidOrWall (0..12), bit 15 = cell is fluid.
+constexpr uint16_t kInfoFluid = 0x8000u;
+
+__host__ __device__ inline float rand01(uint32_t seed) {        // Random.h:18-27
+    uint32_t i = (seed ^ 12345391u) * 2654435769u;
+    i ^= (i << 6) ^ (i >> 26);
+    i *= 2654435769u;
+    i += (i << 5) ^ (i >> 12);
+    return (float)i / 4294967296.0f;                             // (float)UINT_MAX rounds to 2^32
+}
+
+template <class T> __device__ __forceinline__ T rmin(T a, T b) { return (a < b) ? a : b; }   // MathUtils.h:174-178
+template <class T> __device__ __forceinline__ T rmax(T a, T b) { return (a > b) ? a : b; }   // MathUtils.h:195-199
+__device__ __forceinline__ float rabs(float x) { return (x < 0.f) ? -x : x; }                // MathUtils.h:129-132
+template <class T> __device__ __forceinline__ T rclamp(T x, T lo, T hi) {                   // MathUtils.h:286-292
+    if (x < lo) return lo;
+    if (x > hi) return hi;
+    return x;
+}
+
+// Frame seen by kernels.
+struct FrameK {
+    int fx, fy, fz;
+    int nx, ny, nz;          // padded cell dims = f*B
+    const int* slot;         // [fx*fy*fz]
+    const int* coord;        // [Nb] linear block index
+    const int* nbr;          // [Nb*6]
+};
+
+template <int B> struct Blk {
+    static constexpr int B2 = B * B;
+    static constexpr int B3 = B * B * B;
+    static constexpr int SH = (B == 4) ? 2 : 3;
+};
+
+// Storage index of cell (i,j,k) (0 <= i < nx ...), or -1 when its block is inactive
+// (SparseField::findIndex, SparseField.h:89-95).
+template <int B>
+__device__ __forceinline__ int cell_index(const FrameK& f, int i, int j, int k) {
+    constexpr int SH = Blk<B>::SH;
+    const int s = f.slot[((k >> SH) * f.fy + (j >> SH)) * f.fx + (i >> SH)];
+    if (s < 0) return -1;
+    return (s << (3 * SH)) + ((((k & (B - 1)) << SH) + (j & (B - 1))) << SH) + (i & (B - 1));
+}
+
+// Cell coordinates of storage index c (SparseField::findIndices, SparseField.h:103-112).
+template <int B>
+__device__ __forceinline__ void cell_coords(const FrameK& f, int c, int& i, int& j, int& k) {
+    constexpr int SH = Blk<B>::SH;
+    const int s = c >> (3 * SH);
+    const int l = c & (Blk<B>::B3 - 1);
+    const int t = f.coord[s];
+    const int bi = t % f.fx, bj = (t / f.fx) % f.fy, bk = t / (f.fx * f.fy);
+    i = (bi << SH) + (l & (B - 1));
+    j = (bj << SH) + ((l >> SH) & (B - 1));
+    k = (bk << SH) + (l >> (2 * SH));
+}
+
+// Neighbour of cell c in direction q (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z).  Returns the storage
+// index, -1 when the neighbour block is inactive, -2 when the neighbour is outside the domain.
+template <int B>
+__device__ __forceinline__ int cell_neighbor(const FrameK& f, int c, int q) {
+    constexpr int SH = Blk<B>::SH;
+    const int s = c >> (3 * SH);
+    const int l = c & (Blk<B>::B3 - 1);
+    const int li = l & (B - 1), lj = (l >> SH) & (B - 1), lk = l >> (2 * SH);
+    const int axis = q >> 1;
+    const int dir = (q & 1) ? 1 : -1;
+    const int lc = axis == 0 ? li : (axis == 1 ? lj : lk);
+    const int stride = axis == 0 ? 1 : (axis == 1 ? B : B * B);
+    const int nl = lc + dir;
+    if (nl >= 0 && nl < B) return c + dir * stride;
+    const int ns = f.nbr[s * 6 + q];
+    if (ns < 0) return ns;
+    return (ns << (3 * SH)) + l - dir * (B - 1) * stride;
+}
+
+// ---- wave / workgroup reductions (wave64) ----
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over a 256-thread workgroup; result valid in every thread.
+__device__ __forceinline__ double block_sum(double v, double* sh /*[4]*/) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    const double r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+// Exclusive scan over a 256-thread workgroup of one int per thread; total returned in *tot.
+__device__ __forceinline__ int block_exscan(int v, int* sh /*[4]*/, int* tot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    int off = 0;
+    for (int q = 0; q < w; ++q) off += sh[q];
+    *tot = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return off + x - v;
+}
+
+// Agent-scope coherent load of a value written by another workgroup of the same kernel.
+template <class T> __device__ __forceinline__ T load_agent(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// "Last workgroup" detection for single-pass grid reductions: each workgroup stores its partial,
+// then takes a ticket; the workgroup that draws the last ticket sees every partial.
+__device__ __forceinline__ bool last_block(unsigned* ticket, unsigned nblocks, int* sh_flag) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        *sh_flag = (t == nblocks - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    const bool last = *sh_flag != 0;
+    if (last) __threadfence();
+    return last;
+}
+
+// Deterministic sum of partials[0..n) by one workgroup (fixed order: strided per thread, then a
+// fixed tree).
+__device__ __forceinline__ double sum_partials(const double* partials, int n, double* sh) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += WG) s += load_agent(partials + q);
+    return block_sum(s, sh);
+}
+
+}  // namespace vare

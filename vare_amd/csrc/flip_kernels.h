@@ -1,0 +1,720 @@
+// flip_kernels.h -- gfx950 kernels of one FLIP step on the two-level sparse grid.
+//
+// Each kernel cites the reference stage it replaces.  All kernels are bandwidth-bound integer /
+// stencil / scatter work (no MFMA).  Floating-point expressions keep the reference's operation
+// order; the library is compiled with -ffp-contract=off so no FMA is formed behind them.
+#pragma once
+#include "flip_device.h"
+
+namespace vare {
+
+// ------------------------------------------------------------------------------------------------
+// Generic fill kernels (graph-safe replacement for memset).
+__global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Binning + block activation: SparseFrame::buildFromPoints K1 (SparseFrame.cu:141-161).
+// Each particle marks the 3x3x3 block neighbourhood starting at c-1 with c = max((int)p/B, 0).
+// The linear index uses the reference's signed int arithmetic, so a -1 coordinate aliases into
+// the next axis exactly as in the reference; indices below zero (out of bounds in the reference)
+// are dropped.  Only "count > 0" matters downstream, so a plain store of 1 replaces the atomicAdd.
+// Lanes whose block equals the previous lane's block skip the 27 stores (particles arrive in
+// spatially coherent order).
+template <int B>
+__global__ void k_bin_mark(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+                           int np, int fx, int fy, int fz, int* __restrict__ flag) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    int ci = -1000000, cj = 0, ck = 0;
+    if (i < np) {
+        ci = rmax((int)(px[i]) / B, 0);
+        cj = rmax((int)(py[i]) / B, 0);
+        ck = rmax((int)(pz[i]) / B, 0);
+    }
+    const int lane = threadIdx.x & 63;
+    const int pi = __shfl_up(ci, 1, 64), pj = __shfl_up(cj, 1, 64), pk = __shfl_up(ck, 1, 64);
+    if (i >= np) return;
+    if (lane > 0 && pi == ci && pj == cj && pk == ck) return;
+    const int ke = rmin(ck + 1, fz - 1), je = rmin(cj + 1, fy - 1), ie = rmin(ci + 1, fx - 1);
+    for (int k = ck - 1; k <= ke; ++k)
+        for (int j = cj - 1; j <= je; ++j)
+            for (int ii = ci - 1; ii <= ie; ++ii) {
+                const int t = k * fx * fy + j * fx + ii;
+                if (t >= 0) flag[t] = 1;
+            }
+}
+
+// Water-tank activation (SparseFrame.cu:163-173): every block below the wall level is active.
+__global__ void k_bin_tank(int fx, int fy, int fz, int tall, int* __restrict__ flag) {
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (t >= fx * fy * fz) return;
+    const int j = (t / fx) % fy;
+    if (j < tall) flag[t] = 1;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Three-phase workgroup scan (reduce tiles -> scan tile sums -> scan tiles and emit).  A tile is
+// 256 threads x ITEMS consecutive elements.  Op supplies value(i) and emit(i, prefix, value).
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = WG * SCAN_ITEMS;
+
+template <class Op>
+__global__ void k_scan_reduce(Op op, int n, int* __restrict__ tile_sums) {
+    __shared__ int sh[4];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q)
+        if (base + q < n) s += op.value(base + q);
+    int tot;
+    block_exscan(s, sh, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// One workgroup: exclusive scan of tile sums in place; total to *total.
+__global__ void k_scan_tiles(int* __restrict__ tile_sums, int ntiles, int* __restrict__ total) {
+    __shared__ int sh[4];
+    int carry = 0;
+    for (int base = 0; base < ntiles; base += WG) {
+        const int i = base + threadIdx.x;
+        const int v = i < ntiles ? tile_sums[i] : 0;
+        int tot;
+        const int ex = block_exscan(v, sh, &tot);
+        if (i < ntiles) tile_sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+template <class Op>
+__global__ void k_scan_apply(Op op, int n, const int* __restrict__ tile_sums) {
+    __shared__ int sh[4];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        v[q] = (base + q < n) ? op.value(base + q) : 0;
+        s += v[q];
+    }
+    int tot;
+    int run = tile_sums[blockIdx.x] + block_exscan(s, sh, &tot);
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        if (base + q < n) op.emit(base + q, run, v[q]);
+        run += v[q];
+    }
+}
+
+// Block compaction (SparseFrame.cu:175-196): slot[t] = rank or -1, coord[rank] = t.
+struct OpBlocks {
+    const int* flag;
+    int* slot;
+    int* coord;
+    __device__ int value(int t) const { return flag[t] > 0 ? 1 : 0; }
+    __device__ void emit(int t, int pre, int v) const {
+        slot[t] = v ? pre : -1;
+        if (v) coord[pre] = t;
+    }
+};
+
+// Fluid enumeration (FluidProjection.cu:178-186): rank of every fluid cell in storage order.
+struct OpFluid {
+    const uint8_t* type;
+    int* rank;
+    __device__ int value(int c) const { return type[c] == kFluid ? 1 : 0; }
+    __device__ void emit(int c, int pre, int v) const { rank[c] = v ? pre : -1; }
+};
+
+// Removal compaction, part 1: ranks of removed slots below newN; hole[rank] = slot.
+struct OpHoles {
+    const uint8_t* removed;
+    int* hole;
+    __device__ int value(int i) const { return removed[i] ? 1 : 0; }
+    __device__ void emit(int i, int pre, int v) const {
+        if (v) hole[pre] = i;
+    }
+};
+// part 2: survivors at or above newN, in ascending order: src[rank] = newN + offset.
+struct OpSurvivors {
+    const uint8_t* removed;
+    int base;
+    int* src;
+    __device__ int value(int o) const { return removed[base + o] ? 0 : 1; }
+    __device__ void emit(int o, int pre, int v) const {
+        if (v) src[pre] = base + o;
+    }
+};
+
+// Neighbour block table: nbr[s*6+q] = slot of the neighbour block, -1 inactive, -2 outside.
+__global__ void k_block_nbr(int nb, int fx, int fy, int fz, const int* __restrict__ coord, const int* __restrict__ slot,
+                            int* __restrict__ nbr) {
+    const int e = blockIdx.x * WG + threadIdx.x;
+    if (e >= nb * 6) return;
+    const int s = e / 6, q = e - 6 * (e / 6);
+    const int t = coord[s];
+    int bi = t % fx, bj = (t / fx) % fy, bk = t / (fx * fy);
+    const int d = (q & 1) ? 1 : -1;
+    if (q < 2) bi += d; else if (q < 4) bj += d; else bk += d;
+    int r;
+    if (bi < 0 || bi >= fx || bj < 0 || bj >= fy || bk < 0 || bk >= fz) r = -2;
+    else r = slot[(bk * fy + bj) * fx + bi];
+    nbr[e] = r;
+}
+
+// ------------------------------------------------------------------------------------------------
+// P2G: Rasterization::PointsToVelField (Rasterization.cu:121-173) fused with PointsToCountField
+// (Rasterization.cu:198-228).  One thread per particle, float atomics into the block-major
+// fields.  Summation order is that of the atomics (the reference has the same non-determinism).
+struct CellFields {
+    float* mass;
+    float* vx;
+    float* vy;
+    float* vz;
+    float* cnt;
+};
+
+template <int B>
+__global__ void k_p2g_atomic(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+                             const float* __restrict__ ux, const float* __restrict__ uy, const float* __restrict__ uz,
+                             int np, FrameK f, CellFields cf) {
+    const int n = blockIdx.x * WG + threadIdx.x;
+    if (n >= np) return;
+    const float x = px[n], y = py[n], z = pz[n];
+    // Grid::inside (Grid.h:407-417)
+    if (x < 0.f || x > (float)f.nx || y < 0.f || y > (float)f.ny || z < 0.f || z > (float)f.nz) return;
+    const float vx = ux[n], vy = uy[n], vz = uz[n];
+    const int ci = rmax((int)(x - 0.5f), 0), cj = rmax((int)(y - 0.5f), 0), ck = rmax((int)(z - 0.5f), 0);
+    const int ie = rmin(ci + 1, f.nx - 1), je = rmin(cj + 1, f.ny - 1), ke = rmin(ck + 1, f.nz - 1);
+    for (int k = ck; k <= ke; ++k)
+        for (int j = cj; j <= je; ++j)
+            for (int i = ci; i <= ie; ++i) {
+                const int c = cell_index<B>(f, i, j, k);
+                if (c < 0) continue;
+                const float dx = x - ((float)i + 0.5f), dy = y - ((float)j + 0.5f), dz = z - ((float)k + 0.5f);
+                const float xw = rmax(0.f, 1.f - rabs(dx));
+                const float yw = rmax(0.f, 1.f - rabs(dy));
+                const float zw = rmax(0.f, 1.f - rabs(dz));
+                const float w = xw * yw * zw + 1e-05f;
+                unsafeAtomicAdd(cf.mass + c, w);
+                unsafeAtomicAdd(cf.vx + c, w * vx);
+                unsafeAtomicAdd(cf.vy + c, w * vy);
+                unsafeAtomicAdd(cf.vz + c, w * vz);
+                unsafeAtomicAdd(cf.cnt + c, 1.f);
+            }
+}
+
+// P2G normalise (Rasterization.cu:175-188) + cell classification and comp := prev
+// (FLIPMethod.cu:242-280, no collision source: phi = 5, so no solid cells and no collision velocity).
+__global__ void k_cell_finalize(int nc, CellFields cf, float* __restrict__ cx, float* __restrict__ cy,
+                                float* __restrict__ cz, uint8_t* __restrict__ type) {
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= nc) return;
+    const float m = cf.mass[c];
+    float vx = 0.f, vy = 0.f, vz = 0.f;
+    uint8_t t = kAir;
+    if (m > 0.f) {
+        vx = cf.vx[c] / m;
+        vy = cf.vy[c] / m;
+        vz = cf.vz[c] / m;
+        t = kFluid;
+    } else {
+        cf.mass[c] = 0.f;
+    }
+    cf.vx[c] = vx; cf.vy[c] = vy; cf.vz[c] = vz;
+    cx[c] = vx; cy[c] = vy; cz[c] = vz;
+    type[c] = t;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Divergence + stencil descriptor: FluidProjection::buildLinearSystem K6 (FluidProjection.cu:203-267),
+// matrix-free: the ELL row [diag, -1 per fluid neighbour] is encoded in 2 bytes.  Also initialises the
+// CG vectors (x = 0, r = b - A*0 = b, p = r; LinearSystemSolver.cpp:17-19) and reduces |b|^2.
+struct CgVecs {
+    float* x;
+    float* r;
+    float* p;
+    float* ap;
+    float* b;
+    uint16_t* info;
+};
+
+struct CgState {
+    double rr;        // r.r of the current residual (double accumulation)
+    double rz;        // r.z (preconditioned)
+    double bb;        // b.b
+    double pap;
+    float rsold;      // reference-mode scalars, kept in FP32 like LinearSystemSolver.cpp:21-41
+    float alpha;
+    float beta;
+    float rzold;
+    int iter;         // iterations executed
+    int done;
+    int count;        // return value of LinearSystemSolver::cg
+    int converged;
+    unsigned ticket;
+    int mode;         // 0 reference, 1 relative
+    int max_count;    // reference: params.projIteration; relative: max iterations
+    int loop_limit;   // reference: b.size()+5
+    float abs_tol;
+    float rel_tol;
+    int precond;
+    int nfluid;       // fluid cells (rows of the system)
+};
+
+template <int B>
+__global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type, const float* __restrict__ cx,
+                             const float* __restrict__ cy, const float* __restrict__ cz, float wall_level, CgVecs v,
+                             double* __restrict__ partials, double* __restrict__ partials2, CgState* __restrict__ st) {
+    __shared__ double sh[4];
+    __shared__ int sflag;
+    const int c = blockIdx.x * WG + threadIdx.x;
+    double bb = 0.0, nfl = 0.0;
+    if (c < nc) {
+        float d = 0.f;
+        uint16_t info = 0;
+        if (type[c] == kFluid) {
+            int i, j, k;
+            cell_coords<B>(f, c, i, j, k);
+            const uint8_t wall = ((float)j + 0.5f <= wall_level + 1.f) ? kWall : kAir;
+            int nid[6];
+            uint8_t tn[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                nid[q] = cell_neighbor<B>(f, c, q);
+                tn[q] = nid[q] == -2 ? wall : (nid[q] < 0 ? (uint8_t)kAir : type[nid[q]]);
+            }
+            const float vx = cx[c], vy = cy[c], vz = cz[c];
+            auto comp = [&](int q) -> float {       // neighbour velocity component along q's axis
+                const int id = nid[q];
+                const float* a = q < 2 ? cx : (q < 4 ? cy : cz);
+                return id >= 0 ? a[id] : 0.f;
+            };
+            // neighbour velocities with domain-boundary extrapolation (FluidProjection.cu:220-225)
+            float nv[6];
+            const float own[3] = {vx, vy, vz};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int lo = 2 * a, hi = 2 * a + 1;
+                nv[lo] = nid[lo] == -2 ? 2.f * own[a] - comp(hi) : comp(lo);
+                nv[hi] = nid[hi] == -2 ? 2.f * own[a] - comp(lo) : comp(hi);
+            }
+            // face velocities (FluidProjection.cu:227-232)
+            float fv[6];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int lo = 2 * a, hi = 2 * a + 1;
+                fv[lo] = tn[lo] == kAir ? 2.f * own[a] - nv[hi] : (tn[lo] == kWall ? rabs(own[a]) : nv[lo]);
+                fv[hi] = tn[hi] == kAir ? 2.f * own[a] - nv[lo] : (tn[hi] == kWall ? -rabs(own[a]) : nv[hi]);
+            }
+            d = -((fv[1] - fv[0]) * 0.5f + (fv[3] - fv[2]) * 0.5f + (fv[5] - fv[4]) * 0.5f);
+            int diag = 6;
+            uint16_t mask = 0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (tn[q] == kAir) diag += 1;
+                else if (tn[q] == kSolid || tn[q] == kWall) diag -= 1;
+                else mask |= (uint16_t)(1u << q);
+            }
+            info = (uint16_t)(kInfoFluid | mask | (diag << 8));
+            nfl = 1.0;
+        }
+        v.b[c] = d;
+        v.info[c] = info;
+        v.x[c] = 0.f;
+        v.r[c] = d;
+        v.p[c] = d;
+        v.ap[c] = 0.f;
+        bb = (double)d * (double)d;
+    }
+    const double s = block_sum(bb, sh);
+    const double s2 = block_sum(nfl, sh);
+    if (threadIdx.x == 0) { partials[blockIdx.x] = s; partials2[blockIdx.x] = s2; }
+    if (last_block(&st->ticket, gridDim.x, &sflag)) {
+        const double tot = sum_partials(partials, gridDim.x, sh);
+        const double tot2 = sum_partials(partials2, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            st->ticket = 0;
+            st->nfluid = (int)tot2;
+            st->loop_limit = (int)tot2 + 5;     // LinearSystemSolver.cpp:25-27: i < b.size()+5
+            st->bb = tot;
+            st->rr = tot;
+            st->iter = 0;
+            st->count = 0;
+            st->converged = 0;
+            st->done = 0;
+            const float rs = (float)tot;
+            st->rsold = rs;
+            if (st->mode == 0) {
+                if (sqrtf(rs) < 1e-30f) { st->done = 1; st->converged = 1; }    // LinearSystemSolver.cpp:23
+            } else {
+                if (tot == 0.0) { st->done = 1; st->converged = 1; }
+            }
+        }
+    }
+}
+
+// Jacobi preconditioner value for a cell (0 off the fluid set).
+__device__ __forceinline__ float jacobi_inv(uint16_t info) {
+    const int d = (info >> 8) & 0x1f;
+    return ((info & kInfoFluid) && d > 0) ? 1.f / (float)d : 0.f;
+}
+
+// Relative mode set-up: z = M^-1 r, p = z, rz = r.z (after k_divergence).
+__global__ void k_cg_precond_init(int nc, CgVecs v, double* __restrict__ partials, CgState* __restrict__ st) {
+    __shared__ double sh[4];
+    __shared__ int sflag;
+    const int c = blockIdx.x * WG + threadIdx.x;
+    double rz = 0.0;
+    if (c < nc) {
+        const float r = v.r[c];
+        const float z = st->precond == 1 ? r * jacobi_inv(v.info[c]) : r;
+        v.p[c] = z;
+        rz = (double)r * (double)z;
+    }
+    const double s = block_sum(rz, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    if (last_block(&st->ticket, gridDim.x, &sflag)) {
+        const double tot = sum_partials(partials, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            st->ticket = 0;
+            st->rz = tot;
+            st->rzold = (float)tot;
+        }
+    }
+}
+
+// CG step 1: Ap = A p (BLAS::mul, BLAS.cu:12-40, matrix-free 7-point row in the reference's
+// term order) and p.Ap -> alpha (LinearSystemSolver.cpp:29-31).
+template <int B>
+__global__ void k_cg_spmv(int nc, FrameK f, CgVecs v, double* __restrict__ partials, CgState* __restrict__ st) {
+    __shared__ double sh[4];
+    __shared__ int sflag;
+    if (st->done) return;
+    const int c = blockIdx.x * WG + threadIdx.x;
+    double pap = 0.0;
+    if (c < nc) {
+        const uint16_t info = v.info[c];
+        float acc = 0.f;
+        if (info & kInfoFluid) {
+            const float pc = v.p[c];
+            acc = (float)((info >> 8) & 0x1f) * pc;
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (info & (1u << q)) acc = acc - v.p[cell_neighbor<B>(f, c, q)];
+            pap = (double)pc * (double)acc;
+        }
+        v.ap[c] = acc;
+    }
+    const double s = block_sum(pap, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    if (last_block(&st->ticket, gridDim.x, &sflag)) {
+        const double tot = sum_partials(partials, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            st->ticket = 0;
+            st->pap = tot;
+            if (st->mode == 0) st->alpha = st->rsold / (float)tot;
+            else st->alpha = st->rzold / (float)tot;
+        }
+    }
+}
+
+// CG step 2: x += alpha p, r -= alpha Ap, r.r (and r.z), convergence test and beta
+// (LinearSystemSolver.cpp:33-46).
+__global__ void k_cg_update(int nc, CgVecs v, double* __restrict__ partials, double* __restrict__ partials2,
+                            CgState* __restrict__ st) {
+    __shared__ double sh[4];
+    __shared__ int sflag;
+    if (st->done) return;
+    const float alpha = st->alpha;
+    const int precond = st->precond;
+    const int c = blockIdx.x * WG + threadIdx.x;
+    double rr = 0.0, rz = 0.0;
+    if (c < nc) {
+        const float p = v.p[c];
+        v.x[c] = v.x[c] + (alpha * p);
+        const float r = v.r[c] - (alpha * v.ap[c]);
+        v.r[c] = r;
+        rr = (double)r * (double)r;
+        if (precond == 1) rz = (double)r * (double)(r * jacobi_inv(v.info[c]));
+    }
+    const double s = block_sum(rr, sh);
+    const double s2 = precond == 1 ? block_sum(rz, sh) : 0.0;
+    if (threadIdx.x == 0) { partials[blockIdx.x] = s; partials2[blockIdx.x] = s2; }
+    if (last_block(&st->ticket, gridDim.x, &sflag)) {
+        const double tot = sum_partials(partials, gridDim.x, sh);
+        const double tot2 = precond == 1 ? sum_partials(partials2, gridDim.x, sh) : 0.0;
+        if (threadIdx.x == 0) {
+            st->ticket = 0;
+            st->rr = tot;
+            const int i = st->iter;
+            st->iter = i + 1;
+            if (st->mode == 0) {
+                const float rsnew = (float)tot;
+                if (sqrtf(rsnew) < st->abs_tol) {
+                    st->done = 1; st->count = i; st->converged = 1;
+                } else {
+                    st->beta = rsnew / st->rsold;
+                    st->rsold = rsnew;
+                    if (i > st->max_count) { st->done = 1; st->count = i; }
+                    else if (i + 1 >= st->loop_limit) { st->done = 1; st->count = i + 1; }
+                }
+            } else {
+                if (sqrt(tot) <= (double)st->rel_tol * sqrt(st->bb)) {
+                    st->done = 1; st->count = i + 1; st->converged = 1;
+                } else if (i + 1 >= st->max_count) {
+                    st->done = 1; st->count = i + 1;
+                } else {
+                    const float rznew = precond == 1 ? (float)tot2 : (float)tot;
+                    st->beta = rznew / st->rzold;
+                    st->rzold = rznew;
+                }
+            }
+        }
+    }
+}
+
+// CG step 3: p = r + beta p (BLAS::addmul, LinearSystemSolver.cpp:41), or p = z + beta p.
+__global__ void k_cg_pupdate(int nc, CgVecs v, const CgState* __restrict__ st) {
+    if (st->done) return;
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= nc) return;
+    const float beta = st->beta;
+    const float r = v.r[c];
+    const float z = st->precond == 1 ? r * jacobi_inv(v.info[c]) : r;
+    v.p[c] = z + (beta * v.p[c]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Pressure apply.  The solve runs in place on the cell layout, so FluidProjection.cu:297-306
+// (scatter x to the field) has no counterpart.  Air extrapolation, FluidProjection.cu:314-335:
+// each active air cell takes the mean pressure of its fluid neighbours.
+template <int B>
+__global__ void k_press_extrap(int nc, FrameK f, const uint8_t* __restrict__ type, float* __restrict__ pres) {
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= nc) return;
+    if (type[c] != kAir) return;
+    float press = 0.f;
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int id = cell_neighbor<B>(f, c, q);
+        if (id >= 0 && type[id] == kFluid) { press += pres[id]; cnt++; }
+    }
+    if (cnt > 0) pres[c] = press / (float)cnt;
+}
+
+// Gradient subtract, FluidProjection.cu:342-378.
+template <int B>
+__global__ void k_press_grad(int nc, FrameK f, const uint8_t* __restrict__ type, const float* __restrict__ pres,
+                             float wall_level, float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz) {
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= nc) return;
+    if (type[c] != kFluid) return;
+    int i, j, k;
+    cell_coords<B>(f, c, i, j, k);
+    const uint8_t wall = ((float)j + 0.5f <= wall_level + 1.f) ? kSolid : kAir;
+    const float p = pres[c];
+    int nid[6];
+    uint8_t tn[6];
+    float pv[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        nid[q] = cell_neighbor<B>(f, c, q);
+        tn[q] = nid[q] == -2 ? wall : (nid[q] < 0 ? (uint8_t)kAir : type[nid[q]]);
+        pv[q] = nid[q] >= 0 ? pres[nid[q]] : 0.f;
+    }
+    float nb[6];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int lo = 2 * a, hi = 2 * a + 1;
+        nb[lo] = nid[lo] == -2 ? 2.f * p - pv[hi] : pv[lo];
+        nb[hi] = nid[hi] == -2 ? 2.f * p - pv[lo] : pv[hi];
+    }
+    float pf[6];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int lo = 2 * a, hi = 2 * a + 1;
+        pf[lo] = tn[lo] == kSolid ? 2.f * p - nb[hi] : (tn[lo] == kAir ? -nb[lo] : nb[lo]);
+        pf[hi] = tn[hi] == kSolid ? 2.f * p - nb[lo] : (tn[hi] == kAir ? -nb[hi] : nb[hi]);
+    }
+    cx[c] = cx[c] - (pf[1] - pf[0]) * 0.5f;
+    cy[c] = cy[c] - (pf[3] - pf[2]) * 0.5f;
+    cz[c] = cz[c] - (pf[5] - pf[4]) * 0.5f;
+}
+
+// ------------------------------------------------------------------------------------------------
+// G2P + FLIP/PIC blend + advection + wall clamp + removal flag + gravity:
+// FLIPMethod::updateParticles (FLIPMethod.cu:86-171) with SparseField::lerp/lerpMin
+// (SparseField.h:150-196).  No collision source (CollisionSource.h:128-144 with empty fields:
+// signedDistance = 5, so the collision branches are never taken).
+struct G2PParams {
+    float dt;
+    float flip;
+    float pic;
+    float wall;
+    float gx, gy, gz;
+    int max_substeps;
+    int tank;
+    float band;
+};
+
+template <int B>
+__global__ void k_g2p(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz, float* __restrict__ ux,
+                      float* __restrict__ uy, float* __restrict__ uz, float* __restrict__ adj, int np, FrameK f,
+                      const float* __restrict__ cnt, const float* __restrict__ cx, const float* __restrict__ cy,
+                      const float* __restrict__ cz, const float* __restrict__ vx0, const float* __restrict__ vy0,
+                      const float* __restrict__ vz0, G2PParams gp, uint8_t* __restrict__ removed,
+                      int* __restrict__ nremoved) {
+    const int n = blockIdx.x * WG + threadIdx.x;
+    int rem = 0;
+    if (n < np) {
+        float x = px[n], y = py[n], z = pz[n];
+        float vx = ux[n], vy = uy[n], vz = uz[n];
+        // lerp corner (SparseField.h:152-160)
+        const int i = rclamp((int)(x - 0.5f), 0, f.nx - 2);
+        const int j = rclamp((int)(y - 0.5f), 0, f.ny - 2);
+        const int k = rclamp((int)(z - 0.5f), 0, f.nz - 2);
+        const float fxw = rclamp(x - ((float)i + 0.5f), 0.f, 1.f);
+        const float fyw = rclamp(y - ((float)j + 0.5f), 0.f, 1.f);
+        const float fzw = rclamp(z - ((float)k + 0.5f), 0.f, 1.f);
+        int id[8];   // order: (dx,dy,dz) with index dx + 2*dy + 4*dz
+#pragma unroll
+        for (int q = 0; q < 8; ++q) id[q] = cell_index<B>(f, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2));
+        // adjacency = lerpMin of the count field (background 10), SparseField.h:176-196
+        {
+            float cv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) cv[q] = id[q] >= 0 ? cnt[id[q]] : 10.f;
+            float a = rmin(cv[0], cv[1]);
+            a = rmin(a, rmin(cv[4], cv[5]));
+            a = rmin(a, rmin(cv[2], cv[3]));
+            a = rmin(a, rmin(cv[6], cv[7]));
+            adj[n] = a;
+        }
+        auto lerp = [&](const float* fld) -> float {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = id[q] >= 0 ? fld[id[q]] : 0.f;
+            const float c00 = v[0] * (1.f - fxw) + v[1] * fxw;
+            const float c01 = v[4] * (1.f - fxw) + v[5] * fxw;
+            const float c10 = v[2] * (1.f - fxw) + v[3] * fxw;
+            const float c11 = v[6] * (1.f - fxw) + v[7] * fxw;
+            const float c0 = c00 * (1.f - fyw) + c10 * fyw;
+            const float c1 = c01 * (1.f - fyw) + c11 * fyw;
+            return c0 * (1.f - fzw) + c1 * fzw;
+        };
+        // FLIP/PIC blend (FLIPMethod.cu:94-100)
+        const float gvx = lerp(cx), gvy = lerp(cy), gvz = lerp(cz);
+        const float fx_ = gvx - lerp(vx0), fy_ = gvy - lerp(vy0), fz_ = gvz - lerp(vz0);
+        vx += fx_; vy += fy_; vz += fz_;
+        vx = vx * gp.flip + gvx * gp.pic;
+        vy = vy * gp.flip + gvy * gp.pic;
+        vz = vz * gp.flip + gvz * gp.pic;
+        // water-tank damping band (FLIPMethod.cu:102-114)
+        if (gp.tank && y <= gp.wall) {
+            const float lx = (float)f.nx + 0.5f, lz = (float)f.nz + 0.5f, band = gp.band;
+            float w = 1.f;
+            if (x < band) w = rmin(w, rclamp((x - 0.f) / band, 0.f, 1.f));
+            if (x > lx - band) w = rmin(w, rclamp((lx - x) / band, 0.f, 1.f));
+            if (z < band) w = rmin(w, rclamp((z - 0.f) / band, 0.f, 1.f));
+            if (z > lz - band) w = rmin(w, rclamp((lz - z) / band, 0.f, 1.f));
+            vx *= w; vy *= w; vz *= w;
+        }
+        // advection substeps (FLIPMethod.cu:121-159)
+        const float len = sqrtf(vx * vx + vy * vy + vz * vz);
+        const int iter = 1 + rmin((int)(len * gp.dt * 1.5f), rmax(gp.max_substeps, 1));
+        const float sub = gp.dt / (float)iter;
+        const float fnx = (float)f.nx, fny = (float)f.ny, fnz = (float)f.nz;
+        for (int s = 0; s < iter; ++s) {
+            x += vx * sub;
+            y += vy * sub;
+            z += vz * sub;
+            if (y <= gp.wall) {
+                if (x < 0.f) { x = 0.f; if (vx < 0.f) vx = 0.f; }
+                if (y < 0.f) { y = 0.f; if (vy < 0.f) vy = 0.f; }
+                if (z < 0.f) { z = 0.f; if (vz < 0.f) vz = 0.f; }
+                if (x > fnx) { x = fnx; if (vx > 0.f) vx = 0.f; }
+                if (y > fny) { y = fny; if (vy > 0.f) vy = 0.f; }
+                if (z > fnz) { z = fnz; if (vz > 0.f) vz = 0.f; }
+            }
+        }
+        // Grid::inside (FLIPMethod.cu:167)
+        if (x < 0.f || x > fnx || y < 0.f || y > fny || z < 0.f || z > fnz) rem = 1;
+        removed[n] = (uint8_t)rem;
+        // gravity (FLIPMethod.cu:170)
+        vx += gp.gx * gp.dt;
+        vy += gp.gy * gp.dt;
+        vz += gp.gz * gp.dt;
+        px[n] = x; py[n] = y; pz[n] = z;
+        ux[n] = vx; uy[n] = vy; uz[n] = vz;
+    }
+    // wave-aggregated removal count
+    const unsigned long long m = __ballot(rem);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(nremoved, (int)__popcll(m));
+}
+
+// Removal compaction, part 3: survivor src[S-1-k] moves into hole[k] (Array.h:995-1024 closed form:
+// the k-th removed slot below newN, ascending, receives the k-th survivor at or above newN,
+// descending).
+struct PArrays {
+    float* a[9];     // px,py,pz,vx,vy,vz,adj,vort,curv
+};
+__global__ void k_remove_move(int nmove, const int* __restrict__ hole, const int* __restrict__ src, PArrays pa) {
+    const int k = blockIdx.x * WG + threadIdx.x;
+    if (k >= nmove) return;
+    const int d = hole[k];
+    const int s = src[nmove - 1 - k];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) pa.a[q][d] = pa.a[q][s];
+}
+
+// ------------------------------------------------------------------------------------------------
+// Voxel <-> world (Grid.h:431-457), particlesToVoxel / particlesToWorld (FLIPMethod.cu:480-514).
+__global__ void k_to_voxel(int np, PArrays pa, float tx, float ty, float tz, float sx, float sy, float sz) {
+    const int n = blockIdx.x * WG + threadIdx.x;
+    if (n >= np) return;
+    pa.a[0][n] = (pa.a[0][n] - tx) / sx;
+    pa.a[1][n] = (pa.a[1][n] - ty) / sy;
+    pa.a[2][n] = (pa.a[2][n] - tz) / sz;
+    pa.a[3][n] = pa.a[3][n] / sx;
+    pa.a[4][n] = pa.a[4][n] / sy;
+    pa.a[5][n] = pa.a[5][n] / sz;
+}
+__global__ void k_to_world(int np, PArrays pa, float tx, float ty, float tz, float sx, float sy, float sz) {
+    const int n = blockIdx.x * WG + threadIdx.x;
+    if (n >= np) return;
+    pa.a[0][n] = pa.a[0][n] * sx + tx;
+    pa.a[1][n] = pa.a[1][n] * sy + ty;
+    pa.a[2][n] = pa.a[2][n] * sz + tz;
+    pa.a[3][n] = pa.a[3][n] * sx;
+    pa.a[4][n] = pa.a[4][n] * sy;
+    pa.a[5][n] = pa.a[5][n] * sz;
+}
+
+// xyz-interleaved <-> SoA on the device (for device-pointer particle I/O).
+__global__ void k_deinterleave(int n, const float* __restrict__ in, float* __restrict__ a, float* __restrict__ b,
+                               float* __restrict__ c) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    a[i] = in[3 * i]; b[i] = in[3 * i + 1]; c[i] = in[3 * i + 2];
+}
+__global__ void k_interleave(int n, const float* __restrict__ a, const float* __restrict__ b, const float* __restrict__ c,
+                             float* __restrict__ out) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    out[3 * i] = a[i]; out[3 * i + 1] = b[i]; out[3 * i + 2] = c[i];
+}
+
+// Mark field in the reference encoding (uint64: 0 air, 1 solid, 4+fluid rank).
+__global__ void k_mark_u64(int nc, const uint8_t* __restrict__ type, const int* __restrict__ rank,
+                           unsigned long long* __restrict__ out) {
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= nc) return;
+    const uint8_t t = type[c];
+    out[c] = t == kFluid ? (unsigned long long)(4 + rank[c]) : (unsigned long long)t;
+}
+
+}  // namespace vare
