@@ -44,6 +44,7 @@ extern "C" {
 /* preconditioners for VARE_SOLVER_RELATIVE */
 #define VARE_PRECOND_NONE   0
 #define VARE_PRECOND_JACOBI 1
+#define VARE_PRECOND_MG     2   /* geometric multigrid V-cycle on the sparse block layout */
 
 /* Grid description.  Mirrors Bora::Grid (base/header/Grid.h:97-158): cell counts and the world
  * bounding box [min, max].  block_size is the sparse-frame block edge: 4 is the reference's
@@ -78,7 +79,12 @@ typedef struct vare_flip_params {
     float cg_rel_tol;        /* relative mode tolerance, default 1e-5 */
     int max_iterations;      /* relative mode cap, default 2000 */
     int preconditioner;      /* relative mode: VARE_PRECOND_* */
-    int reserved[8];
+    int mg_sweeps;           /* multigrid: Jacobi sweeps before and after the coarse correction (1|2) */
+    float mg_omega;          /* multigrid: Jacobi weight, default 0.8 */
+    int mg_coarse_sweeps;    /* multigrid: sweeps on the coarsest grid, default 40 */
+    int particle_path;       /* 0: block-binned P2G/G2P through LDS tiles (default); 1: one thread per
+                                particle with global atomics / gathers */
+    int reserved[4];
 } vare_flip_params;
 
 /* Frame geometry after SparseFrame::initialize (SparseFrame.cu:72-117): padded dims, block
@@ -176,6 +182,12 @@ int vare_flip_get_stats(const vare_flip* h, vare_step_stats* out);
 int vare_flip_get_block_table(vare_flip* h, int64_t* pointers, int32_t* coords, size_t cap, size_t* nb);
 /* Copy a cell field (VARE_FIELD_*) in storage order; n = number of cells. */
 int vare_flip_get_cell_field(vare_flip* h, int which, void* out, size_t cap_bytes, size_t* n);
+/* Binning permutation of the last prepare stage (particle_path 0): perm[n] lists particle indices
+ * grouped by bucket (bucket b < nb = active block slot b, bucket nb = particles outside the domain),
+ * ascending inside a bucket -- a stable counting sort by bucket.  bstart[nb+2] holds the bucket
+ * starts.  Either pointer may be NULL; n and nbucket (= nb+1) are always written.  No counterpart
+ * in the reference (its particles are never binned). */
+int vare_flip_get_binning(vare_flip* h, int32_t* perm, int32_t* bstart, size_t cap, size_t* n, size_t* nbucket);
 /* Per-particle removal flags of the last step, in the order the particles had before removal. */
 int vare_flip_get_removed_mask(vare_flip* h, uint8_t* out, size_t cap, size_t* n);
 

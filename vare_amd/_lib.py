@@ -28,6 +28,7 @@ SOLVER_REFERENCE = 0
 SOLVER_RELATIVE = 1
 PRECOND_NONE = 0
 PRECOND_JACOBI = 1
+PRECOND_MG = 2
 
 FIELD_MASS = 0
 FIELD_VELOCITY = 1
@@ -69,7 +70,9 @@ class FlipParams(C.Structure):
                 ("pts_per_cell", C.c_int), ("vort_pres_rate", C.c_float), ("adjacency_under", C.c_float),
                 ("proj_iteration", C.c_int), ("redist_iteration", C.c_int), ("flip_ratio", C.c_float),
                 ("solver_mode", C.c_int), ("cg_abs_tol", C.c_float), ("cg_rel_tol", C.c_float),
-                ("max_iterations", C.c_int), ("preconditioner", C.c_int), ("reserved", C.c_int * 8)]
+                ("max_iterations", C.c_int), ("preconditioner", C.c_int), ("mg_sweeps", C.c_int),
+                ("mg_omega", C.c_float), ("mg_coarse_sweeps", C.c_int), ("particle_path", C.c_int),
+                ("reserved", C.c_int * 4)]
 
 
 class FrameInfo(C.Structure):
@@ -113,6 +116,7 @@ SIGNATURES = {
     "vare_flip_get_block_table": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vare_flip_get_cell_field": (C.c_int, [_P, C.c_int, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vare_flip_get_removed_mask": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vare_flip_get_binning": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "vare_seed_box": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_int,
                                 C.c_uint32, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vare_rand": (C.c_float, [C.c_uint32]),
@@ -292,6 +296,15 @@ class FlipHandle:
             out = np.empty(nc, np.float32)
         check(self.lib.vare_flip_get_cell_field(self.h, which, ptr(out), out.nbytes, C.byref(n)))
         return out
+
+    def binning(self):
+        n = C.c_size_t()
+        nbk = C.c_size_t()
+        check(self.lib.vare_flip_get_binning(self.h, None, None, 0, C.byref(n), C.byref(nbk)))
+        perm = np.empty(n.value, np.int32)
+        bstart = np.empty(nbk.value + 1, np.int32)
+        check(self.lib.vare_flip_get_binning(self.h, ptr(perm), ptr(bstart), n.value, C.byref(n), C.byref(nbk)))
+        return perm, bstart
 
     def removed_mask(self):
         n = C.c_size_t()

@@ -14,6 +14,8 @@
 
 #include "../../include/vare_flip.h"
 #include "flip_kernels.h"
+#include "flip_mg.h"
+#include "flip_particles.h"
 
 using namespace vare;
 
@@ -46,6 +48,10 @@ int fail(int code, const char* fmt, ...) {
     } while (0)
 
 inline unsigned nblocks(size_t n) { return (unsigned)((n + WG - 1) / WG); }
+
+// Capped grid for the grid-stride reducing kernels: 8 workgroups of 256 threads per CU.
+constexpr unsigned kReduceGrid = 2048;
+inline unsigned rgrid(size_t n) { return std::max(1u, std::min(nblocks(n), kReduceGrid)); }
 
 template <class T> int dalloc(T** p, size_t n) {
     *p = nullptr;
@@ -83,8 +89,14 @@ struct vare_flip {
     uint8_t* removed = nullptr;
     int* hole = nullptr;
     int* src = nullptr;
+    int* pkey = nullptr;          // bucket (active slot of the particle's block, or Nb)
+    int* prank = nullptr;         // arrival rank inside the bucket
+    int* perm = nullptr;          // binning permutation: particle indices grouped by bucket
+    int* bcount = nullptr;        // [bcap+2]
+    int* bstart = nullptr;        // [bcap+2]
     size_t last_np_before_remove = 0;
     int nremoved = 0;
+    int last_iters = 0;           // solver iterations of the previous step
 
     // frame
     int* flag = nullptr;
@@ -97,12 +109,18 @@ struct vare_flip {
     // cells, capacity bcap*B^3
     float *mass = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *cnt = nullptr;
     float *cx = nullptr, *cy = nullptr, *cz = nullptr;
-    float *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *b = nullptr;
+    float *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *b = nullptr, *z = nullptr;
     uint16_t* info = nullptr;
     uint8_t* type = nullptr;
     int* rank = nullptr;
     double* partials = nullptr;
     double* partials2 = nullptr;
+
+    // multigrid preconditioner: level 0 aliases r/info/z of the CG; zt0 is its second buffer
+    std::vector<MgLevelK> mg;
+    std::vector<float*> mg_z, mg_zt;
+    float* zt0 = nullptr;
+    size_t mg_bcap = 0;
 
     // scan scratch
     int* tiles = nullptr;
@@ -129,11 +147,27 @@ struct vare_flip {
 
 namespace {
 
+void free_mg(vare_flip* h) {
+    for (size_t l = 1; l < h->mg.size(); ++l) {
+        dfree(h->mg[l].r);
+        dfree(h->mg[l].info);
+        dfree(h->mg_z[l]);
+        dfree(h->mg_zt[l]);
+    }
+    dfree(h->zt0);
+    h->mg.clear();
+    h->mg_z.clear();
+    h->mg_zt.clear();
+    h->mg_bcap = 0;
+}
+
 void free_cells(vare_flip* h) {
+    free_mg(h);
     dfree(h->nbr);
+    dfree(h->bcount); dfree(h->bstart);
     dfree(h->mass); dfree(h->vx); dfree(h->vy); dfree(h->vz); dfree(h->cnt);
     dfree(h->cx); dfree(h->cy); dfree(h->cz);
-    dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b);
+    dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b); dfree(h->z);
     dfree(h->info); dfree(h->type); dfree(h->rank);
     dfree(h->partials); dfree(h->partials2);
     h->bcap = 0;
@@ -146,13 +180,14 @@ int ensure_blocks(vare_flip* h, size_t nb) {
     cap = std::max(cap, nb);
     free_cells(h);
     const size_t nc = cap * (size_t)h->B * h->B * h->B;
-    const size_t npart = (nc + WG - 1) / WG + 1;
+    const size_t npart = (size_t)kReduceGrid + 16;
     CHK(dalloc(&h->nbr, cap * 6));
+    CHK(dalloc(&h->bcount, cap + 2)); CHK(dalloc(&h->bstart, cap + 2));
     CHK(dalloc(&h->mass, nc)); CHK(dalloc(&h->vx, nc)); CHK(dalloc(&h->vy, nc)); CHK(dalloc(&h->vz, nc));
     CHK(dalloc(&h->cnt, nc));
     CHK(dalloc(&h->cx, nc)); CHK(dalloc(&h->cy, nc)); CHK(dalloc(&h->cz, nc));
     CHK(dalloc(&h->x, nc)); CHK(dalloc(&h->r, nc)); CHK(dalloc(&h->p, nc)); CHK(dalloc(&h->ap, nc));
-    CHK(dalloc(&h->b, nc));
+    CHK(dalloc(&h->b, nc)); CHK(dalloc(&h->z, nc));
     CHK(dalloc(&h->info, nc)); CHK(dalloc(&h->type, nc)); CHK(dalloc(&h->rank, nc));
     CHK(dalloc(&h->partials, npart)); CHK(dalloc(&h->partials2, npart));
     h->bcap = cap;
@@ -172,7 +207,9 @@ int ensure_particles(vare_flip* h, size_t n) {
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int q = 0; q < 9; ++q) { dfree(h->pa[q]); h->pa[q] = na[q]; }
     dfree(h->removed); dfree(h->hole); dfree(h->src);
+    dfree(h->pkey); dfree(h->prank); dfree(h->perm);
     CHK(dalloc(&h->removed, cap)); CHK(dalloc(&h->hole, cap)); CHK(dalloc(&h->src, cap));
+    CHK(dalloc(&h->pkey, cap)); CHK(dalloc(&h->prank, cap)); CHK(dalloc(&h->perm, cap));
     h->pcap = cap;
     return VARE_OK;
 }
@@ -266,11 +303,33 @@ __global__ void k_cg_setup(CgState* st, int mode, int max_count, float abs_tol, 
     st->abs_tol = abs_tol;
     st->rel_tol = rel_tol;
     st->precond = precond;
-    st->ticket = 0;
     st->done = 0;
     st->iter = 0;
     st->count = 0;
     st->converged = 0;
+}
+
+// Binning permutation: bucket every particle by the active slot of its block (bucket Nb for
+// particles outside the domain), counting sort, then sort each bucket by particle index so the
+// permutation equals a stable counting sort of the particle order.
+template <int B> int bin_particles(vare_flip* h) {
+    const int nb = h->nb;
+    CHK(fill_u32(h, h->bcount, 0, (size_t)nb + 2));
+    if (h->np)
+        hipLaunchKernelGGL(k_pkey<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (const float*)h->pa[0],
+                           (const float*)h->pa[1], (const float*)h->pa[2], (int)h->np, h->frame(), nb, h->pkey, h->prank,
+                           h->bcount);
+    OpBuckets op{h->bcount, h->bstart};
+    CHK(run_scan(h, op, nb + 2, 3));
+    if (h->np) {
+        hipLaunchKernelGGL(k_pscatter, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, (const int*)h->pkey,
+                           (const int*)h->prank, (const int*)h->bstart, h->perm);
+        CHK(fill_u32(h, h->dint + 4, 0, 1));
+        hipLaunchKernelGGL(k_psort, dim3(nb + 1), dim3(1024), 0, h->stream, (const int*)h->bstart, nb + 1, (int)h->np,
+                           h->perm, h->dint + 4);
+    }
+    HIPCHK(hipGetLastError());
+    return VARE_OK;
 }
 
 // ---- stages ----
@@ -297,6 +356,7 @@ template <int B> int stage_prepare(vare_flip* h) {
         hipLaunchKernelGGL(k_block_nbr, dim3(nblocks((size_t)nb * 6)), dim3(WG), 0, h->stream, nb, h->fx, h->fy, h->fz,
                            (const int*)h->coord, (const int*)h->slot, h->nbr);
     HIPCHK(hipGetLastError());
+    if (h->prm.particle_path == 0) CHK(bin_particles<B>(h));
     return VARE_OK;
 }
 
@@ -306,7 +366,11 @@ template <int B> int stage_p2g(vare_flip* h) {
     CHK(fill_u32(h, h->vz, 0, nc)); CHK(fill_u32(h, h->cnt, 0, nc));
     CellFields cf{h->mass, h->vx, h->vy, h->vz, h->cnt};
     tbegin(h, VARE_KERNEL_P2G);
-    if (h->np)
+    if (h->np && h->prm.particle_path == 0 && h->nb)
+        hipLaunchKernelGGL(k_p2g_tile<B>, dim3(h->nb), dim3(WG), 0, h->stream, (const float*)h->pa[0],
+                           (const float*)h->pa[1], (const float*)h->pa[2], (const float*)h->pa[3], (const float*)h->pa[4],
+                           (const float*)h->pa[5], (const int*)h->perm, (const int*)h->bstart, h->frame(), cf);
+    else if (h->np && h->prm.particle_path != 0)
         hipLaunchKernelGGL(k_p2g_atomic<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
                            h->pa[3], h->pa[4], h->pa[5], (int)h->np, h->frame(), cf);
     tend(h, VARE_KERNEL_P2G);
@@ -317,7 +381,10 @@ template <int B> int stage_p2g(vare_flip* h) {
     return VARE_OK;
 }
 
-CgVecs vecs(vare_flip* h) { return CgVecs{h->x, h->r, h->p, h->ap, h->b, h->info}; }
+CgVecs vecs(vare_flip* h) { return CgVecs{h->x, h->r, h->p, h->ap, h->b, h->z, h->info}; }
+
+int mg_setup(vare_flip* h);
+void enqueue_vcycle(vare_flip* h);
 
 template <int B> int stage_system(vare_flip* h) {
     const size_t nc = h->ncells();
@@ -326,55 +393,183 @@ template <int B> int stage_system(vare_flip* h) {
     const int maxc = mode == 0 ? h->prm.proj_iteration : h->prm.max_iterations;
     hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(64), 0, h->stream, h->st, mode, maxc, h->prm.cg_abs_tol,
                        h->prm.cg_rel_tol, precond);
-    const unsigned g = nblocks(std::max<size_t>(nc, 1));
+    const unsigned g = rgrid(nc);
     tbegin(h, VARE_KERNEL_DIV);
     hipLaunchKernelGGL(k_divergence<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const uint8_t*)h->type,
                        (const float*)h->cx, (const float*)h->cy, (const float*)h->cz, h->vox_wall, vecs(h), h->partials,
-                       h->partials2, h->st);
+                       h->partials2);
     tend(h, VARE_KERNEL_DIV);
-    if (mode == 1)
-        hipLaunchKernelGGL(k_cg_precond_init, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials, h->st);
+    hipLaunchKernelGGL(k_fin_divergence, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+                       (const double*)h->partials2, (int)g);
+    if (mode == 1) {
+        if (precond == VARE_PRECOND_MG) {
+            CHK(mg_setup(h));
+            enqueue_vcycle(h);
+        }
+        hipLaunchKernelGGL(k_cg_precond_init, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials,
+                           (const CgState*)h->st);
+        hipLaunchKernelGGL(k_fin_precond_init, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+    }
     HIPCHK(hipGetLastError());
     return VARE_OK;
 }
 
+// ---- multigrid preconditioner (VARE_PRECOND_MG) ----
+// Levels: sparse edges B, B/2, ..., 1 over the active blocks, then dense halvings of the block
+// frame until a level has at most kMgCoarseMax cells.  Arrays are sized for the block capacity.
+int ensure_mg(vare_flip* h) {
+    if (!h->mg.empty() && h->mg_bcap == h->bcap) return VARE_OK;
+    free_mg(h);
+    const int sh0 = h->B == 8 ? 3 : 2;
+    const size_t cap = h->bcap;
+    MgLevelK L0{};
+    L0.dense = 0; L0.sh = sh0; L0.dx = h->fx; L0.dy = h->fy; L0.dz = h->fz; L0.scale_sh = 0;
+    h->mg.push_back(L0);
+    h->mg_z.push_back(nullptr);
+    h->mg_zt.push_back(nullptr);
+    CHK(dalloc(&h->zt0, cap << (3 * sh0)));
+    for (int sh = sh0 - 1; sh >= 0; --sh) {
+        MgLevelK L{};
+        L.dense = 0; L.sh = sh; L.dx = h->fx; L.dy = h->fy; L.dz = h->fz; L.scale_sh = sh0 - sh;
+        const size_t n = cap << (3 * sh);
+        CHK(dalloc(&L.r, n)); CHK(dalloc(&L.info, n));
+        float *z = nullptr, *zt = nullptr;
+        CHK(dalloc(&z, n)); CHK(dalloc(&zt, n));
+        h->mg.push_back(L); h->mg_z.push_back(z); h->mg_zt.push_back(zt);
+    }
+    int dx = h->fx, dy = h->fy, dz = h->fz, ssh = sh0;
+    do {
+        dx = (dx + 1) / 2; dy = (dy + 1) / 2; dz = (dz + 1) / 2; ++ssh;
+        MgLevelK L{};
+        L.dense = 1; L.sh = 0; L.dx = dx; L.dy = dy; L.dz = dz; L.scale_sh = ssh;
+        L.ncell = dx * dy * dz;
+        CHK(dalloc(&L.r, (size_t)L.ncell)); CHK(dalloc(&L.info, (size_t)L.ncell));
+        float *z = nullptr, *zt = nullptr;
+        CHK(dalloc(&z, (size_t)L.ncell)); CHK(dalloc(&zt, (size_t)L.ncell));
+        h->mg.push_back(L); h->mg_z.push_back(z); h->mg_zt.push_back(zt);
+    } while ((size_t)dx * dy * dz > (size_t)kMgCoarseMax);
+    h->mg_bcap = h->bcap;
+    return VARE_OK;
+}
+
+MgFrame mg_frame(const vare_flip* h) { return MgFrame{h->slot, h->coord, h->nbr, h->vox_wall}; }
+const int* done_ptr(const vare_flip* h) { return &h->st->done; }
+
+// Per solve: level sizes for the current block count, coarse fluid flags and stencils.
+int mg_setup(vare_flip* h) {
+    CHK(ensure_mg(h));
+    const MgFrame F = mg_frame(h);
+    MgLevelK& L0 = h->mg[0];
+    L0.ncell = (int)h->ncells();
+    L0.r = h->r;
+    L0.info = h->info;
+    h->mg_z[0] = h->z;
+    h->mg_zt[0] = h->zt0;
+    for (size_t l = 1; l < h->mg.size(); ++l) {
+        MgLevelK& L = h->mg[l];
+        if (!L.dense) L.ncell = h->nb << (3 * L.sh);
+        if (L.ncell)
+            hipLaunchKernelGGL(k_mg_flag, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, h->mg[l - 1], L, F);
+        if (L.ncell) hipLaunchKernelGGL(k_mg_info, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F);
+    }
+    HIPCHK(hipGetLastError());
+    return VARE_OK;
+}
+
+// z = V-cycle(r) into the CG's z vector.
+void enqueue_vcycle(vare_flip* h) {
+    const MgFrame F = mg_frame(h);
+    const float w = h->prm.mg_omega;
+    const int nu = h->prm.mg_sweeps;
+    const int* done = done_ptr(h);
+    const int m = (int)h->mg.size() - 1;
+    for (int l = 0; l < m; ++l) {
+        const MgLevelK& L = h->mg[l];
+        const MgLevelK& C = h->mg[l + 1];
+        if (nu == 1) {
+            if (C.ncell)
+                hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(C.ncell)), dim3(WG), 0, h->stream, L, C, F, w, 1,
+                                   h->mg_zt[l], done);
+        } else {
+            if (L.ncell)
+                hipLaunchKernelGGL(k_mg_smooth_first2, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F, w,
+                                   h->mg_z[l], done);
+            if (C.ncell)
+                hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(C.ncell)), dim3(WG), 0, h->stream, L, C, F, w, 0,
+                                   h->mg_z[l], done);
+        }
+    }
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, h->stream, h->mg[m], w, h->prm.mg_coarse_sweeps,
+                       h->mg_z[m], done);
+    for (int l = m - 1; l >= 0; --l) {
+        const MgLevelK& L = h->mg[l];
+        const MgLevelK& C = h->mg[l + 1];
+        if (!L.ncell) continue;
+        if (nu == 1) {
+            hipLaunchKernelGGL(k_mg_prolong_smooth, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, C, F, w,
+                               (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done);
+        } else {
+            hipLaunchKernelGGL(k_mg_prolong_smooth, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, C, F, w,
+                               (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done);
+            hipLaunchKernelGGL(k_mg_smooth, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F, w,
+                               (const float*)h->mg_zt[l], h->mg_z[l], done);
+        }
+    }
+}
+
+// One CG/PCG iteration (LinearSystemSolver.cpp:29-46); every kernel returns at once when the
+// device-side `done` flag is set, so the host can enqueue iterations ahead without syncing.
+template <int B> void enqueue_iteration(vare_flip* h, size_t nc, unsigned g) {
+    tbegin(h, VARE_KERNEL_SPMV);
+    hipLaunchKernelGGL(k_cg_spmv<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), vecs(h), h->partials,
+                       (const CgState*)h->st);
+    tend(h, VARE_KERNEL_SPMV);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+    tbegin(h, VARE_KERNEL_UPDATE);
+    hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials, h->partials2,
+                       (const CgState*)h->st);
+    tend(h, VARE_KERNEL_UPDATE);
+    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+                       (const double*)h->partials2, (int)g);
+    if (h->prm.solver_mode == VARE_SOLVER_RELATIVE && h->prm.preconditioner == VARE_PRECOND_MG) {
+        enqueue_vcycle(h);
+        hipLaunchKernelGGL(k_cg_rz, dim3(g), dim3(WG), 0, h->stream, (int)nc, (const float*)h->r, (const float*)h->z,
+                           h->partials, (const CgState*)h->st);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+    }
+    tbegin(h, VARE_KERNEL_PUPDATE);
+    hipLaunchKernelGGL(k_cg_pupdate, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), (const CgState*)h->st);
+    tend(h, VARE_KERNEL_PUPDATE);
+}
+
 template <int B> int stage_solve(vare_flip* h) {
     const size_t nc = h->ncells();
-    const unsigned g = nblocks(std::max<size_t>(nc, 1));
+    const unsigned g = rgrid(nc);
     const int mode = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? 1 : 0;
     const int cap = mode == 0 ? h->prm.proj_iteration + 2 : h->prm.max_iterations;
     HIPCHK(hipEventRecord(h->step_ev[2], h->stream));
     int launched = 0;
-    int chunk = 8;
+    // The iteration count changes slowly from step to step: enqueue the previous count (+1) before
+    // the first host check, then small chunks.  Iterations enqueued past convergence return at once.
+    int chunk = h->last_iters > 0 ? h->last_iters + 1 : 8;
     for (;;) {
         HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         if (h->st_host->done) break;
-        if (mode == 0 && launched >= h->st_host->loop_limit + 1) break;   // defensive: cannot be reached
-        if (launched > cap + 1) break;                                     // defensive: cannot be reached
-        for (int q = 0; q < chunk; ++q) {
-            tbegin(h, VARE_KERNEL_SPMV);
-            hipLaunchKernelGGL(k_cg_spmv<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), vecs(h), h->partials,
-                               h->st);
-            tend(h, VARE_KERNEL_SPMV);
-            tbegin(h, VARE_KERNEL_UPDATE);
-            hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials, h->partials2,
-                               h->st);
-            tend(h, VARE_KERNEL_UPDATE);
-            tbegin(h, VARE_KERNEL_PUPDATE);
-            hipLaunchKernelGGL(k_cg_pupdate, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), (const CgState*)h->st);
-            tend(h, VARE_KERNEL_PUPDATE);
-        }
+        if (launched > cap + 1) break;                       // defensive: the device flag ends first
+        for (int q = 0; q < chunk; ++q) enqueue_iteration<B>(h, nc, g);
         HIPCHK(hipGetLastError());
         launched += chunk;
-        chunk = std::min(chunk * 2, 64);
+        chunk = h->last_iters > 0 ? 4 : std::min(chunk * 2, 64);
     }
+    h->last_iters = h->st_host->count;
+    const unsigned ga = nblocks(std::max<size_t>(nc, 1));
     tbegin(h, VARE_KERNEL_APPLY);
     if (nc) {
-        hipLaunchKernelGGL(k_press_extrap<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const uint8_t*)h->type,
-                           h->x);
-        hipLaunchKernelGGL(k_press_grad<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const uint8_t*)h->type,
-                           (const float*)h->x, h->vox_wall, h->cx, h->cy, h->cz);
+        hipLaunchKernelGGL(k_press_extrap<B>, dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+                           (const uint8_t*)h->type, h->x);
+        hipLaunchKernelGGL(k_press_grad<B>, dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+                           (const uint8_t*)h->type, (const float*)h->x, h->vox_wall, h->cx, h->cy, h->cz);
     }
     tend(h, VARE_KERNEL_APPLY);
     HIPCHK(hipGetLastError());
@@ -391,7 +586,12 @@ template <int B> int stage_solve(vare_flip* h) {
 template <int B> int stage_g2p(vare_flip* h, float dt) {
     CHK(fill_u32(h, h->dint + 1, 0, 1));
     tbegin(h, VARE_KERNEL_G2P);
-    if (h->np)
+    if (h->np && h->prm.particle_path == 0) {
+        G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
+        hipLaunchKernelGGL(k_g2p_tile<B>, dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
+                           h->pa[3], h->pa[4], h->pa[5], h->pa[6], (const int*)h->perm, (const int*)h->bstart, h->nb,
+                           h->frame(), gf, g2p_params(h, dt), h->removed, h->dint + 1);
+    } else if (h->np)
         hipLaunchKernelGGL(k_g2p<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2], h->pa[3],
                            h->pa[4], h->pa[5], h->pa[6], (int)h->np, h->frame(), (const float*)h->cnt, (const float*)h->cx,
                            (const float*)h->cy, (const float*)h->cz, (const float*)h->vx, (const float*)h->vy,
@@ -459,10 +659,14 @@ int validate_params(const vare_flip_params* p) {
     if (p->enable_curl) return fail(VARE_ENOTIMPL, "enable_curl: curl noise is outside the built path");
     if (p->solver_mode != VARE_SOLVER_REFERENCE && p->solver_mode != VARE_SOLVER_RELATIVE)
         return fail(VARE_EINVAL, "bad solver_mode %d", p->solver_mode);
-    if (p->preconditioner != VARE_PRECOND_NONE && p->preconditioner != VARE_PRECOND_JACOBI)
+    if (p->preconditioner != VARE_PRECOND_NONE && p->preconditioner != VARE_PRECOND_JACOBI &&
+        p->preconditioner != VARE_PRECOND_MG)
         return fail(VARE_EINVAL, "bad preconditioner %d", p->preconditioner);
     if (p->voxel_size <= 0.f) return fail(VARE_EINVAL, "voxel_size must be positive");
     if (p->max_iterations < 1 || p->proj_iteration < 0) return fail(VARE_EINVAL, "bad iteration cap");
+    if (p->preconditioner == VARE_PRECOND_MG &&
+        (p->mg_sweeps < 1 || p->mg_sweeps > 2 || !(p->mg_omega > 0.f && p->mg_omega < 1.f) || p->mg_coarse_sweeps < 1))
+        return fail(VARE_EINVAL, "bad multigrid settings");
     return VARE_OK;
 }
 
@@ -503,6 +707,9 @@ int vare_flip_params_default(vare_flip_params* p) {
     p->cg_rel_tol = 1e-5f;
     p->max_iterations = 2000;
     p->preconditioner = VARE_PRECOND_NONE;
+    p->mg_sweeps = 2;
+    p->mg_omega = 0.8f;
+    p->mg_coarse_sweeps = 40;
     return VARE_OK;
 }
 
@@ -856,6 +1063,21 @@ int vare_flip_get_cell_field(vare_flip* h, int which, void* out, size_t cap_byte
         return rc;
     }
     return fail(VARE_EINVAL, "bad field id %d", which);
+}
+
+int vare_flip_get_binning(vare_flip* h, int32_t* perm, int32_t* bstart, size_t cap, size_t* n, size_t* nbucket) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (n) *n = h->np;
+    if (nbucket) *nbucket = (size_t)h->nb + 1;
+    if (h->prm.particle_path != 0) return fail(VARE_EINVAL, "binning is only built on particle_path 0");
+    if (perm) {
+        if (cap < h->np) return fail(VARE_EINVAL, "capacity too small");
+        if (h->np) HIPCHK(hipMemcpy(perm, h->perm, h->np * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    if (bstart) HIPCHK(hipMemcpy(bstart, h->bstart, ((size_t)h->nb + 2) * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h->dint_host + 4, h->dint + 4, sizeof(int), hipMemcpyDeviceToHost));
+    if (h->dint_host[4] != 0) return fail(VARE_ENOTIMPL, "%d bucket(s) longer than %d were left unsorted", h->dint_host[4], kSortMax);
+    return VARE_OK;
 }
 
 int vare_flip_get_removed_mask(vare_flip* h, uint8_t* out, size_t cap, size_t* n) {

@@ -140,32 +140,4 @@ __device__ __forceinline__ int block_exscan(int v, int* sh /*[4]*/, int* tot) {
     return off + x - v;
 }
 
-// Agent-scope coherent load of a value written by another workgroup of the same kernel.
-template <class T> __device__ __forceinline__ T load_agent(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// "Last workgroup" detection for single-pass grid reductions: each workgroup stores its partial,
-// then takes a ticket; the workgroup that draws the last ticket sees every partial.
-__device__ __forceinline__ bool last_block(unsigned* ticket, unsigned nblocks, int* sh_flag) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        *sh_flag = (t == nblocks - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    const bool last = *sh_flag != 0;
-    if (last) __threadfence();
-    return last;
-}
-
-// Deterministic sum of partials[0..n) by one workgroup (fixed order: strided per thread, then a
-// fixed tree).
-__device__ __forceinline__ double sum_partials(const double* partials, int n, double* sh) {
-    double s = 0.0;
-    for (int q = threadIdx.x; q < n; q += WG) s += load_agent(partials + q);
-    return block_sum(s, sh);
-}
-
 }  // namespace vare

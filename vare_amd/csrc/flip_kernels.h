@@ -232,12 +232,17 @@ __global__ void k_cell_finalize(int nc, CellFields cf, float* __restrict__ cx, f
 // Divergence + stencil descriptor: FluidProjection::buildLinearSystem K6 (FluidProjection.cu:203-267),
 // matrix-free: the ELL row [diag, -1 per fluid neighbour] is encoded in 2 bytes.  Also initialises the
 // CG vectors (x = 0, r = b - A*0 = b, p = r; LinearSystemSolver.cpp:17-19) and reduces |b|^2.
+//
+// Reductions: every reducing kernel runs a capped grid (grid-stride loop), each workgroup writes one
+// FP64 partial, and a one-workgroup finalize kernel sums the partials in a fixed order (deterministic)
+// and updates the device-resident solver scalars.  No atomics or device-scope fences are needed.
 struct CgVecs {
     float* x;
     float* r;
     float* p;
     float* ap;
     float* b;
+    float* z;         // preconditioned residual (multigrid); unused for none/Jacobi
     uint16_t* info;
 };
 
@@ -254,25 +259,23 @@ struct CgState {
     int done;
     int count;        // return value of LinearSystemSolver::cg
     int converged;
-    unsigned ticket;
     int mode;         // 0 reference, 1 relative
     int max_count;    // reference: params.projIteration; relative: max iterations
     int loop_limit;   // reference: b.size()+5
     float abs_tol;
     float rel_tol;
-    int precond;
+    int precond;      // 0 none, 1 Jacobi, 2 multigrid
     int nfluid;       // fluid cells (rows of the system)
+    int pad;
 };
 
 template <int B>
 __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type, const float* __restrict__ cx,
                              const float* __restrict__ cy, const float* __restrict__ cz, float wall_level, CgVecs v,
-                             double* __restrict__ partials, double* __restrict__ partials2, CgState* __restrict__ st) {
+                             double* __restrict__ partials, double* __restrict__ partials2) {
     __shared__ double sh[4];
-    __shared__ int sflag;
-    const int c = blockIdx.x * WG + threadIdx.x;
     double bb = 0.0, nfl = 0.0;
-    if (c < nc) {
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
         float d = 0.f;
         uint16_t info = 0;
         if (type[c] == kFluid) {
@@ -286,7 +289,7 @@ __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type,
                 nid[q] = cell_neighbor<B>(f, c, q);
                 tn[q] = nid[q] == -2 ? wall : (nid[q] < 0 ? (uint8_t)kAir : type[nid[q]]);
             }
-            const float vx = cx[c], vy = cy[c], vz = cz[c];
+            const float own[3] = {cx[c], cy[c], cz[c]};
             auto comp = [&](int q) -> float {       // neighbour velocity component along q's axis
                 const int id = nid[q];
                 const float* a = q < 2 ? cx : (q < 4 ? cy : cz);
@@ -294,7 +297,6 @@ __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type,
             };
             // neighbour velocities with domain-boundary extrapolation (FluidProjection.cu:220-225)
             float nv[6];
-            const float own[3] = {vx, vy, vz};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const int lo = 2 * a, hi = 2 * a + 1;
@@ -319,7 +321,7 @@ __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type,
                 else mask |= (uint16_t)(1u << q);
             }
             info = (uint16_t)(kInfoFluid | mask | (diag << 8));
-            nfl = 1.0;
+            nfl += 1.0;
         }
         v.b[c] = d;
         v.info[c] = info;
@@ -327,32 +329,41 @@ __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type,
         v.r[c] = d;
         v.p[c] = d;
         v.ap[c] = 0.f;
-        bb = (double)d * (double)d;
+        bb += (double)d * (double)d;
     }
     const double s = block_sum(bb, sh);
     const double s2 = block_sum(nfl, sh);
     if (threadIdx.x == 0) { partials[blockIdx.x] = s; partials2[blockIdx.x] = s2; }
-    if (last_block(&st->ticket, gridDim.x, &sflag)) {
-        const double tot = sum_partials(partials, gridDim.x, sh);
-        const double tot2 = sum_partials(partials2, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            st->ticket = 0;
-            st->nfluid = (int)tot2;
-            st->loop_limit = (int)tot2 + 5;     // LinearSystemSolver.cpp:25-27: i < b.size()+5
-            st->bb = tot;
-            st->rr = tot;
-            st->iter = 0;
-            st->count = 0;
-            st->converged = 0;
-            st->done = 0;
-            const float rs = (float)tot;
-            st->rsold = rs;
-            if (st->mode == 0) {
-                if (sqrtf(rs) < 1e-30f) { st->done = 1; st->converged = 1; }    // LinearSystemSolver.cpp:23
-            } else {
-                if (tot == 0.0) { st->done = 1; st->converged = 1; }
-            }
-        }
+}
+
+// Deterministic sum of n partials by one 256-thread workgroup.
+__device__ __forceinline__ double fin_sum(const double* __restrict__ partials, int n, double* sh) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += WG) s += partials[q];
+    return block_sum(s, sh);
+}
+
+__global__ void k_fin_divergence(CgState* __restrict__ st, const double* __restrict__ partials,
+                                 const double* __restrict__ partials2, int n) {
+    __shared__ double sh[4];
+    const double tot = fin_sum(partials, n, sh);
+    const double tot2 = fin_sum(partials2, n, sh);
+    if (threadIdx.x != 0) return;
+    st->nfluid = (int)tot2;
+    st->loop_limit = (int)tot2 + 5;     // LinearSystemSolver.cpp:25-27: i < b.size()+5
+    st->bb = tot;
+    st->rr = tot;
+    st->iter = 0;
+    st->count = 0;
+    st->converged = 0;
+    st->done = 0;
+    const float rs = (float)tot;
+    st->rsold = rs;
+    st->rzold = rs;
+    if (st->mode == 0) {
+        if (sqrtf(rs) < 1e-30f) { st->done = 1; st->converged = 1; }    // LinearSystemSolver.cpp:23
+    } else {
+        if (tot == 0.0) { st->done = 1; st->converged = 1; }
     }
 }
 
@@ -362,40 +373,39 @@ __device__ __forceinline__ float jacobi_inv(uint16_t info) {
     return ((info & kInfoFluid) && d > 0) ? 1.f / (float)d : 0.f;
 }
 
-// Relative mode set-up: z = M^-1 r, p = z, rz = r.z (after k_divergence).
-__global__ void k_cg_precond_init(int nc, CgVecs v, double* __restrict__ partials, CgState* __restrict__ st) {
+// Relative mode set-up (after k_divergence): z = M^-1 r, p = z, partial r.z.  For multigrid the
+// V-cycle has already written z.
+__global__ void k_cg_precond_init(int nc, CgVecs v, double* __restrict__ partials, const CgState* __restrict__ st) {
     __shared__ double sh[4];
-    __shared__ int sflag;
-    const int c = blockIdx.x * WG + threadIdx.x;
     double rz = 0.0;
-    if (c < nc) {
+    const int pc = st->precond;
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
         const float r = v.r[c];
-        const float z = st->precond == 1 ? r * jacobi_inv(v.info[c]) : r;
+        const float z = pc == 1 ? r * jacobi_inv(v.info[c]) : (pc == 2 ? v.z[c] : r);
         v.p[c] = z;
-        rz = (double)r * (double)z;
+        rz += (double)r * (double)z;
     }
     const double s = block_sum(rz, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
-    if (last_block(&st->ticket, gridDim.x, &sflag)) {
-        const double tot = sum_partials(partials, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            st->ticket = 0;
-            st->rz = tot;
-            st->rzold = (float)tot;
-        }
-    }
 }
 
-// CG step 1: Ap = A p (BLAS::mul, BLAS.cu:12-40, matrix-free 7-point row in the reference's
-// term order) and p.Ap -> alpha (LinearSystemSolver.cpp:29-31).
-template <int B>
-__global__ void k_cg_spmv(int nc, FrameK f, CgVecs v, double* __restrict__ partials, CgState* __restrict__ st) {
+__global__ void k_fin_precond_init(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
     __shared__ double sh[4];
-    __shared__ int sflag;
     if (st->done) return;
-    const int c = blockIdx.x * WG + threadIdx.x;
+    const double tot = fin_sum(partials, n, sh);
+    if (threadIdx.x != 0) return;
+    st->rz = tot;
+    st->rzold = (float)tot;
+}
+
+// CG step 1: Ap = A p (BLAS::mul, BLAS.cu:12-40, matrix-free 7-point row in the reference's term
+// order) and partial p.Ap.
+template <int B>
+__global__ void k_cg_spmv(int nc, FrameK f, CgVecs v, double* __restrict__ partials, const CgState* __restrict__ st) {
+    __shared__ double sh[4];
+    if (st->done) return;
     double pap = 0.0;
-    if (c < nc) {
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
         const uint16_t info = v.info[c];
         float acc = 0.f;
         if (info & kInfoFluid) {
@@ -404,87 +414,112 @@ __global__ void k_cg_spmv(int nc, FrameK f, CgVecs v, double* __restrict__ parti
 #pragma unroll
             for (int q = 0; q < 6; ++q)
                 if (info & (1u << q)) acc = acc - v.p[cell_neighbor<B>(f, c, q)];
-            pap = (double)pc * (double)acc;
+            pap += (double)pc * (double)acc;
         }
         v.ap[c] = acc;
     }
     const double s = block_sum(pap, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
-    if (last_block(&st->ticket, gridDim.x, &sflag)) {
-        const double tot = sum_partials(partials, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            st->ticket = 0;
-            st->pap = tot;
-            if (st->mode == 0) st->alpha = st->rsold / (float)tot;
-            else st->alpha = st->rzold / (float)tot;
-        }
-    }
 }
 
-// CG step 2: x += alpha p, r -= alpha Ap, r.r (and r.z), convergence test and beta
-// (LinearSystemSolver.cpp:33-46).
-__global__ void k_cg_update(int nc, CgVecs v, double* __restrict__ partials, double* __restrict__ partials2,
-                            CgState* __restrict__ st) {
+// alpha = rsold / p.Ap  (LinearSystemSolver.cpp:31)
+__global__ void k_fin_alpha(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
     __shared__ double sh[4];
-    __shared__ int sflag;
+    if (st->done) return;
+    const double tot = fin_sum(partials, n, sh);
+    if (threadIdx.x != 0) return;
+    st->pap = tot;
+    st->alpha = (st->mode == 0 ? st->rsold : st->rzold) / (float)tot;
+}
+
+// CG step 2: x += alpha p, r -= alpha Ap (LinearSystemSolver.cpp:33-35), partial r.r (and r.z for
+// Jacobi).
+__global__ void k_cg_update(int nc, CgVecs v, double* __restrict__ partials, double* __restrict__ partials2,
+                            const CgState* __restrict__ st) {
+    __shared__ double sh[4];
     if (st->done) return;
     const float alpha = st->alpha;
     const int precond = st->precond;
-    const int c = blockIdx.x * WG + threadIdx.x;
     double rr = 0.0, rz = 0.0;
-    if (c < nc) {
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
         const float p = v.p[c];
         v.x[c] = v.x[c] + (alpha * p);
         const float r = v.r[c] - (alpha * v.ap[c]);
         v.r[c] = r;
-        rr = (double)r * (double)r;
-        if (precond == 1) rz = (double)r * (double)(r * jacobi_inv(v.info[c]));
+        rr += (double)r * (double)r;
+        if (precond == 1) rz += (double)r * (double)(r * jacobi_inv(v.info[c]));
     }
     const double s = block_sum(rr, sh);
     const double s2 = precond == 1 ? block_sum(rz, sh) : 0.0;
     if (threadIdx.x == 0) { partials[blockIdx.x] = s; partials2[blockIdx.x] = s2; }
-    if (last_block(&st->ticket, gridDim.x, &sflag)) {
-        const double tot = sum_partials(partials, gridDim.x, sh);
-        const double tot2 = precond == 1 ? sum_partials(partials2, gridDim.x, sh) : 0.0;
-        if (threadIdx.x == 0) {
-            st->ticket = 0;
-            st->rr = tot;
-            const int i = st->iter;
-            st->iter = i + 1;
-            if (st->mode == 0) {
-                const float rsnew = (float)tot;
-                if (sqrtf(rsnew) < st->abs_tol) {
-                    st->done = 1; st->count = i; st->converged = 1;
-                } else {
-                    st->beta = rsnew / st->rsold;
-                    st->rsold = rsnew;
-                    if (i > st->max_count) { st->done = 1; st->count = i; }
-                    else if (i + 1 >= st->loop_limit) { st->done = 1; st->count = i + 1; }
-                }
-            } else {
-                if (sqrt(tot) <= (double)st->rel_tol * sqrt(st->bb)) {
-                    st->done = 1; st->count = i + 1; st->converged = 1;
-                } else if (i + 1 >= st->max_count) {
-                    st->done = 1; st->count = i + 1;
-                } else {
-                    const float rznew = precond == 1 ? (float)tot2 : (float)tot;
-                    st->beta = rznew / st->rzold;
-                    st->rzold = rznew;
-                }
-            }
+}
+
+// Convergence test and beta (LinearSystemSolver.cpp:37-46).  For multigrid, beta is computed after
+// the V-cycle by k_fin_beta.
+__global__ void k_fin_update(CgState* __restrict__ st, const double* __restrict__ partials,
+                             const double* __restrict__ partials2, int n) {
+    __shared__ double sh[4];
+    if (st->done) return;
+    const double tot = fin_sum(partials, n, sh);
+    const double tot2 = st->precond == 1 ? fin_sum(partials2, n, sh) : 0.0;
+    if (threadIdx.x != 0) return;
+    st->rr = tot;
+    const int i = st->iter;
+    st->iter = i + 1;
+    if (st->mode == 0) {
+        const float rsnew = (float)tot;
+        if (sqrtf(rsnew) < st->abs_tol) {
+            st->done = 1; st->count = i; st->converged = 1;
+        } else {
+            st->beta = rsnew / st->rsold;
+            st->rsold = rsnew;
+            if (i > st->max_count) { st->done = 1; st->count = i; }
+            else if (i + 1 >= st->loop_limit) { st->done = 1; st->count = i + 1; }
+        }
+    } else {
+        if (sqrt(tot) <= (double)st->rel_tol * sqrt(st->bb)) {
+            st->done = 1; st->count = i + 1; st->converged = 1;
+        } else if (i + 1 >= st->max_count) {
+            st->done = 1; st->count = i + 1;
+        } else if (st->precond != 2) {
+            const float rznew = st->precond == 1 ? (float)tot2 : (float)tot;
+            st->beta = rznew / st->rzold;
+            st->rzold = rznew;
         }
     }
+}
+
+// Multigrid: partial r.z after the V-cycle, then beta.
+__global__ void k_cg_rz(int nc, const float* __restrict__ r, const float* __restrict__ z, double* __restrict__ partials,
+                        const CgState* __restrict__ st) {
+    __shared__ double sh[4];
+    if (st->done) return;
+    double rz = 0.0;
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) rz += (double)r[c] * (double)z[c];
+    const double s = block_sum(rz, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void k_fin_beta(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
+    __shared__ double sh[4];
+    if (st->done) return;
+    const double tot = fin_sum(partials, n, sh);
+    if (threadIdx.x != 0) return;
+    const float rznew = (float)tot;
+    st->beta = rznew / st->rzold;
+    st->rzold = rznew;
 }
 
 // CG step 3: p = r + beta p (BLAS::addmul, LinearSystemSolver.cpp:41), or p = z + beta p.
 __global__ void k_cg_pupdate(int nc, CgVecs v, const CgState* __restrict__ st) {
     if (st->done) return;
-    const int c = blockIdx.x * WG + threadIdx.x;
-    if (c >= nc) return;
     const float beta = st->beta;
-    const float r = v.r[c];
-    const float z = st->precond == 1 ? r * jacobi_inv(v.info[c]) : r;
-    v.p[c] = z + (beta * v.p[c]);
+    const int pc = st->precond;
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
+        const float r = v.r[c];
+        const float z = pc == 1 ? r * jacobi_inv(v.info[c]) : (pc == 2 ? v.z[c] : r);
+        v.p[c] = z + (beta * v.p[c]);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------

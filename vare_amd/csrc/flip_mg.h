@@ -1,0 +1,291 @@
+// flip_mg.h -- geometric multigrid V-cycle used as the preconditioner of the pressure PCG
+// (VARE_PRECOND_MG).  The reference's only preconditioner is a host-serial MIC(0) that is never
+// built (BLAS.cu:211-331, FluidProjection.cu:273); this one is designed for the GPU layout.
+//
+// Level hierarchy (all levels discretise the same 7-point operator as the fine system):
+//   * sparse levels: the active blocks of the frame with cell edge E = B, B/2, ..., 1 per block,
+//     stored slot-major like the fine cells (cell (slot, local) at slot*E^3 + local);
+//   * dense levels: the block frame itself halved repeatedly ((fx+1)/2, ...), one array each;
+//   * the coarsest dense level (<= kMgCoarseMax cells) is smoothed to convergence by one workgroup.
+// A coarse cell is fluid when any child is fluid; its diagonal is 6 + nAir - nWall counted on the
+// coarse grid with the fine level's boundary rule (FluidProjection.cu:208: a domain face is a wall
+// below the wall level).  Each level solves stencil_l z_l = rt_l where rt_{l+1} = 0.5 * sum of the
+// 8 children's residuals (restriction R = P^T/8 times the 4x operator scaling), prolongation is
+// piecewise constant, smoothing is weighted Jacobi with the same weight and sweep count before and
+// after the coarse correction, so the V-cycle is a symmetric positive-definite preconditioner.
+#pragma once
+#include "flip_device.h"
+
+namespace vare {
+
+constexpr int kMgCoarseMax = 4096;
+
+struct MgLevelK {
+    int dense;          // 0: sparse level of edge 1<<sh; 1: dense grid
+    int sh;             // sparse: log2(edge)
+    int dx, dy, dz;     // dense: grid dims; sparse: block frame dims (fx, fy, fz)
+    int ncell;
+    int scale_sh;       // log2 of this level's cell size in fine cells
+    float* r;           // right-hand side rt_l (level 0: the CG residual)
+    uint16_t* info;     // kInfoFluid | fluid-neighbour mask | diag << 8
+};
+
+struct MgFrame {
+    const int* slot;    // [fx*fy*fz]
+    const int* coord;   // [Nb]
+    const int* nbr;     // [Nb*6]
+    float wall_level;
+};
+
+// ---- indexing ----
+__device__ __forceinline__ void mg_coords(const MgLevelK& L, const MgFrame& F, int c, int& i, int& j, int& k) {
+    if (L.dense) {
+        i = c % L.dx;
+        j = (c / L.dx) % L.dy;
+        k = c / (L.dx * L.dy);
+    } else {
+        const int sh = L.sh;
+        const int s = c >> (3 * sh);
+        const int l = c & ((1 << (3 * sh)) - 1);
+        const int t = F.coord[s];
+        const int bi = t % L.dx, bj = (t / L.dx) % L.dy, bk = t / (L.dx * L.dy);
+        const int m = (1 << sh) - 1;
+        i = (bi << sh) + (l & m);
+        j = (bj << sh) + ((l >> sh) & m);
+        k = (bk << sh) + (l >> (2 * sh));
+    }
+}
+
+// Neighbour index in direction q: >= 0 cell, -1 inactive (non-fluid), -2 outside the domain.
+__device__ __forceinline__ int mg_nbr(const MgLevelK& L, const MgFrame& F, int c, int q) {
+    const int axis = q >> 1;
+    const int dir = (q & 1) ? 1 : -1;
+    if (L.dense) {
+        int i = c % L.dx, j = (c / L.dx) % L.dy, k = c / (L.dx * L.dy);
+        if (axis == 0) i += dir; else if (axis == 1) j += dir; else k += dir;
+        if (i < 0 || i >= L.dx || j < 0 || j >= L.dy || k < 0 || k >= L.dz) return -2;
+        return (k * L.dy + j) * L.dx + i;
+    }
+    const int sh = L.sh;
+    const int E = 1 << sh;
+    const int s = c >> (3 * sh);
+    const int l = c & ((1 << (3 * sh)) - 1);
+    const int lc = axis == 0 ? (l & (E - 1)) : (axis == 1 ? ((l >> sh) & (E - 1)) : (l >> (2 * sh)));
+    const int stride = axis == 0 ? 1 : (axis == 1 ? E : E * E);
+    const int nl = lc + dir;
+    if (nl >= 0 && nl < E) return c + dir * stride;
+    const int ns = F.nbr[s * 6 + q];
+    if (ns < 0) return ns;
+    return (ns << (3 * sh)) + l - dir * (E - 1) * stride;
+}
+
+// Parent (coarse level C) of fine cell c (level L).
+__device__ __forceinline__ int mg_parent(const MgLevelK& L, const MgLevelK& C, const MgFrame& F, int c) {
+    if (!L.dense && L.sh > 0) {
+        const int sh = L.sh, csh = sh - 1, E = 1 << sh;
+        const int s = c >> (3 * sh);
+        const int l = c & ((1 << (3 * sh)) - 1);
+        const int li = (l & (E - 1)) >> 1, lj = ((l >> sh) & (E - 1)) >> 1, lk = (l >> (2 * sh)) >> 1;
+        return (s << (3 * csh)) + (((lk << csh) + lj) << csh) + li;
+    }
+    int i, j, k;
+    if (L.dense) { i = c % L.dx; j = (c / L.dx) % L.dy; k = c / (L.dx * L.dy); }
+    else { const int t = F.coord[c]; i = t % L.dx; j = (t / L.dx) % L.dy; k = t / (L.dx * L.dy); }
+    return ((k >> 1) * C.dy + (j >> 1)) * C.dx + (i >> 1);
+}
+
+// Child a (0..7, x fastest) of coarse cell cc (level C) on fine level L; -1 if absent.
+__device__ __forceinline__ int mg_child(const MgLevelK& L, const MgLevelK& C, const MgFrame& F, int cc, int a) {
+    const int ai = a & 1, aj = (a >> 1) & 1, ak = a >> 2;
+    if (!L.dense && L.sh > 0) {
+        const int sh = L.sh, csh = sh - 1, Ec = 1 << csh;
+        const int s = cc >> (3 * csh);
+        const int l = cc & ((1 << (3 * csh)) - 1);
+        const int li = l & (Ec - 1), lj = (l >> csh) & (Ec - 1), lk = l >> (2 * csh);
+        return (s << (3 * sh)) + ((((2 * lk + ak) << sh) + (2 * lj + aj)) << sh) + (2 * li + ai);
+    }
+    const int I = cc % C.dx, J = (cc / C.dx) % C.dy, K = cc / (C.dx * C.dy);
+    const int i = 2 * I + ai, j = 2 * J + aj, k = 2 * K + ak;
+    if (i >= L.dx || j >= L.dy || k >= L.dz) return -1;
+    if (L.dense) return (k * L.dy + j) * L.dx + i;
+    return F.slot[(k * L.dy + j) * L.dx + i];      // sparse edge-1 level: cell index = block slot
+}
+
+__device__ __forceinline__ bool mg_fluid(uint16_t info) { return (info & kInfoFluid) != 0; }
+__device__ __forceinline__ float mg_diag(uint16_t info) { return (float)((info >> 8) & 0x1f); }
+
+// ---- set-up (once per solve) ----
+// Coarse fluid flag: any child fluid.
+__global__ void k_mg_flag(MgLevelK L, MgLevelK C, MgFrame F) {
+    const int cc = blockIdx.x * WG + threadIdx.x;
+    if (cc >= C.ncell) return;
+    bool fl = false;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int ch = mg_child(L, C, F, cc, a);
+        if (ch >= 0 && mg_fluid(L.info[ch])) fl = true;
+    }
+    C.info[cc] = fl ? kInfoFluid : 0;
+}
+
+// Stencil descriptor of a coarse level from its fluid flags.
+__global__ void k_mg_info(MgLevelK C, MgFrame F) {
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= C.ncell) return;
+    const uint16_t me = C.info[c];
+    if (!mg_fluid(me)) return;
+    int i, j, k;
+    mg_coords(C, F, c, i, j, k);
+    const bool wall = ((float)(j << C.scale_sh) + 0.5f <= F.wall_level + 1.f);
+    int diag = 6;
+    uint16_t mask = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int n = mg_nbr(C, F, c, q);
+        if (n == -2) diag += wall ? -1 : 1;
+        else if (n >= 0 && mg_fluid(C.info[n])) mask |= (uint16_t)(1u << q);
+        else diag += 1;
+    }
+    C.info[c] = (uint16_t)(kInfoFluid | mask | (diag << 8));
+}
+
+// ---- V-cycle kernels; each returns at once when the solver is done ----
+// z at cell c for the "first sweep from zero" case: omega * r / diag.
+__device__ __forceinline__ float mg_z0(const MgLevelK& L, int c, float omega) {
+    const uint16_t in = L.info[c];
+    return mg_fluid(in) ? omega * L.r[c] / mg_diag(in) : 0.f;
+}
+
+// Two Jacobi sweeps from zero in one pass: zout = z1 + omega (r - A z1)/D with z1 = omega D^-1 r.
+__global__ void k_mg_smooth_first2(MgLevelK L, MgFrame F, float omega, float* __restrict__ zout, const int* done) {
+    if (*done) return;
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= L.ncell) return;
+    const uint16_t in = L.info[c];
+    float out = 0.f;
+    if (mg_fluid(in)) {
+        const float d = mg_diag(in);
+        const float z1 = omega * L.r[c] / d;
+        float az = d * z1;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (in & (1u << q)) az -= mg_z0(L, mg_nbr(L, F, c, q), omega);
+        out = z1 + omega * (L.r[c] - az) / d;
+    }
+    zout[c] = out;
+}
+
+// One Jacobi sweep: zout = z + omega (r - A z)/D.
+__global__ void k_mg_smooth(MgLevelK L, MgFrame F, float omega, const float* __restrict__ z, float* __restrict__ zout,
+                            const int* done) {
+    if (*done) return;
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= L.ncell) return;
+    const uint16_t in = L.info[c];
+    float out = 0.f;
+    if (mg_fluid(in)) {
+        const float d = mg_diag(in);
+        const float zc = z[c];
+        float az = d * zc;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (in & (1u << q)) az -= z[mg_nbr(L, F, c, q)];
+        out = zc + omega * (L.r[c] - az) / d;
+    }
+    zout[c] = out;
+}
+
+// Restriction of the residual: rC[cc] = 0.5 * sum over fluid children of (r - A z).  With
+// first != 0 the fine z is the first sweep from zero (omega D^-1 r), computed on the fly and
+// stored to zf.
+__global__ void k_mg_restrict(MgLevelK L, MgLevelK C, MgFrame F, float omega, int first, float* __restrict__ zf,
+                              const int* done) {
+    if (*done) return;
+    const int cc = blockIdx.x * WG + threadIdx.x;
+    if (cc >= C.ncell) return;
+    float acc = 0.f;
+    if (mg_fluid(C.info[cc])) {
+#pragma unroll 1
+        for (int a = 0; a < 8; ++a) {
+            const int ch = mg_child(L, C, F, cc, a);
+            if (ch < 0) continue;
+            const uint16_t in = L.info[ch];
+            if (!mg_fluid(in)) continue;
+            const float d = mg_diag(in);
+            const float zc = first ? omega * L.r[ch] / d : zf[ch];
+            if (first) zf[ch] = zc;
+            float az = d * zc;
+            for (int q = 0; q < 6; ++q) {
+                if (!(in & (1u << q))) continue;
+                const int n = mg_nbr(L, F, ch, q);
+                az -= first ? mg_z0(L, n, omega) : zf[n];
+            }
+            acc += L.r[ch] - az;
+        }
+        acc *= 0.5f;
+    }
+    C.r[cc] = acc;
+}
+
+// Prolongation (piecewise constant) fused with one Jacobi sweep:
+// zp = z + zC[parent]; zout = zp + omega (r - A zp)/D.
+__global__ void k_mg_prolong_smooth(MgLevelK L, MgLevelK C, MgFrame F, float omega, const float* __restrict__ z,
+                                    const float* __restrict__ zc, float* __restrict__ zout, const int* done) {
+    if (*done) return;
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= L.ncell) return;
+    const uint16_t in = L.info[c];
+    float out = 0.f;
+    if (mg_fluid(in)) {
+        const float d = mg_diag(in);
+        const float zp = z[c] + zc[mg_parent(L, C, F, c)];
+        float az = d * zp;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            if (!(in & (1u << q))) continue;
+            const int n = mg_nbr(L, F, c, q);
+            az -= z[n] + zc[mg_parent(L, C, F, n)];
+        }
+        out = zp + omega * (L.r[c] - az) / d;
+    }
+    zout[c] = out;
+}
+
+// Coarsest level (dense, <= kMgCoarseMax cells): `sweeps` Jacobi sweeps from zero by one
+// workgroup of 1024 threads in LDS.
+__global__ void __launch_bounds__(1024) k_mg_coarse(MgLevelK L, float omega, int sweeps, float* __restrict__ zout,
+                                                    const int* done) {
+    if (*done) return;
+    __shared__ float za[kMgCoarseMax];
+    __shared__ float zb[kMgCoarseMax];
+    const int n = L.ncell;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) za[c] = 0.f;
+    __syncthreads();
+    float* cur = za;
+    float* nxt = zb;
+    for (int s = 0; s < sweeps; ++s) {
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            const uint16_t in = L.info[c];
+            float out = 0.f;
+            if (mg_fluid(in)) {
+                const float d = mg_diag(in);
+                float az = d * cur[c];
+                const int i = c % L.dx, j = (c / L.dx) % L.dy;
+                if (in & 1u) az -= cur[c - 1];
+                if (in & 2u) az -= cur[c + 1];
+                if (in & 4u) az -= cur[c - L.dx];
+                if (in & 8u) az -= cur[c + L.dx];
+                if (in & 16u) az -= cur[c - L.dx * L.dy];
+                if (in & 32u) az -= cur[c + L.dx * L.dy];
+                (void)i; (void)j;
+                out = cur[c] + omega * (L.r[c] - az) / d;
+            }
+            nxt[c] = out;
+        }
+        __syncthreads();
+        float* t = cur; cur = nxt; nxt = t;
+    }
+    for (int c = threadIdx.x; c < n; c += blockDim.x) zout[c] = cur[c];
+}
+
+}  // namespace vare
