@@ -84,7 +84,10 @@ typedef struct vare_flip_params {
     int mg_coarse_sweeps;    /* multigrid: sweeps on the coarsest grid, default 40 */
     int particle_path;       /* 0: block-binned P2G/G2P through LDS tiles (default); 1: one thread per
                                 particle with global atomics / gathers */
-    int reserved[4];
+    float refine_tol;        /* > 0: FP64 iterative refinement of the FP32 PCG until the FP64
+                                residual |b - A x|/|b| <= refine_tol (pressure kept in FP64) */
+    int refine_max;          /* refinement step cap, default 8 */
+    int reserved[2];
 } vare_flip_params;
 
 /* Frame geometry after SparseFrame::initialize (SparseFrame.cu:72-117): padded dims, block
@@ -112,6 +115,9 @@ typedef struct vare_step_stats {
     double b_norm;               /* |b|_2 */
     double step_ms;              /* device time of the whole step (HIP events) */
     double solve_ms;             /* device time of the Poisson solve */
+    int refine_steps;            /* FP64 iterative-refinement steps (0 when refinement is off) */
+    int pad;
+    double true_rel_residual;    /* |b - A p|_2 / |b|_2 recomputed in FP64 from the final pressure */
 } vare_step_stats;
 
 /* cell fields for vare_flip_get_cell_field (storage order = block-major, x fastest) */
@@ -219,6 +225,57 @@ int vare_device_name(char* buf, size_t cap);
 int vare_flip_set_kernel_timing(vare_flip* h, int mask);
 int vare_flip_get_kernel_timing(vare_flip* h, double ms_total[VARE_KERNEL_COUNT], int64_t launches[VARE_KERNEL_COUNT],
                                 int reset);
+
+/* ---- Serving: many independent tank solves, continuously batched on one GPU ----
+ * No counterpart in the reference (one Houdini cook drives one FLIPMethod).  A server admits
+ * submitted requests in arrival order whenever a slot and enough of the HBM budget are free; each
+ * admitted tank gets its own handle and HIP stream, driven by a worker thread, so the steps of all
+ * resident tanks overlap on the device; a finished tank releases its slot and memory at once and the
+ * next arrived request is admitted (continuous batching). */
+typedef struct vare_server vare_server;
+
+typedef struct vare_tank_request {
+    vare_grid_desc grid;
+    vare_flip_params params;
+    const float* pos;        /* voxel-space xyz, host memory, copied at submit */
+    const float* vel;
+    size_t n;
+    int steps;
+    float dt;
+    double arrival_s;        /* arrival time in seconds after vare_server_run starts */
+} vare_tank_request;
+
+typedef struct vare_request_stats {
+    double arrival_s, admit_s, finish_s;   /* seconds after run start */
+    double step_ms_mean, step_ms_max;
+    int steps_done;
+    int status;                            /* VARE_OK or the first error of the request */
+    int64_t num_particles;
+    int64_t bytes_reserved;
+} vare_request_stats;
+
+typedef struct vare_server_stats {
+    int64_t requests, steps;
+    double wall_s;                         /* first arrival to last finish */
+    double step_p50_ms, step_p99_ms, step_mean_ms;
+    double particle_steps_per_s;
+    int max_resident;                      /* most tanks resident at once */
+    int pad;
+    int64_t peak_bytes;                    /* peak reserved HBM */
+} vare_server_stats;
+
+int vare_server_create(size_t hbm_budget_bytes, int max_slots, vare_server** out);
+int vare_server_destroy(vare_server* s);
+/* Queue a request (copies the particle arrays); returns its id. */
+int vare_server_submit(vare_server* s, const vare_tank_request* req, int64_t* id);
+/* Run until every submitted request has finished, honouring arrival times. */
+int vare_server_run(vare_server* s);
+/* Final particles of a finished request (voxel space) and its record; any pointer may be NULL. */
+int vare_server_result(vare_server* s, int64_t id, float* pos, float* vel, size_t cap, size_t* n,
+                       vare_request_stats* st);
+int vare_server_get_stats(vare_server* s, vare_server_stats* out);
+/* Estimated device bytes a request reserves (what admission charges against the budget). */
+int vare_server_request_bytes(const vare_tank_request* req, size_t* bytes);
 
 /* Last error message of the calling thread. */
 const char* vare_last_error(void);

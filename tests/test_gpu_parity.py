@@ -359,3 +359,29 @@ def test_binned_path_matches_direct_path(B):
     assert rel(f0[0], f1[0]) < 1e-6 and rel(f0[1], f1[1]) < 1e-6
     np.testing.assert_array_equal(p0["adj"], p1["adj"])
     assert rel(p0["vel"], p1["vel"]) < 1e-5 and rel(p0["pos"], p1["pos"]) < 1e-7
+
+
+def test_iterative_refinement_fp64():
+    # FP32 MG-PCG (inner 1e-4) + FP64 residual recompute: the FP64 residual of the refined pressure
+    # reaches 1e-8 relative; the FP32 pressure handed to the gradient matches the FP64 oracle's
+    # converged solve to FP32 rounding.
+    n = 48
+    pos, vel = make_scene(n, wx=0.5, wy=0.5, seed=11)
+    base = dict(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n, solver_mode=1)
+    g = L.grid_desc(n, n, n, (0, 0, 0), (1, 1, 1), 8)
+    h = L.FlipHandle(g, L.default_params(preconditioner=L.PRECOND_MG, cg_rel_tol=1e-4, refine_tol=1e-8,
+                                         max_iterations=500, **base))
+    h.set_particles(pos, vel)
+    o64 = O.Oracle((n, n, n), B=8, prm=O.params(cg_rel_tol=1e-12, max_iterations=10000, **base), fp64=True)
+    o64.set_particles(pos, vel)
+    for s in range(4):
+        h.stage(s, DT)
+        o64.stage(s, DT)
+    st = h.stats()
+    print("refinement steps=%d inner iters=%d fp64 rel residual=%.3g (fp32 pressure: %.3g)" % (
+        st["refine_steps"], st["cg_iterations"], st["residual_norm"] / st["b_norm"], st["true_rel_residual"]),
+        flush=True)
+    assert st["converged"] == 1 and st["refine_steps"] >= 2
+    assert st["residual_norm"] <= 1e-8 * st["b_norm"]
+    assert rel(h.cell_field(L.FIELD_PRESSURE), o64.field(O.FIELD_PRES)) < 1e-6
+    assert rel(h.cell_field(L.FIELD_COMP), o64.field(O.FIELD_COMP)) < 1e-6

@@ -72,7 +72,7 @@ class FlipParams(C.Structure):
                 ("solver_mode", C.c_int), ("cg_abs_tol", C.c_float), ("cg_rel_tol", C.c_float),
                 ("max_iterations", C.c_int), ("preconditioner", C.c_int), ("mg_sweeps", C.c_int),
                 ("mg_omega", C.c_float), ("mg_coarse_sweeps", C.c_int), ("particle_path", C.c_int),
-                ("reserved", C.c_int * 4)]
+                ("refine_tol", C.c_float), ("refine_max", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class FrameInfo(C.Structure):
@@ -86,10 +86,35 @@ class StepStats(C.Structure):
     _fields_ = [("num_particles", C.c_int64), ("num_removed", C.c_int64), ("num_blocks", C.c_int64),
                 ("num_cells", C.c_int64), ("num_fluid", C.c_int64), ("cg_iterations", C.c_int),
                 ("converged", C.c_int), ("residual_norm", C.c_double), ("b_norm", C.c_double),
-                ("step_ms", C.c_double), ("solve_ms", C.c_double)]
+                ("step_ms", C.c_double), ("solve_ms", C.c_double), ("refine_steps", C.c_int), ("pad", C.c_int),
+                ("true_rel_residual", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TankRequest(C.Structure):
+    _fields_ = [("grid", GridDesc), ("params", FlipParams), ("pos", C.c_void_p), ("vel", C.c_void_p),
+                ("n", C.c_size_t), ("steps", C.c_int), ("dt", C.c_float), ("arrival_s", C.c_double)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double),
+                ("step_ms_mean", C.c_double), ("step_ms_max", C.c_double), ("steps_done", C.c_int),
+                ("status", C.c_int), ("num_particles", C.c_int64), ("bytes_reserved", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_int64), ("steps", C.c_int64), ("wall_s", C.c_double),
+                ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double), ("step_mean_ms", C.c_double),
+                ("particle_steps_per_s", C.c_double), ("max_resident", C.c_int), ("pad", C.c_int),
+                ("peak_bytes", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
 _P = C.c_void_p
@@ -126,6 +151,14 @@ SIGNATURES = {
     "vare_flip_get_kernel_timing": (C.c_int, [_P, C.POINTER(C.c_double * KERNEL_COUNT),
                                               C.POINTER(C.c_int64 * KERNEL_COUNT), C.c_int]),
     "vare_last_error": (C.c_char_p, []),
+    "vare_server_create": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(_P)]),
+    "vare_server_destroy": (C.c_int, [_P]),
+    "vare_server_submit": (C.c_int, [_P, C.POINTER(TankRequest), C.POINTER(C.c_int64)]),
+    "vare_server_run": (C.c_int, [_P]),
+    "vare_server_result": (C.c_int, [_P, C.c_int64, _P, _P, C.c_size_t, C.POINTER(C.c_size_t),
+                                     C.POINTER(RequestStats)]),
+    "vare_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "vare_server_request_bytes": (C.c_int, [C.POINTER(TankRequest), C.POINTER(C.c_size_t)]),
 }
 
 _lib = None
@@ -324,6 +357,77 @@ class FlipHandle:
         cnt = (C.c_int64 * KERNEL_COUNT)()
         check(self.lib.vare_flip_get_kernel_timing(self.h, C.byref(ms), C.byref(cnt), 1 if reset else 0))
         return {KERNEL_NAMES[i]: (ms[i], cnt[i]) for i in range(KERNEL_COUNT)}
+
+
+class Server:
+    """Owning wrapper of a ``vare_server`` (continuous batching of independent tank solves)."""
+
+    def __init__(self, budget_bytes, max_slots):
+        self.lib = load()
+        self.s = C.c_void_p()
+        check(self.lib.vare_server_create(int(budget_bytes), int(max_slots), C.byref(self.s)))
+        self._keep = []
+
+    def close(self):
+        if self.s:
+            self.lib.vare_server_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def make_request(grid, params, pos, vel, steps, dt, arrival_s=0.0):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        r = TankRequest()
+        r.grid = grid
+        r.params = params
+        r.pos = pos.ctypes.data_as(C.c_void_p)
+        r.vel = vel.ctypes.data_as(C.c_void_p)
+        r.n = pos.shape[0]
+        r.steps = int(steps)
+        r.dt = float(dt)
+        r.arrival_s = float(arrival_s)
+        return r, (pos, vel)
+
+    def submit(self, grid, params, pos, vel, steps, dt, arrival_s=0.0) -> int:
+        r, keep = self.make_request(grid, params, pos, vel, steps, dt, arrival_s)
+        rid = C.c_int64()
+        check(self.lib.vare_server_submit(self.s, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def run(self):
+        check(self.lib.vare_server_run(self.s))
+
+    def result(self, rid):
+        n = C.c_size_t()
+        st = RequestStats()
+        check(self.lib.vare_server_result(self.s, rid, None, None, 0, C.byref(n), C.byref(st)))
+        pos = np.empty((n.value, 3), np.float32)
+        vel = np.empty((n.value, 3), np.float32)
+        check(self.lib.vare_server_result(self.s, rid, ptr(pos), ptr(vel), n.value, C.byref(n), C.byref(st)))
+        return pos, vel, st.as_dict()
+
+    def stats(self):
+        st = ServerStats()
+        check(self.lib.vare_server_get_stats(self.s, C.byref(st)))
+        return st.as_dict()
+
+    @staticmethod
+    def request_bytes(grid, params, n, steps=1, dt=1 / 240):
+        r = TankRequest()
+        r.grid = grid
+        r.params = params
+        r.n = int(n)
+        r.steps = steps
+        r.dt = dt
+        b = C.c_size_t()
+        check(load().vare_server_request_bytes(C.byref(r), C.byref(b)))
+        return b.value
 
 
 def device_synchronize():

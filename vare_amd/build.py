@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "flip_api.hip")
+SRCS = [os.path.join(HERE, "csrc", "flip_api.hip"), os.path.join(HERE, "csrc", "flip_server.cpp")]
 OUT_DIR = os.path.join(HERE, "lib")
 OUT = os.path.join(OUT_DIR, "libvareflip.so")
 
@@ -33,7 +33,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp, SRC]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + SRCS
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -110,6 +110,7 @@ struct vare_flip {
     float *mass = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *cnt = nullptr;
     float *cx = nullptr, *cy = nullptr, *cz = nullptr;
     float *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *b = nullptr, *z = nullptr;
+    double* x64 = nullptr;        // FP64 pressure of the iterative refinement
     uint16_t* info = nullptr;
     uint8_t* type = nullptr;
     int* rank = nullptr;
@@ -167,7 +168,7 @@ void free_cells(vare_flip* h) {
     dfree(h->bcount); dfree(h->bstart);
     dfree(h->mass); dfree(h->vx); dfree(h->vy); dfree(h->vz); dfree(h->cnt);
     dfree(h->cx); dfree(h->cy); dfree(h->cz);
-    dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b); dfree(h->z);
+    dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b); dfree(h->z); dfree(h->x64);
     dfree(h->info); dfree(h->type); dfree(h->rank);
     dfree(h->partials); dfree(h->partials2);
     h->bcap = 0;
@@ -187,7 +188,7 @@ int ensure_blocks(vare_flip* h, size_t nb) {
     CHK(dalloc(&h->cnt, nc));
     CHK(dalloc(&h->cx, nc)); CHK(dalloc(&h->cy, nc)); CHK(dalloc(&h->cz, nc));
     CHK(dalloc(&h->x, nc)); CHK(dalloc(&h->r, nc)); CHK(dalloc(&h->p, nc)); CHK(dalloc(&h->ap, nc));
-    CHK(dalloc(&h->b, nc)); CHK(dalloc(&h->z, nc));
+    CHK(dalloc(&h->b, nc)); CHK(dalloc(&h->z, nc)); CHK(dalloc(&h->x64, nc));
     CHK(dalloc(&h->info, nc)); CHK(dalloc(&h->type, nc)); CHK(dalloc(&h->rank, nc));
     CHK(dalloc(&h->partials, npart)); CHK(dalloc(&h->partials2, npart));
     h->bcap = cap;
@@ -542,15 +543,11 @@ template <int B> void enqueue_iteration(vare_flip* h, size_t nc, unsigned g) {
     tend(h, VARE_KERNEL_PUPDATE);
 }
 
-template <int B> int stage_solve(vare_flip* h) {
-    const size_t nc = h->ncells();
-    const unsigned g = rgrid(nc);
-    const int mode = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? 1 : 0;
-    const int cap = mode == 0 ? h->prm.proj_iteration + 2 : h->prm.max_iterations;
-    HIPCHK(hipEventRecord(h->step_ev[2], h->stream));
+// Run CG/PCG iterations until the device-side `done` flag is set.  The iteration count changes
+// slowly from step to step: the previous count (+1) is enqueued before the first host check, then
+// small chunks; iterations enqueued past convergence return at once.
+template <int B> int run_solver(vare_flip* h, size_t nc, unsigned g, int cap, int& total_iters) {
     int launched = 0;
-    // The iteration count changes slowly from step to step: enqueue the previous count (+1) before
-    // the first host check, then small chunks.  Iterations enqueued past convergence return at once.
     int chunk = h->last_iters > 0 ? h->last_iters + 1 : 8;
     for (;;) {
         HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
@@ -563,6 +560,50 @@ template <int B> int stage_solve(vare_flip* h) {
         chunk = h->last_iters > 0 ? 4 : std::min(chunk * 2, 64);
     }
     h->last_iters = h->st_host->count;
+    total_iters += h->st_host->count;
+    return VARE_OK;
+}
+
+template <int B> int stage_solve(vare_flip* h) {
+    const size_t nc = h->ncells();
+    const unsigned g = rgrid(nc);
+    const int mode = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? 1 : 0;
+    const int cap = mode == 0 ? h->prm.proj_iteration + 2 : h->prm.max_iterations;
+    HIPCHK(hipEventRecord(h->step_ev[2], h->stream));
+    const bool refine = mode == 1 && h->prm.refine_tol > 0.f;
+    int total_iters = 0, outer = 0;
+    if (refine) CHK(fill_u32(h, h->x64, 0, 2 * nc));
+    CHK(run_solver<B>(h, nc, g, cap, total_iters));
+    if (refine) {
+        // FP64 iterative refinement: x64 += d, r = b - A x64 in FP64, solve A d = r in FP32
+        for (;;) {
+            hipLaunchKernelGGL(k_ir_accumulate, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->x64, (const float*)h->x,
+                               h->x, 0);
+            ++outer;
+            hipLaunchKernelGGL(k_ir_residual<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+                               (const float*)h->b, (const double*)h->x64, (const uint16_t*)h->info, h->r, h->x, h->p,
+                               h->partials);
+            hipLaunchKernelGGL(k_fin_ir, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g,
+                               h->prm.refine_tol);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            if (h->st_host->ir_done || outer >= h->prm.refine_max) break;
+            if (h->prm.preconditioner == VARE_PRECOND_MG) enqueue_vcycle(h);
+            hipLaunchKernelGGL(k_cg_precond_init, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials,
+                               (const CgState*)h->st);
+            hipLaunchKernelGGL(k_fin_precond_init, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+                               (int)g);
+            CHK(run_solver<B>(h, nc, g, cap, total_iters));
+        }
+        hipLaunchKernelGGL(k_ir_round, dim3(g), dim3(WG), 0, h->stream, (int)nc, (const double*)h->x64, h->x);
+    }
+    // FP64 recompute of the final residual from the FP32 pressure (reported with every solve)
+    hipLaunchKernelGGL(k_true_residual<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const float*)h->b,
+                       (const float*)h->x, (const uint16_t*)h->info, h->partials);
+    hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+    HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     const unsigned ga = nblocks(std::max<size_t>(nc, 1));
     tbegin(h, VARE_KERNEL_APPLY);
     if (nc) {
@@ -575,11 +616,13 @@ template <int B> int stage_solve(vare_flip* h) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(h->step_ev[3], h->stream));
     const CgState& s = *h->st_host;
-    h->stats.cg_iterations = s.count;
-    h->stats.converged = s.converged;
-    h->stats.residual_norm = std::sqrt(std::max(s.rr, 0.0));
-    h->stats.b_norm = std::sqrt(std::max(s.bb, 0.0));
+    h->stats.cg_iterations = refine ? total_iters : s.count;
+    h->stats.converged = refine ? s.ir_done : s.converged;
+    h->stats.residual_norm = std::sqrt(std::max(refine ? s.ir_rr : s.rr, 0.0));
+    h->stats.b_norm = std::sqrt(std::max(s.ir_bb, 0.0));
     h->stats.num_fluid = s.nfluid;
+    h->stats.refine_steps = outer;
+    h->stats.true_rel_residual = s.ir_bb > 0.0 ? std::sqrt(std::max(s.true_rr, 0.0) / s.ir_bb) : 0.0;
     return VARE_OK;
 }
 
@@ -710,6 +753,8 @@ int vare_flip_params_default(vare_flip_params* p) {
     p->mg_sweeps = 2;
     p->mg_omega = 0.8f;
     p->mg_coarse_sweeps = 40;
+    p->refine_tol = 0.f;
+    p->refine_max = 8;
     return VARE_OK;
 }
 

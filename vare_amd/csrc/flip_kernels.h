@@ -266,7 +266,10 @@ struct CgState {
     float rel_tol;
     int precond;      // 0 none, 1 Jacobi, 2 multigrid
     int nfluid;       // fluid cells (rows of the system)
-    int pad;
+    int ir_done;      // iterative refinement reached its tolerance
+    double ir_bb;     // |b|^2 of the original system (refinement)
+    double ir_rr;     // |b - A x64|^2 of the last refinement step
+    double true_rr;   // |b - A x|^2 recomputed in FP64 from the final FP32 pressure
 };
 
 template <int B>
@@ -352,6 +355,10 @@ __global__ void k_fin_divergence(CgState* __restrict__ st, const double* __restr
     st->nfluid = (int)tot2;
     st->loop_limit = (int)tot2 + 5;     // LinearSystemSolver.cpp:25-27: i < b.size()+5
     st->bb = tot;
+    st->ir_bb = tot;
+    st->ir_rr = tot;
+    st->ir_done = 0;
+    st->true_rr = 0.0;
     st->rr = tot;
     st->iter = 0;
     st->count = 0;
@@ -750,6 +757,109 @@ __global__ void k_mark_u64(int nc, const uint8_t* __restrict__ type, const int* 
     if (c >= nc) return;
     const uint8_t t = type[c];
     out[c] = t == kFluid ? (unsigned long long)(4 + rank[c]) : (unsigned long long)t;
+}
+
+}  // namespace vare
+
+namespace vare {
+
+// ------------------------------------------------------------------------------------------------
+// Mixed-precision projection: iterative refinement.  The pressure accumulates in FP64 (x64); each
+// outer step recomputes the residual r = b - A x64 in FP64, hands its FP32 rounding to the FP32
+// PCG as the right-hand side of a correction solve A d = r (d starts at 0), then x64 += d.
+template <int B>
+__global__ void k_ir_residual(int nc, FrameK f, const float* __restrict__ b, const double* __restrict__ x64,
+                              const uint16_t* __restrict__ info, float* __restrict__ r32, float* __restrict__ x,
+                              float* __restrict__ p, double* __restrict__ partials) {
+    __shared__ double sh[4];
+    double rr = 0.0;
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
+        const uint16_t in = info[c];
+        double r = 0.0;
+        if (in & kInfoFluid) {
+            double ax = (double)((in >> 8) & 0x1f) * x64[c];
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (in & (1u << q)) ax -= x64[cell_neighbor<B>(f, c, q)];
+            r = (double)b[c] - ax;
+        }
+        const float rf = (float)r;
+        r32[c] = rf;
+        x[c] = 0.f;
+        p[c] = rf;
+        rr += r * r;
+    }
+    const double s = block_sum(rr, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void k_ir_accumulate(int nc, double* __restrict__ x64, const float* __restrict__ d, float* __restrict__ x32,
+                                int last) {
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
+        const double v = x64[c] + (double)d[c];
+        x64[c] = v;
+        if (last) x32[c] = (float)v;
+    }
+}
+
+__global__ void k_ir_round(int nc, const double* __restrict__ x64, float* __restrict__ x32) {
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) x32[c] = (float)x64[c];
+}
+
+}  // namespace vare
+
+namespace vare {
+
+// Refinement bookkeeping after k_ir_residual: record the FP64 residual and restart the inner solve
+// on the new right-hand side (its relative tolerance is measured against |r|).
+__global__ void k_fin_ir(CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += WG) s += partials[q];
+    const double tot = block_sum(s, sh);
+    if (threadIdx.x != 0) return;
+    st->ir_rr = tot;
+    if (sqrt(tot) <= (double)outer_tol * sqrt(st->ir_bb) || tot == 0.0) {
+        st->ir_done = 1;
+        st->done = 1;
+        return;
+    }
+    st->bb = tot;
+    st->rr = tot;
+    st->rsold = (float)tot;
+    st->rzold = (float)tot;
+    st->iter = 0;
+    st->count = 0;
+    st->converged = 0;
+    st->done = 0;
+}
+
+// |b - A x|^2 in FP64 from the FP32 pressure (reported with every solve).
+template <int B>
+__global__ void k_true_residual(int nc, FrameK f, const float* __restrict__ b, const float* __restrict__ x,
+                                const uint16_t* __restrict__ info, double* __restrict__ partials) {
+    __shared__ double sh[4];
+    double rr = 0.0;
+    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
+        const uint16_t in = info[c];
+        if (!(in & kInfoFluid)) continue;
+        double ax = (double)((in >> 8) & 0x1f) * (double)x[c];
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (in & (1u << q)) ax -= (double)x[cell_neighbor<B>(f, c, q)];
+        const double r = (double)b[c] - ax;
+        rr += r * r;
+    }
+    const double s = block_sum(rr, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void k_fin_true(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += WG) s += partials[q];
+    const double tot = block_sum(s, sh);
+    if (threadIdx.x == 0) st->true_rr = tot;
 }
 
 }  // namespace vare
