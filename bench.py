@@ -1,12 +1,19 @@
-"""Benchmark: FLIP dam break on the two-level sparse grid, one MI355X.
+"""Benchmark of the FLIP step on the two-level sparse grid, one MI355X.
 
-Workload (default): 256^3 cells, dx = 1/256, closed solid walls, water block x<0.5, y<0.5, full z,
-8 jittered particles per cell (33,554,431 particles), FP32, gravity -9.8, dt = 1/240, FLIP 0.95,
-pressure PCG to 1e-5 relative residual, 8^3 blocks.  A "step" is one FLIPMethod::advanceOneStep
+Default (config 2 of the benchmark set): FLIP dam break, 256^3 cells, dx = 1/256, closed solid walls,
+water block x<0.5, y<0.5, full z, 8 jittered particles per cell (33,554,431 particles), FP32,
+gravity -9.8, dt = 1/240, FLIP ratio 0.95, pressure PCG to 1e-5 relative residual (multigrid
+preconditioner), 8^3 blocks, 200 steps from rest.  A "step" is one FLIPMethod::advanceOneStep
 (binning, P2G, divergence, PCG, pressure apply, G2P/advection, removal) with the particles resident
-in HBM.  The timed region runs `--steps` consecutive steps of one simulation started from rest.
+in HBM.
 
-Prints one JSON line (driver contract) with roofline and cpu_baseline objects.
+Other configurations (--config):
+  1  the same dam break at 64^3, water x<0.4, y<0.6, 100 steps; the CPU baseline runs all 100 steps;
+  4  served batch: independent tanks (edges log-uniform 32..128, 4..8 particles per cell, water
+     fractions 0.2..0.7, 10 steps each) arriving as a Poisson process, continuously batched by the
+     library's server, compared with the same requests solved one at a time ("plain GPU").
+
+Prints one JSON line (driver contract) with `roofline` and `cpu_baseline` objects.
 """
 import argparse
 import json
@@ -21,41 +28,57 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/sec + p50/p99 step latency at 1e-5 rel divergence, vs plain GPU"
 HBM_PEAK_GBS = 8000.0
+DT = 1.0 / 240.0
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 4])
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--block", type=int, default=8)
     ap.add_argument("--ppc", type=int, default=8)
     ap.add_argument("--solver", choices=["relative", "reference"], default="relative")
     ap.add_argument("--precond", choices=["none", "jacobi", "mg"], default="mg")
-    ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--tol", type=float, default=1e-5,
+                    help="target |b-Ax|/|b|, met in FP64 through iterative refinement (see --no-refine)")
+    ap.add_argument("--inner-tol", type=float, default=1e-3,
+                    help="relative tolerance of each FP32 PCG solve inside the refinement")
+    ap.add_argument("--no-refine", action="store_true",
+                    help="plain FP32 PCG to --tol (recursive residual; its FP64 residual is reported)")
+    ap.add_argument("--mg-sweeps", type=int, default=2)
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--slots", type=int, default=16)
     return ap.parse_args()
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
-              "sharding design is described in DESIGN.md)" % a.gpus, file=sys.stderr)
-        sys.exit(2)
-    from vare_amd import _lib as L
+def solver_kw(a, L, n):
+    refine = a.solver == "relative" and not a.no_refine
+    return dict(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n,
+                solver_mode=L.SOLVER_RELATIVE if a.solver == "relative" else L.SOLVER_REFERENCE,
+                cg_rel_tol=a.inner_tol if refine else a.tol, refine_tol=a.tol if refine else 0.0,
+                refine_max=16, max_iterations=20000,
+                preconditioner={"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "mg": L.PRECOND_MG}[a.precond],
+                mg_sweeps=a.mg_sweeps)
 
-    n = a.n
-    dt = 1.0 / 240.0
-    kw = dict(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n,
-              solver_mode=L.SOLVER_RELATIVE if a.solver == "relative" else L.SOLVER_REFERENCE,
-              cg_rel_tol=a.tol, max_iterations=20000,
-              preconditioner={"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "mg": L.PRECOND_MG}[a.precond])
+
+def solver_desc(a):
+    if a.solver != "relative":
+        return "reference CG rule"
+    if a.no_refine:
+        return "FP32 %s-PCG to %.0e relative (recursive residual)" % (a.precond, a.tol)
+    return ("FP32 %s-PCG (%.0e per solve) + FP64 iterative refinement to %.0e relative FP64 residual"
+            % (a.precond, a.inner_tol, a.tol))
+
+
+def run_dambreak(a, L, n, wx, wy, steps):
+    kw = solver_kw(a, L, n)
     t0 = time.time()
-    pos = L.seed_box(n, n, n, 0.5 * n, 0.5 * n, float(n), a.ppc, 1)
+    pos = L.seed_box(n, n, n, wx * n, wy * n, float(n), a.ppc, 1)
     vel = np.zeros_like(pos)
     npart = pos.shape[0]
     print("bench: seeded %d particles in %.1fs on %s" % (npart, time.time() - t0, L.device_name()),
@@ -68,7 +91,7 @@ def main():
     # warm-up: run the first steps of the simulation, then restart from the initial state
     h.set_particles(pos, vel)
     for _ in range(a.warmup):
-        rc = h.step_rc(dt)
+        rc = h.step_rc(DT)
         if rc not in (L.VARE_OK, L.VARE_ENOCONV):
             L.check(rc)
     h.set_particles(pos, vel)
@@ -77,15 +100,13 @@ def main():
         h.kernel_timing(reset=True)
     L.device_synchronize()
 
-    step_ms = []
-    iters = []
-    relres = []
+    step_ms, iters, relres, truerel, refines = [], [], [], [], []
     nonconv = 0
     stats = None
     t_start = time.perf_counter()
-    for s in range(a.steps):
+    for s in range(steps):
         ts = time.perf_counter()
-        rc = h.step_rc(dt)
+        rc = h.step_rc(DT)
         if rc == L.VARE_ENOCONV:
             nonconv += 1
         elif rc != L.VARE_OK:
@@ -95,15 +116,15 @@ def main():
         iters.append(stats["cg_iterations"])
         if stats["b_norm"] > 0:
             relres.append(stats["residual_norm"] / stats["b_norm"])
+            truerel.append(stats["true_rel_residual"])
+            refines.append(stats["refine_steps"])
         if (s + 1) % 50 == 0:
-            print("bench: step %d/%d %.2f ms cg=%d" % (s + 1, a.steps, step_ms[-1], iters[-1]),
+            print("bench: step %d/%d %.2f ms cg=%d" % (s + 1, steps, step_ms[-1], iters[-1]),
                   file=sys.stderr, flush=True)
     L.device_synchronize()
     total_s = time.perf_counter() - t_start
     kt = h.kernel_timing(reset=True) if not a.no_kernel_timing else None
 
-    value = npart * a.steps / total_s
-    sm = np.array(step_ms)
     roofline = None
     if kt is not None and kt["spmv"][1] > 0:
         nc = stats["num_cells"]
@@ -112,64 +133,160 @@ def main():
         bytes_per_launch = 10.0 * nc
         avg_ms = kt["spmv"][0] / kt["spmv"][1]
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        roofline = {"kernel": "k_cg_spmv", "bound": "hbm", "achieved": round(achieved, 1),
+        roofline = {"kernel": "k_spmv_blk", "bound": "hbm", "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 2), "launches": kt["spmv"][1],
                     "bytes_per_launch": bytes_per_launch}
-
-    cpu = None
-    if a.cpu_baseline == "on":
-        cpu = cpu_baseline(h, n, B, kw, dt, a.cpu_steps)
-
-    out = {
-        "metric": METRIC,
-        "value": round(value, 1),
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(total_s * 1e3 / a.steps, 3),
+    sm = np.array(step_ms)
+    res = {
+        "value": round(npart * steps / total_s, 1),
+        "ms_per_step": round(total_s * 1e3 / steps, 3),
         "p50_ms": round(float(np.percentile(sm, 50)), 3),
         "p99_ms": round(float(np.percentile(sm, 99)), 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "dam break %d^3, water x<0.5 y<0.5, %d ppc, closed walls, dt=1/240, FLIP 0.95, "
-                               "%s PCG %s to %.0e relative, %d^3 blocks, %d steps from rest"
-                               % (n, a.ppc, a.precond, a.solver, a.tol, B, a.steps),
+        "config": {"workload": "dam break %d^3, water x<%.1f y<%.1f, %d ppc, closed walls, dt=1/240, FLIP 0.95, "
+                               "%s, %d^3 blocks, %d steps from rest"
+                               % (n, wx, wy, a.ppc, solver_desc(a), B, steps),
                    "particles": npart, "cells": n ** 3, "active_blocks_last": stats["num_blocks"],
                    "fluid_cells_last": stats["num_fluid"]},
         "cg_iterations_mean": round(float(np.mean(iters)), 1),
         "cg_iterations_max": int(np.max(iters)),
         "rel_residual_max": float(np.max(relres)) if relres else 0.0,
+        "fp64_rel_residual_max": float(np.max(truerel)) if truerel else 0.0,
+        "refine_steps_mean": round(float(np.mean(refines)), 2) if refines else 0.0,
         "nonconverged_steps": nonconv,
         "roofline": roofline,
-        "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    return h, kw, res
 
 
-def cpu_baseline(h, n, B, kw, dt, steps):
+def cpu_baseline(h, n, B, kw, dt, steps, pos0=None):
     """Time the oracle (the FP32 restatement of the reference step, one host thread) on the same
-    workload: `steps` step(s) of the 256^3 dam break taken from the GPU's current particle state,
-    with the reference's own solver (unpreconditioned CG, |r| < 1e-3 or 202 products)."""
+    workload with the reference's own solver (unpreconditioned CG, |r| < 1e-3 or 202 products):
+    `steps` step(s) from the GPU run's final particle state, or from `pos0` (at rest)."""
     from oracle import oracle as O
-    d = h.get_particles()
-    ref_kw = dict(kw, solver_mode=0, preconditioner=0)
+    if pos0 is None:
+        d = h.get_particles()
+        p, v = d["pos"], d["vel"]
+        start = "the GPU run's final state"
+    else:
+        p, v = pos0, np.zeros_like(pos0)
+        start = "rest"
+    ref_kw = {k: kw[k] for k in ("ext_force", "wall_level", "voxel_size")}
     o = O.Oracle((n, n, n), B=B, prm=O.params(**ref_kw))
-    o.set_particles(d["pos"], d["vel"])
-    npart = d["pos"].shape[0]
+    o.set_particles(p, v)
+    npart = p.shape[0]
+    its = []
     t = time.perf_counter()
     for _ in range(steps):
-        o.step(dt)
+        its.append(o.step(dt))
     el = time.perf_counter() - t
-    st = o.stats()
+    nfinal = o.num_particles()
     o.close()
     return {"value": round(npart * steps / el, 1), "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d step(s) of the %d^3 dam break (%d particles) from the GPU run's final state, "
-                      "reference CG rule (%d iterations), %.1f s" % (steps, n, npart, st["cg_iters"], el)}
+            "sample": "%d step(s) of the %d^3 dam break (%d particles) from %s, reference CG rule "
+                      "(mean %.0f iterations), %.1f s, %d particles at the end"
+                      % (steps, n, npart, start, float(np.mean(its)), el, nfinal)}
+
+
+def run_served(a, L):
+    """Config 4: a Poisson stream of independent tank solves through the continuous-batching server,
+    against the same requests solved one at a time."""
+    rng = np.random.default_rng(2024)
+    nreq = a.requests
+    reqs = []
+    for k in range(nreq):
+        edge = int(8 * round(np.exp(rng.uniform(np.log(32), np.log(128))) / 8))
+        ppc = int(rng.integers(4, 9))
+        fx, fy = rng.uniform(0.2, 0.7, 2)
+        pos = L.seed_box(edge, edge, edge, fx * edge, fy * edge, float(edge), ppc, 100 + k)
+        kw = solver_kw(a, L, edge)
+        reqs.append((L.grid_desc(edge, edge, edge, block_size=a.block), L.default_params(**kw), pos,
+                     np.zeros_like(pos)))
+    total_particles = sum(r[2].shape[0] for r in reqs)
+    print("bench: %d requests, %d particles in total" % (nreq, total_particles), file=sys.stderr, flush=True)
+    budget = 200 << 30
+    # isolated solve time (sequential server, no arrival gaps) sets the Poisson rate
+    warm = L.Server(budget, 1)
+    for g, p, pos, vel in reqs[:8]:
+        warm.submit(g, p, pos, vel, steps=10, dt=DT)
+    t = time.perf_counter()
+    warm.run()
+    iso = (time.perf_counter() - t) / 8
+    mean_gap = 0.5 * iso
+    arrivals = np.cumsum(rng.exponential(mean_gap, nreq))
+    arrivals -= arrivals[0]
+
+    def serve(slots, with_arrivals):
+        srv = L.Server(budget, slots)
+        for k, (g, p, pos, vel) in enumerate(reqs):
+            srv.submit(g, p, pos, vel, steps=10, dt=DT, arrival_s=float(arrivals[k]) if with_arrivals else 0.0)
+        srv.run()
+        return srv.stats()
+
+    plain = serve(1, False)                 # one tank at a time, back to back
+    served = serve(a.slots, True)           # Poisson arrivals, continuous batching
+    return {
+        "value": round(served["particle_steps_per_s"], 1),
+        "ms_per_step": round(served["step_mean_ms"], 3),
+        "p50_ms": round(served["step_p50_ms"], 3),
+        "p99_ms": round(served["step_p99_ms"], 3),
+        "steps": int(served["steps"]),
+        "config": {"workload": "served batch: %d tank solves, edges log-uniform 32..128 (multiple of 8), 4..8 ppc, "
+                               "water fractions U(0.2,0.7), 10 steps each at dt=1/240, %s, Poisson "
+                               "arrivals (mean gap %.1f ms = 0.5 x isolated solve), %d slots, 200 GB budget"
+                               % (nreq, solver_desc(a), 1e3 * mean_gap, a.slots),
+                   "particles_total": total_particles, "max_resident": served["max_resident"],
+                   "peak_reserved_gb": round(served["peak_bytes"] / 2 ** 30, 2), "wall_s": round(served["wall_s"], 3)},
+        "plain_gpu": {"value": round(plain["particle_steps_per_s"], 1), "p50_ms": round(plain["step_p50_ms"], 3),
+                      "p99_ms": round(plain["step_p99_ms"], 3), "wall_s": round(plain["wall_s"], 3),
+                      "note": "same requests, one tank at a time"},
+        "vs_plain_gpu": round(served["particle_steps_per_s"] / max(plain["particle_steps_per_s"], 1e-9), 3),
+        "roofline": None,
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
+              "sharding design is described in DESIGN.md)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    from vare_amd import _lib as L
+
+    if a.config == 4:
+        res = run_served(a, L)
+        steps, cpu = res.pop("steps"), None
+    else:
+        n, wx, wy, dsteps = (64, 0.4, 0.6, 100) if a.config == 1 else (256, 0.5, 0.5, 200)
+        steps = a.steps if a.steps is not None else dsteps
+        h, kw, res = run_dambreak(a, L, n, wx, wy, steps)
+        cpu = None
+        if a.cpu_baseline == "on":
+            if a.config == 1:
+                pos0 = L.seed_box(n, n, n, wx * n, wy * n, float(n), a.ppc, 1)
+                cpu = cpu_baseline(h, n, a.block, kw, DT, a.cpu_steps or steps, pos0=pos0)
+            else:
+                cpu = cpu_baseline(h, n, a.block, kw, DT, a.cpu_steps or 1)
+    out = {
+        "metric": METRIC,
+        "value": res.pop("value"),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": res.pop("ms_per_step"),
+        "p50_ms": res.pop("p50_ms"),
+        "p99_ms": res.pop("p99_ms"),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": res.pop("config"),
+    }
+    out.update(res)
+    out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

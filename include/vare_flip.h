@@ -87,7 +87,9 @@ typedef struct vare_flip_params {
     float refine_tol;        /* > 0: FP64 iterative refinement of the FP32 PCG until the FP64
                                 residual |b - A x|/|b| <= refine_tol (pressure kept in FP64) */
     int refine_max;          /* refinement step cap, default 8 */
-    int reserved[2];
+    int velocity_fp16;       /* 1: particle velocities stored in FP16 (positions stay FP32); fixed at
+                                vare_flip_create */
+    int reserved[1];
 } vare_flip_params;
 
 /* Frame geometry after SparseFrame::initialize (SparseFrame.cu:72-117): padded dims, block
@@ -203,6 +205,11 @@ int vare_flip_get_removed_mask(vare_flip* h, uint8_t* out, size_t cap, size_t* n
  * pos may be NULL to count.  Host-side generator. */
 int vare_seed_box(int64_t nx, int64_t ny, int64_t nz, float wx, float wy, float wz, int ppc, uint32_t seed,
                   float* pos, size_t cap, size_t* n);
+/* Seed a pool of depth `depth` plus a sphere (centre c, radius r), all in voxel units, with the same
+ * per-cell rule as vare_seed_box (a cell is seeded when its centre is inside the union; a point is
+ * kept when it is inside).  The mixed-precision benchmark scene. */
+int vare_seed_pool_sphere(int64_t nx, int64_t ny, int64_t nz, float depth, float cx, float cy, float cz, float r,
+                          int ppc, uint32_t seed, float* pos, size_t cap, size_t* n);
 /* Rand(seed) of Random.h:18-27 (for checking the generator). */
 float vare_rand(uint32_t seed);
 

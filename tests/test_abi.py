@@ -84,3 +84,16 @@ def test_create_rejects_bad_grid_before_device_use():
     g = L.grid_desc(8, 8, 8)
     assert lib.vare_flip_create(C.byref(g), C.byref(p), 0, 0, C.byref(h)) == L.VARE_ENOTIMPL
     assert b"curl" in lib.vare_last_error()
+
+
+def test_seed_pool_sphere_region():
+    n = 32
+    pos = L.seed_pool_sphere(n, n, n, 0.3 * n, (0.5 * n, 0.7 * n, 0.5 * n), 0.2 * n, 4, seed=2)
+    c = np.array([0.5, 0.7, 0.5]) * n
+    inside = (pos[:, 1] < 0.3 * n) | (np.linalg.norm(pos - c, axis=1) < 0.2 * n)
+    assert inside.all()
+    # the pool part is exactly the box seeder's output for the same cells
+    box = L.seed_box(n, n, n, float(n), 0.3 * n, float(n), 4, seed=2)
+    pool = pos[pos[:, 1] < 0.3 * n]
+    assert pool.shape[0] >= box.shape[0]
+    assert {tuple(p) for p in box} <= {tuple(p) for p in pool}

@@ -385,3 +385,25 @@ def test_iterative_refinement_fp64():
     assert st["residual_norm"] <= 1e-8 * st["b_norm"]
     assert rel(h.cell_field(L.FIELD_PRESSURE), o64.field(O.FIELD_PRES)) < 1e-6
     assert rel(h.cell_field(L.FIELD_COMP), o64.field(O.FIELD_COMP)) < 1e-6
+
+
+def test_fp16_velocity_storage():
+    # particle velocities stored in FP16 (positions FP32): the step matches the FP32-storage step
+    # to FP16 rounding of the velocities (unit roundoff 2^-11 ~ 4.9e-4), and the FP16 values come
+    # back exactly representable
+    n = 32
+    pos, vel = make_scene(n, seed=12)
+    out = {}
+    for half in (0, 1):
+        g = L.grid_desc(n, n, n, block_size=8)
+        h = L.FlipHandle(g, L.default_params(ext_force=(0, -9.8, 0), wall_level=1e30, voxel_size=1 / n,
+                                             solver_mode=1, cg_rel_tol=1e-6, preconditioner=L.PRECOND_MG,
+                                             velocity_fp16=half))
+        h.set_particles(pos, vel)
+        for _ in range(3):
+            h.step(DT)
+        out[half] = h.get_particles()
+    v16 = out[1]["vel"]
+    np.testing.assert_array_equal(v16, v16.astype(np.float16).astype(np.float32))
+    assert rel(v16, out[0]["vel"]) < 2e-3
+    assert rel(out[1]["pos"], out[0]["pos"]) < 1e-5

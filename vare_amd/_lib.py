@@ -72,7 +72,8 @@ class FlipParams(C.Structure):
                 ("solver_mode", C.c_int), ("cg_abs_tol", C.c_float), ("cg_rel_tol", C.c_float),
                 ("max_iterations", C.c_int), ("preconditioner", C.c_int), ("mg_sweeps", C.c_int),
                 ("mg_omega", C.c_float), ("mg_coarse_sweeps", C.c_int), ("particle_path", C.c_int),
-                ("refine_tol", C.c_float), ("refine_max", C.c_int), ("reserved", C.c_int * 2)]
+                ("refine_tol", C.c_float), ("refine_max", C.c_int), ("velocity_fp16", C.c_int),
+                ("reserved", C.c_int * 1)]
 
 
 class FrameInfo(C.Structure):
@@ -144,6 +145,8 @@ SIGNATURES = {
     "vare_flip_get_binning": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "vare_seed_box": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_int,
                                 C.c_uint32, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vare_seed_pool_sphere": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
+                                        C.c_float, C.c_int, C.c_uint32, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vare_rand": (C.c_float, [C.c_uint32]),
     "vare_device_synchronize": (C.c_int, []),
     "vare_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
@@ -221,6 +224,17 @@ def seed_box(nx, ny, nz, wx, wy, wz, ppc, seed=1):
     check(lib.vare_seed_box(nx, ny, nz, wx, wy, wz, ppc, seed, None, 0, C.byref(n)))
     pos = np.empty((n.value, 3), np.float32)
     check(lib.vare_seed_box(nx, ny, nz, wx, wy, wz, ppc, seed, ptr(pos), n.value, C.byref(n)))
+    return pos
+
+
+def seed_pool_sphere(nx, ny, nz, depth, centre, radius, ppc, seed=1):
+    """Pool of depth `depth` plus a sphere, voxel space (see vare_seed_pool_sphere)."""
+    lib = load()
+    n = C.c_size_t()
+    args = (nx, ny, nz, depth, centre[0], centre[1], centre[2], radius, ppc, seed)
+    check(lib.vare_seed_pool_sphere(*args, None, 0, C.byref(n)))
+    pos = np.empty((n.value, 3), np.float32)
+    check(lib.vare_seed_pool_sphere(*args, ptr(pos), n.value, C.byref(n)))
     return pos
 
 

@@ -11,11 +11,13 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <thread>
 
 #include "../../include/vare_flip.h"
 #include "flip_kernels.h"
 #include "flip_mg.h"
 #include "flip_particles.h"
+#include "flip_stencil.h"
 
 using namespace vare;
 
@@ -85,7 +87,8 @@ struct vare_flip {
 
     // particles, SoA in reference order: px,py,pz,vx,vy,vz,adj,vort,curv
     size_t np = 0, pcap = 0;
-    float* pa[9] = {};
+    float* pa[9] = {};            // pa[3..5] hold FP16 values when vhalf != 0
+    int vhalf = 0;
     uint8_t* removed = nullptr;
     int* hole = nullptr;
     int* src = nullptr;
@@ -143,6 +146,15 @@ struct vare_flip {
     hipEvent_t step_ev[4] = {};
 
     size_t ncells() const { return (size_t)nb * (size_t)B * B * B; }
+    size_t vsz() const { return vhalf ? 2 : 4; }
+    void* vptr(int a, size_t off) const { return (char*)pa[3 + a] + off * vsz(); }
+    VelPtr vel() const { return VelPtr{{pa[3], pa[4], pa[5]}, vhalf}; }
+    PArrays parrays(size_t off = 0) const {
+        PArrays r;
+        for (int q = 0; q < 9; ++q) r.a[q] = (q >= 3 && q < 6) ? (float*)vptr(q - 3, off) : pa[q] + off;
+        r.half = vhalf;
+        return r;
+    }
     FrameK frame() const { return FrameK{fx, fy, fz, nx, ny, nz, slot, coord, nbr}; }
 };
 
@@ -181,7 +193,7 @@ int ensure_blocks(vare_flip* h, size_t nb) {
     cap = std::max(cap, nb);
     free_cells(h);
     const size_t nc = cap * (size_t)h->B * h->B * h->B;
-    const size_t npart = (size_t)kReduceGrid + 16;
+    const size_t npart = std::max((size_t)kReduceGrid, cap) + 16;
     CHK(dalloc(&h->nbr, cap * 6));
     CHK(dalloc(&h->bcount, cap + 2)); CHK(dalloc(&h->bstart, cap + 2));
     CHK(dalloc(&h->mass, nc)); CHK(dalloc(&h->vx, nc)); CHK(dalloc(&h->vy, nc)); CHK(dalloc(&h->vz, nc));
@@ -201,9 +213,12 @@ int ensure_particles(vare_flip* h, size_t n) {
     cap = std::max<size_t>(cap, 1024);
     if (cap > (size_t)0x7fffffff) return fail(VARE_ENOMEM, "particle capacity %zu exceeds 2^31-1", cap);
     float* na[9] = {};
-    for (int q = 0; q < 9; ++q) CHK(dalloc(&na[q], cap));
     for (int q = 0; q < 9; ++q) {
-        if (h->np) HIPCHK(hipMemcpyAsync(na[q], h->pa[q], h->np * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+        const size_t esz = (q >= 3 && q < 6) ? h->vsz() : sizeof(float);
+        char* p = nullptr;
+        CHK(dalloc(&p, cap * esz));
+        na[q] = (float*)p;
+        if (h->np) HIPCHK(hipMemcpyAsync(na[q], h->pa[q], h->np * esz, hipMemcpyDeviceToDevice, h->stream));
     }
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int q = 0; q < 9; ++q) { dfree(h->pa[q]); h->pa[q] = na[q]; }
@@ -369,11 +384,11 @@ template <int B> int stage_p2g(vare_flip* h) {
     tbegin(h, VARE_KERNEL_P2G);
     if (h->np && h->prm.particle_path == 0 && h->nb)
         hipLaunchKernelGGL(k_p2g_tile<B>, dim3(h->nb), dim3(WG), 0, h->stream, (const float*)h->pa[0],
-                           (const float*)h->pa[1], (const float*)h->pa[2], (const float*)h->pa[3], (const float*)h->pa[4],
-                           (const float*)h->pa[5], (const int*)h->perm, (const int*)h->bstart, h->frame(), cf);
+                           (const float*)h->pa[1], (const float*)h->pa[2], h->vel(), (const int*)h->perm,
+                           (const int*)h->bstart, h->frame(), cf);
     else if (h->np && h->prm.particle_path != 0)
         hipLaunchKernelGGL(k_p2g_atomic<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
-                           h->pa[3], h->pa[4], h->pa[5], (int)h->np, h->frame(), cf);
+                           h->vel(), (int)h->np, h->frame(), cf);
     tend(h, VARE_KERNEL_P2G);
     if (nc)
         hipLaunchKernelGGL(k_cell_finalize, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, cf, h->cx, h->cy, h->cz,
@@ -478,15 +493,30 @@ int mg_setup(vare_flip* h) {
 }
 
 // z = V-cycle(r) into the CG's z vector.
-void enqueue_vcycle(vare_flip* h) {
+template <int B> void enqueue_vcycle_t(vare_flip* h) {
     const MgFrame F = mg_frame(h);
     const float w = h->prm.mg_omega;
     const int nu = h->prm.mg_sweeps;
     const int* done = done_ptr(h);
     const int m = (int)h->mg.size() - 1;
+    const unsigned nb = (unsigned)h->nb;
+    constexpr unsigned B3 = (unsigned)(B * B * B);
     for (int l = 0; l < m; ++l) {
         const MgLevelK& L = h->mg[l];
         const MgLevelK& C = h->mg[l + 1];
+        if (l == 0) {           // fine level: block-aligned kernels with the block staged in LDS
+            if (!nb) continue;
+            if (nu == 1) {
+                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, L, C, F, w, 1, h->mg_zt[0],
+                                   done);
+            } else {
+                hipLaunchKernelGGL(k_mg0_smooth_first2<B>, dim3(nb), dim3(B3), 0, h->stream, L, F, w, h->mg_z[0],
+                                   done);
+                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, L, C, F, w, 0, h->mg_z[0],
+                                   done);
+            }
+            continue;
+        }
         if (nu == 1) {
             if (C.ncell)
                 hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(C.ncell)), dim3(WG), 0, h->stream, L, C, F, w, 1,
@@ -500,12 +530,24 @@ void enqueue_vcycle(vare_flip* h) {
                                    h->mg_z[l], done);
         }
     }
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, h->stream, h->mg[m], w, h->prm.mg_coarse_sweeps,
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kMgCoarseMax), 0, h->stream, h->mg[m], w, h->prm.mg_coarse_sweeps,
                        h->mg_z[m], done);
     for (int l = m - 1; l >= 0; --l) {
         const MgLevelK& L = h->mg[l];
         const MgLevelK& C = h->mg[l + 1];
         if (!L.ncell) continue;
+        if (l == 0) {
+            if (nu == 1) {
+                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, L, C, F, w,
+                                   (const float*)h->mg_zt[0], (const float*)h->mg_z[1], h->mg_z[0], done);
+            } else {
+                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, L, C, F, w,
+                                   (const float*)h->mg_z[0], (const float*)h->mg_z[1], h->mg_zt[0], done);
+                hipLaunchKernelGGL(k_mg0_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, L, F, w,
+                                   (const float*)h->mg_zt[0], h->mg_z[0], done);
+            }
+            continue;
+        }
         if (nu == 1) {
             hipLaunchKernelGGL(k_mg_prolong_smooth, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, C, F, w,
                                (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done);
@@ -518,14 +560,20 @@ void enqueue_vcycle(vare_flip* h) {
     }
 }
 
+void enqueue_vcycle(vare_flip* h) {
+    if (h->B == 4) enqueue_vcycle_t<4>(h);
+    else enqueue_vcycle_t<8>(h);
+}
+
 // One CG/PCG iteration (LinearSystemSolver.cpp:29-46); every kernel returns at once when the
 // device-side `done` flag is set, so the host can enqueue iterations ahead without syncing.
 template <int B> void enqueue_iteration(vare_flip* h, size_t nc, unsigned g) {
     tbegin(h, VARE_KERNEL_SPMV);
-    hipLaunchKernelGGL(k_cg_spmv<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), vecs(h), h->partials,
-                       (const CgState*)h->st);
+    if (h->nb)
+        hipLaunchKernelGGL(k_spmv_blk<B>, dim3(h->nb), dim3(B * B * B), 0, h->stream, h->frame(), vecs(h), h->partials,
+                           (const CgState*)h->st);
     tend(h, VARE_KERNEL_SPMV);
-    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, h->nb);
     tbegin(h, VARE_KERNEL_UPDATE);
     hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials, h->partials2,
                        (const CgState*)h->st);
@@ -596,20 +644,31 @@ template <int B> int stage_solve(vare_flip* h) {
                                (int)g);
             CHK(run_solver<B>(h, nc, g, cap, total_iters));
         }
-        hipLaunchKernelGGL(k_ir_round, dim3(g), dim3(WG), 0, h->stream, (int)nc, (const double*)h->x64, h->x);
     }
-    // FP64 recompute of the final residual from the FP32 pressure (reported with every solve)
-    hipLaunchKernelGGL(k_true_residual<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const float*)h->b,
-                       (const float*)h->x, (const uint16_t*)h->info, h->partials);
+    // FP64 recompute of the final residual from the pressure the gradient will use (the FP64
+    // pressure when refining, else the FP32 one); reported with every solve
+    if (refine)
+        hipLaunchKernelGGL((k_true_residual<B, double>), dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+                           (const float*)h->b, (const double*)h->x64, (const uint16_t*)h->info, h->partials);
+    else
+        hipLaunchKernelGGL((k_true_residual<B, float>), dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+                           (const float*)h->b, (const float*)h->x, (const uint16_t*)h->info, h->partials);
     hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
     HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     const unsigned ga = nblocks(std::max<size_t>(nc, 1));
     tbegin(h, VARE_KERNEL_APPLY);
-    if (nc) {
-        hipLaunchKernelGGL(k_press_extrap<B>, dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+    if (nc && refine) {
+        // pressure apply in FP64 on the refined pressure; the FP32 field keeps its rounding
+        hipLaunchKernelGGL((k_press_extrap<B, double>), dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+                           (const uint8_t*)h->type, h->x64);
+        hipLaunchKernelGGL((k_press_grad<B, double>), dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+                           (const uint8_t*)h->type, (const double*)h->x64, h->vox_wall, h->cx, h->cy, h->cz);
+        hipLaunchKernelGGL(k_ir_round, dim3(g), dim3(WG), 0, h->stream, (int)nc, (const double*)h->x64, h->x);
+    } else if (nc) {
+        hipLaunchKernelGGL((k_press_extrap<B, float>), dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
                            (const uint8_t*)h->type, h->x);
-        hipLaunchKernelGGL(k_press_grad<B>, dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+        hipLaunchKernelGGL((k_press_grad<B, float>), dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
                            (const uint8_t*)h->type, (const float*)h->x, h->vox_wall, h->cx, h->cy, h->cz);
     }
     tend(h, VARE_KERNEL_APPLY);
@@ -632,11 +691,11 @@ template <int B> int stage_g2p(vare_flip* h, float dt) {
     if (h->np && h->prm.particle_path == 0) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
         hipLaunchKernelGGL(k_g2p_tile<B>, dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
-                           h->pa[3], h->pa[4], h->pa[5], h->pa[6], (const int*)h->perm, (const int*)h->bstart, h->nb,
+                           h->vel(), h->pa[6], (const int*)h->perm, (const int*)h->bstart, h->nb,
                            h->frame(), gf, g2p_params(h, dt), h->removed, h->dint + 1);
     } else if (h->np)
-        hipLaunchKernelGGL(k_g2p<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2], h->pa[3],
-                           h->pa[4], h->pa[5], h->pa[6], (int)h->np, h->frame(), (const float*)h->cnt, (const float*)h->cx,
+        hipLaunchKernelGGL(k_g2p<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
+                           h->vel(), h->pa[6], (int)h->np, h->frame(), (const float*)h->cnt, (const float*)h->cx,
                            (const float*)h->cy, (const float*)h->cz, (const float*)h->vx, (const float*)h->vy,
                            (const float*)h->vz, g2p_params(h, dt), h->removed, h->dint + 1);
     tend(h, VARE_KERNEL_G2P);
@@ -662,8 +721,7 @@ int stage_remove(vare_flip* h) {
     HIPCHK(hipStreamSynchronize(h->stream));
     const int nmove = h->dint_host[0];
     if (nmove != h->dint_host[2]) return fail(VARE_EINVAL, "removal compaction mismatch (%d holes, %d survivors)", nmove, h->dint_host[2]);
-    PArrays pa;
-    for (int q = 0; q < 9; ++q) pa.a[q] = h->pa[q];
+    const PArrays pa = h->parrays();
     if (nmove)
         hipLaunchKernelGGL(k_remove_move, dim3(nblocks(nmove)), dim3(WG), 0, h->stream, nmove, (const int*)h->hole,
                            (const int*)h->src, pa);
@@ -807,6 +865,7 @@ int vare_flip_create(const vare_grid_desc* g, const vare_flip_params* prm, size_
         h->s[a] = L / (float)n3[a];
     }
     apply_params(h, p);
+    h->vhalf = p->velocity_fp16 ? 1 : 0;
     if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(VARE_EHIP, "hipStreamCreate failed"); }
     int rc = VARE_OK;
     if (rc == VARE_OK) rc = dalloc(&h->flag, (size_t)h->nblk);
@@ -852,6 +911,8 @@ int vare_flip_destroy(vare_flip* h) {
 int vare_flip_set_params(vare_flip* h, const vare_flip_params* p) {
     if (!h || !p) return fail(VARE_EINVAL, "null argument");
     CHK(validate_params(p));
+    if ((p->velocity_fp16 ? 1 : 0) != h->vhalf)
+        return fail(VARE_EINVAL, "velocity_fp16 is fixed when the handle is created");
     apply_params(h, p);
     return VARE_OK;
 }
@@ -869,40 +930,41 @@ int vare_flip_get_frame(const vare_flip* h, vare_frame_info* o) {
 static int upload_particles(vare_flip* h, size_t off, const float* pos, const float* vel, const float* vort,
                             const float* curv, const float* adj, size_t n, int space, int mem) {
     if (n == 0) return VARE_OK;
+    float* dv = nullptr;          // interleaved velocities on the device (converted to FP16 if stored so)
     if (mem == VARE_MEM_HOST) {
         if (!finite3(pos, n) || !finite3(vel, n)) return fail(VARE_ENONFINITE, "NaN/Inf in particle input");
         std::vector<float> tmp(n);
-        for (int q = 0; q < 6; ++q) {
-            const float* in = q < 3 ? pos : vel;
-            const int a = q % 3;
-            for (size_t i = 0; i < n; ++i) tmp[i] = in[3 * i + a];
-            HIPCHK(hipMemcpy(h->pa[q] + off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        for (int a = 0; a < 3; ++a) {
+            for (size_t i = 0; i < n; ++i) tmp[i] = pos[3 * i + a];
+            HIPCHK(hipMemcpy(h->pa[a] + off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
         }
+        CHK(dalloc(&dv, 3 * n));
+        int rc = hipMemcpy(dv, vel, 3 * n * sizeof(float), hipMemcpyHostToDevice) == hipSuccess
+                     ? VARE_OK : fail(VARE_EHIP, "velocity upload failed");
+        if (rc != VARE_OK) { dfree(dv); return rc; }
         const float* sc[3] = {adj, vort, curv};
         for (int q = 0; q < 3; ++q) {
             if (sc[q]) HIPCHK(hipMemcpy(h->pa[6 + q] + off, sc[q], n * sizeof(float), hipMemcpyHostToDevice));
             else CHK(fill_u32(h, h->pa[6 + q] + off, 0, n));
         }
     } else {
-        hipLaunchKernelGGL(k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, pos, h->pa[0] + off,
-                           h->pa[1] + off, h->pa[2] + off);
-        hipLaunchKernelGGL(k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, vel, h->pa[3] + off,
-                           h->pa[4] + off, h->pa[5] + off);
+        hipLaunchKernelGGL(k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, pos, (void*)(h->pa[0] + off),
+                           (void*)(h->pa[1] + off), (void*)(h->pa[2] + off), 0);
         const float* sc[3] = {adj, vort, curv};
         for (int q = 0; q < 3; ++q) {
             if (sc[q]) HIPCHK(hipMemcpyAsync(h->pa[6 + q] + off, sc[q], n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
             else CHK(fill_u32(h, h->pa[6 + q] + off, 0, n));
         }
-        HIPCHK(hipGetLastError());
     }
-    if (space == VARE_SPACE_WORLD) {
-        PArrays pa;
-        for (int q = 0; q < 9; ++q) pa.a[q] = h->pa[q] + off;
-        hipLaunchKernelGGL(k_to_voxel, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, pa, h->t[0], h->t[1], h->t[2],
-                           h->s[0], h->s[1], h->s[2]);
-        HIPCHK(hipGetLastError());
-    }
-    HIPCHK(hipStreamSynchronize(h->stream));
+    hipLaunchKernelGGL(k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, mem == VARE_MEM_HOST ? dv : vel,
+                       h->vptr(0, off), h->vptr(1, off), h->vptr(2, off), h->vhalf);
+    if (space == VARE_SPACE_WORLD)
+        hipLaunchKernelGGL(k_to_voxel, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, h->parrays(off), h->t[0],
+                           h->t[1], h->t[2], h->s[0], h->s[1], h->s[2]);
+    const hipError_t e = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(h->stream);
+    dfree(dv);
+    if (e != hipSuccess || e2 != hipSuccess) return fail(VARE_EHIP, "particle upload failed");
     return VARE_OK;
 }
 
@@ -932,43 +994,31 @@ int vare_flip_get_particles(vare_flip* h, float* pos, float* vel, float* adj, fl
     if (cap < h->np) return fail(VARE_EINVAL, "capacity %zu < %zu particles", cap, h->np);
     const size_t np = h->np;
     if (np == 0) return VARE_OK;
-    float* tmp[6] = {};
-    if (space == VARE_SPACE_WORLD) {
-        for (int q = 0; q < 6; ++q) CHK(dalloc(&tmp[q], np));
-        for (int q = 0; q < 6; ++q)
-            HIPCHK(hipMemcpyAsync(tmp[q], h->pa[q], np * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
-        PArrays pa;
-        for (int q = 0; q < 9; ++q) pa.a[q] = q < 6 ? tmp[q] : nullptr;
-        hipLaunchKernelGGL(k_to_world, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, pa, h->t[0], h->t[1], h->t[2],
-                           h->s[0], h->s[1], h->s[2]);
-    }
-    float* srcs[6];
-    for (int q = 0; q < 6; ++q) srcs[q] = space == VARE_SPACE_WORLD ? tmp[q] : h->pa[q];
-    int rc = VARE_OK;
+    const int world = space == VARE_SPACE_WORLD ? 1 : 0;
+    float* sc[3] = {adj, vort, curv};
     if (mem == VARE_MEM_HOST) {
-        std::vector<float> buf(np);
-        for (int q = 0; q < 6 && rc == VARE_OK; ++q) {
-            float* outp = q < 3 ? pos : vel;
-            if (!outp) continue;
-            if (hipMemcpyAsync(buf.data(), srcs[q], np * sizeof(float), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-                hipStreamSynchronize(h->stream) != hipSuccess) { rc = fail(VARE_EHIP, "particle download failed"); break; }
-            const int a = q % 3;
-            for (size_t i = 0; i < np; ++i) outp[3 * i + a] = buf[i];
-        }
-        float* sc[3] = {adj, vort, curv};
+        float* d = nullptr;
+        CHK(dalloc(&d, 6 * np));
+        hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(), world, h->t[0],
+                           h->t[1], h->t[2], h->s[0], h->s[1], h->s[2], pos ? d : nullptr, vel ? d + 3 * np : nullptr);
+        int rc = hipStreamSynchronize(h->stream) == hipSuccess ? VARE_OK : fail(VARE_EHIP, "particle export failed");
+        if (rc == VARE_OK && pos && hipMemcpy(pos, d, 3 * np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(VARE_EHIP, "particle download failed");
+        if (rc == VARE_OK && vel && hipMemcpy(vel, d + 3 * np, 3 * np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(VARE_EHIP, "particle download failed");
         for (int q = 0; q < 3 && rc == VARE_OK; ++q)
             if (sc[q] && hipMemcpy(sc[q], h->pa[6 + q], np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
                 rc = fail(VARE_EHIP, "particle download failed");
-    } else {
-        if (pos) hipLaunchKernelGGL(k_interleave, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, srcs[0], srcs[1], srcs[2], pos);
-        if (vel) hipLaunchKernelGGL(k_interleave, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, srcs[3], srcs[4], srcs[5], vel);
-        float* sc[3] = {adj, vort, curv};
-        for (int q = 0; q < 3; ++q)
-            if (sc[q]) (void)hipMemcpyAsync(sc[q], h->pa[6 + q], np * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
-        if (hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(VARE_EHIP, "particle copy failed");
+        dfree(d);
+        return rc;
     }
-    for (int q = 0; q < 6; ++q) dfree(tmp[q]);
-    return rc;
+    hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(), world, h->t[0],
+                       h->t[1], h->t[2], h->s[0], h->s[1], h->s[2], pos, vel);
+    for (int q = 0; q < 3; ++q)
+        if (sc[q]) HIPCHK(hipMemcpyAsync(sc[q], h->pa[6 + q], np * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return VARE_OK;
 }
 
 int vare_flip_num_particles(const vare_flip* h, size_t* n) {
@@ -980,8 +1030,7 @@ int vare_flip_num_particles(const vare_flip* h, size_t* n) {
 static int convert_all(vare_flip* h, bool to_voxel) {
     if (!h) return fail(VARE_EINVAL, "null handle");
     if (!h->np) return VARE_OK;
-    PArrays pa;
-    for (int q = 0; q < 9; ++q) pa.a[q] = h->pa[q];
+    const PArrays pa = h->parrays();
     if (to_voxel)
         hipLaunchKernelGGL(k_to_voxel, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, pa, h->t[0], h->t[1],
                            h->t[2], h->s[0], h->s[1], h->s[2]);
@@ -1135,28 +1184,72 @@ int vare_flip_get_removed_mask(vare_flip* h, uint8_t* out, size_t cap, size_t* n
     return VARE_OK;
 }
 
-int vare_seed_box(int64_t nx, int64_t ny, int64_t nz, float wx, float wy, float wz, int ppc, uint32_t seed, float* pos,
-                  size_t cap, size_t* n) {
+}  // extern "C"
+
+namespace {
+
+// Host seeding shared by the seeders: for every cell whose centre satisfies `inside`, ppc points
+// cell + (Rand(s+3q), Rand(s+3q+1), Rand(s+3q+2)), q = ppc*(i + nx*(j + ny*k)) + n, kept when the
+// point itself satisfies `inside`.  Cells are visited in (k, j, i) order; z-slabs are counted and
+// filled by parallel host threads, which keeps the order.
+template <class Inside>
+int seed_points(int64_t nx, int64_t ny, int64_t nz, int ppc, uint32_t seed, float* pos, size_t cap, size_t* n,
+                Inside inside) {
     if (nx <= 0 || ny <= 0 || nz <= 0 || ppc <= 0) return fail(VARE_EINVAL, "bad seeding arguments");
-    size_t cnt = 0;
-    for (int64_t k = 0; k < nz; ++k)
+    auto slab = [&](int64_t k, float* out, size_t limit) {
+        size_t cnt = 0;
         for (int64_t j = 0; j < ny; ++j)
             for (int64_t i = 0; i < nx; ++i) {
-                if (!((float)i + 0.5f < wx && (float)j + 0.5f < wy && (float)k + 0.5f < wz)) continue;
+                if (!inside((float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f)) continue;
                 const uint64_t cell = (uint64_t)i + (uint64_t)nx * ((uint64_t)j + (uint64_t)ny * (uint64_t)k);
                 for (int q = 0; q < ppc; ++q) {
                     const uint32_t qq = (uint32_t)((uint64_t)ppc * cell + (uint64_t)q);
                     const float px = (float)i + rand01(seed + 3u * qq);
                     const float py = (float)j + rand01(seed + 3u * qq + 1u);
                     const float pz = (float)k + rand01(seed + 3u * qq + 2u);
-                    if (!(px < wx && py < wy && pz < wz)) continue;
-                    if (pos && cnt < cap) { pos[3 * cnt] = px; pos[3 * cnt + 1] = py; pos[3 * cnt + 2] = pz; }
+                    if (!inside(px, py, pz)) continue;
+                    if (out && cnt < limit) { out[3 * cnt] = px; out[3 * cnt + 1] = py; out[3 * cnt + 2] = pz; }
                     cnt++;
                 }
             }
-    if (n) *n = cnt;
-    if (pos && cnt > cap) return fail(VARE_EINVAL, "capacity %zu < %zu points", cap, cnt);
+        return cnt;
+    };
+    const int nt = (int)std::max<unsigned>(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency()));
+    std::vector<size_t> counts((size_t)nz, 0);
+    auto run = [&](auto body) {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t]() { for (int64_t k = t; k < nz; k += nt) body(k); });
+        for (auto& x : th) x.join();
+    };
+    run([&](int64_t k) { counts[(size_t)k] = slab(k, nullptr, 0); });
+    size_t total = 0;
+    std::vector<size_t> offs((size_t)nz, 0);
+    for (int64_t k = 0; k < nz; ++k) { offs[(size_t)k] = total; total += counts[(size_t)k]; }
+    if (n) *n = total;
+    if (!pos) return VARE_OK;
+    if (total > cap) return fail(VARE_EINVAL, "capacity %zu < %zu points", cap, total);
+    run([&](int64_t k) { slab(k, pos + 3 * offs[(size_t)k], counts[(size_t)k]); });
     return VARE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vare_seed_box(int64_t nx, int64_t ny, int64_t nz, float wx, float wy, float wz, int ppc, uint32_t seed, float* pos,
+                  size_t cap, size_t* n) {
+    return seed_points(nx, ny, nz, ppc, seed, pos, cap, n,
+                       [=](float x, float y, float z) { return x < wx && y < wy && z < wz; });
+}
+
+int vare_seed_pool_sphere(int64_t nx, int64_t ny, int64_t nz, float depth, float cx, float cy, float cz, float r, int ppc,
+                          uint32_t seed, float* pos, size_t cap, size_t* n) {
+    const float r2 = r * r;
+    return seed_points(nx, ny, nz, ppc, seed, pos, cap, n, [=](float x, float y, float z) {
+        const float dx = x - cx, dy = y - cy, dz = z - cz;
+        return y < depth || dx * dx + dy * dy + dz * dz < r2;
+    });
 }
 
 int vare_device_synchronize(void) {

@@ -42,6 +42,20 @@ template <class T> __device__ __forceinline__ T rclamp(T x, T lo, T hi) {       
     return x;
 }
 
+// Particle velocity arrays, stored as FP32 or FP16 (half = 1).  The branch is uniform across a
+// launch.  FP16 stores round to nearest even.
+struct VelPtr {
+    void* p[3];
+    int half;
+    __device__ __forceinline__ float ld(int a, int i) const {
+        return half ? (float)((const _Float16*)p[a])[i] : ((const float*)p[a])[i];
+    }
+    __device__ __forceinline__ void st(int a, int i, float v) const {
+        if (half) ((_Float16*)p[a])[i] = (_Float16)v;
+        else ((float*)p[a])[i] = v;
+    }
+};
+
 // Frame seen by kernels.
 struct FrameK {
     int fx, fy, fz;

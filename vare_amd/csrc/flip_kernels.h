@@ -178,14 +178,13 @@ struct CellFields {
 
 template <int B>
 __global__ void k_p2g_atomic(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-                             const float* __restrict__ ux, const float* __restrict__ uy, const float* __restrict__ uz,
-                             int np, FrameK f, CellFields cf) {
+                             VelPtr u, int np, FrameK f, CellFields cf) {
     const int n = blockIdx.x * WG + threadIdx.x;
     if (n >= np) return;
     const float x = px[n], y = py[n], z = pz[n];
     // Grid::inside (Grid.h:407-417)
     if (x < 0.f || x > (float)f.nx || y < 0.f || y > (float)f.ny || z < 0.f || z > (float)f.nz) return;
-    const float vx = ux[n], vy = uy[n], vz = uz[n];
+    const float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
     const int ci = rmax((int)(x - 0.5f), 0), cj = rmax((int)(y - 0.5f), 0), ck = rmax((int)(z - 0.5f), 0);
     const int ie = rmin(ci + 1, f.nx - 1), je = rmin(cj + 1, f.ny - 1), ke = rmin(ck + 1, f.nz - 1);
     for (int k = ck; k <= ke; ++k)
@@ -532,25 +531,26 @@ __global__ void k_cg_pupdate(int nc, CgVecs v, const CgState* __restrict__ st) {
 // ------------------------------------------------------------------------------------------------
 // Pressure apply.  The solve runs in place on the cell layout, so FluidProjection.cu:297-306
 // (scatter x to the field) has no counterpart.  Air extrapolation, FluidProjection.cu:314-335:
-// each active air cell takes the mean pressure of its fluid neighbours.
-template <int B>
-__global__ void k_press_extrap(int nc, FrameK f, const uint8_t* __restrict__ type, float* __restrict__ pres) {
+// each active air cell takes the mean pressure of its fluid neighbours.  PT is the pressure type:
+// float, or double for the FP64 pressure of iterative refinement (arithmetic in PT).
+template <int B, class PT>
+__global__ void k_press_extrap(int nc, FrameK f, const uint8_t* __restrict__ type, PT* __restrict__ pres) {
     const int c = blockIdx.x * WG + threadIdx.x;
     if (c >= nc) return;
     if (type[c] != kAir) return;
-    float press = 0.f;
+    PT press = PT(0);
     int cnt = 0;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int id = cell_neighbor<B>(f, c, q);
         if (id >= 0 && type[id] == kFluid) { press += pres[id]; cnt++; }
     }
-    if (cnt > 0) pres[c] = press / (float)cnt;
+    if (cnt > 0) pres[c] = press / (PT)cnt;
 }
 
 // Gradient subtract, FluidProjection.cu:342-378.
-template <int B>
-__global__ void k_press_grad(int nc, FrameK f, const uint8_t* __restrict__ type, const float* __restrict__ pres,
+template <int B, class PT>
+__global__ void k_press_grad(int nc, FrameK f, const uint8_t* __restrict__ type, const PT* __restrict__ pres,
                              float wall_level, float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz) {
     const int c = blockIdx.x * WG + threadIdx.x;
     if (c >= nc) return;
@@ -558,33 +558,33 @@ __global__ void k_press_grad(int nc, FrameK f, const uint8_t* __restrict__ type,
     int i, j, k;
     cell_coords<B>(f, c, i, j, k);
     const uint8_t wall = ((float)j + 0.5f <= wall_level + 1.f) ? kSolid : kAir;
-    const float p = pres[c];
+    const PT p = pres[c];
     int nid[6];
     uint8_t tn[6];
-    float pv[6];
+    PT pv[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         nid[q] = cell_neighbor<B>(f, c, q);
         tn[q] = nid[q] == -2 ? wall : (nid[q] < 0 ? (uint8_t)kAir : type[nid[q]]);
-        pv[q] = nid[q] >= 0 ? pres[nid[q]] : 0.f;
+        pv[q] = nid[q] >= 0 ? pres[nid[q]] : PT(0);
     }
-    float nb[6];
+    PT nb[6];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const int lo = 2 * a, hi = 2 * a + 1;
-        nb[lo] = nid[lo] == -2 ? 2.f * p - pv[hi] : pv[lo];
-        nb[hi] = nid[hi] == -2 ? 2.f * p - pv[lo] : pv[hi];
+        nb[lo] = nid[lo] == -2 ? PT(2) * p - pv[hi] : pv[lo];
+        nb[hi] = nid[hi] == -2 ? PT(2) * p - pv[lo] : pv[hi];
     }
-    float pf[6];
+    PT pf[6];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const int lo = 2 * a, hi = 2 * a + 1;
-        pf[lo] = tn[lo] == kSolid ? 2.f * p - nb[hi] : (tn[lo] == kAir ? -nb[lo] : nb[lo]);
-        pf[hi] = tn[hi] == kSolid ? 2.f * p - nb[lo] : (tn[hi] == kAir ? -nb[hi] : nb[hi]);
+        pf[lo] = tn[lo] == kSolid ? PT(2) * p - nb[hi] : (tn[lo] == kAir ? -nb[lo] : nb[lo]);
+        pf[hi] = tn[hi] == kSolid ? PT(2) * p - nb[lo] : (tn[hi] == kAir ? -nb[hi] : nb[hi]);
     }
-    cx[c] = cx[c] - (pf[1] - pf[0]) * 0.5f;
-    cy[c] = cy[c] - (pf[3] - pf[2]) * 0.5f;
-    cz[c] = cz[c] - (pf[5] - pf[4]) * 0.5f;
+    cx[c] = (float)((PT)cx[c] - (pf[1] - pf[0]) * PT(0.5));
+    cy[c] = (float)((PT)cy[c] - (pf[3] - pf[2]) * PT(0.5));
+    cz[c] = (float)((PT)cz[c] - (pf[5] - pf[4]) * PT(0.5));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -604,8 +604,8 @@ struct G2PParams {
 };
 
 template <int B>
-__global__ void k_g2p(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz, float* __restrict__ ux,
-                      float* __restrict__ uy, float* __restrict__ uz, float* __restrict__ adj, int np, FrameK f,
+__global__ void k_g2p(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz, VelPtr u,
+                      float* __restrict__ adj, int np, FrameK f,
                       const float* __restrict__ cnt, const float* __restrict__ cx, const float* __restrict__ cy,
                       const float* __restrict__ cz, const float* __restrict__ vx0, const float* __restrict__ vy0,
                       const float* __restrict__ vz0, G2PParams gp, uint8_t* __restrict__ removed,
@@ -614,7 +614,7 @@ __global__ void k_g2p(float* __restrict__ px, float* __restrict__ py, float* __r
     int rem = 0;
     if (n < np) {
         float x = px[n], y = py[n], z = pz[n];
-        float vx = ux[n], vy = uy[n], vz = uz[n];
+        float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
         // lerp corner (SparseField.h:152-160)
         const int i = rclamp((int)(x - 0.5f), 0, f.nx - 2);
         const int j = rclamp((int)(y - 0.5f), 0, f.ny - 2);
@@ -691,7 +691,7 @@ __global__ void k_g2p(float* __restrict__ px, float* __restrict__ py, float* __r
         vy += gp.gy * gp.dt;
         vz += gp.gz * gp.dt;
         px[n] = x; py[n] = y; pz[n] = z;
-        ux[n] = vx; uy[n] = vy; uz[n] = vz;
+        u.st(0, n, vx); u.st(1, n, vy); u.st(2, n, vz);
     }
     // wave-aggregated removal count
     const unsigned long long m = __ballot(rem);
@@ -702,7 +702,9 @@ __global__ void k_g2p(float* __restrict__ px, float* __restrict__ py, float* __r
 // the k-th removed slot below newN, ascending, receives the k-th survivor at or above newN,
 // descending).
 struct PArrays {
-    float* a[9];     // px,py,pz,vx,vy,vz,adj,vort,curv
+    float* a[9];     // px,py,pz,vx,vy,vz,adj,vort,curv (vx..vz are FP16 when half != 0)
+    int half;
+    __device__ __forceinline__ VelPtr vel() const { return VelPtr{{a[3], a[4], a[5]}, half}; }
 };
 __global__ void k_remove_move(int nmove, const int* __restrict__ hole, const int* __restrict__ src, PArrays pa) {
     const int k = blockIdx.x * WG + threadIdx.x;
@@ -710,7 +712,10 @@ __global__ void k_remove_move(int nmove, const int* __restrict__ hole, const int
     const int d = hole[k];
     const int s = src[nmove - 1 - k];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) pa.a[q][d] = pa.a[q][s];
+    for (int q = 0; q < 9; ++q) {
+        if (pa.half && q >= 3 && q < 6) ((_Float16*)pa.a[q])[d] = ((const _Float16*)pa.a[q])[s];
+        else pa.a[q][d] = pa.a[q][s];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -718,36 +723,56 @@ __global__ void k_remove_move(int nmove, const int* __restrict__ hole, const int
 __global__ void k_to_voxel(int np, PArrays pa, float tx, float ty, float tz, float sx, float sy, float sz) {
     const int n = blockIdx.x * WG + threadIdx.x;
     if (n >= np) return;
+    const VelPtr u = pa.vel();
     pa.a[0][n] = (pa.a[0][n] - tx) / sx;
     pa.a[1][n] = (pa.a[1][n] - ty) / sy;
     pa.a[2][n] = (pa.a[2][n] - tz) / sz;
-    pa.a[3][n] = pa.a[3][n] / sx;
-    pa.a[4][n] = pa.a[4][n] / sy;
-    pa.a[5][n] = pa.a[5][n] / sz;
+    u.st(0, n, u.ld(0, n) / sx);
+    u.st(1, n, u.ld(1, n) / sy);
+    u.st(2, n, u.ld(2, n) / sz);
 }
 __global__ void k_to_world(int np, PArrays pa, float tx, float ty, float tz, float sx, float sy, float sz) {
     const int n = blockIdx.x * WG + threadIdx.x;
     if (n >= np) return;
+    const VelPtr u = pa.vel();
     pa.a[0][n] = pa.a[0][n] * sx + tx;
     pa.a[1][n] = pa.a[1][n] * sy + ty;
     pa.a[2][n] = pa.a[2][n] * sz + tz;
-    pa.a[3][n] = pa.a[3][n] * sx;
-    pa.a[4][n] = pa.a[4][n] * sy;
-    pa.a[5][n] = pa.a[5][n] * sz;
+    u.st(0, n, u.ld(0, n) * sx);
+    u.st(1, n, u.ld(1, n) * sy);
+    u.st(2, n, u.ld(2, n) * sz);
 }
 
-// xyz-interleaved <-> SoA on the device (for device-pointer particle I/O).
-__global__ void k_deinterleave(int n, const float* __restrict__ in, float* __restrict__ a, float* __restrict__ b,
-                               float* __restrict__ c) {
-    const int i = blockIdx.x * WG + threadIdx.x;
-    if (i >= n) return;
-    a[i] = in[3 * i]; b[i] = in[3 * i + 1]; c[i] = in[3 * i + 2];
+// Particle export: interleaved FP32 positions/velocities (voxel space, or world space when world
+// != 0: Grid::worldPoint / worldVector, Grid.h:431-443).  Either output may be null.
+__global__ void k_export(int np, PArrays pa, int world, float tx, float ty, float tz, float sx, float sy, float sz,
+                         float* __restrict__ pos, float* __restrict__ vel) {
+    const int n = blockIdx.x * WG + threadIdx.x;
+    if (n >= np) return;
+    const VelPtr u = pa.vel();
+    float x = pa.a[0][n], y = pa.a[1][n], z = pa.a[2][n];
+    float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
+    if (world) {
+        x = x * sx + tx; y = y * sy + ty; z = z * sz + tz;
+        vx = vx * sx; vy = vy * sy; vz = vz * sz;
+    }
+    if (pos) { pos[3 * n] = x; pos[3 * n + 1] = y; pos[3 * n + 2] = z; }
+    if (vel) { vel[3 * n] = vx; vel[3 * n + 1] = vy; vel[3 * n + 2] = vz; }
 }
-__global__ void k_interleave(int n, const float* __restrict__ a, const float* __restrict__ b, const float* __restrict__ c,
-                             float* __restrict__ out) {
+
+// xyz-interleaved FP32 <-> SoA on the device (for device-pointer particle I/O and FP16 velocity
+// storage): in/out are float[3n]; the SoA side is FP32 or FP16 (half).
+__global__ void k_deinterleave(int n, const float* __restrict__ in, void* a, void* b, void* c, int half) {
     const int i = blockIdx.x * WG + threadIdx.x;
     if (i >= n) return;
-    out[3 * i] = a[i]; out[3 * i + 1] = b[i]; out[3 * i + 2] = c[i];
+    const VelPtr u{{a, b, c}, half};
+    u.st(0, i, in[3 * i]); u.st(1, i, in[3 * i + 1]); u.st(2, i, in[3 * i + 2]);
+}
+__global__ void k_interleave(int n, const void* a, const void* b, const void* c, int half, float* __restrict__ out) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    const VelPtr u{{(void*)a, (void*)b, (void*)c}, half};
+    out[3 * i] = u.ld(0, i); out[3 * i + 1] = u.ld(1, i); out[3 * i + 2] = u.ld(2, i);
 }
 
 // Mark field in the reference encoding (uint64: 0 air, 1 solid, 4+fluid rank).
@@ -834,9 +859,9 @@ __global__ void k_fin_ir(CgState* __restrict__ st, const double* __restrict__ pa
     st->done = 0;
 }
 
-// |b - A x|^2 in FP64 from the FP32 pressure (reported with every solve).
-template <int B>
-__global__ void k_true_residual(int nc, FrameK f, const float* __restrict__ b, const float* __restrict__ x,
+// |b - A x|^2 in FP64 from the pressure x (FP32, or FP64 when refining); reported with every solve.
+template <int B, class PT>
+__global__ void k_true_residual(int nc, FrameK f, const float* __restrict__ b, const PT* __restrict__ x,
                                 const uint16_t* __restrict__ info, double* __restrict__ partials) {
     __shared__ double sh[4];
     double rr = 0.0;

@@ -18,7 +18,7 @@
 
 namespace vare {
 
-constexpr int kMgCoarseMax = 4096;
+constexpr int kMgCoarseMax = 512;
 
 struct MgLevelK {
     int dense;          // 0: sparse level of edge 1<<sh; 1: dense grid
@@ -252,40 +252,44 @@ __global__ void k_mg_prolong_smooth(MgLevelK L, MgLevelK C, MgFrame F, float ome
 }
 
 // Coarsest level (dense, <= kMgCoarseMax cells): `sweeps` Jacobi sweeps from zero by one
-// workgroup of 1024 threads in LDS.
-__global__ void __launch_bounds__(1024) k_mg_coarse(MgLevelK L, float omega, int sweeps, float* __restrict__ zout,
-                                                    const int* done) {
+// workgroup, one cell per thread, the iterate in LDS and the cell's row in registers.
+__global__ void __launch_bounds__(kMgCoarseMax) k_mg_coarse(MgLevelK L, float omega, int sweeps,
+                                                            float* __restrict__ zout, const int* done) {
     if (*done) return;
     __shared__ float za[kMgCoarseMax];
     __shared__ float zb[kMgCoarseMax];
     const int n = L.ncell;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) za[c] = 0.f;
+    const int c = threadIdx.x;
+    const bool act = c < n;
+    const uint16_t in = act ? L.info[c] : (uint16_t)0;
+    const float r = act ? L.r[c] : 0.f;
+    const bool fl = mg_fluid(in);
+    const float d = fl ? mg_diag(in) : 1.f;
+    const int sx = 1, sy = L.dx, sz = L.dx * L.dy;
+    if (act) za[c] = 0.f;
     __syncthreads();
     float* cur = za;
     float* nxt = zb;
     for (int s = 0; s < sweeps; ++s) {
-        for (int c = threadIdx.x; c < n; c += blockDim.x) {
-            const uint16_t in = L.info[c];
+        if (act) {
             float out = 0.f;
-            if (mg_fluid(in)) {
-                const float d = mg_diag(in);
-                float az = d * cur[c];
-                const int i = c % L.dx, j = (c / L.dx) % L.dy;
-                if (in & 1u) az -= cur[c - 1];
-                if (in & 2u) az -= cur[c + 1];
-                if (in & 4u) az -= cur[c - L.dx];
-                if (in & 8u) az -= cur[c + L.dx];
-                if (in & 16u) az -= cur[c - L.dx * L.dy];
-                if (in & 32u) az -= cur[c + L.dx * L.dy];
-                (void)i; (void)j;
-                out = cur[c] + omega * (L.r[c] - az) / d;
+            if (fl) {
+                const float zc = cur[c];
+                float az = d * zc;
+                if (in & 1u) az -= cur[c - sx];
+                if (in & 2u) az -= cur[c + sx];
+                if (in & 4u) az -= cur[c - sy];
+                if (in & 8u) az -= cur[c + sy];
+                if (in & 16u) az -= cur[c - sz];
+                if (in & 32u) az -= cur[c + sz];
+                out = zc + omega * (r - az) / d;
             }
             nxt[c] = out;
         }
         __syncthreads();
         float* t = cur; cur = nxt; nxt = t;
     }
-    for (int c = threadIdx.x; c < n; c += blockDim.x) zout[c] = cur[c];
+    if (act) zout[c] = cur[c];
 }
 
 }  // namespace vare

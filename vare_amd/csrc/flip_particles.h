@@ -75,6 +75,10 @@ __global__ void __launch_bounds__(1024) k_psort(const int* __restrict__ bstart, 
     const int s1 = b + 1 < nbucket ? bstart[b + 1] : np;
     const int n = s1 - s0;
     if (n <= 1) return;
+    // arrival order is usually already ascending: skip buckets that need no sorting
+    int unsorted = 0;
+    for (int t = s0 + threadIdx.x; t + 1 < s1; t += blockDim.x) unsorted |= perm[t] > perm[t + 1];
+    if (!__syncthreads_or(unsorted)) return;
     if (n > kSortMax) {
         if (threadIdx.x == 0) atomicAdd(overflow, 1);
         return;
@@ -129,8 +133,7 @@ __device__ __forceinline__ int tile_cell(const int* nsl, int ti, int tj, int tk)
 // :198-228 per particle), then one global atomic add per touched tile cell and field.
 template <int B>
 __global__ void __launch_bounds__(WG) k_p2g_tile(const float* __restrict__ px, const float* __restrict__ py,
-                                                const float* __restrict__ pz, const float* __restrict__ ux,
-                                                const float* __restrict__ uy, const float* __restrict__ uz,
+                                                const float* __restrict__ pz, VelPtr u,
                                                 const int* __restrict__ perm, const int* __restrict__ bstart, FrameK f,
                                                 CellFields cf) {
     constexpr int T = B + 2, T3 = T * T * T;
@@ -144,35 +147,73 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const float* __restrict__ px, c
     __syncthreads();
     const int t0 = f.coord[s];
     const int ox = (t0 % f.fx) * B - 1, oy = ((t0 / f.fx) % f.fy) * B - 1, oz = (t0 / (f.fx * f.fy)) * B - 1;
-    for (int q = p0 + threadIdx.x; q < p1; q += WG) {
+    // Each thread walks a contiguous run of the bucket.  Particles sharing the base cell
+    // c = max((int)(p-0.5), 0) touch the same (up to) 8 cells, so their contributions are summed in
+    // registers and flushed to the LDS tile only when the base cell changes.
+    const int cnt = p1 - p0;
+    const int chunk = (cnt + WG - 1) / WG;
+    const int q0 = p0 + threadIdx.x * chunk;
+    const int q1 = rmin(q0 + chunk, p1);
+    float am[8], ax[8], ay[8], az[8], ac[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) { am[c] = 0.f; ax[c] = 0.f; ay[c] = 0.f; az[c] = 0.f; ac[c] = 0.f; }
+    int cur = -1;                       // packed tile coordinates of the current base cell
+    unsigned cur_mask = 0;              // corners that exist and are writable
+    auto flush = [&]() {
+        if (cur < 0) return;
+        const int bi = cur & 1023, bj = (cur >> 10) & 1023, bk = cur >> 20;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (cur_mask & (1u << c)) {
+                const int t = ((bk + (c >> 2)) * T + (bj + ((c >> 1) & 1))) * T + (bi + (c & 1));
+                atomicAdd(tm + t, am[c]);
+                atomicAdd(tx + t, ax[c]);
+                atomicAdd(ty + t, ay[c]);
+                atomicAdd(tz + t, az[c]);
+                atomicAdd(tc + t, ac[c]);
+            }
+            am[c] = 0.f; ax[c] = 0.f; ay[c] = 0.f; az[c] = 0.f; ac[c] = 0.f;    // every corner restarts
+        }
+    };
+    for (int q = q0; q < q1; ++q) {
         const int n = perm[q];
         const float x = px[n], y = py[n], z = pz[n];
         if (x < 0.f || x > (float)f.nx || y < 0.f || y > (float)f.ny || z < 0.f || z > (float)f.nz) continue;
-        const float vx = ux[n], vy = uy[n], vz = uz[n];
+        const float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
         const int ci = rmax((int)(x - 0.5f), 0), cj = rmax((int)(y - 0.5f), 0), ck = rmax((int)(z - 0.5f), 0);
-        const int ie = rmin(ci + 1, f.nx - 1), je = rmin(cj + 1, f.ny - 1), ke = rmin(ck + 1, f.nz - 1);
-        for (int k = ck; k <= ke; ++k)
-            for (int j = cj; j <= je; ++j)
-                for (int i = ci; i <= ie; ++i) {
-                    const int ti = i - ox, tj = j - oy, tk = k - oz;
-                    const int ai = ti - 1, aj = tj - 1, ak = tk - 1;
-                    const int dx = ai < 0 ? -1 : (ai >= B ? 1 : 0);
-                    const int dy = aj < 0 ? -1 : (aj >= B ? 1 : 0);
-                    const int dz = ak < 0 ? -1 : (ak >= B ? 1 : 0);
-                    if (nsl[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] < 0) continue;     // isWritable
-                    const float ddx = x - ((float)i + 0.5f), ddy = y - ((float)j + 0.5f), ddz = z - ((float)k + 0.5f);
-                    const float xw = rmax(0.f, 1.f - rabs(ddx));
-                    const float yw = rmax(0.f, 1.f - rabs(ddy));
-                    const float zw = rmax(0.f, 1.f - rabs(ddz));
-                    const float w = xw * yw * zw + 1e-05f;
-                    const int t = (tk * T + tj) * T + ti;
-                    atomicAdd(tm + t, w);
-                    atomicAdd(tx + t, w * vx);
-                    atomicAdd(ty + t, w * vy);
-                    atomicAdd(tz + t, w * vz);
-                    atomicAdd(tc + t, 1.f);
-                }
+        const int ti = ci - ox, tj = cj - oy, tk = ck - oz;
+        const int key = (tk << 20) | (tj << 10) | ti;
+        if (key != cur) {
+            flush();
+            cur = key;
+            cur_mask = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int i = ci + (c & 1), j = cj + ((c >> 1) & 1), k = ck + (c >> 2);
+                if (i > f.nx - 1 || j > f.ny - 1 || k > f.nz - 1) continue;    // Min(c+1, n-1) bound
+                const int ai = i - ox - 1, aj = j - oy - 1, ak = k - oz - 1;
+                const int dx = ai < 0 ? -1 : (ai >= B ? 1 : 0);
+                const int dy = aj < 0 ? -1 : (aj >= B ? 1 : 0);
+                const int dz = ak < 0 ? -1 : (ak >= B ? 1 : 0);
+                if (nsl[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] >= 0) cur_mask |= 1u << c;   // isWritable
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int i = ci + (c & 1), j = cj + ((c >> 1) & 1), k = ck + (c >> 2);
+            const float ddx = x - ((float)i + 0.5f), ddy = y - ((float)j + 0.5f), ddz = z - ((float)k + 0.5f);
+            const float xw = rmax(0.f, 1.f - rabs(ddx));
+            const float yw = rmax(0.f, 1.f - rabs(ddy));
+            const float zw = rmax(0.f, 1.f - rabs(ddz));
+            const float w = xw * yw * zw + 1e-05f;
+            am[c] += w;
+            ax[c] += w * vx;
+            ay[c] += w * vy;
+            az[c] += w * vz;
+            ac[c] += 1.f;
+        }
     }
+    flush();
     __syncthreads();
     for (int t = threadIdx.x; t < T3; t += WG) {
         if (tc[t] == 0.f) continue;
@@ -190,11 +231,10 @@ __global__ void __launch_bounds__(WG) k_p2g_tile(const float* __restrict__ px, c
 // G2P for one particle given a cell-value fetcher (FLIPMethod.cu:86-171, SparseField.h:150-196).
 template <class Fetch>
 __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
-                                            float* __restrict__ ux, float* __restrict__ uy, float* __restrict__ uz,
-                                            float* __restrict__ adj, const FrameK& f, const G2PParams& gp,
+                                            const VelPtr& u, float* __restrict__ adj, const FrameK& f, const G2PParams& gp,
                                             uint8_t* __restrict__ removed, Fetch fetch) {
     float x = px[n], y = py[n], z = pz[n];
-    float vx = ux[n], vy = uy[n], vz = uz[n];
+    float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
     const int i = rclamp((int)(x - 0.5f), 0, f.nx - 2);
     const int j = rclamp((int)(y - 0.5f), 0, f.ny - 2);
     const int k = rclamp((int)(z - 0.5f), 0, f.nz - 2);
@@ -258,7 +298,7 @@ __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float
     vy += gp.gy * gp.dt;
     vz += gp.gz * gp.dt;
     px[n] = x; py[n] = y; pz[n] = z;
-    ux[n] = vx; uy[n] = vy; uz[n] = vz;
+    u.st(0, n, vx); u.st(1, n, vy); u.st(2, n, vz);
     return rem;
 }
 
@@ -276,8 +316,7 @@ struct G2PFields {
 // outside the domain) reads the fields through the block table.
 template <int B>
 __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
-                                                float* __restrict__ ux, float* __restrict__ uy, float* __restrict__ uz,
-                                                float* __restrict__ adj, const int* __restrict__ perm,
+                                                VelPtr u, float* __restrict__ adj, const int* __restrict__ perm,
                                                 const int* __restrict__ bstart, int nb, FrameK f, G2PFields gf,
                                                 G2PParams gp, uint8_t* __restrict__ removed, int* __restrict__ nremoved) {
     constexpr int T = B + 2, T3 = T * T * T;
@@ -296,7 +335,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* 
             for (int fi = 0; fi < 7; ++fi) v[fi][q] = c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
         };
         for (int q = p0 + threadIdx.x; q < p1; q += WG)
-            rem_count += g2p_particle(perm[q], px, py, pz, ux, uy, uz, adj, f, gp, removed, fetch);
+            rem_count += g2p_particle(perm[q], px, py, pz, u, adj, f, gp, removed, fetch);
     } else {
         load_nbhd(f, s, nsl);
         __syncthreads();
@@ -321,7 +360,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* 
             }
         };
         for (int q = p0 + threadIdx.x; q < p1; q += WG)
-            rem_count += g2p_particle(perm[q], px, py, pz, ux, uy, uz, adj, f, gp, removed, fetch);
+            rem_count += g2p_particle(perm[q], px, py, pz, u, adj, f, gp, removed, fetch);
     }
     // wave-aggregated removal count
     const int w = wave_sum_i(rem_count);
