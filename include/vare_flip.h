@@ -191,10 +191,11 @@ int vare_flip_get_block_table(vare_flip* h, int64_t* pointers, int32_t* coords, 
 /* Copy a cell field (VARE_FIELD_*) in storage order; n = number of cells. */
 int vare_flip_get_cell_field(vare_flip* h, int which, void* out, size_t cap_bytes, size_t* n);
 /* Binning permutation of the last prepare stage (particle_path 0): perm[n] lists particle indices
- * grouped by bucket (bucket b < nb = active block slot b, bucket nb = particles outside the domain),
- * ascending inside a bucket -- a stable counting sort by bucket.  bstart[nb+2] holds the bucket
- * starts.  Either pointer may be NULL; n and nbucket (= nb+1) are always written.  No counterpart
- * in the reference (its particles are never binned). */
+ * grouped by bin -- bin c < Nc is the cell with storage index c, holding the particles whose P2G
+ * base cell max((int)(p-0.5),0) it is; bin Nc holds the particles outside [0,n]^3 -- ascending
+ * inside a bin: a stable counting sort by bin.  bstart[Nc+2] holds the bin starts.  Either pointer
+ * may be NULL; n and nbucket (= Nc+1) are always written.  No counterpart in the reference (its
+ * particles are never binned). */
 int vare_flip_get_binning(vare_flip* h, int32_t* perm, int32_t* bstart, size_t cap, size_t* n, size_t* nbucket);
 /* Per-particle removal flags of the last step, in the order the particles had before removal. */
 int vare_flip_get_removed_mask(vare_flip* h, uint8_t* out, size_t cap, size_t* n);

@@ -289,8 +289,9 @@ def test_multigrid_pcg_solution(B, nu):
     assert rel(h.cell_field(L.FIELD_COMP), o64.field(O.FIELD_COMP)) < 2e-5
 
 
-def test_multigrid_deterministic():
-    # fixed-order reductions and gather-form restriction: two runs give identical bits
+def test_step_bit_reproducible():
+    # gather P2G, fixed-order reductions and gather-form restriction: two runs of whole steps give
+    # identical bits (pressure, particles, iteration counts)
     n = 32
     pos, vel = make_scene(n, seed=8)
     outs = []
@@ -299,25 +300,32 @@ def test_multigrid_deterministic():
         h = L.FlipHandle(g, L.default_params(ext_force=(0, -9.8, 0), wall_level=1e30, voxel_size=1 / n,
                                              solver_mode=1, cg_rel_tol=1e-6, preconditioner=L.PRECOND_MG))
         h.set_particles(pos, vel)
-        for s in range(4):
-            h.stage(s, DT)
-        outs.append(h.cell_field(L.FIELD_PRESSURE))
-    # P2G float atomics may reorder sums, so compare the solve's iteration structure loosely and
-    # the pressure tightly
-    assert rel(outs[0], outs[1]) < 1e-6
+        iters = []
+        for _ in range(3):
+            h.step(DT)
+            iters.append(h.stats()["cg_iterations"])
+        pt = h.get_particles()
+        outs.append((h.cell_field(L.FIELD_PRESSURE), pt["pos"], pt["vel"], iters))
+    for a, b in zip(outs[0][:3], outs[1][:3]):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert outs[0][3] == outs[1][3]
 
 
-def cpu_binning(pos, pointers, B, f):
-    """Stable counting sort of particle indices by bucket (slot of the clamped block, strays last)."""
+def cpu_binning(pos, pointers, B, f, n):
+    """Stable counting sort of particle indices by bin: the storage index of the P2G base cell
+    max((int)(p-0.5), 0) for particles inside [0,n]^3, Nc (= active cells) for the rest."""
     fx, fy, fz = f
-    nb = int((pointers >= 0).sum())
-    b = np.clip(np.trunc(pos).astype(np.int64) // B, 0, None)        # (int)p / B, then max(., 0)
-    b = np.minimum(b, np.array([fx - 1, fy - 1, fz - 1]))
-    lin = b[:, 0] + fx * (b[:, 1] + fy * b[:, 2])
-    slot = pointers[lin] // B ** 3
-    key = np.where(pointers[lin] >= 0, slot, nb)
+    nc = int((pointers >= 0).sum()) * B ** 3
+    inside = np.all((pos >= 0) & (pos <= np.float32(n)), axis=1)
+    base = np.maximum(np.trunc(pos - np.float32(0.5)).astype(np.int64), 0)
+    base = np.where(inside[:, None], base, 0)
+    blk = base // B
+    loc = base % B
+    lin = blk[:, 0] + fx * (blk[:, 1] + fy * blk[:, 2])
+    ptr = pointers[lin]
+    key = np.where(inside & (ptr >= 0), ptr + loc[:, 0] + B * (loc[:, 1] + B * loc[:, 2]), nc)
     perm = np.argsort(key, kind="stable").astype(np.int32)
-    bstart = np.searchsorted(key[perm], np.arange(nb + 2)).astype(np.int32)
+    bstart = np.searchsorted(key[perm], np.arange(nc + 2)).astype(np.int32)
     return perm, bstart
 
 
@@ -328,14 +336,38 @@ def test_binning_permutation_bit_exact(B):
     rng = np.random.default_rng(9)
     shuffle = rng.permutation(pos.shape[0])           # arbitrary input order
     pos, vel = pos[shuffle], vel[shuffle]
-    pos[:50] += np.float32(40.0)                      # some particles outside the domain (bucket Nb)
+    pos[:50] += np.float32(40.0)                      # some particles outside the domain (bin Nc)
+    pos[50:90] = np.float32([5.7, 3.2, 9.9])          # one bin longer than the register sort
+    pos[90:93] = np.float32([0.2, 0.3, 0.1])          # base cell clamped to 0
+    pos[93:96] = np.float32(n)                        # on the far faces (still inside)
     h, o32, _ = pair(n, B, pos, vel)
     h.stage(0)
     pointers, _ = h.block_table()
     perm, bstart = h.binning()
-    cperm, cbstart = cpu_binning(pos, pointers, B, (n // B,) * 3)
+    cperm, cbstart = cpu_binning(pos, pointers, B, (n // B,) * 3, n)
     np.testing.assert_array_equal(bstart, cbstart)
     np.testing.assert_array_equal(perm, cperm)
+
+
+@pytest.mark.parametrize("B", [4, 8])
+def test_p2g_bit_reproducible_and_clustered(B):
+    # the gather P2G sums in a fixed order: two runs agree bit for bit, also with a crowded cell
+    n = 32
+    pos, vel = make_scene(n, seed=12)
+    pos[:3000] = np.float32([7.3, 4.6, 11.2]) + np.random.default_rng(3).random((3000, 3), np.float32) * 0.4
+    out = []
+    for _ in range(2):
+        h, o32, _ = pair(n, B, pos, vel)
+        h.stage(0)
+        h.stage(1)
+        out.append((h.cell_field(L.FIELD_MASS), h.cell_field(L.FIELD_VELOCITY), h.cell_field(L.FIELD_COUNT)))
+    o32.stage(0)
+    o32.stage(1)
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    np.testing.assert_array_equal(out[0][2], o32.field(O.FIELD_COUNT))
+    assert rel(out[0][0], o32.field(O.FIELD_MASS)) < 1e-6
+    assert rel(out[0][1], o32.field(O.FIELD_VELO)) < 1e-6
 
 
 @pytest.mark.parametrize("B", [4, 8])

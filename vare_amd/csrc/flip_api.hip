@@ -92,11 +92,12 @@ struct vare_flip {
     uint8_t* removed = nullptr;
     int* hole = nullptr;
     int* src = nullptr;
-    int* pkey = nullptr;          // bucket (active slot of the particle's block, or Nb)
-    int* prank = nullptr;         // arrival rank inside the bucket
-    int* perm = nullptr;          // binning permutation: particle indices grouped by bucket
-    int* bcount = nullptr;        // [bcap+2]
-    int* bstart = nullptr;        // [bcap+2]
+    int* pkey = nullptr;          // bin (storage index of the particle's base cell, or Nc)
+    int* prank = nullptr;         // arrival rank inside the bin
+    int* perm = nullptr;          // binning permutation: particle indices grouped by bin
+    int* longbins = nullptr;      // bins longer than kSortReg, [pcap/(kSortReg+1)+2]
+    int* ccount = nullptr;        // particles per bin, [bcap*B^3+2]
+    int* cstart = nullptr;        // bin starts, [bcap*B^3+2]
     size_t last_np_before_remove = 0;
     int nremoved = 0;
     int last_iters = 0;           // solver iterations of the previous step
@@ -129,7 +130,8 @@ struct vare_flip {
     // scan scratch
     int* tiles = nullptr;
     size_t tiles_cap = 0;
-    int* dint = nullptr;          // [0] scan total, [1] removed count, [2] second scan total
+    int* dint = nullptr;          // [0] scan total, [1] removed count, [2] second scan total,
+                                  // [3] bin total, [4] bins left unsorted, [5] long-bin count
     CgState* st = nullptr;
     CgState* st_host = nullptr;   // pinned mirror
     int* dint_host = nullptr;     // pinned mirror of dint
@@ -177,7 +179,7 @@ void free_mg(vare_flip* h) {
 void free_cells(vare_flip* h) {
     free_mg(h);
     dfree(h->nbr);
-    dfree(h->bcount); dfree(h->bstart);
+    dfree(h->ccount); dfree(h->cstart);
     dfree(h->mass); dfree(h->vx); dfree(h->vy); dfree(h->vz); dfree(h->cnt);
     dfree(h->cx); dfree(h->cy); dfree(h->cz);
     dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b); dfree(h->z); dfree(h->x64);
@@ -195,7 +197,7 @@ int ensure_blocks(vare_flip* h, size_t nb) {
     const size_t nc = cap * (size_t)h->B * h->B * h->B;
     const size_t npart = std::max((size_t)kReduceGrid, cap) + 16;
     CHK(dalloc(&h->nbr, cap * 6));
-    CHK(dalloc(&h->bcount, cap + 2)); CHK(dalloc(&h->bstart, cap + 2));
+    CHK(dalloc(&h->ccount, nc + 2)); CHK(dalloc(&h->cstart, nc + 2));
     CHK(dalloc(&h->mass, nc)); CHK(dalloc(&h->vx, nc)); CHK(dalloc(&h->vy, nc)); CHK(dalloc(&h->vz, nc));
     CHK(dalloc(&h->cnt, nc));
     CHK(dalloc(&h->cx, nc)); CHK(dalloc(&h->cy, nc)); CHK(dalloc(&h->cz, nc));
@@ -223,9 +225,10 @@ int ensure_particles(vare_flip* h, size_t n) {
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int q = 0; q < 9; ++q) { dfree(h->pa[q]); h->pa[q] = na[q]; }
     dfree(h->removed); dfree(h->hole); dfree(h->src);
-    dfree(h->pkey); dfree(h->prank); dfree(h->perm);
+    dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins);
     CHK(dalloc(&h->removed, cap)); CHK(dalloc(&h->hole, cap)); CHK(dalloc(&h->src, cap));
     CHK(dalloc(&h->pkey, cap)); CHK(dalloc(&h->prank, cap)); CHK(dalloc(&h->perm, cap));
+    CHK(dalloc(&h->longbins, cap / (kSortReg + 1) + 2));
     h->pcap = cap;
     return VARE_OK;
 }
@@ -325,24 +328,28 @@ __global__ void k_cg_setup(CgState* st, int mode, int max_count, float abs_tol, 
     st->converged = 0;
 }
 
-// Binning permutation: bucket every particle by the active slot of its block (bucket Nb for
-// particles outside the domain), counting sort, then sort each bucket by particle index so the
+// Binning permutation: bin every particle by the storage index of its P2G base cell (bin Nc for
+// particles outside the domain), counting sort, then sort each bin by particle index so the
 // permutation equals a stable counting sort of the particle order.
+constexpr int kSortLongGrid = 512;   // workgroups of k_sort_long (strides over the long-bin list)
+
 template <int B> int bin_particles(vare_flip* h) {
-    const int nb = h->nb;
-    CHK(fill_u32(h, h->bcount, 0, (size_t)nb + 2));
+    const int nc = (int)h->ncells();
+    CHK(fill_u32(h, h->ccount, 0, (size_t)nc + 2));
+    CHK(fill_u32(h, h->dint + 4, 0, 2));           // [4] sort overflow, [5] long-bin count
     if (h->np)
-        hipLaunchKernelGGL(k_pkey<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (const float*)h->pa[0],
-                           (const float*)h->pa[1], (const float*)h->pa[2], (int)h->np, h->frame(), nb, h->pkey, h->prank,
-                           h->bcount);
-    OpBuckets op{h->bcount, h->bstart};
-    CHK(run_scan(h, op, nb + 2, 3));
+        hipLaunchKernelGGL(k_ckey<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (const float*)h->pa[0],
+                           (const float*)h->pa[1], (const float*)h->pa[2], (int)h->np, h->frame(), nc, h->pkey, h->prank,
+                           h->ccount);
+    OpBuckets op{h->ccount, h->cstart};
+    CHK(run_scan(h, op, nc + 2, 3));
     if (h->np) {
         hipLaunchKernelGGL(k_pscatter, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, (const int*)h->pkey,
-                           (const int*)h->prank, (const int*)h->bstart, h->perm);
-        CHK(fill_u32(h, h->dint + 4, 0, 1));
-        hipLaunchKernelGGL(k_psort, dim3(nb + 1), dim3(1024), 0, h->stream, (const int*)h->bstart, nb + 1, (int)h->np,
-                           h->perm, h->dint + 4);
+                           (const int*)h->prank, (const int*)h->cstart, h->perm);
+        hipLaunchKernelGGL(k_sort_short, dim3(nblocks((size_t)nc + 1)), dim3(WG), 0, h->stream, (const int*)h->cstart,
+                           nc + 1, h->perm, h->longbins, h->dint + 5);
+        hipLaunchKernelGGL(k_sort_long, dim3(kSortLongGrid), dim3(1024), 0, h->stream, (const int*)h->cstart,
+                           (const int*)h->longbins, (const int*)(h->dint + 5), h->perm, h->dint + 4);
     }
     HIPCHK(hipGetLastError());
     return VARE_OK;
@@ -378,15 +385,22 @@ template <int B> int stage_prepare(vare_flip* h) {
 
 template <int B> int stage_p2g(vare_flip* h) {
     const size_t nc = h->ncells();
+    CellFields cf{h->mass, h->vx, h->vy, h->vz, h->cnt};
+    if (h->prm.particle_path == 0) {
+        // gather form: every cell of every active block is written, normalised and classified
+        tbegin(h, VARE_KERNEL_P2G);
+        if (h->nb)
+            hipLaunchKernelGGL(k_p2g_gather<B>, dim3(h->nb), dim3(P2GGather<B>::THREADS), 0, h->stream,
+                               (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
+                               (const int*)h->perm, (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type);
+        tend(h, VARE_KERNEL_P2G);
+        HIPCHK(hipGetLastError());
+        return VARE_OK;
+    }
     CHK(fill_u32(h, h->mass, 0, nc)); CHK(fill_u32(h, h->vx, 0, nc)); CHK(fill_u32(h, h->vy, 0, nc));
     CHK(fill_u32(h, h->vz, 0, nc)); CHK(fill_u32(h, h->cnt, 0, nc));
-    CellFields cf{h->mass, h->vx, h->vy, h->vz, h->cnt};
     tbegin(h, VARE_KERNEL_P2G);
-    if (h->np && h->prm.particle_path == 0 && h->nb)
-        hipLaunchKernelGGL(k_p2g_tile<B>, dim3(h->nb), dim3(WG), 0, h->stream, (const float*)h->pa[0],
-                           (const float*)h->pa[1], (const float*)h->pa[2], h->vel(), (const int*)h->perm,
-                           (const int*)h->bstart, h->frame(), cf);
-    else if (h->np && h->prm.particle_path != 0)
+    if (h->np)
         hipLaunchKernelGGL(k_p2g_atomic<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
                            h->vel(), (int)h->np, h->frame(), cf);
     tend(h, VARE_KERNEL_P2G);
@@ -691,7 +705,7 @@ template <int B> int stage_g2p(vare_flip* h, float dt) {
     if (h->np && h->prm.particle_path == 0) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
         hipLaunchKernelGGL(k_g2p_tile<B>, dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
-                           h->vel(), h->pa[6], (const int*)h->perm, (const int*)h->bstart, h->nb,
+                           h->vel(), h->pa[6], (const int*)h->perm, (const int*)h->cstart, h->nb,
                            h->frame(), gf, g2p_params(h, dt), h->removed, h->dint + 1);
     } else if (h->np)
         hipLaunchKernelGGL(k_g2p<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
@@ -896,6 +910,7 @@ int vare_flip_destroy(vare_flip* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (int q = 0; q < 9; ++q) dfree(h->pa[q]);
     dfree(h->removed); dfree(h->hole); dfree(h->src);
+    dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins);
     dfree(h->flag); dfree(h->slot); dfree(h->coord);
     free_cells(h);
     dfree(h->tiles); dfree(h->dint); dfree(h->st);
@@ -1162,15 +1177,15 @@ int vare_flip_get_cell_field(vare_flip* h, int which, void* out, size_t cap_byte
 int vare_flip_get_binning(vare_flip* h, int32_t* perm, int32_t* bstart, size_t cap, size_t* n, size_t* nbucket) {
     if (!h) return fail(VARE_EINVAL, "null handle");
     if (n) *n = h->np;
-    if (nbucket) *nbucket = (size_t)h->nb + 1;
+    if (nbucket) *nbucket = h->ncells() + 1;
     if (h->prm.particle_path != 0) return fail(VARE_EINVAL, "binning is only built on particle_path 0");
     if (perm) {
         if (cap < h->np) return fail(VARE_EINVAL, "capacity too small");
         if (h->np) HIPCHK(hipMemcpy(perm, h->perm, h->np * sizeof(int), hipMemcpyDeviceToHost));
     }
-    if (bstart) HIPCHK(hipMemcpy(bstart, h->bstart, ((size_t)h->nb + 2) * sizeof(int), hipMemcpyDeviceToHost));
+    if (bstart) HIPCHK(hipMemcpy(bstart, h->cstart, (h->ncells() + 2) * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(h->dint_host + 4, h->dint + 4, sizeof(int), hipMemcpyDeviceToHost));
-    if (h->dint_host[4] != 0) return fail(VARE_ENOTIMPL, "%d bucket(s) longer than %d were left unsorted", h->dint_host[4], kSortMax);
+    if (h->dint_host[4] != 0) return fail(VARE_ENOTIMPL, "%d bin(s) longer than %d were left unsorted", h->dint_host[4], kSortMax);
     return VARE_OK;
 }
 

@@ -1,55 +1,57 @@
-// flip_particles.h -- block-binned particle kernels: the binning permutation (a stable counting
-// sort of particle indices by the active slot of the block that contains them), P2G accumulated in
-// an LDS tile per block, and G2P gathering from an LDS tile per block.
+// flip_particles.h -- cell-binned particle kernels: the binning permutation (a stable counting sort
+// of particle indices by the storage index of their P2G base cell), P2G as a gather per block
+// (deterministic: no float atomics), and G2P gathering from an LDS tile per block.
 //
 // The particle arrays stay in the reference's order (Array::remove order); the permutation only
-// says in which order each block's workgroup visits them.  A particle's block is
-// c = clamp((int)p/B, 0, f-1) per axis (SparseFrame.cu:145-147 before the neighbourhood loop); it is
-// always active for a particle inside the domain.  Particles whose clamped block is inactive (they
-// lie outside the domain) go to an extra bucket Nb, handled with global-memory lookups.
+// says in which order each block's workgroup visits them.  A particle's P2G base cell is
+// c = max((int)(p-0.5), 0) per axis (Rasterization.cu:139-147).  For a particle inside [0,n]^3 the
+// block holding c is always active: it is the particle's block or the one below it, both inside the
+// 3x3x3 neighbourhood that SparseFrame.cu:145-152 marks.  Particles outside [0,n]^3 (skipped by P2G)
+// go to an extra bin Nc, handled by G2P with global-memory lookups.
 #pragma once
 #include "flip_kernels.h"
 
 namespace vare {
 
-constexpr int kSortMax = 16384;     // longest bucket sorted in LDS (64 KB)
+constexpr int kSortMax = 16384;     // longest bin sorted in LDS (64 KB)
+constexpr int kSortReg = 16;        // bins up to this length are sorted in registers
 
-__device__ __forceinline__ unsigned long long lanemask_lt() {
-    const int lane = threadIdx.x & 63;
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
-// Bucket of every particle + its arrival rank inside the bucket (wave-aggregated atomics).
+// Bin of every particle (storage index of its base cell, or nc) + its arrival rank inside the bin.
+// Lanes holding equal keys in a row form a run; the first lane of every run reserves the whole run
+// with one atomic add, so a wave issues a single atomic instruction whatever the number of bins.
 template <int B>
-__global__ void k_pkey(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, int np,
-                       FrameK f, int nb, int* __restrict__ key, int* __restrict__ rank, int* __restrict__ bcount) {
+__global__ void k_ckey(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, int np,
+                       FrameK f, int nc, int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count) {
+    constexpr int B3 = B * B * B;
     const int i = blockIdx.x * WG + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const bool valid = i < np;
     int k = -1;
     if (valid) {
-        const int bi = rmin(rmax((int)(px[i]) / B, 0), f.fx - 1);
-        const int bj = rmin(rmax((int)(py[i]) / B, 0), f.fy - 1);
-        const int bk = rmin(rmax((int)(pz[i]) / B, 0), f.fz - 1);
-        const int s = f.slot[(bk * f.fy + bj) * f.fx + bi];
-        k = s >= 0 ? s : nb;
+        const float x = px[i], y = py[i], z = pz[i];
+        k = nc;
+        if (!(x < 0.f || x > (float)f.nx || y < 0.f || y > (float)f.ny || z < 0.f || z > (float)f.nz)) {
+            const int ci = rmax((int)(x - 0.5f), 0), cj = rmax((int)(y - 0.5f), 0), ck = rmax((int)(z - 0.5f), 0);
+            const int s = f.slot[((ck / B) * f.fy + cj / B) * f.fx + ci / B];
+            if (s >= 0) k = s * B3 + ((ck % B) * B + cj % B) * B + ci % B;
+        }
         key[i] = k;
     }
-    unsigned long long pending = __ballot(valid);
-    int myrank = 0;
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int lk = __shfl(k, leader, 64);
-        const unsigned long long m = __ballot(valid && k == lk);
-        int base = 0;
-        if ((threadIdx.x & 63) == leader) base = atomicAdd(bcount + lk, (int)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if (valid && k == lk) myrank = base + (int)__popcll(m & lanemask_lt());
-        pending &= ~m;
-    }
-    if (valid) rank[i] = myrank;
+    const int prev = __shfl(k, lane > 0 ? lane - 1 : 0, 64);
+    const bool head = valid && (lane == 0 || k != prev);
+    const unsigned long long heads = __ballot(head);
+    const int nvalid = (int)__popcll(__ballot(valid));       // valid lanes are a prefix of the wave
+    const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1);
+    const int mine = 63 - __clzll((long long)(upto | 1ull));   // first lane of my run
+    const unsigned long long after = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
+    const int next = after ? __ffsll((long long)after) - 1 : nvalid;
+    int base = 0;
+    if (head) base = atomicAdd(count + k, next - lane);
+    base = __shfl(base, mine, 64);
+    if (valid) rank[i] = base + lane - mine;
 }
 
-// Exclusive scan of bucket counts -> bucket starts.
+// Exclusive scan of bin counts -> bin starts.
 struct OpBuckets {
     const int* count;
     int* start;
@@ -64,43 +66,82 @@ __global__ void k_pscatter(int np, const int* __restrict__ key, const int* __res
     perm[bstart[key[i]] + rank[i]] = i;
 }
 
-// Sort every bucket's indices ascending (makes the permutation a stable counting sort, independent
-// of the atomic arrival order).  One 1024-thread workgroup per bucket, bitonic sort in LDS.
-__global__ void __launch_bounds__(1024) k_psort(const int* __restrict__ bstart, int nbucket, int np,
-                                                int* __restrict__ perm, int* __restrict__ overflow) {
-    __shared__ int sh[kSortMax];
-    const int b = blockIdx.x;
-    if (b >= nbucket) return;
-    const int s0 = bstart[b];
-    const int s1 = b + 1 < nbucket ? bstart[b + 1] : np;
-    const int n = s1 - s0;
+// Sort every bin's indices ascending, which turns the permutation into a stable counting sort that
+// does not depend on the atomic arrival order.  One thread per bin: bins of up to kSortReg entries
+// are sorted in registers (bitonic network); longer bins are listed for k_sort_long.
+__global__ void k_sort_short(const int* __restrict__ bstart, int nbins, int* __restrict__ perm,
+                             int* __restrict__ longbins, int* __restrict__ nlong) {
+    const int c = blockIdx.x * WG + threadIdx.x;
+    if (c >= nbins) return;
+    const int s0 = bstart[c], n = bstart[c + 1] - s0;
     if (n <= 1) return;
-    // arrival order is usually already ascending: skip buckets that need no sorting
-    int unsorted = 0;
-    for (int t = s0 + threadIdx.x; t + 1 < s1; t += blockDim.x) unsorted |= perm[t] > perm[t + 1];
-    if (!__syncthreads_or(unsorted)) return;
-    if (n > kSortMax) {
-        if (threadIdx.x == 0) atomicAdd(overflow, 1);
+    if (n > kSortReg) {
+        longbins[atomicAdd(nlong, 1)] = c;
         return;
     }
-    int m = 1;
-    while (m < n) m <<= 1;
-    for (int t = threadIdx.x; t < m; t += blockDim.x) sh[t] = t < n ? perm[s0 + t] : 0x7fffffff;
-    __syncthreads();
-    for (int k = 2; k <= m; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = threadIdx.x; t < m; t += blockDim.x) {
+    int v[kSortReg];
+#pragma unroll
+    for (int q = 0; q < kSortReg; ++q) v[q] = q < n ? perm[s0 + q] : 0x7fffffff;
+    bool sorted = true;
+#pragma unroll
+    for (int q = 0; q + 1 < kSortReg; ++q) sorted = sorted && v[q] < v[q + 1];
+    if (sorted) return;
+#pragma unroll
+    for (int k = 2; k <= kSortReg; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int t = 0; t < kSortReg; ++t) {
                 const int u = t ^ j;
                 if (u > t) {
-                    const int a = sh[t], c = sh[u];
-                    const bool up = (t & k) == 0;
-                    if ((a > c) == up) { sh[t] = c; sh[u] = a; }
+                    const int lo = rmin(v[t], v[u]), hi = rmax(v[t], v[u]);
+                    if ((t & k) == 0) { v[t] = lo; v[u] = hi; } else { v[t] = hi; v[u] = lo; }
                 }
             }
-            __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kSortReg; ++q)
+        if (q < n) perm[s0 + q] = v[q];
+}
+
+// Bins longer than kSortReg: one 1024-thread workgroup per listed bin, bitonic sort in LDS.  The
+// grid is fixed and strides over the list, whose length is read on the device.  Bins longer than
+// kSortMax keep their arrival order and are counted in *overflow.
+__global__ void __launch_bounds__(1024) k_sort_long(const int* __restrict__ bstart, const int* __restrict__ longbins,
+                                                    const int* __restrict__ nlong, int* __restrict__ perm,
+                                                    int* __restrict__ overflow) {
+    __shared__ int sh[kSortMax];
+    const int nl = *nlong;
+    for (int e = blockIdx.x; e < nl; e += gridDim.x) {
+        const int b = longbins[e];
+        const int s0 = bstart[b], s1 = bstart[b + 1];
+        const int n = s1 - s0;
+        int unsorted = 0;
+        for (int t = s0 + threadIdx.x; t + 1 < s1; t += blockDim.x) unsorted |= perm[t] > perm[t + 1];
+        if (!__syncthreads_or(unsorted)) continue;
+        if (n > kSortMax) {
+            if (threadIdx.x == 0) atomicAdd(overflow, 1);
+            continue;
         }
+        int m = 1;
+        while (m < n) m <<= 1;
+        for (int t = threadIdx.x; t < m; t += blockDim.x) sh[t] = t < n ? perm[s0 + t] : 0x7fffffff;
+        __syncthreads();
+        for (int k = 2; k <= m; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = threadIdx.x; t < m; t += blockDim.x) {
+                    const int u = t ^ j;
+                    if (u > t) {
+                        const int a = sh[t], c = sh[u];
+                        const bool up = (t & k) == 0;
+                        if ((a > c) == up) { sh[t] = c; sh[u] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int t = threadIdx.x; t < n; t += blockDim.x) perm[s0 + t] = sh[t];
+        __syncthreads();
     }
-    for (int t = threadIdx.x; t < n; t += blockDim.x) perm[s0 + t] = sh[t];
 }
 
 // Slots of the 3x3x3 block neighbourhood of block slot s (index (dz+1)*9+(dy+1)*3+(dx+1)),
@@ -129,102 +170,100 @@ __device__ __forceinline__ int tile_cell(const int* nsl, int ti, int tj, int tk)
     return s * Blk<B>::B3 + (lk * B + lj) * B + li;
 }
 
-// P2G of one block's particles into an LDS tile of (B+2)^3 cells (Rasterization.cu:121-173 and
-// :198-228 per particle), then one global atomic add per touched tile cell and field.
+// Workgroup shape of the gather P2G: one thread per base cell of the (B+1)^3 region whose stencils
+// reach the block (local offsets -1..B-1 per axis).
+template <int B> struct P2GGather {
+    static constexpr int B1 = B + 1;
+    static constexpr int NBASE = B1 * B1 * B1;
+    static constexpr int THREADS = (NBASE + 63) / 64 * 64;
+};
+
+// P2G + count field + normalisation + classification of one active block, as a gather
+// (Rasterization.cu:121-190 and :198-230, FLIPMethod.cu:242-280 with no collision source).
+// Each thread sums the 8 corner contributions of its base cell's particles (bin order, i.e.
+// ascending particle index) in registers.  The block's cells then receive the corners in 8 passes
+// (corner offset 000, 100, 010, ... in turn): within one pass distinct base cells hit distinct
+// cells, so the LDS updates need no atomics and every cell sums its terms in a fixed order -- the
+// result is bit-reproducible.  A corner past n-1 is skipped (the Min(c+1, n-1) bound); corners in
+// other blocks are gathered by those blocks' workgroups, and corners in inactive blocks are dropped
+// (isWritable).  Every cell of the block is written, so no zero fill is needed.
 template <int B>
-__global__ void __launch_bounds__(WG) k_p2g_tile(const float* __restrict__ px, const float* __restrict__ py,
-                                                const float* __restrict__ pz, VelPtr u,
-                                                const int* __restrict__ perm, const int* __restrict__ bstart, FrameK f,
-                                                CellFields cf) {
-    constexpr int T = B + 2, T3 = T * T * T;
-    __shared__ float tm[T3], tx[T3], ty[T3], tz[T3], tc[T3];
+__global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
+        const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, VelPtr u,
+        const int* __restrict__ perm, const int* __restrict__ cstart, FrameK f, CellFields cf,
+        float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz, uint8_t* __restrict__ type) {
+    constexpr int B1 = B + 1, NBASE = B1 * B1 * B1, B3 = B * B * B;
+    __shared__ float tm[B3], tx[B3], ty[B3], tz[B3], tc[B3];
     __shared__ int nsl[27];
     const int s = blockIdx.x;
-    const int p0 = bstart[s], p1 = bstart[s + 1];
-    if (p0 == p1) return;
+    const int t = threadIdx.x;
     load_nbhd(f, s, nsl);
-    for (int t = threadIdx.x; t < T3; t += WG) { tm[t] = 0.f; tx[t] = 0.f; ty[t] = 0.f; tz[t] = 0.f; tc[t] = 0.f; }
+    for (int q = t; q < B3; q += blockDim.x) { tm[q] = 0.f; tx[q] = 0.f; ty[q] = 0.f; tz[q] = 0.f; tc[q] = 0.f; }
     __syncthreads();
     const int t0 = f.coord[s];
-    const int ox = (t0 % f.fx) * B - 1, oy = ((t0 / f.fx) % f.fy) * B - 1, oz = (t0 / (f.fx * f.fy)) * B - 1;
-    // Each thread walks a contiguous run of the bucket.  Particles sharing the base cell
-    // c = max((int)(p-0.5), 0) touch the same (up to) 8 cells, so their contributions are summed in
-    // registers and flushed to the LDS tile only when the base cell changes.
-    const int cnt = p1 - p0;
-    const int chunk = (cnt + WG - 1) / WG;
-    const int q0 = p0 + threadIdx.x * chunk;
-    const int q1 = rmin(q0 + chunk, p1);
+    const int ox = (t0 % f.fx) * B, oy = ((t0 / f.fx) % f.fy) * B, oz = (t0 / (f.fx * f.fy)) * B;
+    const int li = t % B1 - 1, lj = (t / B1) % B1 - 1, lk = t / (B1 * B1) - 1;     // base cell - origin
     float am[8], ax[8], ay[8], az[8], ac[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) { am[c] = 0.f; ax[c] = 0.f; ay[c] = 0.f; az[c] = 0.f; ac[c] = 0.f; }
-    int cur = -1;                       // packed tile coordinates of the current base cell
-    unsigned cur_mask = 0;              // corners that exist and are writable
-    auto flush = [&]() {
-        if (cur < 0) return;
-        const int bi = cur & 1023, bj = (cur >> 10) & 1023, bk = cur >> 20;
+    if (t < NBASE) {
+        const int dx = li < 0 ? -1 : 0, dy = lj < 0 ? -1 : 0, dz = lk < 0 ? -1 : 0;
+        const int sl = nsl[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+        if (sl >= 0) {
+            const int cell = sl * B3 + ((lk - dz * B) * B + (lj - dy * B)) * B + (li - dx * B);
+            const int q0 = cstart[cell], q1 = cstart[cell + 1];
+            const int ci = ox + li, cj = oy + lj, ck = oz + lk;
+            for (int q = q0; q < q1; ++q) {
+                const int n = perm[q];
+                const float x = px[n], y = py[n], z = pz[n];
+                const float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            if (cur_mask & (1u << c)) {
-                const int t = ((bk + (c >> 2)) * T + (bj + ((c >> 1) & 1))) * T + (bi + (c & 1));
-                atomicAdd(tm + t, am[c]);
-                atomicAdd(tx + t, ax[c]);
-                atomicAdd(ty + t, ay[c]);
-                atomicAdd(tz + t, az[c]);
-                atomicAdd(tc + t, ac[c]);
+                for (int c = 0; c < 8; ++c) {
+                    const int i = ci + (c & 1), j = cj + ((c >> 1) & 1), k = ck + (c >> 2);
+                    const float ddx = x - ((float)i + 0.5f), ddy = y - ((float)j + 0.5f), ddz = z - ((float)k + 0.5f);
+                    const float xw = rmax(0.f, 1.f - rabs(ddx));
+                    const float yw = rmax(0.f, 1.f - rabs(ddy));
+                    const float zw = rmax(0.f, 1.f - rabs(ddz));
+                    const float w = xw * yw * zw + 1e-05f;
+                    am[c] += w;
+                    ax[c] += w * vx;
+                    ay[c] += w * vy;
+                    az[c] += w * vz;
+                    ac[c] += 1.f;
+                }
             }
-            am[c] = 0.f; ax[c] = 0.f; ay[c] = 0.f; az[c] = 0.f; ac[c] = 0.f;    // every corner restarts
-        }
-    };
-    for (int q = q0; q < q1; ++q) {
-        const int n = perm[q];
-        const float x = px[n], y = py[n], z = pz[n];
-        if (x < 0.f || x > (float)f.nx || y < 0.f || y > (float)f.ny || z < 0.f || z > (float)f.nz) continue;
-        const float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
-        const int ci = rmax((int)(x - 0.5f), 0), cj = rmax((int)(y - 0.5f), 0), ck = rmax((int)(z - 0.5f), 0);
-        const int ti = ci - ox, tj = cj - oy, tk = ck - oz;
-        const int key = (tk << 20) | (tj << 10) | ti;
-        if (key != cur) {
-            flush();
-            cur = key;
-            cur_mask = 0;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const int i = ci + (c & 1), j = cj + ((c >> 1) & 1), k = ck + (c >> 2);
-                if (i > f.nx - 1 || j > f.ny - 1 || k > f.nz - 1) continue;    // Min(c+1, n-1) bound
-                const int ai = i - ox - 1, aj = j - oy - 1, ak = k - oz - 1;
-                const int dx = ai < 0 ? -1 : (ai >= B ? 1 : 0);
-                const int dy = aj < 0 ? -1 : (aj >= B ? 1 : 0);
-                const int dz = ak < 0 ? -1 : (ak >= B ? 1 : 0);
-                if (nsl[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] >= 0) cur_mask |= 1u << c;   // isWritable
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int i = ci + (c & 1), j = cj + ((c >> 1) & 1), k = ck + (c >> 2);
-            const float ddx = x - ((float)i + 0.5f), ddy = y - ((float)j + 0.5f), ddz = z - ((float)k + 0.5f);
-            const float xw = rmax(0.f, 1.f - rabs(ddx));
-            const float yw = rmax(0.f, 1.f - rabs(ddy));
-            const float zw = rmax(0.f, 1.f - rabs(ddz));
-            const float w = xw * yw * zw + 1e-05f;
-            am[c] += w;
-            ax[c] += w * vx;
-            ay[c] += w * vy;
-            az[c] += w * vz;
-            ac[c] += 1.f;
         }
     }
-    flush();
-    __syncthreads();
-    for (int t = threadIdx.x; t < T3; t += WG) {
-        if (tc[t] == 0.f) continue;
-        const int ti = t % T, tj = (t / T) % T, tk = t / (T * T);
-        const int c = tile_cell<B>(nsl, ti, tj, tk);
-        if (c < 0) continue;
-        unsafeAtomicAdd(cf.mass + c, tm[t]);
-        unsafeAtomicAdd(cf.vx + c, tx[t]);
-        unsafeAtomicAdd(cf.vy + c, ty[t]);
-        unsafeAtomicAdd(cf.vz + c, tz[t]);
-        unsafeAtomicAdd(cf.cnt + c, tc[t]);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int ai = li + (c & 1), aj = lj + ((c >> 1) & 1), ak = lk + (c >> 2);
+        if (t < NBASE && ai >= 0 && aj >= 0 && ak >= 0 && ai < B && aj < B && ak < B && ox + ai <= f.nx - 1 &&
+            oy + aj <= f.ny - 1 && oz + ak <= f.nz - 1) {
+            const int q = (ak * B + aj) * B + ai;
+            tm[q] += am[c];
+            tx[q] += ax[c];
+            ty[q] += ay[c];
+            tz[q] += az[c];
+            tc[q] += ac[c];
+        }
+        __syncthreads();
+    }
+    for (int q = t; q < B3; q += blockDim.x) {
+        const int c = s * B3 + q;
+        const float m = tm[q];
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+        uint8_t ty8 = kAir;
+        if (m > 0.f) {
+            vx = tx[q] / m;
+            vy = ty[q] / m;
+            vz = tz[q] / m;
+            ty8 = kFluid;
+        }
+        cf.mass[c] = m > 0.f ? m : 0.f;
+        cf.vx[c] = vx; cf.vy[c] = vy; cf.vz[c] = vz;
+        cf.cnt[c] = tc[q];
+        cx[c] = vx; cy[c] = vy; cz[c] = vz;
+        type[c] = ty8;
     }
 }
 
@@ -312,18 +351,20 @@ struct G2PFields {
     const float* vz0;
 };
 
-// G2P for one block's particles from an LDS tile of the 7 gathered fields; bucket Nb (particles
-// outside the domain) reads the fields through the block table.
+// G2P for the particles whose base cell lies in one block (bins s*B^3 .. (s+1)*B^3-1), from an LDS
+// tile of the 7 gathered fields over the block and a 1-cell halo, which holds every stencil
+// i = clamp((int)(p-0.5), 0, n-2) .. i+1 of those particles.  Workgroup nb takes bin Nc (particles
+// outside the domain) and reads the fields through the block table.
 template <int B>
 __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
                                                 VelPtr u, float* __restrict__ adj, const int* __restrict__ perm,
-                                                const int* __restrict__ bstart, int nb, FrameK f, G2PFields gf,
+                                                const int* __restrict__ cstart, int nb, FrameK f, G2PFields gf,
                                                 G2PParams gp, uint8_t* __restrict__ removed, int* __restrict__ nremoved) {
-    constexpr int T = B + 2, T3 = T * T * T;
+    constexpr int T = B + 2, T3 = T * T * T, B3 = B * B * B;
     __shared__ float tile[7][T3];
     __shared__ int nsl[27];
     const int s = blockIdx.x;
-    const int p0 = bstart[s], p1 = bstart[s + 1];
+    const int p0 = cstart[s * B3], p1 = s < nb ? cstart[(s + 1) * B3] : cstart[nb * B3 + 1];
     if (p0 == p1) return;
     const float* src[7] = {gf.cnt, gf.cx, gf.cy, gf.cz, gf.vx0, gf.vy0, gf.vz0};
     int rem_count = 0;
