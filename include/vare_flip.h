@@ -200,6 +200,39 @@ int vare_flip_get_binning(vare_flip* h, int32_t* perm, int32_t* bstart, size_t c
 /* Per-particle removal flags of the last step, in the order the particles had before removal. */
 int vare_flip_get_removed_mask(vare_flip* h, uint8_t* out, size_t cap, size_t* n);
 
+/* ---- Collision source (CollisionSource, base/header/CollisionSource.h:13-150) ----
+ * Replaces FLIPMethod::updateCollisionSource(openvdb SDF grid, openvdb velocity grid)
+ * (base/header/FLIPMethod.h:105-108 -> CollisionSource::update, CollisionSource.h:66-126).  The
+ * source is a sparse field over the simulation frame: the 3x3x3 block neighbourhood of the block
+ * of every seed (the world positions of the source's active voxels, as the reference feeds the
+ * VDB's active voxels to SparseFrame::buildFromPoints) is activated, and every active cell centre
+ * is sampled once.  The step then uses it exactly where the reference does: solid cells and the
+ * collision velocity in the cell classification (FLIPMethod.cu:242-260), and the push, projection
+ * and removal of particles in G2P (FLIPMethod.cu:116-164).  Until a source is set every query
+ * returns signed distance 5 (no collision), as after CollisionSource::initialize. */
+
+/* Sampler standing in for the OpenVDB grids (BoxSampler reads of the SDF, velocity and SDF-gradient
+ * grids): at world point wp return the signed distance (negative inside), the collision velocity
+ * and the SDF gradient, all in world units.  Called on the host, once per active cell. */
+typedef void (*vare_collision_sampler)(void* user, const float wp[3], float* sdf, float vel[3], float grad[3]);
+/* Set the source from seeds (nseed world xyz triples) and a sampler; background_world is the
+ * signed distance of cells outside the source's active blocks (the VDB background). */
+int vare_flip_set_collision(vare_flip* h, const float* seeds_world, size_t nseed, float background_world,
+                            vare_collision_sampler fn, void* user);
+/* Built-in source: the axis-aligned box [lo,hi] (world) moving with vel (world units/s, NULL = 0),
+ * exact box SDF and gradient.  Seeds = cells whose centre lies within band_voxels of the surface
+ * (a narrow-band level set of that half-width); background = +band_voxels.  Sampled on host
+ * threads. */
+int vare_flip_set_collision_box(vare_flip* h, const float lo[3], const float hi[3], const float vel[3],
+                                float band_voxels);
+/* Remove the source (queries return signed distance 5 again). */
+int vare_flip_clear_collision(vare_flip* h);
+/* Debug/parity accessor: pointers[fx*fy*fz] = cell offset of each block in the source's field or -1;
+ * phi[nc] (voxel units), vel[3*nc] (voxel units), nrm[3*nc] (unit gradient) in storage order;
+ * *nc and *background (voxel units) always written.  Field pointers may be NULL. */
+int vare_flip_get_collision(vare_flip* h, int64_t* pointers, float* phi, float* vel, float* nrm, size_t cap,
+                            size_t* nc, float* background);
+
 /* Seed a box of water in voxel space (the dam-break input of the benchmark): for every cell
  * whose centre is inside [0,wx)x[0,wy)x[0,wz), ppc points at cell + Rand(s+3q+a), a=0,1,2,
  * q = ppc*(i + nx*(j + ny*k)) + n (Rand = Random.h:18-27); only points inside the box are kept.

@@ -142,6 +142,16 @@ template <class R> struct State {
     std::vector<uint64_t> A_ind;
     std::vector<R> x, b;
 
+    // collision source (CollisionSource.h:13-150): its own sparse frame on the same geometry, with
+    // signed distance (voxel units, background col_bg), velocity and normal fields (background 0).
+    // Before the first update the fields are empty and signedDistance returns 5 (CollisionSource.h:56,
+    // :128-131).
+    std::vector<uint64_t> col_pointers;
+    std::vector<Idx3> col_coords;
+    std::vector<R> col_phi;
+    std::vector<V3<R>> col_vel, col_nrm;
+    R col_bg = R(5);
+
     // stats
     int cg_iters = 0;
     double cg_res = 0;              // last recursive residual norm
@@ -295,12 +305,121 @@ template <class R> struct State {
 
     // ---- cell classification, FLIPMethod.cu:242-280 (no collision source: phi = 5, v = 0,
     //      CollisionSource.h:128-144 with an empty field) ----
+    // ---- FLIPMethod::advanceOneStep mm_kernel (FLIPMethod.cu:242-280), curl disabled ----
     void classify() {
+        const R scale = (R)prm.col_vel_scale;
         for (size_t n = 0; n < mark.size(); ++n) {
-            const R phi = R(5);
+            int64_t i, j, k;
+            find_indices(n, i, j, k);
+            const V3<R> cell{(R)((float)i + 0.5f), (R)((float)j + 0.5f), (R)((float)k + 0.5f)};   // Grid.h:274-277
+            const R phi = signed_distance(cell);
+            const V3<R> v = col_velocity(cell);
             if (mass[n] > R(0)) mark[n] = kFluid;
             else mark[n] = (phi < R(0)) ? kSolid : kAir;
+            if (phi < R(0)) {
+                velo[n].x += v.x * scale;
+                velo[n].y += v.y * scale;
+                velo[n].z += v.z * scale;
+            }
             comp[n] = velo[n];
+        }
+    }
+
+    // ---- collision source queries (CollisionSource.h:128-144) ----
+    uint64_t col_index(int64_t i, int64_t j, int64_t k) const {
+        const int B = g.B;
+        const uint64_t p = col_pointers[(k / B) * ((int64_t)g.fx * g.fy) + (j / B) * g.fx + (i / B)];
+        if (p == NULL_MAX) return NULL_MAX;
+        return p + (k % B) * B * B + (j % B) * B + (i % B);
+    }
+    R signed_distance(const V3<R>& p) const {
+        if (col_phi.empty()) return col_bg;
+        int64_t i, j, k;
+        R x, y, z;
+        lerp_corner(p, i, j, k, x, y, z);
+        auto F = [&](int64_t a, int64_t b2, int64_t c) {
+            const uint64_t n = col_index(a, b2, c);
+            return n == NULL_MAX ? col_bg : col_phi[n];
+        };
+        const R c00 = F(i, j, k) * (R(1) - x) + F(i + 1, j, k) * x;
+        const R c01 = F(i, j, k + 1) * (R(1) - x) + F(i + 1, j, k + 1) * x;
+        const R c10 = F(i, j + 1, k) * (R(1) - x) + F(i + 1, j + 1, k) * x;
+        const R c11 = F(i, j + 1, k + 1) * (R(1) - x) + F(i + 1, j + 1, k + 1) * x;
+        const R c0 = c00 * (R(1) - y) + c10 * y;
+        const R c1 = c01 * (R(1) - y) + c11 * y;
+        return c0 * (R(1) - z) + c1 * z;
+    }
+    V3<R> col_lerp(const std::vector<V3<R>>& f, const V3<R>& p) const {
+        if (f.empty()) return V3<R>{0, 0, 0};
+        int64_t i, j, k;
+        R x, y, z;
+        lerp_corner(p, i, j, k, x, y, z);
+        auto F = [&](int64_t a, int64_t b2, int64_t c) {
+            const uint64_t n = col_index(a, b2, c);
+            return n == NULL_MAX ? V3<R>{0, 0, 0} : f[n];
+        };
+        auto L = [&](const V3<R>& a, const V3<R>& c, R t) {
+            return V3<R>{a.x * (R(1) - t) + c.x * t, a.y * (R(1) - t) + c.y * t, a.z * (R(1) - t) + c.z * t};
+        };
+        const V3<R> c00 = L(F(i, j, k), F(i + 1, j, k), x);
+        const V3<R> c01 = L(F(i, j, k + 1), F(i + 1, j, k + 1), x);
+        const V3<R> c10 = L(F(i, j + 1, k), F(i + 1, j + 1, k), x);
+        const V3<R> c11 = L(F(i, j + 1, k + 1), F(i + 1, j + 1, k + 1), x);
+        return L(L(c00, c10, y), L(c01, c11, y), z);
+    }
+    V3<R> col_velocity(const V3<R>& p) const { return col_lerp(col_vel, p); }
+    V3<R> col_normal(const V3<R>& p) const { return col_lerp(col_nrm, p); }
+    // Vector3::normalized (Vector3.h:354-358): d = 1/Sqrt(x^2+y^2+z^2+EPSILON), EPSILON = 1e-30f
+    static V3<R> normalized(const V3<R>& v) {
+        const R d = R(1) / std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z + R(1e-30f));
+        return V3<R>{v.x * d, v.y * d, v.z * d};
+    }
+
+    // CollisionSource::update (CollisionSource.h:66-126).  The OpenVDB inputs are replaced by
+    // `seeds` (world positions of the source's active voxels) and a sampler returning the signed
+    // distance, velocity and gradient (world units) at a world point.
+    template <class Sampler> void set_collision(const std::vector<float>& seeds, float bg_world, Sampler sample) {
+        const float h = g.s[0];                                  // CollisionSource.h:53-54 (Lx / nx)
+        const int fx = g.fx, fy = g.fy, fz = g.fz, B = g.B;
+        std::vector<int> mk((size_t)nblk(), 0);
+        for (size_t q = 0; q + 2 < seeds.size(); q += 3) {
+            float vp[3];                                          // Grid::voxelPoint (Grid.h:445-450)
+            for (int a = 0; a < 3; ++a) vp[a] = (seeds[q + a] - g.t[a]) / g.s[a];
+            const int ci = Max((int)(vp[0]) / B, 0), cj = Max((int)(vp[1]) / B, 0), ck = Max((int)(vp[2]) / B, 0);
+            for (int k = ck - 1; k <= Min(ck + 1, fz - 1); ++k)       // SparseFrame.cu:145-157
+                for (int j = cj - 1; j <= Min(cj + 1, fy - 1); ++j)
+                    for (int i = ci - 1; i <= Min(ci + 1, fx - 1); ++i) {
+                        const int64_t t = (int64_t)(k * fx * fy + j * fx + i);
+                        if (t < 0) continue;
+                        mk[(size_t)t] += 1;
+                    }
+        }
+        col_pointers.assign((size_t)nblk(), NULL_MAX);
+        col_coords.clear();
+        uint64_t count = 0;
+        for (size_t n = 0; n < mk.size(); ++n)
+            if (mk[n] > 0) {
+                col_pointers[n] = count;
+                col_coords.push_back(Idx3{(int64_t)(n % fx), (int64_t)((n / fx) % fy), (int64_t)(n / ((size_t)fx * fy))});
+                count += (uint64_t)B3();
+            }
+        col_bg = (R)(bg_world / h);                               // CollisionSource.h:90
+        const size_t nc = col_coords.size() * (size_t)B3();
+        col_phi.assign(nc, col_bg);
+        col_vel.assign(nc, V3<R>{0, 0, 0});
+        col_nrm.assign(nc, V3<R>{0, 0, 0});
+        for (size_t n = 0; n < nc; ++n) {
+            const Idx3& c = col_coords[n / B3()];
+            const uint64_t b = n % B3();
+            const int64_t i = c.i * B + b % B, j = c.j * B + (b / B) % B, k = c.k * B + b / (B * B);
+            const float ctr[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+            float wp[3];                                          // Grid::worldPoint (Grid.h:431-436)
+            for (int a = 0; a < 3; ++a) wp[a] = ctr[a] * g.s[a] + g.t[a];
+            float sdf = 0.f, v[3] = {0, 0, 0}, gr[3] = {0, 0, 0};
+            sample(wp, &sdf, v, gr);
+            col_phi[n] = (R)sdf / (R)h;                           // :108
+            col_vel[n] = V3<R>{(R)v[0] / (R)g.s[0], (R)v[1] / (R)g.s[1], (R)v[2] / (R)g.s[2]};   // :113 voxelVector
+            col_nrm[n] = normalized(V3<R>{(R)gr[0] / (R)g.s[0], (R)gr[1] / (R)g.s[1], (R)gr[2] / (R)g.s[2]});   // :120
         }
     }
 
@@ -541,7 +660,7 @@ template <class R> struct State {
         return v;
     }
 
-    // ---- FLIPMethod::updateParticles (FLIPMethod.cu:54-175), no collision source ----
+    // ---- FLIPMethod::updateParticles (FLIPMethod.cu:54-175) ----
     void update_particles(R dt) {
         const R lx = (R)((float)g.nx + 0.5f), lz = (R)((float)g.nz + 0.5f);
         const bool tank = prm.enable_water_tank != 0;
@@ -571,10 +690,17 @@ template <class R> struct State {
                 if (ps.z > lz - band) w = Min(w, Clamp((lz - ps.z) / band, R(0), R(1)));
                 v.x *= w; v.y *= w; v.z *= w;
             }
+            if (signed_distance(ps) < R(0)) {                      // :116-119
+                const V3<R> cv = col_velocity(ps);
+                ps.x += cv.x * dt;
+                ps.y += cv.y * dt;
+                ps.z += cv.z * dt;
+            }
             {
                 const R len = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
                 const int iter = 1 + Min((int)(len * dt * R(1.5)), Max(prm.max_substeps, 1));
                 const R sub_dt = dt / (R)iter;
+                bool col = false;
                 for (int s = 0; s < iter; ++s) {
                     ps.x += v.x * sub_dt;
                     ps.y += v.y * sub_dt;
@@ -587,6 +713,18 @@ template <class R> struct State {
                         if (ps.y > (R)(float)g.ny) { ps.y = (R)(float)g.ny; if (v.y > R(0)) v.y = R(0); }
                         if (ps.z > (R)(float)g.nz) { ps.z = (R)(float)g.nz; if (v.z > R(0)) v.z = R(0); }
                     }
+                    const R phi = signed_distance(ps);                  // :146-157
+                    if (phi < R(0)) {
+                        const V3<R> nor = normalized(col_normal(ps));
+                        ps.x -= nor.x * phi;
+                        ps.y -= nor.y * phi;
+                        ps.z -= nor.z * phi;
+                        col = true;
+                    }
+                }
+                if (col) {                                              // :161-164
+                    const R phi = signed_distance(ps);
+                    if (phi > R(3) || phi < R(0)) removed[ix] = 1;
                 }
                 if (!inside(ps)) removed[ix] = 1;
             }
@@ -851,6 +989,101 @@ void orc_get_stats(void* hp, orc_stats* st) {
         st->cg_iters = s->cg_iters;
         st->cg_residual = s->cg_res;
         st->b_norm = s->b_norm;
+    };
+    if (h->fp64) run(h->s64); else run(h->s32);
+}
+
+// ---- collision source ----
+// Axis-aligned box [lo,hi] (world): exact signed distance (negative inside) and its gradient,
+// evaluated in FP64 and rounded once: outside, |max(q,0)| with q = |p-c| - e per axis and the
+// gradient along sign(p-c)*max(q,0); inside, max(q) (<= 0) with the gradient along the axis of the
+// largest q (first of x, y, z on ties).
+static void box_sample(const float lo[3], const float hi[3], const float p[3], float* sdf, float grad[3]) {
+    double d[3], q[3], o[3];
+    for (int a = 0; a < 3; ++a) {
+        const double c = 0.5 * ((double)lo[a] + (double)hi[a]);
+        const double e = 0.5 * ((double)hi[a] - (double)lo[a]);
+        d[a] = (double)p[a] - c;
+        q[a] = std::fabs(d[a]) - e;
+        o[a] = q[a] > 0.0 ? q[a] : 0.0;
+    }
+    const double out = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+    const double qm = std::max(q[0], std::max(q[1], q[2]));
+    *sdf = (float)(out + std::min(qm, 0.0));
+    if (out > 0.0) {
+        for (int a = 0; a < 3; ++a) grad[a] = (float)((d[a] < 0.0 ? -o[a] : o[a]) / out);
+    } else {
+        const int ax = q[0] >= q[1] && q[0] >= q[2] ? 0 : (q[1] >= q[2] ? 1 : 2);
+        for (int a = 0; a < 3; ++a) grad[a] = a == ax ? (d[a] < 0.0 ? -1.f : 1.f) : 0.f;
+    }
+}
+
+// Box obstacle moving with velocity vel (world units): the source's active voxels are the cells
+// whose centre lies within `band` voxels of the surface (|sdf|/h < band, as in a narrow-band level
+// set of half-width `band`); the background is +band voxels.
+void orc_set_collision_box(void* hp, const float lo[3], const float hi[3], const float vel[3], float band) {
+    Handle* h = (Handle*)hp;
+    auto run = [&](auto* s) {
+        const Geom& g = s->g;
+        const float hv = g.s[0];
+        std::vector<float> seeds;
+        for (int64_t k = 0; k < g.nz; ++k)
+            for (int64_t j = 0; j < g.ny; ++j)
+                for (int64_t i = 0; i < g.nx; ++i) {
+                    const float ctr[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+                    float wp[3], sdf, gr[3];
+                    for (int a = 0; a < 3; ++a) wp[a] = ctr[a] * g.s[a] + g.t[a];
+                    box_sample(lo, hi, wp, &sdf, gr);
+                    if (std::fabs(sdf / hv) < band)
+                        for (int a = 0; a < 3; ++a) seeds.push_back(wp[a]);
+                }
+        s->set_collision(seeds, band * hv, [&](const float* wp, float* sdf, float* v, float* gr) {
+            box_sample(lo, hi, wp, sdf, gr);
+            for (int a = 0; a < 3; ++a) v[a] = vel[a];
+        });
+    };
+    if (h->fp64) run(h->s64); else run(h->s32);
+}
+
+// General source: seeds (world xyz, n points) and a sampler callback.
+typedef void (*orc_sampler)(void* user, const float* wp, float* sdf, float* vel, float* grad);
+void orc_set_collision(void* hp, const float* seeds, size_t n, float bg_world, orc_sampler fn, void* user) {
+    Handle* h = (Handle*)hp;
+    std::vector<float> sd(seeds, seeds + 3 * n);
+    auto cb = [&](const float* wp, float* sdf, float* v, float* gr) { fn(user, wp, sdf, v, gr); };
+    if (h->fp64) h->s64->set_collision(sd, bg_world, cb); else h->s32->set_collision(sd, bg_world, cb);
+}
+
+// Collision frame + fields: pointers[fx*fy*fz] (-1 inactive), phi[nc], vel[3nc], nrm[3nc]; returns nc.
+size_t orc_get_collision(void* hp, int64_t* pointers, double* phi, double* vel, double* nrm, double* bg) {
+    Handle* h = (Handle*)hp;
+    auto run = [&](auto* s) {
+        if (pointers)
+            for (size_t n = 0; n < s->col_pointers.size(); ++n)
+                pointers[n] = s->col_pointers[n] == NULL_MAX ? -1 : (int64_t)s->col_pointers[n];
+        for (size_t n = 0; n < s->col_phi.size(); ++n) {
+            if (phi) phi[n] = s->col_phi[n];
+            if (vel) { vel[3 * n] = s->col_vel[n].x; vel[3 * n + 1] = s->col_vel[n].y; vel[3 * n + 2] = s->col_vel[n].z; }
+            if (nrm) { nrm[3 * n] = s->col_nrm[n].x; nrm[3 * n + 1] = s->col_nrm[n].y; nrm[3 * n + 2] = s->col_nrm[n].z; }
+        }
+        if (bg) *bg = s->col_bg;
+        return s->col_phi.size();
+    };
+    return h->fp64 ? run(h->s64) : run(h->s32);
+}
+
+// Signed distance, velocity and normal lerps at voxel-space points (CollisionSource.h:128-144).
+void orc_collision_query(void* hp, const float* pts, size_t n, double* phi, double* vel, double* nrm) {
+    Handle* h = (Handle*)hp;
+    auto run = [&](auto* s) {
+        using R = typename std::remove_reference<decltype(s->mass.data()[0])>::type;
+        for (size_t q = 0; q < n; ++q) {
+            const V3<R> p{(R)pts[3 * q], (R)pts[3 * q + 1], (R)pts[3 * q + 2]};
+            if (phi) phi[q] = s->signed_distance(p);
+            const V3<R> v = s->col_velocity(p), nr = s->col_normal(p);
+            if (vel) { vel[3 * q] = v.x; vel[3 * q + 1] = v.y; vel[3 * q + 2] = v.z; }
+            if (nrm) { nrm[3 * q] = nr.x; nrm[3 * q + 1] = nr.y; nrm[3 * q + 2] = nr.z; }
+        }
     };
     if (h->fp64) run(h->s64); else run(h->s32);
 }

@@ -33,6 +33,9 @@ class OrcStats(C.Structure):
 
 _lib = None
 _V = C.c_void_p
+# sampler(user, world_xyz[3], *sdf, vel[3], grad[3]) -- world units
+SAMPLER = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                      C.POINTER(C.c_float))
 
 
 def load():
@@ -62,6 +65,10 @@ def load():
         "orc_remove_perm": (C.c_size_t, [_V, C.c_size_t, _V]),
         "orc_seed_box": (C.c_size_t, [C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_int,
                                       C.c_uint32, _V, C.c_size_t]),
+        "orc_set_collision_box": (None, [_V, _V, _V, _V, C.c_float]),
+        "orc_set_collision": (None, [_V, _V, C.c_size_t, C.c_float, SAMPLER, _V]),
+        "orc_get_collision": (C.c_size_t, [_V, _V, _V, _V, _V, _V]),
+        "orc_collision_query": (None, [_V, _V, C.c_size_t, _V, _V, _V]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -233,3 +240,39 @@ class Oracle:
         s = OrcStats()
         self.lib.orc_get_stats(self.h, C.byref(s))
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def set_collision_box(self, lo, hi, vel=(0.0, 0.0, 0.0), band=3.0):
+        """Box obstacle [lo,hi] (world), moving with `vel` (world units); narrow band of `band` voxels."""
+        lo, hi, v = (np.ascontiguousarray(a, np.float32) for a in (lo, hi, vel))
+        self.lib.orc_set_collision_box(self.h, _p(lo), _p(hi), _p(v), float(band))
+
+    def set_collision(self, seeds, background, fn):
+        """General source: seeds (n,3 world) and fn(world_xyz) -> (sdf, vel[3], grad[3]) in world units."""
+        seeds = np.ascontiguousarray(seeds, np.float32).reshape(-1)
+
+        def cb(user, wp, sdf, vel, grad):
+            d, v, g = fn(np.array([wp[0], wp[1], wp[2]], np.float32))
+            sdf[0] = d
+            for a in range(3):
+                vel[a] = v[a]
+                grad[a] = g[a]
+
+        self._sampler = SAMPLER(cb)
+        self.lib.orc_set_collision(self.h, _p(seeds), seeds.size // 3, float(background), self._sampler, None)
+
+    def collision(self):
+        """(pointers, phi, vel, nrm, background) of the collision source's sparse field."""
+        f = self.geom["f"]
+        pointers = np.empty(int(f[0]) * int(f[1]) * int(f[2]), np.int64)
+        n = self.lib.orc_get_collision(self.h, None, None, None, None, None)
+        phi, vel, nrm, bg = np.empty(n), np.empty((n, 3)), np.empty((n, 3)), np.empty(1)
+        self.lib.orc_get_collision(self.h, _p(pointers), _p(phi), _p(vel), _p(nrm), _p(bg))
+        return pointers, phi, vel, nrm, float(bg[0])
+
+    def collision_query(self, pts):
+        """Signed distance, velocity and normal lerps at voxel-space points."""
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        n = pts.shape[0]
+        phi, vel, nrm = np.empty(n), np.empty((n, 3)), np.empty((n, 3))
+        self.lib.orc_collision_query(self.h, _p(pts), n, _p(phi), _p(vel), _p(nrm))
+        return phi, vel, nrm

@@ -380,3 +380,173 @@ def test_fp32_fp64_instances_agree_one_step():
     a, b = o32.get_particles(), o64.get_particles()
     assert np.linalg.norm(a["vel"] - b["vel"]) / np.linalg.norm(b["vel"]) < 1e-4
     np.testing.assert_array_equal(a["adj"], b["adj"])
+
+
+# ---------------------------------------------------------------- collision source (CollisionSource.h)
+def _box_sdf(p, lo, hi):
+    """Exact box signed distance in float64 (negative inside); corners as the FP32 values passed."""
+    lo = np.asarray(lo, np.float32).astype(np.float64)
+    hi = np.asarray(hi, np.float32).astype(np.float64)
+    c, e = (lo + hi) / 2, (hi - lo) / 2
+    q = np.abs(np.asarray(p, np.float64) - c) - e
+    return np.linalg.norm(np.maximum(q, 0), axis=-1) + np.minimum(q.max(-1), 0)
+
+
+def _storage_ijk(pointers, f, B):
+    """(i,j,k) of every cell of a sparse field in storage order."""
+    act = np.nonzero(pointers >= 0)[0]
+    order = np.argsort(pointers[act])
+    blk = act[order]
+    bi, bj, bk = blk % f[0], (blk // f[0]) % f[1], blk // (f[0] * f[1])
+    l = np.arange(B ** 3)
+    li, lj, lk = l % B, (l // B) % B, l // (B * B)
+    return np.stack([(bi[:, None] * B + li).ravel(), (bj[:, None] * B + lj).ravel(),
+                     (bk[:, None] * B + lk).ravel()], 1)
+
+
+@pytest.mark.parametrize("B", [4, 8])
+def test_collision_box_field_closed_form(B):
+    n, band = 32, 3.0
+    lo, hi, v = (0.3, 0.1, 0.25), (0.55, 0.4, 0.7), (0.5, 0.0, -0.25)
+    o = O.Oracle((n, n, n), B=B, prm=dam_params(n))
+    o.set_collision_box(lo, hi, v, band)
+    pointers, phi, vel, nrm, bg = o.collision()
+    h = 1.0 / n
+    f = o.geom["f"]
+    assert bg == band
+    # activation = 3x3x3 block neighbourhoods of the cells within `band` voxels of the surface
+    ii = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    ctr = (ii + 0.5).astype(np.float32)
+    sdf32 = _box_sdf(ctr * np.float32(h), lo, hi).astype(np.float32)
+    seeds = ctr[np.abs(sdf32 / np.float32(h)) < band]
+    np.testing.assert_array_equal(pointers >= 0, _brute_active(seeds, f, B) > 0)
+    ijk = _storage_ijk(pointers, f, B)
+    wp = (ijk + 0.5) * h
+    np.testing.assert_allclose(phi, _box_sdf(wp, lo, hi) / h, rtol=0, atol=2e-5)
+    np.testing.assert_allclose(vel, np.tile(np.array(v) / h, (phi.size, 1)), rtol=1e-6)
+    # normals: unit gradient of the exact SDF
+    lo64, hi64 = np.float32(lo).astype(np.float64), np.float32(hi).astype(np.float64)
+    c = (lo64 + hi64) / 2
+    e = (hi64 - lo64) / 2
+    d = wp - c
+    q = np.abs(d) - e
+    out = q.max(1) > 0
+    g = np.sign(d) * np.maximum(q, 0)
+    g[out] /= np.linalg.norm(g[out], axis=1)[:, None]
+    ax = np.argmax(q, 1)
+    gi = np.zeros_like(g)
+    gi[np.arange(len(ax)), ax] = np.where(d[np.arange(len(ax)), ax] < 0, -1.0, 1.0)
+    g[~out] = gi[~out]
+    np.testing.assert_allclose(nrm, g, atol=1e-6)
+
+
+def _scene_particles(n, seed=4):
+    pos = O.seed_box(n, n, n, 0.5 * n, 0.5 * n, n, 8, 1)
+    vel = (np.random.default_rng(seed).standard_normal(pos.shape) * 5).astype(np.float32)
+    return pos, vel
+
+
+def test_collision_classification_and_velocity():
+    # FLIPMethod.cu:242-260: solid = no mass and phi < 0 at the cell centre; where phi < 0 the
+    # collision velocity times col_vel_scale is added to the P2G velocity (fluid cells included)
+    n, B, scale = 16, 4, 0.5
+    pos, vel = _scene_particles(n)
+    vbox = np.array([0.3, -0.2, 0.1])
+    with_col = O.Oracle((n, n, n), B=B, prm=dam_params(n, col_vel_scale=scale), fp64=True)
+    no_col = O.Oracle((n, n, n), B=B, prm=dam_params(n, col_vel_scale=scale), fp64=True)
+    with_col.set_collision_box((0.3, 0.05, 0.2), (0.8, 0.45, 0.6), vbox, 3.0)
+    for o in (with_col, no_col):
+        o.set_particles(pos, vel)
+        o.stage(0)
+        o.stage(1)
+    pointers, _ = with_col.block_table()
+    ijk = _storage_ijk(pointers, with_col.geom["f"], B)
+    phi, _, _ = with_col.collision_query((ijk + 0.5).astype(np.float32))
+    mass = with_col.field(O.FIELD_MASS)
+    expect = np.where(mass > 0, 4, np.where(phi < 0, 1, 0))
+    np.testing.assert_array_equal(with_col.field(O.FIELD_MARK), expect)
+    assert (expect == 1).sum() > 50 and ((expect == 4) & (phi < 0)).sum() > 50
+    dv = with_col.field(O.FIELD_VELO) - no_col.field(O.FIELD_VELO)
+    np.testing.assert_allclose(dv, (phi < 0)[:, None] * (vbox * n * scale)[None, :], atol=1e-9)
+    np.testing.assert_array_equal(with_col.field(O.FIELD_COMP), with_col.field(O.FIELD_VELO))
+
+
+def test_collision_projection_and_push_kat():
+    # FLIPMethod.cu:116-164 with zero grid velocity and no gravity: a particle inside the box is
+    # projected along the (exact, axis-aligned) normal onto the face; a particle outside is left
+    # alone; a moving box pushes an inside particle by v*dt first (here out of the box, so no
+    # projection and no removal).
+    n, B = 16, 4
+    dt = 1 / 240
+    o = O.Oracle((n, n, n), B=B, prm=dam_params(n, ext_force=(0, 0, 0)), fp64=True)
+    pos = np.array([[5.3, 8.2, 7.9],       # 1.3 voxels inside the -x face (x = 4)
+                    [2.0, 8.0, 8.0],       # outside
+                    [8.1, 11.4, 7.6],      # 0.6 inside the +y face (y = 12)
+                    [11.95, 8.3, 8.4]], np.float32)   # 0.05 inside +x; pushed out by the moving box
+    o.set_collision_box((0.25, 0.25, 0.25), (0.75, 0.75, 0.75), (0.0, 0.0, 0.0), 3.0)
+    o.set_particles(pos, np.zeros_like(pos))
+    o.stage(0); o.stage(1); o.stage(2)
+    o.set_field(O.FIELD_COMP, o.field(O.FIELD_VELO))
+    o.stage(4, dt)
+    p = o.get_particles()["pos"]
+    assert abs(p[0, 0] - 4.0) < 1e-9 and p[0, 1] == np.float32(8.2) and p[0, 2] == np.float32(7.9)
+    np.testing.assert_array_equal(p[1], pos[1].astype(np.float64))
+    assert abs(p[2, 1] - 12.0) < 1e-9 and p[2, 0] == np.float32(8.1) and p[2, 2] == np.float32(7.6)
+    assert abs(p[3, 0] - 12.0) < 1e-9
+    phi_after, _, _ = o.collision_query(p.astype(np.float32))
+    rem = o.removed()
+    assert rem[1] == 0
+    # a particle that collided is removed iff its final distance reads < 0 or > 3
+    assert all(rem[q] == (phi_after[q] < 0 or phi_after[q] > 3) for q in (0, 2, 3))
+    # the same box moving at +2 world units/s in x: the +x-face particle is pushed out first
+    # (col_vel_scale 0 keeps the box velocity out of the grid velocity, so the particle's own
+    # velocity stays 0 and the push is the only motion)
+    o2 = O.Oracle((n, n, n), B=B, prm=dam_params(n, ext_force=(0, 0, 0), col_vel_scale=0.0), fp64=True)
+    o2.set_collision_box((0.25, 0.25, 0.25), (0.75, 0.75, 0.75), (2.0, 0.0, 0.0), 3.0)
+    o2.set_particles(pos[3:], np.zeros_like(pos[3:]))
+    o2.stage(0); o2.stage(1); o2.stage(2)
+    o2.set_field(O.FIELD_COMP, o2.field(O.FIELD_VELO))
+    o2.stage(4, dt)
+    p2 = o2.get_particles()["pos"]
+    assert abs(p2[0, 0] - (np.float64(np.float32(11.95)) + 2.0 * n * np.float64(np.float32(dt)))) < 1e-9
+    assert o2.removed()[0] == 0
+
+
+def test_collision_sampler_callback_matches_box():
+    # the general (seeds + sampler) entry reproduces the built-in box when fed the same seeds
+    n, B = 16, 4
+    lo, hi, v = (0.3, 0.2, 0.25), (0.6, 0.5, 0.7), (0.1, 0.2, 0.3)
+    a = O.Oracle((n, n, n), B=B, prm=dam_params(n))
+    a.set_collision_box(lo, hi, v, 2.0)
+    h = np.float32(1.0 / n)
+    ii = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    ctr = (ii + 0.5).astype(np.float32)
+    wp = ctr * h
+    sdf = _box_sdf(wp, lo, hi).astype(np.float32)
+    seeds = wp[np.abs(sdf / h) < 2.0]
+    b = O.Oracle((n, n, n), B=B, prm=dam_params(n))
+    lo64, hi64 = np.float32(lo).astype(np.float64), np.float32(hi).astype(np.float64)
+    c = 0.5 * (lo64 + hi64)
+    e = 0.5 * (hi64 - lo64)
+
+    def sample(p):
+        d = p.astype(np.float64) - c
+        q = np.abs(d) - e
+        o = np.maximum(q, 0)
+        out = np.sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2])
+        qm = max(q[0], max(q[1], q[2]))
+        s = np.float32(out + min(qm, 0.0))
+        if out > 0:
+            g = [np.float32((-o[x] if d[x] < 0 else o[x]) / out) for x in range(3)]
+        else:
+            ax = 0 if (q[0] >= q[1] and q[0] >= q[2]) else (1 if q[1] >= q[2] else 2)
+            g = [0.0, 0.0, 0.0]
+            g[ax] = -1.0 if d[ax] < 0 else 1.0
+        return s, v, g
+
+    b.set_collision(seeds, 2.0 / n, sample)
+    ra, rb = a.collision(), b.collision()
+    np.testing.assert_array_equal(ra[0], rb[0])
+    for x, y in zip(ra[1:4], rb[1:4]):
+        np.testing.assert_array_equal(x, y)
+    assert ra[4] == rb[4]

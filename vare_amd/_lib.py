@@ -120,6 +120,8 @@ class ServerStats(C.Structure):
 
 _P = C.c_void_p
 _FP = C.POINTER(C.c_float)
+# vare_collision_sampler(user, wp[3], *sdf, vel[3], grad[3]) -- world units
+COLLISION_SAMPLER = C.CFUNCTYPE(None, C.c_void_p, _FP, _FP, _FP, _FP)
 
 # name -> (restype, argtypes); every symbol declared in include/vare_flip.h
 SIGNATURES = {
@@ -142,6 +144,10 @@ SIGNATURES = {
     "vare_flip_get_block_table": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vare_flip_get_cell_field": (C.c_int, [_P, C.c_int, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vare_flip_get_removed_mask": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vare_flip_set_collision": (C.c_int, [_P, _P, C.c_size_t, C.c_float, COLLISION_SAMPLER, _P]),
+    "vare_flip_set_collision_box": (C.c_int, [_P, C.c_float * 3, C.c_float * 3, C.c_float * 3, C.c_float]),
+    "vare_flip_clear_collision": (C.c_int, [_P]),
+    "vare_flip_get_collision": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), _FP]),
     "vare_flip_get_binning": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "vare_seed_box": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_int,
                                 C.c_uint32, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -359,6 +365,41 @@ class FlipHandle:
         out = np.empty(n.value, np.uint8)
         check(self.lib.vare_flip_get_removed_mask(self.h, ptr(out), n.value, C.byref(n)))
         return out
+
+    def set_collision_box(self, lo, hi, vel=(0.0, 0.0, 0.0), band=3.0):
+        """Box obstacle [lo,hi] (world) moving with `vel` (world units/s); narrow band in voxels."""
+        f3 = C.c_float * 3
+        check(self.lib.vare_flip_set_collision_box(self.h, f3(*lo), f3(*hi), f3(*vel), float(band)))
+
+    def set_collision(self, seeds, background, fn):
+        """General source: seeds (n,3 world) and fn(world_xyz) -> (sdf, vel[3], grad[3]) in world units."""
+        seeds = np.ascontiguousarray(seeds, np.float32).reshape(-1)
+
+        def cb(user, wp, sdf, vel, grad):
+            d, v, g = fn(np.array([wp[0], wp[1], wp[2]], np.float32))
+            sdf[0] = d
+            for a in range(3):
+                vel[a] = v[a]
+                grad[a] = g[a]
+
+        sampler = COLLISION_SAMPLER(cb)
+        check(self.lib.vare_flip_set_collision(self.h, ptr(seeds), seeds.size // 3, float(background), sampler, None))
+
+    def clear_collision(self):
+        check(self.lib.vare_flip_clear_collision(self.h))
+
+    def collision(self):
+        """(pointers, phi, vel, nrm, background) of the collision source's sparse field."""
+        fr = self.frame()
+        nblk = fr.fx * fr.fy * fr.fz
+        n = C.c_size_t()
+        bg = C.c_float()
+        pointers = np.empty(nblk, np.int64)
+        check(self.lib.vare_flip_get_collision(self.h, ptr(pointers), None, None, None, 0, C.byref(n), C.byref(bg)))
+        m = n.value
+        phi, vel, nrm = np.empty(m, np.float32), np.empty((m, 3), np.float32), np.empty((m, 3), np.float32)
+        check(self.lib.vare_flip_get_collision(self.h, None, ptr(phi), ptr(vel), ptr(nrm), m, C.byref(n), C.byref(bg)))
+        return pointers, phi, vel, nrm, bg.value
 
     def set_kernel_timing(self, kernels=()):
         mask = 0

@@ -136,6 +136,15 @@ struct vare_flip {
     CgState* st_host = nullptr;   // pinned mirror
     int* dint_host = nullptr;     // pinned mirror of dint
 
+    // collision source (CollisionSource): block table over the frame + fields of its active cells
+    int* col_slot = nullptr;      // [nblk]
+    float* col_phi = nullptr;
+    float* col_v[3] = {};
+    float* col_n[3] = {};
+    size_t col_cells = 0;
+    float col_bg = 5.f;           // CollisionSource.h:56, the value before the first update
+    int col_active = 0;
+
     // stats + timing
     vare_step_stats stats;
     int stage_next = 0;
@@ -158,6 +167,10 @@ struct vare_flip {
         return r;
     }
     FrameK frame() const { return FrameK{fx, fy, fz, nx, ny, nz, slot, coord, nbr}; }
+    ColK colk() const {
+        return ColK{col_slot, col_phi, {col_v[0], col_v[1], col_v[2]}, {col_n[0], col_n[1], col_n[2]}, col_bg,
+                    col_active};
+    }
 };
 
 namespace {
@@ -174,6 +187,14 @@ void free_mg(vare_flip* h) {
     h->mg_z.clear();
     h->mg_zt.clear();
     h->mg_bcap = 0;
+}
+
+void free_collision(vare_flip* h) {
+    dfree(h->col_slot);
+    dfree(h->col_phi);
+    for (int a = 0; a < 3; ++a) { dfree(h->col_v[a]); dfree(h->col_n[a]); }
+    h->col_cells = 0;
+    h->col_active = 0;
 }
 
 void free_cells(vare_flip* h) {
@@ -392,7 +413,8 @@ template <int B> int stage_p2g(vare_flip* h) {
         if (h->nb)
             hipLaunchKernelGGL(k_p2g_gather<B>, dim3(h->nb), dim3(P2GGather<B>::THREADS), 0, h->stream,
                                (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
-                               (const int*)h->perm, (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type);
+                               (const int*)h->perm, (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type,
+                               h->colk(), h->prm.col_vel_scale);
         tend(h, VARE_KERNEL_P2G);
         HIPCHK(hipGetLastError());
         return VARE_OK;
@@ -405,8 +427,8 @@ template <int B> int stage_p2g(vare_flip* h) {
                            h->vel(), (int)h->np, h->frame(), cf);
     tend(h, VARE_KERNEL_P2G);
     if (nc)
-        hipLaunchKernelGGL(k_cell_finalize, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, cf, h->cx, h->cy, h->cz,
-                           h->type);
+        hipLaunchKernelGGL(k_cell_finalize<B>, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, cf, h->cx, h->cy,
+                           h->cz, h->type, h->frame(), h->colk(), h->prm.col_vel_scale);
     HIPCHK(hipGetLastError());
     return VARE_OK;
 }
@@ -706,12 +728,13 @@ template <int B> int stage_g2p(vare_flip* h, float dt) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
         hipLaunchKernelGGL(k_g2p_tile<B>, dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
                            h->vel(), h->pa[6], (const int*)h->perm, (const int*)h->cstart, h->nb,
-                           h->frame(), gf, g2p_params(h, dt), h->removed, h->dint + 1);
-    } else if (h->np)
+                           h->frame(), gf, g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
+    } else if (h->np) {
+        G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
         hipLaunchKernelGGL(k_g2p<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
-                           h->vel(), h->pa[6], (int)h->np, h->frame(), (const float*)h->cnt, (const float*)h->cx,
-                           (const float*)h->cy, (const float*)h->cz, (const float*)h->vx, (const float*)h->vy,
-                           (const float*)h->vz, g2p_params(h, dt), h->removed, h->dint + 1);
+                           h->vel(), h->pa[6], (int)h->np, h->frame(), gf, g2p_params(h, dt), h->colk(), h->removed,
+                           h->dint + 1);
+    }
     tend(h, VARE_KERNEL_G2P);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h->dint_host + 1, h->dint + 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -913,6 +936,7 @@ int vare_flip_destroy(vare_flip* h) {
     dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins);
     dfree(h->flag); dfree(h->slot); dfree(h->coord);
     free_cells(h);
+    free_collision(h);
     dfree(h->tiles); dfree(h->dint); dfree(h->st);
     if (h->st_host) (void)hipHostFree(h->st_host);
     if (h->dint_host) (void)hipHostFree(h->dint_host);
@@ -1248,6 +1272,118 @@ int seed_points(int64_t nx, int64_t ny, int64_t nz, int ppc, uint32_t seed, floa
     return VARE_OK;
 }
 
+// CollisionSource::update (CollisionSource.h:66-126), host-side as in the reference.  The OpenVDB
+// grids are replaced by `seeds` (world points of the source's active voxels, nseed of them) and a
+// sampler of signed distance, velocity and gradient (world units) at a world point:
+//   * the source's block table marks the 3x3x3 block neighbourhood of every seed's block
+//     (SparseFrame::buildFromPoints, SparseFrame.cu:145-157, with its index aliasing; negative
+//     indices are dropped), active blocks numbered in ascending linear order;
+//   * every active cell centre is sampled: phi = sdf / h (h = Lx / nx), velocity and gradient are
+//     converted with voxelVector and the gradient is normalised (Vector3::normalized);
+//   * inactive cells read background / h.
+// `threads` > 1 samples blocks on several host threads (the sampler must then be reentrant).
+template <class Sampler>
+int build_collision(vare_flip* h, const float* seeds, size_t nseed, float bg_world, Sampler sample, int threads) {
+    const int B = h->B, fx = h->fx, fy = h->fy, fz = h->fz;
+    const int B3 = B * B * B;
+    const float hv = h->s[0];
+    std::vector<char> mark((size_t)h->nblk, 0);
+    for (size_t q = 0; q < nseed; ++q) {
+        float vp[3];
+        for (int a = 0; a < 3; ++a) vp[a] = (seeds[3 * q + a] - h->t[a]) / h->s[a];      // Grid::voxelPoint
+        const int ci = std::max((int)vp[0] / B, 0), cj = std::max((int)vp[1] / B, 0), ck = std::max((int)vp[2] / B, 0);
+        for (int k = ck - 1; k <= std::min(ck + 1, fz - 1); ++k)
+            for (int j = cj - 1; j <= std::min(cj + 1, fy - 1); ++j)
+                for (int i = ci - 1; i <= std::min(ci + 1, fx - 1); ++i) {
+                    const long long t = (long long)(k * fx * fy + j * fx + i);
+                    if (t >= 0) mark[(size_t)t] = 1;
+                }
+    }
+    std::vector<int> slot((size_t)h->nblk, -1), blocks;
+    for (size_t t = 0; t < mark.size(); ++t)
+        if (mark[t]) { slot[t] = (int)blocks.size(); blocks.push_back((int)t); }
+    const size_t nbc = blocks.size(), nc = nbc * (size_t)B3;
+    std::vector<float> phi(nc), vel(3 * nc), nrm(3 * nc);
+    auto fill_block = [&](size_t b) {
+        const int t = blocks[b];
+        const int bi = t % fx, bj = (t / fx) % fy, bk = t / (fx * fy);
+        for (int l = 0; l < B3; ++l) {
+            const size_t c = b * (size_t)B3 + (size_t)l;
+            const int i = bi * B + l % B, j = bj * B + (l / B) % B, k = bk * B + l / (B * B);
+            const float ctr[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};   // Grid::cellCenter
+            float wp[3];
+            for (int a = 0; a < 3; ++a) wp[a] = ctr[a] * h->s[a] + h->t[a];              // Grid::worldPoint
+            float sdf = 0.f, v[3] = {0.f, 0.f, 0.f}, g[3] = {0.f, 0.f, 0.f};
+            sample(wp, &sdf, v, g);
+            phi[c] = sdf / hv;
+            for (int a = 0; a < 3; ++a) { vel[3 * c + a] = v[a] / h->s[a]; g[a] = g[a] / h->s[a]; }
+            const float d = 1.f / std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + 1e-30f);
+            for (int a = 0; a < 3; ++a) nrm[3 * c + a] = g[a] * d;
+        }
+    };
+    if (threads > 1 && nbc > 1) {
+        std::vector<std::thread> th;
+        for (int w = 0; w < threads; ++w)
+            th.emplace_back([&, w]() { for (size_t b = (size_t)w; b < nbc; b += (size_t)threads) fill_block(b); });
+        for (auto& x : th) x.join();
+    } else {
+        for (size_t b = 0; b < nbc; ++b) fill_block(b);
+    }
+    // upload: block table + structure-of-arrays fields
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_collision(h);
+    h->col_bg = bg_world / hv;                                   // CollisionSource.h:90
+    if (nbc == 0) return VARE_OK;                                // size()==0: every query returns background
+    CHK(dalloc(&h->col_slot, (size_t)h->nblk));
+    CHK(dalloc(&h->col_phi, nc));
+    for (int a = 0; a < 3; ++a) { CHK(dalloc(&h->col_v[a], nc)); CHK(dalloc(&h->col_n[a], nc)); }
+    HIPCHK(hipMemcpy(h->col_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->col_phi, phi.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> tmp(nc);
+    for (int a = 0; a < 3; ++a) {
+        for (size_t c = 0; c < nc; ++c) tmp[c] = vel[3 * c + a];
+        HIPCHK(hipMemcpy(h->col_v[a], tmp.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+        for (size_t c = 0; c < nc; ++c) tmp[c] = nrm[3 * c + a];
+        HIPCHK(hipMemcpy(h->col_n[a], tmp.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    }
+    h->col_cells = nc;
+    h->col_active = 1;
+    return VARE_OK;
+}
+
+// Axis-aligned box [lo,hi] (world): exact signed distance (negative inside) and gradient, in FP64,
+// rounded once to FP32.  With q = |p - c| - e per axis (c centre, e half extent): outside,
+// sdf = |max(q,0)| and grad = sign(p-c) max(q,0) / sdf; inside, sdf = max(q) and grad is the unit
+// vector along the axis holding max(q) (x before y before z on ties), signed by p - c.
+struct BoxSdf {
+    double c[3], e[3];
+    BoxSdf(const float lo[3], const float hi[3]) {
+        for (int a = 0; a < 3; ++a) {
+            c[a] = 0.5 * ((double)lo[a] + (double)hi[a]);
+            e[a] = 0.5 * ((double)hi[a] - (double)lo[a]);
+        }
+    }
+    void operator()(const float p[3], float* sdf, float grad[3]) const {
+        double d[3], q[3], o[3];
+        for (int a = 0; a < 3; ++a) {
+            d[a] = (double)p[a] - c[a];
+            q[a] = std::fabs(d[a]) - e[a];
+            o[a] = q[a] > 0.0 ? q[a] : 0.0;
+        }
+        const double out = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+        const double qmax = std::max(q[0], std::max(q[1], q[2]));
+        *sdf = (float)(out + std::min(qmax, 0.0));
+        if (out > 0.0) {
+            for (int a = 0; a < 3; ++a) grad[a] = (float)((d[a] < 0.0 ? -o[a] : o[a]) / out);
+            return;
+        }
+        const int ax = (q[0] >= q[1] && q[0] >= q[2]) ? 0 : (q[1] >= q[2] ? 1 : 2);
+        for (int a = 0; a < 3; ++a) grad[a] = a != ax ? 0.f : (d[a] < 0.0 ? -1.f : 1.f);
+    }
+};
+
+int host_threads() { return (int)std::max<unsigned>(1u, std::min<unsigned>(16u, std::thread::hardware_concurrency())); }
+
 }  // namespace
 
 extern "C" {
@@ -1265,6 +1401,88 @@ int vare_seed_pool_sphere(int64_t nx, int64_t ny, int64_t nz, float depth, float
         const float dx = x - cx, dy = y - cy, dz = z - cz;
         return y < depth || dx * dx + dy * dy + dz * dz < r2;
     });
+}
+
+int vare_flip_set_collision(vare_flip* h, const float* seeds, size_t nseed, float background,
+                            vare_collision_sampler fn, void* user) {
+    if (!h || !fn || (nseed && !seeds)) return fail(VARE_EINVAL, "null argument");
+    return build_collision(
+        h, seeds, nseed, background,
+        [&](const float* wp, float* sdf, float* v, float* g) { fn(user, wp, sdf, v, g); }, 1);
+}
+
+int vare_flip_set_collision_box(vare_flip* h, const float lo[3], const float hi[3], const float vel[3], float band) {
+    if (!h || !lo || !hi) return fail(VARE_EINVAL, "null argument");
+    if (!(band > 0.f) || !std::isfinite(band)) return fail(VARE_EINVAL, "band must be a positive number of voxels");
+    for (int a = 0; a < 3; ++a)
+        if (!(lo[a] <= hi[a]) || !std::isfinite(lo[a]) || !std::isfinite(hi[a]))
+            return fail(VARE_EINVAL, "box needs finite lo <= hi");
+    const BoxSdf box(lo, hi);
+    const float v[3] = {vel ? vel[0] : 0.f, vel ? vel[1] : 0.f, vel ? vel[2] : 0.f};
+    const float hv = h->s[0];
+    // active voxels: cells whose centre lies within `band` voxels of the surface; only cells of the
+    // box's bounding region grown by band+2 voxels can qualify
+    int lo_i[3], hi_i[3];
+    const int64_t n[3] = {h->nx, h->ny, h->nz};
+    for (int a = 0; a < 3; ++a) {
+        const double vl = ((double)lo[a] - h->t[a]) / h->s[a], vh = ((double)hi[a] - h->t[a]) / h->s[a];
+        lo_i[a] = (int)std::max<double>(0.0, std::min<double>((double)n[a], std::floor(vl - band - 2.0)));
+        hi_i[a] = (int)std::max<double>(0.0, std::min<double>((double)n[a], std::ceil(vh + band + 2.0)));
+    }
+    std::vector<float> seeds;
+    for (int k = lo_i[2]; k < hi_i[2]; ++k)
+        for (int j = lo_i[1]; j < hi_i[1]; ++j)
+            for (int i = lo_i[0]; i < hi_i[0]; ++i) {
+                const float ctr[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+                float wp[3], sdf, g[3];
+                for (int a = 0; a < 3; ++a) wp[a] = ctr[a] * h->s[a] + h->t[a];
+                box(wp, &sdf, g);
+                if (std::fabs(sdf / hv) < band) seeds.insert(seeds.end(), wp, wp + 3);
+            }
+    return build_collision(
+        h, seeds.data(), seeds.size() / 3, band * hv,
+        [&](const float* wp, float* sdf, float* vo, float* g) {
+            box(wp, sdf, g);
+            for (int a = 0; a < 3; ++a) vo[a] = v[a];
+        },
+        host_threads());
+}
+
+int vare_flip_clear_collision(vare_flip* h) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_collision(h);
+    h->col_bg = 5.f;
+    return VARE_OK;
+}
+
+int vare_flip_get_collision(vare_flip* h, int64_t* pointers, float* phi, float* vel, float* nrm, size_t cap, size_t* nc,
+                            float* background) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    const size_t m = h->col_cells;
+    if (nc) *nc = m;
+    if (background) *background = h->col_bg;
+    const int B3 = h->B * h->B * h->B;
+    if (pointers) {
+        std::vector<int> slot((size_t)h->nblk, -1);
+        if (h->col_slot) HIPCHK(hipMemcpy(slot.data(), h->col_slot, slot.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < slot.size(); ++t) pointers[t] = slot[t] < 0 ? -1 : (int64_t)slot[t] * B3;
+    }
+    if ((phi || vel || nrm) && cap < m) return fail(VARE_EINVAL, "capacity %zu < %zu cells", cap, m);
+    if (m == 0) return VARE_OK;
+    if (phi) HIPCHK(hipMemcpy(phi, h->col_phi, m * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<float> tmp(m);
+    for (int a = 0; a < 3; ++a) {
+        if (vel) {
+            HIPCHK(hipMemcpy(tmp.data(), h->col_v[a], m * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t c = 0; c < m; ++c) vel[3 * c + a] = tmp[c];
+        }
+        if (nrm) {
+            HIPCHK(hipMemcpy(tmp.data(), h->col_n[a], m * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t c = 0; c < m; ++c) nrm[3 * c + a] = tmp[c];
+        }
+    }
+    return VARE_OK;
 }
 
 int vare_device_synchronize(void) {

@@ -5,6 +5,7 @@
 // order; the library is compiled with -ffp-contract=off so no FMA is formed behind them.
 #pragma once
 #include "flip_device.h"
+#include "flip_collision.h"
 
 namespace vare {
 
@@ -206,9 +207,10 @@ __global__ void k_p2g_atomic(const float* __restrict__ px, const float* __restri
 }
 
 // P2G normalise (Rasterization.cu:175-188) + cell classification and comp := prev
-// (FLIPMethod.cu:242-280, no collision source: phi = 5, so no solid cells and no collision velocity).
+// (FLIPMethod.cu:242-280, curl disabled).
+template <int B>
 __global__ void k_cell_finalize(int nc, CellFields cf, float* __restrict__ cx, float* __restrict__ cy,
-                                float* __restrict__ cz, uint8_t* __restrict__ type) {
+                                float* __restrict__ cz, uint8_t* __restrict__ type, FrameK f, ColK col, float col_scale) {
     const int c = blockIdx.x * WG + threadIdx.x;
     if (c >= nc) return;
     const float m = cf.mass[c];
@@ -221,6 +223,12 @@ __global__ void k_cell_finalize(int nc, CellFields cf, float* __restrict__ cx, f
         t = kFluid;
     } else {
         cf.mass[c] = 0.f;
+    }
+    if (col.any()) {
+        const int t0 = f.coord[c / Blk<B>::B3], l = c % Blk<B>::B3;
+        const int i = (t0 % f.fx) * B + l % B, j = ((t0 / f.fx) % f.fy) * B + (l / B) % B,
+                  k = (t0 / (f.fx * f.fy)) * B + l / (B * B);
+        t = classify_cell<B>(col, f, col_scale, i, j, k, m, vx, vy, vz);
     }
     cf.vx[c] = vx; cf.vy[c] = vy; cf.vz[c] = vz;
     cx[c] = vx; cy[c] = vy; cz[c] = vz;
@@ -602,101 +610,6 @@ struct G2PParams {
     int tank;
     float band;
 };
-
-template <int B>
-__global__ void k_g2p(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz, VelPtr u,
-                      float* __restrict__ adj, int np, FrameK f,
-                      const float* __restrict__ cnt, const float* __restrict__ cx, const float* __restrict__ cy,
-                      const float* __restrict__ cz, const float* __restrict__ vx0, const float* __restrict__ vy0,
-                      const float* __restrict__ vz0, G2PParams gp, uint8_t* __restrict__ removed,
-                      int* __restrict__ nremoved) {
-    const int n = blockIdx.x * WG + threadIdx.x;
-    int rem = 0;
-    if (n < np) {
-        float x = px[n], y = py[n], z = pz[n];
-        float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
-        // lerp corner (SparseField.h:152-160)
-        const int i = rclamp((int)(x - 0.5f), 0, f.nx - 2);
-        const int j = rclamp((int)(y - 0.5f), 0, f.ny - 2);
-        const int k = rclamp((int)(z - 0.5f), 0, f.nz - 2);
-        const float fxw = rclamp(x - ((float)i + 0.5f), 0.f, 1.f);
-        const float fyw = rclamp(y - ((float)j + 0.5f), 0.f, 1.f);
-        const float fzw = rclamp(z - ((float)k + 0.5f), 0.f, 1.f);
-        int id[8];   // order: (dx,dy,dz) with index dx + 2*dy + 4*dz
-#pragma unroll
-        for (int q = 0; q < 8; ++q) id[q] = cell_index<B>(f, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2));
-        // adjacency = lerpMin of the count field (background 10), SparseField.h:176-196
-        {
-            float cv[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) cv[q] = id[q] >= 0 ? cnt[id[q]] : 10.f;
-            float a = rmin(cv[0], cv[1]);
-            a = rmin(a, rmin(cv[4], cv[5]));
-            a = rmin(a, rmin(cv[2], cv[3]));
-            a = rmin(a, rmin(cv[6], cv[7]));
-            adj[n] = a;
-        }
-        auto lerp = [&](const float* fld) -> float {
-            float v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = id[q] >= 0 ? fld[id[q]] : 0.f;
-            const float c00 = v[0] * (1.f - fxw) + v[1] * fxw;
-            const float c01 = v[4] * (1.f - fxw) + v[5] * fxw;
-            const float c10 = v[2] * (1.f - fxw) + v[3] * fxw;
-            const float c11 = v[6] * (1.f - fxw) + v[7] * fxw;
-            const float c0 = c00 * (1.f - fyw) + c10 * fyw;
-            const float c1 = c01 * (1.f - fyw) + c11 * fyw;
-            return c0 * (1.f - fzw) + c1 * fzw;
-        };
-        // FLIP/PIC blend (FLIPMethod.cu:94-100)
-        const float gvx = lerp(cx), gvy = lerp(cy), gvz = lerp(cz);
-        const float fx_ = gvx - lerp(vx0), fy_ = gvy - lerp(vy0), fz_ = gvz - lerp(vz0);
-        vx += fx_; vy += fy_; vz += fz_;
-        vx = vx * gp.flip + gvx * gp.pic;
-        vy = vy * gp.flip + gvy * gp.pic;
-        vz = vz * gp.flip + gvz * gp.pic;
-        // water-tank damping band (FLIPMethod.cu:102-114)
-        if (gp.tank && y <= gp.wall) {
-            const float lx = (float)f.nx + 0.5f, lz = (float)f.nz + 0.5f, band = gp.band;
-            float w = 1.f;
-            if (x < band) w = rmin(w, rclamp((x - 0.f) / band, 0.f, 1.f));
-            if (x > lx - band) w = rmin(w, rclamp((lx - x) / band, 0.f, 1.f));
-            if (z < band) w = rmin(w, rclamp((z - 0.f) / band, 0.f, 1.f));
-            if (z > lz - band) w = rmin(w, rclamp((lz - z) / band, 0.f, 1.f));
-            vx *= w; vy *= w; vz *= w;
-        }
-        // advection substeps (FLIPMethod.cu:121-159)
-        const float len = sqrtf(vx * vx + vy * vy + vz * vz);
-        const int iter = 1 + rmin((int)(len * gp.dt * 1.5f), rmax(gp.max_substeps, 1));
-        const float sub = gp.dt / (float)iter;
-        const float fnx = (float)f.nx, fny = (float)f.ny, fnz = (float)f.nz;
-        for (int s = 0; s < iter; ++s) {
-            x += vx * sub;
-            y += vy * sub;
-            z += vz * sub;
-            if (y <= gp.wall) {
-                if (x < 0.f) { x = 0.f; if (vx < 0.f) vx = 0.f; }
-                if (y < 0.f) { y = 0.f; if (vy < 0.f) vy = 0.f; }
-                if (z < 0.f) { z = 0.f; if (vz < 0.f) vz = 0.f; }
-                if (x > fnx) { x = fnx; if (vx > 0.f) vx = 0.f; }
-                if (y > fny) { y = fny; if (vy > 0.f) vy = 0.f; }
-                if (z > fnz) { z = fnz; if (vz > 0.f) vz = 0.f; }
-            }
-        }
-        // Grid::inside (FLIPMethod.cu:167)
-        if (x < 0.f || x > fnx || y < 0.f || y > fny || z < 0.f || z > fnz) rem = 1;
-        removed[n] = (uint8_t)rem;
-        // gravity (FLIPMethod.cu:170)
-        vx += gp.gx * gp.dt;
-        vy += gp.gy * gp.dt;
-        vz += gp.gz * gp.dt;
-        px[n] = x; py[n] = y; pz[n] = z;
-        u.st(0, n, vx); u.st(1, n, vy); u.st(2, n, vz);
-    }
-    // wave-aggregated removal count
-    const unsigned long long m = __ballot(rem);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(nremoved, (int)__popcll(m));
-}
 
 // Removal compaction, part 3: survivor src[S-1-k] moves into hole[k] (Array.h:995-1024 closed form:
 // the k-th removed slot below newN, ascending, receives the k-th survivor at or above newN,

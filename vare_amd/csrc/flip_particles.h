@@ -179,7 +179,7 @@ template <int B> struct P2GGather {
 };
 
 // P2G + count field + normalisation + classification of one active block, as a gather
-// (Rasterization.cu:121-190 and :198-230, FLIPMethod.cu:242-280 with no collision source).
+// (Rasterization.cu:121-190 and :198-230, FLIPMethod.cu:242-280 with curl disabled).
 // Each thread sums the 8 corner contributions of its base cell's particles (bin order, i.e.
 // ascending particle index) in registers.  The block's cells then receive the corners in 8 passes
 // (corner offset 000, 100, 010, ... in turn): within one pass distinct base cells hit distinct
@@ -191,7 +191,8 @@ template <int B>
 __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
         const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, VelPtr u,
         const int* __restrict__ perm, const int* __restrict__ cstart, FrameK f, CellFields cf,
-        float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz, uint8_t* __restrict__ type) {
+        float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz, uint8_t* __restrict__ type, ColK col,
+        float col_scale) {
     constexpr int B1 = B + 1, NBASE = B1 * B1 * B1, B3 = B * B * B;
     __shared__ float tm[B3], tx[B3], ty[B3], tz[B3], tc[B3];
     __shared__ int nsl[27];
@@ -259,6 +260,8 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
             vz = tz[q] / m;
             ty8 = kFluid;
         }
+        if (col.any())
+            ty8 = classify_cell<B>(col, f, col_scale, ox + q % B, oy + (q / B) % B, oz + q / (B * B), m, vx, vy, vz);
         cf.mass[c] = m > 0.f ? m : 0.f;
         cf.vx[c] = vx; cf.vy[c] = vy; cf.vz[c] = vz;
         cf.cnt[c] = tc[q];
@@ -268,10 +271,10 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
 }
 
 // G2P for one particle given a cell-value fetcher (FLIPMethod.cu:86-171, SparseField.h:150-196).
-template <class Fetch>
+template <int B, class Fetch>
 __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
                                             const VelPtr& u, float* __restrict__ adj, const FrameK& f, const G2PParams& gp,
-                                            uint8_t* __restrict__ removed, Fetch fetch) {
+                                            const ColK& col, uint8_t* __restrict__ removed, Fetch fetch) {
     float x = px[n], y = py[n], z = pz[n];
     float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
     const int i = rclamp((int)(x - 0.5f), 0, f.nx - 2);
@@ -314,10 +317,20 @@ __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float
         if (z > lz - band) w = rmin(w, rclamp((lz - z) / band, 0.f, 1.f));
         vx *= w; vy *= w; vz *= w;
     }
+    const bool cols = col.any();
+    if (cols && col_phi<B>(col, f, x, y, z) < 0.f) {          // collision push (FLIPMethod.cu:116-119)
+        float cv[3];
+        col_vec<B>(col, f, 0, x, y, z, cv);
+        x += cv[0] * gp.dt;
+        y += cv[1] * gp.dt;
+        z += cv[2] * gp.dt;
+    }
+    // advection substeps (FLIPMethod.cu:121-159)
     const float len = sqrtf(vx * vx + vy * vy + vz * vz);
     const int iter = 1 + rmin((int)(len * gp.dt * 1.5f), rmax(gp.max_substeps, 1));
     const float sub = gp.dt / (float)iter;
     const float fnx = (float)f.nx, fny = (float)f.ny, fnz = (float)f.nz;
+    bool hit = false;
     for (int s = 0; s < iter; ++s) {
         x += vx * sub;
         y += vy * sub;
@@ -330,8 +343,25 @@ __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float
             if (y > fny) { y = fny; if (vy > 0.f) vy = 0.f; }
             if (z > fnz) { z = fnz; if (vz > 0.f) vz = 0.f; }
         }
+        if (cols) {                                             // projection out of the source
+            const float phi = col_phi<B>(col, f, x, y, z);
+            if (phi < 0.f) {
+                float nr[3];
+                col_vec<B>(col, f, 1, x, y, z, nr);
+                normalize3(nr);
+                x -= nr[0] * phi;
+                y -= nr[1] * phi;
+                z -= nr[2] * phi;
+                hit = true;
+            }
+        }
     }
-    const int rem = (x < 0.f || x > fnx || y < 0.f || y > fny || z < 0.f || z > fnz) ? 1 : 0;
+    int rem = 0;
+    if (hit) {                                                  // FLIPMethod.cu:161-164
+        const float phi = col_phi<B>(col, f, x, y, z);
+        if (phi > 3.f || phi < 0.f) rem = 1;
+    }
+    if (x < 0.f || x > fnx || y < 0.f || y > fny || z < 0.f || z > fnz) rem = 1;   // Grid::inside
     removed[n] = (uint8_t)rem;
     vx += gp.gx * gp.dt;
     vy += gp.gy * gp.dt;
@@ -359,7 +389,8 @@ template <int B>
 __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
                                                 VelPtr u, float* __restrict__ adj, const int* __restrict__ perm,
                                                 const int* __restrict__ cstart, int nb, FrameK f, G2PFields gf,
-                                                G2PParams gp, uint8_t* __restrict__ removed, int* __restrict__ nremoved) {
+                                                G2PParams gp, ColK col, uint8_t* __restrict__ removed,
+                                                int* __restrict__ nremoved) {
     constexpr int T = B + 2, T3 = T * T * T, B3 = B * B * B;
     __shared__ float tile[7][T3];
     __shared__ int nsl[27];
@@ -376,7 +407,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* 
             for (int fi = 0; fi < 7; ++fi) v[fi][q] = c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
         };
         for (int q = p0 + threadIdx.x; q < p1; q += WG)
-            rem_count += g2p_particle(perm[q], px, py, pz, u, adj, f, gp, removed, fetch);
+            rem_count += g2p_particle<B>(perm[q], px, py, pz, u, adj, f, gp, col, removed, fetch);
     } else {
         load_nbhd(f, s, nsl);
         __syncthreads();
@@ -401,11 +432,29 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* 
             }
         };
         for (int q = p0 + threadIdx.x; q < p1; q += WG)
-            rem_count += g2p_particle(perm[q], px, py, pz, u, adj, f, gp, removed, fetch);
+            rem_count += g2p_particle<B>(perm[q], px, py, pz, u, adj, f, gp, col, removed, fetch);
     }
     // wave-aggregated removal count
     const int w = wave_sum_i(rem_count);
     if ((threadIdx.x & 63) == 0 && w) atomicAdd(nremoved, w);
+}
+
+// G2P with one thread per particle in storage order and every field read through the block table
+// (particle_path 1).
+template <int B>
+__global__ void k_g2p(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz, VelPtr u,
+                      float* __restrict__ adj, int np, FrameK f, G2PFields gf, G2PParams gp, ColK col,
+                      uint8_t* __restrict__ removed, int* __restrict__ nremoved) {
+    const int n = blockIdx.x * WG + threadIdx.x;
+    const float* src[7] = {gf.cnt, gf.cx, gf.cy, gf.cz, gf.vx0, gf.vy0, gf.vz0};
+    auto fetch = [&](int i, int j, int k, int q, float (&v)[7][8]) {
+        const int c = cell_index<B>(f, i, j, k);
+#pragma unroll
+        for (int fi = 0; fi < 7; ++fi) v[fi][q] = c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
+    };
+    const int rem = n < np ? g2p_particle<B>(n, px, py, pz, u, adj, f, gp, col, removed, fetch) : 0;
+    const unsigned long long m = __ballot(rem);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(nremoved, (int)__popcll(m));
 }
 
 }  // namespace vare
