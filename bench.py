@@ -9,9 +9,18 @@ in HBM.
 
 Other configurations (--config):
   1  the same dam break at 64^3, water x<0.4, y<0.6, 100 steps; the CPU baseline runs all 100 steps;
+  3  mixed precision at 512^3: a pool of depth 0.3 plus a sphere of water (radius 0.2, centre
+     (0.5, 0.7, 0.5)), 8 particles per cell, velocities stored in FP16, FP32 PCG with FP64 dot products
+     and FP64 iterative refinement to 1e-8 relative, 10 steps;
   4  served batch: independent tanks (edges log-uniform 32..128, 4..8 particles per cell, water
      fractions 0.2..0.7, 10 steps each) arriving as a Poisson process, continuously batched by the
-     library's server, compared with the same requests solved one at a time ("plain GPU").
+     library's server, compared with the same requests solved one at a time ("plain GPU");
+  5  sparse wave on an obstacle: 1024x512x1024 cells (dx = 1/512, world 2x1x2), a wall of water
+     x<0.25, y<0.3, full z moving +x at 4 m/s, a static box obstacle x in [0.5,0.7], y in [0,0.5],
+     z in [0.6,1.4] given as a signed distance field, FP16 velocities, FP64 refinement to 1e-6, 100
+     steps at dt = 1/1000 (4 m/s is 8.5 cells per step at 1/240; 1/1000 keeps it near 2).
+For configs 3 and 5 the CPU baseline times one step of the same scene at a quarter of the
+resolution per axis (a bounded sample; the rate is per particle-step).
 
 Prints one JSON line (driver contract) with `roofline` and `cpu_baseline` objects.
 """
@@ -34,15 +43,16 @@ DT = 1.0 / 240.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 4])
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--block", type=int, default=8)
     ap.add_argument("--ppc", type=int, default=8)
     ap.add_argument("--solver", choices=["relative", "reference"], default="relative")
     ap.add_argument("--precond", choices=["none", "jacobi", "mg"], default="mg")
-    ap.add_argument("--tol", type=float, default=1e-5,
-                    help="target |b-Ax|/|b|, met in FP64 through iterative refinement (see --no-refine)")
+    ap.add_argument("--tol", type=float, default=None,
+                    help="target |b-Ax|/|b|, met in FP64 through iterative refinement (see --no-refine); "
+                         "default 1e-5 (1e-8 for config 3, 1e-6 for config 5)")
     ap.add_argument("--inner-tol", type=float, default=1e-3,
                     help="relative tolerance of each FP32 PCG solve inside the refinement")
     ap.add_argument("--no-refine", action="store_true",
@@ -57,6 +67,7 @@ def parse():
 
 
 def solver_kw(a, L, n):
+    """Step parameters for a scene with voxel size 1/n (world units)."""
     refine = a.solver == "relative" and not a.no_refine
     return dict(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n,
                 solver_mode=L.SOLVER_RELATIVE if a.solver == "relative" else L.SOLVER_REFERENCE,
@@ -66,7 +77,7 @@ def solver_kw(a, L, n):
                 mg_sweeps=a.mg_sweeps)
 
 
-def solver_desc(a):
+def solver_desc(a, cfg=2):
     if a.solver != "relative":
         return "reference CG rule"
     if a.no_refine:
@@ -75,38 +86,86 @@ def solver_desc(a):
             % (a.precond, a.inner_tol, a.tol))
 
 
-def run_dambreak(a, L, n, wx, wy, steps):
-    kw = solver_kw(a, L, n)
+def scene(cfg, scale=1):
+    """Scene of a benchmark configuration; `scale` divides the resolution per axis (CPU sample)."""
+    if cfg in (1, 2):
+        n, wx, wy = (64, 0.4, 0.6) if cfg == 1 else (256, 0.5, 0.5)
+        n //= scale
+        return dict(dims=(n, n, n), world=(1.0, 1.0, 1.0), res=n, kind="box", water=(wx, wy, 1.0), vel=(0, 0, 0),
+                    obstacle=None, fp16=False, steps=100 if cfg == 1 else 200, dt=DT,
+                    desc="dam break %d^3, water x<%.1f y<%.1f, closed walls" % (n, wx, wy))
+    if cfg == 3:
+        n = 512 // scale
+        return dict(dims=(n, n, n), world=(1.0, 1.0, 1.0), res=n, kind="pool_sphere",
+                    pool=(0.3, (0.5, 0.7, 0.5), 0.2), vel=(0, 0, 0), obstacle=None, fp16=True, steps=10, dt=DT,
+                    desc="pool depth 0.3 + sphere r=0.2 at (0.5,0.7,0.5), %d^3, closed walls" % n)
+    if cfg == 5:
+        r = 512 // scale
+        return dict(dims=(2 * r, r, 2 * r), world=(2.0, 1.0, 2.0), res=r, kind="box", water=(0.25, 0.3, 2.0),
+                    vel=(4.0, 0.0, 0.0), obstacle=((0.5, 0.0, 0.6), (0.7, 0.5, 1.4)), fp16=True, steps=100,
+                    dt=1.0 / 1000.0,
+                    desc="wave x<0.25 y<0.3 at +4 m/s on a static box [0.5,0.7]x[0,0.5]x[0.6,1.4], %dx%dx%d (dx=1/%d)"
+                         % (2 * r, r, 2 * r, r))
+    raise ValueError(cfg)
+
+
+def seed_scene(L, sc, ppc):
+    """Particles (voxel space) and their initial velocities (voxel units per second)."""
+    nx, ny, nz = sc["dims"]
+    r = sc["res"]
+    if sc["kind"] == "box":
+        wx, wy, wz = sc["water"]
+        pos = L.seed_box(nx, ny, nz, wx * r, wy * r, wz * r, ppc, 1)
+    else:
+        depth, c, rad = sc["pool"]
+        pos = L.seed_pool_sphere(nx, ny, nz, depth * r, [x * r for x in c], rad * r, ppc, 1)
+    vel = np.empty_like(pos)
+    vel[:] = np.array(sc["vel"], np.float32) * np.float32(r)
+    return pos, vel
+
+
+def scene_handle(L, sc, kw, B, npart):
+    nx, ny, nz = sc["dims"]
+    kw = dict(kw, velocity_fp16=1 if sc["fp16"] else 0)
+    h = L.FlipHandle(L.grid_desc(nx, ny, nz, (0.0, 0.0, 0.0), sc["world"], block_size=B), L.default_params(**kw),
+                     particle_capacity=npart, block_capacity=(nx // B) * (ny // B) * (nz // B))
+    if sc["obstacle"] is not None:
+        h.set_collision_box(sc["obstacle"][0], sc["obstacle"][1], (0.0, 0.0, 0.0), 3.0)
+    return h
+
+
+def run_scene(a, L, cfg, steps):
+    sc = scene(cfg)
+    kw = solver_kw(a, L, sc["res"])
     t0 = time.time()
-    pos = L.seed_box(n, n, n, wx * n, wy * n, float(n), a.ppc, 1)
-    vel = np.zeros_like(pos)
+    pos, vel = seed_scene(L, sc, a.ppc)
     npart = pos.shape[0]
     print("bench: seeded %d particles in %.1fs on %s" % (npart, time.time() - t0, L.device_name()),
           file=sys.stderr, flush=True)
     B = a.block
-    nblk = (n // B) ** 3
-    h = L.FlipHandle(L.grid_desc(n, n, n, block_size=B), L.default_params(**kw),
-                     particle_capacity=npart, block_capacity=nblk)
+    h = scene_handle(L, sc, kw, B, npart)
 
     # warm-up: run the first steps of the simulation, then restart from the initial state
     h.set_particles(pos, vel)
     for _ in range(a.warmup):
-        rc = h.step_rc(DT)
+        rc = h.step_rc(sc["dt"])
         if rc not in (L.VARE_OK, L.VARE_ENOCONV):
             L.check(rc)
     h.set_particles(pos, vel)
+    del pos, vel
     if not a.no_kernel_timing:
         h.set_kernel_timing(["spmv"])
         h.kernel_timing(reset=True)
     L.device_synchronize()
 
-    step_ms, iters, relres, truerel, refines = [], [], [], [], []
+    step_ms, iters, relres, truerel, refines, nps = [], [], [], [], [], []
     nonconv = 0
     stats = None
     t_start = time.perf_counter()
     for s in range(steps):
         ts = time.perf_counter()
-        rc = h.step_rc(DT)
+        nps.append(h.num_particles())
+        rc = h.step_rc(sc["dt"])
         if rc == L.VARE_ENOCONV:
             nonconv += 1
         elif rc != L.VARE_OK:
@@ -118,7 +177,7 @@ def run_dambreak(a, L, n, wx, wy, steps):
             relres.append(stats["residual_norm"] / stats["b_norm"])
             truerel.append(stats["true_rel_residual"])
             refines.append(stats["refine_steps"])
-        if (s + 1) % 50 == 0:
+        if (s + 1) % 10 == 0:
             print("bench: step %d/%d %.2f ms cg=%d" % (s + 1, steps, step_ms[-1], iters[-1]),
                   file=sys.stderr, flush=True)
     L.device_synchronize()
@@ -127,10 +186,9 @@ def run_dambreak(a, L, n, wx, wy, steps):
 
     roofline = None
     if kt is not None and kt["spmv"][1] > 0:
-        nc = stats["num_cells"]
         # algorithmic bytes of one A*p launch: read p (4 B) + stencil descriptor (2 B) + write Ap (4 B)
-        # per active cell; neighbour values of p are re-reads of the same vector.
-        bytes_per_launch = 10.0 * nc
+        # per active cell of the last step; neighbour values of p are re-reads of the same vector.
+        bytes_per_launch = 10.0 * stats["num_cells"]
         avg_ms = kt["spmv"][0] / kt["spmv"][1]
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"kernel": "k_spmv_blk", "bound": "hbm", "achieved": round(achieved, 1),
@@ -138,15 +196,17 @@ def run_dambreak(a, L, n, wx, wy, steps):
                     "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 2), "launches": kt["spmv"][1],
                     "bytes_per_launch": bytes_per_launch}
     sm = np.array(step_ms)
+    nx, ny, nz = sc["dims"]
     res = {
-        "value": round(npart * steps / total_s, 1),
+        "value": round(float(np.sum(nps)) / total_s, 1),
         "ms_per_step": round(total_s * 1e3 / steps, 3),
         "p50_ms": round(float(np.percentile(sm, 50)), 3),
         "p99_ms": round(float(np.percentile(sm, 99)), 3),
-        "config": {"workload": "dam break %d^3, water x<%.1f y<%.1f, %d ppc, closed walls, dt=1/240, FLIP 0.95, "
-                               "%s, %d^3 blocks, %d steps from rest"
-                               % (n, wx, wy, a.ppc, solver_desc(a), B, steps),
-                   "particles": npart, "cells": n ** 3, "active_blocks_last": stats["num_blocks"],
+        "config": {"workload": "%s, %d ppc, dt=1/%d, FLIP 0.95, %s%s, %d^3 blocks, %d steps from the initial state"
+                               % (sc["desc"], a.ppc, round(1.0 / sc["dt"]), solver_desc(a, cfg),
+                                  ", FP16 particle velocities" if sc["fp16"] else "", B, steps),
+                   "particles": npart, "particles_last": int(nps[-1]), "cells": nx * ny * nz,
+                   "active_blocks_last": stats["num_blocks"], "frame_blocks": (nx // B) * (ny // B) * (nz // B),
                    "fluid_cells_last": stats["num_fluid"]},
         "cg_iterations_mean": round(float(np.mean(iters)), 1),
         "cg_iterations_max": int(np.max(iters)),
@@ -159,33 +219,47 @@ def run_dambreak(a, L, n, wx, wy, steps):
     return h, kw, res
 
 
-def cpu_baseline(h, n, B, kw, dt, steps, pos0=None):
-    """Time the oracle (the FP32 restatement of the reference step, one host thread) on the same
-    workload with the reference's own solver (unpreconditioned CG, |r| < 1e-3 or 202 products):
-    `steps` step(s) from the GPU run's final particle state, or from `pos0` (at rest)."""
+def cpu_baseline(a, L, cfg, h, kw):
+    """Time the oracle (the FP32 restatement of the reference step, one host thread) with the
+    reference's own solver (unpreconditioned CG, |r| < 1e-3 or 202 products) on a bounded sample of
+    the workload: config 1 all its steps from rest; config 2 one step from the GPU run's final
+    particle state; configs 3 and 5 two steps of the same scene at a quarter of the resolution, after
+    four untimed ones."""
     from oracle import oracle as O
-    if pos0 is None:
+    if cfg == 1:
+        sc = scene(1)
+        p, v = seed_scene(L, sc, a.ppc)
+        steps, start = a.cpu_steps or sc["steps"], "rest"
+    elif cfg == 2:
+        sc = scene(2)
         d = h.get_particles()
         p, v = d["pos"], d["vel"]
-        start = "the GPU run's final state"
+        steps, start = a.cpu_steps or 1, "the GPU run's final state"
     else:
-        p, v = pos0, np.zeros_like(pos0)
-        start = "rest"
-    ref_kw = {k: kw[k] for k in ("ext_force", "wall_level", "voxel_size")}
-    o = O.Oracle((n, n, n), B=B, prm=O.params(**ref_kw))
+        sc = scene(cfg, scale=4)
+        p, v = seed_scene(L, sc, a.ppc)
+        steps, start = a.cpu_steps or 2, "the state after 4 untimed steps"
+    ref_kw = dict(ext_force=kw["ext_force"], wall_level=kw["wall_level"], voxel_size=1.0 / sc["res"])
+    o = O.Oracle(sc["dims"], (0.0, 0.0, 0.0), sc["world"], B=a.block, prm=O.params(**ref_kw))
+    if sc["obstacle"] is not None:
+        o.set_collision_box(sc["obstacle"][0], sc["obstacle"][1], (0.0, 0.0, 0.0), 3.0)
     o.set_particles(p, v)
+    if cfg in (3, 5):
+        for _ in range(4):          # leave the trivial first steps from rest out of the sample
+            o.step(sc["dt"])
     npart = p.shape[0]
-    its = []
+    its, counts = [], []
     t = time.perf_counter()
     for _ in range(steps):
-        its.append(o.step(dt))
+        counts.append(o.num_particles())
+        its.append(o.step(sc["dt"]))
     el = time.perf_counter() - t
     nfinal = o.num_particles()
     o.close()
-    return {"value": round(npart * steps / el, 1), "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d step(s) of the %d^3 dam break (%d particles) from %s, reference CG rule "
-                      "(mean %.0f iterations), %.1f s, %d particles at the end"
-                      % (steps, n, npart, start, float(np.mean(its)), el, nfinal)}
+    return {"value": round(float(np.sum(counts)) / el, 1), "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d step(s) of %s (%d particles, %d ppc) from %s, reference CG rule (mean %.0f iterations), "
+                      "%.1f s, %d particles at the end" % (steps, sc["desc"], npart, a.ppc, start,
+                                                           float(np.mean(its)), el, nfinal)}
 
 
 def run_served(a, L):
@@ -234,7 +308,7 @@ def run_served(a, L):
         "config": {"workload": "served batch: %d tank solves, edges log-uniform 32..128 (multiple of 8), 4..8 ppc, "
                                "water fractions U(0.2,0.7), 10 steps each at dt=1/240, %s, Poisson "
                                "arrivals (mean gap %.1f ms = 0.5 x isolated solve), %d slots, 200 GB budget"
-                               % (nreq, solver_desc(a), 1e3 * mean_gap, a.slots),
+                               % (nreq, solver_desc(a, 4), 1e3 * mean_gap, a.slots),
                    "particles_total": total_particles, "max_resident": served["max_resident"],
                    "peak_reserved_gb": round(served["peak_bytes"] / 2 ** 30, 2), "wall_s": round(served["wall_s"], 3)},
         "plain_gpu": {"value": round(plain["particle_steps_per_s"], 1), "p50_ms": round(plain["step_p50_ms"], 3),
@@ -253,20 +327,15 @@ def main():
         sys.exit(2)
     from vare_amd import _lib as L
 
+    if a.tol is None:
+        a.tol = {3: 1e-8, 5: 1e-6}.get(a.config, 1e-5)
     if a.config == 4:
         res = run_served(a, L)
         steps, cpu = res.pop("steps"), None
     else:
-        n, wx, wy, dsteps = (64, 0.4, 0.6, 100) if a.config == 1 else (256, 0.5, 0.5, 200)
-        steps = a.steps if a.steps is not None else dsteps
-        h, kw, res = run_dambreak(a, L, n, wx, wy, steps)
-        cpu = None
-        if a.cpu_baseline == "on":
-            if a.config == 1:
-                pos0 = L.seed_box(n, n, n, wx * n, wy * n, float(n), a.ppc, 1)
-                cpu = cpu_baseline(h, n, a.block, kw, DT, a.cpu_steps or steps, pos0=pos0)
-            else:
-                cpu = cpu_baseline(h, n, a.block, kw, DT, a.cpu_steps or 1)
+        steps = a.steps if a.steps is not None else scene(a.config)["steps"]
+        h, kw, res = run_scene(a, L, a.config, steps)
+        cpu = cpu_baseline(a, L, a.config, h, kw) if a.cpu_baseline == "on" else None
     out = {
         "metric": METRIC,
         "value": res.pop("value"),
