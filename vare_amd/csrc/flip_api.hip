@@ -100,7 +100,7 @@ struct vare_flip {
     int* cstart = nullptr;        // bin starts, [bcap*B^3+2]
     size_t last_np_before_remove = 0;
     int nremoved = 0;
-    int last_iters = 0;           // solver iterations of the previous step
+    int last_iters[20] = {};      // iterations of each solve (first, then each refinement) of the previous step
 
     // frame
     int* flag = nullptr;
@@ -109,6 +109,9 @@ struct vare_flip {
     int* nbr = nullptr;       // capacity bcap*6
     size_t bcap = 0;
     int nb = 0;
+    int* bfluid = nullptr;        // [bcap] 1 when the block holds a fluid cell (set by k_divergence)
+    int* blist = nullptr;         // [bcap] slots of those blocks, ascending: the solver's blocks
+    int ns = 0;                   // number of listed blocks
 
     // cells, capacity bcap*B^3
     float *mass = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *cnt = nullptr;
@@ -131,7 +134,8 @@ struct vare_flip {
     int* tiles = nullptr;
     size_t tiles_cap = 0;
     int* dint = nullptr;          // [0] scan total, [1] removed count, [2] second scan total,
-                                  // [3] bin total, [4] bins left unsorted, [5] long-bin count
+                                  // [3] bin total, [4] bins left unsorted, [5] long-bin count,
+                                  // [6] solver block count
     CgState* st = nullptr;
     CgState* st_host = nullptr;   // pinned mirror
     int* dint_host = nullptr;     // pinned mirror of dint
@@ -167,6 +171,8 @@ struct vare_flip {
         return r;
     }
     FrameK frame() const { return FrameK{fx, fy, fz, nx, ny, nz, slot, coord, nbr}; }
+    CellList clist() const { return CellList{blist, B == 8 ? 9 : 6}; }
+    int nvirt() const { return ns * B * B * B; }          // cells of the listed blocks
     ColK colk() const {
         return ColK{col_slot, col_phi, {col_v[0], col_v[1], col_v[2]}, {col_n[0], col_n[1], col_n[2]}, col_bg,
                     col_active};
@@ -201,6 +207,7 @@ void free_cells(vare_flip* h) {
     free_mg(h);
     dfree(h->nbr);
     dfree(h->ccount); dfree(h->cstart);
+    dfree(h->bfluid); dfree(h->blist);
     dfree(h->mass); dfree(h->vx); dfree(h->vy); dfree(h->vz); dfree(h->cnt);
     dfree(h->cx); dfree(h->cy); dfree(h->cz);
     dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b); dfree(h->z); dfree(h->x64);
@@ -219,6 +226,7 @@ int ensure_blocks(vare_flip* h, size_t nb) {
     const size_t npart = std::max((size_t)kReduceGrid, cap) + 16;
     CHK(dalloc(&h->nbr, cap * 6));
     CHK(dalloc(&h->ccount, nc + 2)); CHK(dalloc(&h->cstart, nc + 2));
+    CHK(dalloc(&h->bfluid, cap + 1)); CHK(dalloc(&h->blist, cap + 1));
     CHK(dalloc(&h->mass, nc)); CHK(dalloc(&h->vx, nc)); CHK(dalloc(&h->vy, nc)); CHK(dalloc(&h->vz, nc));
     CHK(dalloc(&h->cnt, nc));
     CHK(dalloc(&h->cx, nc)); CHK(dalloc(&h->cy, nc)); CHK(dalloc(&h->cz, nc));
@@ -438,6 +446,17 @@ CgVecs vecs(vare_flip* h) { return CgVecs{h->x, h->r, h->p, h->ap, h->b, h->z, h
 int mg_setup(vare_flip* h);
 void enqueue_vcycle(vare_flip* h);
 
+// z = M^-1 r (multigrid: already in z), p = z and r.z over the solver's blocks.
+void enqueue_precond_init(vare_flip* h) {
+    const int nv = h->nvirt();
+    const unsigned gs = rgrid((size_t)nv);
+    if (nv)
+        hipLaunchKernelGGL(k_cg_precond_init, dim3(gs), dim3(WG), 0, h->stream, nv, h->clist(), vecs(h), h->partials,
+                           (const CgState*)h->st);
+    hipLaunchKernelGGL(k_fin_precond_init, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+                       nv ? (int)gs : 0);
+}
+
 template <int B> int stage_system(vare_flip* h) {
     const size_t nc = h->ncells();
     const int mode = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? 1 : 0;
@@ -446,21 +465,26 @@ template <int B> int stage_system(vare_flip* h) {
     hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(64), 0, h->stream, h->st, mode, maxc, h->prm.cg_abs_tol,
                        h->prm.cg_rel_tol, precond);
     const unsigned g = rgrid(nc);
+    CHK(fill_u32(h, h->bfluid, 0, (size_t)h->nb));
     tbegin(h, VARE_KERNEL_DIV);
     hipLaunchKernelGGL(k_divergence<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const uint8_t*)h->type,
                        (const float*)h->cx, (const float*)h->cy, (const float*)h->cz, h->vox_wall, vecs(h), h->partials,
-                       h->partials2);
+                       h->partials2, h->bfluid);
     tend(h, VARE_KERNEL_DIV);
     hipLaunchKernelGGL(k_fin_divergence, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
                        (const double*)h->partials2, (int)g);
+    // the solver's blocks: those holding a fluid cell
+    OpFluidBlocks ofb{h->bfluid, h->blist};
+    CHK(run_scan(h, ofb, h->nb, 6));
+    HIPCHK(hipMemcpyAsync(h->dint_host + 6, h->dint + 6, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->ns = h->dint_host[6];
     if (mode == 1) {
         if (precond == VARE_PRECOND_MG) {
             CHK(mg_setup(h));
             enqueue_vcycle(h);
         }
-        hipLaunchKernelGGL(k_cg_precond_init, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials,
-                           (const CgState*)h->st);
-        hipLaunchKernelGGL(k_fin_precond_init, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+        enqueue_precond_init(h);
     }
     HIPCHK(hipGetLastError());
     return VARE_OK;
@@ -535,7 +559,8 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
     const int nu = h->prm.mg_sweeps;
     const int* done = done_ptr(h);
     const int m = (int)h->mg.size() - 1;
-    const unsigned nb = (unsigned)h->nb;
+    const unsigned nb = (unsigned)h->ns;                     // level 0 runs over the solver's blocks
+    const int* bl = h->blist;
     constexpr unsigned B3 = (unsigned)(B * B * B);
     for (int l = 0; l < m; ++l) {
         const MgLevelK& L = h->mg[l];
@@ -543,27 +568,27 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
         if (l == 0) {           // fine level: block-aligned kernels with the block staged in LDS
             if (!nb) continue;
             if (nu == 1) {
-                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, L, C, F, w, 1, h->mg_zt[0],
-                                   done);
+                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, C, F, w, 1,
+                                   h->mg_zt[0], done);
             } else {
-                hipLaunchKernelGGL(k_mg0_smooth_first2<B>, dim3(nb), dim3(B3), 0, h->stream, L, F, w, h->mg_z[0],
-                                   done);
-                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, L, C, F, w, 0, h->mg_z[0],
-                                   done);
+                hipLaunchKernelGGL(k_mg0_smooth_first2<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, F, w,
+                                   h->mg_z[0], done);
+                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, C, F, w, 0,
+                                   h->mg_z[0], done);
             }
             continue;
         }
         if (nu == 1) {
             if (C.ncell)
-                hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(C.ncell)), dim3(WG), 0, h->stream, L, C, F, w, 1,
-                                   h->mg_zt[l], done);
+                hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks((size_t)C.ncell * 8)), dim3(WG), 0, h->stream, L, C, F,
+                                   w, 1, h->mg_zt[l], done);
         } else {
             if (L.ncell)
                 hipLaunchKernelGGL(k_mg_smooth_first2, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F, w,
                                    h->mg_z[l], done);
             if (C.ncell)
-                hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks(C.ncell)), dim3(WG), 0, h->stream, L, C, F, w, 0,
-                                   h->mg_z[l], done);
+                hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks((size_t)C.ncell * 8)), dim3(WG), 0, h->stream, L, C, F,
+                                   w, 0, h->mg_z[l], done);
         }
     }
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kMgCoarseMax), 0, h->stream, h->mg[m], w, h->prm.mg_coarse_sweeps,
@@ -573,13 +598,14 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
         const MgLevelK& C = h->mg[l + 1];
         if (!L.ncell) continue;
         if (l == 0) {
+            if (!nb) continue;
             if (nu == 1) {
-                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, L, C, F, w,
+                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, C, F, w,
                                    (const float*)h->mg_zt[0], (const float*)h->mg_z[1], h->mg_z[0], done);
             } else {
-                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, L, C, F, w,
+                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, C, F, w,
                                    (const float*)h->mg_z[0], (const float*)h->mg_z[1], h->mg_zt[0], done);
-                hipLaunchKernelGGL(k_mg0_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, L, F, w,
+                hipLaunchKernelGGL(k_mg0_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, F, w,
                                    (const float*)h->mg_zt[0], h->mg_z[0], done);
             }
             continue;
@@ -601,49 +627,55 @@ void enqueue_vcycle(vare_flip* h) {
     else enqueue_vcycle_t<8>(h);
 }
 
-// One CG/PCG iteration (LinearSystemSolver.cpp:29-46); every kernel returns at once when the
-// device-side `done` flag is set, so the host can enqueue iterations ahead without syncing.
-template <int B> void enqueue_iteration(vare_flip* h, size_t nc, unsigned g) {
+// One CG/PCG iteration (LinearSystemSolver.cpp:29-46) over the solver's blocks; every kernel
+// returns at once when the device-side `done` flag is set, so the host can enqueue iterations ahead
+// without syncing.
+template <int B> void enqueue_iteration(vare_flip* h) {
+    const int nv = h->nvirt();
+    if (!nv) return;
+    const unsigned gs = rgrid((size_t)nv);
+    const CellList cl = h->clist();
     tbegin(h, VARE_KERNEL_SPMV);
-    if (h->nb)
-        hipLaunchKernelGGL(k_spmv_blk<B>, dim3(h->nb), dim3(B * B * B), 0, h->stream, h->frame(), vecs(h), h->partials,
-                           (const CgState*)h->st);
+    hipLaunchKernelGGL(k_spmv_blk<B>, dim3(h->ns), dim3(B * B * B), 0, h->stream, (const int*)h->blist, h->frame(),
+                       vecs(h), h->partials, (const CgState*)h->st);
     tend(h, VARE_KERNEL_SPMV);
-    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, h->nb);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, h->ns);
     tbegin(h, VARE_KERNEL_UPDATE);
-    hipLaunchKernelGGL(k_cg_update, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials, h->partials2,
+    hipLaunchKernelGGL(k_cg_update, dim3(gs), dim3(WG), 0, h->stream, nv, cl, vecs(h), h->partials, h->partials2,
                        (const CgState*)h->st);
     tend(h, VARE_KERNEL_UPDATE);
     hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
-                       (const double*)h->partials2, (int)g);
+                       (const double*)h->partials2, (int)gs);
     if (h->prm.solver_mode == VARE_SOLVER_RELATIVE && h->prm.preconditioner == VARE_PRECOND_MG) {
         enqueue_vcycle(h);
-        hipLaunchKernelGGL(k_cg_rz, dim3(g), dim3(WG), 0, h->stream, (int)nc, (const float*)h->r, (const float*)h->z,
+        hipLaunchKernelGGL(k_cg_rz, dim3(gs), dim3(WG), 0, h->stream, nv, cl, (const float*)h->r, (const float*)h->z,
                            h->partials, (const CgState*)h->st);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)gs);
     }
     tbegin(h, VARE_KERNEL_PUPDATE);
-    hipLaunchKernelGGL(k_cg_pupdate, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), (const CgState*)h->st);
+    hipLaunchKernelGGL(k_cg_pupdate, dim3(gs), dim3(WG), 0, h->stream, nv, cl, vecs(h), (const CgState*)h->st);
     tend(h, VARE_KERNEL_PUPDATE);
 }
 
-// Run CG/PCG iterations until the device-side `done` flag is set.  The iteration count changes
-// slowly from step to step: the previous count (+1) is enqueued before the first host check, then
-// small chunks; iterations enqueued past convergence return at once.
-template <int B> int run_solver(vare_flip* h, size_t nc, unsigned g, int cap, int& total_iters) {
+// Run CG/PCG iterations until the device-side `done` flag is set.  The iteration count of a solve
+// changes slowly from step to step: the count of the same solve (the k-th of the step) in the
+// previous step is enqueued before the first host check, then small chunks; iterations enqueued
+// past convergence return at once.
+template <int B> int run_solver(vare_flip* h, int cap, int k, int& total_iters) {
+    int& last = h->last_iters[std::min(k, 19)];
     int launched = 0;
-    int chunk = h->last_iters > 0 ? h->last_iters + 1 : 8;
+    int chunk = last > 0 ? last : 8;
     for (;;) {
         HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         if (h->st_host->done) break;
         if (launched > cap + 1) break;                       // defensive: the device flag ends first
-        for (int q = 0; q < chunk; ++q) enqueue_iteration<B>(h, nc, g);
+        for (int q = 0; q < chunk; ++q) enqueue_iteration<B>(h);
         HIPCHK(hipGetLastError());
         launched += chunk;
-        chunk = h->last_iters > 0 ? 4 : std::min(chunk * 2, 64);
+        chunk = last > 0 ? std::max(2, last / 8) : std::min(chunk * 2, 64);
     }
-    h->last_iters = h->st_host->count;
+    last = h->st_host->count;
     total_iters += h->st_host->count;
     return VARE_OK;
 }
@@ -657,7 +689,7 @@ template <int B> int stage_solve(vare_flip* h) {
     const bool refine = mode == 1 && h->prm.refine_tol > 0.f;
     int total_iters = 0, outer = 0;
     if (refine) CHK(fill_u32(h, h->x64, 0, 2 * nc));
-    CHK(run_solver<B>(h, nc, g, cap, total_iters));
+    CHK(run_solver<B>(h, cap, 0, total_iters));
     if (refine) {
         // FP64 iterative refinement: x64 += d, r = b - A x64 in FP64, solve A d = r in FP32
         for (;;) {
@@ -668,17 +700,14 @@ template <int B> int stage_solve(vare_flip* h) {
                                (const float*)h->b, (const double*)h->x64, (const uint16_t*)h->info, h->r, h->x, h->p,
                                h->partials);
             hipLaunchKernelGGL(k_fin_ir, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g,
-                               h->prm.refine_tol);
+                               h->prm.refine_tol, h->prm.cg_rel_tol);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
             if (h->st_host->ir_done || outer >= h->prm.refine_max) break;
             if (h->prm.preconditioner == VARE_PRECOND_MG) enqueue_vcycle(h);
-            hipLaunchKernelGGL(k_cg_precond_init, dim3(g), dim3(WG), 0, h->stream, (int)nc, vecs(h), h->partials,
-                               (const CgState*)h->st);
-            hipLaunchKernelGGL(k_fin_precond_init, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
-                               (int)g);
-            CHK(run_solver<B>(h, nc, g, cap, total_iters));
+            enqueue_precond_init(h);
+            CHK(run_solver<B>(h, cap, outer, total_iters));
         }
     }
     // FP64 recompute of the final residual from the pressure the gradient will use (the FP64

@@ -279,12 +279,31 @@ struct CgState {
     double true_rr;   // |b - A x|^2 recomputed in FP64 from the final FP32 pressure
 };
 
+// Cells visited by the solver's vector kernels: the cells of the listed blocks (those holding a
+// fluid cell), virtual index v -> cell (blk[v >> sh3] << sh3) | (v & (2^sh3 - 1)).  Blocks without
+// fluid keep x = r = p = 0 from k_divergence and are never read by the stencils.
+struct CellList {
+    const int* blk;
+    int sh3;
+    __device__ __forceinline__ int cell(int v) const { return (blk[v >> sh3] << sh3) | (v & ((1 << sh3) - 1)); }
+};
+
+// Exclusive scan of the per-block fluid flags -> list of blocks holding a fluid cell.
+struct OpFluidBlocks {
+    const int* flag;
+    int* list;
+    __device__ int value(int b) const { return flag[b] != 0 ? 1 : 0; }
+    __device__ void emit(int b, int pre, int v) const { if (v) list[pre] = b; }
+};
+
 template <int B>
 __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type, const float* __restrict__ cx,
                              const float* __restrict__ cy, const float* __restrict__ cz, float wall_level, CgVecs v,
-                             double* __restrict__ partials, double* __restrict__ partials2) {
+                             double* __restrict__ partials, double* __restrict__ partials2, int* __restrict__ bfluid) {
     __shared__ double sh[4];
     double bb = 0.0, nfl = 0.0;
+    // nc is a multiple of 64 and the loop stride a multiple of WG, so every wave stays whole and
+    // its 64 cells lie in one block
     for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
         float d = 0.f;
         uint16_t info = 0;
@@ -340,6 +359,7 @@ __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type,
         v.p[c] = d;
         v.ap[c] = 0.f;
         bb += (double)d * (double)d;
+        if (__ballot(info != 0) && (threadIdx.x & 63) == 0) bfluid[c / Blk<B>::B3] = 1;
     }
     const double s = block_sum(bb, sh);
     const double s2 = block_sum(nfl, sh);
@@ -389,11 +409,13 @@ __device__ __forceinline__ float jacobi_inv(uint16_t info) {
 
 // Relative mode set-up (after k_divergence): z = M^-1 r, p = z, partial r.z.  For multigrid the
 // V-cycle has already written z.
-__global__ void k_cg_precond_init(int nc, CgVecs v, double* __restrict__ partials, const CgState* __restrict__ st) {
+__global__ void k_cg_precond_init(int nv, CellList cl, CgVecs v, double* __restrict__ partials,
+                                  const CgState* __restrict__ st) {
     __shared__ double sh[4];
     double rz = 0.0;
     const int pc = st->precond;
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
+    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
+        const int c = cl.cell(vi);
         const float r = v.r[c];
         const float z = pc == 1 ? r * jacobi_inv(v.info[c]) : (pc == 2 ? v.z[c] : r);
         v.p[c] = z;
@@ -412,31 +434,7 @@ __global__ void k_fin_precond_init(CgState* __restrict__ st, const double* __res
     st->rzold = (float)tot;
 }
 
-// CG step 1: Ap = A p (BLAS::mul, BLAS.cu:12-40, matrix-free 7-point row in the reference's term
-// order) and partial p.Ap.
-template <int B>
-__global__ void k_cg_spmv(int nc, FrameK f, CgVecs v, double* __restrict__ partials, const CgState* __restrict__ st) {
-    __shared__ double sh[4];
-    if (st->done) return;
-    double pap = 0.0;
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
-        const uint16_t info = v.info[c];
-        float acc = 0.f;
-        if (info & kInfoFluid) {
-            const float pc = v.p[c];
-            acc = (float)((info >> 8) & 0x1f) * pc;
-#pragma unroll
-            for (int q = 0; q < 6; ++q)
-                if (info & (1u << q)) acc = acc - v.p[cell_neighbor<B>(f, c, q)];
-            pap += (double)pc * (double)acc;
-        }
-        v.ap[c] = acc;
-    }
-    const double s = block_sum(pap, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
-
-// alpha = rsold / p.Ap  (LinearSystemSolver.cpp:31)
+// alpha = rsold / p.Ap  (LinearSystemSolver.cpp:31); the partials are the products of k_spmv_blk.
 __global__ void k_fin_alpha(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
     __shared__ double sh[4];
     if (st->done) return;
@@ -448,14 +446,15 @@ __global__ void k_fin_alpha(CgState* __restrict__ st, const double* __restrict__
 
 // CG step 2: x += alpha p, r -= alpha Ap (LinearSystemSolver.cpp:33-35), partial r.r (and r.z for
 // Jacobi).
-__global__ void k_cg_update(int nc, CgVecs v, double* __restrict__ partials, double* __restrict__ partials2,
-                            const CgState* __restrict__ st) {
+__global__ void k_cg_update(int nv, CellList cl, CgVecs v, double* __restrict__ partials,
+                            double* __restrict__ partials2, const CgState* __restrict__ st) {
     __shared__ double sh[4];
     if (st->done) return;
     const float alpha = st->alpha;
     const int precond = st->precond;
     double rr = 0.0, rz = 0.0;
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
+    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
+        const int c = cl.cell(vi);
         const float p = v.p[c];
         v.x[c] = v.x[c] + (alpha * p);
         const float r = v.r[c] - (alpha * v.ap[c]);
@@ -504,12 +503,15 @@ __global__ void k_fin_update(CgState* __restrict__ st, const double* __restrict_
 }
 
 // Multigrid: partial r.z after the V-cycle, then beta.
-__global__ void k_cg_rz(int nc, const float* __restrict__ r, const float* __restrict__ z, double* __restrict__ partials,
-                        const CgState* __restrict__ st) {
+__global__ void k_cg_rz(int nv, CellList cl, const float* __restrict__ r, const float* __restrict__ z,
+                        double* __restrict__ partials, const CgState* __restrict__ st) {
     __shared__ double sh[4];
     if (st->done) return;
     double rz = 0.0;
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) rz += (double)r[c] * (double)z[c];
+    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
+        const int c = cl.cell(vi);
+        rz += (double)r[c] * (double)z[c];
+    }
     const double s = block_sum(rz, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
@@ -525,11 +527,12 @@ __global__ void k_fin_beta(CgState* __restrict__ st, const double* __restrict__ 
 }
 
 // CG step 3: p = r + beta p (BLAS::addmul, LinearSystemSolver.cpp:41), or p = z + beta p.
-__global__ void k_cg_pupdate(int nc, CgVecs v, const CgState* __restrict__ st) {
+__global__ void k_cg_pupdate(int nv, CellList cl, CgVecs v, const CgState* __restrict__ st) {
     if (st->done) return;
     const float beta = st->beta;
     const int pc = st->precond;
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
+    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
+        const int c = cl.cell(vi);
         const float r = v.r[c];
         const float z = pc == 1 ? r * jacobi_inv(v.info[c]) : (pc == 2 ? v.z[c] : r);
         v.p[c] = z + (beta * v.p[c]);
@@ -750,7 +753,10 @@ namespace vare {
 
 // Refinement bookkeeping after k_ir_residual: record the FP64 residual and restart the inner solve
 // on the new right-hand side (its relative tolerance is measured against |r|).
-__global__ void k_fin_ir(CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol) {
+// The next correction solve only has to close the gap to the target: its relative tolerance is
+// max(inner_tol, 0.5 * outer_tol * |b| / |r|), at most 0.5.
+__global__ void k_fin_ir(CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol,
+                         float inner_tol) {
     __shared__ double sh[4];
     double s = 0.0;
     for (int q = threadIdx.x; q < n; q += WG) s += partials[q];
@@ -762,6 +768,8 @@ __global__ void k_fin_ir(CgState* __restrict__ st, const double* __restrict__ pa
         st->done = 1;
         return;
     }
+    const double need = 0.5 * (double)outer_tol * sqrt(st->ir_bb / tot);
+    st->rel_tol = (float)fmin(0.5, fmax((double)inner_tol, need));
     st->bb = tot;
     st->rr = tot;
     st->rsold = (float)tot;

@@ -197,34 +197,38 @@ __global__ void k_mg_smooth(MgLevelK L, MgFrame F, float omega, const float* __r
 
 // Restriction of the residual: rC[cc] = 0.5 * sum over fluid children of (r - A z).  With
 // first != 0 the fine z is the first sweep from zero (omega D^-1 r), computed on the fly and
-// stored to zf.
+// stored to zf.  One thread per (coarse cell, child) pair: the 8 lanes of a coarse cell each
+// compute one child's residual and are summed with three butterfly shuffles, (c0+c1)+(c2+c3) ...
+// in a fixed order, so the result is deterministic.
 __global__ void k_mg_restrict(MgLevelK L, MgLevelK C, MgFrame F, float omega, int first, float* __restrict__ zf,
                               const int* done) {
     if (*done) return;
-    const int cc = blockIdx.x * WG + threadIdx.x;
-    if (cc >= C.ncell) return;
-    float acc = 0.f;
-    if (mg_fluid(C.info[cc])) {
-#pragma unroll 1
-        for (int a = 0; a < 8; ++a) {
-            const int ch = mg_child(L, C, F, cc, a);
-            if (ch < 0) continue;
-            const uint16_t in = L.info[ch];
-            if (!mg_fluid(in)) continue;
+    const int t = blockIdx.x * WG + threadIdx.x;
+    const int cc = t >> 3, a = t & 7;
+    const bool valid = cc < C.ncell;
+    float res = 0.f;
+    const bool cfl = valid && mg_fluid(C.info[cc]);
+    if (cfl) {
+        const int ch = mg_child(L, C, F, cc, a);
+        const uint16_t in = ch >= 0 ? L.info[ch] : (uint16_t)0;
+        if (mg_fluid(in)) {
             const float d = mg_diag(in);
             const float zc = first ? omega * L.r[ch] / d : zf[ch];
             if (first) zf[ch] = zc;
             float az = d * zc;
+#pragma unroll
             for (int q = 0; q < 6; ++q) {
                 if (!(in & (1u << q))) continue;
                 const int n = mg_nbr(L, F, ch, q);
                 az -= first ? mg_z0(L, n, omega) : zf[n];
             }
-            acc += L.r[ch] - az;
+            res = L.r[ch] - az;
         }
-        acc *= 0.5f;
     }
-    C.r[cc] = acc;
+    res += __shfl_xor(res, 1, 64);
+    res += __shfl_xor(res, 2, 64);
+    res += __shfl_xor(res, 4, 64);
+    if (valid && a == 0) C.r[cc] = cfl ? 0.5f * res : 0.f;
 }
 
 // Prolongation (piecewise constant) fused with one Jacobi sweep:

@@ -1,6 +1,7 @@
 // flip_stencil.h -- block-aligned 7-point stencil kernels on the fine level (edge B).
 //
-// One workgroup per active block, one thread per cell (B^3 threads: 512 for B=8, 64 for B=4).
+// One workgroup per listed block (the blocks holding a fluid cell, `blist`), one thread per cell
+// (B^3 threads: 512 for B=8, 64 for B=4).  Blocks without fluid hold no unknown and are skipped.
 // The block's own values are staged in LDS; neighbours inside the block are read from LDS, the
 // face neighbours of the 6 adjacent blocks from global memory through the block's neighbour slots.
 // Reductions are one FP64 partial per block, summed in a fixed order by the finalize kernels.
@@ -48,14 +49,14 @@ __device__ __forceinline__ double fine_block_sum(double v, double* sh) {
 
 // Ap = A p and partial p.Ap (BLAS::mul, BLAS.cu:12-40).
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(FrameK f, CgVecs v, double* __restrict__ partials,
+__global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict__ blist, FrameK f, CgVecs v, double* __restrict__ partials,
                                                          const CgState* __restrict__ st) {
     constexpr int B3 = Fine<B>::B3;
     __shared__ float sp[B3];
     __shared__ int fnb[6];
     __shared__ double sh[Fine<B>::NW];
     if (st->done) return;
-    const int s = blockIdx.x, l = threadIdx.x, c = s * B3 + l;
+    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
     if (l < 6) fnb[l] = f.nbr[s * 6 + l];
     const uint16_t info = v.info[c];
     const float pc = v.p[c];
@@ -76,19 +77,19 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(FrameK f, CgVecs v, do
     }
     v.ap[c] = acc;
     const double tot = fine_block_sum<B>(pap, sh);
-    if (l == 0) partials[s] = tot;
+    if (l == 0) partials[blockIdx.x] = tot;
 }
 
 // ---- multigrid, level 0 (the CG residual r, stencil descriptor info) ----
 // Two Jacobi sweeps from zero: zout = z1 + w (r - A z1)/D, z1 = w D^-1 r.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(MgLevelK L, MgFrame F, float omega,
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __restrict__ blist, MgLevelK L, MgFrame F, float omega,
                                                                   float* __restrict__ zout, const int* done) {
     constexpr int B3 = Fine<B>::B3;
     __shared__ float sz[B3];
     __shared__ int fnb[6];
     if (*done) return;
-    const int s = blockIdx.x, l = threadIdx.x, c = s * B3 + l;
+    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
     if (l < 6) fnb[l] = F.nbr[s * 6 + l];
     const uint16_t in = L.info[c];
     const float rc = L.r[c];
@@ -122,7 +123,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(MgLevelK L, M
 // the fixed order of mg_child.  With first != 0, z is the first sweep from zero (w D^-1 r) and is
 // stored to zf.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(MgLevelK L, MgLevelK C, MgFrame F, float omega,
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(const int* __restrict__ blist, MgLevelK L, MgLevelK C, MgFrame F, float omega,
                                                              int first, float* __restrict__ zf, const int* done) {
     constexpr int B3 = Fine<B>::B3;
     constexpr int SH = Fine<B>::SH;
@@ -131,7 +132,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(MgLevelK L, MgLeve
     __shared__ float sres[B3];
     __shared__ int fnb[6];
     if (*done) return;
-    const int s = blockIdx.x, l = threadIdx.x, c = s * B3 + l;
+    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
     if (l < 6) fnb[l] = F.nbr[s * 6 + l];
     const uint16_t in = L.info[c];
     const float rc = L.r[c];
@@ -187,7 +188,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(MgLevelK L, MgLeve
 
 // Prolongation (piecewise constant from level 1) fused with one Jacobi sweep on level 0.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(MgLevelK L, MgLevelK C, MgFrame F, float omega,
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* __restrict__ blist, MgLevelK L, MgLevelK C, MgFrame F, float omega,
                                                                    const float* __restrict__ z,
                                                                    const float* __restrict__ zc,
                                                                    float* __restrict__ zout, const int* done) {
@@ -197,7 +198,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(MgLevelK L, 
     __shared__ float szp[B3];
     __shared__ int fnb[6];
     if (*done) return;
-    const int s = blockIdx.x, l = threadIdx.x, c = s * B3 + l;
+    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
     if (l < 6) fnb[l] = F.nbr[s * 6 + l];
     auto parent_local = [](int ll) {
         const int li = (ll & (B - 1)) >> 1, lj = ((ll >> SH) & (B - 1)) >> 1, lk = (ll >> (2 * SH)) >> 1;
@@ -230,14 +231,14 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(MgLevelK L, 
 
 // One Jacobi sweep on level 0: zout = z + w (r - A z)/D.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(MgLevelK L, MgFrame F, float omega,
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restrict__ blist, MgLevelK L, MgFrame F, float omega,
                                                            const float* __restrict__ z, float* __restrict__ zout,
                                                            const int* done) {
     constexpr int B3 = Fine<B>::B3;
     __shared__ float sz[B3];
     __shared__ int fnb[6];
     if (*done) return;
-    const int s = blockIdx.x, l = threadIdx.x, c = s * B3 + l;
+    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
     if (l < 6) fnb[l] = F.nbr[s * 6 + l];
     const uint16_t in = L.info[c];
     const float zc = z[c];
