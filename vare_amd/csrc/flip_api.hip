@@ -559,8 +559,10 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
     const int nu = h->prm.mg_sweeps;
     const int* done = done_ptr(h);
     const int m = (int)h->mg.size() - 1;
-    const unsigned nb = (unsigned)h->ns;                     // level 0 runs over the solver's blocks
+    // level 0 runs over the solver's blocks, Fine<B>::K of them per workgroup
+    const unsigned nb = (unsigned)((h->ns + Fine<B>::K - 1) / Fine<B>::K);
     const int* bl = h->blist;
+    const int nsb = h->ns;
     constexpr unsigned B3 = (unsigned)(B * B * B);
     for (int l = 0; l < m; ++l) {
         const MgLevelK& L = h->mg[l];
@@ -568,12 +570,12 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
         if (l == 0) {           // fine level: block-aligned kernels with the block staged in LDS
             if (!nb) continue;
             if (nu == 1) {
-                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, C, F, w, 1,
+                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, C, F, w, 1,
                                    h->mg_zt[0], done);
             } else {
-                hipLaunchKernelGGL(k_mg0_smooth_first2<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, F, w,
+                hipLaunchKernelGGL(k_mg0_smooth_first2<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, F, w,
                                    h->mg_z[0], done);
-                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, C, F, w, 0,
+                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, C, F, w, 0,
                                    h->mg_z[0], done);
             }
             continue;
@@ -600,12 +602,12 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
         if (l == 0) {
             if (!nb) continue;
             if (nu == 1) {
-                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, C, F, w,
+                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, C, F, w,
                                    (const float*)h->mg_zt[0], (const float*)h->mg_z[1], h->mg_z[0], done);
             } else {
-                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, C, F, w,
+                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, C, F, w,
                                    (const float*)h->mg_z[0], (const float*)h->mg_z[1], h->mg_zt[0], done);
-                hipLaunchKernelGGL(k_mg0_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, L, F, w,
+                hipLaunchKernelGGL(k_mg0_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, F, w,
                                    (const float*)h->mg_zt[0], h->mg_z[0], done);
             }
             continue;
@@ -635,11 +637,12 @@ template <int B> void enqueue_iteration(vare_flip* h) {
     if (!nv) return;
     const unsigned gs = rgrid((size_t)nv);
     const CellList cl = h->clist();
+    const int nwg = (h->ns + Fine<B>::K - 1) / Fine<B>::K;
     tbegin(h, VARE_KERNEL_SPMV);
-    hipLaunchKernelGGL(k_spmv_blk<B>, dim3(h->ns), dim3(B * B * B), 0, h->stream, (const int*)h->blist, h->frame(),
-                       vecs(h), h->partials, (const CgState*)h->st);
+    hipLaunchKernelGGL(k_spmv_blk<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, (const int*)h->blist, h->ns,
+                       h->frame(), vecs(h), h->partials, (const CgState*)h->st);
     tend(h, VARE_KERNEL_SPMV);
-    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, h->ns);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
     tbegin(h, VARE_KERNEL_UPDATE);
     hipLaunchKernelGGL(k_cg_update, dim3(gs), dim3(WG), 0, h->stream, nv, cl, vecs(h), h->partials, h->partials2,
                        (const CgState*)h->st);

@@ -1,11 +1,13 @@
 // flip_stencil.h -- block-aligned 7-point stencil kernels on the fine level (edge B).
 //
-// One workgroup per listed block (the blocks holding a fluid cell, `blist`), one thread per cell
-// (B^3 threads: 512 for B=8, 64 for B=4).  Blocks without fluid hold no unknown and are skipped.
-// The block's own values are staged in LDS; neighbours inside the block are read from LDS, the
-// face neighbours of the 6 adjacent blocks from global memory through the block's neighbour slots.
-// Reductions are one FP64 partial per block, summed in a fixed order by the finalize kernels.
-// The arithmetic (term order included) is identical to the generic grid-stride kernels.
+// Each workgroup takes K consecutive entries of the solver's block list (`blist`: the blocks
+// holding a fluid cell; blocks without fluid hold no unknown and are skipped) with one thread per
+// cell position (B^3 threads: 512 for B=8, 64 for B=4), so every thread serves K cells.  A pass is
+// two dependent memory round trips (own values, then the face neighbours in adjacent blocks), so
+// the K blocks' loads are issued together in each round: the latency is paid once per K blocks.
+// Own values are staged in LDS; neighbours inside a block are read from LDS, those across a face
+// from global memory through the block's neighbour slots.  Reductions are one FP64 partial per
+// workgroup, summed in a fixed order by the finalize kernels.
 #pragma once
 #include "flip_kernels.h"
 #include "flip_mg.h"
@@ -16,7 +18,25 @@ template <int B> struct Fine {
     static constexpr int B3 = B * B * B;
     static constexpr int SH = (B == 4) ? 2 : 3;
     static constexpr int NW = B3 / 64;         // waves per workgroup
+    static constexpr int K = 4;                // blocks per workgroup
 };
+
+// Slots of this workgroup's K blocks (-1 past the end of the list) and, in LDS, their 6 neighbour
+// slots each.  The caller synchronises before reading fnb.
+template <int B>
+__device__ __forceinline__ void wg_blocks(const int* __restrict__ blist, int nsb, const int* __restrict__ nbr,
+                                          int (&s)[Fine<B>::K], int* fnb) {
+    constexpr int K = Fine<B>::K;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        const int v = blockIdx.x * K + kb;
+        s[kb] = v < nsb ? blist[v] : -1;
+    }
+    if (threadIdx.x < 6 * K) {
+        const int v = blockIdx.x * K + threadIdx.x / 6;
+        fnb[threadIdx.x] = v < nsb ? nbr[blist[v] * 6 + threadIdx.x % 6] : -1;
+    }
+}
 
 // Local coordinates of thread l and whether the neighbour in direction q stays in the block;
 // returns the local index inside the block (nl) or, across the face, the local index inside the
@@ -49,33 +69,56 @@ __device__ __forceinline__ double fine_block_sum(double v, double* sh) {
 
 // Ap = A p and partial p.Ap (BLAS::mul, BLAS.cu:12-40).
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict__ blist, FrameK f, CgVecs v, double* __restrict__ partials,
+__global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict__ blist, int nsb, FrameK f, CgVecs v,
+                                                         double* __restrict__ partials,
                                                          const CgState* __restrict__ st) {
-    constexpr int B3 = Fine<B>::B3;
-    __shared__ float sp[B3];
-    __shared__ int fnb[6];
+    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    __shared__ float sp[K][B3];
+    __shared__ int fnb[6 * K];
     __shared__ double sh[Fine<B>::NW];
     if (st->done) return;
-    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
-    if (l < 6) fnb[l] = f.nbr[s * 6 + l];
-    const uint16_t info = v.info[c];
-    const float pc = v.p[c];
-    sp[l] = pc;
+    const int l = threadIdx.x;
+    int s[K];
+    wg_blocks<B>(blist, nsb, f.nbr, s, fnb);
+    uint16_t in[K];
+    float pc[K];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        in[kb] = 0;
+        pc[kb] = 0.f;
+        if (s[kb] >= 0) {
+            in[kb] = v.info[s[kb] * B3 + l];
+            pc[kb] = v.p[s[kb] * B3 + l];
+        }
+        sp[kb][l] = pc[kb];
+    }
     __syncthreads();
-    float acc = 0.f;
-    double pap = 0.0;
-    if (info & kInfoFluid) {
-        acc = (float)((info >> 8) & 0x1f) * pc;
+    float hv[K][6];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb)
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            if (!(info & (1u << q))) continue;
             int nl;
-            const float pv = fine_nbr<B>(l, q, nl) ? sp[nl] : v.p[fnb[q] * B3 + nl];
-            acc = acc - pv;
+            hv[kb][q] = 0.f;
+            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) hv[kb][q] = v.p[fnb[kb * 6 + q] * B3 + nl];
         }
-        pap = (double)pc * (double)acc;
+    double pap = 0.0;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        if (s[kb] < 0) continue;
+        float acc = 0.f;
+        if (in[kb] & kInfoFluid) {
+            acc = (float)((in[kb] >> 8) & 0x1f) * pc[kb];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (!(in[kb] & (1u << q))) continue;
+                int nl;
+                acc = acc - (fine_nbr<B>(l, q, nl) ? sp[kb][nl] : hv[kb][q]);
+            }
+            pap += (double)pc[kb] * (double)acc;
+        }
+        v.ap[s[kb] * B3 + l] = acc;
     }
-    v.ap[c] = acc;
     const double tot = fine_block_sum<B>(pap, sh);
     if (l == 0) partials[blockIdx.x] = tot;
 }
@@ -83,39 +126,63 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict_
 // ---- multigrid, level 0 (the CG residual r, stencil descriptor info) ----
 // Two Jacobi sweeps from zero: zout = z1 + w (r - A z1)/D, z1 = w D^-1 r.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __restrict__ blist, MgLevelK L, MgFrame F, float omega,
-                                                                  float* __restrict__ zout, const int* done) {
-    constexpr int B3 = Fine<B>::B3;
-    __shared__ float sz[B3];
-    __shared__ int fnb[6];
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __restrict__ blist, int nsb, MgLevelK L,
+                                                                  MgFrame F, float omega, float* __restrict__ zout,
+                                                                  const int* done) {
+    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    __shared__ float sz[K][B3];
+    __shared__ int fnb[6 * K];
     if (*done) return;
-    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
-    if (l < 6) fnb[l] = F.nbr[s * 6 + l];
-    const uint16_t in = L.info[c];
-    const float rc = L.r[c];
-    const float z1 = mg_fluid(in) ? omega * rc / mg_diag(in) : 0.f;
-    sz[l] = z1;
+    const int l = threadIdx.x;
+    int s[K];
+    wg_blocks<B>(blist, nsb, F.nbr, s, fnb);
+    uint16_t in[K];
+    float rc[K], z1[K];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        in[kb] = 0;
+        rc[kb] = 0.f;
+        if (s[kb] >= 0) {
+            in[kb] = L.info[s[kb] * B3 + l];
+            rc[kb] = L.r[s[kb] * B3 + l];
+        }
+        z1[kb] = mg_fluid(in[kb]) ? omega * rc[kb] / mg_diag(in[kb]) : 0.f;
+        sz[kb][l] = z1[kb];
+    }
     __syncthreads();
-    float out = 0.f;
-    if (mg_fluid(in)) {
-        const float d = mg_diag(in);
-        float az = d * z1;
+    uint16_t hi[K][6];
+    float hr[K][6];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb)
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            if (!(in & (1u << q))) continue;
             int nl;
-            float zn;
-            if (fine_nbr<B>(l, q, nl)) zn = sz[nl];
-            else {
-                const int n = fnb[q] * B3 + nl;
-                const uint16_t ni = L.info[n];
-                zn = mg_fluid(ni) ? omega * L.r[n] / mg_diag(ni) : 0.f;
+            hi[kb][q] = 0;
+            hr[kb][q] = 0.f;
+            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) {
+                const int n = fnb[kb * 6 + q] * B3 + nl;
+                hi[kb][q] = L.info[n];
+                hr[kb][q] = L.r[n];
             }
-            az -= zn;
         }
-        out = z1 + omega * (rc - az) / d;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        if (s[kb] < 0) continue;
+        float out = 0.f;
+        if (mg_fluid(in[kb])) {
+            const float d = mg_diag(in[kb]);
+            float az = d * z1[kb];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (!(in[kb] & (1u << q))) continue;
+                int nl;
+                az -= fine_nbr<B>(l, q, nl) ? sz[kb][nl]
+                                            : (mg_fluid(hi[kb][q]) ? omega * hr[kb][q] / mg_diag(hi[kb][q]) : 0.f);
+            }
+            out = z1[kb] + omega * (rc[kb] - az) / d;
+        }
+        zout[s[kb] * B3 + l] = out;
     }
-    zout[c] = out;
 }
 
 // Residual of level 0 and restriction to level 1 inside the block (the 2^3 children of a level-1
@@ -123,140 +190,215 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __
 // the fixed order of mg_child.  With first != 0, z is the first sweep from zero (w D^-1 r) and is
 // stored to zf.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(const int* __restrict__ blist, MgLevelK L, MgLevelK C, MgFrame F, float omega,
-                                                             int first, float* __restrict__ zf, const int* done) {
-    constexpr int B3 = Fine<B>::B3;
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(const int* __restrict__ blist, int nsb, MgLevelK L,
+                                                             MgLevelK C, MgFrame F, float omega, int first,
+                                                             float* __restrict__ zf, const int* done) {
+    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
     constexpr int SH = Fine<B>::SH;
     constexpr int BC = B / 2, CSH = SH - 1, C3 = BC * BC * BC;
-    __shared__ float sz[B3];
-    __shared__ float sres[B3];
-    __shared__ int fnb[6];
+    __shared__ float sz[K][B3];
+    __shared__ float sres[K][B3];
+    __shared__ int fnb[6 * K];
     if (*done) return;
-    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
-    if (l < 6) fnb[l] = F.nbr[s * 6 + l];
-    const uint16_t in = L.info[c];
-    const float rc = L.r[c];
-    float zc;
-    if (first) {
-        zc = mg_fluid(in) ? omega * rc / mg_diag(in) : 0.f;
-        if (mg_fluid(in)) zf[c] = zc;
-    } else {
-        zc = zf[c];
+    const int l = threadIdx.x;
+    int s[K];
+    wg_blocks<B>(blist, nsb, F.nbr, s, fnb);
+    uint16_t in[K];
+    float rc[K], zc[K];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        in[kb] = 0;
+        rc[kb] = 0.f;
+        zc[kb] = 0.f;
+        if (s[kb] >= 0) {
+            const int c = s[kb] * B3 + l;
+            in[kb] = L.info[c];
+            rc[kb] = L.r[c];
+            if (first) {
+                zc[kb] = mg_fluid(in[kb]) ? omega * rc[kb] / mg_diag(in[kb]) : 0.f;
+                if (mg_fluid(in[kb])) zf[c] = zc[kb];
+            } else {
+                zc[kb] = zf[c];
+            }
+        }
+        sz[kb][l] = zc[kb];
     }
-    sz[l] = zc;
     __syncthreads();
-    float res = 0.f;
-    if (mg_fluid(in)) {
-        const float d = mg_diag(in);
-        float az = d * zc;
+    float hz[K][6];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb)
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            if (!(in & (1u << q))) continue;
             int nl;
-            float zn;
-            if (fine_nbr<B>(l, q, nl)) zn = sz[nl];
-            else {
-                const int n = fnb[q] * B3 + nl;
+            hz[kb][q] = 0.f;
+            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) {
+                const int n = fnb[kb * 6 + q] * B3 + nl;
                 if (first) {
                     const uint16_t ni = L.info[n];
-                    zn = mg_fluid(ni) ? omega * L.r[n] / mg_diag(ni) : 0.f;
+                    hz[kb][q] = mg_fluid(ni) ? omega * L.r[n] / mg_diag(ni) : 0.f;
                 } else {
-                    zn = zf[n];
+                    hz[kb][q] = zf[n];
                 }
             }
-            az -= zn;
         }
-        res = rc - az;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        float res = 0.f;
+        if (mg_fluid(in[kb])) {
+            const float d = mg_diag(in[kb]);
+            float az = d * zc[kb];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (!(in[kb] & (1u << q))) continue;
+                int nl;
+                az -= fine_nbr<B>(l, q, nl) ? sz[kb][nl] : hz[kb][q];
+            }
+            res = rc[kb] - az;
+        }
+        sres[kb][l] = res;
     }
-    sres[l] = mg_fluid(in) ? res : 0.f;
     __syncthreads();
     if (l < C3) {
-        const int cc = s * C3 + l;
-        float acc = 0.f;
-        if (mg_fluid(C.info[cc])) {
-            const int li = l & (BC - 1), lj = (l >> CSH) & (BC - 1), lk = l >> (2 * CSH);
+        const int li = l & (BC - 1), lj = (l >> CSH) & (BC - 1), lk = l >> (2 * CSH);
 #pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                const int fl = (((2 * lk + (a >> 2)) << SH) + (2 * lj + ((a >> 1) & 1))) * B + (2 * li + (a & 1));
-                acc += sres[fl];
+        for (int kb = 0; kb < K; ++kb) {
+            if (s[kb] < 0) continue;
+            const int cc = s[kb] * C3 + l;
+            float acc = 0.f;
+            if (mg_fluid(C.info[cc])) {
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    const int fl = (((2 * lk + (a >> 2)) << SH) + (2 * lj + ((a >> 1) & 1))) * B + (2 * li + (a & 1));
+                    acc += sres[kb][fl];
+                }
+                acc *= 0.5f;
             }
-            acc *= 0.5f;
+            C.r[cc] = acc;
         }
-        C.r[cc] = acc;
     }
 }
 
 // Prolongation (piecewise constant from level 1) fused with one Jacobi sweep on level 0.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* __restrict__ blist, MgLevelK L, MgLevelK C, MgFrame F, float omega,
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* __restrict__ blist, int nsb,
+                                                                   MgLevelK L, MgLevelK C, MgFrame F, float omega,
                                                                    const float* __restrict__ z,
                                                                    const float* __restrict__ zc,
                                                                    float* __restrict__ zout, const int* done) {
-    constexpr int B3 = Fine<B>::B3;
+    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
     constexpr int SH = Fine<B>::SH;
     constexpr int BC = B / 2, CSH = SH - 1, C3 = BC * BC * BC;
-    __shared__ float szp[B3];
-    __shared__ int fnb[6];
+    __shared__ float szp[K][B3];
+    __shared__ int fnb[6 * K];
     if (*done) return;
-    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
-    if (l < 6) fnb[l] = F.nbr[s * 6 + l];
+    const int l = threadIdx.x;
+    int s[K];
+    wg_blocks<B>(blist, nsb, F.nbr, s, fnb);
     auto parent_local = [](int ll) {
         const int li = (ll & (B - 1)) >> 1, lj = ((ll >> SH) & (B - 1)) >> 1, lk = (ll >> (2 * SH)) >> 1;
         return (((lk << CSH) + lj) << CSH) + li;
     };
-    const uint16_t in = L.info[c];
-    const float zp = z[c] + zc[s * C3 + parent_local(l)];
-    szp[l] = zp;
+    uint16_t in[K];
+    float zp[K], rc[K];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        in[kb] = 0;
+        zp[kb] = 0.f;
+        rc[kb] = 0.f;
+        if (s[kb] >= 0) {
+            const int c = s[kb] * B3 + l;
+            in[kb] = L.info[c];
+            zp[kb] = z[c] + zc[s[kb] * C3 + parent_local(l)];
+            rc[kb] = L.r[c];
+        }
+        szp[kb][l] = zp[kb];
+    }
     __syncthreads();
-    float out = 0.f;
-    if (mg_fluid(in)) {
-        const float d = mg_diag(in);
-        float az = d * zp;
+    float hz[K][6];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb)
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            if (!(in & (1u << q))) continue;
             int nl;
-            float zn;
-            if (fine_nbr<B>(l, q, nl)) zn = szp[nl];
-            else {
-                const int ns = fnb[q];
-                zn = z[ns * B3 + nl] + zc[ns * C3 + parent_local(nl)];
+            hz[kb][q] = 0.f;
+            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) {
+                const int ns = fnb[kb * 6 + q];
+                hz[kb][q] = z[ns * B3 + nl] + zc[ns * C3 + parent_local(nl)];
             }
-            az -= zn;
         }
-        out = zp + omega * (L.r[c] - az) / d;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        if (s[kb] < 0) continue;
+        float out = 0.f;
+        if (mg_fluid(in[kb])) {
+            const float d = mg_diag(in[kb]);
+            float az = d * zp[kb];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (!(in[kb] & (1u << q))) continue;
+                int nl;
+                az -= fine_nbr<B>(l, q, nl) ? szp[kb][nl] : hz[kb][q];
+            }
+            out = zp[kb] + omega * (rc[kb] - az) / d;
+        }
+        zout[s[kb] * B3 + l] = out;
     }
-    zout[c] = out;
 }
 
 // One Jacobi sweep on level 0: zout = z + w (r - A z)/D.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restrict__ blist, MgLevelK L, MgFrame F, float omega,
-                                                           const float* __restrict__ z, float* __restrict__ zout,
-                                                           const int* done) {
-    constexpr int B3 = Fine<B>::B3;
-    __shared__ float sz[B3];
-    __shared__ int fnb[6];
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restrict__ blist, int nsb, MgLevelK L,
+                                                           MgFrame F, float omega, const float* __restrict__ z,
+                                                           float* __restrict__ zout, const int* done) {
+    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    __shared__ float sz[K][B3];
+    __shared__ int fnb[6 * K];
     if (*done) return;
-    const int s = blist[blockIdx.x], l = threadIdx.x, c = s * B3 + l;
-    if (l < 6) fnb[l] = F.nbr[s * 6 + l];
-    const uint16_t in = L.info[c];
-    const float zc = z[c];
-    sz[l] = zc;
+    const int l = threadIdx.x;
+    int s[K];
+    wg_blocks<B>(blist, nsb, F.nbr, s, fnb);
+    uint16_t in[K];
+    float zc[K], rc[K];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        in[kb] = 0;
+        zc[kb] = 0.f;
+        rc[kb] = 0.f;
+        if (s[kb] >= 0) {
+            const int c = s[kb] * B3 + l;
+            in[kb] = L.info[c];
+            zc[kb] = z[c];
+            rc[kb] = L.r[c];
+        }
+        sz[kb][l] = zc[kb];
+    }
     __syncthreads();
-    float out = 0.f;
-    if (mg_fluid(in)) {
-        const float d = mg_diag(in);
-        float az = d * zc;
+    float hz[K][6];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb)
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            if (!(in & (1u << q))) continue;
             int nl;
-            az -= fine_nbr<B>(l, q, nl) ? sz[nl] : z[fnb[q] * B3 + nl];
+            hz[kb][q] = 0.f;
+            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) hz[kb][q] = z[fnb[kb * 6 + q] * B3 + nl];
         }
-        out = zc + omega * (L.r[c] - az) / d;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        if (s[kb] < 0) continue;
+        float out = 0.f;
+        if (mg_fluid(in[kb])) {
+            const float d = mg_diag(in[kb]);
+            float az = d * zc[kb];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (!(in[kb] & (1u << q))) continue;
+                int nl;
+                az -= fine_nbr<B>(l, q, nl) ? sz[kb][nl] : hz[kb][q];
+            }
+            out = zc[kb] + omega * (rc[kb] - az) / d;
+        }
+        zout[s[kb] * B3 + l] = out;
     }
-    zout[c] = out;
 }
 
 }  // namespace vare
