@@ -85,16 +85,22 @@ struct vare_flip {
     float vox_wall = 0.f;
     hipStream_t stream = nullptr;
 
-    // particles, SoA in reference order: px,py,pz,vx,vy,vz,adj,vort,curv
+    // particles, SoA: px,py,pz,vx,vy,vz,adj in physical order (bin order after each binning),
+    // vort,curv in the reference (logical) order; pid[q] = logical index of physical particle q
     size_t np = 0, pcap = 0;
     float* pa[9] = {};            // pa[3..5] hold FP16 values when vhalf != 0
+    float* pb[7] = {};            // second buffers of pa[0..6] for the reorder
+    int* pid = nullptr;
+    int* pid2 = nullptr;
     int vhalf = 0;
-    uint8_t* removed = nullptr;
+    uint8_t* removed = nullptr;   // per physical particle (written by G2P)
+    uint8_t* removed_log = nullptr;   // per logical index: the mask of the last step
     int* hole = nullptr;
     int* src = nullptr;
+    int* remap = nullptr;         // new logical index of the survivors moved by the last compaction
     int* pkey = nullptr;          // bin (storage index of the particle's base cell, or Nc)
     int* prank = nullptr;         // arrival rank inside the bin
-    int* perm = nullptr;          // binning permutation: particle indices grouped by bin
+    unsigned long long* perm = nullptr;   // binning: (logical index << 32 | physical index) per bin slot
     int* longbins = nullptr;      // bins longer than kSortReg, [pcap/(kSortReg+1)+2]
     int* ccount = nullptr;        // particles per bin, [bcap*B^3+2]
     int* cstart = nullptr;        // bin starts, [bcap*B^3+2]
@@ -251,11 +257,24 @@ int ensure_particles(vare_flip* h, size_t n) {
         na[q] = (float*)p;
         if (h->np) HIPCHK(hipMemcpyAsync(na[q], h->pa[q], h->np * esz, hipMemcpyDeviceToDevice, h->stream));
     }
+    int* npid = nullptr;
+    CHK(dalloc(&npid, cap));
+    if (h->np) HIPCHK(hipMemcpyAsync(npid, h->pid, h->np * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int q = 0; q < 9; ++q) { dfree(h->pa[q]); h->pa[q] = na[q]; }
-    dfree(h->removed); dfree(h->hole); dfree(h->src);
+    dfree(h->pid);
+    h->pid = npid;
+    for (int q = 0; q < 7; ++q) dfree(h->pb[q]);
+    dfree(h->pid2); dfree(h->removed); dfree(h->removed_log); dfree(h->hole); dfree(h->src); dfree(h->remap);
     dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins);
-    CHK(dalloc(&h->removed, cap)); CHK(dalloc(&h->hole, cap)); CHK(dalloc(&h->src, cap));
+    for (int q = 0; q < 7; ++q) {
+        char* p = nullptr;
+        CHK(dalloc(&p, cap * ((q >= 3 && q < 6) ? h->vsz() : sizeof(float))));
+        h->pb[q] = (float*)p;
+    }
+    CHK(dalloc(&h->pid2, cap));
+    CHK(dalloc(&h->removed, cap)); CHK(dalloc(&h->removed_log, cap + 4));
+    CHK(dalloc(&h->hole, cap)); CHK(dalloc(&h->src, cap)); CHK(dalloc(&h->remap, cap));
     CHK(dalloc(&h->pkey, cap)); CHK(dalloc(&h->prank, cap)); CHK(dalloc(&h->perm, cap));
     CHK(dalloc(&h->longbins, cap / (kSortReg + 1) + 2));
     h->pcap = cap;
@@ -357,9 +376,9 @@ __global__ void k_cg_setup(CgState* st, int mode, int max_count, float abs_tol, 
     st->converged = 0;
 }
 
-// Binning permutation: bin every particle by the storage index of its P2G base cell (bin Nc for
-// particles outside the domain), counting sort, then sort each bin by particle index so the
-// permutation equals a stable counting sort of the particle order.
+// Binning: bin every particle by the storage index of its P2G base cell (bin Nc for particles
+// outside the domain), counting sort, then sort each bin by logical index so the order equals a
+// stable counting sort of the reference order, and reorder the physical arrays into it.
 constexpr int kSortLongGrid = 512;   // workgroups of k_sort_long (strides over the long-bin list)
 
 template <int B> int bin_particles(vare_flip* h) {
@@ -374,11 +393,19 @@ template <int B> int bin_particles(vare_flip* h) {
     CHK(run_scan(h, op, nc + 2, 3));
     if (h->np) {
         hipLaunchKernelGGL(k_pscatter, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, (const int*)h->pkey,
-                           (const int*)h->prank, (const int*)h->cstart, h->perm);
+                           (const int*)h->prank, (const int*)h->cstart, (const int*)h->pid, h->perm);
         hipLaunchKernelGGL(k_sort_short, dim3(nblocks((size_t)nc + 1)), dim3(WG), 0, h->stream, (const int*)h->cstart,
                            nc + 1, h->perm, h->longbins, h->dint + 5);
         hipLaunchKernelGGL(k_sort_long, dim3(kSortLongGrid), dim3(1024), 0, h->stream, (const int*)h->cstart,
                            (const int*)h->longbins, (const int*)(h->dint + 5), h->perm, h->dint + 4);
+        // move the particles into bin order (double-buffered) and keep their logical indices
+        ReorderArrays ra;
+        for (int a = 0; a < 7; ++a) { ra.src[a] = h->pa[a]; ra.dst[a] = h->pb[a]; }
+        ra.vsz = (int)h->vsz();
+        hipLaunchKernelGGL(k_reorder, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np,
+                           (const unsigned long long*)h->perm, ra, h->pid2);
+        for (int a = 0; a < 7; ++a) std::swap(h->pa[a], h->pb[a]);
+        std::swap(h->pid, h->pid2);
     }
     HIPCHK(hipGetLastError());
     return VARE_OK;
@@ -421,7 +448,7 @@ template <int B> int stage_p2g(vare_flip* h) {
         if (h->nb)
             hipLaunchKernelGGL(k_p2g_gather<B>, dim3(h->nb), dim3(P2GGather<B>::THREADS), 0, h->stream,
                                (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
-                               (const int*)h->perm, (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type,
+                               (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type,
                                h->colk(), h->prm.col_vel_scale);
         tend(h, VARE_KERNEL_P2G);
         HIPCHK(hipGetLastError());
@@ -552,6 +579,54 @@ int mg_setup(vare_flip* h) {
     return VARE_OK;
 }
 
+// Block-aligned kernels for a sparse level of edge E (E^3 cells per slot, slot-major like the fine
+// cells): level 0, and for B = 8 also level 1 (edge 4), run over the solver's blocks, Fine<E>::K
+// per workgroup.  A level-l cell is fluid only if a level-0 cell of its block is, so the listed
+// blocks hold every fluid cell of these levels.
+template <int E> void enqueue_block_down(vare_flip* h, int l, int nu) {
+    const MgFrame F = mg_frame(h);
+    const float w = h->prm.mg_omega;
+    const int* done = done_ptr(h);
+    const MgLevelK& L = h->mg[l];
+    const MgLevelK& C = h->mg[l + 1];
+    const unsigned nwg = (unsigned)((h->ns + Fine<E>::K - 1) / Fine<E>::K);
+    const int* bl = h->blist;
+    const int nsb = h->ns;
+    constexpr unsigned E3 = (unsigned)(E * E * E);
+    if (!nwg) return;
+    if (nu == 1) {
+        hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w, 1, h->mg_zt[l],
+                           done);
+    } else {
+        hipLaunchKernelGGL(k_mg0_smooth_first2<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, F, w, h->mg_z[l],
+                           done);
+        hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w, 0, h->mg_z[l],
+                           done);
+    }
+}
+
+template <int E> void enqueue_block_up(vare_flip* h, int l, int nu) {
+    const MgFrame F = mg_frame(h);
+    const float w = h->prm.mg_omega;
+    const int* done = done_ptr(h);
+    const MgLevelK& L = h->mg[l];
+    const MgLevelK& C = h->mg[l + 1];
+    const unsigned nwg = (unsigned)((h->ns + Fine<E>::K - 1) / Fine<E>::K);
+    const int* bl = h->blist;
+    const int nsb = h->ns;
+    constexpr unsigned E3 = (unsigned)(E * E * E);
+    if (!nwg) return;
+    if (nu == 1) {
+        hipLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w,
+                           (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done);
+    } else {
+        hipLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w,
+                           (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done);
+        hipLaunchKernelGGL(k_mg0_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, F, w,
+                           (const float*)h->mg_zt[l], h->mg_z[l], done);
+    }
+}
+
 // z = V-cycle(r) into the CG's z vector.
 template <int B> void enqueue_vcycle_t(vare_flip* h) {
     const MgFrame F = mg_frame(h);
@@ -559,25 +634,13 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
     const int nu = h->prm.mg_sweeps;
     const int* done = done_ptr(h);
     const int m = (int)h->mg.size() - 1;
-    // level 0 runs over the solver's blocks, Fine<B>::K of them per workgroup
-    const unsigned nb = (unsigned)((h->ns + Fine<B>::K - 1) / Fine<B>::K);
-    const int* bl = h->blist;
-    const int nsb = h->ns;
-    constexpr unsigned B3 = (unsigned)(B * B * B);
+    const int nblk_levels = B == 8 ? 2 : 1;                    // levels run by the block kernels
     for (int l = 0; l < m; ++l) {
         const MgLevelK& L = h->mg[l];
         const MgLevelK& C = h->mg[l + 1];
-        if (l == 0) {           // fine level: block-aligned kernels with the block staged in LDS
-            if (!nb) continue;
-            if (nu == 1) {
-                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, C, F, w, 1,
-                                   h->mg_zt[0], done);
-            } else {
-                hipLaunchKernelGGL(k_mg0_smooth_first2<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, F, w,
-                                   h->mg_z[0], done);
-                hipLaunchKernelGGL(k_mg0_restrict<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, C, F, w, 0,
-                                   h->mg_z[0], done);
-            }
+        if (l < nblk_levels) {
+            if (l == 0) enqueue_block_down<B>(h, 0, nu);
+            else enqueue_block_down<4>(h, 1, nu);
             continue;
         }
         if (nu == 1) {
@@ -599,17 +662,9 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
         const MgLevelK& L = h->mg[l];
         const MgLevelK& C = h->mg[l + 1];
         if (!L.ncell) continue;
-        if (l == 0) {
-            if (!nb) continue;
-            if (nu == 1) {
-                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, C, F, w,
-                                   (const float*)h->mg_zt[0], (const float*)h->mg_z[1], h->mg_z[0], done);
-            } else {
-                hipLaunchKernelGGL(k_mg0_prolong_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, C, F, w,
-                                   (const float*)h->mg_z[0], (const float*)h->mg_z[1], h->mg_zt[0], done);
-                hipLaunchKernelGGL(k_mg0_smooth<B>, dim3(nb), dim3(B3), 0, h->stream, bl, nsb, L, F, w,
-                                   (const float*)h->mg_zt[0], h->mg_z[0], done);
-            }
+        if (l < nblk_levels) {
+            if (l == 0) enqueue_block_up<B>(h, 0, nu);
+            else enqueue_block_up<4>(h, 1, nu);
             continue;
         }
         if (nu == 1) {
@@ -755,11 +810,12 @@ template <int B> int stage_solve(vare_flip* h) {
 
 template <int B> int stage_g2p(vare_flip* h, float dt) {
     CHK(fill_u32(h, h->dint + 1, 0, 1));
+    CHK(fill_u32(h, h->removed_log, 0, (h->np + 3) / 4));
     tbegin(h, VARE_KERNEL_G2P);
     if (h->np && h->prm.particle_path == 0) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
         hipLaunchKernelGGL(k_g2p_tile<B>, dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
-                           h->vel(), h->pa[6], (const int*)h->perm, (const int*)h->cstart, h->nb,
+                           h->vel(), h->pa[6], (const int*)h->cstart, h->nb,
                            h->frame(), gf, g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
     } else if (h->np) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
@@ -773,6 +829,28 @@ template <int B> int stage_g2p(vare_flip* h, float dt) {
     HIPCHK(hipStreamSynchronize(h->stream));
     h->nremoved = h->dint_host[1];
     h->last_np_before_remove = h->np;
+    if (h->nremoved > 0)
+        hipLaunchKernelGGL(k_removed_logical, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np,
+                           (const uint8_t*)h->removed, (const int*)h->pid, h->removed_log);
+    HIPCHK(hipGetLastError());
+    return VARE_OK;
+}
+
+// Array::remove (Array.h:995-1024) on the reference order, in closed form: the k-th removed
+// logical index below newN (ascending) receives the k-th surviving logical index at or above newN
+// (descending).  Vorticity and curvature (logical storage) move accordingly; the physical arrays are
+// compacted the same way on their own order, and every surviving particle above newN gets its new
+// logical index through `remap`.
+int compaction_scans(vare_flip* h, const uint8_t* mask, int n, int newn, int& nmove) {
+    OpHoles oh{mask, h->hole};
+    CHK(run_scan(h, oh, newn, 0));
+    OpSurvivors os{mask, newn, h->src};
+    CHK(run_scan(h, os, n - newn, 2));
+    HIPCHK(hipMemcpyAsync(h->dint_host, h->dint, 3 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    nmove = h->dint_host[0];
+    if (nmove != h->dint_host[2])
+        return fail(VARE_EINVAL, "removal compaction mismatch (%d holes, %d survivors)", nmove, h->dint_host[2]);
     return VARE_OK;
 }
 
@@ -782,18 +860,34 @@ int stage_remove(vare_flip* h) {
     if (nrem <= 0) return VARE_OK;
     const int n = (int)h->np;
     const int newn = n - nrem;
-    OpHoles oh{h->removed, h->hole};
-    CHK(run_scan(h, oh, newn, 0));
-    OpSurvivors os{h->removed, newn, h->src};
-    CHK(run_scan(h, os, n - newn, 2));
-    HIPCHK(hipMemcpyAsync(h->dint_host, h->dint, 3 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    const int nmove = h->dint_host[0];
-    if (nmove != h->dint_host[2]) return fail(VARE_EINVAL, "removal compaction mismatch (%d holes, %d survivors)", nmove, h->dint_host[2]);
-    const PArrays pa = h->parrays();
-    if (nmove)
+    // logical order: vorticity and curvature, and the new index of each moved survivor
+    int nmove = 0;
+    CHK(compaction_scans(h, h->removed_log, n, newn, nmove));
+    if (nmove) {
+        MoveSet lg{};
+        lg.a[0] = h->pa[7]; lg.a[1] = h->pa[8];
+        lg.esz[0] = 4; lg.esz[1] = 4;
+        lg.n = 2;
         hipLaunchKernelGGL(k_remove_move, dim3(nblocks(nmove)), dim3(WG), 0, h->stream, nmove, (const int*)h->hole,
-                           (const int*)h->src, pa);
+                           (const int*)h->src, lg);
+        hipLaunchKernelGGL(k_remap_build, dim3(nblocks(nmove)), dim3(WG), 0, h->stream, nmove, newn,
+                           (const int*)h->hole, (const int*)h->src, h->remap);
+    }
+    // physical order: positions, velocities, adjacency and logical indices
+    int pmove = 0;
+    CHK(compaction_scans(h, h->removed, n, newn, pmove));
+    if (pmove) {
+        MoveSet ph{};
+        for (int a = 0; a < 7; ++a) { ph.a[a] = h->pa[a]; ph.esz[a] = (a >= 3 && a < 6) ? (int)h->vsz() : 4; }
+        ph.a[7] = h->pid;
+        ph.esz[7] = 4;
+        ph.n = 8;
+        hipLaunchKernelGGL(k_remove_move, dim3(nblocks(pmove)), dim3(WG), 0, h->stream, pmove, (const int*)h->hole,
+                           (const int*)h->src, ph);
+    }
+    if (nmove && newn)
+        hipLaunchKernelGGL(k_pid_remap, dim3(nblocks(newn)), dim3(WG), 0, h->stream, newn, newn,
+                           (const int*)h->remap, h->pid);
     HIPCHK(hipGetLastError());
     h->np = (size_t)newn;
     return VARE_OK;
@@ -964,8 +1058,9 @@ int vare_flip_destroy(vare_flip* h) {
     if (!h) return VARE_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (int q = 0; q < 9; ++q) dfree(h->pa[q]);
-    dfree(h->removed); dfree(h->hole); dfree(h->src);
-    dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins);
+    for (int q = 0; q < 7; ++q) dfree(h->pb[q]);
+    dfree(h->pid); dfree(h->pid2); dfree(h->removed); dfree(h->removed_log); dfree(h->hole); dfree(h->src);
+    dfree(h->remap); dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins);
     dfree(h->flag); dfree(h->slot); dfree(h->coord);
     free_cells(h);
     free_collision(h);
@@ -1029,6 +1124,8 @@ static int upload_particles(vare_flip* h, size_t off, const float* pos, const fl
     }
     hipLaunchKernelGGL(k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, mem == VARE_MEM_HOST ? dv : vel,
                        h->vptr(0, off), h->vptr(1, off), h->vptr(2, off), h->vhalf);
+    // new particles are stored at the end in their reference order
+    hipLaunchKernelGGL(k_iota, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, (int)off, h->pid + off);
     if (space == VARE_SPACE_WORLD)
         hipLaunchKernelGGL(k_to_voxel, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, h->parrays(off), h->t[0],
                            h->t[1], h->t[2], h->s[0], h->s[1], h->s[2]);
@@ -1066,27 +1163,30 @@ int vare_flip_get_particles(vare_flip* h, float* pos, float* vel, float* adj, fl
     const size_t np = h->np;
     if (np == 0) return VARE_OK;
     const int world = space == VARE_SPACE_WORLD ? 1 : 0;
-    float* sc[3] = {adj, vort, curv};
+    float* lg[2] = {vort, curv};                  // stored in the reference order already
     if (mem == VARE_MEM_HOST) {
         float* d = nullptr;
-        CHK(dalloc(&d, 6 * np));
-        hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(), world, h->t[0],
-                           h->t[1], h->t[2], h->s[0], h->s[1], h->s[2], pos ? d : nullptr, vel ? d + 3 * np : nullptr);
+        CHK(dalloc(&d, 7 * np));
+        hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(),
+                           (const int*)h->pid, world, h->t[0], h->t[1], h->t[2], h->s[0], h->s[1], h->s[2],
+                           pos ? d : nullptr, vel ? d + 3 * np : nullptr, adj ? d + 6 * np : nullptr);
         int rc = hipStreamSynchronize(h->stream) == hipSuccess ? VARE_OK : fail(VARE_EHIP, "particle export failed");
         if (rc == VARE_OK && pos && hipMemcpy(pos, d, 3 * np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(VARE_EHIP, "particle download failed");
         if (rc == VARE_OK && vel && hipMemcpy(vel, d + 3 * np, 3 * np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(VARE_EHIP, "particle download failed");
-        for (int q = 0; q < 3 && rc == VARE_OK; ++q)
-            if (sc[q] && hipMemcpy(sc[q], h->pa[6 + q], np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        if (rc == VARE_OK && adj && hipMemcpy(adj, d + 6 * np, np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(VARE_EHIP, "particle download failed");
+        for (int q = 0; q < 2 && rc == VARE_OK; ++q)
+            if (lg[q] && hipMemcpy(lg[q], h->pa[7 + q], np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
                 rc = fail(VARE_EHIP, "particle download failed");
         dfree(d);
         return rc;
     }
-    hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(), world, h->t[0],
-                       h->t[1], h->t[2], h->s[0], h->s[1], h->s[2], pos, vel);
-    for (int q = 0; q < 3; ++q)
-        if (sc[q]) HIPCHK(hipMemcpyAsync(sc[q], h->pa[6 + q], np * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(), (const int*)h->pid,
+                       world, h->t[0], h->t[1], h->t[2], h->s[0], h->s[1], h->s[2], pos, vel, adj);
+    for (int q = 0; q < 2; ++q)
+        if (lg[q]) HIPCHK(hipMemcpyAsync(lg[q], h->pa[7 + q], np * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     return VARE_OK;
@@ -1235,9 +1335,9 @@ int vare_flip_get_binning(vare_flip* h, int32_t* perm, int32_t* bstart, size_t c
     if (n) *n = h->np;
     if (nbucket) *nbucket = h->ncells() + 1;
     if (h->prm.particle_path != 0) return fail(VARE_EINVAL, "binning is only built on particle_path 0");
-    if (perm) {
+    if (perm) {                                   // the physical order is the bin order: its logical indices
         if (cap < h->np) return fail(VARE_EINVAL, "capacity too small");
-        if (h->np) HIPCHK(hipMemcpy(perm, h->perm, h->np * sizeof(int), hipMemcpyDeviceToHost));
+        if (h->np) HIPCHK(hipMemcpy(perm, h->pid, h->np * sizeof(int), hipMemcpyDeviceToHost));
     }
     if (bstart) HIPCHK(hipMemcpy(bstart, h->cstart, (h->ncells() + 2) * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(h->dint_host + 4, h->dint + 4, sizeof(int), hipMemcpyDeviceToHost));
@@ -1251,7 +1351,7 @@ int vare_flip_get_removed_mask(vare_flip* h, uint8_t* out, size_t cap, size_t* n
     if (n) *n = m;
     if (!out) return VARE_OK;
     if (cap < m) return fail(VARE_EINVAL, "capacity too small");
-    if (m) HIPCHK(hipMemcpy(out, h->removed, m, hipMemcpyDeviceToHost));
+    if (m) HIPCHK(hipMemcpy(out, h->removed_log, m, hipMemcpyDeviceToHost));
     return VARE_OK;
 }
 

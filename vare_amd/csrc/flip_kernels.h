@@ -622,16 +622,52 @@ struct PArrays {
     int half;
     __device__ __forceinline__ VelPtr vel() const { return VelPtr{{a[3], a[4], a[5]}, half}; }
 };
-__global__ void k_remove_move(int nmove, const int* __restrict__ hole, const int* __restrict__ src, PArrays pa) {
+// Arrays moved together by one compaction: up to 8, each of 4- or 2-byte elements.
+struct MoveSet {
+    void* a[8];
+    int esz[8];
+    int n;
+};
+// The k-th hole (ascending) receives the k-th survivor from the end (descending): the closed form of
+// Array::remove's swap-from-back loop (Array.h:995-1024).
+__global__ void k_remove_move(int nmove, const int* __restrict__ hole, const int* __restrict__ src, MoveSet ms) {
     const int k = blockIdx.x * WG + threadIdx.x;
     if (k >= nmove) return;
     const int d = hole[k];
     const int s = src[nmove - 1 - k];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        if (pa.half && q >= 3 && q < 6) ((_Float16*)pa.a[q])[d] = ((const _Float16*)pa.a[q])[s];
-        else pa.a[q][d] = pa.a[q][s];
+    for (int q = 0; q < 8; ++q) {
+        if (q >= ms.n) break;
+        if (ms.esz[q] == 2) ((_Float16*)ms.a[q])[d] = ((const _Float16*)ms.a[q])[s];
+        else ((float*)ms.a[q])[d] = ((const float*)ms.a[q])[s];
     }
+}
+
+// Logical removal mask from the physical one (G2P writes flags per physical particle).
+__global__ void k_removed_logical(int np, const uint8_t* __restrict__ removed, const int* __restrict__ pid,
+                                  uint8_t* __restrict__ removed_log) {
+    const int q = blockIdx.x * WG + threadIdx.x;
+    if (q >= np) return;
+    if (removed[q]) removed_log[pid[q]] = 1;
+}
+
+// New logical index of every survivor at or above newN: the one the compaction moves into hole k.
+__global__ void k_remap_build(int nmove, int newn, const int* __restrict__ hole, const int* __restrict__ src,
+                              int* __restrict__ remap) {
+    const int k = blockIdx.x * WG + threadIdx.x;
+    if (k >= nmove) return;
+    remap[src[nmove - 1 - k] - newn] = hole[k];
+}
+__global__ void k_pid_remap(int n, int newn, const int* __restrict__ remap, int* __restrict__ pid) {
+    const int q = blockIdx.x * WG + threadIdx.x;
+    if (q >= n) return;
+    const int l = pid[q];
+    if (l >= newn) pid[q] = remap[l - newn];
+}
+
+__global__ void k_iota(int n, int base, int* __restrict__ out) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i < n) out[i] = base + i;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -661,19 +697,24 @@ __global__ void k_to_world(int np, PArrays pa, float tx, float ty, float tz, flo
 
 // Particle export: interleaved FP32 positions/velocities (voxel space, or world space when world
 // != 0: Grid::worldPoint / worldVector, Grid.h:431-443).  Either output may be null.
-__global__ void k_export(int np, PArrays pa, int world, float tx, float ty, float tz, float sx, float sy, float sz,
-                         float* __restrict__ pos, float* __restrict__ vel) {
-    const int n = blockIdx.x * WG + threadIdx.x;
-    if (n >= np) return;
+// Particle export in the reference's order: physical particle q goes to slot pid[q] of the
+// xyz-interleaved outputs (and of adj); optional world transform.
+__global__ void k_export(int np, PArrays pa, const int* __restrict__ pid, int world, float tx, float ty, float tz,
+                         float sx, float sy, float sz, float* __restrict__ pos, float* __restrict__ vel,
+                         float* __restrict__ adj) {
+    const int q = blockIdx.x * WG + threadIdx.x;
+    if (q >= np) return;
+    const int n = pid[q];
     const VelPtr u = pa.vel();
-    float x = pa.a[0][n], y = pa.a[1][n], z = pa.a[2][n];
-    float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
+    float x = pa.a[0][q], y = pa.a[1][q], z = pa.a[2][q];
+    float vx = u.ld(0, q), vy = u.ld(1, q), vz = u.ld(2, q);
     if (world) {
         x = x * sx + tx; y = y * sy + ty; z = z * sz + tz;
         vx = vx * sx; vy = vy * sy; vz = vz * sz;
     }
     if (pos) { pos[3 * n] = x; pos[3 * n + 1] = y; pos[3 * n + 2] = z; }
     if (vel) { vel[3 * n] = vx; vel[3 * n + 1] = vy; vel[3 * n + 2] = vz; }
+    if (adj) adj[n] = pa.a[6][q];
 }
 
 // xyz-interleaved FP32 <-> SoA on the device (for device-pointer particle I/O and FP16 velocity

@@ -1,20 +1,23 @@
-// flip_particles.h -- cell-binned particle kernels: the binning permutation (a stable counting sort
-// of particle indices by the storage index of their P2G base cell), P2G as a gather per block
-// (deterministic: no float atomics), and G2P gathering from an LDS tile per block.
+// flip_particles.h -- cell-binned particle kernels: the binning (a stable counting sort of the
+// particles by the storage index of their P2G base cell, ties in the reference's particle order),
+// the reorder of the particle arrays into that order, P2G as a gather per block (deterministic: no
+// float atomics), and G2P from an LDS tile per block.
 //
-// The particle arrays stay in the reference's order (Array::remove order); the permutation only
-// says in which order each block's workgroup visits them.  A particle's P2G base cell is
-// c = max((int)(p-0.5), 0) per axis (Rasterization.cu:139-147).  For a particle inside [0,n]^3 the
-// block holding c is always active: it is the particle's block or the one below it, both inside the
-// 3x3x3 neighbourhood that SparseFrame.cu:145-152 marks.  Particles outside [0,n]^3 (skipped by P2G)
-// go to an extra bin Nc, handled by G2P with global-memory lookups.
+// Storage: positions, velocities and adjacency live in physical order, which each step's binning
+// sets to bin order so that a block's particles are contiguous; pid[q] is the reference (logical)
+// index of physical particle q.  Vorticity and curvature stay in logical order.  A particle's P2G
+// base cell is c = max((int)(p-0.5), 0) per axis (Rasterization.cu:139-147).  For a particle inside
+// [0,n]^3 the block holding c is always active: it is the particle's block or the one below it,
+// both inside the 3x3x3 neighbourhood that SparseFrame.cu:145-152 marks.  Particles outside [0,n]^3
+// (skipped by P2G) go to an extra bin Nc, handled by G2P with global-memory lookups.
 #pragma once
 #include "flip_kernels.h"
 
 namespace vare {
 
-constexpr int kSortMax = 16384;     // longest bin sorted in LDS (64 KB)
+constexpr int kSortMax = 8192;      // longest bin sorted in LDS (64 KB of 64-bit keys)
 constexpr int kSortReg = 16;        // bins up to this length are sorted in registers
+constexpr unsigned long long kSortPad = ~0ull;
 
 // Bin of every particle (storage index of its base cell, or nc) + its arrival rank inside the bin.
 // Lanes holding equal keys in a row form a run; the first lane of every run reserves the whole run
@@ -59,17 +62,20 @@ struct OpBuckets {
     __device__ void emit(int b, int pre, int v) const { start[b] = pre; }
 };
 
+// Sort key of physical particle q: its logical index above, q below (both < 2^31).
 __global__ void k_pscatter(int np, const int* __restrict__ key, const int* __restrict__ rank,
-                           const int* __restrict__ bstart, int* __restrict__ perm) {
+                           const int* __restrict__ bstart, const int* __restrict__ pid,
+                           unsigned long long* __restrict__ perm) {
     const int i = blockIdx.x * WG + threadIdx.x;
     if (i >= np) return;
-    perm[bstart[key[i]] + rank[i]] = i;
+    perm[bstart[key[i]] + rank[i]] = ((unsigned long long)(unsigned)pid[i] << 32) | (unsigned)i;
 }
 
-// Sort every bin's indices ascending, which turns the permutation into a stable counting sort that
-// does not depend on the atomic arrival order.  One thread per bin: bins of up to kSortReg entries
-// are sorted in registers (bitonic network); longer bins are listed for k_sort_long.
-__global__ void k_sort_short(const int* __restrict__ bstart, int nbins, int* __restrict__ perm,
+// Sort every bin's keys ascending, i.e. by logical index, which turns the permutation into a stable
+// counting sort of the reference order that does not depend on the atomic arrival order.  One
+// thread per bin: bins of up to kSortReg entries are sorted in registers (bitonic network); longer
+// bins are listed for k_sort_long.
+__global__ void k_sort_short(const int* __restrict__ bstart, int nbins, unsigned long long* __restrict__ perm,
                              int* __restrict__ longbins, int* __restrict__ nlong) {
     const int c = blockIdx.x * WG + threadIdx.x;
     if (c >= nbins) return;
@@ -79,9 +85,9 @@ __global__ void k_sort_short(const int* __restrict__ bstart, int nbins, int* __r
         longbins[atomicAdd(nlong, 1)] = c;
         return;
     }
-    int v[kSortReg];
+    unsigned long long v[kSortReg];
 #pragma unroll
-    for (int q = 0; q < kSortReg; ++q) v[q] = q < n ? perm[s0 + q] : 0x7fffffff;
+    for (int q = 0; q < kSortReg; ++q) v[q] = q < n ? perm[s0 + q] : kSortPad;
     bool sorted = true;
 #pragma unroll
     for (int q = 0; q + 1 < kSortReg; ++q) sorted = sorted && v[q] < v[q + 1];
@@ -94,7 +100,7 @@ __global__ void k_sort_short(const int* __restrict__ bstart, int nbins, int* __r
             for (int t = 0; t < kSortReg; ++t) {
                 const int u = t ^ j;
                 if (u > t) {
-                    const int lo = rmin(v[t], v[u]), hi = rmax(v[t], v[u]);
+                    const unsigned long long lo = v[t] < v[u] ? v[t] : v[u], hi = v[t] < v[u] ? v[u] : v[t];
                     if ((t & k) == 0) { v[t] = lo; v[u] = hi; } else { v[t] = hi; v[u] = lo; }
                 }
             }
@@ -107,9 +113,9 @@ __global__ void k_sort_short(const int* __restrict__ bstart, int nbins, int* __r
 // grid is fixed and strides over the list, whose length is read on the device.  Bins longer than
 // kSortMax keep their arrival order and are counted in *overflow.
 __global__ void __launch_bounds__(1024) k_sort_long(const int* __restrict__ bstart, const int* __restrict__ longbins,
-                                                    const int* __restrict__ nlong, int* __restrict__ perm,
-                                                    int* __restrict__ overflow) {
-    __shared__ int sh[kSortMax];
+                                                    const int* __restrict__ nlong,
+                                                    unsigned long long* __restrict__ perm, int* __restrict__ overflow) {
+    __shared__ unsigned long long sh[kSortMax];
     const int nl = *nlong;
     for (int e = blockIdx.x; e < nl; e += gridDim.x) {
         const int b = longbins[e];
@@ -124,14 +130,14 @@ __global__ void __launch_bounds__(1024) k_sort_long(const int* __restrict__ bsta
         }
         int m = 1;
         while (m < n) m <<= 1;
-        for (int t = threadIdx.x; t < m; t += blockDim.x) sh[t] = t < n ? perm[s0 + t] : 0x7fffffff;
+        for (int t = threadIdx.x; t < m; t += blockDim.x) sh[t] = t < n ? perm[s0 + t] : kSortPad;
         __syncthreads();
         for (int k = 2; k <= m; k <<= 1) {
             for (int j = k >> 1; j > 0; j >>= 1) {
                 for (int t = threadIdx.x; t < m; t += blockDim.x) {
                     const int u = t ^ j;
                     if (u > t) {
-                        const int a = sh[t], c = sh[u];
+                        const unsigned long long a = sh[t], c = sh[u];
                         const bool up = (t & k) == 0;
                         if ((a > c) == up) { sh[t] = c; sh[u] = a; }
                     }
@@ -141,6 +147,28 @@ __global__ void __launch_bounds__(1024) k_sort_long(const int* __restrict__ bsta
         }
         for (int t = threadIdx.x; t < n; t += blockDim.x) perm[s0 + t] = sh[t];
         __syncthreads();
+    }
+}
+
+// Physical reorder into bin order: new particle q takes the old particle in the low half of
+// perm[q] and its logical index from the high half.  `vsz` is the byte size of the velocity
+// elements (4, or 2 for FP16 storage).
+struct ReorderArrays {
+    const void* src[7];     // px, py, pz, vx, vy, vz, adj
+    void* dst[7];
+    int vsz;
+};
+__global__ void k_reorder(int np, const unsigned long long* __restrict__ perm, ReorderArrays ra,
+                          int* __restrict__ pid_out) {
+    const int q = blockIdx.x * WG + threadIdx.x;
+    if (q >= np) return;
+    const unsigned long long k = perm[q];
+    const int o = (int)(unsigned)(k & 0xffffffffull);
+    pid_out[q] = (int)(k >> 32);
+#pragma unroll
+    for (int a = 0; a < 7; ++a) {
+        if (a >= 3 && a < 6 && ra.vsz == 2) ((_Float16*)ra.dst[a])[q] = ((const _Float16*)ra.src[a])[o];
+        else ((float*)ra.dst[a])[q] = ((const float*)ra.src[a])[o];
     }
 }
 
@@ -181,7 +209,7 @@ template <int B> struct P2GGather {
 // P2G + count field + normalisation + classification of one active block, as a gather
 // (Rasterization.cu:121-190 and :198-230, FLIPMethod.cu:242-280 with curl disabled).
 // Each thread sums the 8 corner contributions of its base cell's particles (bin order, i.e.
-// ascending particle index) in registers.  The block's cells then receive the corners in 8 passes
+// ascending reference index) in registers.  The block's cells then receive the corners in 8 passes
 // (corner offset 000, 100, 010, ... in turn): within one pass distinct base cells hit distinct
 // cells, so the LDS updates need no atomics and every cell sums its terms in a fixed order -- the
 // result is bit-reproducible.  A corner past n-1 is skipped (the Min(c+1, n-1) bound); corners in
@@ -190,7 +218,7 @@ template <int B> struct P2GGather {
 template <int B>
 __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
         const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, VelPtr u,
-        const int* __restrict__ perm, const int* __restrict__ cstart, FrameK f, CellFields cf,
+        const int* __restrict__ cstart, FrameK f, CellFields cf,
         float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz, uint8_t* __restrict__ type, ColK col,
         float col_scale) {
     constexpr int B1 = B + 1, NBASE = B1 * B1 * B1, B3 = B * B * B;
@@ -214,8 +242,7 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
             const int cell = sl * B3 + ((lk - dz * B) * B + (lj - dy * B)) * B + (li - dx * B);
             const int q0 = cstart[cell], q1 = cstart[cell + 1];
             const int ci = ox + li, cj = oy + lj, ck = oz + lk;
-            for (int q = q0; q < q1; ++q) {
-                const int n = perm[q];
+            for (int n = q0; n < q1; ++n) {              // the bin's particles are contiguous
                 const float x = px[n], y = py[n], z = pz[n];
                 const float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
 #pragma unroll
@@ -387,7 +414,7 @@ struct G2PFields {
 // outside the domain) and reads the fields through the block table.
 template <int B>
 __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
-                                                VelPtr u, float* __restrict__ adj, const int* __restrict__ perm,
+                                                VelPtr u, float* __restrict__ adj,
                                                 const int* __restrict__ cstart, int nb, FrameK f, G2PFields gf,
                                                 G2PParams gp, ColK col, uint8_t* __restrict__ removed,
                                                 int* __restrict__ nremoved) {
@@ -407,7 +434,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* 
             for (int fi = 0; fi < 7; ++fi) v[fi][q] = c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
         };
         for (int q = p0 + threadIdx.x; q < p1; q += WG)
-            rem_count += g2p_particle<B>(perm[q], px, py, pz, u, adj, f, gp, col, removed, fetch);
+            rem_count += g2p_particle<B>(q, px, py, pz, u, adj, f, gp, col, removed, fetch);
     } else {
         load_nbhd(f, s, nsl);
         __syncthreads();
@@ -432,7 +459,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* 
             }
         };
         for (int q = p0 + threadIdx.x; q < p1; q += WG)
-            rem_count += g2p_particle<B>(perm[q], px, py, pz, u, adj, f, gp, col, removed, fetch);
+            rem_count += g2p_particle<B>(q, px, py, pz, u, adj, f, gp, col, removed, fetch);
     }
     // wave-aggregated removal count
     const int w = wave_sum_i(rem_count);
