@@ -101,7 +101,8 @@ struct vare_flip {
     int* pkey = nullptr;          // bin (storage index of the particle's base cell, or Nc)
     int* prank = nullptr;         // arrival rank inside the bin
     unsigned long long* perm = nullptr;   // binning: (logical index << 32 | physical index) per bin slot
-    int* longbins = nullptr;      // bins longer than kSortReg, [pcap/(kSortReg+1)+2]
+    int* longbins = nullptr;      // bins longer than 64, [pcap/(kSortReg+1)+2]
+    int* midbins = nullptr;       // bins of kSortReg+1 .. 64 particles, same capacity
     int* ccount = nullptr;        // particles per bin, [bcap*B^3+2]
     int* cstart = nullptr;        // bin starts, [bcap*B^3+2]
     size_t last_np_before_remove = 0;
@@ -141,7 +142,7 @@ struct vare_flip {
     size_t tiles_cap = 0;
     int* dint = nullptr;          // [0] scan total, [1] removed count, [2] second scan total,
                                   // [3] bin total, [4] bins left unsorted, [5] long-bin count,
-                                  // [6] solver block count
+                                  // [6] solver block count, [7] mid-bin count
     CgState* st = nullptr;
     CgState* st_host = nullptr;   // pinned mirror
     int* dint_host = nullptr;     // pinned mirror of dint
@@ -266,7 +267,7 @@ int ensure_particles(vare_flip* h, size_t n) {
     h->pid = npid;
     for (int q = 0; q < 7; ++q) dfree(h->pb[q]);
     dfree(h->pid2); dfree(h->removed); dfree(h->removed_log); dfree(h->hole); dfree(h->src); dfree(h->remap);
-    dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins);
+    dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins); dfree(h->midbins);
     for (int q = 0; q < 7; ++q) {
         char* p = nullptr;
         CHK(dalloc(&p, cap * ((q >= 3 && q < 6) ? h->vsz() : sizeof(float))));
@@ -277,6 +278,7 @@ int ensure_particles(vare_flip* h, size_t n) {
     CHK(dalloc(&h->hole, cap)); CHK(dalloc(&h->src, cap)); CHK(dalloc(&h->remap, cap));
     CHK(dalloc(&h->pkey, cap)); CHK(dalloc(&h->prank, cap)); CHK(dalloc(&h->perm, cap));
     CHK(dalloc(&h->longbins, cap / (kSortReg + 1) + 2));
+    CHK(dalloc(&h->midbins, cap / (kSortReg + 1) + 2));
     h->pcap = cap;
     return VARE_OK;
 }
@@ -385,6 +387,7 @@ template <int B> int bin_particles(vare_flip* h) {
     const int nc = (int)h->ncells();
     CHK(fill_u32(h, h->ccount, 0, (size_t)nc + 2));
     CHK(fill_u32(h, h->dint + 4, 0, 2));           // [4] sort overflow, [5] long-bin count
+    CHK(fill_u32(h, h->dint + 7, 0, 1));           // [7] mid-bin count
     if (h->np)
         hipLaunchKernelGGL(k_ckey<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (const float*)h->pa[0],
                            (const float*)h->pa[1], (const float*)h->pa[2], (int)h->np, h->frame(), nc, h->pkey, h->prank,
@@ -395,7 +398,9 @@ template <int B> int bin_particles(vare_flip* h) {
         hipLaunchKernelGGL(k_pscatter, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, (const int*)h->pkey,
                            (const int*)h->prank, (const int*)h->cstart, (const int*)h->pid, h->perm);
         hipLaunchKernelGGL(k_sort_short, dim3(nblocks((size_t)nc + 1)), dim3(WG), 0, h->stream, (const int*)h->cstart,
-                           nc + 1, h->perm, h->longbins, h->dint + 5);
+                           nc + 1, h->perm, h->midbins, h->dint + 7, h->longbins, h->dint + 5);
+        hipLaunchKernelGGL(k_sort_wave, dim3(kSortLongGrid * 4), dim3(WG), 0, h->stream, (const int*)h->cstart,
+                           (const int*)h->midbins, (const int*)(h->dint + 7), h->perm);
         hipLaunchKernelGGL(k_sort_long, dim3(kSortLongGrid), dim3(1024), 0, h->stream, (const int*)h->cstart,
                            (const int*)h->longbins, (const int*)(h->dint + 5), h->perm, h->dint + 4);
         // move the particles into bin order (double-buffered) and keep their logical indices
@@ -1060,7 +1065,7 @@ int vare_flip_destroy(vare_flip* h) {
     for (int q = 0; q < 9; ++q) dfree(h->pa[q]);
     for (int q = 0; q < 7; ++q) dfree(h->pb[q]);
     dfree(h->pid); dfree(h->pid2); dfree(h->removed); dfree(h->removed_log); dfree(h->hole); dfree(h->src);
-    dfree(h->remap); dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins);
+    dfree(h->remap); dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins); dfree(h->midbins);
     dfree(h->flag); dfree(h->slot); dfree(h->coord);
     free_cells(h);
     free_collision(h);

@@ -73,16 +73,18 @@ __global__ void k_pscatter(int np, const int* __restrict__ key, const int* __res
 
 // Sort every bin's keys ascending, i.e. by logical index, which turns the permutation into a stable
 // counting sort of the reference order that does not depend on the atomic arrival order.  One
-// thread per bin: bins of up to kSortReg entries are sorted in registers (bitonic network); longer
-// bins are listed for k_sort_long.
+// thread per bin: bins of up to kSortReg entries are sorted in registers (bitonic network); bins of
+// up to 64 are listed for k_sort_wave, longer ones for k_sort_long.
 __global__ void k_sort_short(const int* __restrict__ bstart, int nbins, unsigned long long* __restrict__ perm,
-                             int* __restrict__ longbins, int* __restrict__ nlong) {
+                             int* __restrict__ midbins, int* __restrict__ nmid, int* __restrict__ longbins,
+                             int* __restrict__ nlong) {
     const int c = blockIdx.x * WG + threadIdx.x;
     if (c >= nbins) return;
     const int s0 = bstart[c], n = bstart[c + 1] - s0;
     if (n <= 1) return;
     if (n > kSortReg) {
-        longbins[atomicAdd(nlong, 1)] = c;
+        if (n <= 64) midbins[atomicAdd(nmid, 1)] = c;
+        else longbins[atomicAdd(nlong, 1)] = c;
         return;
     }
     unsigned long long v[kSortReg];
@@ -109,7 +111,32 @@ __global__ void k_sort_short(const int* __restrict__ bstart, int nbins, unsigned
         if (q < n) perm[s0 + q] = v[q];
 }
 
-// Bins longer than kSortReg: one 1024-thread workgroup per listed bin, bitonic sort in LDS.  The
+// Bins of kSortReg+1 .. 64 entries: one wave per bin, one key per lane, bitonic sort across the
+// lanes with butterfly shuffles (no LDS, no barrier).  The grid is fixed and strides over the list.
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+    const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)(v & 0xffffffffull), m, 64);
+    const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(v >> 32), m, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__global__ void __launch_bounds__(WG) k_sort_wave(const int* __restrict__ bstart, const int* __restrict__ bins,
+                                                  const int* __restrict__ nbins, unsigned long long* __restrict__ perm) {
+    const int lane = threadIdx.x & 63;
+    const int nw = *nbins;
+    for (int e = blockIdx.x * (WG / 64) + (threadIdx.x >> 6); e < nw; e += gridDim.x * (WG / 64)) {
+        const int b = bins[e];
+        const int s0 = bstart[b], n = bstart[b + 1] - s0;
+        unsigned long long v = lane < n ? perm[s0 + lane] : kSortPad;
+        for (int k = 2; k <= 64; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const unsigned long long o = shfl_xor_u64(v, j);
+                const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+                v = keep_min ? (o < v ? o : v) : (o < v ? v : o);
+            }
+        if (lane < n) perm[s0 + lane] = v;
+    }
+}
+
+// Bins longer than 64: one 1024-thread workgroup per listed bin, bitonic sort in LDS.  The
 // grid is fixed and strides over the list, whose length is read on the device.  Bins longer than
 // kSortMax keep their arrival order and are counted in *overflow.
 __global__ void __launch_bounds__(1024) k_sort_long(const int* __restrict__ bstart, const int* __restrict__ longbins,
