@@ -65,6 +65,15 @@ struct FrameK {
     const int* nbr;          // [Nb*6]
 };
 
+// MI355X deals workgroups to its 8 XCDs round-robin by id, and each XCD has its own L2.  This
+// bijection of [0, g) gives each XCD a contiguous range of logical workgroup indices, so blocks that
+// are neighbours in storage order (and share halos) run on the same XCD.
+__device__ __forceinline__ int xcd_swizzle(int w, int g) {
+    constexpr int X = 8;
+    const int x = w % X, i = w / X, q = g / X, r = g % X;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 template <int B> struct Blk {
     static constexpr int B2 = B * B;
     static constexpr int B3 = B * B * B;

@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
     constexpr int B1 = B + 1, NBASE = B1 * B1 * B1, B3 = B * B * B;
     __shared__ float tm[B3], tx[B3], ty[B3], tz[B3], tc[B3];
     __shared__ int nsl[27];
-    const int s = blockIdx.x;
+    const int s = xcd_swizzle(blockIdx.x, gridDim.x);
     const int t = threadIdx.x;
     load_nbhd(f, s, nsl);
     for (int q = t; q < B3; q += blockDim.x) { tm[q] = 0.f; tx[q] = 0.f; ty[q] = 0.f; tz[q] = 0.f; tc[q] = 0.f; }
@@ -448,7 +448,7 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* 
     constexpr int T = B + 2, T3 = T * T * T, B3 = B * B * B;
     __shared__ float tile[7][T3];
     __shared__ int nsl[27];
-    const int s = blockIdx.x;
+    const int s = xcd_swizzle(blockIdx.x, gridDim.x);
     const int p0 = cstart[s * B3], p1 = s < nb ? cstart[(s + 1) * B3] : cstart[nb * B3 + 1];
     if (p0 == p1) return;
     const float* src[7] = {gf.cnt, gf.cx, gf.cy, gf.cz, gf.vx0, gf.vy0, gf.vz0};

@@ -27,13 +27,14 @@ template <int B>
 __device__ __forceinline__ void wg_blocks(const int* __restrict__ blist, int nsb, const int* __restrict__ nbr,
                                           int (&s)[Fine<B>::K], int* fnb) {
     constexpr int K = Fine<B>::K;
+    const int w = xcd_swizzle(blockIdx.x, gridDim.x);
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
-        const int v = blockIdx.x * K + kb;
+        const int v = w * K + kb;
         s[kb] = v < nsb ? blist[v] : -1;
     }
     if (threadIdx.x < 6 * K) {
-        const int v = blockIdx.x * K + threadIdx.x / 6;
+        const int v = w * K + threadIdx.x / 6;
         fnb[threadIdx.x] = v < nsb ? nbr[blist[v] * 6 + threadIdx.x % 6] : -1;
     }
 }
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict_
         v.ap[s[kb] * B3 + l] = acc;
     }
     const double tot = fine_block_sum<B>(pap, sh);
-    if (l == 0) partials[blockIdx.x] = tot;
+    if (l == 0) partials[blockIdx.x] = tot;     // any fixed slot per workgroup keeps the sum order fixed
 }
 
 // ---- multigrid, level 0 (the CG residual r, stencil descriptor info) ----
