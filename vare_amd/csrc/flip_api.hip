@@ -476,7 +476,7 @@ template <int B> int stage_p2g(vare_flip* h) {
 CgVecs vecs(vare_flip* h) { return CgVecs{h->x, h->r, h->p, h->ap, h->b, h->z, h->info}; }
 
 int mg_setup(vare_flip* h);
-void enqueue_vcycle(vare_flip* h);
+void enqueue_vcycle(vare_flip* h, double* rz = nullptr);
 
 // z = M^-1 r (multigrid: already in z), p = z and r.z over the solver's blocks.
 void enqueue_precond_init(vare_flip* h) {
@@ -610,7 +610,8 @@ template <int E> void enqueue_block_down(vare_flip* h, int l, int nu) {
     }
 }
 
-template <int E> void enqueue_block_up(vare_flip* h, int l, int nu) {
+// rz != nullptr (level 0 only): the last pass also writes per-workgroup partials of r.z.
+template <int E> void enqueue_block_up(vare_flip* h, int l, int nu, double* rz) {
     const MgFrame F = mg_frame(h);
     const float w = h->prm.mg_omega;
     const int* done = done_ptr(h);
@@ -623,17 +624,19 @@ template <int E> void enqueue_block_up(vare_flip* h, int l, int nu) {
     if (!nwg) return;
     if (nu == 1) {
         hipLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w,
-                           (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done);
+                           (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done, rz);
     } else {
         hipLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w,
-                           (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done);
+                           (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done,
+                           (double*)nullptr);
         hipLaunchKernelGGL(k_mg0_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, F, w,
-                           (const float*)h->mg_zt[l], h->mg_z[l], done);
+                           (const float*)h->mg_zt[l], h->mg_z[l], done, rz);
     }
 }
 
-// z = V-cycle(r) into the CG's z vector.
-template <int B> void enqueue_vcycle_t(vare_flip* h) {
+// z = V-cycle(r) into the CG's z vector; with rz != nullptr the last level-0 pass also writes the
+// per-workgroup partials of r.z (one per level-0 workgroup).
+template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz) {
     const MgFrame F = mg_frame(h);
     const float w = h->prm.mg_omega;
     const int nu = h->prm.mg_sweeps;
@@ -668,8 +671,8 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
         const MgLevelK& C = h->mg[l + 1];
         if (!L.ncell) continue;
         if (l < nblk_levels) {
-            if (l == 0) enqueue_block_up<B>(h, 0, nu);
-            else enqueue_block_up<4>(h, 1, nu);
+            if (l == 0) enqueue_block_up<B>(h, 0, nu, rz);
+            else enqueue_block_up<4>(h, 1, nu, (double*)nullptr);
             continue;
         }
         if (nu == 1) {
@@ -684,9 +687,9 @@ template <int B> void enqueue_vcycle_t(vare_flip* h) {
     }
 }
 
-void enqueue_vcycle(vare_flip* h) {
-    if (h->B == 4) enqueue_vcycle_t<4>(h);
-    else enqueue_vcycle_t<8>(h);
+void enqueue_vcycle(vare_flip* h, double* rz) {
+    if (h->B == 4) enqueue_vcycle_t<4>(h, rz);
+    else enqueue_vcycle_t<8>(h, rz);
 }
 
 // One CG/PCG iteration (LinearSystemSolver.cpp:29-46) over the solver's blocks; every kernel
@@ -710,10 +713,9 @@ template <int B> void enqueue_iteration(vare_flip* h) {
     hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
                        (const double*)h->partials2, (int)gs);
     if (h->prm.solver_mode == VARE_SOLVER_RELATIVE && h->prm.preconditioner == VARE_PRECOND_MG) {
-        enqueue_vcycle(h);
-        hipLaunchKernelGGL(k_cg_rz, dim3(gs), dim3(WG), 0, h->stream, nv, cl, (const float*)h->r, (const float*)h->z,
-                           h->partials, (const CgState*)h->st);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)gs);
+        // the V-cycle's last level-0 pass reduces r.z as well (one partial per workgroup)
+        enqueue_vcycle(h, h->partials);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
     }
     tbegin(h, VARE_KERNEL_PUPDATE);
     hipLaunchKernelGGL(k_cg_pupdate, dim3(gs), dim3(WG), 0, h->stream, nv, cl, vecs(h), (const CgState*)h->st);

@@ -502,20 +502,7 @@ __global__ void k_fin_update(CgState* __restrict__ st, const double* __restrict_
     }
 }
 
-// Multigrid: partial r.z after the V-cycle, then beta.
-__global__ void k_cg_rz(int nv, CellList cl, const float* __restrict__ r, const float* __restrict__ z,
-                        double* __restrict__ partials, const CgState* __restrict__ st) {
-    __shared__ double sh[4];
-    if (st->done) return;
-    double rz = 0.0;
-    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
-        const int c = cl.cell(vi);
-        rz += (double)r[c] * (double)z[c];
-    }
-    const double s = block_sum(rz, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
-
+// Multigrid: beta from the r.z partials written by the V-cycle's last level-0 pass.
 __global__ void k_fin_beta(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
     __shared__ double sh[4];
     if (st->done) return;

@@ -285,12 +285,14 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* _
                                                                    MgLevelK L, MgLevelK C, MgFrame F, float omega,
                                                                    const float* __restrict__ z,
                                                                    const float* __restrict__ zc,
-                                                                   float* __restrict__ zout, const int* done) {
+                                                                   float* __restrict__ zout, const int* done,
+                                                                   double* __restrict__ rz_partials) {
     constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
     constexpr int SH = Fine<B>::SH;
     constexpr int BC = B / 2, CSH = SH - 1, C3 = BC * BC * BC;
     __shared__ float szp[K][B3];
     __shared__ int fnb[6 * K];
+    __shared__ double sh[Fine<B>::NW];
     if (*done) return;
     const int l = threadIdx.x;
     int s[K];
@@ -301,6 +303,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* _
     };
     uint16_t in[K];
     float zp[K], rc[K];
+    double rz = 0.0;             // r.z of the output (the PCG's r.z when this is the last level-0 pass)
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         in[kb] = 0;
@@ -343,6 +346,11 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* _
             out = zp[kb] + omega * (rc[kb] - az) / d;
         }
         zout[s[kb] * B3 + l] = out;
+        rz += (double)rc[kb] * (double)out;
+    }
+    if (rz_partials) {
+        const double tot = fine_block_sum<B>(rz, sh);
+        if (l == 0) rz_partials[blockIdx.x] = tot;
     }
 }
 
@@ -350,16 +358,19 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* _
 template <int B>
 __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restrict__ blist, int nsb, MgLevelK L,
                                                            MgFrame F, float omega, const float* __restrict__ z,
-                                                           float* __restrict__ zout, const int* done) {
+                                                           float* __restrict__ zout, const int* done,
+                                                           double* __restrict__ rz_partials) {
     constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
     __shared__ float sz[K][B3];
     __shared__ int fnb[6 * K];
+    __shared__ double sh[Fine<B>::NW];
     if (*done) return;
     const int l = threadIdx.x;
     int s[K];
     wg_blocks<B>(blist, nsb, F.nbr, s, fnb);
     uint16_t in[K];
     float zc[K], rc[K];
+    double rz = 0.0;             // r.z of the output (the PCG's r.z when this is the last level-0 pass)
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         in[kb] = 0;
@@ -399,6 +410,11 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restric
             out = zc[kb] + omega * (rc[kb] - az) / d;
         }
         zout[s[kb] * B3 + l] = out;
+        rz += (double)rc[kb] * (double)out;
+    }
+    if (rz_partials) {
+        const double tot = fine_block_sum<B>(rz, sh);
+        if (l == 0) rz_partials[blockIdx.x] = tot;
     }
 }
 
