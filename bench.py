@@ -37,6 +37,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/sec + p50/p99 step latency at 1e-5 rel divergence, vs plain GPU"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per processed cell of one launch, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# on this benchmark (20 steps) and calibrated against plain 1 GiB copies (FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0); profiles/hbm_traffic_dam_break_256.md.
+SPMV_TRAFFIC_PER_CELL = 9.29      # k_spmv_blk: 2 x 2.81 + 3.66
+MG0_TRAFFIC_PER_CELL = 12.41      # k_mg0_prolong_smooth: 2 x 4.54 + 3.33
 DT = 1.0 / 240.0
 
 
@@ -154,7 +159,7 @@ def run_scene(a, L, cfg, steps):
     h.set_particles(pos, vel)
     del pos, vel
     if not a.no_kernel_timing:
-        h.set_kernel_timing(["spmv"])
+        h.set_kernel_timing(["spmv", "mg0"])
         h.kernel_timing(reset=True)
     L.device_synchronize()
 
@@ -185,16 +190,22 @@ def run_scene(a, L, cfg, steps):
     kt = h.kernel_timing(reset=True) if not a.no_kernel_timing else None
 
     roofline = None
-    if kt is not None and kt["spmv"][1] > 0:
-        # algorithmic bytes of one A*p launch: read p (4 B) + stencil descriptor (2 B) + write Ap (4 B)
-        # per active cell of the last step; neighbour values of p are re-reads of the same vector.
-        bytes_per_launch = 10.0 * stats["num_cells"]
-        avg_ms = kt["spmv"][0] / kt["spmv"][1]
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        roofline = {"kernel": "k_spmv_blk", "bound": "hbm", "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 2), "launches": kt["spmv"][1],
-                    "bytes_per_launch": bytes_per_launch}
+    if kt is not None and kt["mg0"][1] > 0 and kt["spmv"][1] > 0:
+        def line(kernel, cls, bytes_per_cell, traffic_per_cell):
+            ms, launches, cells = kt[cls]
+            bytes_total = bytes_per_cell * cells
+            achieved = bytes_total / (ms * 1e-3) / 1e9
+            return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": round(traffic_per_cell * cells / launches, 1) if traffic_per_cell else None,
+                    "avg_launch_us": round(ms * 1e3 / launches, 2), "launches": launches,
+                    "bytes_per_launch": round(bytes_total / launches, 1)}
+        # dominant kernel: the level-0 prolongation + Jacobi sweep of the V-cycle.  Algorithmic bytes
+        # per cell it processes: read z 4 + coarse z 0.5 (one per 8 cells) + r 4 + descriptor 2, write
+        # the smoothed z 4.  Neighbour values are re-reads.
+        roofline = line("k_mg0_prolong_smooth", "mg0", 14.5, MG0_TRAFFIC_PER_CELL)
+        # the PCG's A*p: read p 4 + descriptor 2, write Ap 4
+        roofline["others"] = [line("k_spmv_blk", "spmv", 10.0, SPMV_TRAFFIC_PER_CELL)]
     sm = np.array(step_ms)
     nx, ny, nz = sc["dims"]
     res = {

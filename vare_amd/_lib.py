@@ -38,8 +38,8 @@ FIELD_PRESSURE = 4
 FIELD_COUNT = 5
 FIELD_DIV = 6
 
-KERNEL_NAMES = ["spmv", "update", "p2g", "g2p", "bin", "div", "apply", "pupdate"]
-KERNEL_COUNT = 8
+KERNEL_NAMES = ["spmv", "update", "p2g", "g2p", "bin", "div", "apply", "pupdate", "mg0"]
+KERNEL_COUNT = 9
 
 ERROR_NAMES = {
     VARE_EINVAL: "VARE_EINVAL",
@@ -158,6 +158,7 @@ SIGNATURES = {
     "vare_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
     "vare_flip_set_kernel_timing": (C.c_int, [_P, C.c_int]),
     "vare_flip_get_kernel_timing": (C.c_int, [_P, C.POINTER(C.c_double * KERNEL_COUNT),
+                                              C.POINTER(C.c_int64 * KERNEL_COUNT),
                                               C.POINTER(C.c_int64 * KERNEL_COUNT), C.c_int]),
     "vare_last_error": (C.c_char_p, []),
     "vare_server_create": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(_P)]),
@@ -408,10 +409,13 @@ class FlipHandle:
         check(self.lib.vare_flip_set_kernel_timing(self.h, mask))
 
     def kernel_timing(self, reset=False):
+        """{class: (total ms, launches, work items)} since the last reset."""
         ms = (C.c_double * KERNEL_COUNT)()
         cnt = (C.c_int64 * KERNEL_COUNT)()
-        check(self.lib.vare_flip_get_kernel_timing(self.h, C.byref(ms), C.byref(cnt), 1 if reset else 0))
-        return {KERNEL_NAMES[i]: (ms[i], cnt[i]) for i in range(KERNEL_COUNT)}
+        units = (C.c_int64 * KERNEL_COUNT)()
+        check(self.lib.vare_flip_get_kernel_timing(self.h, C.byref(ms), C.byref(cnt), C.byref(units),
+                                                   1 if reset else 0))
+        return {KERNEL_NAMES[i]: (ms[i], cnt[i], units[i]) for i in range(KERNEL_COUNT)}
 
 
 class Server:

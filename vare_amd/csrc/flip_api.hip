@@ -4,6 +4,7 @@
 // the reference's FLIPMethod::advanceOneStep (base/source/FLIPMethod.cu:208-298) split into six
 // stages; each stage enqueues hand-written kernels from flip_kernels.h on the handle's stream.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -162,9 +163,11 @@ struct vare_flip {
     unsigned timing = 0;          // bitmask of VARE_KERNEL_* classes bracketed by events
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kid;
+    std::vector<int64_t> ev_units;
     size_t ev_used = 0;
     double kms[VARE_KERNEL_COUNT] = {};
     int64_t klaunch[VARE_KERNEL_COUNT] = {};
+    int64_t kunits[VARE_KERNEL_COUNT] = {};
     hipEvent_t step_ev[4] = {};
 
     size_t ncells() const { return (size_t)nb * (size_t)B * B * B; }
@@ -311,19 +314,36 @@ template <class Op> int run_scan(vare_flip* h, const Op& op, int n, int total_sl
 }
 
 // ---- kernel timing (HIP events around launches of one class on the handle's stream) ----
+bool tgrow(vare_flip* h) {
+    if (h->ev_used + 2 <= h->ev_pool.size()) return true;
+    for (int q = 0; q < 64; ++q) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return false;
+        h->ev_pool.push_back(e);
+        h->ev_kid.push_back(-1);
+        h->ev_units.push_back(0);
+    }
+    return true;
+}
 void tbegin(vare_flip* h, int kid) {
     if (!(h->timing & (1u << kid))) return;
-    if (h->ev_used + 2 > h->ev_pool.size()) {
-        for (int q = 0; q < 64; ++q) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return;
-            h->ev_pool.push_back(e);
-            h->ev_kid.push_back(-1);
-        }
-    }
+    if (!tgrow(h)) return;
     h->ev_kid[h->ev_used] = kid;
+    h->ev_units[h->ev_used] = 0;
     (void)hipEventRecord(h->ev_pool[h->ev_used], h->stream);
     h->ev_used++;
+}
+// Start/stop events for a launch through hipExtLaunchKernelGGL, which records them on the kernel's
+// own dispatch (its execution time alone); `units` = work items of the launch (cells or
+// particles).  Both null when the class is not timed.
+void tpair(vare_flip* h, int kid, int64_t units, hipEvent_t& a, hipEvent_t& b) {
+    a = b = nullptr;
+    if (!(h->timing & (1u << kid)) || !tgrow(h)) return;
+    h->ev_kid[h->ev_used] = kid;
+    h->ev_units[h->ev_used] = units;
+    a = h->ev_pool[h->ev_used++];
+    h->ev_kid[h->ev_used] = -2;
+    b = h->ev_pool[h->ev_used++];
 }
 void tend(vare_flip* h, int kid) {
     if (!(h->timing & (1u << kid))) return;
@@ -339,7 +359,7 @@ int tcollect(vare_flip* h) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, h->ev_pool[q], h->ev_pool[q + 1]));
         const int kid = h->ev_kid[q];
-        if (kid >= 0 && kid < VARE_KERNEL_COUNT) { h->kms[kid] += ms; h->klaunch[kid]++; }
+        if (kid >= 0 && kid < VARE_KERNEL_COUNT) { h->kms[kid] += ms; h->klaunch[kid]++; h->kunits[kid] += h->ev_units[q]; }
     }
     h->ev_used = 0;
     return VARE_OK;
@@ -622,13 +642,15 @@ template <int E> void enqueue_block_up(vare_flip* h, int l, int nu, double* rz) 
     const int nsb = h->ns;
     constexpr unsigned E3 = (unsigned)(E * E * E);
     if (!nwg) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (l == 0) tpair(h, VARE_KERNEL_MG0, (int64_t)nsb * E3, e0, e1);
     if (nu == 1) {
-        hipLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w,
-                           (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done, rz);
+        hipExtLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, bl, nsb, L, C,
+                              F, w, (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done, rz);
     } else {
-        hipLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w,
-                           (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done,
-                           (double*)nullptr);
+        hipExtLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, bl, nsb, L, C,
+                              F, w, (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done,
+                              (double*)nullptr);
         hipLaunchKernelGGL(k_mg0_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, F, w,
                            (const float*)h->mg_zt[l], h->mg_z[l], done, rz);
     }
@@ -701,10 +723,10 @@ template <int B> void enqueue_iteration(vare_flip* h) {
     const unsigned gs = rgrid((size_t)nv);
     const CellList cl = h->clist();
     const int nwg = (h->ns + Fine<B>::K - 1) / Fine<B>::K;
-    tbegin(h, VARE_KERNEL_SPMV);
-    hipLaunchKernelGGL(k_spmv_blk<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, (const int*)h->blist, h->ns,
-                       h->frame(), vecs(h), h->partials, (const CgState*)h->st);
-    tend(h, VARE_KERNEL_SPMV);
+    hipEvent_t e0, e1;
+    tpair(h, VARE_KERNEL_SPMV, (int64_t)nv, e0, e1);
+    hipExtLaunchKernelGGL(k_spmv_blk<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, e0, e1, 0, (const int*)h->blist,
+                          h->ns, h->frame(), vecs(h), h->partials, (const CgState*)h->st);
     hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
     tbegin(h, VARE_KERNEL_UPDATE);
     hipLaunchKernelGGL(k_cg_update, dim3(gs), dim3(WG), 0, h->stream, nv, cl, vecs(h), h->partials, h->partials2,
@@ -1645,13 +1667,15 @@ int vare_flip_set_kernel_timing(vare_flip* h, int enable) {
     return VARE_OK;
 }
 
-int vare_flip_get_kernel_timing(vare_flip* h, double ms[VARE_KERNEL_COUNT], int64_t launches[VARE_KERNEL_COUNT], int reset) {
+int vare_flip_get_kernel_timing(vare_flip* h, double ms[VARE_KERNEL_COUNT], int64_t launches[VARE_KERNEL_COUNT],
+                                int64_t units[VARE_KERNEL_COUNT], int reset) {
     if (!h) return fail(VARE_EINVAL, "null handle");
     CHK(tcollect(h));
     for (int q = 0; q < VARE_KERNEL_COUNT; ++q) {
         if (ms) ms[q] = h->kms[q];
         if (launches) launches[q] = h->klaunch[q];
-        if (reset) { h->kms[q] = 0; h->klaunch[q] = 0; }
+        if (units) units[q] = h->kunits[q];
+        if (reset) { h->kms[q] = 0; h->klaunch[q] = 0; h->kunits[q] = 0; }
     }
     return VARE_OK;
 }
