@@ -120,6 +120,7 @@ struct vare_flip {
     int* bfluid = nullptr;        // [bcap] 1 when the block holds a fluid cell (set by k_divergence)
     int* blist = nullptr;         // [bcap] slots of those blocks, ascending: the solver's blocks
     int ns = 0;                   // number of listed blocks
+    int* wgtab = nullptr;         // [(bcap/K+1)*kTab] per-workgroup block and neighbour slots (k_wg_table)
 
     // cells, capacity bcap*B^3
     float *mass = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *cnt = nullptr;
@@ -217,7 +218,7 @@ void free_cells(vare_flip* h) {
     free_mg(h);
     dfree(h->nbr);
     dfree(h->ccount); dfree(h->cstart);
-    dfree(h->bfluid); dfree(h->blist);
+    dfree(h->bfluid); dfree(h->blist); dfree(h->wgtab);
     dfree(h->mass); dfree(h->vx); dfree(h->vy); dfree(h->vz); dfree(h->cnt);
     dfree(h->cx); dfree(h->cy); dfree(h->cz);
     dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b); dfree(h->z); dfree(h->x64);
@@ -237,6 +238,7 @@ int ensure_blocks(vare_flip* h, size_t nb) {
     CHK(dalloc(&h->nbr, cap * 6));
     CHK(dalloc(&h->ccount, nc + 2)); CHK(dalloc(&h->cstart, nc + 2));
     CHK(dalloc(&h->bfluid, cap + 1)); CHK(dalloc(&h->blist, cap + 1));
+    CHK(dalloc(&h->wgtab, (cap / kWgBlocks + 1) * kTab));
     CHK(dalloc(&h->mass, nc)); CHK(dalloc(&h->vx, nc)); CHK(dalloc(&h->vy, nc)); CHK(dalloc(&h->vz, nc));
     CHK(dalloc(&h->cnt, nc));
     CHK(dalloc(&h->cx, nc)); CHK(dalloc(&h->cy, nc)); CHK(dalloc(&h->cz, nc));
@@ -531,6 +533,11 @@ template <int B> int stage_system(vare_flip* h) {
     HIPCHK(hipMemcpyAsync(h->dint_host + 6, h->dint + 6, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->ns = h->dint_host[6];
+    if (h->ns) {
+        const int nwg = (h->ns + kWgBlocks - 1) / kWgBlocks;
+        hipLaunchKernelGGL(k_wg_table, dim3(nblocks((size_t)nwg * kTab)), dim3(WG), 0, h->stream,
+                           (const int*)h->blist, h->ns, (const int*)h->nbr, nwg, h->wgtab);
+    }
     if (mode == 1) {
         if (precond == VARE_PRECOND_MG) {
             CHK(mg_setup(h));
@@ -609,50 +616,42 @@ int mg_setup(vare_flip* h) {
 // per workgroup.  A level-l cell is fluid only if a level-0 cell of its block is, so the listed
 // blocks hold every fluid cell of these levels.
 template <int E> void enqueue_block_down(vare_flip* h, int l, int nu) {
-    const MgFrame F = mg_frame(h);
     const float w = h->prm.mg_omega;
     const int* done = done_ptr(h);
     const MgLevelK& L = h->mg[l];
     const MgLevelK& C = h->mg[l + 1];
     const unsigned nwg = (unsigned)((h->ns + Fine<E>::K - 1) / Fine<E>::K);
-    const int* bl = h->blist;
-    const int nsb = h->ns;
+    const int* tab = h->wgtab;
     constexpr unsigned E3 = (unsigned)(E * E * E);
     if (!nwg) return;
     if (nu == 1) {
-        hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w, 1, h->mg_zt[l],
-                           done);
+        hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 1, h->mg_zt[l], done);
     } else {
-        hipLaunchKernelGGL(k_mg0_smooth_first2<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, F, w, h->mg_z[l],
-                           done);
-        hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, C, F, w, 0, h->mg_z[l],
-                           done);
+        hipLaunchKernelGGL(k_mg0_smooth_first2<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, h->mg_z[l], done);
+        hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 0, h->mg_z[l], done);
     }
 }
 
 // rz != nullptr (level 0 only): the last pass also writes per-workgroup partials of r.z.
 template <int E> void enqueue_block_up(vare_flip* h, int l, int nu, double* rz) {
-    const MgFrame F = mg_frame(h);
     const float w = h->prm.mg_omega;
     const int* done = done_ptr(h);
     const MgLevelK& L = h->mg[l];
-    const MgLevelK& C = h->mg[l + 1];
     const unsigned nwg = (unsigned)((h->ns + Fine<E>::K - 1) / Fine<E>::K);
-    const int* bl = h->blist;
-    const int nsb = h->ns;
+    const int* tab = h->wgtab;
     constexpr unsigned E3 = (unsigned)(E * E * E);
     if (!nwg) return;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (l == 0) tpair(h, VARE_KERNEL_MG0, (int64_t)nsb * E3, e0, e1);
+    if (l == 0) tpair(h, VARE_KERNEL_MG0, (int64_t)h->ns * E3, e0, e1);
     if (nu == 1) {
-        hipExtLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, bl, nsb, L, C,
-                              F, w, (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done, rz);
+        hipExtLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, w,
+                              (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done, rz);
     } else {
-        hipExtLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, bl, nsb, L, C,
-                              F, w, (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done,
+        hipExtLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, w,
+                              (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done,
                               (double*)nullptr);
-        hipLaunchKernelGGL(k_mg0_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, bl, nsb, L, F, w,
-                           (const float*)h->mg_zt[l], h->mg_z[l], done, rz);
+        hipLaunchKernelGGL(k_mg0_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, (const float*)h->mg_zt[l],
+                           h->mg_z[l], done, rz);
     }
 }
 
@@ -725,8 +724,8 @@ template <int B> void enqueue_iteration(vare_flip* h) {
     const int nwg = (h->ns + Fine<B>::K - 1) / Fine<B>::K;
     hipEvent_t e0, e1;
     tpair(h, VARE_KERNEL_SPMV, (int64_t)nv, e0, e1);
-    hipExtLaunchKernelGGL(k_spmv_blk<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, e0, e1, 0, (const int*)h->blist,
-                          h->ns, h->frame(), vecs(h), h->partials, (const CgState*)h->st);
+    hipExtLaunchKernelGGL(k_spmv_blk<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, e0, e1, 0, (const int*)h->wgtab,
+                          vecs(h), h->partials, (const CgState*)h->st);
     hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
     tbegin(h, VARE_KERNEL_UPDATE);
     hipLaunchKernelGGL(k_cg_update, dim3(gs), dim3(WG), 0, h->stream, nv, cl, vecs(h), h->partials, h->partials2,

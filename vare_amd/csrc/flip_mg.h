@@ -111,8 +111,8 @@ __device__ __forceinline__ int mg_child(const MgLevelK& L, const MgLevelK& C, co
     return F.slot[(k * L.dy + j) * L.dx + i];      // sparse edge-1 level: cell index = block slot
 }
 
-__device__ __forceinline__ bool mg_fluid(uint16_t info) { return (info & kInfoFluid) != 0; }
-__device__ __forceinline__ float mg_diag(uint16_t info) { return (float)((info >> 8) & 0x1f); }
+__device__ __forceinline__ bool mg_fluid(unsigned info) { return (info & kInfoFluid) != 0; }
+__device__ __forceinline__ float mg_diag(unsigned info) { return (float)((info >> 8) & 0x1f); }
 
 // ---- set-up (once per solve) ----
 // Coarse fluid flag: any child fluid.

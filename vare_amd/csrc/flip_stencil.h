@@ -1,12 +1,18 @@
-// flip_stencil.h -- block-aligned 7-point stencil kernels on the fine level (edge B).
+// flip_stencil.h -- block-aligned 7-point stencil kernels on the fine level (edge B) and, for
+// B = 8, on the first multigrid level (edge 4) over the same blocks.
 //
-// Each workgroup takes K consecutive entries of the solver's block list (`blist`: the blocks
-// holding a fluid cell; blocks without fluid hold no unknown and are skipped) with one thread per
-// cell position (B^3 threads: 512 for B=8, 64 for B=4), so every thread serves K cells.  A pass is
-// two dependent memory round trips (own values, then the face neighbours in adjacent blocks), so
-// the K blocks' loads are issued together in each round: the latency is paid once per K blocks.
-// Own values are staged in LDS; neighbours inside a block are read from LDS, those across a face
-// from global memory through the block's neighbour slots.  Reductions are one FP64 partial per
+// Each workgroup takes K consecutive entries of the solver's block list (the blocks holding a
+// fluid cell; blocks without fluid hold no unknown and are skipped) with one thread per cell
+// position (E^3 threads: 512 for E=8, 64 for E=4), so every thread serves K cells.
+//
+// Latency structure.  The slots of a workgroup's K blocks and their 6 face-neighbour slots are
+// gathered once per solve into a table (k_wg_table, kTab ints per workgroup).  A kernel reads its
+// entry with uniform loads (scalar registers) together with the solver's `done` flag, and then
+// issues every value load of the pass at once: the K blocks' own values and the values across
+// their faces.  A value across a face is loaded whenever the neighbour block is active and used
+// only where the stencil descriptor marks that neighbour fluid, so it need not wait for the
+// descriptor.  A pass is therefore one round trip for the table and one for the data.  Own values
+// are staged in LDS for the neighbours inside a block.  Reductions are one FP64 partial per
 // workgroup, summed in a fixed order by the finalize kernels.
 #pragma once
 #include "flip_kernels.h"
@@ -21,39 +27,77 @@ template <int B> struct Fine {
     static constexpr int K = 4;                // blocks per workgroup
 };
 
-// Slots of this workgroup's K blocks (-1 past the end of the list) and, in LDS, their 6 neighbour
-// slots each.  The caller synchronises before reading fnb.
+constexpr int kWgBlocks = 4;                   // K of every block-aligned kernel
+constexpr int kTab = 32;                       // ints per workgroup in the table (K + 6K <= kTab)
+static_assert(7 * kWgBlocks <= kTab, "workgroup table entry too small");
+
+// Table entry of workgroup w: [0, K) the block slots (-1 past the end of the list), then the 6
+// neighbour slots (-x,+x,-y,+y,-z,+z; -1 inactive, -2 outside the domain) of each block.
+__global__ void k_wg_table(const int* __restrict__ blist, int nsb, const int* __restrict__ nbr, int nwg,
+                           int* __restrict__ tab) {
+    constexpr int K = kWgBlocks;
+    const int t = blockIdx.x * WG + threadIdx.x;
+    if (t >= nwg * kTab) return;
+    const int w = t / kTab, e = t % kTab;
+    int v = -1;
+    if (e < K) {
+        const int i = w * K + e;
+        if (i < nsb) v = blist[i];
+    } else if (e < 7 * K) {
+        const int i = w * K + (e - K) / 6;
+        if (i < nsb) v = nbr[blist[i] * 6 + (e - K) % 6];
+    }
+    tab[t] = v;
+}
+
+// This workgroup's entry (uniform addresses: loaded into scalar registers).  The empty asm
+// statements make every value live at this point, so the loads are issued here, before the
+// caller's early-exit test on the solver flag, and their latency overlaps the flag's.
 template <int B>
-__device__ __forceinline__ void wg_blocks(const int* __restrict__ blist, int nsb, const int* __restrict__ nbr,
-                                          int (&s)[Fine<B>::K], int* fnb) {
+__device__ __forceinline__ void wg_load(const int* __restrict__ tab, int (&s)[Fine<B>::K], int (&nb)[Fine<B>::K][6]) {
     constexpr int K = Fine<B>::K;
-    const int w = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int* t = tab + xcd_swizzle(blockIdx.x, gridDim.x) * kTab;
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
-        const int v = w * K + kb;
-        s[kb] = v < nsb ? blist[v] : -1;
+        s[kb] = t[kb];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nb[kb][q] = t[K + 6 * kb + q];
     }
-    if (threadIdx.x < 6 * K) {
-        const int v = w * K + threadIdx.x / 6;
-        fnb[threadIdx.x] = v < nsb ? nbr[blist[v] * 6 + threadIdx.x % 6] : -1;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        asm volatile("" ::"s"(s[kb]));
+#pragma unroll
+        for (int q = 0; q < 6; ++q) asm volatile("" ::"s"(nb[kb][q]));
     }
 }
 
-// Local coordinates of thread l and whether the neighbour in direction q stays in the block;
-// returns the local index inside the block (nl) or, across the face, the local index inside the
-// neighbour block.
+// Faces of the block that thread l lies on: per axis a, side[a] = 0 on the low face (direction
+// 2a), 1 on the high face (2a+1), -1 for an interior position; fl[a] = the local index of the cell
+// across that face in the neighbour block.
 template <int B>
-__device__ __forceinline__ bool fine_nbr(int l, int q, int& nl) {
+__device__ __forceinline__ void thread_faces(int l, int (&side)[3], int (&fl)[3]) {
     constexpr int SH = Fine<B>::SH;
-    const int li = l & (B - 1), lj = (l >> SH) & (B - 1), lk = l >> (2 * SH);
-    const int axis = q >> 1;
-    const int dir = (q & 1) ? 1 : -1;
-    const int lc = axis == 0 ? li : (axis == 1 ? lj : lk);
-    const int stride = axis == 0 ? 1 : (axis == 1 ? B : B * B);
-    const int t = lc + dir;
-    if (t >= 0 && t < B) { nl = l + dir * stride; return true; }
-    nl = l - dir * (B - 1) * stride;
-    return false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int lc = (l >> (a * SH)) & (B - 1);
+        const int stride = 1 << (a * SH);
+        side[a] = lc == 0 ? 0 : (lc == B - 1 ? 1 : -1);
+        fl[a] = lc == 0 ? l + (B - 1) * stride : l - (B - 1) * stride;
+    }
+}
+
+// Slot of the neighbour block across this thread's face on axis a (negative: none).  `a` is a
+// compile-time constant after unrolling, so both candidates are fixed scalar registers.
+__device__ __forceinline__ int face_slot(const int (&nb)[6], int a, int side) {
+    return side < 0 ? -1 : (side ? nb[2 * a + 1] : nb[2 * a]);
+}
+
+// Local index of the neighbour inside the block in direction q.
+template <int B>
+__device__ __forceinline__ int inner_nbr(int l, int q) {
+    constexpr int SH = Fine<B>::SH;
+    const int stride = 1 << ((q >> 1) * SH);
+    return (q & 1) ? l + stride : l - stride;
 }
 
 template <int B>
@@ -68,21 +112,38 @@ __device__ __forceinline__ double fine_block_sum(double v, double* sh) {
     return r;
 }
 
+// A applied to the staged field: diag * own - sum of the fluid neighbours (in the order -x,+x,
+// -y,+y,-z,+z), the neighbour across a face from `halo`, inside the block from LDS.
+template <int B>
+__device__ __forceinline__ float apply_row(unsigned in, float own, const float* lds, int l, const int (&side)[3],
+                                           const float (&halo)[3]) {
+    float az = mg_diag(in) * own;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        if (!(in & (1u << q))) continue;
+        const int a = q >> 1;
+        az -= side[a] == (q & 1) ? halo[a] : lds[inner_nbr<B>(l, q)];
+    }
+    return az;
+}
+
 // Ap = A p and partial p.Ap (BLAS::mul, BLAS.cu:12-40).
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict__ blist, int nsb, FrameK f, CgVecs v,
+__global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict__ tab, CgVecs v,
                                                          double* __restrict__ partials,
                                                          const CgState* __restrict__ st) {
     constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
     __shared__ float sp[K][B3];
-    __shared__ int fnb[6 * K];
     __shared__ double sh[Fine<B>::NW];
-    if (st->done) return;
+    const int dn = st->done;
+    int s[K], nb[K][6];
+    wg_load<B>(tab, s, nb);
+    if (dn) return;
     const int l = threadIdx.x;
-    int s[K];
-    wg_blocks<B>(blist, nsb, f.nbr, s, fnb);
-    uint16_t in[K];
-    float pc[K];
+    int side[3], fl[3];
+    thread_faces<B>(l, side, fl);
+    unsigned in[K];
+    float pc[K], hv[K][3];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         in[kb] = 0;
@@ -91,31 +152,24 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict_
             in[kb] = v.info[s[kb] * B3 + l];
             pc[kb] = v.p[s[kb] * B3 + l];
         }
-        sp[kb][l] = pc[kb];
     }
-    __syncthreads();
-    float hv[K][6];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            int nl;
-            hv[kb][q] = 0.f;
-            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) hv[kb][q] = v.p[fnb[kb * 6 + q] * B3 + nl];
+        for (int a = 0; a < 3; ++a) {
+            const int ns = face_slot(nb[kb], a, side[a]);
+            hv[kb][a] = ns >= 0 ? v.p[ns * B3 + fl[a]] : 0.f;
         }
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) sp[kb][l] = pc[kb];
+    __syncthreads();
     double pap = 0.0;
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         if (s[kb] < 0) continue;
         float acc = 0.f;
         if (in[kb] & kInfoFluid) {
-            acc = (float)((in[kb] >> 8) & 0x1f) * pc[kb];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                if (!(in[kb] & (1u << q))) continue;
-                int nl;
-                acc = acc - (fine_nbr<B>(l, q, nl) ? sp[kb][nl] : hv[kb][q]);
-            }
+            acc = apply_row<B>(in[kb], pc[kb], sp[kb], l, side, hv[kb]);
             pap += (double)pc[kb] * (double)acc;
         }
         v.ap[s[kb] * B3 + l] = acc;
@@ -124,21 +178,28 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict_
     if (l == 0) partials[blockIdx.x] = tot;     // any fixed slot per workgroup keeps the sum order fixed
 }
 
-// ---- multigrid, level 0 (the CG residual r, stencil descriptor info) ----
+// ---- multigrid, level 0 (the CG residual r, stencil descriptor info) and level 1 for B = 8 ----
+// First Jacobi sweep from zero at a cell: omega * r / D on fluid cells, 0 elsewhere.
+__device__ __forceinline__ float mg_first(unsigned in, float r, float omega) {
+    return mg_fluid(in) ? omega * r / mg_diag(in) : 0.f;
+}
+
 // Two Jacobi sweeps from zero: zout = z1 + w (r - A z1)/D, z1 = w D^-1 r.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __restrict__ blist, int nsb, MgLevelK L,
-                                                                  MgFrame F, float omega, float* __restrict__ zout,
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __restrict__ tab, MgLevelK L,
+                                                                  float omega, float* __restrict__ zout,
                                                                   const int* done) {
     constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
     __shared__ float sz[K][B3];
-    __shared__ int fnb[6 * K];
-    if (*done) return;
+    const int dn = *done;
+    int s[K], nb[K][6];
+    wg_load<B>(tab, s, nb);
+    if (dn) return;
     const int l = threadIdx.x;
-    int s[K];
-    wg_blocks<B>(blist, nsb, F.nbr, s, fnb);
-    uint16_t in[K];
-    float rc[K], z1[K];
+    int side[3], fl[3];
+    thread_faces<B>(l, side, fl);
+    unsigned in[K], hi[K][3];
+    float rc[K], hr[K][3];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         in[kb] = 0;
@@ -147,65 +208,63 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __
             in[kb] = L.info[s[kb] * B3 + l];
             rc[kb] = L.r[s[kb] * B3 + l];
         }
-        z1[kb] = mg_fluid(in[kb]) ? omega * rc[kb] / mg_diag(in[kb]) : 0.f;
-        sz[kb][l] = z1[kb];
     }
-    __syncthreads();
-    uint16_t hi[K][6];
-    float hr[K][6];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            int nl;
-            hi[kb][q] = 0;
-            hr[kb][q] = 0.f;
-            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) {
-                const int n = fnb[kb * 6 + q] * B3 + nl;
-                hi[kb][q] = L.info[n];
-                hr[kb][q] = L.r[n];
+        for (int a = 0; a < 3; ++a) {
+            const int ns = face_slot(nb[kb], a, side[a]);
+            hi[kb][a] = 0;
+            hr[kb][a] = 0.f;
+            if (ns >= 0) {
+                hi[kb][a] = L.info[ns * B3 + fl[a]];
+                hr[kb][a] = L.r[ns * B3 + fl[a]];
             }
         }
+    float z1[K];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        z1[kb] = mg_first(in[kb], rc[kb], omega);
+        sz[kb][l] = z1[kb];
+    }
+    __syncthreads();
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         if (s[kb] < 0) continue;
         float out = 0.f;
         if (mg_fluid(in[kb])) {
-            const float d = mg_diag(in[kb]);
-            float az = d * z1[kb];
+            float hz[3];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                if (!(in[kb] & (1u << q))) continue;
-                int nl;
-                az -= fine_nbr<B>(l, q, nl) ? sz[kb][nl]
-                                            : (mg_fluid(hi[kb][q]) ? omega * hr[kb][q] / mg_diag(hi[kb][q]) : 0.f);
-            }
-            out = z1[kb] + omega * (rc[kb] - az) / d;
+            for (int a = 0; a < 3; ++a) hz[a] = mg_first(hi[kb][a], hr[kb][a], omega);
+            const float az = apply_row<B>(in[kb], z1[kb], sz[kb], l, side, hz);
+            out = z1[kb] + omega * (rc[kb] - az) / mg_diag(in[kb]);
         }
         zout[s[kb] * B3 + l] = out;
     }
 }
 
-// Residual of level 0 and restriction to level 1 inside the block (the 2^3 children of a level-1
-// cell lie in the same block): rC = 0.5 * sum over fluid children of (r - A z), children summed in
-// the fixed order of mg_child.  With first != 0, z is the first sweep from zero (w D^-1 r) and is
+// Residual and restriction to the next level inside the block (the 2^3 children of a coarse cell
+// lie in the same block): rC = 0.5 * sum over fluid children of (r - A z), children summed in the
+// fixed order of mg_child.  With first != 0, z is the first sweep from zero (w D^-1 r) and is
 // stored to zf.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(const int* __restrict__ blist, int nsb, MgLevelK L,
-                                                             MgLevelK C, MgFrame F, float omega, int first,
-                                                             float* __restrict__ zf, const int* done) {
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(const int* __restrict__ tab, MgLevelK L, MgLevelK C,
+                                                             float omega, int first, float* __restrict__ zf,
+                                                             const int* done) {
     constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
     constexpr int SH = Fine<B>::SH;
     constexpr int BC = B / 2, CSH = SH - 1, C3 = BC * BC * BC;
     __shared__ float sz[K][B3];
     __shared__ float sres[K][B3];
-    __shared__ int fnb[6 * K];
-    if (*done) return;
+    const int dn = *done;
+    int s[K], nb[K][6];
+    wg_load<B>(tab, s, nb);
+    if (dn) return;
     const int l = threadIdx.x;
-    int s[K];
-    wg_blocks<B>(blist, nsb, F.nbr, s, fnb);
-    uint16_t in[K];
-    float rc[K], zc[K];
+    int side[3], fl[3];
+    thread_faces<B>(l, side, fl);
+    unsigned in[K];
+    float rc[K], zc[K], hz[K][3];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         in[kb] = 0;
@@ -215,47 +274,47 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(const int* __restr
             const int c = s[kb] * B3 + l;
             in[kb] = L.info[c];
             rc[kb] = L.r[c];
-            if (first) {
-                zc[kb] = mg_fluid(in[kb]) ? omega * rc[kb] / mg_diag(in[kb]) : 0.f;
-                if (mg_fluid(in[kb])) zf[c] = zc[kb];
-            } else {
-                zc[kb] = zf[c];
-            }
+            if (!first) zc[kb] = zf[c];
         }
-        sz[kb][l] = zc[kb];
     }
-    __syncthreads();
-    float hz[K][6];
+    if (first) {
+        unsigned hi[K][3];
+        float hr[K][3];
 #pragma unroll
-    for (int kb = 0; kb < K; ++kb)
+        for (int kb = 0; kb < K; ++kb)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            int nl;
-            hz[kb][q] = 0.f;
-            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) {
-                const int n = fnb[kb * 6 + q] * B3 + nl;
-                if (first) {
-                    const uint16_t ni = L.info[n];
-                    hz[kb][q] = mg_fluid(ni) ? omega * L.r[n] / mg_diag(ni) : 0.f;
-                } else {
-                    hz[kb][q] = zf[n];
+            for (int a = 0; a < 3; ++a) {
+                const int ns = face_slot(nb[kb], a, side[a]);
+                hi[kb][a] = 0;
+                hr[kb][a] = 0.f;
+                if (ns >= 0) {
+                    hi[kb][a] = L.info[ns * B3 + fl[a]];
+                    hr[kb][a] = L.r[ns * B3 + fl[a]];
                 }
             }
+#pragma unroll
+        for (int kb = 0; kb < K; ++kb) {
+            zc[kb] = mg_first(in[kb], rc[kb], omega);
+            if (s[kb] >= 0 && mg_fluid(in[kb])) zf[s[kb] * B3 + l] = zc[kb];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) hz[kb][a] = mg_first(hi[kb][a], hr[kb][a], omega);
         }
+    } else {
+#pragma unroll
+        for (int kb = 0; kb < K; ++kb)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int ns = face_slot(nb[kb], a, side[a]);
+                hz[kb][a] = ns >= 0 ? zf[ns * B3 + fl[a]] : 0.f;
+            }
+    }
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) sz[kb][l] = zc[kb];
+    __syncthreads();
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         float res = 0.f;
-        if (mg_fluid(in[kb])) {
-            const float d = mg_diag(in[kb]);
-            float az = d * zc[kb];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                if (!(in[kb] & (1u << q))) continue;
-                int nl;
-                az -= fine_nbr<B>(l, q, nl) ? sz[kb][nl] : hz[kb][q];
-            }
-            res = rc[kb] - az;
-        }
+        if (mg_fluid(in[kb])) res = rc[kb] - apply_row<B>(in[kb], zc[kb], sz[kb], l, side, hz[kb]);
         sres[kb][l] = res;
     }
     __syncthreads();
@@ -269,8 +328,8 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(const int* __restr
             if (mg_fluid(C.info[cc])) {
 #pragma unroll
                 for (int a = 0; a < 8; ++a) {
-                    const int fl = (((2 * lk + (a >> 2)) << SH) + (2 * lj + ((a >> 1) & 1))) * B + (2 * li + (a & 1));
-                    acc += sres[kb][fl];
+                    const int fl8 = (((2 * lk + (a >> 2)) << SH) + (2 * lj + ((a >> 1) & 1))) * B + (2 * li + (a & 1));
+                    acc += sres[kb][fl8];
                 }
                 acc *= 0.5f;
             }
@@ -279,71 +338,78 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(const int* __restr
     }
 }
 
-// Prolongation (piecewise constant from level 1) fused with one Jacobi sweep on level 0.
+// Local index of the parent (edge B/2 level) of local cell ll of an edge-B block.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* __restrict__ blist, int nsb,
-                                                                   MgLevelK L, MgLevelK C, MgFrame F, float omega,
-                                                                   const float* __restrict__ z,
+__device__ __forceinline__ int parent_local(int ll) {
+    constexpr int SH = Fine<B>::SH, CSH = SH - 1;
+    const int li = (ll & (B - 1)) >> 1, lj = ((ll >> SH) & (B - 1)) >> 1, lk = (ll >> (2 * SH)) >> 1;
+    return (((lk << CSH) + lj) << CSH) + li;
+}
+
+// Prolongation (piecewise constant from the next level) fused with one Jacobi sweep.
+template <int B>
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* __restrict__ tab, MgLevelK L,
+                                                                   float omega, const float* __restrict__ z,
                                                                    const float* __restrict__ zc,
                                                                    float* __restrict__ zout, const int* done,
                                                                    double* __restrict__ rz_partials) {
     constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
-    constexpr int SH = Fine<B>::SH;
-    constexpr int BC = B / 2, CSH = SH - 1, C3 = BC * BC * BC;
+    constexpr int BC = B / 2, C3 = BC * BC * BC;
     __shared__ float szp[K][B3];
-    __shared__ int fnb[6 * K];
     __shared__ double sh[Fine<B>::NW];
-    if (*done) return;
+    const int dn = *done;
+    int s[K], nb[K][6];
+    wg_load<B>(tab, s, nb);
+    if (dn) return;
     const int l = threadIdx.x;
-    int s[K];
-    wg_blocks<B>(blist, nsb, F.nbr, s, fnb);
-    auto parent_local = [](int ll) {
-        const int li = (ll & (B - 1)) >> 1, lj = ((ll >> SH) & (B - 1)) >> 1, lk = (ll >> (2 * SH)) >> 1;
-        return (((lk << CSH) + lj) << CSH) + li;
-    };
-    uint16_t in[K];
-    float zp[K], rc[K];
-    double rz = 0.0;             // r.z of the output (the PCG's r.z when this is the last level-0 pass)
+    int side[3], fl[3];
+    thread_faces<B>(l, side, fl);
+    unsigned in[K];
+    float zo[K], zq[K], rc[K], hz[K][3], hq[K][3];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         in[kb] = 0;
-        zp[kb] = 0.f;
+        zo[kb] = 0.f;
+        zq[kb] = 0.f;
         rc[kb] = 0.f;
         if (s[kb] >= 0) {
             const int c = s[kb] * B3 + l;
             in[kb] = L.info[c];
-            zp[kb] = z[c] + zc[s[kb] * C3 + parent_local(l)];
+            zo[kb] = z[c];
+            zq[kb] = zc[s[kb] * C3 + parent_local<B>(l)];
             rc[kb] = L.r[c];
         }
-        szp[kb][l] = zp[kb];
     }
-    __syncthreads();
-    float hz[K][6];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            int nl;
-            hz[kb][q] = 0.f;
-            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) {
-                const int ns = fnb[kb * 6 + q];
-                hz[kb][q] = z[ns * B3 + nl] + zc[ns * C3 + parent_local(nl)];
+        for (int a = 0; a < 3; ++a) {
+            const int ns = face_slot(nb[kb], a, side[a]);
+            hz[kb][a] = 0.f;
+            hq[kb][a] = 0.f;
+            if (ns >= 0) {
+                hz[kb][a] = z[ns * B3 + fl[a]];
+                hq[kb][a] = zc[ns * C3 + parent_local<B>(fl[a])];
             }
         }
+    float zp[K];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        zp[kb] = zo[kb] + zq[kb];
+        szp[kb][l] = zp[kb];
+    }
+    __syncthreads();
+    double rz = 0.0;             // r.z of the output (the PCG's r.z when this is the last level-0 pass)
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         if (s[kb] < 0) continue;
         float out = 0.f;
         if (mg_fluid(in[kb])) {
-            const float d = mg_diag(in[kb]);
-            float az = d * zp[kb];
+            float h[3];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                if (!(in[kb] & (1u << q))) continue;
-                int nl;
-                az -= fine_nbr<B>(l, q, nl) ? szp[kb][nl] : hz[kb][q];
-            }
-            out = zp[kb] + omega * (rc[kb] - az) / d;
+            for (int a = 0; a < 3; ++a) h[a] = hz[kb][a] + hq[kb][a];
+            const float az = apply_row<B>(in[kb], zp[kb], szp[kb], l, side, h);
+            out = zp[kb] + omega * (rc[kb] - az) / mg_diag(in[kb]);
         }
         zout[s[kb] * B3 + l] = out;
         rz += (double)rc[kb] * (double)out;
@@ -354,23 +420,23 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* _
     }
 }
 
-// One Jacobi sweep on level 0: zout = z + w (r - A z)/D.
+// One Jacobi sweep: zout = z + w (r - A z)/D.
 template <int B>
-__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restrict__ blist, int nsb, MgLevelK L,
-                                                           MgFrame F, float omega, const float* __restrict__ z,
-                                                           float* __restrict__ zout, const int* done,
-                                                           double* __restrict__ rz_partials) {
+__global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restrict__ tab, MgLevelK L, float omega,
+                                                           const float* __restrict__ z, float* __restrict__ zout,
+                                                           const int* done, double* __restrict__ rz_partials) {
     constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
     __shared__ float sz[K][B3];
-    __shared__ int fnb[6 * K];
     __shared__ double sh[Fine<B>::NW];
-    if (*done) return;
+    const int dn = *done;
+    int s[K], nb[K][6];
+    wg_load<B>(tab, s, nb);
+    if (dn) return;
     const int l = threadIdx.x;
-    int s[K];
-    wg_blocks<B>(blist, nsb, F.nbr, s, fnb);
-    uint16_t in[K];
-    float zc[K], rc[K];
-    double rz = 0.0;             // r.z of the output (the PCG's r.z when this is the last level-0 pass)
+    int side[3], fl[3];
+    thread_faces<B>(l, side, fl);
+    unsigned in[K];
+    float zc[K], rc[K], hz[K][3];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         in[kb] = 0;
@@ -382,32 +448,25 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restric
             zc[kb] = z[c];
             rc[kb] = L.r[c];
         }
-        sz[kb][l] = zc[kb];
     }
-    __syncthreads();
-    float hz[K][6];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            int nl;
-            hz[kb][q] = 0.f;
-            if ((in[kb] & (1u << q)) && !fine_nbr<B>(l, q, nl)) hz[kb][q] = z[fnb[kb * 6 + q] * B3 + nl];
+        for (int a = 0; a < 3; ++a) {
+            const int ns = face_slot(nb[kb], a, side[a]);
+            hz[kb][a] = ns >= 0 ? z[ns * B3 + fl[a]] : 0.f;
         }
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) sz[kb][l] = zc[kb];
+    __syncthreads();
+    double rz = 0.0;             // r.z of the output (the PCG's r.z when this is the last level-0 pass)
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         if (s[kb] < 0) continue;
         float out = 0.f;
         if (mg_fluid(in[kb])) {
-            const float d = mg_diag(in[kb]);
-            float az = d * zc[kb];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                if (!(in[kb] & (1u << q))) continue;
-                int nl;
-                az -= fine_nbr<B>(l, q, nl) ? sz[kb][nl] : hz[kb][q];
-            }
-            out = zc[kb] + omega * (rc[kb] - az) / d;
+            const float az = apply_row<B>(in[kb], zc[kb], sz[kb], l, side, hz[kb]);
+            out = zc[kb] + omega * (rc[kb] - az) / mg_diag(in[kb]);
         }
         zout[s[kb] * B3 + l] = out;
         rz += (double)rc[kb] * (double)out;
