@@ -114,18 +114,33 @@ __device__ __forceinline__ int mg_child(const MgLevelK& L, const MgLevelK& C, co
 __device__ __forceinline__ bool mg_fluid(unsigned info) { return (info & kInfoFluid) != 0; }
 __device__ __forceinline__ float mg_diag(unsigned info) { return (float)((info >> 8) & 0x1f); }
 
+// Whether child a of coarse cell cc lies inside the fine level's grid (an inactive block inside the
+// frame is inside: it is air; a position past an odd-sized grid's last cell is outside the domain).
+__device__ __forceinline__ bool mg_child_inside(const MgLevelK& L, const MgLevelK& C, int cc, int a) {
+    if (!L.dense && L.sh > 0) return true;
+    const int I = cc % C.dx, J = (cc / C.dx) % C.dy, K = cc / (C.dx * C.dy);
+    return 2 * I + (a & 1) < L.dx && 2 * J + ((a >> 1) & 1) < L.dy && 2 * K + (a >> 2) < L.dz;
+}
+
 // ---- set-up (once per solve) ----
-// Coarse fluid flag: any child fluid.
+// Coarse fluid flag: at least half of the children inside the grid are fluid.  Coarse cells that
+// are mostly air stay air (Dirichlet), so the coarse free surface lies where the fine one does; a
+// fluid cell whose parent is air is corrected by the smoother alone, which converges fast next to
+// a Dirichlet boundary.  (Marking a coarse cell fluid when any child is fluid pushes the coarse
+// surface outwards by up to a coarse cell per level and, measured on 256^3 dam-break systems,
+// needs 2.5x the PCG iterations.)
 __global__ void k_mg_flag(MgLevelK L, MgLevelK C, MgFrame F) {
     const int cc = blockIdx.x * WG + threadIdx.x;
     if (cc >= C.ncell) return;
-    bool fl = false;
+    int nin = 0, nfl = 0;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
+        if (!mg_child_inside(L, C, cc, a)) continue;
+        ++nin;
         const int ch = mg_child(L, C, F, cc, a);
-        if (ch >= 0 && mg_fluid(L.info[ch])) fl = true;
+        if (ch >= 0 && mg_fluid(L.info[ch])) ++nfl;
     }
-    C.info[cc] = fl ? kInfoFluid : 0;
+    C.info[cc] = (nfl > 0 && 2 * nfl >= nin) ? kInfoFluid : 0;
 }
 
 // Stencil descriptor of a coarse level from its fluid flags.
