@@ -40,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per processed cell of one launch, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # on this benchmark (20 steps) and calibrated against plain 1 GiB copies (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0); profiles/hbm_traffic_dam_break_256.md.
-SPMV_TRAFFIC_PER_CELL = 9.29      # k_spmv_blk: 2 x 2.81 + 3.66
-MG0_TRAFFIC_PER_CELL = 12.41      # k_mg0_prolong_smooth: 2 x 4.54 + 3.33
+SPMV_TRAFFIC_PER_CELL = 9.58      # k_spmv_blk: (2 x 13273.9 + 16366.3) KiB per 4.588 M cells
+MG0_TRAFFIC_PER_CELL = 12.61      # k_mg0_prolong_smooth: (2 x 20825.3 + 14803.1) KiB per 4.586 M cells
 DT = 1.0 / 240.0
 
 

@@ -5,6 +5,7 @@
 // stages; each stage enqueues hand-written kernels from flip_kernels.h on the handle's stream.
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -112,6 +113,8 @@ struct vare_flip {
 
     // frame
     int* flag = nullptr;
+    int* occ = nullptr;           // [(fx+1)(fy+1)(fz+1)] step stamp of the blocks holding a particle (k_bin_occ)
+    int occ_stamp = 0;
     int* slot = nullptr;
     int* coord = nullptr;     // capacity nblk (4 bytes per frame block)
     int* nbr = nullptr;       // capacity bcap*6
@@ -184,6 +187,7 @@ struct vare_flip {
     FrameK frame() const { return FrameK{fx, fy, fz, nx, ny, nz, slot, coord, nbr}; }
     CellList clist() const { return CellList{blist, B == 8 ? 9 : 6}; }
     int nvirt() const { return ns * B * B * B; }          // cells of the listed blocks
+    size_t nocc() const { return (size_t)(fx + 1) * (fy + 1) * (fz + 1); }
     ColK colk() const {
         return ColK{col_slot, col_phi, {col_v[0], col_v[1], col_v[2]}, {col_n[0], col_n[1], col_n[2]}, col_bg,
                     col_active};
@@ -442,9 +446,14 @@ template <int B> int bin_particles(vare_flip* h) {
 template <int B> int stage_prepare(vare_flip* h) {
     CHK(fill_u32(h, h->flag, 0, (size_t)h->nblk));
     tbegin(h, VARE_KERNEL_BIN);
-    if (h->np)
-        hipLaunchKernelGGL(k_bin_mark<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
-                           (int)h->np, h->fx, h->fy, h->fz, h->flag);
+    if (h->np) {
+        h->occ_stamp = h->occ_stamp == INT_MAX ? 1 : h->occ_stamp + 1;
+        if (h->occ_stamp == 1) CHK(fill_u32(h, h->occ, 0, h->nocc()));
+        hipLaunchKernelGGL(k_bin_occ<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
+                           (int)h->np, h->fx, h->fy, h->fz, h->occ_stamp, h->occ);
+        hipLaunchKernelGGL(k_bin_dilate, dim3(nblocks(h->nocc())), dim3(WG), 0, h->stream, h->fx, h->fy, h->fz,
+                           h->occ_stamp, (const int*)h->occ, h->flag);
+    }
     tend(h, VARE_KERNEL_BIN);
     if (h->prm.enable_water_tank) {
         const int tall = std::min(std::max((int)(h->vox_wall) / B, 0), h->fy);   // SparseFrame.cu:165
@@ -1060,6 +1069,7 @@ int vare_flip_create(const vare_grid_desc* g, const vare_flip_params* prm, size_
     if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(VARE_EHIP, "hipStreamCreate failed"); }
     int rc = VARE_OK;
     if (rc == VARE_OK) rc = dalloc(&h->flag, (size_t)h->nblk);
+    if (rc == VARE_OK) rc = dalloc(&h->occ, h->nocc());
     if (rc == VARE_OK) rc = dalloc(&h->slot, (size_t)h->nblk);
     if (rc == VARE_OK) rc = dalloc(&h->coord, (size_t)h->nblk);
     if (rc == VARE_OK) rc = dalloc(&h->dint, 8);
@@ -1089,7 +1099,7 @@ int vare_flip_destroy(vare_flip* h) {
     for (int q = 0; q < 7; ++q) dfree(h->pb[q]);
     dfree(h->pid); dfree(h->pid2); dfree(h->removed); dfree(h->removed_log); dfree(h->hole); dfree(h->src);
     dfree(h->remap); dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins); dfree(h->midbins);
-    dfree(h->flag); dfree(h->slot); dfree(h->coord);
+    dfree(h->flag); dfree(h->occ); dfree(h->slot); dfree(h->coord);
     free_cells(h);
     free_collision(h);
     dfree(h->tiles); dfree(h->dint); dfree(h->st);

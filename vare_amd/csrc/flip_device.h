@@ -1,8 +1,9 @@
 // flip_device.h -- device-side building blocks of the gfx950 FLIP step.
 //
 // Layout in HBM (one handle):
-//   particles  : structure of arrays px,py,pz,vx,vy,vz,adj,vort,curv (float), in the reference's
-//                particle order (so Array::remove's order is kept without a permutation).
+//   particles  : structure of arrays px,py,pz,vx,vy,vz,adj (float; velocities optionally FP16) in
+//                physical order, which each step's binning sets to bin order; pid[q] is the
+//                reference (logical) index of physical particle q.  vort,curv stay in logical order.
 //   frame      : slot[fx*fy*fz] (int32: active slot or -1), coord[slot] (linear block index),
 //                nbr[slot*6] (slot of the -x,+x,-y,+y,-z,+z neighbour block, -1 inactive,
 //                -2 outside the domain).

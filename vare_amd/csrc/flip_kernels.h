@@ -21,23 +21,35 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
 // Each particle marks the 3x3x3 block neighbourhood starting at c-1 with c = max((int)p/B, 0).
 // The linear index uses the reference's signed int arithmetic, so a -1 coordinate aliases into
 // the next axis exactly as in the reference; indices below zero (out of bounds in the reference)
-// are dropped.  Only "count > 0" matters downstream, so a plain store of 1 replaces the atomicAdd.
-// Lanes whose block equals the previous lane's block skip the 27 stores (particles arrive in
-// spatially coherent order).
+// are dropped.  Only "count > 0" matters downstream, so a plain store replaces the atomicAdd.
+//
+// Two passes.  k_bin_occ records which blocks c hold a particle (occ, stamped with the step
+// number, on the frame extended by one block per axis: a particle with c = f still marks the
+// layer c-1 = f-1, one with c > f marks nothing); lanes whose block equals the previous lane's skip
+// the store, and particles stored in bin order give one store per wave.  k_bin_dilate then marks the
+// neighbourhood of every occupied block once, so a block's 27 flags are written once per step
+// rather than once per wave of its particles.
 template <int B>
-__global__ void k_bin_mark(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-                           int np, int fx, int fy, int fz, int* __restrict__ flag) {
+__global__ void k_bin_occ(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+                          int np, int fx, int fy, int fz, int stamp, int* __restrict__ occ) {
     const int i = blockIdx.x * WG + threadIdx.x;
-    int ci = -1000000, cj = 0, ck = 0;
+    int e = -1;
     if (i < np) {
-        ci = rmax((int)(px[i]) / B, 0);
-        cj = rmax((int)(py[i]) / B, 0);
-        ck = rmax((int)(pz[i]) / B, 0);
+        const int ci = rmax((int)(px[i]) / B, 0);
+        const int cj = rmax((int)(py[i]) / B, 0);
+        const int ck = rmax((int)(pz[i]) / B, 0);
+        if (ci <= fx && cj <= fy && ck <= fz) e = (ck * (fy + 1) + cj) * (fx + 1) + ci;
     }
     const int lane = threadIdx.x & 63;
-    const int pi = __shfl_up(ci, 1, 64), pj = __shfl_up(cj, 1, 64), pk = __shfl_up(ck, 1, 64);
-    if (i >= np) return;
-    if (lane > 0 && pi == ci && pj == cj && pk == ck) return;
+    const int prev = __shfl_up(e, 1, 64);
+    if (e >= 0 && (lane == 0 || prev != e)) occ[e] = stamp;
+}
+
+__global__ void k_bin_dilate(int fx, int fy, int fz, int stamp, const int* __restrict__ occ, int* __restrict__ flag) {
+    const int e = blockIdx.x * WG + threadIdx.x;
+    if (e >= (fx + 1) * (fy + 1) * (fz + 1)) return;
+    if (occ[e] != stamp) return;
+    const int ci = e % (fx + 1), cj = (e / (fx + 1)) % (fy + 1), ck = e / ((fx + 1) * (fy + 1));
     const int ke = rmin(ck + 1, fz - 1), je = rmin(cj + 1, fy - 1), ie = rmin(ci + 1, fx - 1);
     for (int k = ck - 1; k <= ke; ++k)
         for (int j = cj - 1; j <= je; ++j)
