@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per processed cell of one launch, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # on this benchmark (20 steps) and calibrated against plain 1 GiB copies (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0); profiles/hbm_traffic_dam_break_256.md.
-SPMV_TRAFFIC_PER_CELL = 9.58      # k_spmv_blk: (2 x 13273.9 + 16366.3) KiB per 4.588 M cells
+PSPMV_TRAFFIC_PER_CELL = None     # k_pupdate_spmv: not measured yet
 MG0_TRAFFIC_PER_CELL = 12.61      # k_mg0_prolong_smooth: (2 x 20825.3 + 14803.1) KiB per 4.586 M cells
 DT = 1.0 / 240.0
 
@@ -204,8 +204,8 @@ def run_scene(a, L, cfg, steps):
         # per cell it processes: read z 4 + coarse z 0.5 (one per 8 cells) + r 4 + descriptor 2, write
         # the smoothed z 4.  Neighbour values are re-reads.
         roofline = line("k_mg0_prolong_smooth", "mg0", 14.5, MG0_TRAFFIC_PER_CELL)
-        # the PCG's A*p: read p 4 + descriptor 2, write Ap 4
-        roofline["others"] = [line("k_spmv_blk", "spmv", 10.0, SPMV_TRAFFIC_PER_CELL)]
+        # the PCG's direction update fused with A*p: read old p 4 + z 4 + descriptor 2, write p 4 + Ap 4
+        roofline["others"] = [line("k_pupdate_spmv", "spmv", 18.0, PSPMV_TRAFFIC_PER_CELL)]
     sm = np.array(step_ms)
     nx, ny, nz = sc["dims"]
     res = {

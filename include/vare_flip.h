@@ -254,14 +254,14 @@ int vare_device_name(char* buf, size_t cap);
 /* Kernel timing.  `mask` is a bitmask of kernel classes (1 << VARE_KERNEL_*); each launch of a
  * selected class is bracketed by HIP events on the handle's stream (the stream the kernel runs
  * on) and the durations are accumulated until reset.  0 disables. */
-#define VARE_KERNEL_SPMV     0   /* CG: A*p + p.Ap */
-#define VARE_KERNEL_UPDATE   1   /* CG: x,r update + r.r */
+#define VARE_KERNEL_SPMV     0   /* CG: p = z + beta p fused with A*p + p.Ap (iterations after a solve's first) */
+#define VARE_KERNEL_UPDATE   1   /* CG: x,r update + r.r (with multigrid: fused with the first level-0 sweeps) */
 #define VARE_KERNEL_P2G      2
 #define VARE_KERNEL_G2P      3
 #define VARE_KERNEL_BIN      4
 #define VARE_KERNEL_DIV      5
 #define VARE_KERNEL_APPLY    6
-#define VARE_KERNEL_PUPDATE  7   /* CG: p = r + beta p */
+#define VARE_KERNEL_PUPDATE  7   /* unused: the direction update is part of VARE_KERNEL_SPMV */
 #define VARE_KERNEL_MG0      8   /* multigrid: level-0 prolongation + Jacobi sweep */
 #define VARE_KERNEL_COUNT    9
 int vare_flip_set_kernel_timing(vare_flip* h, int mask);

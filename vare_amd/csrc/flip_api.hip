@@ -129,6 +129,9 @@ struct vare_flip {
     float *mass = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *cnt = nullptr;
     float *cx = nullptr, *cy = nullptr, *cz = nullptr;
     float *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *b = nullptr, *z = nullptr;
+    // second buffers of p and r: the fused iteration kernels read the old vector across block faces
+    // while writing the new one, so iteration k of a solve writes p and r to the buffer of parity k
+    float *p2 = nullptr, *r2 = nullptr;
     double* x64 = nullptr;        // FP64 pressure of the iterative refinement
     uint16_t* info = nullptr;
     uint8_t* type = nullptr;
@@ -226,6 +229,7 @@ void free_cells(vare_flip* h) {
     dfree(h->mass); dfree(h->vx); dfree(h->vy); dfree(h->vz); dfree(h->cnt);
     dfree(h->cx); dfree(h->cy); dfree(h->cz);
     dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b); dfree(h->z); dfree(h->x64);
+    dfree(h->r2); dfree(h->p2);
     dfree(h->info); dfree(h->type); dfree(h->rank);
     dfree(h->partials); dfree(h->partials2);
     h->bcap = 0;
@@ -247,6 +251,7 @@ int ensure_blocks(vare_flip* h, size_t nb) {
     CHK(dalloc(&h->cnt, nc));
     CHK(dalloc(&h->cx, nc)); CHK(dalloc(&h->cy, nc)); CHK(dalloc(&h->cz, nc));
     CHK(dalloc(&h->x, nc)); CHK(dalloc(&h->r, nc)); CHK(dalloc(&h->p, nc)); CHK(dalloc(&h->ap, nc));
+    CHK(dalloc(&h->r2, nc)); CHK(dalloc(&h->p2, nc));
     CHK(dalloc(&h->b, nc)); CHK(dalloc(&h->z, nc)); CHK(dalloc(&h->x64, nc));
     CHK(dalloc(&h->info, nc)); CHK(dalloc(&h->type, nc)); CHK(dalloc(&h->rank, nc));
     CHK(dalloc(&h->partials, npart)); CHK(dalloc(&h->partials2, npart));
@@ -507,7 +512,7 @@ template <int B> int stage_p2g(vare_flip* h) {
 CgVecs vecs(vare_flip* h) { return CgVecs{h->x, h->r, h->p, h->ap, h->b, h->z, h->info}; }
 
 int mg_setup(vare_flip* h);
-void enqueue_vcycle(vare_flip* h, double* rz = nullptr);
+void enqueue_vcycle(vare_flip* h, double* rz = nullptr, float* r0 = nullptr, bool skip_first = false);
 
 // z = M^-1 r (multigrid: already in z), p = z and r.z over the solver's blocks.
 void enqueue_precond_init(vare_flip* h) {
@@ -624,7 +629,9 @@ int mg_setup(vare_flip* h) {
 // cells): level 0, and for B = 8 also level 1 (edge 4), run over the solver's blocks, Fine<E>::K
 // per workgroup.  A level-l cell is fluid only if a level-0 cell of its block is, so the listed
 // blocks hold every fluid cell of these levels.
-template <int E> void enqueue_block_down(vare_flip* h, int l, int nu) {
+// skip_first: the level-0 pass of two Jacobi sweeps from zero has already written mg_z[0]
+// (k_update_smooth2), so only the restriction runs.
+template <int E> void enqueue_block_down(vare_flip* h, int l, int nu, bool skip_first = false) {
     const float w = h->prm.mg_omega;
     const int* done = done_ptr(h);
     const MgLevelK& L = h->mg[l];
@@ -636,7 +643,8 @@ template <int E> void enqueue_block_down(vare_flip* h, int l, int nu) {
     if (nu == 1) {
         hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 1, h->mg_zt[l], done);
     } else {
-        hipLaunchKernelGGL(k_mg0_smooth_first2<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, h->mg_z[l], done);
+        if (!skip_first)
+            hipLaunchKernelGGL(k_mg0_smooth_first2<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, h->mg_z[l], done);
         hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 0, h->mg_z[l], done);
     }
 }
@@ -666,7 +674,11 @@ template <int E> void enqueue_block_up(vare_flip* h, int l, int nu, double* rz) 
 
 // z = V-cycle(r) into the CG's z vector; with rz != nullptr the last level-0 pass also writes the
 // per-workgroup partials of r.z (one per level-0 workgroup).
-template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz) {
+// r0 != nullptr: the level-0 right-hand side (the CG residual) is read from r0 instead of h->r;
+// skip_first: the first level-0 pass is already done (see enqueue_block_down).
+template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool skip_first) {
+    float* const r0_saved = h->mg[0].r;
+    if (r0) h->mg[0].r = r0;
     const MgFrame F = mg_frame(h);
     const float w = h->prm.mg_omega;
     const int nu = h->prm.mg_sweeps;
@@ -677,7 +689,7 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz) {
         const MgLevelK& L = h->mg[l];
         const MgLevelK& C = h->mg[l + 1];
         if (l < nblk_levels) {
-            if (l == 0) enqueue_block_down<B>(h, 0, nu);
+            if (l == 0) enqueue_block_down<B>(h, 0, nu, skip_first);
             else enqueue_block_down<4>(h, 1, nu);
             continue;
         }
@@ -715,41 +727,74 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz) {
                                (const float*)h->mg_zt[l], h->mg_z[l], done);
         }
     }
+    h->mg[0].r = r0_saved;
 }
 
-void enqueue_vcycle(vare_flip* h, double* rz) {
-    if (h->B == 4) enqueue_vcycle_t<4>(h, rz);
-    else enqueue_vcycle_t<8>(h, rz);
+void enqueue_vcycle(vare_flip* h, double* rz, float* r0, bool skip_first) {
+    if (h->B == 4) enqueue_vcycle_t<4>(h, rz, r0, skip_first);
+    else enqueue_vcycle_t<8>(h, rz, r0, skip_first);
 }
 
-// One CG/PCG iteration (LinearSystemSolver.cpp:29-46) over the solver's blocks; every kernel
-// returns at once when the device-side `done` flag is set, so the host can enqueue iterations ahead
-// without syncing.
-template <int B> void enqueue_iteration(vare_flip* h) {
+// One CG/PCG iteration (LinearSystemSolver.cpp:29-46), iteration k of the current solve, over the
+// solver's blocks; every kernel returns at once when the device-side `done` flag is set, so the host
+// can enqueue iterations ahead without syncing.
+//   * Iteration k >= 1 starts with the previous iteration's direction update p = z + beta p fused
+//     into A p (k_pupdate_spmv); iteration 0 uses the p of the solve's set-up.  p of iteration k
+//     lives in the buffer of parity k (p, p2).
+//   * With the multigrid preconditioner and two sweeps, the x, r update is fused with the V-cycle's
+//     first level-0 pass (k_update_smooth2), and r alternates too: the residual before iteration k
+//     is in the buffer of parity k (r, r2).  Otherwise r is updated in place by k_cg_update.
+template <int B> void enqueue_iteration(vare_flip* h, int k) {
     const int nv = h->nvirt();
     if (!nv) return;
-    const unsigned gs = rgrid((size_t)nv);
-    const CellList cl = h->clist();
     const int nwg = (h->ns + Fine<B>::K - 1) / Fine<B>::K;
-    hipEvent_t e0, e1;
-    tpair(h, VARE_KERNEL_SPMV, (int64_t)nv, e0, e1);
-    hipExtLaunchKernelGGL(k_spmv_blk<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, e0, e1, 0, (const int*)h->wgtab,
-                          vecs(h), h->partials, (const CgState*)h->st);
+    const bool mg = h->prm.solver_mode == VARE_SOLVER_RELATIVE && h->prm.preconditioner == VARE_PRECOND_MG;
+    const bool fuse = mg && h->prm.mg_sweeps == 2;
+    float* const pbuf[2] = {h->p, h->p2};
+    float* const rbuf[2] = {h->r, h->r2};
+    float* const pk = pbuf[k & 1];
+    float* const rin = fuse ? rbuf[k & 1] : h->r;
+    float* const rout = fuse ? rbuf[(k + 1) & 1] : h->r;
+    const int* tab = h->wgtab;
+    CgVecs v = vecs(h);
+    v.r = rin;
+    if (k == 0) {
+        v.p = pk;
+        hipLaunchKernelGGL(k_spmv_blk<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, tab, v, h->partials,
+                           (const CgState*)h->st);
+    } else {
+        v.p = pbuf[(k + 1) & 1];
+        hipEvent_t e0, e1;
+        tpair(h, VARE_KERNEL_SPMV, (int64_t)nv, e0, e1);
+        hipExtLaunchKernelGGL(k_pupdate_spmv<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, e0, e1, 0, tab, v, pk,
+                              h->partials, (const CgState*)h->st);
+    }
     hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
+    v.p = pk;
+    if (fuse) {
+        tbegin(h, VARE_KERNEL_UPDATE);
+        hipLaunchKernelGGL(k_update_smooth2<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, tab, v, rout,
+                           h->prm.mg_omega, h->mg_z[0], h->partials, (const CgState*)h->st);
+        tend(h, VARE_KERNEL_UPDATE);
+        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+                           (const double*)h->partials2, nwg);
+        // the rest of the V-cycle on the new residual; its last level-0 pass reduces r.z as well
+        enqueue_vcycle(h, h->partials, rout, true);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
+        return;
+    }
+    const unsigned gs = rgrid((size_t)nv);
     tbegin(h, VARE_KERNEL_UPDATE);
-    hipLaunchKernelGGL(k_cg_update, dim3(gs), dim3(WG), 0, h->stream, nv, cl, vecs(h), h->partials, h->partials2,
+    hipLaunchKernelGGL(k_cg_update, dim3(gs), dim3(WG), 0, h->stream, nv, h->clist(), v, h->partials, h->partials2,
                        (const CgState*)h->st);
     tend(h, VARE_KERNEL_UPDATE);
     hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
                        (const double*)h->partials2, (int)gs);
-    if (h->prm.solver_mode == VARE_SOLVER_RELATIVE && h->prm.preconditioner == VARE_PRECOND_MG) {
+    if (mg) {
         // the V-cycle's last level-0 pass reduces r.z as well (one partial per workgroup)
         enqueue_vcycle(h, h->partials);
         hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
     }
-    tbegin(h, VARE_KERNEL_PUPDATE);
-    hipLaunchKernelGGL(k_cg_pupdate, dim3(gs), dim3(WG), 0, h->stream, nv, cl, vecs(h), (const CgState*)h->st);
-    tend(h, VARE_KERNEL_PUPDATE);
 }
 
 // Run CG/PCG iterations until the device-side `done` flag is set.  The iteration count of a solve
@@ -765,7 +810,7 @@ template <int B> int run_solver(vare_flip* h, int cap, int k, int& total_iters) 
         HIPCHK(hipStreamSynchronize(h->stream));
         if (h->st_host->done) break;
         if (launched > cap + 1) break;                       // defensive: the device flag ends first
-        for (int q = 0; q < chunk; ++q) enqueue_iteration<B>(h);
+        for (int q = 0; q < chunk; ++q) enqueue_iteration<B>(h, launched + q);
         HIPCHK(hipGetLastError());
         launched += chunk;
         chunk = last > 0 ? std::max(2, last / 8) : std::min(chunk * 2, 64);

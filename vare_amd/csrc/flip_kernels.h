@@ -525,19 +525,6 @@ __global__ void k_fin_beta(CgState* __restrict__ st, const double* __restrict__ 
     st->rzold = rznew;
 }
 
-// CG step 3: p = r + beta p (BLAS::addmul, LinearSystemSolver.cpp:41), or p = z + beta p.
-__global__ void k_cg_pupdate(int nv, CellList cl, CgVecs v, const CgState* __restrict__ st) {
-    if (st->done) return;
-    const float beta = st->beta;
-    const int pc = st->precond;
-    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
-        const int c = cl.cell(vi);
-        const float r = v.r[c];
-        const float z = pc == 1 ? r * jacobi_inv(v.info[c]) : (pc == 2 ? v.z[c] : r);
-        v.p[c] = z + (beta * v.p[c]);
-    }
-}
-
 // ------------------------------------------------------------------------------------------------
 // Pressure apply.  The solve runs in place on the cell layout, so FluidProjection.cu:297-306
 // (scatter x to the field) has no counterpart.  Air extrapolation, FluidProjection.cu:314-335:

@@ -178,6 +178,91 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict_
     if (l == 0) partials[blockIdx.x] = tot;     // any fixed slot per workgroup keeps the sum order fixed
 }
 
+// Search direction of one cell (BLAS::addmul, LinearSystemSolver.cpp:41): p = z + beta p, with z the
+// preconditioned residual (r for no preconditioner, r / diag for Jacobi, the V-cycle output for
+// multigrid).
+__device__ __forceinline__ float pdir(int pc, float beta, unsigned in, float r, float z, float p) {
+    const float zz = pc == 1 ? r * jacobi_inv((uint16_t)in) : (pc == 2 ? z : r);
+    return zz + (beta * p);
+}
+
+// p = z + beta p (k_cg_pupdate) fused with Ap = A p and partial p.Ap (k_spmv_blk) for every CG
+// iteration after the first of a solve.  The new p across a face is recomputed from the neighbour's
+// z (or r) and old p, exactly as the neighbour's own workgroup computes it, so no pass over p is
+// needed between the two.  The old p is read from v.p and the new one written to pout, a second
+// buffer: other workgroups still read the old values across their faces.
+template <int B>
+__global__ void __launch_bounds__(Fine<B>::B3) k_pupdate_spmv(const int* __restrict__ tab, CgVecs v,
+                                                             float* __restrict__ pout,
+                                                             double* __restrict__ partials,
+                                                             const CgState* __restrict__ st) {
+    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    __shared__ float sp[K][B3];
+    __shared__ double sh[Fine<B>::NW];
+    const int dn = st->done;
+    int s[K], nb[K][6];
+    wg_load<B>(tab, s, nb);
+    if (dn) return;
+    const float beta = st->beta;
+    const int pc = st->precond;
+    const float* __restrict__ zs = pc == 2 ? v.z : v.r;
+    const int l = threadIdx.x;
+    int side[3], fl[3];
+    thread_faces<B>(l, side, fl);
+    unsigned in[K];
+    float po[K], zo[K], hp[K][3], hz[K][3];
+    unsigned hi[K][3];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        in[kb] = 0;
+        po[kb] = 0.f;
+        zo[kb] = 0.f;
+        if (s[kb] >= 0) {
+            in[kb] = v.info[s[kb] * B3 + l];
+            po[kb] = v.p[s[kb] * B3 + l];
+            zo[kb] = zs[s[kb] * B3 + l];
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int ns = face_slot(nb[kb], a, side[a]);
+            hp[kb][a] = 0.f;
+            hz[kb][a] = 0.f;
+            hi[kb][a] = 0;
+            if (ns >= 0) {
+                hp[kb][a] = v.p[ns * B3 + fl[a]];
+                hz[kb][a] = zs[ns * B3 + fl[a]];
+                if (pc == 1) hi[kb][a] = v.info[ns * B3 + fl[a]];
+            }
+        }
+    float pn[K];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        pn[kb] = pdir(pc, beta, in[kb], zo[kb], zo[kb], po[kb]);
+        sp[kb][l] = pn[kb];
+    }
+    __syncthreads();
+    double pap = 0.0;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        if (s[kb] < 0) continue;
+        float acc = 0.f;
+        if (in[kb] & kInfoFluid) {
+            float h[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) h[a] = pdir(pc, beta, hi[kb][a], hz[kb][a], hz[kb][a], hp[kb][a]);
+            acc = apply_row<B>(in[kb], pn[kb], sp[kb], l, side, h);
+            pap += (double)pn[kb] * (double)acc;
+        }
+        pout[s[kb] * B3 + l] = pn[kb];
+        v.ap[s[kb] * B3 + l] = acc;
+    }
+    const double tot = fine_block_sum<B>(pap, sh);
+    if (l == 0) partials[blockIdx.x] = tot;
+}
+
 // ---- multigrid, level 0 (the CG residual r, stencil descriptor info) and level 1 for B = 8 ----
 // First Jacobi sweep from zero at a cell: omega * r / D on fluid cells, 0 elsewhere.
 __device__ __forceinline__ float mg_first(unsigned in, float r, float omega) {
@@ -241,6 +326,88 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __
         }
         zout[s[kb] * B3 + l] = out;
     }
+}
+
+// x += alpha p, r -= alpha Ap and partial r.r (k_cg_update, LinearSystemSolver.cpp:33-35) fused with
+// the V-cycle's first level-0 pass (two Jacobi sweeps from zero on the new r, k_mg0_smooth_first2),
+// for multigrid with two sweeps.  The new r across a face is recomputed from the neighbour's r and
+// Ap, exactly as the neighbour's own workgroup computes it.  The old r is read from v.r and the new
+// one written to rout, a second buffer, because other workgroups still read the old values across
+// their faces; x is read and written only at the workgroup's own cells and is updated in place.
+template <int B>
+__global__ void __launch_bounds__(Fine<B>::B3) k_update_smooth2(const int* __restrict__ tab, CgVecs v,
+                                                               float* __restrict__ rout, float omega,
+                                                               float* __restrict__ zout,
+                                                               double* __restrict__ partials,
+                                                               const CgState* __restrict__ st) {
+    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    __shared__ float sz[K][B3];
+    __shared__ double sh[Fine<B>::NW];
+    const int dn = st->done;
+    int s[K], nb[K][6];
+    wg_load<B>(tab, s, nb);
+    if (dn) return;
+    const float alpha = st->alpha;
+    const int l = threadIdx.x;
+    int side[3], fl[3];
+    thread_faces<B>(l, side, fl);
+    unsigned in[K], hi[K][3];
+    float xo[K], po[K], ro[K], ao[K], hr[K][3], ha[K][3];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        in[kb] = 0;
+        xo[kb] = po[kb] = ro[kb] = ao[kb] = 0.f;
+        if (s[kb] >= 0) {
+            const int c = s[kb] * B3 + l;
+            in[kb] = v.info[c];
+            xo[kb] = v.x[c];
+            po[kb] = v.p[c];
+            ro[kb] = v.r[c];
+            ao[kb] = v.ap[c];
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int ns = face_slot(nb[kb], a, side[a]);
+            hi[kb][a] = 0;
+            hr[kb][a] = 0.f;
+            ha[kb][a] = 0.f;
+            if (ns >= 0) {
+                hi[kb][a] = v.info[ns * B3 + fl[a]];
+                hr[kb][a] = v.r[ns * B3 + fl[a]];
+                ha[kb][a] = v.ap[ns * B3 + fl[a]];
+            }
+        }
+    float rn[K], z1[K];
+    double rr = 0.0;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        rn[kb] = ro[kb] - (alpha * ao[kb]);
+        z1[kb] = mg_first(in[kb], rn[kb], omega);
+        sz[kb][l] = z1[kb];
+        if (s[kb] >= 0) rr += (double)rn[kb] * (double)rn[kb];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        if (s[kb] < 0) continue;
+        float out = 0.f;
+        if (mg_fluid(in[kb])) {
+            float hz[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) hz[a] = mg_first(hi[kb][a], hr[kb][a] - (alpha * ha[kb][a]), omega);
+            const float az = apply_row<B>(in[kb], z1[kb], sz[kb], l, side, hz);
+            out = z1[kb] + omega * (rn[kb] - az) / mg_diag(in[kb]);
+        }
+        const int c = s[kb] * B3 + l;
+        v.x[c] = xo[kb] + (alpha * po[kb]);
+        rout[c] = rn[kb];
+        zout[c] = out;
+    }
+    const double tot = fine_block_sum<B>(rr, sh);
+    if (l == 0) partials[blockIdx.x] = tot;
 }
 
 // Residual and restriction to the next level inside the block (the 2^3 children of a coarse cell
