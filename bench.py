@@ -321,6 +321,7 @@ def run_served(a, L):
                                "arrivals (mean gap %.1f ms = 0.5 x isolated solve), %d slots, 200 GB budget"
                                % (nreq, solver_desc(a, 4), 1e3 * mean_gap, a.slots),
                    "particles_total": total_particles, "max_resident": served["max_resident"],
+                   "handles_created": served["handles_created"], "handles_reused": served["handles_reused"],
                    "peak_reserved_gb": round(served["peak_bytes"] / 2 ** 30, 2), "wall_s": round(served["wall_s"], 3)},
         "plain_gpu": {"value": round(plain["particle_steps_per_s"], 1), "p50_ms": round(plain["step_p50_ms"], 3),
                       "p99_ms": round(plain["step_p99_ms"], 3), "wall_s": round(plain["wall_s"], 3),

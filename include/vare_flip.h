@@ -306,7 +306,9 @@ typedef struct vare_server_stats {
     double particle_steps_per_s;
     int max_resident;                      /* most tanks resident at once */
     int pad;
-    int64_t peak_bytes;                    /* peak reserved HBM */
+    int64_t peak_bytes;                    /* peak reserved HBM (resident tanks + pooled handles) */
+    int64_t handles_created;               /* device states allocated by the last run */
+    int64_t handles_reused;                /* requests served by a pooled device state of the same grid */
 } vare_server_stats;
 
 int vare_server_create(size_t hbm_budget_bytes, int max_slots, vare_server** out);

@@ -76,3 +76,25 @@ def test_request_over_budget_rejected():
     with pytest.raises(L.VareError) as e:
         srv.submit(g, params(n), pos, vel, steps=1, dt=DT)
     assert e.value.code == L.VARE_EBUDGET
+
+
+def test_pooled_handle_reuse_same_result():
+    # one slot: the second tank (same grid) is served by the first tank's pooled device state and must
+    # give exactly what a fresh handle gives (every kernel of the step is deterministic)
+    n = 40
+    g = L.grid_desc(n, n, n, block_size=8)
+    pa, va = tank(n, 0.6, 0.5, 6, 21)
+    pb, vb = tank(n, 0.3, 0.6, 5, 22)
+    fresh = L.Server(8 << 30, 1)
+    b0 = fresh.submit(g, params(n), pb, vb, steps=4, dt=DT)
+    fresh.run()
+    p0, v0, s0 = fresh.result(b0)
+    srv = L.Server(8 << 30, 1)
+    srv.submit(g, params(n), pa, va, steps=4, dt=DT, arrival_s=0.0)
+    b1 = srv.submit(g, params(n), pb, vb, steps=4, dt=DT, arrival_s=0.001)
+    srv.run()
+    st = srv.stats()
+    assert st["handles_created"] == 1 and st["handles_reused"] == 1, st
+    p1, v1, s1 = srv.result(b1)
+    assert s0["status"] == 0 and s1["status"] == 0
+    assert np.array_equal(p1, p0) and np.array_equal(v1, v0)

@@ -112,7 +112,7 @@ class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_int64), ("steps", C.c_int64), ("wall_s", C.c_double),
                 ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double), ("step_mean_ms", C.c_double),
                 ("particle_steps_per_s", C.c_double), ("max_resident", C.c_int), ("pad", C.c_int),
-                ("peak_bytes", C.c_int64)]
+                ("peak_bytes", C.c_int64), ("handles_created", C.c_int64), ("handles_reused", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}

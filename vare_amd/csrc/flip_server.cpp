@@ -5,7 +5,13 @@
 // request has arrived, (b) a slot is free and (c) its reserved bytes fit in what is left of the HBM
 // budget.  Each slot is a worker thread that owns one vare_flip handle (its own HIP stream) for the
 // lifetime of the request, so all resident tanks step concurrently on the device.  A finished tank
-// releases its slot and budget immediately and the next arrived request is admitted.
+// releases its slot immediately and the next arrived request is admitted.
+//
+// Handle pool: a finished tank's handle (device state sized for its grid) is kept, not destroyed, and
+// the next request with the same grid and velocity storage takes it over, so serving does not
+// allocate or free device memory per request (hipFree waits for the whole device, which would stall
+// every other resident tank).  Pooled handles keep their bytes counted against the budget; when a
+// request does not fit, pooled handles are destroyed, least recently used first, until it does.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -45,9 +51,27 @@ size_t request_bytes(const vare_tank_request& r) {
 
 }  // namespace
 
+struct Pooled {
+    vare_flip* h = nullptr;
+    vare_grid_desc grid;
+    int fp16 = 0;
+    size_t bytes = 0;
+    uint64_t last_use = 0;
+};
+
+bool same_grid(const vare_grid_desc& a, const vare_grid_desc& b) {
+    return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.block_size == b.block_size && a.min[0] == b.min[0] &&
+           a.min[1] == b.min[1] && a.min[2] == b.min[2] && a.max[0] == b.max[0] && a.max[1] == b.max[1] &&
+           a.max[2] == b.max[2];
+}
+
 struct vare_server {
     size_t budget = 0;
     int max_slots = 1;
+    std::vector<Pooled> pool;         // idle handles, their bytes included in `used`
+    uint64_t use_clock = 0;
+    int64_t handles_created = 0;
+    int64_t handles_reused = 0;
     std::vector<std::unique_ptr<Request>> reqs;
     std::mutex mu;
     std::condition_variable cv;
@@ -67,11 +91,48 @@ double since(const Clock::time_point& t0) {
     return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
-void run_request(vare_server* s, Request* q) {
+// Take the pooled handle for this request's grid out of the pool (h == nullptr if there is none).
+// Called with s->mu held; its bytes leave `used` with it.
+Pooled take_pooled(vare_server* s, Request* q) {
+    const int fp16 = q->req.params.velocity_fp16 ? 1 : 0;
+    for (size_t i = 0; i < s->pool.size(); ++i) {
+        if (s->pool[i].fp16 != fp16 || !same_grid(s->pool[i].grid, q->req.grid)) continue;
+        Pooled pd = s->pool[i];
+        s->used -= pd.bytes;
+        s->pool.erase(s->pool.begin() + (long)i);
+        return pd;
+    }
+    return Pooled();
+}
+
+// Put a handle back into the pool.  Called with s->mu held.
+void put_pooled(vare_server* s, const Pooled& pd) {
+    s->pool.push_back(pd);
+    s->used += pd.bytes;
+    s->peak = std::max(s->peak, s->used);
+}
+
+// Destroy pooled handles, least recently used first, until `need` more bytes fit in the budget.
+// Called with s->mu held.
+void evict_for(vare_server* s, size_t need) {
+    while (!s->pool.empty() && s->used + need > s->budget) {
+        size_t lru = 0;
+        for (size_t i = 1; i < s->pool.size(); ++i)
+            if (s->pool[i].last_use < s->pool[lru].last_use) lru = i;
+        vare_flip_destroy(s->pool[lru].h);
+        s->used -= s->pool[lru].bytes;
+        s->pool.erase(s->pool.begin() + (long)lru);
+    }
+}
+
+// Runs the request on h (a pooled handle) or on a new handle; returns the handle to pool afterwards
+// (nullptr when it failed and was destroyed).
+vare_flip* run_request(vare_server* s, Request* q, vare_flip* h) {
     const double admit = since(s->t0);
     q->st.admit_s = admit;
-    vare_flip* h = nullptr;
-    int rc = vare_flip_create(&q->req.grid, &q->req.params, q->req.n, 0, &h);
+    int rc = VARE_OK;
+    if (h) rc = vare_flip_set_params(h, &q->req.params);
+    else rc = vare_flip_create(&q->req.grid, &q->req.params, q->req.n, 0, &h);
     if (rc == VARE_OK)
         rc = vare_flip_set_particles(h, q->pos.data(), q->vel.data(), nullptr, nullptr, nullptr, q->req.n,
                                      VARE_SPACE_VOXEL, VARE_MEM_HOST);
@@ -90,7 +151,10 @@ void run_request(vare_server* s, Request* q) {
                                      VARE_SPACE_VOXEL, VARE_MEM_HOST);
         q->st.num_particles = (int64_t)n;
     }
-    if (h) vare_flip_destroy(h);
+    if (h && rc != VARE_OK) {
+        vare_flip_destroy(h);
+        h = nullptr;
+    }
     q->st.status = rc;
     q->st.steps_done = (int)q->step_ms.size();
     double sum = 0.0, mx = 0.0;
@@ -98,11 +162,13 @@ void run_request(vare_server* s, Request* q) {
     q->st.step_ms_mean = q->step_ms.empty() ? 0.0 : sum / q->step_ms.size();
     q->st.step_ms_max = mx;
     q->st.finish_s = since(s->t0);
+    return h;
 }
 
 void worker(vare_server* s) {
     for (;;) {
         Request* q = nullptr;
+        vare_flip* h = nullptr;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
@@ -113,10 +179,16 @@ void worker(vare_server* s) {
                     s->cv.wait_for(lk, std::chrono::duration<double>(head->req.arrival_s - now));
                     continue;
                 }
-                if (s->used + head->bytes > s->budget) {   // FIFO: wait for memory to free up
+                const Pooled pd = take_pooled(s, head);
+                evict_for(s, head->bytes);
+                if (s->used + head->bytes > s->budget) {   // FIFO: wait for resident tanks to finish
+                    if (pd.h) put_pooled(s, pd);
                     s->cv.wait(lk);
                     continue;
                 }
+                h = pd.h;
+                if (h) s->handles_reused++;
+                else s->handles_created++;
                 q = head;
                 s->next++;
                 s->used += q->bytes;
@@ -126,10 +198,21 @@ void worker(vare_server* s) {
                 break;
             }
         }
-        run_request(s, q);
+        h = run_request(s, q, h);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->used -= q->bytes;
+            if (h && (int)s->pool.size() < s->max_slots) {      // the request's bytes pass to the pool
+                Pooled pd;
+                pd.h = h;
+                pd.grid = q->req.grid;
+                pd.fp16 = q->req.params.velocity_fp16 ? 1 : 0;
+                pd.bytes = q->bytes;
+                pd.last_use = ++s->use_clock;
+                put_pooled(s, pd);
+            } else if (h) {
+                vare_flip_destroy(h);
+            }
             s->resident--;
             q->done = true;
         }
@@ -157,6 +240,8 @@ int vare_server_create(size_t budget, int max_slots, vare_server** out) {
 }
 
 int vare_server_destroy(vare_server* s) {
+    if (!s) return VARE_OK;
+    for (auto& pd : s->pool) vare_flip_destroy(pd.h);
     delete s;
     return VARE_OK;
 }
@@ -188,8 +273,11 @@ int vare_server_run(vare_server* s) {
     std::stable_sort(s->order.begin(), s->order.end(),
                      [&](size_t a, size_t b) { return s->reqs[a]->req.arrival_s < s->reqs[b]->req.arrival_s; });
     s->next = 0;
+    s->handles_created = 0;
+    s->handles_reused = 0;
     s->used = 0;
-    s->peak = 0;
+    for (const Pooled& pd : s->pool) s->used += pd.bytes;
+    s->peak = s->used;
     s->resident = 0;
     s->max_resident = 0;
     s->t0 = Clock::now();
@@ -248,6 +336,8 @@ int vare_server_get_stats(vare_server* s, vare_server_stats* o) {
     o->particle_steps_per_s = o->wall_s > 0 ? psteps / o->wall_s : 0.0;
     o->max_resident = s->max_resident;
     o->peak_bytes = (int64_t)s->peak;
+    o->handles_created = s->handles_created;
+    o->handles_reused = s->handles_reused;
     return VARE_OK;
 }
 
