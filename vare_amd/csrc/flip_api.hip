@@ -896,9 +896,14 @@ template <int B> int stage_g2p(vare_flip* h, float dt) {
     tbegin(h, VARE_KERNEL_G2P);
     if (h->np && h->prm.particle_path == 0) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
-        hipLaunchKernelGGL(k_g2p_tile<B>, dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
-                           h->vel(), h->pa[6], (const int*)h->cstart, h->nb,
-                           h->frame(), gf, g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
+        if (h->colk().any())
+            hipLaunchKernelGGL((k_g2p_tile<B, true>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1],
+                               h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
+                               g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
+        else
+            hipLaunchKernelGGL((k_g2p_tile<B, false>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1],
+                               h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
+                               g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
     } else if (h->np) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
         hipLaunchKernelGGL(k_g2p<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],

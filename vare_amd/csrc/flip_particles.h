@@ -325,7 +325,9 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
 }
 
 // G2P for one particle given a cell-value fetcher (FLIPMethod.cu:86-171, SparseField.h:150-196).
-template <int B, class Fetch>
+// COL = false compiles the collision queries out (no collision source is active), which leaves the
+// kernel with fewer registers and so more waves in flight.
+template <int B, bool COL, class Fetch>
 __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
                                             const VelPtr& u, float* __restrict__ adj, const FrameK& f, const G2PParams& gp,
                                             const ColK& col, uint8_t* __restrict__ removed, Fetch fetch) {
@@ -337,21 +339,24 @@ __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float
     const float fxw = rclamp(x - ((float)i + 0.5f), 0.f, 1.f);
     const float fyw = rclamp(y - ((float)j + 0.5f), 0.f, 1.f);
     const float fzw = rclamp(z - ((float)k + 0.5f), 0.f, 1.f);
-    float v[7][8];     // fields: cnt, cx, cy, cz, vx0, vy0, vz0 at the 8 corners (dx + 2dy + 4dz)
-#pragma unroll
-    for (int q = 0; q < 8; ++q) fetch(i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2), q, v);
+    // fields fi: 0 cnt, 1-3 cx, cy, cz, 4-6 vx0, vy0, vz0; corner q = dx + 2dy + 4dz.  One field is
+    // fetched and interpolated at a time, which keeps 8 values live instead of 56.
+    auto corner = [&](int fi, int q) -> float { return fetch(fi, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2)); };
     {
-        float a = rmin(v[0][0], v[0][1]);
-        a = rmin(a, rmin(v[0][4], v[0][5]));
-        a = rmin(a, rmin(v[0][2], v[0][3]));
-        a = rmin(a, rmin(v[0][6], v[0][7]));
+        float a = rmin(corner(0, 0), corner(0, 1));
+        a = rmin(a, rmin(corner(0, 4), corner(0, 5)));
+        a = rmin(a, rmin(corner(0, 2), corner(0, 3)));
+        a = rmin(a, rmin(corner(0, 6), corner(0, 7)));
         adj[n] = a;
     }
     auto lerp = [&](int fi) -> float {
-        const float c00 = v[fi][0] * (1.f - fxw) + v[fi][1] * fxw;
-        const float c01 = v[fi][4] * (1.f - fxw) + v[fi][5] * fxw;
-        const float c10 = v[fi][2] * (1.f - fxw) + v[fi][3] * fxw;
-        const float c11 = v[fi][6] * (1.f - fxw) + v[fi][7] * fxw;
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = corner(fi, q);
+        const float c00 = v[0] * (1.f - fxw) + v[1] * fxw;
+        const float c01 = v[4] * (1.f - fxw) + v[5] * fxw;
+        const float c10 = v[2] * (1.f - fxw) + v[3] * fxw;
+        const float c11 = v[6] * (1.f - fxw) + v[7] * fxw;
         const float c0 = c00 * (1.f - fyw) + c10 * fyw;
         const float c1 = c01 * (1.f - fyw) + c11 * fyw;
         return c0 * (1.f - fzw) + c1 * fzw;
@@ -371,7 +376,7 @@ __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float
         if (z > lz - band) w = rmin(w, rclamp((lz - z) / band, 0.f, 1.f));
         vx *= w; vy *= w; vz *= w;
     }
-    const bool cols = col.any();
+    const bool cols = COL && col.any();
     if (cols && col_phi<B>(col, f, x, y, z) < 0.f) {          // collision push (FLIPMethod.cu:116-119)
         float cv[3];
         col_vec<B>(col, f, 0, x, y, z, cv);
@@ -439,8 +444,8 @@ struct G2PFields {
 // tile of the 7 gathered fields over the block and a 1-cell halo, which holds every stencil
 // i = clamp((int)(p-0.5), 0, n-2) .. i+1 of those particles.  Workgroup nb takes bin Nc (particles
 // outside the domain) and reads the fields through the block table.
-template <int B>
-__global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
+template <int B, bool COL>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(COL ? 4 : 5))) k_g2p_tile(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
                                                 VelPtr u, float* __restrict__ adj,
                                                 const int* __restrict__ cstart, int nb, FrameK f, G2PFields gf,
                                                 G2PParams gp, ColK col, uint8_t* __restrict__ removed,
@@ -455,13 +460,12 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* 
     int rem_count = 0;
     if (s == nb) {
         // particles outside the domain: global lookups (cells of inactive blocks read background)
-        auto fetch = [&](int i, int j, int k, int q, float (&v)[7][8]) {
+        auto fetch = [&](int fi, int i, int j, int k) -> float {
             const int c = cell_index<B>(f, i, j, k);
-#pragma unroll
-            for (int fi = 0; fi < 7; ++fi) v[fi][q] = c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
+            return c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
         };
         for (int q = p0 + threadIdx.x; q < p1; q += WG)
-            rem_count += g2p_particle<B>(q, px, py, pz, u, adj, f, gp, col, removed, fetch);
+            rem_count += g2p_particle<B, COL>(q, px, py, pz, u, adj, f, gp, col, removed, fetch);
     } else {
         load_nbhd(f, s, nsl);
         __syncthreads();
@@ -473,20 +477,16 @@ __global__ void __launch_bounds__(WG) k_g2p_tile(float* __restrict__ px, float* 
         __syncthreads();
         const int t0 = f.coord[s];
         const int ox = (t0 % f.fx) * B - 1, oy = ((t0 / f.fx) % f.fy) * B - 1, oz = (t0 / (f.fx * f.fy)) * B - 1;
-        auto fetch = [&](int i, int j, int k, int q, float (&v)[7][8]) {
-            const int ti = i - ox, tj = j - oy, tk = k - oz;
-            if (ti >= 0 && tj >= 0 && tk >= 0 && ti < T && tj < T && tk < T) {
-                const int t = (tk * T + tj) * T + ti;
-#pragma unroll
-                for (int fi = 0; fi < 7; ++fi) v[fi][q] = tile[fi][t];
-            } else {
-                const int c = cell_index<B>(f, i, j, k);
-#pragma unroll
-                for (int fi = 0; fi < 7; ++fi) v[fi][q] = c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
-            }
+        // Every corner lies in the tile: a particle binned here has its P2G base cell c (per axis
+        // max((int)(p-0.5), 0)) in this block, and its G2P stencil base clamp((int)(p-0.5), 0, n-2)
+        // is c, or c-1 when c = n-1, so its corners lie in [c-1, c+1], inside the block +- 1 cell.
+        // The clamp only keeps the LDS index in range.
+        auto fetch = [&](int fi, int i, int j, int k) -> float {
+            const int ti = rclamp(i - ox, 0, T - 1), tj = rclamp(j - oy, 0, T - 1), tk = rclamp(k - oz, 0, T - 1);
+            return tile[fi][(tk * T + tj) * T + ti];
         };
         for (int q = p0 + threadIdx.x; q < p1; q += WG)
-            rem_count += g2p_particle<B>(q, px, py, pz, u, adj, f, gp, col, removed, fetch);
+            rem_count += g2p_particle<B, COL>(q, px, py, pz, u, adj, f, gp, col, removed, fetch);
     }
     // wave-aggregated removal count
     const int w = wave_sum_i(rem_count);
@@ -501,12 +501,11 @@ __global__ void k_g2p(float* __restrict__ px, float* __restrict__ py, float* __r
                       uint8_t* __restrict__ removed, int* __restrict__ nremoved) {
     const int n = blockIdx.x * WG + threadIdx.x;
     const float* src[7] = {gf.cnt, gf.cx, gf.cy, gf.cz, gf.vx0, gf.vy0, gf.vz0};
-    auto fetch = [&](int i, int j, int k, int q, float (&v)[7][8]) {
+    auto fetch = [&](int fi, int i, int j, int k) -> float {
         const int c = cell_index<B>(f, i, j, k);
-#pragma unroll
-        for (int fi = 0; fi < 7; ++fi) v[fi][q] = c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
+        return c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
     };
-    const int rem = n < np ? g2p_particle<B>(n, px, py, pz, u, adj, f, gp, col, removed, fetch) : 0;
+    const int rem = n < np ? g2p_particle<B, true>(n, px, py, pz, u, adj, f, gp, col, removed, fetch) : 0;
     const unsigned long long m = __ballot(rem);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(nremoved, (int)__popcll(m));
 }
