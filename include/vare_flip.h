@@ -81,7 +81,7 @@ typedef struct vare_flip_params {
     int preconditioner;      /* relative mode: VARE_PRECOND_* */
     int mg_sweeps;           /* multigrid: Jacobi sweeps before and after the coarse correction (1|2) */
     float mg_omega;          /* multigrid: Jacobi weight, default 0.8 */
-    int mg_coarse_sweeps;    /* multigrid: sweeps on the coarsest grid, default 40 */
+    int mg_coarse_sweeps;    /* multigrid: sweeps on the coarsest grid, default 20 */
     int particle_path;       /* 0: block-binned P2G/G2P through LDS tiles (default); 1: one thread per
                                 particle with global atomics / gathers */
     float refine_tol;        /* > 0: FP64 iterative refinement of the FP32 PCG until the FP64

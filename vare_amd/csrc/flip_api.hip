@@ -685,7 +685,14 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
     const int* done = done_ptr(h);
     const int m = (int)h->mg.size() - 1;
     const int nblk_levels = B == 8 ? 2 : 1;                    // levels run by the block kernels
-    for (int l = 0; l < m; ++l) {
+    // levels lb..m (dense, together at most kMgBottomCells cells) run inside k_mg_bottom
+    int lb = m, tot = h->mg[m].ncell;
+    while (lb - 1 >= nblk_levels && h->mg[lb - 1].dense && tot + h->mg[lb - 1].ncell <= kMgBottomCells &&
+           m - lb + 2 <= kMgBottomLevels) {
+        --lb;
+        tot += h->mg[lb].ncell;
+    }
+    for (int l = 0; l < lb; ++l) {
         const MgLevelK& L = h->mg[l];
         const MgLevelK& C = h->mg[l + 1];
         if (l < nblk_levels) {
@@ -706,9 +713,20 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
                                    w, 0, h->mg_z[l], done);
         }
     }
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kMgCoarseMax), 0, h->stream, h->mg[m], w, h->prm.mg_coarse_sweeps,
-                       h->mg_z[m], done);
-    for (int l = m - 1; l >= 0; --l) {
+    MgBottom bt{};
+    bt.nlev = m - lb + 1;
+    for (int q = 0, off = 0; q < bt.nlev; ++q) {
+        const MgLevelK& L = h->mg[lb + q];
+        bt.dx[q] = L.dx; bt.dy[q] = L.dy; bt.dz[q] = L.dz;
+        bt.off[q] = off;
+        bt.info[q] = L.info;
+        off += L.ncell;
+    }
+    bt.r0 = h->mg[lb].r;
+    bt.z0 = h->mg_z[lb];
+    hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(kMgBottomThreads), 0, h->stream, bt, w, nu, h->prm.mg_coarse_sweeps,
+                       done);
+    for (int l = lb - 1; l >= 0; --l) {
         const MgLevelK& L = h->mg[l];
         const MgLevelK& C = h->mg[l + 1];
         if (!L.ncell) continue;
@@ -1060,7 +1078,7 @@ int vare_flip_params_default(vare_flip_params* p) {
     p->preconditioner = VARE_PRECOND_NONE;
     p->mg_sweeps = 2;
     p->mg_omega = 0.8f;
-    p->mg_coarse_sweeps = 40;
+    p->mg_coarse_sweeps = 20;
     p->refine_tol = 0.f;
     p->refine_max = 8;
     return VARE_OK;

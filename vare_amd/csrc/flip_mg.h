@@ -6,8 +6,10 @@
 //   * sparse levels: the active blocks of the frame with cell edge E = B, B/2, ..., 1 per block,
 //     stored slot-major like the fine cells (cell (slot, local) at slot*E^3 + local);
 //   * dense levels: the block frame itself halved repeatedly ((fx+1)/2, ...), one array each;
-//   * the coarsest dense level (<= kMgCoarseMax cells) is smoothed to convergence by one workgroup.
-// A coarse cell is fluid when any child is fluid; its diagonal is 6 + nAir - nWall counted on the
+//   * the coarsest dense level (<= kMgCoarseMax cells) gets a fixed number of Jacobi sweeps, and the
+//     last dense levels (at most kMgBottomCells cells together) run inside one workgroup
+//     (k_mg_bottom).
+// A coarse cell is fluid when at least half of its children are (k_mg_flag); its diagonal is 6 + nAir - nWall counted on the
 // coarse grid with the fine level's boundary rule (FluidProjection.cu:208: a domain face is a wall
 // below the wall level).  Each level solves stencil_l z_l = rt_l where rt_{l+1} = 0.5 * sum of the
 // 8 children's residuals (restriction R = P^T/8 times the 4x operator scaling), prolongation is
@@ -270,45 +272,157 @@ __global__ void k_mg_prolong_smooth(MgLevelK L, MgLevelK C, MgFrame F, float ome
     zout[c] = out;
 }
 
-// Coarsest level (dense, <= kMgCoarseMax cells): `sweeps` Jacobi sweeps from zero by one
-// workgroup, one cell per thread, the iterate in LDS and the cell's row in registers.
-__global__ void __launch_bounds__(kMgCoarseMax) k_mg_coarse(MgLevelK L, float omega, int sweeps,
-                                                            float* __restrict__ zout, const int* done) {
+// ---- bottom of the hierarchy: the last dense levels, at most kMgBottomCells cells, in one workgroup ----
+// The V-cycle on these levels (down: two Jacobi sweeps from zero and the restriction; the
+// coarsest level: `coarse_sweeps` Jacobi sweeps from zero; up: prolongation and the post-sweeps)
+// runs in one 1024-thread workgroup with every level's r, z and descriptors in LDS, replacing four
+// launches per level.  Each level's arithmetic is that of k_mg_smooth_first2 / k_mg_restrict /
+// k_mg_prolong_smooth / k_mg_smooth, in the same order; the coarsest level gets `coarse_sweeps`
+// Jacobi sweeps from zero.
+constexpr int kMgBottomLevels = 6;                     // levels handled at most
+constexpr int kMgBottomCells = 5120;                   // cells of those levels together (LDS: 14 B each)
+constexpr int kMgBottomThreads = 1024;
+
+struct MgBottom {
+    int nlev;                                          // levels handled; the last is the coarsest
+    int dx[kMgBottomLevels], dy[kMgBottomLevels], dz[kMgBottomLevels];
+    int off[kMgBottomLevels];                          // first cell of each level in the LDS arrays
+    const uint16_t* info[kMgBottomLevels];
+    const float* r0;                                   // right-hand side of the first level
+    float* z0;                                         // its correction (output)
+};
+
+// diag * z - sum of the fluid neighbours' z on a dense level held in LDS (q order -x,+x,-y,+y,-z,+z).
+__device__ __forceinline__ float bottom_az(unsigned in, const float* z, int c, int sy, int sz) {
+    float az = mg_diag(in) * z[c];
+    if (in & 1u) az -= z[c - 1];
+    if (in & 2u) az -= z[c + 1];
+    if (in & 4u) az -= z[c - sy];
+    if (in & 8u) az -= z[c + sy];
+    if (in & 16u) az -= z[c - sz];
+    if (in & 32u) az -= z[c + sz];
+    return az;
+}
+
+__global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, float omega, int nu, int coarse_sweeps,
+                                                                const int* done) {
     if (*done) return;
-    __shared__ float za[kMgCoarseMax];
-    __shared__ float zb[kMgCoarseMax];
-    const int n = L.ncell;
-    const int c = threadIdx.x;
-    const bool act = c < n;
-    const uint16_t in = act ? L.info[c] : (uint16_t)0;
-    const float r = act ? L.r[c] : 0.f;
-    const bool fl = mg_fluid(in);
-    const float d = fl ? mg_diag(in) : 1.f;
-    const int sx = 1, sy = L.dx, sz = L.dx * L.dy;
-    if (act) za[c] = 0.f;
+    __shared__ float sr[kMgBottomCells];
+    __shared__ float sz[kMgBottomCells];
+    __shared__ float st[kMgBottomCells];
+    __shared__ uint16_t si[kMgBottomCells];
+    const int tid = threadIdx.x;
+    constexpr int T = kMgBottomThreads;
+    const int L = bt.nlev;
+    for (int l = 0; l < L; ++l) {
+        const int n = bt.dx[l] * bt.dy[l] * bt.dz[l];
+        for (int c = tid; c < n; c += T) si[bt.off[l] + c] = bt.info[l][c];
+    }
+    {
+        const int n = bt.dx[0] * bt.dy[0] * bt.dz[0];
+        for (int c = tid; c < n; c += T) sr[c] = bt.r0[c];
+    }
     __syncthreads();
-    float* cur = za;
-    float* nxt = zb;
-    for (int s = 0; s < sweeps; ++s) {
-        if (act) {
-            float out = 0.f;
-            if (fl) {
-                const float zc = cur[c];
-                float az = d * zc;
-                if (in & 1u) az -= cur[c - sx];
-                if (in & 2u) az -= cur[c + sx];
-                if (in & 4u) az -= cur[c - sy];
-                if (in & 8u) az -= cur[c + sy];
-                if (in & 16u) az -= cur[c - sz];
-                if (in & 32u) az -= cur[c + sz];
-                out = zc + omega * (r - az) / d;
+    // down
+    for (int l = 0; l + 1 < L; ++l) {
+        const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
+        float* r = sr + o;
+        float* z = sz + o;
+        float* zt = st + o;
+        const uint16_t* in = si + o;
+        // first sweep from zero: omega r / D (k_mg_smooth_first2's z1, or the whole pre-smoothing for nu = 1)
+        for (int c = tid; c < n; c += T) (nu == 1 ? z : zt)[c] = mg_fluid(in[c]) ? omega * r[c] / mg_diag(in[c]) : 0.f;
+        __syncthreads();
+        if (nu == 2) {
+            for (int c = tid; c < n; c += T) {
+                float out = 0.f;
+                if (mg_fluid(in[c])) out = zt[c] + omega * (r[c] - bottom_az(in[c], zt, c, sy, szz)) / mg_diag(in[c]);
+                z[c] = out;
             }
-            nxt[c] = out;
+            __syncthreads();
+        }
+        // restriction: rC = 0.5 * sum of the fluid children's residuals r - A z, children summed as
+        // ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) like k_mg_restrict's butterflies
+        const int oc = bt.off[l + 1], cdx = bt.dx[l + 1], cdy = bt.dy[l + 1], nc = cdx * cdy * bt.dz[l + 1];
+        for (int cc = tid; cc < nc; cc += T) {
+            float acc = 0.f;
+            if (mg_fluid(si[oc + cc])) {
+                const int I = cc % cdx, J = (cc / cdx) % cdy, K = cc / (cdx * cdy);
+                float ch[8];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    const int i = 2 * I + (a & 1), j = 2 * J + ((a >> 1) & 1), k = 2 * K + (a >> 2);
+                    ch[a] = 0.f;
+                    if (i < dx && j < dy && k < bt.dz[l]) {
+                        const int c = (k * dy + j) * dx + i;
+                        if (mg_fluid(in[c])) ch[a] = r[c] - bottom_az(in[c], z, c, sy, szz);
+                    }
+                }
+                acc = 0.5f * (((ch[0] + ch[1]) + (ch[2] + ch[3])) + ((ch[4] + ch[5]) + (ch[6] + ch[7])));
+            }
+            sr[oc + cc] = acc;
         }
         __syncthreads();
-        float* t = cur; cur = nxt; nxt = t;
     }
-    if (act) zout[c] = cur[c];
+    // coarsest level: Jacobi sweeps from zero
+    {
+        const int l = L - 1;
+        const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
+        float* cur = sz + o;
+        float* nxt = st + o;
+        for (int c = tid; c < n; c += T) cur[c] = 0.f;
+        __syncthreads();
+        for (int s = 0; s < coarse_sweeps; ++s) {
+            for (int c = tid; c < n; c += T) {
+                const unsigned in = si[o + c];
+                float out = 0.f;
+                if (mg_fluid(in)) out = cur[c] + omega * (sr[o + c] - bottom_az(in, cur, c, sy, szz)) / mg_diag(in);
+                nxt[c] = out;
+            }
+            __syncthreads();
+            float* t = cur; cur = nxt; nxt = t;
+        }
+        if (cur != sz + o) {
+            for (int c = tid; c < n; c += T) sz[o + c] = cur[c];
+            __syncthreads();
+        }
+    }
+    // up
+    for (int l = L - 2; l >= 0; --l) {
+        const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
+        const int oc = bt.off[l + 1], cdx = bt.dx[l + 1], cdy = bt.dy[l + 1];
+        float* r = sr + o;
+        float* z = sz + o;
+        float* zt = st + o;
+        const uint16_t* in = si + o;
+        // zp = z + zC[parent] on every cell (k_mg_prolong_smooth adds the same two terms per cell)
+        for (int c = tid; c < n; c += T) {
+            const int i = c % dx, j = (c / dx) % dy, k = c / (dx * dy);
+            zt[c] = z[c] + sz[oc + ((k >> 1) * cdy + (j >> 1)) * cdx + (i >> 1)];
+        }
+        __syncthreads();
+        // one sweep from zp; nu = 2 adds a second (k_mg_smooth)
+        for (int c = tid; c < n; c += T) {
+            float out = 0.f;
+            if (mg_fluid(in[c])) out = zt[c] + omega * (r[c] - bottom_az(in[c], zt, c, sy, szz)) / mg_diag(in[c]);
+            z[c] = out;
+        }
+        __syncthreads();
+        if (nu == 2) {
+            for (int c = tid; c < n; c += T) {
+                float out = 0.f;
+                if (mg_fluid(in[c])) out = z[c] + omega * (r[c] - bottom_az(in[c], z, c, sy, szz)) / mg_diag(in[c]);
+                zt[c] = out;
+            }
+            __syncthreads();
+            for (int c = tid; c < n; c += T) z[c] = zt[c];
+            __syncthreads();
+        }
+    }
+    {
+        const int n = bt.dx[0] * bt.dy[0] * bt.dz[0];
+        for (int c = tid; c < n; c += T) bt.z0[c] = sz[c];
+    }
 }
 
 }  // namespace vare
