@@ -685,12 +685,17 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
     const int* done = done_ptr(h);
     const int m = (int)h->mg.size() - 1;
     const int nblk_levels = B == 8 ? 2 : 1;                    // levels run by the block kernels
-    // levels lb..m (dense, together at most kMgBottomCells cells) run inside k_mg_bottom
-    int lb = m, tot = h->mg[m].ncell;
-    while (lb - 1 >= nblk_levels && h->mg[lb - 1].dense && tot + h->mg[lb - 1].ncell <= kMgBottomCells &&
+    // levels lb..m run inside k_mg_bottom: every level below the block-kernel levels whose dense view
+    // (a sparse level: the frame at its resolution) fits, together at most kMgBottomCells cells
+    auto view_cells = [&](const MgLevelK& L) -> long long {
+        return L.dense ? (long long)L.ncell : ((long long)h->fx * h->fy * h->fz) << (3 * L.sh);
+    };
+    int lb = m;
+    long long tot = view_cells(h->mg[m]);
+    while (lb - 1 >= nblk_levels && tot + view_cells(h->mg[lb - 1]) <= kMgBottomCells &&
            m - lb + 2 <= kMgBottomLevels) {
         --lb;
-        tot += h->mg[lb].ncell;
+        tot += view_cells(h->mg[lb]);
     }
     for (int l = 0; l < lb; ++l) {
         const MgLevelK& L = h->mg[l];
@@ -715,12 +720,18 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
     }
     MgBottom bt{};
     bt.nlev = m - lb + 1;
+    bt.slot = h->slot;
+    bt.fx = h->fx;
+    bt.fy = h->fy;
     for (int q = 0, off = 0; q < bt.nlev; ++q) {
         const MgLevelK& L = h->mg[lb + q];
-        bt.dx[q] = L.dx; bt.dy[q] = L.dy; bt.dz[q] = L.dz;
+        bt.sh[q] = L.dense ? -1 : L.sh;
+        bt.dx[q] = L.dense ? L.dx : h->fx << L.sh;
+        bt.dy[q] = L.dense ? L.dy : h->fy << L.sh;
+        bt.dz[q] = L.dense ? L.dz : h->fz << L.sh;
         bt.off[q] = off;
         bt.info[q] = L.info;
-        off += L.ncell;
+        off += bt.dx[q] * bt.dy[q] * bt.dz[q];
     }
     bt.r0 = h->mg[lb].r;
     bt.z0 = h->mg_z[lb];

@@ -272,7 +272,7 @@ __global__ void k_mg_prolong_smooth(MgLevelK L, MgLevelK C, MgFrame F, float ome
     zout[c] = out;
 }
 
-// ---- bottom of the hierarchy: the last dense levels, at most kMgBottomCells cells, in one workgroup ----
+// ---- bottom of the hierarchy: the last levels, at most kMgBottomCells cells, in one workgroup ----
 // The V-cycle on these levels (down: two Jacobi sweeps from zero and the restriction; the
 // coarsest level: `coarse_sweeps` Jacobi sweeps from zero; up: prolongation and the post-sweeps)
 // runs in one 1024-thread workgroup with every level's r, z and descriptors in LDS, replacing four
@@ -283,14 +283,32 @@ constexpr int kMgBottomLevels = 6;                     // levels handled at most
 constexpr int kMgBottomCells = 5120;                   // cells of those levels together (LDS: 14 B each)
 constexpr int kMgBottomThreads = 1024;
 
+// A sparse level (edge 2^sh per block) enters as its dense view: the block frame at that level's
+// resolution, dims (fx, fy, fz) << sh, whose cell (i, j, k) is stored at slot*E^3 + local for an
+// active block and is air (not fluid) in an inactive one.
 struct MgBottom {
     int nlev;                                          // levels handled; the last is the coarsest
     int dx[kMgBottomLevels], dy[kMgBottomLevels], dz[kMgBottomLevels];
     int off[kMgBottomLevels];                          // first cell of each level in the LDS arrays
+    int sh[kMgBottomLevels];                           // sparse level: log2 of its block edge; dense: -1
     const uint16_t* info[kMgBottomLevels];
+    const int* slot;                                   // frame slot table (sparse levels)
+    int fx, fy;                                        // frame dims
     const float* r0;                                   // right-hand side of the first level
     float* z0;                                         // its correction (output)
 };
+
+// Storage index of dense-view cell c of bottom level l (-1: inactive block).
+__device__ __forceinline__ int bottom_index(const MgBottom& bt, int l, int c) {
+    const int sh = bt.sh[l];
+    if (sh < 0) return c;
+    const int dx = bt.dx[l], dy = bt.dy[l];
+    const int i = c % dx, j = (c / dx) % dy, k = c / (dx * dy);
+    const int s = bt.slot[((k >> sh) * bt.fy + (j >> sh)) * bt.fx + (i >> sh)];
+    if (s < 0) return -1;
+    const int m = (1 << sh) - 1;
+    return (s << (3 * sh)) + ((((k & m) << sh) + (j & m)) << sh) + (i & m);
+}
 
 // diag * z - sum of the fluid neighbours' z on a dense level held in LDS (q order -x,+x,-y,+y,-z,+z).
 __device__ __forceinline__ float bottom_az(unsigned in, const float* z, int c, int sy, int sz) {
@@ -316,11 +334,12 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
     const int L = bt.nlev;
     for (int l = 0; l < L; ++l) {
         const int n = bt.dx[l] * bt.dy[l] * bt.dz[l];
-        for (int c = tid; c < n; c += T) si[bt.off[l] + c] = bt.info[l][c];
-    }
-    {
-        const int n = bt.dx[0] * bt.dy[0] * bt.dz[0];
-        for (int c = tid; c < n; c += T) sr[c] = bt.r0[c];
+        for (int c = tid; c < n; c += T) {
+            const int g = bottom_index(bt, l, c);
+            const uint16_t in = g >= 0 ? bt.info[l][g] : (uint16_t)0;
+            si[bt.off[l] + c] = in;
+            if (l == 0) sr[c] = mg_fluid(in) ? bt.r0[g] : 0.f;
+        }
     }
     __syncthreads();
     // down
@@ -421,7 +440,10 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
     }
     {
         const int n = bt.dx[0] * bt.dy[0] * bt.dz[0];
-        for (int c = tid; c < n; c += T) bt.z0[c] = sz[c];
+        for (int c = tid; c < n; c += T) {
+            const int g = bottom_index(bt, 0, c);
+            if (g >= 0) bt.z0[g] = sz[c];
+        }
     }
 }
 
