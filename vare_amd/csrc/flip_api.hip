@@ -123,7 +123,7 @@ struct vare_flip {
     int* bfluid = nullptr;        // [bcap] 1 when the block holds a fluid cell (set by k_divergence)
     int* blist = nullptr;         // [bcap] slots of those blocks, ascending: the solver's blocks
     int ns = 0;                   // number of listed blocks
-    int* wgtab = nullptr;         // [(bcap/K+1)*kTab] per-workgroup block and neighbour slots (k_wg_table)
+    int* wgtab = nullptr;         // [wg_table_entries(bcap)*kTab] block and neighbour slots per pair (k_wg_table)
 
     // cells, capacity bcap*B^3
     float *mass = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr, *cnt = nullptr;
@@ -246,7 +246,7 @@ int ensure_blocks(vare_flip* h, size_t nb) {
     CHK(dalloc(&h->nbr, cap * 6));
     CHK(dalloc(&h->ccount, nc + 2)); CHK(dalloc(&h->cstart, nc + 2));
     CHK(dalloc(&h->bfluid, cap + 1)); CHK(dalloc(&h->blist, cap + 1));
-    CHK(dalloc(&h->wgtab, (cap / kWgBlocks + 1) * kTab));
+    CHK(dalloc(&h->wgtab, (size_t)wg_table_entries((int)cap) * kTab));
     CHK(dalloc(&h->mass, nc)); CHK(dalloc(&h->vx, nc)); CHK(dalloc(&h->vy, nc)); CHK(dalloc(&h->vz, nc));
     CHK(dalloc(&h->cnt, nc));
     CHK(dalloc(&h->cx, nc)); CHK(dalloc(&h->cy, nc)); CHK(dalloc(&h->cz, nc));
@@ -548,9 +548,9 @@ template <int B> int stage_system(vare_flip* h) {
     HIPCHK(hipStreamSynchronize(h->stream));
     h->ns = h->dint_host[6];
     if (h->ns) {
-        const int nwg = (h->ns + kWgBlocks - 1) / kWgBlocks;
-        hipLaunchKernelGGL(k_wg_table, dim3(nblocks((size_t)nwg * kTab)), dim3(WG), 0, h->stream,
-                           (const int*)h->blist, h->ns, (const int*)h->nbr, nwg, h->wgtab);
+        const int nent = wg_table_entries(h->ns);
+        hipLaunchKernelGGL(k_wg_table, dim3(nblocks((size_t)nent * kTab)), dim3(WG), 0, h->stream,
+                           (const int*)h->blist, h->ns, (const int*)h->nbr, nent, h->wgtab);
     }
     if (mode == 1) {
         if (precond == VARE_PRECOND_MG) {
@@ -625,8 +625,16 @@ int mg_setup(vare_flip* h) {
     return VARE_OK;
 }
 
+// Blocks per workgroup (K) of the block-aligned kernels, chosen per kernel by measurement: the
+// V-cycle passes at edge 8 load few values per cell and run best with K = 4; those at edge 4 and the
+// fused CG passes (k_pupdate_spmv, k_update_smooth2) with K = 2.
+template <int E> constexpr int mg_block_k() { return E == 8 ? 4 : 2; }
+constexpr int kIterK = 2;
+constexpr int kSpmvK = 4;
+int nwg_k(const vare_flip* h, int K) { return (h->ns + K - 1) / K; }
+
 // Block-aligned kernels for a sparse level of edge E (E^3 cells per slot, slot-major like the fine
-// cells): level 0, and for B = 8 also level 1 (edge 4), run over the solver's blocks, Fine<E>::K
+// cells): level 0, and for B = 8 also level 1 (edge 4), run over the solver's blocks, mg_block_k<E>()
 // per workgroup.  A level-l cell is fluid only if a level-0 cell of its block is, so the listed
 // blocks hold every fluid cell of these levels.
 // skip_first: the level-0 pass of two Jacobi sweeps from zero has already written mg_z[0]
@@ -636,16 +644,18 @@ template <int E> void enqueue_block_down(vare_flip* h, int l, int nu, bool skip_
     const int* done = done_ptr(h);
     const MgLevelK& L = h->mg[l];
     const MgLevelK& C = h->mg[l + 1];
-    const unsigned nwg = (unsigned)((h->ns + Fine<E>::K - 1) / Fine<E>::K);
+    constexpr int K = mg_block_k<E>();
+    const unsigned nwg = (unsigned)nwg_k(h, K);
     const int* tab = h->wgtab;
     constexpr unsigned E3 = (unsigned)(E * E * E);
     if (!nwg) return;
     if (nu == 1) {
-        hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 1, h->mg_zt[l], done);
+        hipLaunchKernelGGL((k_mg0_restrict<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 1, h->mg_zt[l], done);
     } else {
         if (!skip_first)
-            hipLaunchKernelGGL(k_mg0_smooth_first2<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, h->mg_z[l], done);
-        hipLaunchKernelGGL(k_mg0_restrict<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 0, h->mg_z[l], done);
+            hipLaunchKernelGGL((k_mg0_smooth_first2<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, h->mg_z[l],
+                               done);
+        hipLaunchKernelGGL((k_mg0_restrict<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 0, h->mg_z[l], done);
     }
 }
 
@@ -654,20 +664,21 @@ template <int E> void enqueue_block_up(vare_flip* h, int l, int nu, double* rz) 
     const float w = h->prm.mg_omega;
     const int* done = done_ptr(h);
     const MgLevelK& L = h->mg[l];
-    const unsigned nwg = (unsigned)((h->ns + Fine<E>::K - 1) / Fine<E>::K);
+    constexpr int K = mg_block_k<E>();
+    const unsigned nwg = (unsigned)nwg_k(h, K);
     const int* tab = h->wgtab;
     constexpr unsigned E3 = (unsigned)(E * E * E);
     if (!nwg) return;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (l == 0) tpair(h, VARE_KERNEL_MG0, (int64_t)h->ns * E3, e0, e1);
     if (nu == 1) {
-        hipExtLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, w,
+        hipExtLaunchKernelGGL((k_mg0_prolong_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, w,
                               (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done, rz);
     } else {
-        hipExtLaunchKernelGGL(k_mg0_prolong_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, w,
+        hipExtLaunchKernelGGL((k_mg0_prolong_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, w,
                               (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done,
                               (double*)nullptr);
-        hipLaunchKernelGGL(k_mg0_smooth<E>, dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, (const float*)h->mg_zt[l],
+        hipLaunchKernelGGL((k_mg0_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, (const float*)h->mg_zt[l],
                            h->mg_z[l], done, rz);
     }
 }
@@ -776,7 +787,9 @@ void enqueue_vcycle(vare_flip* h, double* rz, float* r0, bool skip_first) {
 template <int B> void enqueue_iteration(vare_flip* h, int k) {
     const int nv = h->nvirt();
     if (!nv) return;
-    const int nwg = (h->ns + Fine<B>::K - 1) / Fine<B>::K;
+    const int nwg_it = nwg_k(h, kIterK);                   // fused CG passes
+    const int nwg_sp = nwg_k(h, kSpmvK);                   // plain A p of a solve's first iteration
+    const int nwg_v0 = nwg_k(h, mg_block_k<B>());         // the V-cycle's level-0 passes (r.z partials)
     const bool mg = h->prm.solver_mode == VARE_SOLVER_RELATIVE && h->prm.preconditioner == VARE_PRECOND_MG;
     const bool fuse = mg && h->prm.mg_sweeps == 2;
     float* const pbuf[2] = {h->p, h->p2};
@@ -789,27 +802,28 @@ template <int B> void enqueue_iteration(vare_flip* h, int k) {
     v.r = rin;
     if (k == 0) {
         v.p = pk;
-        hipLaunchKernelGGL(k_spmv_blk<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, tab, v, h->partials,
+        hipLaunchKernelGGL((k_spmv_blk<B, kSpmvK>), dim3(nwg_sp), dim3(B * B * B), 0, h->stream, tab, v, h->partials,
                            (const CgState*)h->st);
     } else {
         v.p = pbuf[(k + 1) & 1];
         hipEvent_t e0, e1;
         tpair(h, VARE_KERNEL_SPMV, (int64_t)nv, e0, e1);
-        hipExtLaunchKernelGGL(k_pupdate_spmv<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, e0, e1, 0, tab, v, pk,
-                              h->partials, (const CgState*)h->st);
+        hipExtLaunchKernelGGL((k_pupdate_spmv<B, kIterK>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, e0, e1, 0, tab,
+                              v, pk, h->partials, (const CgState*)h->st);
     }
-    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+                       k == 0 ? nwg_sp : nwg_it);
     v.p = pk;
     if (fuse) {
         tbegin(h, VARE_KERNEL_UPDATE);
-        hipLaunchKernelGGL(k_update_smooth2<B>, dim3(nwg), dim3(B * B * B), 0, h->stream, tab, v, rout,
+        hipLaunchKernelGGL((k_update_smooth2<B, kIterK>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, tab, v, rout,
                            h->prm.mg_omega, h->mg_z[0], h->partials, (const CgState*)h->st);
         tend(h, VARE_KERNEL_UPDATE);
         hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
-                           (const double*)h->partials2, nwg);
+                           (const double*)h->partials2, nwg_it);
         // the rest of the V-cycle on the new residual; its last level-0 pass reduces r.z as well
         enqueue_vcycle(h, h->partials, rout, true);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg_v0);
         return;
     }
     const unsigned gs = rgrid((size_t)nv);
@@ -822,7 +836,7 @@ template <int B> void enqueue_iteration(vare_flip* h, int k) {
     if (mg) {
         // the V-cycle's last level-0 pass reduces r.z as well (one partial per workgroup)
         enqueue_vcycle(h, h->partials);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg_v0);
     }
 }
 

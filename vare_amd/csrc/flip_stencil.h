@@ -3,11 +3,13 @@
 //
 // Each workgroup takes K consecutive entries of the solver's block list (the blocks holding a
 // fluid cell; blocks without fluid hold no unknown and are skipped) with one thread per cell
-// position (E^3 threads: 512 for E=8, 64 for E=4), so every thread serves K cells.
+// position (E^3 threads: 512 for E=8, 64 for E=4), so every thread serves K cells.  K is 2 or 4 per
+// kernel: the passes that load many values per cell run shorter workgroups with K = 2, the lighter
+// ones keep more bytes in flight with K = 4 (measured per kernel on the 256^3 dam break).
 //
 // Latency structure.  The slots of a workgroup's K blocks and their 6 face-neighbour slots are
-// gathered once per solve into a table (k_wg_table, kTab ints per workgroup).  A kernel reads its
-// entry with uniform loads (scalar registers) together with the solver's `done` flag, and then
+// gathered once per solve into a table (k_wg_table, kTab ints per pair of blocks).  A kernel reads its
+// entries with uniform loads (scalar registers) together with the solver's `done` flag, and then
 // issues every value load of the pass at once: the K blocks' own values and the values across
 // their faces.  A value across a face is loaded whenever the neighbour block is active and used
 // only where the stencil descriptor marks that neighbour fluid, so it need not wait for the
@@ -24,20 +26,23 @@ template <int B> struct Fine {
     static constexpr int B3 = B * B * B;
     static constexpr int SH = (B == 4) ? 2 : 3;
     static constexpr int NW = B3 / 64;         // waves per workgroup
-    static constexpr int K = 4;                // blocks per workgroup
 };
 
-constexpr int kWgBlocks = 4;                   // K of every block-aligned kernel
-constexpr int kTab = 32;                       // ints per workgroup in the table (K + 6K <= kTab)
-static_assert(7 * kWgBlocks <= kTab, "workgroup table entry too small");
+constexpr int kTabBlocks = 2;                  // blocks per table entry
+constexpr int kTab = 16;                       // ints per table entry (kTabBlocks * 7 <= kTab)
+static_assert(7 * kTabBlocks <= kTab, "workgroup table entry too small");
 
-// Table entry of workgroup w: [0, K) the block slots (-1 past the end of the list), then the 6
-// neighbour slots (-x,+x,-y,+y,-z,+z; -1 inactive, -2 outside the domain) of each block.
-__global__ void k_wg_table(const int* __restrict__ blist, int nsb, const int* __restrict__ nbr, int nwg,
+// Table entries needed for the solver's ns blocks: one per pair, plus one entry of -1 so that a
+// 4-block workgroup reading two entries never passes the end.
+inline int wg_table_entries(int ns) { return (ns + kTabBlocks - 1) / kTabBlocks + 1; }
+
+// Table entry w: [0, 2) the slots of blocks 2w, 2w+1 of the list (-1 past its end), then the 6
+// neighbour slots (-x,+x,-y,+y,-z,+z; -1 inactive, -2 outside the domain) of each.
+__global__ void k_wg_table(const int* __restrict__ blist, int nsb, const int* __restrict__ nbr, int nent,
                            int* __restrict__ tab) {
-    constexpr int K = kWgBlocks;
+    constexpr int K = kTabBlocks;
     const int t = blockIdx.x * WG + threadIdx.x;
-    if (t >= nwg * kTab) return;
+    if (t >= nent * kTab) return;
     const int w = t / kTab, e = t % kTab;
     int v = -1;
     if (e < K) {
@@ -50,19 +55,22 @@ __global__ void k_wg_table(const int* __restrict__ blist, int nsb, const int* __
     tab[t] = v;
 }
 
-// This workgroup's entry (uniform addresses: loaded into scalar registers).  The empty asm
-// statements make every value live at this point, so the loads are issued here, before the
-// caller's early-exit test on the solver flag, and their latency overlaps the flag's.
-template <int B>
-__device__ __forceinline__ void wg_load(const int* __restrict__ tab, int (&s)[Fine<B>::K], int (&nb)[Fine<B>::K][6]) {
-    constexpr int K = Fine<B>::K;
-    const int* t = tab + xcd_swizzle(blockIdx.x, gridDim.x) * kTab;
+// The K blocks of this workgroup (K/2 consecutive table entries; uniform addresses, so they are
+// loaded into scalar registers).  The empty asm statements make every value live at this point, so
+// the loads are issued here, before the caller's early-exit test on the solver flag, and their
+// latency overlaps the flag's.
+template <int K>
+__device__ __forceinline__ void wg_load(const int* __restrict__ tab, int (&s)[K], int (&nb)[K][6]) {
+    static_assert(K % kTabBlocks == 0, "K must be a multiple of the table entry");
+    const int* t = tab + xcd_swizzle(blockIdx.x, gridDim.x) * (K / kTabBlocks) * kTab;
 #pragma unroll
-    for (int kb = 0; kb < K; ++kb) {
-        s[kb] = t[kb];
+    for (int g = 0; g < K / kTabBlocks; ++g)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) nb[kb][q] = t[K + 6 * kb + q];
-    }
+        for (int kb = 0; kb < kTabBlocks; ++kb) {
+            s[g * kTabBlocks + kb] = t[g * kTab + kb];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) nb[g * kTabBlocks + kb][q] = t[g * kTab + kTabBlocks + 6 * kb + q];
+        }
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         asm volatile("" ::"s"(s[kb]));
@@ -128,16 +136,16 @@ __device__ __forceinline__ float apply_row(unsigned in, float own, const float* 
 }
 
 // Ap = A p and partial p.Ap (BLAS::mul, BLAS.cu:12-40).
-template <int B>
+template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict__ tab, CgVecs v,
                                                          double* __restrict__ partials,
                                                          const CgState* __restrict__ st) {
-    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    constexpr int B3 = Fine<B>::B3;
     __shared__ float sp[K][B3];
     __shared__ double sh[Fine<B>::NW];
     const int dn = st->done;
     int s[K], nb[K][6];
-    wg_load<B>(tab, s, nb);
+    wg_load<K>(tab, s, nb);
     if (dn) return;
     const int l = threadIdx.x;
     int side[3], fl[3];
@@ -191,17 +199,17 @@ __device__ __forceinline__ float pdir(int pc, float beta, unsigned in, float r, 
 // z (or r) and old p, exactly as the neighbour's own workgroup computes it, so no pass over p is
 // needed between the two.  The old p is read from v.p and the new one written to pout, a second
 // buffer: other workgroups still read the old values across their faces.
-template <int B>
+template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_pupdate_spmv(const int* __restrict__ tab, CgVecs v,
                                                              float* __restrict__ pout,
                                                              double* __restrict__ partials,
                                                              const CgState* __restrict__ st) {
-    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    constexpr int B3 = Fine<B>::B3;
     __shared__ float sp[K][B3];
     __shared__ double sh[Fine<B>::NW];
     const int dn = st->done;
     int s[K], nb[K][6];
-    wg_load<B>(tab, s, nb);
+    wg_load<K>(tab, s, nb);
     if (dn) return;
     const float beta = st->beta;
     const int pc = st->precond;
@@ -270,15 +278,15 @@ __device__ __forceinline__ float mg_first(unsigned in, float r, float omega) {
 }
 
 // Two Jacobi sweeps from zero: zout = z1 + w (r - A z1)/D, z1 = w D^-1 r.
-template <int B>
+template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __restrict__ tab, MgLevelK L,
                                                                   float omega, float* __restrict__ zout,
                                                                   const int* done) {
-    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    constexpr int B3 = Fine<B>::B3;
     __shared__ float sz[K][B3];
     const int dn = *done;
     int s[K], nb[K][6];
-    wg_load<B>(tab, s, nb);
+    wg_load<K>(tab, s, nb);
     if (dn) return;
     const int l = threadIdx.x;
     int side[3], fl[3];
@@ -334,18 +342,18 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __
 // Ap, exactly as the neighbour's own workgroup computes it.  The old r is read from v.r and the new
 // one written to rout, a second buffer, because other workgroups still read the old values across
 // their faces; x is read and written only at the workgroup's own cells and is updated in place.
-template <int B>
+template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_update_smooth2(const int* __restrict__ tab, CgVecs v,
                                                                float* __restrict__ rout, float omega,
                                                                float* __restrict__ zout,
                                                                double* __restrict__ partials,
                                                                const CgState* __restrict__ st) {
-    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    constexpr int B3 = Fine<B>::B3;
     __shared__ float sz[K][B3];
     __shared__ double sh[Fine<B>::NW];
     const int dn = st->done;
     int s[K], nb[K][6];
-    wg_load<B>(tab, s, nb);
+    wg_load<K>(tab, s, nb);
     if (dn) return;
     const float alpha = st->alpha;
     const int l = threadIdx.x;
@@ -414,18 +422,18 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_update_smooth2(const int* __res
 // lie in the same block): rC = 0.5 * sum over fluid children of (r - A z), children summed in the
 // fixed order of mg_child.  With first != 0, z is the first sweep from zero (w D^-1 r) and is
 // stored to zf.
-template <int B>
+template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_restrict(const int* __restrict__ tab, MgLevelK L, MgLevelK C,
                                                              float omega, int first, float* __restrict__ zf,
                                                              const int* done) {
-    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    constexpr int B3 = Fine<B>::B3;
     constexpr int SH = Fine<B>::SH;
     constexpr int BC = B / 2, CSH = SH - 1, C3 = BC * BC * BC;
     __shared__ float sz[K][B3];
     __shared__ float sres[K][B3];
     const int dn = *done;
     int s[K], nb[K][6];
-    wg_load<B>(tab, s, nb);
+    wg_load<K>(tab, s, nb);
     if (dn) return;
     const int l = threadIdx.x;
     int side[3], fl[3];
@@ -514,19 +522,19 @@ __device__ __forceinline__ int parent_local(int ll) {
 }
 
 // Prolongation (piecewise constant from the next level) fused with one Jacobi sweep.
-template <int B>
+template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* __restrict__ tab, MgLevelK L,
                                                                    float omega, const float* __restrict__ z,
                                                                    const float* __restrict__ zc,
                                                                    float* __restrict__ zout, const int* done,
                                                                    double* __restrict__ rz_partials) {
-    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    constexpr int B3 = Fine<B>::B3;
     constexpr int BC = B / 2, C3 = BC * BC * BC;
     __shared__ float szp[K][B3];
     __shared__ double sh[Fine<B>::NW];
     const int dn = *done;
     int s[K], nb[K][6];
-    wg_load<B>(tab, s, nb);
+    wg_load<K>(tab, s, nb);
     if (dn) return;
     const int l = threadIdx.x;
     int side[3], fl[3];
@@ -588,16 +596,16 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* _
 }
 
 // One Jacobi sweep: zout = z + w (r - A z)/D.
-template <int B>
+template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restrict__ tab, MgLevelK L, float omega,
                                                            const float* __restrict__ z, float* __restrict__ zout,
                                                            const int* done, double* __restrict__ rz_partials) {
-    constexpr int B3 = Fine<B>::B3, K = Fine<B>::K;
+    constexpr int B3 = Fine<B>::B3;
     __shared__ float sz[K][B3];
     __shared__ double sh[Fine<B>::NW];
     const int dn = *done;
     int s[K], nb[K][6];
-    wg_load<B>(tab, s, nb);
+    wg_load<K>(tab, s, nb);
     if (dn) return;
     const int l = threadIdx.x;
     int side[3], fl[3];
