@@ -378,10 +378,17 @@ __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type,
     if (threadIdx.x == 0) { partials[blockIdx.x] = s; partials2[blockIdx.x] = s2; }
 }
 
-// Deterministic sum of n partials by one 256-thread workgroup.
+// Deterministic sum of n partials by one 256-thread workgroup.  Each thread keeps 8 running sums over
+// strided partials, so 8 loads are in flight per thread, and combines them in a fixed order.
 __device__ __forceinline__ double fin_sum(const double* __restrict__ partials, int n, double* sh) {
-    double s = 0.0;
-    for (int q = threadIdx.x; q < n; q += WG) s += partials[q];
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int q = threadIdx.x;
+    for (; q + 7 * WG < n; q += 8 * WG) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] += partials[q + j * WG];
+    }
+    for (int j = 0; q < n; q += WG, ++j) a[j] += partials[q];
+    const double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     return block_sum(s, sh);
 }
 
@@ -785,9 +792,7 @@ namespace vare {
 __global__ void k_fin_ir(CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol,
                          float inner_tol) {
     __shared__ double sh[4];
-    double s = 0.0;
-    for (int q = threadIdx.x; q < n; q += WG) s += partials[q];
-    const double tot = block_sum(s, sh);
+    const double tot = fin_sum(partials, n, sh);
     if (threadIdx.x != 0) return;
     st->ir_rr = tot;
     if (sqrt(tot) <= (double)outer_tol * sqrt(st->ir_bb) || tot == 0.0) {
@@ -829,9 +834,7 @@ __global__ void k_true_residual(int nc, FrameK f, const float* __restrict__ b, c
 
 __global__ void k_fin_true(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
     __shared__ double sh[4];
-    double s = 0.0;
-    for (int q = threadIdx.x; q < n; q += WG) s += partials[q];
-    const double tot = block_sum(s, sh);
+    const double tot = fin_sum(partials, n, sh);
     if (threadIdx.x == 0) st->true_rr = tot;
 }
 
