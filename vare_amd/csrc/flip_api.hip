@@ -420,13 +420,15 @@ template <int B> int bin_particles(vare_flip* h) {
     CHK(fill_u32(h, h->dint + 4, 0, 2));           // [4] sort overflow, [5] long-bin count
     CHK(fill_u32(h, h->dint + 7, 0, 1));           // [7] mid-bin count
     if (h->np)
-        hipLaunchKernelGGL(k_ckey<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (const float*)h->pa[0],
+        hipLaunchKernelGGL(k_ckey<B>, dim3(nblocks((h->np + kBinSlices - 1) / kBinSlices)), dim3(WG), 0, h->stream,
+                           (const float*)h->pa[0],
                            (const float*)h->pa[1], (const float*)h->pa[2], (int)h->np, h->frame(), nc, h->pkey, h->prank,
                            h->ccount);
     OpBuckets op{h->ccount, h->cstart};
     CHK(run_scan(h, op, nc + 2, 3));
     if (h->np) {
-        hipLaunchKernelGGL(k_pscatter, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, (const int*)h->pkey,
+        hipLaunchKernelGGL(k_pscatter, dim3(nblocks((h->np + kBinSlices - 1) / kBinSlices)), dim3(WG), 0, h->stream,
+                           (int)h->np, (const int*)h->pkey,
                            (const int*)h->prank, (const int*)h->cstart, (const int*)h->pid, h->perm);
         hipLaunchKernelGGL(k_sort_short, dim3(nblocks((size_t)nc + 1)), dim3(WG), 0, h->stream, (const int*)h->cstart,
                            nc + 1, h->perm, h->midbins, h->dint + 7, h->longbins, h->dint + 5);
@@ -871,18 +873,21 @@ template <int B> int stage_solve(vare_flip* h) {
     HIPCHK(hipEventRecord(h->step_ev[2], h->stream));
     const bool refine = mode == 1 && h->prm.refine_tol > 0.f;
     int total_iters = 0, outer = 0;
+    const int nv = h->nvirt();
+    const unsigned gv = rgrid((size_t)nv);
     if (refine) CHK(fill_u32(h, h->x64, 0, 2 * nc));
     CHK(run_solver<B>(h, cap, 0, total_iters));
     if (refine) {
         // FP64 iterative refinement: x64 += d, r = b - A x64 in FP64, solve A d = r in FP32
         for (;;) {
-            hipLaunchKernelGGL(k_ir_accumulate, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->x64, (const float*)h->x,
-                               h->x, 0);
+            // over the solver's blocks only: the other blocks hold no fluid cell, and x and x64 stay 0 there
+            hipLaunchKernelGGL(k_ir_accumulate, dim3(gv), dim3(WG), 0, h->stream, nv, h->clist(), h->x64,
+                               (const float*)h->x, h->x, 0);
             ++outer;
-            hipLaunchKernelGGL(k_ir_residual<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+            hipLaunchKernelGGL(k_ir_residual<B>, dim3(gv), dim3(WG), 0, h->stream, nv, h->clist(), h->frame(),
                                (const float*)h->b, (const double*)h->x64, (const uint16_t*)h->info, h->r, h->x, h->p,
                                h->partials);
-            hipLaunchKernelGGL(k_fin_ir, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g,
+            hipLaunchKernelGGL(k_fin_ir, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)gv,
                                h->prm.refine_tol, h->prm.cg_rel_tol);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
@@ -893,15 +898,14 @@ template <int B> int stage_solve(vare_flip* h) {
             CHK(run_solver<B>(h, cap, outer, total_iters));
         }
     }
-    // FP64 recompute of the final residual from the pressure the gradient will use (the FP64
-    // pressure when refining, else the FP32 one); reported with every solve
-    if (refine)
-        hipLaunchKernelGGL((k_true_residual<B, double>), dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
-                           (const float*)h->b, (const double*)h->x64, (const uint16_t*)h->info, h->partials);
-    else
+    // FP64 recompute of the final residual from the pressure the gradient will use, reported with
+    // every solve.  When refining, the last k_ir_residual computed it from the final FP64 pressure
+    // (k_fin_ir stored it); otherwise it is recomputed from the FP32 pressure.
+    if (!refine) {
         hipLaunchKernelGGL((k_true_residual<B, float>), dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
                            (const float*)h->b, (const float*)h->x, (const uint16_t*)h->info, h->partials);
-    hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+        hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+    }
     HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     const unsigned ga = nblocks(std::max<size_t>(nc, 1));

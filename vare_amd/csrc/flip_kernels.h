@@ -743,12 +743,13 @@ namespace vare {
 // outer step recomputes the residual r = b - A x64 in FP64, hands its FP32 rounding to the FP32
 // PCG as the right-hand side of a correction solve A d = r (d starts at 0), then x64 += d.
 template <int B>
-__global__ void k_ir_residual(int nc, FrameK f, const float* __restrict__ b, const double* __restrict__ x64,
+__global__ void k_ir_residual(int nv, CellList cl, FrameK f, const float* __restrict__ b, const double* __restrict__ x64,
                               const uint16_t* __restrict__ info, float* __restrict__ r32, float* __restrict__ x,
                               float* __restrict__ p, double* __restrict__ partials) {
     __shared__ double sh[4];
     double rr = 0.0;
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
+    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
+        const int c = cl.cell(vi);
         const uint16_t in = info[c];
         double r = 0.0;
         if (in & kInfoFluid) {
@@ -768,9 +769,10 @@ __global__ void k_ir_residual(int nc, FrameK f, const float* __restrict__ b, con
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-__global__ void k_ir_accumulate(int nc, double* __restrict__ x64, const float* __restrict__ d, float* __restrict__ x32,
-                                int last) {
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
+__global__ void k_ir_accumulate(int nv, CellList cl, double* __restrict__ x64, const float* __restrict__ d,
+                                float* __restrict__ x32, int last) {
+    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
+        const int c = cl.cell(vi);
         const double v = x64[c] + (double)d[c];
         x64[c] = v;
         if (last) x32[c] = (float)v;
@@ -795,6 +797,7 @@ __global__ void k_fin_ir(CgState* __restrict__ st, const double* __restrict__ pa
     const double tot = fin_sum(partials, n, sh);
     if (threadIdx.x != 0) return;
     st->ir_rr = tot;
+    st->true_rr = tot;            // |b - A x64|^2 of the current FP64 pressure, in FP64
     if (sqrt(tot) <= (double)outer_tol * sqrt(st->ir_bb) || tot == 0.0) {
         st->ir_done = 1;
         st->done = 1;

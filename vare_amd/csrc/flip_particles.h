@@ -22,36 +22,62 @@ constexpr unsigned long long kSortPad = ~0ull;
 // Bin of every particle (storage index of its base cell, or nc) + its arrival rank inside the bin.
 // Lanes holding equal keys in a row form a run; the first lane of every run reserves the whole run
 // with one atomic add, so a wave issues a single atomic instruction whatever the number of bins.
+// A workgroup covers kBinSlices consecutive slices of WG particles, one particle per thread per
+// slice; the slices' loads, slot lookups and atomics are independent, so their latencies overlap.
+constexpr int kBinSlices = 4;
+
 template <int B>
 __global__ void k_ckey(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, int np,
                        FrameK f, int nc, int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count) {
     constexpr int B3 = B * B * B;
-    const int i = blockIdx.x * WG + threadIdx.x;
+    constexpr int J = kBinSlices;
     const int lane = threadIdx.x & 63;
-    const bool valid = i < np;
-    int k = -1;
-    if (valid) {
-        const float x = px[i], y = py[i], z = pz[i];
-        k = nc;
-        if (!(x < 0.f || x > (float)f.nx || y < 0.f || y > (float)f.ny || z < 0.f || z > (float)f.nz)) {
-            const int ci = rmax((int)(x - 0.5f), 0), cj = rmax((int)(y - 0.5f), 0), ck = rmax((int)(z - 0.5f), 0);
-            const int s = f.slot[((ck / B) * f.fy + cj / B) * f.fx + ci / B];
-            if (s >= 0) k = s * B3 + ((ck % B) * B + cj % B) * B + ci % B;
-        }
-        key[i] = k;
+    int idx[J], k[J];
+    bool valid[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        idx[j] = (blockIdx.x * J + j) * WG + threadIdx.x;
+        valid[j] = idx[j] < np;
+        k[j] = -1;
     }
-    const int prev = __shfl(k, lane > 0 ? lane - 1 : 0, 64);
-    const bool head = valid && (lane == 0 || k != prev);
-    const unsigned long long heads = __ballot(head);
-    const int nvalid = (int)__popcll(__ballot(valid));       // valid lanes are a prefix of the wave
-    const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1);
-    const int mine = 63 - __clzll((long long)(upto | 1ull));   // first lane of my run
-    const unsigned long long after = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
-    const int next = after ? __ffsll((long long)after) - 1 : nvalid;
-    int base = 0;
-    if (head) base = atomicAdd(count + k, next - lane);
-    base = __shfl(base, mine, 64);
-    if (valid) rank[i] = base + lane - mine;
+    float x[J], y[J], z[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int i = valid[j] ? idx[j] : 0;
+        x[j] = px[i]; y[j] = py[i]; z[j] = pz[i];
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        if (!valid[j]) continue;
+        k[j] = nc;
+        if (!(x[j] < 0.f || x[j] > (float)f.nx || y[j] < 0.f || y[j] > (float)f.ny || z[j] < 0.f ||
+              z[j] > (float)f.nz)) {
+            const int ci = rmax((int)(x[j] - 0.5f), 0), cj = rmax((int)(y[j] - 0.5f), 0),
+                      ck = rmax((int)(z[j] - 0.5f), 0);
+            const int s = f.slot[((ck / B) * f.fy + cj / B) * f.fx + ci / B];
+            if (s >= 0) k[j] = s * B3 + ((ck % B) * B + cj % B) * B + ci % B;
+        }
+        key[idx[j]] = k[j];
+    }
+    int base[J], mine[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int prev = __shfl(k[j], lane > 0 ? lane - 1 : 0, 64);
+        const bool head = valid[j] && (lane == 0 || k[j] != prev);
+        const unsigned long long heads = __ballot(head);
+        const int nvalid = (int)__popcll(__ballot(valid[j]));   // valid lanes are a prefix of the wave
+        const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1);
+        mine[j] = 63 - __clzll((long long)(upto | 1ull));       // first lane of my run
+        const unsigned long long after = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
+        const int next = after ? __ffsll((long long)after) - 1 : nvalid;
+        base[j] = 0;
+        if (head) base[j] = atomicAdd(count + k[j], next - lane);
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int b = __shfl(base[j], mine[j], 64);
+        if (valid[j]) rank[idx[j]] = b + lane - mine[j];
+    }
 }
 
 // Exclusive scan of bin counts -> bin starts.
@@ -62,13 +88,24 @@ struct OpBuckets {
     __device__ void emit(int b, int pre, int v) const { start[b] = pre; }
 };
 
-// Sort key of physical particle q: its logical index above, q below (both < 2^31).
+// Sort key of physical particle q: its logical index above, q below (both < 2^31).  Same slices as
+// k_ckey.
 __global__ void k_pscatter(int np, const int* __restrict__ key, const int* __restrict__ rank,
                            const int* __restrict__ bstart, const int* __restrict__ pid,
                            unsigned long long* __restrict__ perm) {
-    const int i = blockIdx.x * WG + threadIdx.x;
-    if (i >= np) return;
-    perm[bstart[key[i]] + rank[i]] = ((unsigned long long)(unsigned)pid[i] << 32) | (unsigned)i;
+    constexpr int J = kBinSlices;
+    int idx[J], kk[J], rk[J], pd[J], st[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        idx[j] = (blockIdx.x * J + j) * WG + threadIdx.x;
+        const int i = idx[j] < np ? idx[j] : 0;
+        kk[j] = key[i]; rk[j] = rank[i]; pd[j] = pid[i];
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) st[j] = bstart[kk[j]];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+        if (idx[j] < np) perm[st[j] + rk[j]] = ((unsigned long long)(unsigned)pd[j] << 32) | (unsigned)idx[j];
 }
 
 // Sort every bin's keys ascending, i.e. by logical index, which turns the permutation into a stable
