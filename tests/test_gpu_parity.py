@@ -439,3 +439,32 @@ def test_fp16_velocity_storage():
     np.testing.assert_array_equal(v16, v16.astype(np.float16).astype(np.float32))
     assert rel(v16, out[0]["vel"]) < 2e-3
     assert rel(out[1]["pos"], out[0]["pos"]) < 1e-5
+
+
+def test_multigrid_deep_hierarchy_matches_jacobi():
+    # 160^3 with 8^3 blocks: the edge-2 and edge-1 levels are too large for the single-workgroup
+    # bottom V-cycle, which takes the two dense levels (10^3, 5^3) below them.  The multigrid
+    # preconditioner must give the same pressure as Jacobi-PCG on the same step, in few iterations.
+    n = 160
+    pos, vel = make_scene(n, wx=0.5, wy=0.5, seed=11)
+    g = L.grid_desc(n, n, n, (0, 0, 0), (1, 1, 1), 8)
+    base = dict(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n, solver_mode=1,
+                cg_rel_tol=1e-7, max_iterations=20000)
+    h0 = L.FlipHandle(g, L.default_params(preconditioner=L.PRECOND_MG, **base))
+    h0.set_particles(pos, vel)
+    for _ in range(3):                          # a few steps so the surface is no longer flat
+        h0.step(DT)
+    d = h0.get_particles()
+    out = {}
+    for name, pc in (("mg", L.PRECOND_MG), ("jacobi", L.PRECOND_JACOBI)):
+        h = L.FlipHandle(g, L.default_params(preconditioner=pc, **base))
+        h.set_particles(d["pos"], d["vel"])
+        h.step(DT)
+        st = h.stats()
+        out[name] = (h.cell_field(L.FIELD_PRESSURE), st)
+        print(name, "iterations", st["cg_iterations"], "converged", st["converged"], flush=True)
+        assert st["converged"] == 1
+    pm, sm = out["mg"]
+    pj, sj = out["jacobi"]
+    assert sm["cg_iterations"] < 40 < sj["cg_iterations"]
+    assert rel(pm, pj) < 1e-5
