@@ -116,6 +116,13 @@ __device__ __forceinline__ int mg_child(const MgLevelK& L, const MgLevelK& C, co
 __device__ __forceinline__ bool mg_fluid(unsigned info) { return (info & kInfoFluid) != 0; }
 __device__ __forceinline__ float mg_diag(unsigned info) { return (float)((info >> 8) & 0x1f); }
 
+// Jacobi weight omega / D of a cell, as omega times the hardware reciprocal of its integer diagonal.
+// The smoother belongs to the preconditioner, not to the discretised system, so it only has to be a
+// fixed symmetric operator: the reciprocal is deterministic and the same at every use of a cell, so
+// the V-cycle stays symmetric, and it costs one instruction where a correctly rounded division
+// costs about ten.
+__device__ __forceinline__ float mg_wd(unsigned info, float omega) { return omega * __builtin_amdgcn_rcpf(mg_diag(info)); }
+
 // Whether child a of coarse cell cc lies inside the fine level's grid (an inactive block inside the
 // frame is inside: it is air; a position past an odd-sized grid's last cell is outside the domain).
 __device__ __forceinline__ bool mg_child_inside(const MgLevelK& L, const MgLevelK& C, int cc, int a) {
@@ -170,7 +177,7 @@ __global__ void k_mg_info(MgLevelK C, MgFrame F) {
 // z at cell c for the "first sweep from zero" case: omega * r / diag.
 __device__ __forceinline__ float mg_z0(const MgLevelK& L, int c, float omega) {
     const uint16_t in = L.info[c];
-    return mg_fluid(in) ? omega * L.r[c] / mg_diag(in) : 0.f;
+    return mg_fluid(in) ? L.r[c] * mg_wd(in, omega) : 0.f;
 }
 
 // Two Jacobi sweeps from zero in one pass: zout = z1 + omega (r - A z1)/D with z1 = omega D^-1 r.
@@ -181,13 +188,13 @@ __global__ void k_mg_smooth_first2(MgLevelK L, MgFrame F, float omega, float* __
     const uint16_t in = L.info[c];
     float out = 0.f;
     if (mg_fluid(in)) {
-        const float d = mg_diag(in);
-        const float z1 = omega * L.r[c] / d;
+        const float d = mg_diag(in), wd = mg_wd(in, omega);
+        const float z1 = L.r[c] * wd;
         float az = d * z1;
 #pragma unroll
         for (int q = 0; q < 6; ++q)
             if (in & (1u << q)) az -= mg_z0(L, mg_nbr(L, F, c, q), omega);
-        out = z1 + omega * (L.r[c] - az) / d;
+        out = z1 + (L.r[c] - az) * wd;
     }
     zout[c] = out;
 }
@@ -207,7 +214,7 @@ __global__ void k_mg_smooth(MgLevelK L, MgFrame F, float omega, const float* __r
 #pragma unroll
         for (int q = 0; q < 6; ++q)
             if (in & (1u << q)) az -= z[mg_nbr(L, F, c, q)];
-        out = zc + omega * (L.r[c] - az) / d;
+        out = zc + (L.r[c] - az) * mg_wd(in, omega);
     }
     zout[c] = out;
 }
@@ -230,7 +237,7 @@ __global__ void k_mg_restrict(MgLevelK L, MgLevelK C, MgFrame F, float omega, in
         const uint16_t in = ch >= 0 ? L.info[ch] : (uint16_t)0;
         if (mg_fluid(in)) {
             const float d = mg_diag(in);
-            const float zc = first ? omega * L.r[ch] / d : zf[ch];
+            const float zc = first ? L.r[ch] * mg_wd(in, omega) : zf[ch];
             if (first) zf[ch] = zc;
             float az = d * zc;
 #pragma unroll
@@ -267,7 +274,7 @@ __global__ void k_mg_prolong_smooth(MgLevelK L, MgLevelK C, MgFrame F, float ome
             const int n = mg_nbr(L, F, c, q);
             az -= z[n] + zc[mg_parent(L, C, F, n)];
         }
-        out = zp + omega * (L.r[c] - az) / d;
+        out = zp + (L.r[c] - az) * mg_wd(in, omega);
     }
     zout[c] = out;
 }
@@ -350,12 +357,12 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
         float* zt = st + o;
         const uint16_t* in = si + o;
         // first sweep from zero: omega r / D (k_mg_smooth_first2's z1, or the whole pre-smoothing for nu = 1)
-        for (int c = tid; c < n; c += T) (nu == 1 ? z : zt)[c] = mg_fluid(in[c]) ? omega * r[c] / mg_diag(in[c]) : 0.f;
+        for (int c = tid; c < n; c += T) (nu == 1 ? z : zt)[c] = mg_fluid(in[c]) ? r[c] * mg_wd(in[c], omega) : 0.f;
         __syncthreads();
         if (nu == 2) {
             for (int c = tid; c < n; c += T) {
                 float out = 0.f;
-                if (mg_fluid(in[c])) out = zt[c] + omega * (r[c] - bottom_az(in[c], zt, c, sy, szz)) / mg_diag(in[c]);
+                if (mg_fluid(in[c])) out = zt[c] + (r[c] - bottom_az(in[c], zt, c, sy, szz)) * mg_wd(in[c], omega);
                 z[c] = out;
             }
             __syncthreads();
@@ -395,7 +402,7 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
             for (int c = tid; c < n; c += T) {
                 const unsigned in = si[o + c];
                 float out = 0.f;
-                if (mg_fluid(in)) out = cur[c] + omega * (sr[o + c] - bottom_az(in, cur, c, sy, szz)) / mg_diag(in);
+                if (mg_fluid(in)) out = cur[c] + (sr[o + c] - bottom_az(in, cur, c, sy, szz)) * mg_wd(in, omega);
                 nxt[c] = out;
             }
             __syncthreads();
@@ -423,14 +430,14 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
         // one sweep from zp; nu = 2 adds a second (k_mg_smooth)
         for (int c = tid; c < n; c += T) {
             float out = 0.f;
-            if (mg_fluid(in[c])) out = zt[c] + omega * (r[c] - bottom_az(in[c], zt, c, sy, szz)) / mg_diag(in[c]);
+            if (mg_fluid(in[c])) out = zt[c] + (r[c] - bottom_az(in[c], zt, c, sy, szz)) * mg_wd(in[c], omega);
             z[c] = out;
         }
         __syncthreads();
         if (nu == 2) {
             for (int c = tid; c < n; c += T) {
                 float out = 0.f;
-                if (mg_fluid(in[c])) out = z[c] + omega * (r[c] - bottom_az(in[c], z, c, sy, szz)) / mg_diag(in[c]);
+                if (mg_fluid(in[c])) out = z[c] + (r[c] - bottom_az(in[c], z, c, sy, szz)) * mg_wd(in[c], omega);
                 zt[c] = out;
             }
             __syncthreads();

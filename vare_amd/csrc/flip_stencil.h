@@ -272,9 +272,9 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_pupdate_spmv(const int* __restr
 }
 
 // ---- multigrid, level 0 (the CG residual r, stencil descriptor info) and level 1 for B = 8 ----
-// First Jacobi sweep from zero at a cell: omega * r / D on fluid cells, 0 elsewhere.
+// First Jacobi sweep from zero at a cell: (omega / D) r on fluid cells, 0 elsewhere.
 __device__ __forceinline__ float mg_first(unsigned in, float r, float omega) {
-    return mg_fluid(in) ? omega * r / mg_diag(in) : 0.f;
+    return mg_fluid(in) ? r * mg_wd(in, omega) : 0.f;
 }
 
 // Two Jacobi sweeps from zero: zout = z1 + w (r - A z1)/D, z1 = w D^-1 r.
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __
 #pragma unroll
             for (int a = 0; a < 3; ++a) hz[a] = mg_first(hi[kb][a], hr[kb][a], omega);
             const float az = apply_row<B>(in[kb], z1[kb], sz[kb], l, side, hz);
-            out = z1[kb] + omega * (rc[kb] - az) / mg_diag(in[kb]);
+            out = z1[kb] + (rc[kb] - az) * mg_wd(in[kb], omega);
         }
         zout[s[kb] * B3 + l] = out;
     }
@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_update_smooth2(const int* __res
 #pragma unroll
             for (int a = 0; a < 3; ++a) hz[a] = mg_first(hi[kb][a], hr[kb][a] - (alpha * ha[kb][a]), omega);
             const float az = apply_row<B>(in[kb], z1[kb], sz[kb], l, side, hz);
-            out = z1[kb] + omega * (rn[kb] - az) / mg_diag(in[kb]);
+            out = z1[kb] + (rn[kb] - az) * mg_wd(in[kb], omega);
         }
         const int c = s[kb] * B3 + l;
         v.x[c] = xo[kb] + (alpha * po[kb]);
@@ -584,7 +584,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_prolong_smooth(const int* _
 #pragma unroll
             for (int a = 0; a < 3; ++a) h[a] = hz[kb][a] + hq[kb][a];
             const float az = apply_row<B>(in[kb], zp[kb], szp[kb], l, side, h);
-            out = zp[kb] + omega * (rc[kb] - az) / mg_diag(in[kb]);
+            out = zp[kb] + (rc[kb] - az) * mg_wd(in[kb], omega);
         }
         zout[s[kb] * B3 + l] = out;
         rz += (double)rc[kb] * (double)out;
@@ -641,7 +641,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth(const int* __restric
         float out = 0.f;
         if (mg_fluid(in[kb])) {
             const float az = apply_row<B>(in[kb], zc[kb], sz[kb], l, side, hz[kb]);
-            out = zc[kb] + omega * (rc[kb] - az) / mg_diag(in[kb]);
+            out = zc[kb] + (rc[kb] - az) * mg_wd(in[kb], omega);
         }
         zout[s[kb] * B3 + l] = out;
         rz += (double)rc[kb] * (double)out;
