@@ -810,8 +810,16 @@ template <int B> void enqueue_iteration(vare_flip* h, int k) {
         v.p = pbuf[(k + 1) & 1];
         hipEvent_t e0, e1;
         tpair(h, VARE_KERNEL_SPMV, (int64_t)nv, e0, e1);
-        hipExtLaunchKernelGGL((k_pupdate_spmv<B, kIterK>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, e0, e1, 0, tab,
-                              v, pk, h->partials, (const CgState*)h->st);
+        const int pc = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? h->prm.preconditioner : VARE_PRECOND_NONE;
+        if (pc == VARE_PRECOND_MG)
+            hipExtLaunchKernelGGL((k_pupdate_spmv<B, kIterK, 2>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, e0, e1,
+                                  0, tab, v, pk, h->partials, (const CgState*)h->st);
+        else if (pc == VARE_PRECOND_JACOBI)
+            hipExtLaunchKernelGGL((k_pupdate_spmv<B, kIterK, 1>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, e0, e1,
+                                  0, tab, v, pk, h->partials, (const CgState*)h->st);
+        else
+            hipExtLaunchKernelGGL((k_pupdate_spmv<B, kIterK, 0>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, e0, e1,
+                                  0, tab, v, pk, h->partials, (const CgState*)h->st);
     }
     hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
                        k == 0 ? nwg_sp : nwg_it);

@@ -187,10 +187,11 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict_
 }
 
 // Search direction of one cell (BLAS::addmul, LinearSystemSolver.cpp:41): p = z + beta p, with z the
-// preconditioned residual (r for no preconditioner, r / diag for Jacobi, the V-cycle output for
+// preconditioned residual (PC 0: r, no preconditioner; 1: r / diag, Jacobi; 2: the V-cycle output,
 // multigrid).
-__device__ __forceinline__ float pdir(int pc, float beta, unsigned in, float r, float z, float p) {
-    const float zz = pc == 1 ? r * jacobi_inv((uint16_t)in) : (pc == 2 ? z : r);
+template <int PC>
+__device__ __forceinline__ float pdir(float beta, unsigned in, float r, float z, float p) {
+    const float zz = PC == 1 ? r * jacobi_inv((uint16_t)in) : (PC == 2 ? z : r);
     return zz + (beta * p);
 }
 
@@ -199,7 +200,9 @@ __device__ __forceinline__ float pdir(int pc, float beta, unsigned in, float r, 
 // z (or r) and old p, exactly as the neighbour's own workgroup computes it, so no pass over p is
 // needed between the two.  The old p is read from v.p and the new one written to pout, a second
 // buffer: other workgroups still read the old values across their faces.
-template <int B, int K>
+// PC is the preconditioner (the solve's st->precond), a template parameter so that each instance
+// carries only its own form of z.
+template <int B, int K, int PC>
 __global__ void __launch_bounds__(Fine<B>::B3) k_pupdate_spmv(const int* __restrict__ tab, CgVecs v,
                                                              float* __restrict__ pout,
                                                              double* __restrict__ partials,
@@ -212,8 +215,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_pupdate_spmv(const int* __restr
     wg_load<K>(tab, s, nb);
     if (dn) return;
     const float beta = st->beta;
-    const int pc = st->precond;
-    const float* __restrict__ zs = pc == 2 ? v.z : v.r;
+    const float* __restrict__ zs = PC == 2 ? v.z : v.r;
     const int l = threadIdx.x;
     int side[3], fl[3];
     thread_faces<B>(l, side, fl);
@@ -242,13 +244,13 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_pupdate_spmv(const int* __restr
             if (ns >= 0) {
                 hp[kb][a] = v.p[ns * B3 + fl[a]];
                 hz[kb][a] = zs[ns * B3 + fl[a]];
-                if (pc == 1) hi[kb][a] = v.info[ns * B3 + fl[a]];
+                if (PC == 1) hi[kb][a] = v.info[ns * B3 + fl[a]];
             }
         }
     float pn[K];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
-        pn[kb] = pdir(pc, beta, in[kb], zo[kb], zo[kb], po[kb]);
+        pn[kb] = pdir<PC>(beta, in[kb], zo[kb], zo[kb], po[kb]);
         sp[kb][l] = pn[kb];
     }
     __syncthreads();
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_pupdate_spmv(const int* __restr
         if (in[kb] & kInfoFluid) {
             float h[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) h[a] = pdir(pc, beta, hi[kb][a], hz[kb][a], hz[kb][a], hp[kb][a]);
+            for (int a = 0; a < 3; ++a) h[a] = pdir<PC>(beta, hi[kb][a], hz[kb][a], hz[kb][a], hp[kb][a]);
             acc = apply_row<B>(in[kb], pn[kb], sp[kb], l, side, h);
             pap += (double)pn[kb] * (double)acc;
         }
