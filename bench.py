@@ -41,6 +41,7 @@ HBM_PEAK_GBS = 8000.0
 # on this benchmark (20 steps) and calibrated against plain 1 GiB copies (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0); profiles/hbm_traffic_dam_break_256.md.
 PSPMV_TRAFFIC_PER_CELL = 17.11    # k_pupdate_spmv: (2 x 22310.3 + 32056.6) KiB per 4.590 M cells
+UPD_TRAFFIC_PER_CELL = 29.12     # k_update_smooth2: (2 x 40949.5 + 48564.0) KiB per 4.587 M cells
 MG0_TRAFFIC_PER_CELL = 12.51      # k_mg0_prolong_smooth: (2 x 20662.6 + 14684.8) KiB per 4.585 M cells
 DT = 1.0 / 240.0
 
@@ -159,7 +160,7 @@ def run_scene(a, L, cfg, steps):
     h.set_particles(pos, vel)
     del pos, vel
     if not a.no_kernel_timing:
-        h.set_kernel_timing(["spmv", "mg0"])
+        h.set_kernel_timing(["spmv", "mg0", "update"])
         h.kernel_timing(reset=True)
     L.device_synchronize()
 
@@ -190,7 +191,7 @@ def run_scene(a, L, cfg, steps):
     kt = h.kernel_timing(reset=True) if not a.no_kernel_timing else None
 
     roofline = None
-    if kt is not None and kt["mg0"][1] > 0 and kt["spmv"][1] > 0:
+    if kt is not None and kt["mg0"][1] > 0 and kt["spmv"][1] > 0 and kt["update"][1] > 0:
         def line(kernel, cls, bytes_per_cell, traffic_per_cell):
             ms, launches, cells = kt[cls]
             bytes_total = bytes_per_cell * cells
@@ -200,12 +201,15 @@ def run_scene(a, L, cfg, steps):
                     "traffic": round(traffic_per_cell * cells / launches, 1) if traffic_per_cell else None,
                     "avg_launch_us": round(ms * 1e3 / launches, 2), "launches": launches,
                     "bytes_per_launch": round(bytes_total / launches, 1)}
-        # dominant kernel: the level-0 prolongation + Jacobi sweep of the V-cycle.  Algorithmic bytes
-        # per cell it processes: read z 4 + coarse z 0.5 (one per 8 cells) + r 4 + descriptor 2, write
-        # the smoothed z 4.  Neighbour values are re-reads.
-        roofline = line("k_mg0_prolong_smooth", "mg0", 14.5, MG0_TRAFFIC_PER_CELL)
-        # the PCG's direction update fused with A*p: read old p 4 + z 4 + descriptor 2, write p 4 + Ap 4
-        roofline["others"] = [line("k_pupdate_spmv", "spmv", 18.0, PSPMV_TRAFFIC_PER_CELL)]
+        # dominant kernel (the most time per step): the CG's x, r update fused with the V-cycle's first
+        # level-0 pass.  Algorithmic bytes per solver cell: read x, p, r, Ap 16 + descriptor 2, write
+        # x, r, z 12.  Neighbour values across a block face are re-reads.
+        roofline = line("k_update_smooth2", "update", 30.0, UPD_TRAFFIC_PER_CELL)
+        roofline["others"] = [
+            # the level-0 prolongation + Jacobi pass: read z 4 + coarse z 0.5 + r 4 + descriptor 2, write z 4
+            line("k_mg0_prolong_smooth", "mg0", 14.5, MG0_TRAFFIC_PER_CELL),
+            # the direction update fused with A*p: read old p 4 + z 4 + descriptor 2, write p 4 + Ap 4
+            line("k_pupdate_spmv", "spmv", 18.0, PSPMV_TRAFFIC_PER_CELL)]
     sm = np.array(step_ms)
     nx, ny, nz = sc["dims"]
     res = {

@@ -266,7 +266,8 @@ int vare_device_name(char* buf, size_t cap);
 #define VARE_KERNEL_COUNT    9
 int vare_flip_set_kernel_timing(vare_flip* h, int mask);
 /* Totals per class since the last reset: milliseconds, launches and work items (cells or particles
- * the timed launches processed; filled for VARE_KERNEL_SPMV and VARE_KERNEL_MG0, whose events are
+ * the timed launches processed; filled for VARE_KERNEL_SPMV, VARE_KERNEL_MG0 and, with multigrid and two
+ * sweeps, VARE_KERNEL_UPDATE, whose events are
  * recorded on the kernel's own dispatch and so time its execution alone).  Any array may be NULL. */
 int vare_flip_get_kernel_timing(vare_flip* h, double ms_total[VARE_KERNEL_COUNT], int64_t launches[VARE_KERNEL_COUNT],
                                 int64_t units[VARE_KERNEL_COUNT], int reset);

@@ -825,10 +825,10 @@ template <int B> void enqueue_iteration(vare_flip* h, int k) {
                        k == 0 ? nwg_sp : nwg_it);
     v.p = pk;
     if (fuse) {
-        tbegin(h, VARE_KERNEL_UPDATE);
-        hipLaunchKernelGGL((k_update_smooth2<B, kIterK>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, tab, v, rout,
-                           h->prm.mg_omega, h->mg_z[0], h->partials, (const CgState*)h->st);
-        tend(h, VARE_KERNEL_UPDATE);
+        hipEvent_t u0, u1;
+        tpair(h, VARE_KERNEL_UPDATE, (int64_t)nv, u0, u1);
+        hipExtLaunchKernelGGL((k_update_smooth2<B, kIterK>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, u0, u1, 0,
+                              tab, v, rout, h->prm.mg_omega, h->mg_z[0], h->partials, (const CgState*)h->st);
         hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
                            (const double*)h->partials2, nwg_it);
         // the rest of the V-cycle on the new residual; its last level-0 pass reduces r.z as well
