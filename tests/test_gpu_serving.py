@@ -98,3 +98,27 @@ def test_pooled_handle_reuse_same_result():
     p1, v1, s1 = srv.result(b1)
     assert s0["status"] == 0 and s1["status"] == 0
     assert np.array_equal(p1, p0) and np.array_equal(v1, v0)
+
+
+def test_pool_eviction_under_budget():
+    # one slot; tanks of grids A, B, A in turn.  With room for only one device state the pooled A is
+    # destroyed to admit B and B to admit the second A (three states created, none reused, the
+    # reserved bytes never above the budget); with room for both, the second A reuses the first's.
+    na, nb = 40, 48
+    ga, gb = L.grid_desc(na, na, na, block_size=8), L.grid_desc(nb, nb, nb, block_size=8)
+    pa, va = tank(na, 0.5, 0.5, 4, 31)
+    pb, vb = tank(nb, 0.5, 0.5, 4, 32)
+    bya = L.Server.request_bytes(ga, params(na), pa.shape[0])
+    byb = L.Server.request_bytes(gb, params(nb), pb.shape[0])
+    for budget, created, reused in ((max(bya, byb) + 1024, 3, 0), (bya + byb + 1024, 2, 1)):
+        srv = L.Server(budget, 1)
+        ids = [srv.submit(ga, params(na), pa, va, steps=2, dt=DT, arrival_s=0.0),
+               srv.submit(gb, params(nb), pb, vb, steps=2, dt=DT, arrival_s=0.001),
+               srv.submit(ga, params(na), pa, va, steps=2, dt=DT, arrival_s=0.002)]
+        srv.run()
+        st = srv.stats()
+        assert (st["handles_created"], st["handles_reused"]) == (created, reused), st
+        assert st["requests"] == 3 and st["peak_bytes"] <= budget
+        r0, r2 = srv.result(ids[0]), srv.result(ids[2])
+        assert r0[2]["status"] == 0 and r2[2]["status"] == 0
+        assert np.array_equal(r0[0], r2[0]) and np.array_equal(r0[1], r2[1])   # same tank, same result
