@@ -685,6 +685,47 @@ template <int E> void enqueue_block_up(vare_flip* h, int l, int nu, double* rz) 
     }
 }
 
+// Down and up passes of level l (l >= the block-kernel levels, above the bottom workgroup) with the
+// generic kernels instantiated for the level's kind: two Jacobi sweeps from zero and the restriction
+// (nu = 1: the restriction with the one sweep computed inside), then the prolongation fused with the
+// first post-sweep and, for nu = 2, the second post-sweep.
+template <int KIND> void enqueue_level_down(vare_flip* h, int l, int nu) {
+    const MgFrame F = mg_frame(h);
+    const float w = h->prm.mg_omega;
+    const int* done = done_ptr(h);
+    const MgLevelK& L = h->mg[l];
+    const MgLevelK& C = h->mg[l + 1];
+    const dim3 gr(nblocks((size_t)C.ncell * 8));
+    if (nu == 1) {
+        if (C.ncell)
+            hipLaunchKernelGGL((k_mg_restrict<KIND, true>), gr, dim3(WG), 0, h->stream, L, C, F, w, h->mg_zt[l], done);
+        return;
+    }
+    if (L.ncell)
+        hipLaunchKernelGGL((k_mg_smooth_first2<KIND>), dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F, w,
+                           h->mg_z[l], done);
+    if (C.ncell)
+        hipLaunchKernelGGL((k_mg_restrict<KIND, false>), gr, dim3(WG), 0, h->stream, L, C, F, w, h->mg_z[l], done);
+}
+
+template <int KIND> void enqueue_level_up(vare_flip* h, int l, int nu) {
+    const MgFrame F = mg_frame(h);
+    const float w = h->prm.mg_omega;
+    const int* done = done_ptr(h);
+    const MgLevelK& L = h->mg[l];
+    const MgLevelK& C = h->mg[l + 1];
+    const dim3 gr(nblocks(L.ncell));
+    if (nu == 1) {
+        hipLaunchKernelGGL((k_mg_prolong_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, C, F, w,
+                           (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done);
+        return;
+    }
+    hipLaunchKernelGGL((k_mg_prolong_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, C, F, w, (const float*)h->mg_z[l],
+                       (const float*)h->mg_z[l + 1], h->mg_zt[l], done);
+    hipLaunchKernelGGL((k_mg_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, F, w, (const float*)h->mg_zt[l], h->mg_z[l],
+                       done);
+}
+
 // z = V-cycle(r) into the CG's z vector; with rz != nullptr the last level-0 pass also writes the
 // per-workgroup partials of r.z (one per level-0 workgroup).
 // r0 != nullptr: the level-0 right-hand side (the CG residual) is read from r0 instead of h->r;
@@ -712,23 +753,15 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
     }
     for (int l = 0; l < lb; ++l) {
         const MgLevelK& L = h->mg[l];
-        const MgLevelK& C = h->mg[l + 1];
         if (l < nblk_levels) {
             if (l == 0) enqueue_block_down<B>(h, 0, nu, skip_first);
             else enqueue_block_down<4>(h, 1, nu);
             continue;
         }
-        if (nu == 1) {
-            if (C.ncell)
-                hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks((size_t)C.ncell * 8)), dim3(WG), 0, h->stream, L, C, F,
-                                   w, 1, h->mg_zt[l], done);
-        } else {
-            if (L.ncell)
-                hipLaunchKernelGGL(k_mg_smooth_first2, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F, w,
-                                   h->mg_z[l], done);
-            if (C.ncell)
-                hipLaunchKernelGGL(k_mg_restrict, dim3(nblocks((size_t)C.ncell * 8)), dim3(WG), 0, h->stream, L, C, F,
-                                   w, 0, h->mg_z[l], done);
+        switch (mg_kind(L)) {
+            case kMgBlk: enqueue_level_down<kMgBlk>(h, l, nu); break;
+            case kMgEdge1: enqueue_level_down<kMgEdge1>(h, l, nu); break;
+            default: enqueue_level_down<kMgDense>(h, l, nu); break;
         }
     }
     MgBottom bt{};
@@ -752,21 +785,16 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
                        done);
     for (int l = lb - 1; l >= 0; --l) {
         const MgLevelK& L = h->mg[l];
-        const MgLevelK& C = h->mg[l + 1];
         if (!L.ncell) continue;
         if (l < nblk_levels) {
             if (l == 0) enqueue_block_up<B>(h, 0, nu, rz);
             else enqueue_block_up<4>(h, 1, nu, (double*)nullptr);
             continue;
         }
-        if (nu == 1) {
-            hipLaunchKernelGGL(k_mg_prolong_smooth, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, C, F, w,
-                               (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done);
-        } else {
-            hipLaunchKernelGGL(k_mg_prolong_smooth, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, C, F, w,
-                               (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done);
-            hipLaunchKernelGGL(k_mg_smooth, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F, w,
-                               (const float*)h->mg_zt[l], h->mg_z[l], done);
+        switch (mg_kind(L)) {
+            case kMgBlk: enqueue_level_up<kMgBlk>(h, l, nu); break;
+            case kMgEdge1: enqueue_level_up<kMgEdge1>(h, l, nu); break;
+            default: enqueue_level_up<kMgDense>(h, l, nu); break;
         }
     }
     h->mg[0].r = r0_saved;

@@ -173,109 +173,331 @@ __global__ void k_mg_info(MgLevelK C, MgFrame F) {
     C.info[c] = (uint16_t)(kInfoFluid | mask | (diag << 8));
 }
 
-// ---- V-cycle kernels; each returns at once when the solver is done ----
+// ---- V-cycle kernels of the levels below the block-aligned ones ----
+// These levels hold few cells, so a launch costs the latency of its chain of dependent loads rather than
+// bandwidth.  Each kernel issues every load whose address it knows at once (the done flag, the cell's
+// descriptor and values, its neighbour-table row), then every load that needs a neighbour index, and
+// tests the done flag only before its store: two round trips to memory where a load per step of the
+// chain (flag, descriptor, table, neighbour) took four or five.  The neighbour loads are unconditional,
+// from the cell itself where the descriptor has no fluid neighbour, so they sit in one basic block and
+// issue together; a skipped term subtracts +0, which leaves the sum bit-identical.  mg_pin makes a value
+// live before the done branch, so the compiler cannot sink the loads behind it.  When the solver is done
+// the kernels store nothing.  The level's kind is a template argument, so each instance is straight-line
+// code with all of its loads in one basic block.
+__device__ __forceinline__ void mg_pin(float v) { asm volatile("" ::"v"(v)); }
+
+// Kinds of level: sparse with edge 2 or more (a cell's children and parent lie in its block), sparse
+// with edge 1 (cell index = block slot; the next level is dense), dense grid.
+enum MgKind : int { kMgBlk = 0, kMgEdge1 = 1, kMgDense = 2 };
+inline int mg_kind(const MgLevelK& L) { return L.dense ? kMgDense : (L.sh == 0 ? kMgEdge1 : kMgBlk); }
+
 // z at cell c for the "first sweep from zero" case: omega * r / diag.
-__device__ __forceinline__ float mg_z0(const MgLevelK& L, int c, float omega) {
-    const uint16_t in = L.info[c];
-    return mg_fluid(in) ? L.r[c] * mg_wd(in, omega) : 0.f;
+__device__ __forceinline__ float mg_z0(unsigned in, float r, float omega) {
+    return mg_fluid(in) ? r * mg_wd(in, omega) : 0.f;
 }
 
+// The neighbour-table row of cell c (mg_nbr's table entries for its block; on a sparse level of edge 1
+// the cell index is the block slot).  A dense level has no table.
+template <int KIND>
+__device__ __forceinline__ void mg_row(const MgLevelK& L, const MgFrame& F, int c, int t[6]) {
+    const int sh = KIND == kMgEdge1 ? 0 : L.sh;
+    const int s = c >> (3 * sh);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) t[q] = KIND == kMgDense ? 0 : F.nbr[s * 6 + q];
+}
+
+// Makes the done flag, a descriptor and six indices complete at this point, so their loads are in
+// flight together before it and nothing formed from them is reordered ahead of it.
+// The values are read-write operands of the (empty) asm statement, so whatever is computed from them
+// comes after it.
+__device__ __forceinline__ void mg_pin7(int& dn, unsigned& a, int t[6]) {
+    asm volatile("" : "+s"(dn), "+v"(a), "+v"(t[0]), "+v"(t[1]), "+v"(t[2]), "+v"(t[3]), "+v"(t[4]), "+v"(t[5]));
+}
+// The same for six loaded values (a neighbour round): issued together, complete here, so the arithmetic
+// that consumes them is not interleaved with waits for individual loads.
+template <class T> __device__ __forceinline__ void mg_pin6(T v[6]) {
+    asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]));
+}
+template <class T> __device__ __forceinline__ void mg_pin1(T& v) { asm volatile("" : "+v"(v)); }
+
+// mg_nbr for all six directions from the cell's table row t.
+template <int KIND>
+__device__ __forceinline__ void mg_nbrs(const MgLevelK& L, int c, const int t[6], int nb[6]) {
+    if (KIND == kMgDense) {
+        const int i = c % L.dx, j = (c / L.dx) % L.dy, k = c / (L.dx * L.dy);
+        const int sy = L.dx, sz = L.dx * L.dy;
+        nb[0] = i > 0 ? c - 1 : -2;
+        nb[1] = i + 1 < L.dx ? c + 1 : -2;
+        nb[2] = j > 0 ? c - sy : -2;
+        nb[3] = j + 1 < L.dy ? c + sy : -2;
+        nb[4] = k > 0 ? c - sz : -2;
+        nb[5] = k + 1 < L.dz ? c + sz : -2;
+        return;
+    }
+    if (KIND == kMgEdge1) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nb[q] = t[q];
+        return;
+    }
+    const int sh = L.sh, E = 1 << sh;
+    const int l = c & ((1 << (3 * sh)) - 1);
+    const int lc[3] = {l & (E - 1), (l >> sh) & (E - 1), l >> (2 * sh)};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int axis = q >> 1, dir = (q & 1) ? 1 : -1;
+        const int stride = 1 << (axis * sh);
+        const int nl = lc[axis] + dir;
+        nb[q] = (nl >= 0 && nl < E) ? c + dir * stride : (t[q] < 0 ? t[q] : (t[q] << (3 * sh)) + l - dir * (E - 1) * stride);
+    }
+}
+
+// The index to load for a neighbour: the neighbour when it exists, else the cell itself (always valid).
+// The value is used only where the descriptor marks the neighbour fluid; choosing the index without the
+// descriptor keeps these loads off the descriptor's round trip.
+__device__ __forceinline__ int mg_ldx(int n, int c) { return n >= 0 ? n : c; }
+
 // Two Jacobi sweeps from zero in one pass: zout = z1 + omega (r - A z1)/D with z1 = omega D^-1 r.
+template <int KIND>
 __global__ void k_mg_smooth_first2(MgLevelK L, MgFrame F, float omega, float* __restrict__ zout, const int* done) {
-    if (*done) return;
+    int dn = *done;
     const int c = blockIdx.x * WG + threadIdx.x;
     if (c >= L.ncell) return;
-    const uint16_t in = L.info[c];
-    float out = 0.f;
-    if (mg_fluid(in)) {
-        const float d = mg_diag(in), wd = mg_wd(in, omega);
-        const float z1 = L.r[c] * wd;
-        float az = d * z1;
+    unsigned in = L.info[c];
+    float rc = L.r[c];
+    int t[6], nb[6];
+    mg_row<KIND>(L, F, c, t);
+    mg_pin7(dn, in, t);
+    mg_pin1(rc);
+    mg_nbrs<KIND>(L, c, t, nb);
+    unsigned ni[6];
+    float nr[6], zn[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
-            if (in & (1u << q)) az -= mg_z0(L, mg_nbr(L, F, c, q), omega);
-        out = z1 + (L.r[c] - az) * wd;
+    for (int q = 0; q < 6; ++q) {
+        const int n = mg_ldx(nb[q], c);
+        ni[q] = L.info[n];
+        nr[q] = L.r[n];
     }
+    mg_pin6(ni);
+    mg_pin6(nr);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) zn[q] = mg_z0(ni[q], nr[q], omega);
+    const float d = mg_diag(in), wd = mg_wd(in, omega);
+    const float z1 = rc * wd;
+    float az = d * z1;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
+    const float out = mg_fluid(in) ? z1 + (rc - az) * wd : 0.f;
+    mg_pin(out);
+    if (dn) return;
     zout[c] = out;
 }
 
 // One Jacobi sweep: zout = z + omega (r - A z)/D.
+template <int KIND>
 __global__ void k_mg_smooth(MgLevelK L, MgFrame F, float omega, const float* __restrict__ z, float* __restrict__ zout,
                             const int* done) {
-    if (*done) return;
+    int dn = *done;
     const int c = blockIdx.x * WG + threadIdx.x;
     if (c >= L.ncell) return;
-    const uint16_t in = L.info[c];
-    float out = 0.f;
-    if (mg_fluid(in)) {
-        const float d = mg_diag(in);
-        const float zc = z[c];
-        float az = d * zc;
+    unsigned in = L.info[c];
+    float rc = L.r[c], zc = z[c];
+    int t[6], nb[6];
+    mg_row<KIND>(L, F, c, t);
+    mg_pin7(dn, in, t);
+    mg_pin1(rc);
+    mg_pin1(zc);
+    mg_nbrs<KIND>(L, c, t, nb);
+    float zn[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
-            if (in & (1u << q)) az -= z[mg_nbr(L, F, c, q)];
-        out = zc + (L.r[c] - az) * mg_wd(in, omega);
-    }
+    for (int q = 0; q < 6; ++q) zn[q] = z[mg_ldx(nb[q], c)];
+    mg_pin6(zn);
+    float az = mg_diag(in) * zc;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
+    const float out = mg_fluid(in) ? zc + (rc - az) * mg_wd(in, omega) : 0.f;
+    mg_pin(out);
+    if (dn) return;
     zout[c] = out;
 }
 
-// Restriction of the residual: rC[cc] = 0.5 * sum over fluid children of (r - A z).  With
-// first != 0 the fine z is the first sweep from zero (omega D^-1 r), computed on the fly and
-// stored to zf.  One thread per (coarse cell, child) pair: the 8 lanes of a coarse cell each
-// compute one child's residual and are summed with three butterfly shuffles, (c0+c1)+(c2+c3) ...
-// in a fixed order, so the result is deterministic.
-__global__ void k_mg_restrict(MgLevelK L, MgLevelK C, MgFrame F, float omega, int first, float* __restrict__ zf,
+// Restriction of the residual: rC[cc] = 0.5 * sum over fluid children of (r - A z).  With FIRST the
+// fine z is the first sweep from zero (omega D^-1 r), computed on the fly and stored to zf for every
+// fluid fine cell (a fluid cell under an air coarse cell too: the prolongation reads it).  One thread
+// per (coarse cell, child) pair: the 8 lanes of a coarse cell each compute one child's residual and
+// are summed with three butterfly shuffles, (c0+c1)+(c2+c3) ... in a fixed order, so the result is
+// deterministic.  The child (mg_child's value) and its six neighbours come in one round of loads: on
+// a sparse fine level of edge 2 or more the child is arithmetic and its block's table row is loaded;
+// on one of edge 1 the child's slot and its neighbours' slots are read from the frame's slot table at
+// their coordinates, not through the child's table row, which would add a round trip.
+template <int KIND, bool FIRST>
+__global__ void k_mg_restrict(MgLevelK L, MgLevelK C, MgFrame F, float omega, float* __restrict__ zf,
                               const int* done) {
-    if (*done) return;
-    const int t = blockIdx.x * WG + threadIdx.x;
-    const int cc = t >> 3, a = t & 7;
+    int dn = *done;
+    const int tid = blockIdx.x * WG + threadIdx.x;
+    const int cc = tid >> 3, a = tid & 7;
     const bool valid = cc < C.ncell;
-    float res = 0.f;
-    const bool cfl = valid && mg_fluid(C.info[cc]);
-    if (cfl) {
-        const int ch = mg_child(L, C, F, cc, a);
-        const uint16_t in = ch >= 0 ? L.info[ch] : (uint16_t)0;
-        if (mg_fluid(in)) {
-            const float d = mg_diag(in);
-            const float zc = first ? L.r[ch] * mg_wd(in, omega) : zf[ch];
-            if (first) zf[ch] = zc;
-            float az = d * zc;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                if (!(in & (1u << q))) continue;
-                const int n = mg_nbr(L, F, ch, q);
-                az -= first ? mg_z0(L, n, omega) : zf[n];
-            }
-            res = L.r[ch] - az;
+    const int ccv = valid ? cc : 0;
+    const unsigned cin = C.info[ccv];
+    int ch, t[6], nb[6];
+    unsigned in0;
+    float rch, zch0;
+    if (KIND == kMgBlk) {                                // mg_child's in-block case
+        const int sh = L.sh, csh = sh - 1, Ec = 1 << csh;
+        const int s = ccv >> (3 * csh);
+        const int l = ccv & ((1 << (3 * csh)) - 1);
+        const int li = l & (Ec - 1), lj = (l >> csh) & (Ec - 1), lk = l >> (2 * csh);
+        ch = (s << (3 * sh)) + ((((2 * lk + (a >> 2)) << sh) + (2 * lj + ((a >> 1) & 1))) << sh) + (2 * li + (a & 1));
+        mg_row<KIND>(L, F, ch, t);
+        in0 = L.info[ch];
+        rch = L.r[ch];
+        zch0 = FIRST ? 0.f : zf[ch];
+        mg_pin7(dn, in0, t);
+        mg_nbrs<KIND>(L, ch, t, nb);
+    } else {
+        const int I = ccv % C.dx, J = (ccv / C.dx) % C.dy, K = ccv / (C.dx * C.dy);
+        const int i = 2 * I + (a & 1), j = 2 * J + ((a >> 1) & 1), k = 2 * K + (a >> 2);
+        const bool inside = i < L.dx && j < L.dy && k < L.dz;
+        if (KIND == kMgDense) {
+            ch = inside ? (k * L.dy + j) * L.dx + i : -1;
+            mg_nbrs<KIND>(L, inside ? ch : 0, t, nb);
+        } else {
+            // slot-table entries at clamped coordinates, all loaded unconditionally, then masked
+            auto slot_raw = [&](int x, int y, int z) -> int {
+                x = x < 0 ? 0 : (x >= L.dx ? L.dx - 1 : x);
+                y = y < 0 ? 0 : (y >= L.dy ? L.dy - 1 : y);
+                z = z < 0 ? 0 : (z >= L.dz ? L.dz - 1 : z);
+                return F.slot[(z * L.dy + y) * L.dx + x];
+            };
+            auto in_frame = [&](int x, int y, int z) {
+                return x >= 0 && y >= 0 && z >= 0 && x < L.dx && y < L.dy && z < L.dz;
+            };
+            unsigned cs = (unsigned)slot_raw(i, j, k);
+            t[0] = slot_raw(i - 1, j, k);
+            t[1] = slot_raw(i + 1, j, k);
+            t[2] = slot_raw(i, j - 1, k);
+            t[3] = slot_raw(i, j + 1, k);
+            t[4] = slot_raw(i, j, k - 1);
+            t[5] = slot_raw(i, j, k + 1);
+            mg_pin7(dn, cs, t);
+            nb[0] = in_frame(i - 1, j, k) ? t[0] : -2;        // -2 outside the frame, else slot or -1
+            nb[1] = in_frame(i + 1, j, k) ? t[1] : -2;
+            nb[2] = in_frame(i, j - 1, k) ? t[2] : -2;
+            nb[3] = in_frame(i, j + 1, k) ? t[3] : -2;
+            nb[4] = in_frame(i, j, k - 1) ? t[4] : -2;
+            nb[5] = in_frame(i, j, k + 1) ? t[5] : -2;
+            ch = (inside && (int)cs >= 0) ? (int)cs : -1;
         }
+        const int cv = ch >= 0 ? ch : 0;
+        in0 = L.info[cv];
+        rch = L.r[cv];
+        zch0 = FIRST ? 0.f : zf[cv];
     }
+    const int chv = ch >= 0 ? ch : 0;
+    const unsigned in = ch >= 0 ? in0 : 0u;
+    float zch, zn[6];
+    if (FIRST) {
+        unsigned ni[6];
+        float nr[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int n = mg_ldx(nb[q], chv);
+            ni[q] = L.info[n];
+            nr[q] = L.r[n];
+        }
+        mg_pin6(ni);
+        mg_pin6(nr);
+        zch = rch * mg_wd(in, omega);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) zn[q] = mg_z0(ni[q], nr[q], omega);
+    } else {
+        zch = zch0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) zn[q] = zf[mg_ldx(nb[q], chv)];
+        mg_pin6(zn);
+    }
+    mg_pin1(rch);
+    float az = mg_diag(in) * zch;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
+    const bool cfl = valid && mg_fluid(cin);
+    const bool fine = valid && mg_fluid(in);
+    float res = (cfl && fine) ? rch - az : 0.f;
     res += __shfl_xor(res, 1, 64);
     res += __shfl_xor(res, 2, 64);
     res += __shfl_xor(res, 4, 64);
+    mg_pin(res);
+    if (dn) return;
+    if (FIRST && fine) zf[ch] = zch;
     if (valid && a == 0) C.r[cc] = cfl ? 0.5f * res : 0.f;
 }
 
 // Prolongation (piecewise constant) fused with one Jacobi sweep:
-// zp = z + zC[parent]; zout = zp + omega (r - A zp)/D.
+// zp = z + zC[parent]; zout = zp + omega (r - A zp)/D.  On a sparse level of edge 2 or more a cell's
+// parent is arithmetic in its index; otherwise (edge 1, dense) the parents of the cell and of its
+// neighbours follow from the cell's coordinates, so no parent needs its neighbour's index first.
+template <int KIND>
 __global__ void k_mg_prolong_smooth(MgLevelK L, MgLevelK C, MgFrame F, float omega, const float* __restrict__ z,
                                     const float* __restrict__ zc, float* __restrict__ zout, const int* done) {
-    if (*done) return;
+    int dn = *done;
     const int c = blockIdx.x * WG + threadIdx.x;
     if (c >= L.ncell) return;
-    const uint16_t in = L.info[c];
-    float out = 0.f;
-    if (mg_fluid(in)) {
-        const float d = mg_diag(in);
-        const float zp = z[c] + zc[mg_parent(L, C, F, c)];
-        float az = d * zp;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            if (!(in & (1u << q))) continue;
-            const int n = mg_nbr(L, F, c, q);
-            az -= z[n] + zc[mg_parent(L, C, F, n)];
-        }
-        out = zp + (L.r[c] - az) * mg_wd(in, omega);
+    constexpr bool inblk = KIND == kMgBlk;
+    auto parent_ijk = [&](int x, int y, int w) { return ((w >> 1) * C.dy + (y >> 1)) * C.dx + (x >> 1); };
+    auto parent_blk = [&](int x) {                       // mg_parent's in-block case
+        const int sh = L.sh, csh = sh - 1, E = 1 << sh;
+        const int s = x >> (3 * sh);
+        const int l = x & ((1 << (3 * sh)) - 1);
+        const int li = (l & (E - 1)) >> 1, lj = ((l >> sh) & (E - 1)) >> 1, lk = (l >> (2 * sh)) >> 1;
+        return (s << (3 * csh)) + (((lk << csh) + lj) << csh) + li;
+    };
+    unsigned in = L.info[c];
+    float rc = L.r[c], zcc = z[c];
+    int i = 0, j = 0, k = 0;
+    float zpc = 0.f;
+    if (KIND == kMgEdge1) {
+        const int tc = F.coord[c];
+        i = tc % L.dx; j = (tc / L.dx) % L.dy; k = tc / (L.dx * L.dy);
+    } else if (KIND == kMgDense) {
+        i = c % L.dx; j = (c / L.dx) % L.dy; k = c / (L.dx * L.dy);
+        zpc = zc[parent_ijk(i, j, k)];
+    } else {
+        zpc = zc[parent_blk(c)];
     }
+    int t[6], nb[6];
+    mg_row<KIND>(L, F, c, t);
+    mg_pin7(dn, in, t);
+    mg_pin1(rc);
+    mg_pin1(zcc);
+    if (KIND == kMgEdge1) zpc = zc[parent_ijk(i, j, k)];
+    mg_nbrs<KIND>(L, c, t, nb);
+    float za[6], zb[6], zn[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int n = mg_ldx(nb[q], c);
+        int pn;
+        if (inblk) {
+            pn = parent_blk(n);
+        } else {
+            const int dir = (q & 1) ? 1 : -1;
+            const bool use = nb[q] >= 0;                 // else the cell itself, like n
+            pn = parent_ijk(i + (use && q < 2 ? dir : 0), j + (use && (q >> 1) == 1 ? dir : 0),
+                            k + (use && q >= 4 ? dir : 0));
+        }
+        za[q] = z[n];
+        zb[q] = zc[pn];
+    }
+    mg_pin6(za);
+    mg_pin6(zb);
+    mg_pin1(zpc);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) zn[q] = za[q] + zb[q];
+    const float zp = zcc + zpc;
+    float az = mg_diag(in) * zp;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
+    const float out = mg_fluid(in) ? zp + (rc - az) * mg_wd(in, omega) : 0.f;
+    mg_pin(out);
+    if (dn) return;
     zout[c] = out;
 }
 
@@ -284,8 +506,9 @@ __global__ void k_mg_prolong_smooth(MgLevelK L, MgLevelK C, MgFrame F, float ome
 // coarsest level: `coarse_sweeps` Jacobi sweeps from zero; up: prolongation and the post-sweeps)
 // runs in one 1024-thread workgroup with every level's r, z and descriptors in LDS, replacing four
 // launches per level.  Each level's arithmetic is that of k_mg_smooth_first2 / k_mg_restrict /
-// k_mg_prolong_smooth / k_mg_smooth, in the same order; the coarsest level gets `coarse_sweeps`
-// Jacobi sweeps from zero.
+// k_mg_prolong_smooth / k_mg_smooth, in the same order, and like those kernels the first sweep from
+// zero and the prolonged values are formed on the fly, so a level costs two barrier-separated passes
+// down and one (nu = 1) or two up.  The coarsest level gets `coarse_sweeps` Jacobi sweeps from zero.
 constexpr int kMgBottomLevels = 6;                     // levels handled at most
 constexpr int kMgBottomCells = 5120;                   // cells of those levels together (LDS: 14 B each)
 constexpr int kMgBottomThreads = 1024;
@@ -318,15 +541,31 @@ __device__ __forceinline__ int bottom_index(const MgBottom& bt, int l, int c) {
 }
 
 // diag * z - sum of the fluid neighbours' z on a dense level held in LDS (q order -x,+x,-y,+y,-z,+z).
+// The six neighbour offsets are read unconditionally, from the cell itself where the descriptor has no
+// fluid neighbour, so the seven reads issue together; a skipped term subtracts +0 (bit-identical).
+__device__ __forceinline__ void bottom_nbrs(unsigned in, int c, int sy, int sz, int nb[6]) {
+    nb[0] = (in & 1u) ? c - 1 : c;
+    nb[1] = (in & 2u) ? c + 1 : c;
+    nb[2] = (in & 4u) ? c - sy : c;
+    nb[3] = (in & 8u) ? c + sy : c;
+    nb[4] = (in & 16u) ? c - sz : c;
+    nb[5] = (in & 32u) ? c + sz : c;
+}
 __device__ __forceinline__ float bottom_az(unsigned in, const float* z, int c, int sy, int sz) {
+    int nb[6];
+    bottom_nbrs(in, c, sy, sz, nb);
+    float v[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v[q] = z[nb[q]];
     float az = mg_diag(in) * z[c];
-    if (in & 1u) az -= z[c - 1];
-    if (in & 2u) az -= z[c + 1];
-    if (in & 4u) az -= z[c - sy];
-    if (in & 8u) az -= z[c + sy];
-    if (in & 16u) az -= z[c - sz];
-    if (in & 32u) az -= z[c + sz];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? v[q] : 0.f;
     return az;
+}
+
+// One Jacobi sweep at cell c of a bottom level held in LDS: zout = z + omega (r - A z)/D (k_mg_smooth).
+__device__ __forceinline__ float bottom_sweep(unsigned in, float r, const float* z, int c, int sy, int sz, float omega) {
+    return mg_fluid(in) ? z[c] + (r - bottom_az(in, z, c, sy, sz)) * mg_wd(in, omega) : 0.f;
 }
 
 __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, float omega, int nu, int coarse_sweeps,
@@ -338,35 +577,80 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
     __shared__ uint16_t si[kMgBottomCells];
     const int tid = threadIdx.x;
     constexpr int T = kMgBottomThreads;
+    constexpr int kPer = (kMgBottomCells + T - 1) / T;      // cells per thread of any one level, at most
     const int L = bt.nlev;
+    // load: a level has at most kPer cells per thread; their indices, then all of their descriptors
+    // and right-hand sides, are loaded in unrolled rounds (safe index 0 where there is no cell), so a
+    // level costs one round trip to memory, two for a sparse level's slot lookup, not one per cell
     for (int l = 0; l < L; ++l) {
         const int n = bt.dx[l] * bt.dy[l] * bt.dz[l];
-        for (int c = tid; c < n; c += T) {
-            const int g = bottom_index(bt, l, c);
-            const uint16_t in = g >= 0 ? bt.info[l][g] : (uint16_t)0;
-            si[bt.off[l] + c] = in;
-            if (l == 0) sr[c] = mg_fluid(in) ? bt.r0[g] : 0.f;
+        int g[kPer];
+        uint16_t iv[kPer];
+        float rv[kPer];
+        const int sh = bt.sh[l];
+        if (sh < 0) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) g[k] = tid + k * T < n ? tid + k * T : -1;
+        } else {                                         // bottom_index with the slot loads batched
+            const int dx = bt.dx[l], dy = bt.dy[l], m = (1 << sh) - 1;
+            int raw[kPer];
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int c = tid + k * T < n ? tid + k * T : 0;
+                const int i = c % dx, j = (c / dx) % dy, kk = c / (dx * dy);
+                raw[k] = bt.slot[((kk >> sh) * bt.fy + (j >> sh)) * bt.fx + (i >> sh)];
+            }
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int c = tid + k * T;
+                const int i = c % dx, j = (c / dx) % dy, kk = c / (dx * dy);
+                g[k] = (c < n && raw[k] >= 0) ? (raw[k] << (3 * sh)) + ((((kk & m) << sh) + (j & m)) << sh) + (i & m) : -1;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int gs = g[k] >= 0 ? g[k] : 0;
+            iv[k] = bt.info[l][gs];
+            rv[k] = l == 0 ? bt.r0[gs] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int c = tid + k * T;
+            if (c < n) {
+                const uint16_t in = g[k] >= 0 ? iv[k] : (uint16_t)0;
+                si[bt.off[l] + c] = in;
+                if (l == 0) sr[c] = mg_fluid(in) ? rv[k] : 0.f;
+            }
         }
     }
     __syncthreads();
-    // down
+    // down: the pre-smoothed z of level l goes to sz + off[l]
     for (int l = 0; l + 1 < L; ++l) {
         const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
-        float* r = sr + o;
+        const float* r = sr + o;
         float* z = sz + o;
-        float* zt = st + o;
         const uint16_t* in = si + o;
-        // first sweep from zero: omega r / D (k_mg_smooth_first2's z1, or the whole pre-smoothing for nu = 1)
-        for (int c = tid; c < n; c += T) (nu == 1 ? z : zt)[c] = mg_fluid(in[c]) ? r[c] * mg_wd(in[c], omega) : 0.f;
-        __syncthreads();
-        if (nu == 2) {
-            for (int c = tid; c < n; c += T) {
-                float out = 0.f;
-                if (mg_fluid(in[c])) out = zt[c] + (r[c] - bottom_az(in[c], zt, c, sy, szz)) * mg_wd(in[c], omega);
-                z[c] = out;
+        // nu = 1: z = omega r / D; nu = 2: a second sweep from that z, whose neighbour values are formed
+        // on the fly (k_mg_smooth_first2)
+        for (int c = tid; c < n; c += T) {
+            const unsigned ic = in[c];
+            const float wd = mg_wd(ic, omega);
+            const float z1 = r[c] * wd;
+            float out = z1;
+            if (nu == 2) {
+                int nb[6];
+                bottom_nbrs(ic, c, sy, szz, nb);
+                float v[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) v[q] = r[nb[q]] * mg_wd(in[nb[q]], omega);
+                float az = mg_diag(ic) * z1;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) az -= (ic & (1u << q)) ? v[q] : 0.f;
+                out = z1 + (r[c] - az) * wd;
             }
-            __syncthreads();
+            z[c] = mg_fluid(ic) ? out : 0.f;
         }
+        __syncthreads();
         // restriction: rC = 0.5 * sum of the fluid children's residuals r - A z, children summed as
         // ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) like k_mg_restrict's butterflies
         const int oc = bt.off[l + 1], cdx = bt.dx[l + 1], cdy = bt.dy[l + 1], nc = cdx * cdy * bt.dz[l + 1];
@@ -378,11 +662,11 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
 #pragma unroll
                 for (int a = 0; a < 8; ++a) {
                     const int i = 2 * I + (a & 1), j = 2 * J + ((a >> 1) & 1), k = 2 * K + (a >> 2);
-                    ch[a] = 0.f;
-                    if (i < dx && j < dy && k < bt.dz[l]) {
-                        const int c = (k * dy + j) * dx + i;
-                        if (mg_fluid(in[c])) ch[a] = r[c] - bottom_az(in[c], z, c, sy, szz);
-                    }
+                    const bool inside = i < dx && j < dy && k < bt.dz[l];
+                    const int c = inside ? (k * dy + j) * dx + i : 0;
+                    const unsigned ic = inside ? in[c] : 0u;
+                    const float res = r[c] - bottom_az(ic, z, c, sy, szz);
+                    ch[a] = mg_fluid(ic) ? res : 0.f;
                 }
                 acc = 0.5f * (((ch[0] + ch[1]) + (ch[2] + ch[3])) + ((ch[4] + ch[5]) + (ch[6] + ch[7])));
             }
@@ -390,66 +674,72 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
         }
         __syncthreads();
     }
-    // coarsest level: Jacobi sweeps from zero
+    // coarsest level: `coarse_sweeps` Jacobi sweeps from zero (the first is omega r / D), alternating
+    // between sz and st; zfin = where the last one is
+    float* zfin;
     {
         const int l = L - 1;
         const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
+        const float* r = sr + o;
+        const uint16_t* in = si + o;
         float* cur = sz + o;
         float* nxt = st + o;
-        for (int c = tid; c < n; c += T) cur[c] = 0.f;
+        for (int c = tid; c < n; c += T) cur[c] = (coarse_sweeps > 0 && mg_fluid(in[c])) ? r[c] * mg_wd(in[c], omega) : 0.f;
+        for (int s = 1; s < coarse_sweeps; ++s) {
+            __syncthreads();
+            for (int c = tid; c < n; c += T) nxt[c] = bottom_sweep(in[c], r[c], cur, c, sy, szz, omega);
+            float* tmp = cur; cur = nxt; nxt = tmp;
+        }
+        zfin = ((coarse_sweeps < 1 ? 1 : coarse_sweeps) & 1) ? sz + o : st + o;
         __syncthreads();
-        for (int s = 0; s < coarse_sweeps; ++s) {
-            for (int c = tid; c < n; c += T) {
-                const unsigned in = si[o + c];
-                float out = 0.f;
-                if (mg_fluid(in)) out = cur[c] + (sr[o + c] - bottom_az(in, cur, c, sy, szz)) * mg_wd(in, omega);
-                nxt[c] = out;
-            }
-            __syncthreads();
-            float* t = cur; cur = nxt; nxt = t;
-        }
-        if (cur != sz + o) {
-            for (int c = tid; c < n; c += T) sz[o + c] = cur[c];
-            __syncthreads();
-        }
     }
-    // up
+    // up: level l's post-smoothed z ends in st (nu = 1) or sz (nu = 2) + off[l]
     for (int l = L - 2; l >= 0; --l) {
         const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
-        const int oc = bt.off[l + 1], cdx = bt.dx[l + 1], cdy = bt.dy[l + 1];
-        float* r = sr + o;
+        const int cdx = bt.dx[l + 1], cdy = bt.dy[l + 1];
+        const float* r = sr + o;
         float* z = sz + o;
         float* zt = st + o;
         const uint16_t* in = si + o;
-        // zp = z + zC[parent] on every cell (k_mg_prolong_smooth adds the same two terms per cell)
+        const float* zc = zfin;
+        // prolongation fused with the first post-sweep: zp = z + zC[parent] on the cell and its fluid
+        // neighbours, formed on the fly (k_mg_prolong_smooth), into zt
         for (int c = tid; c < n; c += T) {
+            const unsigned ic = in[c];
             const int i = c % dx, j = (c / dx) % dy, k = c / (dx * dy);
-            zt[c] = z[c] + sz[oc + ((k >> 1) * cdy + (j >> 1)) * cdx + (i >> 1)];
+            auto par = [&](int x, int y, int w) { return ((w >> 1) * cdy + (y >> 1)) * cdx + (x >> 1); };
+            int nb[6], pn[6];
+            bottom_nbrs(ic, c, sy, szz, nb);
+            pn[0] = par(i - ((ic & 1u) ? 1 : 0), j, k);
+            pn[1] = par(i + ((ic & 2u) ? 1 : 0), j, k);
+            pn[2] = par(i, j - ((ic & 4u) ? 1 : 0), k);
+            pn[3] = par(i, j + ((ic & 8u) ? 1 : 0), k);
+            pn[4] = par(i, j, k - ((ic & 16u) ? 1 : 0));
+            pn[5] = par(i, j, k + ((ic & 32u) ? 1 : 0));
+            float v[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v[q] = z[nb[q]] + zc[pn[q]];
+            const float zpc = z[c] + zc[par(i, j, k)];
+            float az = mg_diag(ic) * zpc;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) az -= (ic & (1u << q)) ? v[q] : 0.f;
+            zt[c] = mg_fluid(ic) ? zpc + (r[c] - az) * mg_wd(ic, omega) : 0.f;
         }
         __syncthreads();
-        // one sweep from zp; nu = 2 adds a second (k_mg_smooth)
-        for (int c = tid; c < n; c += T) {
-            float out = 0.f;
-            if (mg_fluid(in[c])) out = zt[c] + (r[c] - bottom_az(in[c], zt, c, sy, szz)) * mg_wd(in[c], omega);
-            z[c] = out;
-        }
-        __syncthreads();
-        if (nu == 2) {
-            for (int c = tid; c < n; c += T) {
-                float out = 0.f;
-                if (mg_fluid(in[c])) out = z[c] + (r[c] - bottom_az(in[c], z, c, sy, szz)) * mg_wd(in[c], omega);
-                zt[c] = out;
-            }
+        zfin = zt;
+        if (nu == 2) {                                   // second post-sweep (k_mg_smooth), into z
+            for (int c = tid; c < n; c += T) z[c] = bottom_sweep(in[c], r[c], zt, c, sy, szz, omega);
             __syncthreads();
-            for (int c = tid; c < n; c += T) z[c] = zt[c];
-            __syncthreads();
+            zfin = z;
         }
     }
     {
         const int n = bt.dx[0] * bt.dy[0] * bt.dz[0];
-        for (int c = tid; c < n; c += T) {
-            const int g = bottom_index(bt, 0, c);
-            if (g >= 0) bt.z0[g] = sz[c];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int c = tid + k * T;
+            const int g = c < n ? bottom_index(bt, 0, c) : -1;
+            if (g >= 0) bt.z0[g] = zfin[c];
         }
     }
 }
