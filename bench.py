@@ -43,6 +43,8 @@ HBM_PEAK_GBS = 8000.0
 PSPMV_TRAFFIC_PER_CELL = 17.11    # k_pupdate_spmv: (2 x 22310.3 + 32056.6) KiB per 4.590 M cells
 UPD_TRAFFIC_PER_CELL = 29.12     # k_update_smooth2: (2 x 40949.5 + 48564.0) KiB per 4.587 M cells
 MG0_TRAFFIC_PER_CELL = 12.51      # k_mg0_prolong_smooth: (2 x 20662.6 + 14684.8) KiB per 4.585 M cells
+TIMING_STRIDE = 4                  # the dominant kernel is timed on every 4th step of the timed region
+OTHER_TIMING_STEPS = 10            # steps after the timed region that time the two secondary kernels
 DT = 1.0 / 240.0
 
 
@@ -159,8 +161,10 @@ def run_scene(a, L, cfg, steps):
             L.check(rc)
     h.set_particles(pos, vel)
     del pos, vel
+    # HIP events around a launch cost it about 5 us, so inside the timed region only the dominant
+    # kernel is timed, on every TIMING_STRIDE-th step (a uniform sample of its launches over the
+    # region); the other kernels' lines come from a short run after it
     if not a.no_kernel_timing:
-        h.set_kernel_timing(["spmv", "mg0", "update"])
         h.kernel_timing(reset=True)
     L.device_synchronize()
 
@@ -169,6 +173,8 @@ def run_scene(a, L, cfg, steps):
     stats = None
     t_start = time.perf_counter()
     for s in range(steps):
+        if not a.no_kernel_timing:
+            h.set_kernel_timing(["update"] if s % TIMING_STRIDE == 0 else [])
         ts = time.perf_counter()
         nps.append(h.num_particles())
         rc = h.step_rc(sc["dt"])
@@ -189,6 +195,20 @@ def run_scene(a, L, cfg, steps):
     L.device_synchronize()
     total_s = time.perf_counter() - t_start
     kt = h.kernel_timing(reset=True) if not a.no_kernel_timing else None
+    if kt is not None:
+        # untimed: OTHER_TIMING_STEPS more steps with events on the other two reported kernels
+        h.set_kernel_timing(["spmv", "mg0"])
+        h.kernel_timing(reset=True)
+        for _ in range(OTHER_TIMING_STEPS):
+            rc = h.step_rc(sc["dt"])
+            if rc not in (L.VARE_OK, L.VARE_ENOCONV):
+                L.check(rc)
+        L.device_synchronize()
+        ko = h.kernel_timing(reset=True)
+        h.set_kernel_timing([])
+        kt["spmv"], kt["mg0"] = ko["spmv"], ko["mg0"]
+        sampled = "every %d-th step of the timed region" % TIMING_STRIDE
+        after = "%d untimed steps after the timed region" % OTHER_TIMING_STEPS
 
     roofline = None
     if kt is not None and kt["mg0"][1] > 0 and kt["spmv"][1] > 0 and kt["update"][1] > 0:
@@ -205,11 +225,14 @@ def run_scene(a, L, cfg, steps):
         # level-0 pass.  Algorithmic bytes per solver cell: read x, p, r, Ap 16 + descriptor 2, write
         # x, r, z 12.  Neighbour values across a block face are re-reads.
         roofline = line("k_update_smooth2", "update", 30.0, UPD_TRAFFIC_PER_CELL)
+        roofline["timed_on"] = sampled
         roofline["others"] = [
             # the level-0 prolongation + Jacobi pass: read z 4 + coarse z 0.5 + r 4 + descriptor 2, write z 4
             line("k_mg0_prolong_smooth", "mg0", 14.5, MG0_TRAFFIC_PER_CELL),
             # the direction update fused with A*p: read old p 4 + z 4 + descriptor 2, write p 4 + Ap 4
             line("k_pupdate_spmv", "spmv", 18.0, PSPMV_TRAFFIC_PER_CELL)]
+        for o in roofline["others"]:
+            o["timed_on"] = after
     sm = np.array(step_ms)
     nx, ny, nz = sc["dims"]
     res = {
