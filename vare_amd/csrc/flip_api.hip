@@ -420,7 +420,7 @@ template <int B> int bin_particles(vare_flip* h) {
     CHK(fill_u32(h, h->dint + 4, 0, 2));           // [4] sort overflow, [5] long-bin count
     CHK(fill_u32(h, h->dint + 7, 0, 1));           // [7] mid-bin count
     if (h->np)
-        hipLaunchKernelGGL(k_ckey<B>, dim3(nblocks((h->np + kBinSlices - 1) / kBinSlices)), dim3(WG), 0, h->stream,
+        hipLaunchKernelGGL(k_ckey<B>, dim3(nblocks((h->np + kKeySlices - 1) / kKeySlices)), dim3(WG), 0, h->stream,
                            (const float*)h->pa[0],
                            (const float*)h->pa[1], (const float*)h->pa[2], (int)h->np, h->frame(), nc, h->pkey, h->prank,
                            h->ccount);

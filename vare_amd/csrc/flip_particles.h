@@ -20,18 +20,28 @@ constexpr int kSortReg = 16;        // bins up to this length are sorted in regi
 constexpr unsigned long long kSortPad = ~0ull;
 
 // Bin of every particle (storage index of its base cell, or nc) + its arrival rank inside the bin.
-// Lanes holding equal keys in a row form a run; the first lane of every run reserves the whole run
-// with one atomic add, so a wave issues a single atomic instruction whatever the number of bins.
-// A workgroup covers kBinSlices consecutive slices of WG particles, one particle per thread per
-// slice; the slices' loads, slot lookups and atomics are independent, so their latencies overlap.
-constexpr int kBinSlices = 4;
+// A workgroup covers kKeySlices consecutive slices of WG particles, one particle per thread per slice
+// (the slices' loads and slot lookups are independent, so their latencies overlap).  The counts are
+// aggregated in LDS first: every key of the workgroup is entered in an open-addressing hash table
+// (load factor at most 1/2) whose LDS counter hands out the rank among the workgroup's particles of
+// that bin, then one global atomic per distinct key reserves the workgroup's run in the bin.  The
+// particles of a workgroup come from a few neighbouring cells, so this takes several times fewer
+// global atomics than one per particle or per run of equal keys, and global atomic throughput is what
+// bounds this kernel.  Ranks depend on the atomics' order; the bins are sorted by reference index
+// afterwards, so the permutation does not.
+constexpr int kBinSlices = 4;                    // k_pscatter
+constexpr int kKeySlices = 8;                    // k_ckey
+constexpr int kKeyHash = 2 * kKeySlices * WG;    // LDS hash slots of k_ckey (32 KB of keys and counts)
 
 template <int B>
-__global__ void k_ckey(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, int np,
-                       FrameK f, int nc, int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count) {
+__global__ void __launch_bounds__(WG) k_ckey(const float* __restrict__ px, const float* __restrict__ py,
+                                             const float* __restrict__ pz, int np, FrameK f, int nc,
+                                             int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count) {
     constexpr int B3 = B * B * B;
-    constexpr int J = kBinSlices;
-    const int lane = threadIdx.x & 63;
+    constexpr int J = kKeySlices, H = kKeyHash;
+    __shared__ int hk[H];                        // key of the slot, -1 when free
+    __shared__ int hc[H];                        // count, then the global base of the slot's run
+    for (int i = threadIdx.x; i < H; i += WG) { hk[i] = -1; hc[i] = 0; }
     int idx[J], k[J];
     bool valid[J];
 #pragma unroll
@@ -59,25 +69,40 @@ __global__ void k_ckey(const float* __restrict__ px, const float* __restrict__ p
         }
         key[idx[j]] = k[j];
     }
-    int base[J], mine[J];
+    __syncthreads();
+    int hs[J], loc[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const int prev = __shfl(k[j], lane > 0 ? lane - 1 : 0, 64);
-        const bool head = valid[j] && (lane == 0 || k[j] != prev);
-        const unsigned long long heads = __ballot(head);
-        const int nvalid = (int)__popcll(__ballot(valid[j]));   // valid lanes are a prefix of the wave
-        const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1);
-        mine[j] = 63 - __clzll((long long)(upto | 1ull));       // first lane of my run
-        const unsigned long long after = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
-        const int next = after ? __ffsll((long long)after) - 1 : nvalid;
-        base[j] = 0;
-        if (head) base[j] = atomicAdd(count + k[j], next - lane);
+        hs[j] = 0;
+        loc[j] = 0;
+        if (!valid[j]) continue;
+        // keys are >= 0 and at most J*WG are entered, so a free slot is always found
+        int q = (int)(((unsigned)k[j] * 2654435761u) >> 8) & (H - 1);
+        while (true) {
+            const int old = atomicCAS(&hk[q], -1, k[j]);
+            if (old == -1 || old == k[j]) break;
+            q = (q + 1) & (H - 1);
+        }
+        hs[j] = q;
+        loc[j] = atomicAdd(&hc[q], 1);
     }
+    __syncthreads();
+    {
+        constexpr int U = H / WG;                // slots per thread; their atomics are in flight together
+        int res[U];
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int b = __shfl(base[j], mine[j], 64);
-        if (valid[j]) rank[idx[j]] = b + lane - mine[j];
+        for (int u = 0; u < U; ++u) {
+            const int i = threadIdx.x + u * WG;
+            const int c = hc[i];
+            res[u] = c > 0 ? atomicAdd(count + hk[i], c) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) hc[threadIdx.x + u * WG] = res[u];
     }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+        if (valid[j]) rank[idx[j]] = hc[hs[j]] + loc[j];
 }
 
 // Exclusive scan of bin counts -> bin starts.
