@@ -430,8 +430,8 @@ template <int B> int bin_particles(vare_flip* h) {
         hipLaunchKernelGGL(k_pscatter, dim3(nblocks((h->np + kBinSlices - 1) / kBinSlices)), dim3(WG), 0, h->stream,
                            (int)h->np, (const int*)h->pkey,
                            (const int*)h->prank, (const int*)h->cstart, (const int*)h->pid, h->perm);
-        hipLaunchKernelGGL(k_sort_short, dim3(nblocks((size_t)nc + 1)), dim3(WG), 0, h->stream, (const int*)h->cstart,
-                           nc + 1, h->perm, h->midbins, h->dint + 7, h->longbins, h->dint + 5);
+        hipLaunchKernelGGL(k_sort_block<B>, dim3(h->nb + 1), dim3(WG), 0, h->stream, (const int*)h->cstart, (int)h->nb,
+                           nc, h->perm, h->midbins, h->dint + 7, h->longbins, h->dint + 5);
         hipLaunchKernelGGL(k_sort_wave, dim3(kSortLongGrid * 4), dim3(WG), 0, h->stream, (const int*)h->cstart,
                            (const int*)h->midbins, (const int*)(h->dint + 7), h->perm);
         hipLaunchKernelGGL(k_sort_long, dim3(kSortLongGrid), dim3(1024), 0, h->stream, (const int*)h->cstart,

@@ -133,25 +133,12 @@ __global__ void k_pscatter(int np, const int* __restrict__ key, const int* __res
         if (idx[j] < np) perm[st[j] + rk[j]] = ((unsigned long long)(unsigned)pd[j] << 32) | (unsigned)idx[j];
 }
 
-// Sort every bin's keys ascending, i.e. by logical index, which turns the permutation into a stable
-// counting sort of the reference order that does not depend on the atomic arrival order.  One
-// thread per bin: bins of up to kSortReg entries are sorted in registers (bitonic network); bins of
-// up to 64 are listed for k_sort_wave, longer ones for k_sort_long.
-__global__ void k_sort_short(const int* __restrict__ bstart, int nbins, unsigned long long* __restrict__ perm,
-                             int* __restrict__ midbins, int* __restrict__ nmid, int* __restrict__ longbins,
-                             int* __restrict__ nlong) {
-    const int c = blockIdx.x * WG + threadIdx.x;
-    if (c >= nbins) return;
-    const int s0 = bstart[c], n = bstart[c + 1] - s0;
-    if (n <= 1) return;
-    if (n > kSortReg) {
-        if (n <= 64) midbins[atomicAdd(nmid, 1)] = c;
-        else longbins[atomicAdd(nlong, 1)] = c;
-        return;
-    }
+// Sort of one bin of 2..kSortReg keys in place at p (LDS or global memory), in registers with a
+// bitonic network; an already sorted bin is left as it is.
+__device__ __forceinline__ void sort_bin_regs(unsigned long long* p, int n) {
     unsigned long long v[kSortReg];
 #pragma unroll
-    for (int q = 0; q < kSortReg; ++q) v[q] = q < n ? perm[s0 + q] : kSortPad;
+    for (int q = 0; q < kSortReg; ++q) v[q] = q < n ? p[q] : kSortPad;
     bool sorted = true;
 #pragma unroll
     for (int q = 0; q + 1 < kSortReg; ++q) sorted = sorted && v[q] < v[q + 1];
@@ -170,7 +157,67 @@ __global__ void k_sort_short(const int* __restrict__ bstart, int nbins, unsigned
             }
 #pragma unroll
     for (int q = 0; q < kSortReg; ++q)
-        if (q < n) perm[s0 + q] = v[q];
+        if (q < n) p[q] = v[q];
+}
+
+// Sort every bin's keys ascending, i.e. by logical index, which turns the permutation into a stable
+// counting sort of the reference order that does not depend on the atomic arrival order.  One
+// workgroup per block (its B^3 bins are consecutive, so its keys are one contiguous range) and one
+// for the extra bin Nc: the range is staged through LDS with coalesced loads and stores, and each
+// bin of up to kSortReg keys is sorted there by one thread in registers.  Bins of up to 64 keys are
+// listed for k_sort_wave, longer ones for k_sort_long (both sort in global memory afterwards; the
+// range written back holds their keys unchanged).  A block with more than kSortBlockCap keys is
+// sorted bin by bin in global memory.
+constexpr int kSortBlockCap = 6144;                // keys of one block staged in LDS (48 KB)
+constexpr int kStageLoads = 8;
+template <int B>
+__global__ void __launch_bounds__(WG) k_sort_block(const int* __restrict__ bstart, int nb, int nc,
+                                                   unsigned long long* __restrict__ perm, int* __restrict__ midbins,
+                                                   int* __restrict__ nmid, int* __restrict__ longbins,
+                                                   int* __restrict__ nlong) {
+    constexpr int B3 = B * B * B;
+    __shared__ unsigned long long sh[kSortBlockCap];
+    __shared__ int cs[B3 + 1];
+    const int s = blockIdx.x;
+    const int b0 = s < nb ? s * B3 : nc;           // first bin of this workgroup
+    const int nbin = s < nb ? B3 : 1;
+    for (int l = threadIdx.x; l <= nbin; l += WG) cs[l] = bstart[b0 + l];
+    __syncthreads();
+    const int r0 = cs[0], n = cs[nbin] - r0;
+    if (n <= 1) return;
+    const bool staged = n <= kSortBlockCap;
+    if (staged) {                                  // kStageLoads loads per thread in flight at a time
+        constexpr int U = kStageLoads;
+        for (int t0 = 0; t0 < n; t0 += U * WG) {
+            unsigned long long v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u * WG + threadIdx.x;
+                v[u] = t < n ? perm[r0 + t] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u * WG + threadIdx.x;
+                if (t < n) sh[t] = v[u];
+            }
+        }
+        __syncthreads();
+    }
+    for (int l = threadIdx.x; l < nbin; l += WG) {
+        const int a = cs[l] - r0, m = cs[l + 1] - cs[l];
+        if (m <= 1) continue;
+        if (m > kSortReg) {
+            if (m <= 64) midbins[atomicAdd(nmid, 1)] = b0 + l;
+            else longbins[atomicAdd(nlong, 1)] = b0 + l;
+            continue;
+        }
+        if (staged) sort_bin_regs(sh + a, m);
+        else sort_bin_regs(perm + r0 + a, m);
+    }
+    if (staged) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < n; t += WG) perm[r0 + t] = sh[t];
+    }
 }
 
 // Bins of kSortReg+1 .. 64 entries: one wave per bin, one key per lane, bitonic sort across the
