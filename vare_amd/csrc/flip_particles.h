@@ -368,9 +368,10 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
     const int t0 = f.coord[s];
     const int ox = (t0 % f.fx) * B, oy = ((t0 / f.fx) % f.fy) * B, oz = (t0 / (f.fx * f.fy)) * B;
     const int li = t % B1 - 1, lj = (t / B1) % B1 - 1, lk = t / (B1 * B1) - 1;     // base cell - origin
-    float am[8], ax[8], ay[8], az[8], ac[8];
+    float am[8], ax[8], ay[8], az[8];
+    float ac = 0.f;                                  // count term: 1 per particle at every corner
 #pragma unroll
-    for (int c = 0; c < 8; ++c) { am[c] = 0.f; ax[c] = 0.f; ay[c] = 0.f; az[c] = 0.f; ac[c] = 0.f; }
+    for (int c = 0; c < 8; ++c) { am[c] = 0.f; ax[c] = 0.f; ay[c] = 0.f; az[c] = 0.f; }
     if (t < NBASE) {
         const int dx = li < 0 ? -1 : 0, dy = lj < 0 ? -1 : 0, dz = lk < 0 ? -1 : 0;
         const int sl = nsl[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
@@ -378,6 +379,7 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
             const int cell = sl * B3 + ((lk - dz * B) * B + (lj - dy * B)) * B + (li - dx * B);
             const int q0 = cstart[cell], q1 = cstart[cell + 1];
             const int ci = ox + li, cj = oy + lj, ck = oz + lk;
+            ac = (float)(q1 - q0);                   // the sum of q1-q0 ones, exact below 2^24
             for (int n = q0; n < q1; ++n) {              // the bin's particles are contiguous
                 const float x = px[n], y = py[n], z = pz[n];
                 const float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
@@ -393,7 +395,6 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
                     ax[c] += w * vx;
                     ay[c] += w * vy;
                     az[c] += w * vz;
-                    ac[c] += 1.f;
                 }
             }
         }
@@ -408,7 +409,7 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
             tx[q] += ax[c];
             ty[q] += ay[c];
             tz[q] += az[c];
-            tc[q] += ac[c];
+            tc[q] += ac;
         }
         __syncthreads();
     }
