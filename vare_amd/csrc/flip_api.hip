@@ -523,7 +523,7 @@ void enqueue_precond_init(vare_flip* h) {
     if (nv)
         hipLaunchKernelGGL(k_cg_precond_init, dim3(gs), dim3(WG), 0, h->stream, nv, h->clist(), vecs(h), h->partials,
                            (const CgState*)h->st);
-    hipLaunchKernelGGL(k_fin_precond_init, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+    hipLaunchKernelGGL(k_fin_precond_init, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
                        nv ? (int)gs : 0);
 }
 
@@ -541,7 +541,7 @@ template <int B> int stage_system(vare_flip* h) {
                        (const float*)h->cx, (const float*)h->cy, (const float*)h->cz, h->vox_wall, vecs(h), h->partials,
                        h->partials2, h->bfluid);
     tend(h, VARE_KERNEL_DIV);
-    hipLaunchKernelGGL(k_fin_divergence, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+    hipLaunchKernelGGL(k_fin_divergence, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
                        (const double*)h->partials2, (int)g);
     // the solver's blocks: those holding a fluid cell
     OpFluidBlocks ofb{h->bfluid, h->blist};
@@ -849,7 +849,7 @@ template <int B> void enqueue_iteration(vare_flip* h, int k) {
             hipExtLaunchKernelGGL((k_pupdate_spmv<B, kIterK, 0>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, e0, e1,
                                   0, tab, v, pk, h->partials, (const CgState*)h->st);
     }
-    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
                        k == 0 ? nwg_sp : nwg_it);
     v.p = pk;
     if (fuse) {
@@ -857,11 +857,11 @@ template <int B> void enqueue_iteration(vare_flip* h, int k) {
         tpair(h, VARE_KERNEL_UPDATE, (int64_t)nv, u0, u1);
         hipExtLaunchKernelGGL((k_update_smooth2<B, kIterK>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, u0, u1, 0,
                               tab, v, rout, h->prm.mg_omega, h->mg_z[0], h->partials, (const CgState*)h->st);
-        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
                            (const double*)h->partials2, nwg_it);
         // the rest of the V-cycle on the new residual; its last level-0 pass reduces r.z as well
         enqueue_vcycle(h, h->partials, rout, true);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg_v0);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials, nwg_v0);
         return;
     }
     const unsigned gs = rgrid((size_t)nv);
@@ -869,12 +869,12 @@ template <int B> void enqueue_iteration(vare_flip* h, int k) {
     hipLaunchKernelGGL(k_cg_update, dim3(gs), dim3(WG), 0, h->stream, nv, h->clist(), v, h->partials, h->partials2,
                        (const CgState*)h->st);
     tend(h, VARE_KERNEL_UPDATE);
-    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials,
+    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
                        (const double*)h->partials2, (int)gs);
     if (mg) {
         // the V-cycle's last level-0 pass reduces r.z as well (one partial per workgroup)
         enqueue_vcycle(h, h->partials);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, nwg_v0);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials, nwg_v0);
     }
 }
 
@@ -923,7 +923,7 @@ template <int B> int stage_solve(vare_flip* h) {
             hipLaunchKernelGGL(k_ir_residual<B>, dim3(gv), dim3(WG), 0, h->stream, nv, h->clist(), h->frame(),
                                (const float*)h->b, (const double*)h->x64, (const uint16_t*)h->info, h->r, h->x, h->p,
                                h->partials);
-            hipLaunchKernelGGL(k_fin_ir, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)gv,
+            hipLaunchKernelGGL(k_fin_ir, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials, (int)gv,
                                h->prm.refine_tol, h->prm.cg_rel_tol);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
@@ -940,7 +940,7 @@ template <int B> int stage_solve(vare_flip* h) {
     if (!refine) {
         hipLaunchKernelGGL((k_true_residual<B, float>), dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
                            (const float*)h->b, (const float*)h->x, (const uint16_t*)h->info, h->partials);
-        hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(WG), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+        hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials, (int)g);
     }
     HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));

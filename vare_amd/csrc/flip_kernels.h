@@ -380,21 +380,37 @@ __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type,
 
 // Deterministic sum of n partials by one 256-thread workgroup.  Each thread keeps 8 running sums over
 // strided partials, so 8 loads are in flight per thread, and combines them in a fixed order.
-__device__ __forceinline__ double fin_sum(const double* __restrict__ partials, int n, double* sh) {
+// The finalize kernels run as one workgroup of kFinThreads threads: each thread sums its partials in
+// rounds of 8 loads issued together (kFinThreads * 8 partials per round, one round for up to 8192
+// producer workgroups), then the waves' sums are added in a fixed order.  The order of every addition
+// is fixed, so the result is deterministic.
+constexpr int kFinThreads = 1024;
+constexpr int kFinWaves = kFinThreads / 64;
+__device__ __forceinline__ double fin_sum(const double* __restrict__ partials, int n, double* sh /*[kFinWaves]*/) {
+    constexpr int T = kFinThreads;
     double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int q = threadIdx.x;
-    for (; q + 7 * WG < n; q += 8 * WG) {
+    for (int base = 0; base < n; base += 8 * T) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] += partials[q + j * WG];
+        for (int j = 0; j < 8; ++j) {
+            const int q = base + j * T + threadIdx.x;
+            a[j] += q < n ? partials[q] : 0.0;
+        }
     }
-    for (int j = 0; q < n; q += WG, ++j) a[j] += partials[q];
-    const double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-    return block_sum(s, sh);
+    double v = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int q = 0; q < kFinWaves; ++q) r += sh[q];
+    __syncthreads();
+    return r;
 }
 
-__global__ void k_fin_divergence(CgState* __restrict__ st, const double* __restrict__ partials,
+__global__ void __launch_bounds__(kFinThreads) k_fin_divergence(CgState* __restrict__ st, const double* __restrict__ partials,
                                  const double* __restrict__ partials2, int n) {
-    __shared__ double sh[4];
+    __shared__ double sh[kFinWaves];
     const double tot = fin_sum(partials, n, sh);
     const double tot2 = fin_sum(partials2, n, sh);
     if (threadIdx.x != 0) return;
@@ -444,8 +460,8 @@ __global__ void k_cg_precond_init(int nv, CellList cl, CgVecs v, double* __restr
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-__global__ void k_fin_precond_init(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
-    __shared__ double sh[4];
+__global__ void __launch_bounds__(kFinThreads) k_fin_precond_init(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
+    __shared__ double sh[kFinWaves];
     if (st->done) return;
     const double tot = fin_sum(partials, n, sh);
     if (threadIdx.x != 0) return;
@@ -454,8 +470,8 @@ __global__ void k_fin_precond_init(CgState* __restrict__ st, const double* __res
 }
 
 // alpha = rsold / p.Ap  (LinearSystemSolver.cpp:31); the partials are the products of k_spmv_blk.
-__global__ void k_fin_alpha(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
-    __shared__ double sh[4];
+__global__ void __launch_bounds__(kFinThreads) k_fin_alpha(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
+    __shared__ double sh[kFinWaves];
     if (st->done) return;
     const double tot = fin_sum(partials, n, sh);
     if (threadIdx.x != 0) return;
@@ -488,9 +504,9 @@ __global__ void k_cg_update(int nv, CellList cl, CgVecs v, double* __restrict__ 
 
 // Convergence test and beta (LinearSystemSolver.cpp:37-46).  For multigrid, beta is computed after
 // the V-cycle by k_fin_beta.
-__global__ void k_fin_update(CgState* __restrict__ st, const double* __restrict__ partials,
+__global__ void __launch_bounds__(kFinThreads) k_fin_update(CgState* __restrict__ st, const double* __restrict__ partials,
                              const double* __restrict__ partials2, int n) {
-    __shared__ double sh[4];
+    __shared__ double sh[kFinWaves];
     if (st->done) return;
     const double tot = fin_sum(partials, n, sh);
     const double tot2 = st->precond == 1 ? fin_sum(partials2, n, sh) : 0.0;
@@ -522,8 +538,8 @@ __global__ void k_fin_update(CgState* __restrict__ st, const double* __restrict_
 }
 
 // Multigrid: beta from the r.z partials written by the V-cycle's last level-0 pass.
-__global__ void k_fin_beta(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
-    __shared__ double sh[4];
+__global__ void __launch_bounds__(kFinThreads) k_fin_beta(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
+    __shared__ double sh[kFinWaves];
     if (st->done) return;
     const double tot = fin_sum(partials, n, sh);
     if (threadIdx.x != 0) return;
@@ -791,9 +807,9 @@ namespace vare {
 // on the new right-hand side (its relative tolerance is measured against |r|).
 // The next correction solve only has to close the gap to the target: its relative tolerance is
 // max(inner_tol, 0.5 * outer_tol * |b| / |r|), at most 0.5.
-__global__ void k_fin_ir(CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol,
+__global__ void __launch_bounds__(kFinThreads) k_fin_ir(CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol,
                          float inner_tol) {
-    __shared__ double sh[4];
+    __shared__ double sh[kFinWaves];
     const double tot = fin_sum(partials, n, sh);
     if (threadIdx.x != 0) return;
     st->ir_rr = tot;
@@ -835,8 +851,8 @@ __global__ void k_true_residual(int nc, FrameK f, const float* __restrict__ b, c
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-__global__ void k_fin_true(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
-    __shared__ double sh[4];
+__global__ void __launch_bounds__(kFinThreads) k_fin_true(CgState* __restrict__ st, const double* __restrict__ partials, int n) {
+    __shared__ double sh[kFinWaves];
     const double tot = fin_sum(partials, n, sh);
     if (threadIdx.x == 0) st->true_rr = tot;
 }
