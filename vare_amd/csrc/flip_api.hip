@@ -920,11 +920,13 @@ template <int B> int stage_solve(vare_flip* h) {
             hipLaunchKernelGGL(k_ir_accumulate, dim3(gv), dim3(WG), 0, h->stream, nv, h->clist(), h->x64,
                                (const float*)h->x, h->x, 0);
             ++outer;
-            hipLaunchKernelGGL(k_ir_residual<B>, dim3(gv), dim3(WG), 0, h->stream, nv, h->clist(), h->frame(),
-                               (const float*)h->b, (const double*)h->x64, (const uint16_t*)h->info, h->r, h->x, h->p,
-                               h->partials);
-            hipLaunchKernelGGL(k_fin_ir, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials, (int)gv,
-                               h->prm.refine_tol, h->prm.cg_rel_tol);
+            const int nwg_ir = nwg_k(h, kIterK);
+            if (nwg_ir)
+                hipLaunchKernelGGL((k_ir_residual_blk<B, kIterK>), dim3(nwg_ir), dim3(B * B * B), 0, h->stream,
+                                   (const int*)h->wgtab, (const float*)h->b, (const double*)h->x64,
+                                   (const uint16_t*)h->info, h->r, h->x, h->p, h->partials);
+            hipLaunchKernelGGL(k_fin_ir, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
+                               nwg_ir, h->prm.refine_tol, h->prm.cg_rel_tol);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
@@ -935,7 +937,7 @@ template <int B> int stage_solve(vare_flip* h) {
         }
     }
     // FP64 recompute of the final residual from the pressure the gradient will use, reported with
-    // every solve.  When refining, the last k_ir_residual computed it from the final FP64 pressure
+    // every solve.  When refining, the last k_ir_residual_blk computed it from the final FP64 pressure
     // (k_fin_ir stored it); otherwise it is recomputed from the FP32 pressure.
     if (!refine) {
         hipLaunchKernelGGL((k_true_residual<B, float>), dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),

@@ -756,35 +756,9 @@ namespace vare {
 
 // ------------------------------------------------------------------------------------------------
 // Mixed-precision projection: iterative refinement.  The pressure accumulates in FP64 (x64); each
-// outer step recomputes the residual r = b - A x64 in FP64, hands its FP32 rounding to the FP32
-// PCG as the right-hand side of a correction solve A d = r (d starts at 0), then x64 += d.
-template <int B>
-__global__ void k_ir_residual(int nv, CellList cl, FrameK f, const float* __restrict__ b, const double* __restrict__ x64,
-                              const uint16_t* __restrict__ info, float* __restrict__ r32, float* __restrict__ x,
-                              float* __restrict__ p, double* __restrict__ partials) {
-    __shared__ double sh[4];
-    double rr = 0.0;
-    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
-        const int c = cl.cell(vi);
-        const uint16_t in = info[c];
-        double r = 0.0;
-        if (in & kInfoFluid) {
-            double ax = (double)((in >> 8) & 0x1f) * x64[c];
-#pragma unroll
-            for (int q = 0; q < 6; ++q)
-                if (in & (1u << q)) ax -= x64[cell_neighbor<B>(f, c, q)];
-            r = (double)b[c] - ax;
-        }
-        const float rf = (float)r;
-        r32[c] = rf;
-        x[c] = 0.f;
-        p[c] = rf;
-        rr += r * r;
-    }
-    const double s = block_sum(rr, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
-
+// outer step recomputes the residual r = b - A x64 in FP64 (k_ir_residual_blk), hands its FP32
+// rounding to the FP32 PCG as the right-hand side of a correction solve A d = r (d starts at 0), then
+// x64 += d.
 __global__ void k_ir_accumulate(int nv, CellList cl, double* __restrict__ x64, const float* __restrict__ d,
                                 float* __restrict__ x32, int last) {
     for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
@@ -803,7 +777,7 @@ __global__ void k_ir_round(int nc, const double* __restrict__ x64, float* __rest
 
 namespace vare {
 
-// Refinement bookkeeping after k_ir_residual: record the FP64 residual and restart the inner solve
+// Refinement bookkeeping after k_ir_residual_blk: record the FP64 residual and restart the inner solve
 // on the new right-hand side (its relative tolerance is measured against |r|).
 // The next correction solve only has to close the gap to the target: its relative tolerance is
 // max(inner_tol, 0.5 * outer_tol * |b| / |r|), at most 0.5.

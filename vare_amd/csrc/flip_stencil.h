@@ -135,6 +135,77 @@ __device__ __forceinline__ float apply_row(unsigned in, float own, const float* 
     return az;
 }
 
+// Iterative refinement's FP64 residual r = b - A x64 over the solver's blocks (diag * x first, then
+// the fluid neighbours subtracted in the order -x,+x,-y,+y,-z,+z), handing the FP32 rounding of r to
+// the next correction solve as its right-hand side and first direction (x = 0, p = r), with one
+// partial |r|^2 per workgroup.
+// Not gated by the solver flag: it runs between solves.
+template <int B, int K>
+__global__ void __launch_bounds__(Fine<B>::B3) k_ir_residual_blk(const int* __restrict__ tab,
+                                                                const float* __restrict__ b,
+                                                                const double* __restrict__ x64,
+                                                                const uint16_t* __restrict__ info,
+                                                                float* __restrict__ r32, float* __restrict__ x,
+                                                                float* __restrict__ p, double* __restrict__ partials) {
+    constexpr int B3 = Fine<B>::B3;
+    __shared__ double sx[K][B3];
+    __shared__ double sh[Fine<B>::NW];
+    int s[K], nb[K][6];
+    wg_load<K>(tab, s, nb);
+    const int l = threadIdx.x;
+    int side[3], fl[3];
+    thread_faces<B>(l, side, fl);
+    unsigned in[K];
+    float bc[K];
+    double xc[K], hx[K][3];
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        in[kb] = 0;
+        bc[kb] = 0.f;
+        xc[kb] = 0.0;
+        if (s[kb] >= 0) {
+            const int c = s[kb] * B3 + l;
+            in[kb] = info[c];
+            bc[kb] = b[c];
+            xc[kb] = x64[c];
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int ns = face_slot(nb[kb], a, side[a]);
+            hx[kb][a] = ns >= 0 ? x64[ns * B3 + fl[a]] : 0.0;
+        }
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) sx[kb][l] = xc[kb];
+    __syncthreads();
+    double rr = 0.0;
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        if (s[kb] < 0) continue;
+        double r = 0.0;
+        if (in[kb] & kInfoFluid) {
+            double ax = (double)((in[kb] >> 8) & 0x1f) * xc[kb];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (!(in[kb] & (1u << q))) continue;
+                const int a = q >> 1;
+                ax -= side[a] == (q & 1) ? hx[kb][a] : sx[kb][inner_nbr<B>(l, q)];
+            }
+            r = (double)bc[kb] - ax;
+        }
+        const float rf = (float)r;
+        const int c = s[kb] * B3 + l;
+        r32[c] = rf;
+        x[c] = 0.f;
+        p[c] = rf;
+        rr += r * r;
+    }
+    const double tot = fine_block_sum<B>(rr, sh);
+    if (l == 0) partials[blockIdx.x] = tot;
+}
+
 // Ap = A p and partial p.Ap (BLAS::mul, BLAS.cu:12-40).
 template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict__ tab, CgVecs v,
