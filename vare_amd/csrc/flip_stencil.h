@@ -266,7 +266,7 @@ __device__ __forceinline__ float pdir(float beta, unsigned in, float r, float z,
     return zz + (beta * p);
 }
 
-// p = z + beta p (k_cg_pupdate) fused with Ap = A p and partial p.Ap (k_spmv_blk) for every CG
+// p = z + beta p (the direction update) fused with Ap = A p and partial p.Ap (k_spmv_blk) for every CG
 // iteration after the first of a solve.  The new p across a face is recomputed from the neighbour's
 // z (or r) and old p, exactly as the neighbour's own workgroup computes it, so no pass over p is
 // needed between the two.  The old p is read from v.p and the new one written to pout, a second
