@@ -4,6 +4,7 @@
 // the reference's FLIPMethod::advanceOneStep (base/source/FLIPMethod.cu:208-298) split into six
 // stages; each stage enqueues hand-written kernels from flip_kernels.h on the handle's stream.
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include <hip/hip_ext.h>
 #include <climits>
 #include <cmath>
@@ -514,7 +515,8 @@ template <int B> int stage_p2g(vare_flip* h) {
 CgVecs vecs(vare_flip* h) { return CgVecs{h->x, h->r, h->p, h->ap, h->b, h->z, h->info}; }
 
 int mg_setup(vare_flip* h);
-void enqueue_vcycle(vare_flip* h, double* rz = nullptr, float* r0 = nullptr, bool skip_first = false);
+void enqueue_vcycle(vare_flip* h, double* rz = nullptr, float* r0 = nullptr, bool skip_first = false,
+                    XTerm xt = XTerm{});
 
 // z = M^-1 r (multigrid: already in z), p = z and r.z over the solver's blocks.
 void enqueue_precond_init(vare_flip* h) {
@@ -730,7 +732,8 @@ template <int KIND> void enqueue_level_up(vare_flip* h, int l, int nu) {
 // per-workgroup partials of r.z (one per level-0 workgroup).
 // r0 != nullptr: the level-0 right-hand side (the CG residual) is read from r0 instead of h->r;
 // skip_first: the first level-0 pass is already done (see enqueue_block_down).
-template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool skip_first) {
+// xt.x != nullptr: the extra workgroups of k_mg_bottom apply the CG's x += alpha p (see XTerm).
+template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool skip_first, XTerm xt) {
     float* const r0_saved = h->mg[0].r;
     if (r0) h->mg[0].r = r0;
     const MgFrame F = mg_frame(h);
@@ -781,8 +784,13 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
     }
     bt.r0 = h->mg[lb].r;
     bt.z0 = h->mg_z[lb];
-    hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(kMgBottomThreads), 0, h->stream, bt, w, nu, h->prm.mg_coarse_sweeps,
-                       done);
+    unsigned nxwg = 0;
+    if (xt.x && xt.nblk > 0) {
+        const int per = (kMgBottomThreads >> xt.sh3) * kXUpdPasses;     // blocks per x-update workgroup
+        nxwg = (unsigned)((xt.nblk + per - 1) / per);
+    }
+    hipLaunchKernelGGL(k_mg_bottom, dim3(1 + nxwg), dim3(kMgBottomThreads), 0, h->stream, bt, w, nu,
+                       h->prm.mg_coarse_sweeps, done, xt);
     for (int l = lb - 1; l >= 0; --l) {
         const MgLevelK& L = h->mg[l];
         if (!L.ncell) continue;
@@ -800,9 +808,9 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
     h->mg[0].r = r0_saved;
 }
 
-void enqueue_vcycle(vare_flip* h, double* rz, float* r0, bool skip_first) {
-    if (h->B == 4) enqueue_vcycle_t<4>(h, rz, r0, skip_first);
-    else enqueue_vcycle_t<8>(h, rz, r0, skip_first);
+void enqueue_vcycle(vare_flip* h, double* rz, float* r0, bool skip_first, XTerm xt) {
+    if (h->B == 4) enqueue_vcycle_t<4>(h, rz, r0, skip_first, xt);
+    else enqueue_vcycle_t<8>(h, rz, r0, skip_first, xt);
 }
 
 // One CG/PCG iteration (LinearSystemSolver.cpp:29-46), iteration k of the current solve, over the
@@ -860,7 +868,10 @@ template <int B> void enqueue_iteration(vare_flip* h, int k) {
         hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
                            (const double*)h->partials2, nwg_it);
         // the rest of the V-cycle on the new residual; its last level-0 pass reduces r.z as well
-        enqueue_vcycle(h, h->partials, rout, true);
+        // the V-cycle's bottom launch also applies this iteration's x += alpha p (p_k, alpha_k)
+        const XTerm xt{h->x, pk, (const float*)((const char*)h->st + offsetof(CgState, alpha)), h->blist,
+                       B == 8 ? 9 : 6, h->ns};
+        enqueue_vcycle(h, h->partials, rout, true, xt);
         hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials, nwg_v0);
         return;
     }
@@ -912,13 +923,18 @@ template <int B> int stage_solve(vare_flip* h) {
     const int nv = h->nvirt();
     const unsigned gv = rgrid((size_t)nv);
     if (refine) CHK(fill_u32(h, h->x64, 0, 2 * nc));
+    // the fused multigrid iteration leaves the last iteration's x += alpha p to after the solve (XTail)
+    const bool fused = mode == 1 && h->prm.preconditioner == VARE_PRECOND_MG && h->prm.mg_sweeps == 2;
+    const XTail tail{fused ? (const float*)h->p : nullptr, (const float*)h->p2, (const CgState*)h->st};
     CHK(run_solver<B>(h, cap, 0, total_iters));
+    if (!refine && fused && nv)
+        hipLaunchKernelGGL(k_cg_xflush, dim3(gv), dim3(WG), 0, h->stream, nv, h->clist(), h->x, tail);
     if (refine) {
         // FP64 iterative refinement: x64 += d, r = b - A x64 in FP64, solve A d = r in FP32
         for (;;) {
             // over the solver's blocks only: the other blocks hold no fluid cell, and x and x64 stay 0 there
             hipLaunchKernelGGL(k_ir_accumulate, dim3(gv), dim3(WG), 0, h->stream, nv, h->clist(), h->x64,
-                               (const float*)h->x, h->x, 0);
+                               (const float*)h->x, h->x, 0, tail);
             ++outer;
             const int nwg_ir = nwg_k(h, kIterK);
             if (nwg_ir)

@@ -759,13 +759,39 @@ namespace vare {
 // outer step recomputes the residual r = b - A x64 in FP64 (k_ir_residual_blk), hands its FP32
 // rounding to the FP32 PCG as the right-hand side of a correction solve A d = r (d starts at 0), then
 // x64 += d.
+// The fused multigrid CG iteration leaves x += alpha p to the extra workgroups of its V-cycle's bottom
+// launch, which does not run in the iteration that ends the solve; that last term is added here, after
+// the solve: k = st->iter - 1 is the last iteration, alpha_k the state's alpha and p_k the direction
+// buffer of its parity (p0 for even k).  p0 == nullptr: x is complete.
+struct XTail {
+    const float* p0;
+    const float* p1;
+    const CgState* st;
+    __device__ __forceinline__ float apply(float x, int c) const {
+        if (!p0) return x;
+        const int it = st->iter;
+        if (it <= 0) return x;
+        const float* pk = (it & 1) ? p0 : p1;
+        return x + (st->alpha * pk[c]);
+    }
+};
+
+// x64 += d, with d the correction solve's x (its last CG term added, see XTail).
 __global__ void k_ir_accumulate(int nv, CellList cl, double* __restrict__ x64, const float* __restrict__ d,
-                                float* __restrict__ x32, int last) {
+                                float* __restrict__ x32, int last, XTail tail) {
     for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
         const int c = cl.cell(vi);
-        const double v = x64[c] + (double)d[c];
+        const double v = x64[c] + (double)tail.apply(d[c], c);
         x64[c] = v;
         if (last) x32[c] = (float)v;
+    }
+}
+
+// x = x + alpha_k p_k of the last iteration (XTail) when the solve is not refined.
+__global__ void k_cg_xflush(int nv, CellList cl, float* __restrict__ x, XTail tail) {
+    for (int vi = blockIdx.x * WG + threadIdx.x; vi < nv; vi += gridDim.x * WG) {
+        const int c = cl.cell(vi);
+        x[c] = tail.apply(x[c], c);
     }
 }
 

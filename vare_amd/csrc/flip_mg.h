@@ -528,6 +528,20 @@ struct MgBottom {
     float* z0;                                         // its correction (output)
 };
 
+// The CG's x += alpha p over the solver's blocks (block list blk, 2^sh3 cells each), applied by the
+// extra workgroups of k_mg_bottom; x == nullptr: none.  A workgroup takes kXUpdPasses consecutive
+// groups of blocks, one block per 2^sh3 threads (whole waves, so a block's slot is a uniform scalar
+// load), and issues all of its x and p loads at once.
+struct XTerm {
+    float* x;
+    const float* p;                                    // this iteration's direction
+    const float* alpha;                                // this iteration's alpha (solver state)
+    const int* blk;
+    int sh3;                                           // log2 of the cells per block (>= 6)
+    int nblk;                                          // listed blocks
+};
+constexpr int kXUpdPasses = 16;
+
 // Storage index of dense-view cell c of bottom level l (-1: inactive block).
 __device__ __forceinline__ int bottom_index(const MgBottom& bt, int l, int c) {
     const int sh = bt.sh[l];
@@ -568,8 +582,10 @@ __device__ __forceinline__ float bottom_sweep(unsigned in, float r, const float*
     return mg_fluid(in) ? z[c] + (r - bottom_az(in, z, c, sy, sz)) * mg_wd(in, omega) : 0.f;
 }
 
+// Workgroup 0 runs the bottom V-cycle; workgroups 1.. (when xt.x is set) apply the CG's x += alpha p
+// meanwhile (see XTerm).
 __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, float omega, int nu, int coarse_sweeps,
-                                                                const int* done) {
+                                                                const int* done, XTerm xt) {
     if (*done) return;
     __shared__ float sr[kMgBottomCells];
     __shared__ float sz[kMgBottomCells];
@@ -577,6 +593,29 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
     __shared__ uint16_t si[kMgBottomCells];
     const int tid = threadIdx.x;
     constexpr int T = kMgBottomThreads;
+    if (blockIdx.x > 0) {
+        const float a = *xt.alpha;
+        const int bpp = T >> xt.sh3;                   // blocks per pass
+        const int l = tid & ((1 << xt.sh3) - 1);
+        const int b0 = (blockIdx.x - 1) * bpp * kXUpdPasses + (tid >> xt.sh3);
+        int c[kXUpdPasses];
+        float xv[kXUpdPasses], pv[kXUpdPasses];
+#pragma unroll
+        for (int u = 0; u < kXUpdPasses; ++u) {
+            const int b = __builtin_amdgcn_readfirstlane(b0 + u * bpp);     // uniform: whole waves per block
+            c[u] = b < xt.nblk ? (xt.blk[b] << xt.sh3) | l : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kXUpdPasses; ++u) {
+            const int cs = c[u] >= 0 ? c[u] : 0;
+            xv[u] = xt.x[cs];
+            pv[u] = xt.p[cs];
+        }
+#pragma unroll
+        for (int u = 0; u < kXUpdPasses; ++u)
+            if (c[u] >= 0) xt.x[c[u]] = xv[u] + (a * pv[u]);
+        return;
+    }
     constexpr int kPer = (kMgBottomCells + T - 1) / T;      // cells per thread of any one level, at most
     const int L = bt.nlev;
     // load: a level has at most kPer cells per thread; their indices, then all of their descriptors

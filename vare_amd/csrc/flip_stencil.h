@@ -409,12 +409,16 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __
     }
 }
 
-// x += alpha p, r -= alpha Ap and partial r.r (k_cg_update, LinearSystemSolver.cpp:33-35) fused with
-// the V-cycle's first level-0 pass (two Jacobi sweeps from zero on the new r, k_mg0_smooth_first2),
-// for multigrid with two sweeps.  The new r across a face is recomputed from the neighbour's r and
-// Ap, exactly as the neighbour's own workgroup computes it.  The old r is read from v.r and the new
-// one written to rout, a second buffer, because other workgroups still read the old values across
-// their faces; x is read and written only at the workgroup's own cells and is updated in place.
+// r -= alpha Ap and partial r.r (k_cg_update, LinearSystemSolver.cpp:34-35) fused with the V-cycle's
+// first level-0 pass (two Jacobi sweeps from zero on the new r, k_mg0_smooth_first2), for multigrid
+// with two sweeps.  The new r across a face is recomputed from the neighbour's r and Ap, exactly as
+// the neighbour's own workgroup computes it.  The old r is read from v.r and the new one written to
+// rout, a second buffer, because other workgroups still read the old values across their faces.
+// x += alpha p (LinearSystemSolver.cpp:33) is not done here: nothing reads x before the solve ends, so
+// the extra workgroups of the same iteration's k_mg_bottom apply it while the bottom workgroup runs on
+// an otherwise idle GPU, and the last iteration's term, whose V-cycle does not run once the solver is
+// done, is applied after the solve (k_ir_accumulate, k_cg_xflush); the additions happen in the same
+// order, so x is bit-identical.
 template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_update_smooth2(const int* __restrict__ tab, CgVecs v,
                                                                float* __restrict__ rout, float omega,
@@ -433,16 +437,14 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_update_smooth2(const int* __res
     int side[3], fl[3];
     thread_faces<B>(l, side, fl);
     unsigned in[K], hi[K][3];
-    float xo[K], po[K], ro[K], ao[K], hr[K][3], ha[K][3];
+    float ro[K], ao[K], hr[K][3], ha[K][3];
 #pragma unroll
     for (int kb = 0; kb < K; ++kb) {
         in[kb] = 0;
-        xo[kb] = po[kb] = ro[kb] = ao[kb] = 0.f;
+        ro[kb] = ao[kb] = 0.f;
         if (s[kb] >= 0) {
             const int c = s[kb] * B3 + l;
             in[kb] = v.info[c];
-            xo[kb] = v.x[c];
-            po[kb] = v.p[c];
             ro[kb] = v.r[c];
             ao[kb] = v.ap[c];
         }
@@ -483,7 +485,6 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_update_smooth2(const int* __res
             out = z1[kb] + (rn[kb] - az) * mg_wd(in[kb], omega);
         }
         const int c = s[kb] * B3 + l;
-        v.x[c] = xo[kb] + (alpha * po[kb]);
         rout[c] = rn[kb];
         zout[c] = out;
     }
