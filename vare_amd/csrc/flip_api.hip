@@ -489,11 +489,18 @@ template <int B> int stage_p2g(vare_flip* h) {
     if (h->prm.particle_path == 0) {
         // gather form: every cell of every active block is written, normalised and classified
         tbegin(h, VARE_KERNEL_P2G);
-        if (h->nb)
-            hipLaunchKernelGGL(k_p2g_gather<B>, dim3(h->nb), dim3(P2GGather<B>::THREADS), 0, h->stream,
-                               (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
-                               (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type,
-                               h->colk(), h->prm.col_vel_scale);
+        if (h->nb) {
+            if (h->vhalf)
+                hipLaunchKernelGGL((k_p2g_gather<B, true>), dim3(h->nb), dim3(P2GGather<B>::THREADS), 0, h->stream,
+                                   (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
+                                   (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type,
+                                   h->colk(), h->prm.col_vel_scale);
+            else
+                hipLaunchKernelGGL((k_p2g_gather<B, false>), dim3(h->nb), dim3(P2GGather<B>::THREADS), 0, h->stream,
+                                   (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
+                                   (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type,
+                                   h->colk(), h->prm.col_vel_scale);
+        }
         tend(h, VARE_KERNEL_P2G);
         HIPCHK(hipGetLastError());
         return VARE_OK;
@@ -997,13 +1004,23 @@ template <int B> int stage_g2p(vare_flip* h, float dt) {
     tbegin(h, VARE_KERNEL_G2P);
     if (h->np && h->prm.particle_path == 0) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
-        if (h->colk().any())
-            hipLaunchKernelGGL((k_g2p_tile<B, true>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1],
-                               h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
+        const bool col = h->colk().any();
+        // collision queries and the velocity element type are compile-time choices of the kernel
+        if (col && h->vhalf)
+            hipLaunchKernelGGL((k_g2p_tile<B, true, true>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0],
+                               h->pa[1], h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
+                               g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
+        else if (col)
+            hipLaunchKernelGGL((k_g2p_tile<B, true, false>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0],
+                               h->pa[1], h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
+                               g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
+        else if (h->vhalf)
+            hipLaunchKernelGGL((k_g2p_tile<B, false, true>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0],
+                               h->pa[1], h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
                                g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
         else
-            hipLaunchKernelGGL((k_g2p_tile<B, false>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0], h->pa[1],
-                               h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
+            hipLaunchKernelGGL((k_g2p_tile<B, false, false>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0],
+                               h->pa[1], h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
                                g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
     } else if (h->np) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};

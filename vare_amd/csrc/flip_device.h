@@ -57,6 +57,21 @@ struct VelPtr {
     }
 };
 
+// The same arrays with the element type fixed at compile time (H: FP16), for the hot particle
+// kernels: with no run-time branch per load, every load of a particle issues before any conversion.
+template <bool H> struct VelT {
+    void* p[3];
+    __device__ __forceinline__ float ld(int a, int i) const {
+        if constexpr (H) return (float)((const _Float16*)p[a])[i];
+        else return ((const float*)p[a])[i];
+    }
+    __device__ __forceinline__ void st(int a, int i, float v) const {
+        if constexpr (H) ((_Float16*)p[a])[i] = (_Float16)v;
+        else ((float*)p[a])[i] = v;
+    }
+};
+template <bool H> __device__ __forceinline__ VelT<H> vel_typed(const VelPtr& u) { return VelT<H>{{u.p[0], u.p[1], u.p[2]}}; }
+
 // Frame seen by kernels.
 struct FrameK {
     int fx, fy, fz;

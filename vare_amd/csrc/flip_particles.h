@@ -350,14 +350,16 @@ template <int B> struct P2GGather {
 // cells, so the LDS updates need no atomics and every cell sums its terms in a fixed order -- the
 // result is bit-reproducible.  A corner past n-1 is skipped (the Min(c+1, n-1) bound); corners in
 // other blocks are gathered by those blocks' workgroups, and corners in inactive blocks are dropped
-// (isWritable).  Every cell of the block is written, so no zero fill is needed.
-template <int B>
-__global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
-        const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, VelPtr u,
+// (isWritable).  Every cell of the block is written, so no zero fill is needed.  At 6 waves per SIMD
+// (at most 80 registers) two 768-thread workgroups fit on a CU.
+template <int B, bool VH>
+__global__ void __launch_bounds__(P2GGather<B>::THREADS) __attribute__((amdgpu_waves_per_eu(6))) k_p2g_gather(
+        const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, VelPtr uv,
         const int* __restrict__ cstart, FrameK f, CellFields cf,
         float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz, uint8_t* __restrict__ type, ColK col,
         float col_scale) {
     constexpr int B1 = B + 1, NBASE = B1 * B1 * B1, B3 = B * B * B;
+    const VelT<VH> u = vel_typed<VH>(uv);
     __shared__ float tm[B3], tx[B3], ty[B3], tz[B3], tc[B3];
     __shared__ int nsl[27];
     const int s = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -437,9 +439,9 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) k_p2g_gather(
 // G2P for one particle given a cell-value fetcher (FLIPMethod.cu:86-171, SparseField.h:150-196).
 // COL = false compiles the collision queries out (no collision source is active), which leaves the
 // kernel with fewer registers and so more waves in flight.
-template <int B, bool COL, class Fetch>
+template <int B, bool COL, class Vel, class Fetch>
 __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
-                                            const VelPtr& u, float* __restrict__ adj, const FrameK& f, const G2PParams& gp,
+                                            const Vel& u, float* __restrict__ adj, const FrameK& f, const G2PParams& gp,
                                             const ColK& col, uint8_t* __restrict__ removed, Fetch fetch) {
     float x = px[n], y = py[n], z = pz[n];
     float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
@@ -554,13 +556,14 @@ struct G2PFields {
 // tile of the 7 gathered fields over the block and a 1-cell halo, which holds every stencil
 // i = clamp((int)(p-0.5), 0, n-2) .. i+1 of those particles.  Workgroup nb takes bin Nc (particles
 // outside the domain) and reads the fields through the block table.
-template <int B, bool COL>
+template <int B, bool COL, bool VH>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(COL ? 4 : 5))) k_g2p_tile(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
-                                                VelPtr u, float* __restrict__ adj,
+                                                VelPtr uv, float* __restrict__ adj,
                                                 const int* __restrict__ cstart, int nb, FrameK f, G2PFields gf,
                                                 G2PParams gp, ColK col, uint8_t* __restrict__ removed,
                                                 int* __restrict__ nremoved) {
     constexpr int T = B + 2, T3 = T * T * T, B3 = B * B * B;
+    const VelT<VH> u = vel_typed<VH>(uv);
     __shared__ float tile[7][T3];
     __shared__ int nsl[27];
     const int s = xcd_swizzle(blockIdx.x, gridDim.x);
