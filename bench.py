@@ -221,10 +221,11 @@ def run_scene(a, L, cfg, steps):
                     "traffic": round(traffic_per_cell * cells / launches, 1) if traffic_per_cell else None,
                     "avg_launch_us": round(ms * 1e3 / launches, 2), "launches": launches,
                     "bytes_per_launch": round(bytes_total / launches, 1)}
-        # dominant kernel (the most time per step): the CG's x, r update fused with the V-cycle's first
-        # level-0 pass.  Algorithmic bytes per solver cell: read x, p, r, Ap 16 + descriptor 2, write
-        # x, r, z 12.  Neighbour values across a block face are re-reads.
-        roofline = line("k_update_smooth2", "update", 30.0, UPD_TRAFFIC_PER_CELL)
+        # dominant kernel (the most time per step): the CG's r update fused with the V-cycle's first
+        # level-0 pass (x += alpha p runs in the same iteration's bottom V-cycle launch).  Algorithmic
+        # bytes per solver cell: read r, Ap 8 + descriptor 2, write r, z 8.  Neighbour values across a
+        # block face are re-reads.
+        roofline = line("k_update_smooth2", "update", 18.0, UPD_TRAFFIC_PER_CELL)
         roofline["timed_on"] = sampled
         roofline["others"] = [
             # the level-0 prolongation + Jacobi pass: read z 4 + coarse z 0.5 + r 4 + descriptor 2, write z 4
