@@ -40,9 +40,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per processed cell of one launch, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # on this benchmark (20 steps) and calibrated against plain 1 GiB copies (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0); profiles/hbm_traffic_dam_break_256.md.
-PSPMV_TRAFFIC_PER_CELL = 17.11    # k_pupdate_spmv: (2 x 22307.2 + 32056.6) KiB per 4.590 M cells
-UPD_TRAFFIC_PER_CELL = 29.13     # k_update_smooth2: (2 x 40954.8 + 48564.0) KiB per 4.587 M cells
-MG0_TRAFFIC_PER_CELL = 12.51      # k_mg0_prolong_smooth: (2 x 20662.4 + 14684.8) KiB per 4.585 M cells
+PSPMV_TRAFFIC_PER_CELL = 17.10    # k_pupdate_spmv: (2 x 22302.6 + 32056.6) KiB per 4.590 M cells
+UPD_TRAFFIC_PER_CELL = 17.60     # k_update_smooth2: (2 x 23206.2 + 32417.6) KiB per 4.587 M cells
+MG0_TRAFFIC_PER_CELL = 12.51      # k_mg0_prolong_smooth: (2 x 20663.5 + 14684.8) KiB per 4.585 M cells
 TIMING_STRIDE = 4                  # the dominant kernel is timed on every 4th step of the timed region
 OTHER_TIMING_STEPS = 10            # steps after the timed region that time the two secondary kernels
 DT = 1.0 / 240.0
