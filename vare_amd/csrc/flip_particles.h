@@ -582,10 +582,26 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(COL ? 4
     } else {
         load_nbhd(f, s, nsl);
         __syncthreads();
-        for (int t = threadIdx.x; t < T3; t += WG) {
-            const int c = tile_cell<B>(nsl, t % T, (t / T) % T, t / (T * T));
+        // the tile's T^3 cells, R per thread: every load of a thread is issued before any is
+        // stored to LDS (stand-in cell 0 where the cell's block is inactive)
+        constexpr int R = (T3 + WG - 1) / WG;
+        float v[R][7];
+        int cl[R];
 #pragma unroll
-            for (int fi = 0; fi < 7; ++fi) tile[fi][t] = c >= 0 ? src[fi][c] : (fi == 0 ? 10.f : 0.f);
+        for (int u = 0; u < R; ++u) {
+            const int t = threadIdx.x + u * WG;
+            cl[u] = t < T3 ? tile_cell<B>(nsl, t % T, (t / T) % T, t / (T * T)) : -1;
+            const int cs = cl[u] >= 0 ? cl[u] : 0;
+#pragma unroll
+            for (int fi = 0; fi < 7; ++fi) v[u][fi] = src[fi][cs];
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int t = threadIdx.x + u * WG;
+            if (t < T3) {
+#pragma unroll
+                for (int fi = 0; fi < 7; ++fi) tile[fi][t] = cl[u] >= 0 ? v[u][fi] : (fi == 0 ? 10.f : 0.f);
+            }
         }
         __syncthreads();
         const int t0 = f.coord[s];
