@@ -73,14 +73,21 @@ __global__ void k_bin_tank(int fx, int fy, int fz, int tall, int* __restrict__ f
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = WG * SCAN_ITEMS;
 
+// The tile's total only: any order of the sum, so the loads are coalesced (element q*WG + t of the
+// tile for thread t) and all issued at once.
 template <class Op>
 __global__ void k_scan_reduce(Op op, int n, int* __restrict__ tile_sums) {
     __shared__ int sh[4];
-    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const int tile0 = blockIdx.x * SCAN_TILE;
+    int v[SCAN_ITEMS];
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        const int e = tile0 + q * WG + threadIdx.x;
+        v[q] = e < n ? op.value(e) : 0;
+    }
     int s = 0;
 #pragma unroll
-    for (int q = 0; q < SCAN_ITEMS; ++q)
-        if (base + q < n) s += op.value(base + q);
+    for (int q = 0; q < SCAN_ITEMS; ++q) s += v[q];
     int tot;
     block_exscan(s, sh, &tot);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
@@ -101,23 +108,45 @@ __global__ void k_scan_tiles(int* __restrict__ tile_sums, int ntiles, int* __res
     if (threadIdx.x == 0) *total = carry;
 }
 
+// The tile is loaded and emitted in coalesced stripes (element q*WG + t for thread t) and staged in
+// LDS, where thread t scans its SCAN_ITEMS consecutive elements; the LDS rows are padded to
+// SCAN_ITEMS + 1 words so that those reads do not conflict on the banks.
 template <class Op>
 __global__ void k_scan_apply(Op op, int n, const int* __restrict__ tile_sums) {
+    constexpr int RW = SCAN_ITEMS + 1;                 // padded row of one thread's elements
     __shared__ int sh[4];
-    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    __shared__ int sv[WG * RW];
+    const int tile0 = blockIdx.x * SCAN_TILE;
+    auto at = [](int el) { return (el / SCAN_ITEMS) * RW + el % SCAN_ITEMS; };
+    int lv[SCAN_ITEMS];
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        const int e = tile0 + q * WG + threadIdx.x;
+        lv[q] = e < n ? op.value(e) : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) sv[at(q * WG + threadIdx.x)] = lv[q];
+    __syncthreads();
     int v[SCAN_ITEMS];
     int s = 0;
 #pragma unroll
     for (int q = 0; q < SCAN_ITEMS; ++q) {
-        v[q] = (base + q < n) ? op.value(base + q) : 0;
+        v[q] = sv[threadIdx.x * RW + q];
         s += v[q];
     }
     int tot;
     int run = tile_sums[blockIdx.x] + block_exscan(s, sh, &tot);
+    // exclusive prefixes of this thread's elements replace their values
 #pragma unroll
     for (int q = 0; q < SCAN_ITEMS; ++q) {
-        if (base + q < n) op.emit(base + q, run, v[q]);
+        sv[threadIdx.x * RW + q] = run;
         run += v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        const int el = q * WG + threadIdx.x, e = tile0 + el;
+        if (e < n) op.emit(e, sv[at(el)], lv[q]);
     }
 }
 
