@@ -969,20 +969,23 @@ template <int B> int stage_solve(vare_flip* h) {
     }
     HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    const unsigned ga = nblocks(std::max<size_t>(nc, 1));
     tbegin(h, VARE_KERNEL_APPLY);
     if (nc && refine) {
         // pressure apply in FP64 on the refined pressure; the FP32 field keeps its rounding
-        hipLaunchKernelGGL((k_press_extrap<B, double>), dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+        hipLaunchKernelGGL((k_press_extrap_blk<B, double>), dim3(h->nb), dim3(B * B * B), 0, h->stream, h->frame(),
                            (const uint8_t*)h->type, h->x64);
-        hipLaunchKernelGGL((k_press_grad<B, double>), dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
-                           (const uint8_t*)h->type, (const double*)h->x64, h->vox_wall, h->cx, h->cy, h->cz);
+        if (h->ns)
+            hipLaunchKernelGGL((k_press_grad_blk<B, kIterK, double>), dim3(nwg_k(h, kIterK)), dim3(B * B * B), 0,
+                               h->stream, (const int*)h->wgtab, h->frame(), (const uint8_t*)h->type,
+                               (const double*)h->x64, h->vox_wall, h->cx, h->cy, h->cz);
         hipLaunchKernelGGL(k_ir_round, dim3(g), dim3(WG), 0, h->stream, (int)nc, (const double*)h->x64, h->x);
     } else if (nc) {
-        hipLaunchKernelGGL((k_press_extrap<B, float>), dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+        hipLaunchKernelGGL((k_press_extrap_blk<B, float>), dim3(h->nb), dim3(B * B * B), 0, h->stream, h->frame(),
                            (const uint8_t*)h->type, h->x);
-        hipLaunchKernelGGL((k_press_grad<B, float>), dim3(ga), dim3(WG), 0, h->stream, (int)nc, h->frame(),
-                           (const uint8_t*)h->type, (const float*)h->x, h->vox_wall, h->cx, h->cy, h->cz);
+        if (h->ns)
+            hipLaunchKernelGGL((k_press_grad_blk<B, kIterK, float>), dim3(nwg_k(h, kIterK)), dim3(B * B * B), 0,
+                               h->stream, (const int*)h->wgtab, h->frame(), (const uint8_t*)h->type,
+                               (const float*)h->x, h->vox_wall, h->cx, h->cy, h->cz);
     }
     tend(h, VARE_KERNEL_APPLY);
     HIPCHK(hipGetLastError());

@@ -580,62 +580,9 @@ __global__ void __launch_bounds__(kFinThreads) k_fin_beta(CgState* __restrict__ 
 // ------------------------------------------------------------------------------------------------
 // Pressure apply.  The solve runs in place on the cell layout, so FluidProjection.cu:297-306
 // (scatter x to the field) has no counterpart.  Air extrapolation, FluidProjection.cu:314-335:
-// each active air cell takes the mean pressure of its fluid neighbours.  PT is the pressure type:
-// float, or double for the FP64 pressure of iterative refinement (arithmetic in PT).
-template <int B, class PT>
-__global__ void k_press_extrap(int nc, FrameK f, const uint8_t* __restrict__ type, PT* __restrict__ pres) {
-    const int c = blockIdx.x * WG + threadIdx.x;
-    if (c >= nc) return;
-    if (type[c] != kAir) return;
-    PT press = PT(0);
-    int cnt = 0;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int id = cell_neighbor<B>(f, c, q);
-        if (id >= 0 && type[id] == kFluid) { press += pres[id]; cnt++; }
-    }
-    if (cnt > 0) pres[c] = press / (PT)cnt;
-}
-
-// Gradient subtract, FluidProjection.cu:342-378.
-template <int B, class PT>
-__global__ void k_press_grad(int nc, FrameK f, const uint8_t* __restrict__ type, const PT* __restrict__ pres,
-                             float wall_level, float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz) {
-    const int c = blockIdx.x * WG + threadIdx.x;
-    if (c >= nc) return;
-    if (type[c] != kFluid) return;
-    int i, j, k;
-    cell_coords<B>(f, c, i, j, k);
-    const uint8_t wall = ((float)j + 0.5f <= wall_level + 1.f) ? kSolid : kAir;
-    const PT p = pres[c];
-    int nid[6];
-    uint8_t tn[6];
-    PT pv[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        nid[q] = cell_neighbor<B>(f, c, q);
-        tn[q] = nid[q] == -2 ? wall : (nid[q] < 0 ? (uint8_t)kAir : type[nid[q]]);
-        pv[q] = nid[q] >= 0 ? pres[nid[q]] : PT(0);
-    }
-    PT nb[6];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int lo = 2 * a, hi = 2 * a + 1;
-        nb[lo] = nid[lo] == -2 ? PT(2) * p - pv[hi] : pv[lo];
-        nb[hi] = nid[hi] == -2 ? PT(2) * p - pv[lo] : pv[hi];
-    }
-    PT pf[6];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int lo = 2 * a, hi = 2 * a + 1;
-        pf[lo] = tn[lo] == kSolid ? PT(2) * p - nb[hi] : (tn[lo] == kAir ? -nb[lo] : nb[lo]);
-        pf[hi] = tn[hi] == kSolid ? PT(2) * p - nb[lo] : (tn[hi] == kAir ? -nb[hi] : nb[hi]);
-    }
-    cx[c] = (float)((PT)cx[c] - (pf[1] - pf[0]) * PT(0.5));
-    cy[c] = (float)((PT)cy[c] - (pf[3] - pf[2]) * PT(0.5));
-    cz[c] = (float)((PT)cz[c] - (pf[5] - pf[4]) * PT(0.5));
-}
-
+// each active air cell takes the mean pressure of its fluid neighbours (k_press_extrap_blk), then the
+// gradient is subtracted at the fluid cells (k_press_grad_blk, FluidProjection.cu:342-378).  PT is the
+// pressure type: float, or double for the FP64 pressure of iterative refinement (arithmetic in PT).
 // ------------------------------------------------------------------------------------------------
 // G2P + FLIP/PIC blend + advection + wall clamp + removal flag + gravity:
 // FLIPMethod::updateParticles (FLIPMethod.cu:86-171) with SparseField::lerp/lerpMin

@@ -206,6 +206,116 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_ir_residual_blk(const int* __re
     if (l == 0) partials[blockIdx.x] = tot;
 }
 
+// Storage indices of the six neighbours of cell l of block s (cell_neighbor's values: -1 inactive, -2
+// outside the domain), from the block's neighbour slots nb.
+template <int B>
+__device__ __forceinline__ void block_cell_nbrs(int s, int l, const int (&nb)[6], const int (&side)[3],
+                                                const int (&fl)[3], int (&id)[6]) {
+    constexpr int B3 = Fine<B>::B3;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int a = q >> 1;
+        if (side[a] == (q & 1)) id[q] = nb[q] < 0 ? nb[q] : nb[q] * B3 + fl[a];
+        else id[q] = s * B3 + inner_nbr<B>(l, q);
+    }
+}
+
+// Air extrapolation of the pressure (FluidProjection.cu:314-335: the mean of the fluid neighbours), one
+// workgroup per active block: the block's neighbour slots are uniform loads, and an air cell loads its
+// six neighbours' type and pressure together (the cell itself as a stand-in where a neighbour does not
+// exist).  Only air cells are written, only fluid cells' values are used, so there is no race.
+template <int B, class PT>
+__global__ void __launch_bounds__(Fine<B>::B3) k_press_extrap_blk(FrameK f, const uint8_t* __restrict__ type,
+                                                                 PT* __restrict__ pres) {
+    constexpr int B3 = Fine<B>::B3;
+    const int s = xcd_swizzle(blockIdx.x, gridDim.x);
+    int nb[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) nb[q] = f.nbr[s * 6 + q];
+    const int l = threadIdx.x, c = s * B3 + l;
+    if (type[c] != kAir) return;
+    int side[3], fl[3], id[6];
+    thread_faces<B>(l, side, fl);
+    block_cell_nbrs<B>(s, l, nb, side, fl, id);
+    uint8_t tn[6];
+    PT pv[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int ids = id[q] >= 0 ? id[q] : c;
+        tn[q] = type[ids];
+        pv[q] = pres[ids];
+    }
+    PT press = PT(0);
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        if (id[q] >= 0 && tn[q] == kFluid) { press += pv[q]; cnt++; }
+    if (cnt > 0) pres[c] = press / (PT)cnt;
+}
+
+// Pressure gradient subtract (FluidProjection.cu:342-378, same operation order) over the solver's
+// blocks (the only ones holding fluid cells) from the workgroup table: a fluid cell loads its
+// pressure, velocity and its neighbours' type and pressure in one round.
+template <int B, int K, class PT>
+__global__ void __launch_bounds__(Fine<B>::B3) k_press_grad_blk(const int* __restrict__ tab, FrameK f,
+                                                               const uint8_t* __restrict__ type,
+                                                               const PT* __restrict__ pres, float wall_level,
+                                                               float* __restrict__ cx, float* __restrict__ cy,
+                                                               float* __restrict__ cz) {
+    constexpr int B3 = Fine<B>::B3;
+    constexpr int SH = Fine<B>::SH;
+    int s[K], nb[K][6];
+    wg_load<K>(tab, s, nb);
+    const int l = threadIdx.x;
+    int side[3], fl[3];
+    thread_faces<B>(l, side, fl);
+#pragma unroll
+    for (int kb = 0; kb < K; ++kb) {
+        if (s[kb] < 0) continue;
+        const int c = s[kb] * B3 + l;
+        if (type[c] != kFluid) continue;
+        const int t = f.coord[s[kb]];
+        const int j = ((t / f.fx) % f.fy) * B + ((l >> SH) & (B - 1));
+        const uint8_t wall = ((float)j + 0.5f <= wall_level + 1.f) ? kSolid : kAir;
+        int nid[6];
+        block_cell_nbrs<B>(s[kb], l, nb[kb], side, fl, nid);
+        const PT p = pres[c];
+        const float vx = cx[c], vy = cy[c], vz = cz[c];
+        uint8_t tl[6];
+        PT pl[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int ids = nid[q] >= 0 ? nid[q] : c;
+            tl[q] = type[ids];
+            pl[q] = pres[ids];
+        }
+        uint8_t tn[6];
+        PT pv[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            tn[q] = nid[q] == -2 ? wall : (nid[q] < 0 ? (uint8_t)kAir : tl[q]);
+            pv[q] = nid[q] >= 0 ? pl[q] : PT(0);
+        }
+        PT nbv[6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int lo = 2 * a, hi = 2 * a + 1;
+            nbv[lo] = nid[lo] == -2 ? PT(2) * p - pv[hi] : pv[lo];
+            nbv[hi] = nid[hi] == -2 ? PT(2) * p - pv[lo] : pv[hi];
+        }
+        PT pf[6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int lo = 2 * a, hi = 2 * a + 1;
+            pf[lo] = tn[lo] == kSolid ? PT(2) * p - nbv[hi] : (tn[lo] == kAir ? -nbv[lo] : nbv[lo]);
+            pf[hi] = tn[hi] == kSolid ? PT(2) * p - nbv[lo] : (tn[hi] == kAir ? -nbv[hi] : nbv[hi]);
+        }
+        cx[c] = (float)((PT)vx - (pf[1] - pf[0]) * PT(0.5));
+        cy[c] = (float)((PT)vy - (pf[3] - pf[2]) * PT(0.5));
+        cz[c] = (float)((PT)vz - (pf[5] - pf[4]) * PT(0.5));
+    }
+}
+
 // Ap = A p and partial p.Ap (BLAS::mul, BLAS.cu:12-40).
 template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_spmv_blk(const int* __restrict__ tab, CgVecs v,
