@@ -262,9 +262,11 @@ int ensure_blocks(vare_flip* h, size_t nb) {
 
 int ensure_particles(vare_flip* h, size_t n) {
     if (n <= h->pcap && h->pa[0]) return VARE_OK;
+    // kernels address particle arrays by 32-bit byte offsets: at most 2^30 particles
+    constexpr size_t kMaxParticles = (size_t)1 << 30;
+    if (n > kMaxParticles) return fail(VARE_ENOMEM, "%zu particles exceed the limit of 2^30", n);
     size_t cap = std::max(n, h->pcap + h->pcap / 2);
-    cap = std::max<size_t>(cap, 1024);
-    if (cap > (size_t)0x7fffffff) return fail(VARE_ENOMEM, "particle capacity %zu exceeds 2^31-1", cap);
+    cap = std::min(std::max<size_t>(cap, 1024), kMaxParticles);
     float* na[9] = {};
     for (int q = 0; q < 9; ++q) {
         const size_t esz = (q >= 3 && q < 6) ? h->vsz() : sizeof(float);

@@ -37,6 +37,9 @@ __host__ __device__ inline float rand01(uint32_t seed) {        // Random.h:18-2
 template <class T> __device__ __forceinline__ T rmin(T a, T b) { return (a < b) ? a : b; }   // MathUtils.h:174-178
 template <class T> __device__ __forceinline__ T rmax(T a, T b) { return (a > b) ? a : b; }   // MathUtils.h:195-199
 __device__ __forceinline__ float rabs(float x) { return (x < 0.f) ? -x : x; }                // MathUtils.h:129-132
+// Max(0, 1 - Abs(d)) (MathUtils.h:129-132, :195-199) as one v_max_f32 with an abs source modifier:
+// the same value for every d that is not NaN (a particle with a NaN coordinate has no bin).
+__device__ __forceinline__ float tent(float d) { return __builtin_fmaxf(0.f, 1.f - __builtin_fabsf(d)); }
 template <class T> __device__ __forceinline__ T rclamp(T x, T lo, T hi) {                   // MathUtils.h:286-292
     if (x < lo) return lo;
     if (x > hi) return hi;
@@ -59,11 +62,22 @@ struct VelPtr {
 
 // The same arrays with the element type fixed at compile time (H: FP16), for the hot particle
 // kernels: with no run-time branch per load, every load of a particle issues before any conversion.
+// Element i of an array by a 32-bit byte offset from its base pointer, which lets a load take a
+// uniform base in scalar registers and one vector offset.  Particle arrays hold at most 2^30
+// elements (ensure_particles), so i * sizeof(T) < 2^32.
+template <class T> __device__ __forceinline__ T ld_u32(const T* p, unsigned i) {
+    return *(const T*)((const char*)p + i * (unsigned)sizeof(T));
+}
+
 template <bool H> struct VelT {
     void* p[3];
     __device__ __forceinline__ float ld(int a, int i) const {
         if constexpr (H) return (float)((const _Float16*)p[a])[i];
         else return ((const float*)p[a])[i];
+    }
+    __device__ __forceinline__ float ldu(int a, unsigned i) const {
+        if constexpr (H) return (float)ld_u32((const _Float16*)p[a], i);
+        else return ld_u32((const float*)p[a], i);
     }
     __device__ __forceinline__ void st(int a, int i, float v) const {
         if constexpr (H) ((_Float16*)p[a])[i] = (_Float16)v;

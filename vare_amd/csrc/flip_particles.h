@@ -382,16 +382,16 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) __attribute__((amdgpu_w
             const int q0 = cstart[cell], q1 = cstart[cell + 1];
             const int ci = ox + li, cj = oy + lj, ck = oz + lk;
             ac = (float)(q1 - q0);                   // the sum of q1-q0 ones, exact below 2^24
-            for (int n = q0; n < q1; ++n) {              // the bin's particles are contiguous
-                const float x = px[n], y = py[n], z = pz[n];
-                const float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
+            for (unsigned n = q0; n < (unsigned)q1; ++n) {     // the bin's particles are contiguous
+                const float x = ld_u32(px, n), y = ld_u32(py, n), z = ld_u32(pz, n);
+                const float vx = u.ldu(0, n), vy = u.ldu(1, n), vz = u.ldu(2, n);
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     const int i = ci + (c & 1), j = cj + ((c >> 1) & 1), k = ck + (c >> 2);
                     const float ddx = x - ((float)i + 0.5f), ddy = y - ((float)j + 0.5f), ddz = z - ((float)k + 0.5f);
-                    const float xw = rmax(0.f, 1.f - rabs(ddx));
-                    const float yw = rmax(0.f, 1.f - rabs(ddy));
-                    const float zw = rmax(0.f, 1.f - rabs(ddz));
+                    const float xw = tent(ddx);
+                    const float yw = tent(ddy);
+                    const float zw = tent(ddz);
                     const float w = xw * yw * zw + 1e-05f;
                     am[c] += w;
                     ax[c] += w * vx;
