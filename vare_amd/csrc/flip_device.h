@@ -58,6 +58,8 @@ struct VelPtr {
         if (half) ((_Float16*)p[a])[i] = (_Float16)v;
         else ((float*)p[a])[i] = v;
     }
+    __device__ __forceinline__ float ldu(int a, unsigned i) const { return ld(a, (int)i); }
+    __device__ __forceinline__ void stu(int a, unsigned i, float v) const { st(a, (int)i, v); }
 };
 
 // The same arrays with the element type fixed at compile time (H: FP16), for the hot particle
@@ -67,6 +69,9 @@ struct VelPtr {
 // elements (ensure_particles), so i * sizeof(T) < 2^32.
 template <class T> __device__ __forceinline__ T ld_u32(const T* p, unsigned i) {
     return *(const T*)((const char*)p + i * (unsigned)sizeof(T));
+}
+template <class T> __device__ __forceinline__ void st_u32(T* p, unsigned i, T v) {
+    *(T*)((char*)p + i * (unsigned)sizeof(T)) = v;
 }
 
 template <bool H> struct VelT {
@@ -78,6 +83,10 @@ template <bool H> struct VelT {
     __device__ __forceinline__ float ldu(int a, unsigned i) const {
         if constexpr (H) return (float)ld_u32((const _Float16*)p[a], i);
         else return ld_u32((const float*)p[a], i);
+    }
+    __device__ __forceinline__ void stu(int a, unsigned i, float v) const {
+        if constexpr (H) st_u32((_Float16*)p[a], i, (_Float16)v);
+        else st_u32((float*)p[a], i, v);
     }
     __device__ __forceinline__ void st(int a, int i, float v) const {
         if constexpr (H) ((_Float16*)p[a])[i] = (_Float16)v;

@@ -443,23 +443,27 @@ template <int B, bool COL, class Vel, class Fetch>
 __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
                                             const Vel& u, float* __restrict__ adj, const FrameK& f, const G2PParams& gp,
                                             const ColK& col, uint8_t* __restrict__ removed, Fetch fetch) {
-    float x = px[n], y = py[n], z = pz[n];
-    float vx = u.ld(0, n), vy = u.ld(1, n), vz = u.ld(2, n);
+    const unsigned un = (unsigned)n;
+    float x = ld_u32(px, un), y = ld_u32(py, un), z = ld_u32(pz, un);
+    float vx = u.ldu(0, un), vy = u.ldu(1, un), vz = u.ldu(2, un);
     const int i = rclamp((int)(x - 0.5f), 0, f.nx - 2);
     const int j = rclamp((int)(y - 0.5f), 0, f.ny - 2);
     const int k = rclamp((int)(z - 0.5f), 0, f.nz - 2);
-    const float fxw = rclamp(x - ((float)i + 0.5f), 0.f, 1.f);
-    const float fyw = rclamp(y - ((float)j + 0.5f), 0.f, 1.f);
-    const float fzw = rclamp(z - ((float)k + 0.5f), 0.f, 1.f);
+    // Clamp(d, 0, 1) as max/min (one clamped max): the same value, since d = p - (i + 0.5) is never
+    // -0 and never NaN for a particle that has a bin
+    const float fxw = __builtin_fminf(__builtin_fmaxf(x - ((float)i + 0.5f), 0.f), 1.f);
+    const float fyw = __builtin_fminf(__builtin_fmaxf(y - ((float)j + 0.5f), 0.f), 1.f);
+    const float fzw = __builtin_fminf(__builtin_fmaxf(z - ((float)k + 0.5f), 0.f), 1.f);
     // fields fi: 0 cnt, 1-3 cx, cy, cz, 4-6 vx0, vy0, vz0; corner q = dx + 2dy + 4dz.  One field is
     // fetched and interpolated at a time, which keeps 8 values live instead of 56.
     auto corner = [&](int fi, int q) -> float { return fetch(fi, i + (q & 1), j + ((q >> 1) & 1), k + (q >> 2)); };
     {
-        float a = rmin(corner(0, 0), corner(0, 1));
-        a = rmin(a, rmin(corner(0, 4), corner(0, 5)));
-        a = rmin(a, rmin(corner(0, 2), corner(0, 3)));
-        a = rmin(a, rmin(corner(0, 6), corner(0, 7)));
-        adj[n] = a;
+        // counts are sums of +1 (or the stand-in 10): never NaN or -0, so fminf is the reference's Min
+        float a = __builtin_fminf(corner(0, 0), corner(0, 1));
+        a = __builtin_fminf(a, __builtin_fminf(corner(0, 4), corner(0, 5)));
+        a = __builtin_fminf(a, __builtin_fminf(corner(0, 2), corner(0, 3)));
+        a = __builtin_fminf(a, __builtin_fminf(corner(0, 6), corner(0, 7)));
+        st_u32(adj, un, a);
     }
     auto lerp = [&](int fi) -> float {
         float v[8];
@@ -533,12 +537,12 @@ __device__ __forceinline__ int g2p_particle(int n, float* __restrict__ px, float
         if (phi > 3.f || phi < 0.f) rem = 1;
     }
     if (x < 0.f || x > fnx || y < 0.f || y > fny || z < 0.f || z > fnz) rem = 1;   // Grid::inside
-    removed[n] = (uint8_t)rem;
+    st_u32(removed, un, (uint8_t)rem);
     vx += gp.gx * gp.dt;
     vy += gp.gy * gp.dt;
     vz += gp.gz * gp.dt;
-    px[n] = x; py[n] = y; pz[n] = z;
-    u.st(0, n, vx); u.st(1, n, vy); u.st(2, n, vz);
+    st_u32(px, un, x); st_u32(py, un, y); st_u32(pz, un, z);
+    u.stu(0, un, vx); u.stu(1, un, vy); u.stu(2, un, vz);
     return rem;
 }
 
