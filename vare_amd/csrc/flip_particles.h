@@ -24,7 +24,8 @@ constexpr unsigned long long kSortPad = ~0ull;
 // (the slices' loads and slot lookups are independent, so their latencies overlap).  The counts are
 // aggregated in LDS first: every key of the workgroup is entered in an open-addressing hash table
 // (load factor at most 1/2) whose LDS counter hands out the rank among the workgroup's particles of
-// that bin, then one global atomic per distinct key reserves the workgroup's run in the bin.  The
+// that bin (one entry per run of equal keys in adjacent lanes), then one global atomic per distinct
+// key reserves the workgroup's run in the bin.  The
 // particles of a workgroup come from a few neighbouring cells, so this takes several times fewer
 // global atomics than one per particle or per run of equal keys, and global atomic throughput is what
 // bounds this kernel.  Ranks depend on the atomics' order; the bins are sorted by reference index
@@ -70,21 +71,34 @@ __global__ void __launch_bounds__(WG) k_ckey(const float* __restrict__ px, const
         key[idx[j]] = k[j];
     }
     __syncthreads();
+    // Runs of equal keys in adjacent lanes (particles arrive in the previous step's bin order) enter
+    // the table once: the run's first lane finds the slot and adds the run length, the others take
+    // that slot and base plus their distance from the first lane.
+    const int lane = threadIdx.x & 63;
+    const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;     // lanes <= this one
     int hs[J], loc[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        hs[j] = 0;
-        loc[j] = 0;
-        if (!valid[j]) continue;
-        // keys are >= 0 and at most J*WG are entered, so a free slot is always found
-        int q = (int)(((unsigned)k[j] * 2654435761u) >> 8) & (H - 1);
-        while (true) {
-            const int old = atomicCAS(&hk[q], -1, k[j]);
-            if (old == -1 || old == k[j]) break;
-            q = (q + 1) & (H - 1);
+        const int kj = valid[j] ? k[j] : -1;
+        const int pk = __shfl_up(kj, 1);
+        const bool head = lane == 0 || kj != pk;
+        const unsigned long long hm = __ballot(head);
+        const int h = 63 - __clzll(hm & upto);                 // first lane of this lane's run
+        const unsigned long long after = hm & ~upto;
+        const int next = after ? __ffsll((long long)after) - 1 : 64;
+        int q = 0, base = 0;
+        if (head && kj >= 0) {
+            // keys are >= 0 and at most J*WG are entered, so a free slot is always found
+            q = (int)(((unsigned)kj * 2654435761u) >> 8) & (H - 1);
+            while (true) {
+                const int old = atomicCAS(&hk[q], -1, kj);
+                if (old == -1 || old == kj) break;
+                q = (q + 1) & (H - 1);
+            }
+            base = atomicAdd(&hc[q], next - lane);
         }
-        hs[j] = q;
-        loc[j] = atomicAdd(&hc[q], 1);
+        hs[j] = __shfl(q, h);
+        loc[j] = __shfl(base, h) + (lane - h);
     }
     __syncthreads();
     {
