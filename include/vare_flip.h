@@ -142,7 +142,8 @@ int vare_grid_from_voxel_size(float voxel_size, const float min[3], const float 
 
 /* FLIPMethod::initialize(const Grid&) (FLIPMethod.cu:16-47).  particle_capacity and
  * block_capacity reserve device memory (0 = choose from the grid); both grow on demand
- * outside of graph capture. */
+ * outside of graph capture.  A handle holds at most 2^30 particles (the kernels address particle
+ * arrays by 32-bit byte offsets); adding or setting more returns VARE_ENOMEM. */
 int vare_flip_create(const vare_grid_desc* grid, const vare_flip_params* params, size_t particle_capacity,
                      size_t block_capacity, vare_flip** out);
 int vare_flip_destroy(vare_flip* h);
