@@ -234,6 +234,12 @@ def test_empty_and_error_paths():
     with pytest.raises(L.VareError) as e:
         h.set_particles(bad, np.zeros_like(bad))
     assert e.value.code == L.VARE_ENONFINITE
+    # more than 2^30 particles is refused before any input is read
+    one = np.zeros((1, 3), np.float32)
+    rc = h.lib.vare_flip_set_particles(h.h, L.ptr(one), L.ptr(one), None, None, None, (1 << 30) + 1,
+                                       L.SPACE_VOXEL, L.MEM_HOST)
+    assert rc == L.VARE_ENOMEM
+    assert h.num_particles() == 0
     pos, vel = make_scene(n, seed=6)
     h2 = L.FlipHandle(g, L.default_params(voxel_size=1 / n, ext_force=(0, -9.8, 0), wall_level=1e30,
                                           solver_mode=1, cg_rel_tol=1e-9, max_iterations=2))
