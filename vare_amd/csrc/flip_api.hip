@@ -459,7 +459,8 @@ template <int B> int stage_prepare(vare_flip* h) {
     if (h->np) {
         h->occ_stamp = h->occ_stamp == INT_MAX ? 1 : h->occ_stamp + 1;
         if (h->occ_stamp == 1) CHK(fill_u32(h, h->occ, 0, h->nocc()));
-        hipLaunchKernelGGL(k_bin_occ<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
+        hipLaunchKernelGGL(k_bin_occ<B>, dim3(nblocks((h->np + kOccSlices - 1) / kOccSlices)), dim3(WG), 0, h->stream,
+                           h->pa[0], h->pa[1], h->pa[2],
                            (int)h->np, h->fx, h->fy, h->fz, h->occ_stamp, h->occ);
         hipLaunchKernelGGL(k_bin_dilate, dim3(nblocks(h->nocc())), dim3(WG), 0, h->stream, h->fx, h->fy, h->fz,
                            h->occ_stamp, (const int*)h->occ, h->flag);

@@ -29,20 +29,34 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
 // the store, and particles stored in bin order give one store per wave.  k_bin_dilate then marks the
 // neighbourhood of every occupied block once, so a block's 27 flags are written once per step
 // rather than once per wave of its particles.
+// A workgroup covers kOccSlices consecutive slices of WG particles, one particle per thread per slice,
+// with every slice's loads issued before any is used.
+constexpr int kOccSlices = 4;
 template <int B>
 __global__ void k_bin_occ(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
                           int np, int fx, int fy, int fz, int stamp, int* __restrict__ occ) {
-    const int i = blockIdx.x * WG + threadIdx.x;
-    int e = -1;
-    if (i < np) {
-        const int ci = rmax((int)(px[i]) / B, 0);
-        const int cj = rmax((int)(py[i]) / B, 0);
-        const int ck = rmax((int)(pz[i]) / B, 0);
-        if (ci <= fx && cj <= fy && ck <= fz) e = (ck * (fy + 1) + cj) * (fx + 1) + ci;
+    constexpr int J = kOccSlices;
+    float x[J], y[J], z[J];
+    int idx[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        idx[j] = (blockIdx.x * J + j) * WG + threadIdx.x;
+        const int i = idx[j] < np ? idx[j] : 0;
+        x[j] = px[i]; y[j] = py[i]; z[j] = pz[i];
     }
     const int lane = threadIdx.x & 63;
-    const int prev = __shfl_up(e, 1, 64);
-    if (e >= 0 && (lane == 0 || prev != e)) occ[e] = stamp;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        int e = -1;
+        if (idx[j] < np) {
+            const int ci = rmax((int)(x[j]) / B, 0);
+            const int cj = rmax((int)(y[j]) / B, 0);
+            const int ck = rmax((int)(z[j]) / B, 0);
+            if (ci <= fx && cj <= fy && ck <= fz) e = (ck * (fy + 1) + cj) * (fx + 1) + ci;
+        }
+        const int prev = __shfl_up(e, 1, 64);
+        if (e >= 0 && (lane == 0 || prev != e)) occ[e] = stamp;
+    }
 }
 
 __global__ void k_bin_dilate(int fx, int fy, int fz, int stamp, const int* __restrict__ occ, int* __restrict__ flag) {
