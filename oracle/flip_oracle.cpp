@@ -570,6 +570,11 @@ template <class R> struct State {
     // ---- FluidProjection::solve, sparse (FluidProjection.cu:277-381) ----
     void solve_and_apply() {
         cg_iters = prm.solver_mode == 0 ? cg_reference(prm.proj_iteration) : pcg_relative(prm.max_iterations);
+        apply_pressure();
+    }
+
+    // K10-K12 on the current solution x (FluidProjection.cu:297-378)
+    void apply_pressure() {
         for (size_t n = 0; n < fluid.size(); ++n) {               // K10, :297-306
             const uint64_t idx = fluid[n];
             pres[idx] = x[mark[idx] - kFluid];
@@ -960,7 +965,9 @@ size_t orc_get_field(void* hp, int which, double* out) {
     return h->fp64 ? field_t(h->s64, which, out) : field_t(h->s32, which, out);
 }
 
-// Overwrite the pre-projection/projected velocity fields (testing a stage on given inputs).
+// Overwrite a field (testing a stage on given inputs): 0 mass, 1 velo, 2 comp, 3 mark (cell types
+// 0 air / 1 solid / 4 fluid before build_linear_system numbers the fluid cells), 4 pres, 7 x (the
+// solution per fluid rank, for apply_pressure).
 void orc_set_field(void* hp, int which, const double* in) {
     Handle* h = (Handle*)hp;
     auto run = [&](auto* s) {
@@ -968,8 +975,18 @@ void orc_set_field(void* hp, int which, const double* in) {
         if (which == 1) for (size_t n = 0; n < s->velo.size(); ++n) s->velo[n] = {(R)in[3*n], (R)in[3*n+1], (R)in[3*n+2]};
         if (which == 2) for (size_t n = 0; n < s->comp.size(); ++n) s->comp[n] = {(R)in[3*n], (R)in[3*n+1], (R)in[3*n+2]};
         if (which == 0) for (size_t n = 0; n < s->mass.size(); ++n) s->mass[n] = (R)in[n];
+        if (which == 3) for (size_t n = 0; n < s->mark.size(); ++n) s->mark[n] = (uint64_t)in[n];
+        if (which == 4) for (size_t n = 0; n < s->pres.size(); ++n) s->pres[n] = (R)in[n];
+        if (which == 7) for (size_t n = 0; n < s->x.size(); ++n) s->x[n] = (R)in[n];
     };
     if (h->fp64) run(h->s64); else run(h->s32);
+}
+
+// Pressure scatter, air extrapolation and gradient subtract (FluidProjection.cu:297-378) on the
+// current x, without solving (x set by orc_set_field(7) after build_linear_system).
+void orc_apply_pressure(void* hp) {
+    Handle* h = (Handle*)hp;
+    if (h->fp64) h->s64->apply_pressure(); else h->s32->apply_pressure();
 }
 
 size_t orc_get_removed(void* hp, uint8_t* out) {

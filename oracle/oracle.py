@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libvare_oracle.so")
+# VARE_ORACLE_LIB points the checker at another build of the same source (the mutation test of
+# tests/test_oracle_mutants.py loads deliberately broken copies this way)
+LIB_PATH = os.environ.get("VARE_ORACLE_LIB") or os.path.join(HERE, "lib", "libvare_oracle.so")
 
 FIELD_MASS, FIELD_VELO, FIELD_COMP, FIELD_MARK, FIELD_PRES, FIELD_COUNT = 0, 1, 2, 3, 4, 5
 FIELD_B, FIELD_X, FIELD_FLUID, FIELD_AVAL, FIELD_AIND = 6, 7, 8, 9, 10
@@ -60,6 +62,7 @@ def load():
         "orc_get_markers": (None, [_V, _V]),
         "orc_get_field": (C.c_size_t, [_V, C.c_int, _V]),
         "orc_set_field": (None, [_V, C.c_int, _V]),
+        "orc_apply_pressure": (None, [_V]),
         "orc_get_removed": (C.c_size_t, [_V, _V]),
         "orc_get_stats": (None, [_V, C.POINTER(OrcStats)]),
         "orc_remove_perm": (C.c_size_t, [_V, C.c_size_t, _V]),
@@ -230,6 +233,10 @@ class Oracle:
     def set_field(self, which, values):
         v = np.ascontiguousarray(values, np.float64).reshape(-1)
         self.lib.orc_set_field(self.h, which, _p(v))
+
+    def apply_pressure(self):
+        """K10-K12 (FluidProjection.cu:297-378) on the x set with set_field(FIELD_X, ...)."""
+        self.lib.orc_apply_pressure(self.h)
 
     def removed(self):
         out = np.empty(self.lib.orc_get_removed(self.h, None), np.uint8)
