@@ -147,6 +147,15 @@ int vare_grid_from_voxel_size(float voxel_size, const float min[3], const float 
 int vare_flip_create(const vare_grid_desc* grid, const vare_flip_params* params, size_t particle_capacity,
                      size_t block_capacity, vare_flip** out);
 int vare_flip_destroy(vare_flip* h);
+/* Device bytes vare_flip_create(grid, params, particle_capacity, block_capacity) allocates, plus what
+ * the first steps add (multigrid levels, scan tiles) and the staging of one host upload or download of
+ * the particles: the peak of the same allocation sequence, counted on the host without a GPU.  No
+ * counterpart in the reference (it allocates managed memory as it goes); the server admits requests
+ * with it. */
+int vare_flip_plan_bytes(const vare_grid_desc* grid, const vare_flip_params* params, size_t particle_capacity,
+                         size_t block_capacity, size_t* bytes);
+/* Device bytes the handle holds now (live) and the most it has held (peak).  Any pointer may be NULL. */
+int vare_flip_get_device_bytes(const vare_flip* h, size_t* live, size_t* peak);
 int vare_flip_set_params(vare_flip* h, const vare_flip_params* params);
 int vare_flip_get_frame(const vare_flip* h, vare_frame_info* out);
 
@@ -325,6 +334,11 @@ int vare_server_result(vare_server* s, int64_t id, float* pos, float* vel, size_
 int vare_server_get_stats(vare_server* s, vare_server_stats* out);
 /* Estimated device bytes a request reserves (what admission charges against the budget). */
 int vare_server_request_bytes(const vare_tank_request* req, size_t* bytes);
+
+/* Test hook (process-wide): after the next `skip` device allocations of any handle, the following
+ * `count` fail with VARE_ENOMEM as if hipMalloc had failed; count <= 0 switches it off.  A failed
+ * allocation leaves a handle usable with its previous capacity. */
+int vare_debug_inject_alloc_failure(long skip, long count);
 
 /* Last error message of the calling thread. */
 const char* vare_last_error(void);

@@ -97,3 +97,24 @@ def test_seed_pool_sphere_region():
     pool = pos[pos[:, 1] < 0.3 * n]
     assert pool.shape[0] >= box.shape[0]
     assert {tuple(p) for p in box} <= {tuple(p) for p in pool}
+
+
+def test_plan_bytes_counts_every_array_without_a_gpu():
+    # vare_flip_plan_bytes runs the allocation sequence of create + first steps on a planning handle
+    # (no device call); per particle it must count at least the 9 attribute arrays, their 7 reorder
+    # buffers, the binning/removal arrays and the 7-float download staging (about 130 bytes)
+    g = L.grid_desc(64, 64, 64, block_size=8)
+    prm = L.default_params(solver_mode=1, preconditioner=L.PRECOND_MG)
+    a = L.plan_bytes(g, prm, 100000)
+    b = L.plan_bytes(g, prm, 1100000)
+    assert (b - a) / 1e6 >= 125
+    # velocity storage in FP16 saves 2 bytes per component in the arrays and their reorder buffers
+    h16 = L.default_params(solver_mode=1, preconditioner=L.PRECOND_MG, velocity_fp16=1)
+    assert L.plan_bytes(g, prm, 1100000) - L.plan_bytes(g, h16, 1100000) == 2 * 2 * 3 * 1100000
+    # the multigrid levels are counted only with the multigrid preconditioner
+    cg = L.default_params(solver_mode=1, preconditioner=L.PRECOND_JACOBI)
+    assert L.plan_bytes(g, cg, 100000) < a
+    # a full-frame block capacity of 8^3 blocks: at least the 14 FP32 cell fields per cell
+    assert a >= 14 * 4 * 64 ** 3
+    # the server charges exactly the plan
+    assert L.Server.request_bytes(g, prm, 100000) == a

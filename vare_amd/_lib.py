@@ -129,6 +129,10 @@ SIGNATURES = {
     "vare_grid_from_voxel_size": (C.c_int, [C.c_float, C.c_float * 3, C.c_float * 3, C.c_int, C.POINTER(GridDesc)]),
     "vare_flip_create": (C.c_int, [C.POINTER(GridDesc), C.POINTER(FlipParams), C.c_size_t, C.c_size_t, C.POINTER(_P)]),
     "vare_flip_destroy": (C.c_int, [_P]),
+    "vare_flip_plan_bytes": (C.c_int, [C.POINTER(GridDesc), C.POINTER(FlipParams), C.c_size_t, C.c_size_t,
+                                       C.POINTER(C.c_size_t)]),
+    "vare_flip_get_device_bytes": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "vare_debug_inject_alloc_failure": (C.c_int, [C.c_long, C.c_long]),
     "vare_flip_set_params": (C.c_int, [_P, C.POINTER(FlipParams)]),
     "vare_flip_get_frame": (C.c_int, [_P, C.POINTER(FrameInfo)]),
     "vare_flip_add_particles": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_int, C.c_int]),
@@ -266,6 +270,12 @@ class FlipHandle:
             self.close()
         except Exception:
             pass
+
+    def device_bytes(self):
+        """(live, peak) device bytes of the handle."""
+        live, peak = C.c_size_t(), C.c_size_t()
+        check(self.lib.vare_flip_get_device_bytes(self.h, C.byref(live), C.byref(peak)))
+        return live.value, peak.value
 
     def set_params(self, params: FlipParams):
         check(self.lib.vare_flip_set_params(self.h, C.byref(params)))
@@ -487,6 +497,19 @@ class Server:
         b = C.c_size_t()
         check(load().vare_server_request_bytes(C.byref(r), C.byref(b)))
         return b.value
+
+
+def plan_bytes(grid, params, particle_capacity=0, block_capacity=0):
+    """Device bytes vare_flip_create + the first steps allocate (vare_flip_plan_bytes; no GPU needed)."""
+    b = C.c_size_t()
+    check(load().vare_flip_plan_bytes(C.byref(grid), C.byref(params), particle_capacity, block_capacity,
+                                      C.byref(b)))
+    return b.value
+
+
+def inject_alloc_failure(skip, count):
+    """Test hook: after `skip` device allocations, fail the next `count` (count <= 0: off)."""
+    check(load().vare_debug_inject_alloc_failure(int(skip), int(count)))
 
 
 def device_synchronize():

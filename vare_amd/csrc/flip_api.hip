@@ -15,6 +15,8 @@
 #include <vector>
 #include <algorithm>
 #include <thread>
+#include <atomic>
+#include <unordered_map>
 
 #include "../../include/vare_flip.h"
 #include "flip_kernels.h"
@@ -57,22 +59,6 @@ inline unsigned nblocks(size_t n) { return (unsigned)((n + WG - 1) / WG); }
 // Capped grid for the grid-stride reducing kernels: 8 workgroups of 256 threads per CU.
 constexpr unsigned kReduceGrid = 2048;
 inline unsigned rgrid(size_t n) { return std::max(1u, std::min(nblocks(n), kReduceGrid)); }
-
-template <class T> int dalloc(T** p, size_t n) {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        *p = nullptr;
-        return fail(VARE_ENOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
-    }
-    return VARE_OK;
-}
-template <class T> void dfree(T*& p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
 
 }  // namespace
 
@@ -172,11 +158,20 @@ struct vare_flip {
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kid;
     std::vector<int64_t> ev_units;
+    std::vector<int> ev_iter;     // solver iteration of the timed launch (-1 outside the iterations)
     size_t ev_used = 0;
+    int cur_iter = -1;            // iteration being enqueued (enqueue_iteration), -1 otherwise
     double kms[VARE_KERNEL_COUNT] = {};
     int64_t klaunch[VARE_KERNEL_COUNT] = {};
     int64_t kunits[VARE_KERNEL_COUNT] = {};
     hipEvent_t step_ev[4] = {};
+
+    // device memory accounting: every allocation of the handle goes through dalloc/dfree
+    std::unordered_map<void*, size_t> allocs;
+    size_t dev_bytes = 0;         // live bytes
+    size_t peak_bytes = 0;        // most live bytes at any time
+    int dry = 0;                  // planning handle (vare_flip_plan_bytes): count, allocate nothing
+    uintptr_t dry_next = 0;
 
     size_t ncells() const { return (size_t)nb * (size_t)B * B * B; }
     size_t vsz() const { return vhalf ? 2 : 4; }
@@ -200,14 +195,68 @@ struct vare_flip {
 
 namespace {
 
+// Debug hook (vare_debug_inject_alloc_failure): let `skip` more allocations through, then fail `fail`.
+std::atomic<long> g_alloc_skip{-1};
+std::atomic<long> g_alloc_fail{0};
+
+bool injected_failure() {
+    if (g_alloc_skip.load() < 0) return false;
+    if (g_alloc_skip.load() > 0) { g_alloc_skip.fetch_sub(1); return false; }
+    if (g_alloc_fail.load() <= 0) { g_alloc_skip.store(-1); return false; }
+    g_alloc_fail.fetch_sub(1);
+    return true;
+}
+
+// Every device array of a handle is allocated here, so the handle knows its live and peak bytes.  A
+// planning handle (dry) hands out distinct fake addresses that are never dereferenced.
+int dalloc_bytes(vare_flip* h, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 1;
+    if (injected_failure()) return fail(VARE_ENOMEM, "hipMalloc of %zu bytes failed: injected failure", bytes);
+    if (h->dry) {
+        h->dry_next += (bytes + 255) & ~(size_t)255;
+        *p = (void*)h->dry_next;
+    } else {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            return fail(VARE_ENOMEM, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        }
+    }
+    h->allocs[*p] = bytes;
+    h->dev_bytes += bytes;
+    h->peak_bytes = std::max(h->peak_bytes, h->dev_bytes);
+    return VARE_OK;
+}
+void dfree_bytes(vare_flip* h, void* p) {
+    if (!p) return;
+    auto it = h->allocs.find(p);
+    if (it != h->allocs.end()) {
+        h->dev_bytes -= it->second;
+        h->allocs.erase(it);
+    }
+    if (!h->dry) (void)hipFree(p);
+}
+template <class T> int dalloc(vare_flip* h, T** p, size_t n) {
+    void* v = nullptr;
+    const int rc = dalloc_bytes(h, &v, (n ? n : 1) * sizeof(T));
+    *p = (T*)v;
+    return rc;
+}
+template <class T> void dfree(vare_flip* h, T*& p) {
+    dfree_bytes(h, (void*)p);
+    p = nullptr;
+}
+
 void free_mg(vare_flip* h) {
     for (size_t l = 1; l < h->mg.size(); ++l) {
-        dfree(h->mg[l].r);
-        dfree(h->mg[l].info);
-        dfree(h->mg_z[l]);
-        dfree(h->mg_zt[l]);
+        dfree(h, h->mg[l].r);
+        dfree(h, h->mg[l].info);
+        dfree(h, h->mg_z[l]);
+        dfree(h, h->mg_zt[l]);
     }
-    dfree(h->zt0);
+    dfree(h, h->zt0);
     h->mg.clear();
     h->mg_z.clear();
     h->mg_zt.clear();
@@ -215,47 +264,56 @@ void free_mg(vare_flip* h) {
 }
 
 void free_collision(vare_flip* h) {
-    dfree(h->col_slot);
-    dfree(h->col_phi);
-    for (int a = 0; a < 3; ++a) { dfree(h->col_v[a]); dfree(h->col_n[a]); }
+    dfree(h, h->col_slot);
+    dfree(h, h->col_phi);
+    for (int a = 0; a < 3; ++a) { dfree(h, h->col_v[a]); dfree(h, h->col_n[a]); }
     h->col_cells = 0;
     h->col_active = 0;
 }
 
 void free_cells(vare_flip* h) {
     free_mg(h);
-    dfree(h->nbr);
-    dfree(h->ccount); dfree(h->cstart);
-    dfree(h->bfluid); dfree(h->blist); dfree(h->wgtab);
-    dfree(h->mass); dfree(h->vx); dfree(h->vy); dfree(h->vz); dfree(h->cnt);
-    dfree(h->cx); dfree(h->cy); dfree(h->cz);
-    dfree(h->x); dfree(h->r); dfree(h->p); dfree(h->ap); dfree(h->b); dfree(h->z); dfree(h->x64);
-    dfree(h->r2); dfree(h->p2);
-    dfree(h->info); dfree(h->type); dfree(h->rank);
-    dfree(h->partials); dfree(h->partials2);
+    dfree(h, h->nbr);
+    dfree(h, h->ccount); dfree(h, h->cstart);
+    dfree(h, h->bfluid); dfree(h, h->blist); dfree(h, h->wgtab);
+    dfree(h, h->mass); dfree(h, h->vx); dfree(h, h->vy); dfree(h, h->vz); dfree(h, h->cnt);
+    dfree(h, h->cx); dfree(h, h->cy); dfree(h, h->cz);
+    dfree(h, h->x); dfree(h, h->r); dfree(h, h->p); dfree(h, h->ap); dfree(h, h->b); dfree(h, h->z); dfree(h, h->x64);
+    dfree(h, h->r2); dfree(h, h->p2);
+    dfree(h, h->info); dfree(h, h->type); dfree(h, h->rank);
+    dfree(h, h->partials); dfree(h, h->partials2);
     h->bcap = 0;
 }
 
+int alloc_cells(vare_flip* h, size_t cap);
+
+// A failed allocation leaves no cell arrays (capacity 0), so the next call allocates again.
 int ensure_blocks(vare_flip* h, size_t nb) {
     if (nb <= h->bcap && h->mass) return VARE_OK;
     size_t cap = std::max(nb, h->bcap + h->bcap / 4);
     cap = std::min(cap, (size_t)h->nblk);
     cap = std::max(cap, nb);
     free_cells(h);
+    const int rc = alloc_cells(h, cap);
+    if (rc != VARE_OK) free_cells(h);
+    return rc;
+}
+
+int alloc_cells(vare_flip* h, size_t cap) {
     const size_t nc = cap * (size_t)h->B * h->B * h->B;
     const size_t npart = std::max((size_t)kReduceGrid, cap) + 16;
-    CHK(dalloc(&h->nbr, cap * 6));
-    CHK(dalloc(&h->ccount, nc + 2)); CHK(dalloc(&h->cstart, nc + 2));
-    CHK(dalloc(&h->bfluid, cap + 1)); CHK(dalloc(&h->blist, cap + 1));
-    CHK(dalloc(&h->wgtab, (size_t)wg_table_entries((int)cap) * kTab));
-    CHK(dalloc(&h->mass, nc)); CHK(dalloc(&h->vx, nc)); CHK(dalloc(&h->vy, nc)); CHK(dalloc(&h->vz, nc));
-    CHK(dalloc(&h->cnt, nc));
-    CHK(dalloc(&h->cx, nc)); CHK(dalloc(&h->cy, nc)); CHK(dalloc(&h->cz, nc));
-    CHK(dalloc(&h->x, nc)); CHK(dalloc(&h->r, nc)); CHK(dalloc(&h->p, nc)); CHK(dalloc(&h->ap, nc));
-    CHK(dalloc(&h->r2, nc)); CHK(dalloc(&h->p2, nc));
-    CHK(dalloc(&h->b, nc)); CHK(dalloc(&h->z, nc)); CHK(dalloc(&h->x64, nc));
-    CHK(dalloc(&h->info, nc)); CHK(dalloc(&h->type, nc)); CHK(dalloc(&h->rank, nc));
-    CHK(dalloc(&h->partials, npart)); CHK(dalloc(&h->partials2, npart));
+    CHK(dalloc(h, &h->nbr, cap * 6));
+    CHK(dalloc(h, &h->ccount, nc + 2)); CHK(dalloc(h, &h->cstart, nc + 2));
+    CHK(dalloc(h, &h->bfluid, cap + 1)); CHK(dalloc(h, &h->blist, cap + 1));
+    CHK(dalloc(h, &h->wgtab, (size_t)wg_table_entries((int)cap) * kTab));
+    CHK(dalloc(h, &h->mass, nc)); CHK(dalloc(h, &h->vx, nc)); CHK(dalloc(h, &h->vy, nc)); CHK(dalloc(h, &h->vz, nc));
+    CHK(dalloc(h, &h->cnt, nc));
+    CHK(dalloc(h, &h->cx, nc)); CHK(dalloc(h, &h->cy, nc)); CHK(dalloc(h, &h->cz, nc));
+    CHK(dalloc(h, &h->x, nc)); CHK(dalloc(h, &h->r, nc)); CHK(dalloc(h, &h->p, nc)); CHK(dalloc(h, &h->ap, nc));
+    CHK(dalloc(h, &h->r2, nc)); CHK(dalloc(h, &h->p2, nc));
+    CHK(dalloc(h, &h->b, nc)); CHK(dalloc(h, &h->z, nc)); CHK(dalloc(h, &h->x64, nc));
+    CHK(dalloc(h, &h->info, nc)); CHK(dalloc(h, &h->type, nc)); CHK(dalloc(h, &h->rank, nc));
+    CHK(dalloc(h, &h->partials, npart)); CHK(dalloc(h, &h->partials2, npart));
     h->bcap = cap;
     return VARE_OK;
 }
@@ -267,43 +325,57 @@ int ensure_particles(vare_flip* h, size_t n) {
     if (n > kMaxParticles) return fail(VARE_ENOMEM, "%zu particles exceed the limit of 2^30", n);
     size_t cap = std::max(n, h->pcap + h->pcap / 2);
     cap = std::min(std::max<size_t>(cap, 1024), kMaxParticles);
-    float* na[9] = {};
+    // Every new array is allocated before an old one is released, so a failed allocation leaves the
+    // handle with its old arrays, capacity and particles.
+    struct Arr { void* member; size_t bytes; size_t keep; };     // keep: live bytes to copy over
+    const size_t vs = h->vsz();
+    std::vector<Arr> arrs;
     for (int q = 0; q < 9; ++q) {
-        const size_t esz = (q >= 3 && q < 6) ? h->vsz() : sizeof(float);
-        char* p = nullptr;
-        CHK(dalloc(&p, cap * esz));
-        na[q] = (float*)p;
-        if (h->np) HIPCHK(hipMemcpyAsync(na[q], h->pa[q], h->np * esz, hipMemcpyDeviceToDevice, h->stream));
+        const size_t esz = (q >= 3 && q < 6) ? vs : sizeof(float);
+        arrs.push_back({&h->pa[q], cap * esz, h->np * esz});
     }
-    int* npid = nullptr;
-    CHK(dalloc(&npid, cap));
-    if (h->np) HIPCHK(hipMemcpyAsync(npid, h->pid, h->np * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    for (int q = 0; q < 9; ++q) { dfree(h->pa[q]); h->pa[q] = na[q]; }
-    dfree(h->pid);
-    h->pid = npid;
-    for (int q = 0; q < 7; ++q) dfree(h->pb[q]);
-    dfree(h->pid2); dfree(h->removed); dfree(h->removed_log); dfree(h->hole); dfree(h->src); dfree(h->remap);
-    dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins); dfree(h->midbins);
-    for (int q = 0; q < 7; ++q) {
-        char* p = nullptr;
-        CHK(dalloc(&p, cap * ((q >= 3 && q < 6) ? h->vsz() : sizeof(float))));
-        h->pb[q] = (float*)p;
+    for (int q = 0; q < 7; ++q) arrs.push_back({&h->pb[q], cap * ((q >= 3 && q < 6) ? vs : sizeof(float)), 0});
+    arrs.push_back({&h->pid, cap * sizeof(int), h->np * sizeof(int)});
+    arrs.push_back({&h->pid2, cap * sizeof(int), 0});
+    arrs.push_back({&h->removed, cap, 0});
+    arrs.push_back({&h->removed_log, cap + 4, 0});
+    for (int** m : {&h->hole, &h->src, &h->remap, &h->pkey, &h->prank}) arrs.push_back({m, cap * sizeof(int), 0});
+    arrs.push_back({&h->perm, cap * sizeof(unsigned long long), 0});
+    arrs.push_back({&h->longbins, (cap / (kSortReg + 1) + 2) * sizeof(int), 0});
+    arrs.push_back({&h->midbins, (cap / (kSortReg + 1) + 2) * sizeof(int), 0});
+    std::vector<void*> fresh(arrs.size(), nullptr);
+    auto release = [&]() { for (void*& f : fresh) { dfree_bytes(h, f); f = nullptr; } };
+    for (size_t i = 0; i < arrs.size(); ++i) {
+        const int rc = dalloc_bytes(h, &fresh[i], arrs[i].bytes);
+        if (rc != VARE_OK) { release(); return rc; }
     }
-    CHK(dalloc(&h->pid2, cap));
-    CHK(dalloc(&h->removed, cap)); CHK(dalloc(&h->removed_log, cap + 4));
-    CHK(dalloc(&h->hole, cap)); CHK(dalloc(&h->src, cap)); CHK(dalloc(&h->remap, cap));
-    CHK(dalloc(&h->pkey, cap)); CHK(dalloc(&h->prank, cap)); CHK(dalloc(&h->perm, cap));
-    CHK(dalloc(&h->longbins, cap / (kSortReg + 1) + 2));
-    CHK(dalloc(&h->midbins, cap / (kSortReg + 1) + 2));
+    if (!h->dry && h->np) {
+        for (size_t i = 0; i < arrs.size(); ++i) {
+            if (!arrs[i].keep) continue;
+            void* old;
+            std::memcpy(&old, arrs[i].member, sizeof old);
+            if (hipMemcpyAsync(fresh[i], old, arrs[i].keep, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) {
+                (void)hipStreamSynchronize(h->stream);
+                release();
+                return fail(VARE_EHIP, "particle array copy failed");
+            }
+        }
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { release(); return fail(VARE_EHIP, "particle array copy failed"); }
+    }
+    for (size_t i = 0; i < arrs.size(); ++i) {
+        void* old;
+        std::memcpy(&old, arrs[i].member, sizeof old);
+        dfree_bytes(h, old);
+        std::memcpy(arrs[i].member, &fresh[i], sizeof(void*));
+    }
     h->pcap = cap;
     return VARE_OK;
 }
 
 int ensure_tiles(vare_flip* h, size_t ntiles) {
     if (ntiles <= h->tiles_cap && h->tiles) return VARE_OK;
-    dfree(h->tiles);
-    CHK(dalloc(&h->tiles, ntiles + 16));
+    dfree(h, h->tiles);
+    CHK(dalloc(h, &h->tiles, ntiles + 16));
     h->tiles_cap = ntiles + 16;
     return VARE_OK;
 }
@@ -336,6 +408,7 @@ bool tgrow(vare_flip* h) {
         h->ev_pool.push_back(e);
         h->ev_kid.push_back(-1);
         h->ev_units.push_back(0);
+        h->ev_iter.push_back(-1);
     }
     return true;
 }
@@ -344,6 +417,7 @@ void tbegin(vare_flip* h, int kid) {
     if (!tgrow(h)) return;
     h->ev_kid[h->ev_used] = kid;
     h->ev_units[h->ev_used] = 0;
+    h->ev_iter[h->ev_used] = -1;
     (void)hipEventRecord(h->ev_pool[h->ev_used], h->stream);
     h->ev_used++;
 }
@@ -355,6 +429,7 @@ void tpair(vare_flip* h, int kid, int64_t units, hipEvent_t& a, hipEvent_t& b) {
     if (!(h->timing & (1u << kid)) || !tgrow(h)) return;
     h->ev_kid[h->ev_used] = kid;
     h->ev_units[h->ev_used] = units;
+    h->ev_iter[h->ev_used] = h->cur_iter;
     a = h->ev_pool[h->ev_used++];
     h->ev_kid[h->ev_used] = -2;
     b = h->ev_pool[h->ev_used++];
@@ -366,6 +441,18 @@ void tend(vare_flip* h, int kid) {
     (void)hipEventRecord(h->ev_pool[h->ev_used], h->stream);
     h->ev_used++;
 }
+// After a solve of `iters` iterations: drop the timed launches of iterations enqueued past its end
+// (they return at the device-side done flag).  The iteration that ends the solve runs its A*p and
+// update kernels but not its V-cycle.  Events from index `from` on belong to this solve.
+void tdrop_noop(vare_flip* h, size_t from, int iters) {
+    for (size_t q = from; q + 1 < h->ev_used; q += 2) {
+        const int k = h->ev_iter[q], kid = h->ev_kid[q];
+        if (k < 0 || kid < 0) continue;
+        const bool ran = kid == VARE_KERNEL_MG0 ? k < iters - 1 : k < iters;
+        if (!ran) h->ev_kid[q] = -3;
+    }
+}
+
 int tcollect(vare_flip* h) {
     if (h->ev_used == 0) return VARE_OK;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -580,9 +667,18 @@ template <int B> int stage_system(vare_flip* h) {
 // ---- multigrid preconditioner (VARE_PRECOND_MG) ----
 // Levels: sparse edges B, B/2, ..., 1 over the active blocks, then dense halvings of the block
 // frame until a level has at most kMgCoarseMax cells.  Arrays are sized for the block capacity.
+int alloc_mg(vare_flip* h);
+
+// A failed allocation leaves no levels, so the next call allocates again.
 int ensure_mg(vare_flip* h) {
     if (!h->mg.empty() && h->mg_bcap == h->bcap) return VARE_OK;
     free_mg(h);
+    const int rc = alloc_mg(h);
+    if (rc != VARE_OK) free_mg(h);
+    return rc;
+}
+
+int alloc_mg(vare_flip* h) {
     const int sh0 = h->B == 8 ? 3 : 2;
     const size_t cap = h->bcap;
     MgLevelK L0{};
@@ -590,15 +686,23 @@ int ensure_mg(vare_flip* h) {
     h->mg.push_back(L0);
     h->mg_z.push_back(nullptr);
     h->mg_zt.push_back(nullptr);
-    CHK(dalloc(&h->zt0, cap << (3 * sh0)));
+    CHK(dalloc(h, &h->zt0, cap << (3 * sh0)));
+    // a level joins the list before its arrays are allocated, so free_mg releases a partial level
+    auto add_level = [&](const MgLevelK& L, size_t n) -> int {
+        h->mg.push_back(L);
+        h->mg_z.push_back(nullptr);
+        h->mg_zt.push_back(nullptr);
+        MgLevelK& M = h->mg.back();
+        M.r = nullptr;
+        M.info = nullptr;
+        CHK(dalloc(h, &M.r, n)); CHK(dalloc(h, &M.info, n));
+        CHK(dalloc(h, &h->mg_z.back(), n)); CHK(dalloc(h, &h->mg_zt.back(), n));
+        return VARE_OK;
+    };
     for (int sh = sh0 - 1; sh >= 0; --sh) {
         MgLevelK L{};
         L.dense = 0; L.sh = sh; L.dx = h->fx; L.dy = h->fy; L.dz = h->fz; L.scale_sh = sh0 - sh;
-        const size_t n = cap << (3 * sh);
-        CHK(dalloc(&L.r, n)); CHK(dalloc(&L.info, n));
-        float *z = nullptr, *zt = nullptr;
-        CHK(dalloc(&z, n)); CHK(dalloc(&zt, n));
-        h->mg.push_back(L); h->mg_z.push_back(z); h->mg_zt.push_back(zt);
+        CHK(add_level(L, cap << (3 * sh)));
     }
     int dx = h->fx, dy = h->fy, dz = h->fz, ssh = sh0;
     do {
@@ -606,10 +710,7 @@ int ensure_mg(vare_flip* h) {
         MgLevelK L{};
         L.dense = 1; L.sh = 0; L.dx = dx; L.dy = dy; L.dz = dz; L.scale_sh = ssh;
         L.ncell = dx * dy * dz;
-        CHK(dalloc(&L.r, (size_t)L.ncell)); CHK(dalloc(&L.info, (size_t)L.ncell));
-        float *z = nullptr, *zt = nullptr;
-        CHK(dalloc(&z, (size_t)L.ncell)); CHK(dalloc(&zt, (size_t)L.ncell));
-        h->mg.push_back(L); h->mg_z.push_back(z); h->mg_zt.push_back(zt);
+        CHK(add_level(L, (size_t)L.ncell));
     } while ((size_t)dx * dy * dz > (size_t)kMgCoarseMax);
     h->mg_bcap = h->bcap;
     return VARE_OK;
@@ -832,7 +933,13 @@ void enqueue_vcycle(vare_flip* h, double* rz, float* r0, bool skip_first, XTerm 
 //   * With the multigrid preconditioner and two sweeps, the x, r update is fused with the V-cycle's
 //     first level-0 pass (k_update_smooth2), and r alternates too: the residual before iteration k
 //     is in the buffer of parity k (r, r2).  Otherwise r is updated in place by k_cg_update.
+template <int B> void enqueue_iteration_k(vare_flip* h, int k);
 template <int B> void enqueue_iteration(vare_flip* h, int k) {
+    h->cur_iter = k;
+    enqueue_iteration_k<B>(h, k);
+    h->cur_iter = -1;
+}
+template <int B> void enqueue_iteration_k(vare_flip* h, int k) {
     const int nv = h->nvirt();
     if (!nv) return;
     const int nwg_it = nwg_k(h, kIterK);                   // fused CG passes
@@ -905,6 +1012,7 @@ template <int B> void enqueue_iteration(vare_flip* h, int k) {
 // past convergence return at once.
 template <int B> int run_solver(vare_flip* h, int cap, int k, int& total_iters) {
     int& last = h->last_iters[std::min(k, 19)];
+    const size_t ev_from = h->ev_used;
     int launched = 0;
     int chunk = last > 0 ? last : 8;
     for (;;) {
@@ -919,6 +1027,7 @@ template <int B> int run_solver(vare_flip* h, int cap, int k, int& total_iters) 
     }
     last = h->st_host->count;
     total_iters += h->st_host->count;
+    tdrop_noop(h, ev_from, h->st_host->iter);
     return VARE_OK;
 }
 
@@ -1145,6 +1254,63 @@ int validate_params(const vare_flip_params* p) {
     return VARE_OK;
 }
 
+// SparseFrame::initialize (SparseFrame.cu:72-117) + Grid::initialize (Grid.h:123-138): pad to a block
+// multiple, isotropic cell size h = min(dx, dy, dz), recentred bounding box.
+int init_geometry(vare_flip* h, const vare_grid_desc* g) {
+    const int B = g->block_size;
+    h->B = B;
+    h->fx = (int)(g->nx / B + ((g->nx % B) ? 1 : 0));
+    h->fy = (int)(g->ny / B + ((g->ny % B) ? 1 : 0));
+    h->fz = (int)(g->nz / B + ((g->nz % B) ? 1 : 0));
+    const int64_t nblk = (int64_t)h->fx * h->fy * h->fz;
+    const int64_t ncell = nblk * B * B * B;
+    if (ncell > (int64_t)0x7fffffff) return fail(VARE_EINVAL, "frame of %lld cells exceeds 2^31-1", (long long)ncell);
+    h->nblk = (int)nblk;
+    h->nx = h->fx * B; h->ny = h->fy * B; h->nz = h->fz * B;
+    float cen[3], d[3];
+    for (int a = 0; a < 3; ++a) cen[a] = (g->min[a] + g->max[a]) * 0.5f;
+    d[0] = (g->max[0] - g->min[0]) / (float)g->nx;
+    d[1] = (g->max[1] - g->min[1]) / (float)g->ny;
+    d[2] = (g->max[2] - g->min[2]) / (float)g->nz;
+    const float hh = std::min(std::min(d[0], d[1]), d[2]);
+    const int n3[3] = {h->nx, h->ny, h->nz};
+    for (int a = 0; a < 3; ++a) {
+        const float l = (float)n3[a] * hh;
+        const float mn = cen[a] - l * 0.5f;
+        const float mx = mn + l;
+        float L = mx - mn;
+        if (a == 2 && n3[2] == 1) L = 1.f;     // Grid.h:131
+        h->t[a] = mn;
+        h->s[a] = L / (float)n3[a];
+    }
+    return VARE_OK;
+}
+
+// Frame tables and solver scalars, allocated once per handle.
+int alloc_frame(vare_flip* h) {
+    CHK(dalloc(h, &h->flag, (size_t)h->nblk));
+    CHK(dalloc(h, &h->occ, h->nocc()));
+    CHK(dalloc(h, &h->slot, (size_t)h->nblk));
+    CHK(dalloc(h, &h->coord, (size_t)h->nblk));
+    CHK(dalloc(h, &h->dint, 8));
+    CHK(dalloc(h, &h->st, 1));
+    return VARE_OK;
+}
+
+// Block capacity reserved at create: the caller's, or the whole frame up to 64M cells.
+size_t initial_block_capacity(const vare_flip* h, size_t bcap) {
+    const size_t B3 = (size_t)h->B * h->B * h->B;
+    const size_t bc = bcap ? std::min(bcap, (size_t)h->nblk) : std::min((size_t)h->nblk, (size_t)(64u << 20) / B3);
+    return std::max<size_t>(bc, 1);
+}
+
+// Scan tiles the longest scan of a step needs: over the frame's blocks, the bins (cells + 2) or the
+// particles (removal compaction).
+size_t step_scan_tiles(const vare_flip* h, size_t np) {
+    const size_t n = std::max(std::max((size_t)h->nblk, h->bcap * h->B * h->B * h->B + 2), np);
+    return (n + SCAN_TILE - 1) / SCAN_TILE;
+}
+
 void apply_params(vare_flip* h, const vare_flip_params* p) {
     h->prm = *p;
     for (int a = 0; a < 3; ++a) h->vox_force[a] = p->ext_force[a] / h->s[a];   // FLIPMethod.cu:42
@@ -1211,52 +1377,17 @@ int vare_flip_create(const vare_grid_desc* g, const vare_flip_params* prm, size_
     CHK(validate_params(p));
     vare_flip* h = new vare_flip();
     std::memset(&h->stats, 0, sizeof h->stats);
-    const int B = g->block_size;
-    h->B = B;
-    // SparseFrame::initialize (SparseFrame.cu:72-117)
-    h->fx = (int)(g->nx / B + ((g->nx % B) ? 1 : 0));
-    h->fy = (int)(g->ny / B + ((g->ny % B) ? 1 : 0));
-    h->fz = (int)(g->nz / B + ((g->nz % B) ? 1 : 0));
-    const int64_t nblk = (int64_t)h->fx * h->fy * h->fz;
-    const int64_t ncell = nblk * B * B * B;
-    if (ncell > (int64_t)0x7fffffff) { delete h; return fail(VARE_EINVAL, "frame of %lld cells exceeds 2^31-1", (long long)ncell); }
-    h->nblk = (int)nblk;
-    h->nx = h->fx * B; h->ny = h->fy * B; h->nz = h->fz * B;
-    float cen[3], d[3];
-    for (int a = 0; a < 3; ++a) cen[a] = (g->min[a] + g->max[a]) * 0.5f;
-    d[0] = (g->max[0] - g->min[0]) / (float)g->nx;
-    d[1] = (g->max[1] - g->min[1]) / (float)g->ny;
-    d[2] = (g->max[2] - g->min[2]) / (float)g->nz;
-    const float hh = std::min(std::min(d[0], d[1]), d[2]);
-    const int n3[3] = {h->nx, h->ny, h->nz};
-    for (int a = 0; a < 3; ++a) {
-        const float l = (float)n3[a] * hh;
-        const float mn = cen[a] - l * 0.5f;
-        const float mx = mn + l;
-        float L = mx - mn;
-        if (a == 2 && n3[2] == 1) L = 1.f;     // Grid.h:131
-        h->t[a] = mn;
-        h->s[a] = L / (float)n3[a];
-    }
+    if (int rc0 = init_geometry(h, g)) { delete h; return rc0; }
     apply_params(h, p);
     h->vhalf = p->velocity_fp16 ? 1 : 0;
     if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(VARE_EHIP, "hipStreamCreate failed"); }
-    int rc = VARE_OK;
-    if (rc == VARE_OK) rc = dalloc(&h->flag, (size_t)h->nblk);
-    if (rc == VARE_OK) rc = dalloc(&h->occ, h->nocc());
-    if (rc == VARE_OK) rc = dalloc(&h->slot, (size_t)h->nblk);
-    if (rc == VARE_OK) rc = dalloc(&h->coord, (size_t)h->nblk);
-    if (rc == VARE_OK) rc = dalloc(&h->dint, 8);
-    if (rc == VARE_OK) rc = dalloc(&h->st, 1);
+    int rc = alloc_frame(h);
     if (rc == VARE_OK && hipHostMalloc((void**)&h->st_host, sizeof(CgState), 0) != hipSuccess) rc = fail(VARE_EHIP, "hipHostMalloc failed");
     if (rc == VARE_OK && hipHostMalloc((void**)&h->dint_host, 8 * sizeof(int), 0) != hipSuccess) rc = fail(VARE_EHIP, "hipHostMalloc failed");
     for (int q = 0; q < 4 && rc == VARE_OK; ++q)
         if (hipEventCreate(&h->step_ev[q]) != hipSuccess) rc = fail(VARE_EHIP, "hipEventCreate failed");
     if (rc == VARE_OK && pcap) rc = ensure_particles(h, pcap);
-    if (rc == VARE_OK) {
-        size_t bc = bcap ? std::min(bcap, (size_t)h->nblk) : std::min((size_t)h->nblk, (size_t)((64u << 20) / (B * B * B)));
-        rc = ensure_blocks(h, std::max<size_t>(bc, 1));
-    }
+    if (rc == VARE_OK) rc = ensure_blocks(h, initial_block_capacity(h, bcap));
     if (rc == VARE_OK) {
         rc = fill_u32(h, h->slot, 0xffffffffu, (size_t)h->nblk);
         if (rc == VARE_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(VARE_EHIP, "init sync failed");
@@ -1266,17 +1397,60 @@ int vare_flip_create(const vare_grid_desc* g, const vare_flip_params* prm, size_
     return VARE_OK;
 }
 
+int vare_flip_plan_bytes(const vare_grid_desc* g, const vare_flip_params* prm, size_t pcap, size_t bcap,
+                         size_t* bytes) {
+    if (!g || !bytes) return fail(VARE_EINVAL, "null argument");
+    *bytes = 0;
+    if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0) return fail(VARE_EINVAL, "grid has zero cells");
+    if (g->block_size != 4 && g->block_size != 8) return fail(VARE_EINVAL, "block_size must be 4 or 8");
+    vare_flip_params def;
+    vare_flip_params_default(&def);
+    const vare_flip_params* p = prm ? prm : &def;
+    CHK(validate_params(p));
+    // the allocations of vare_flip_create and of the first steps, run on a planning handle that
+    // counts bytes and allocates nothing
+    vare_flip plan;
+    plan.dry = 1;
+    CHK(init_geometry(&plan, g));
+    apply_params(&plan, p);
+    plan.vhalf = p->velocity_fp16 ? 1 : 0;
+    CHK(alloc_frame(&plan));
+    if (pcap) CHK(ensure_particles(&plan, pcap));
+    CHK(ensure_blocks(&plan, initial_block_capacity(&plan, bcap)));
+    if (p->solver_mode == VARE_SOLVER_RELATIVE && p->preconditioner == VARE_PRECOND_MG) CHK(ensure_mg(&plan));
+    CHK(ensure_tiles(&plan, step_scan_tiles(&plan, plan.pcap)));
+    // staging of a host upload (3 floats per particle) or download (7 floats per particle)
+    float* staging = nullptr;
+    CHK(dalloc(&plan, &staging, 7 * std::max<size_t>(plan.pcap, 1)));
+    dfree(&plan, staging);
+    *bytes = plan.peak_bytes;
+    return VARE_OK;
+}
+
+int vare_flip_get_device_bytes(const vare_flip* h, size_t* live, size_t* peak) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (live) *live = h->dev_bytes;
+    if (peak) *peak = h->peak_bytes;
+    return VARE_OK;
+}
+
+int vare_debug_inject_alloc_failure(long skip, long count) {
+    g_alloc_fail.store(count > 0 ? count : 0);
+    g_alloc_skip.store(count > 0 ? std::max(skip, 0L) : -1);
+    return VARE_OK;
+}
+
 int vare_flip_destroy(vare_flip* h) {
     if (!h) return VARE_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    for (int q = 0; q < 9; ++q) dfree(h->pa[q]);
-    for (int q = 0; q < 7; ++q) dfree(h->pb[q]);
-    dfree(h->pid); dfree(h->pid2); dfree(h->removed); dfree(h->removed_log); dfree(h->hole); dfree(h->src);
-    dfree(h->remap); dfree(h->pkey); dfree(h->prank); dfree(h->perm); dfree(h->longbins); dfree(h->midbins);
-    dfree(h->flag); dfree(h->occ); dfree(h->slot); dfree(h->coord);
+    for (int q = 0; q < 9; ++q) dfree(h, h->pa[q]);
+    for (int q = 0; q < 7; ++q) dfree(h, h->pb[q]);
+    dfree(h, h->pid); dfree(h, h->pid2); dfree(h, h->removed); dfree(h, h->removed_log); dfree(h, h->hole); dfree(h, h->src);
+    dfree(h, h->remap); dfree(h, h->pkey); dfree(h, h->prank); dfree(h, h->perm); dfree(h, h->longbins); dfree(h, h->midbins);
+    dfree(h, h->flag); dfree(h, h->occ); dfree(h, h->slot); dfree(h, h->coord);
     free_cells(h);
     free_collision(h);
-    dfree(h->tiles); dfree(h->dint); dfree(h->st);
+    dfree(h, h->tiles); dfree(h, h->dint); dfree(h, h->st);
     if (h->st_host) (void)hipHostFree(h->st_host);
     if (h->dint_host) (void)hipHostFree(h->dint_host);
     for (auto e : h->ev_pool) (void)hipEventDestroy(e);
@@ -1316,10 +1490,10 @@ static int upload_particles(vare_flip* h, size_t off, const float* pos, const fl
             for (size_t i = 0; i < n; ++i) tmp[i] = pos[3 * i + a];
             HIPCHK(hipMemcpy(h->pa[a] + off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
         }
-        CHK(dalloc(&dv, 3 * n));
+        CHK(dalloc(h, &dv, 3 * n));
         int rc = hipMemcpy(dv, vel, 3 * n * sizeof(float), hipMemcpyHostToDevice) == hipSuccess
                      ? VARE_OK : fail(VARE_EHIP, "velocity upload failed");
-        if (rc != VARE_OK) { dfree(dv); return rc; }
+        if (rc != VARE_OK) { dfree(h, dv); return rc; }
         const float* sc[3] = {adj, vort, curv};
         for (int q = 0; q < 3; ++q) {
             if (sc[q]) HIPCHK(hipMemcpy(h->pa[6 + q] + off, sc[q], n * sizeof(float), hipMemcpyHostToDevice));
@@ -1343,7 +1517,7 @@ static int upload_particles(vare_flip* h, size_t off, const float* pos, const fl
                            h->t[1], h->t[2], h->s[0], h->s[1], h->s[2]);
     const hipError_t e = hipGetLastError();
     const hipError_t e2 = hipStreamSynchronize(h->stream);
-    dfree(dv);
+    dfree(h, dv);
     if (e != hipSuccess || e2 != hipSuccess) return fail(VARE_EHIP, "particle upload failed");
     return VARE_OK;
 }
@@ -1378,7 +1552,7 @@ int vare_flip_get_particles(vare_flip* h, float* pos, float* vel, float* adj, fl
     float* lg[2] = {vort, curv};                  // stored in the reference order already
     if (mem == VARE_MEM_HOST) {
         float* d = nullptr;
-        CHK(dalloc(&d, 7 * np));
+        CHK(dalloc(h, &d, 7 * np));
         hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(),
                            (const int*)h->pid, world, h->t[0], h->t[1], h->t[2], h->s[0], h->s[1], h->s[2],
                            pos ? d : nullptr, vel ? d + 3 * np : nullptr, adj ? d + 6 * np : nullptr);
@@ -1392,7 +1566,7 @@ int vare_flip_get_particles(vare_flip* h, float* pos, float* vel, float* adj, fl
         for (int q = 0; q < 2 && rc == VARE_OK; ++q)
             if (lg[q] && hipMemcpy(lg[q], h->pa[7 + q], np * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
                 rc = fail(VARE_EHIP, "particle download failed");
-        dfree(d);
+        dfree(h, d);
         return rc;
     }
     hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(), (const int*)h->pid,
@@ -1530,13 +1704,13 @@ int vare_flip_get_cell_field(vare_flip* h, int which, void* out, size_t cap_byte
         OpFluid op{h->type, h->rank};
         CHK(run_scan(h, op, (int)nc, 2));
         unsigned long long* tmp = nullptr;
-        CHK(dalloc(&tmp, nc));
+        CHK(dalloc(h, &tmp, nc));
         hipLaunchKernelGGL(k_mark_u64, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, (const uint8_t*)h->type,
                            (const int*)h->rank, tmp);
         int rc = VARE_OK;
         if (hipStreamSynchronize(h->stream) != hipSuccess || hipMemcpy(out, tmp, nc * 8, hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(VARE_EHIP, "mark download failed");
-        dfree(tmp);
+        dfree(h, tmp);
         return rc;
     }
     return fail(VARE_EINVAL, "bad field id %d", which);
@@ -1678,9 +1852,9 @@ int build_collision(vare_flip* h, const float* seeds, size_t nseed, float bg_wor
     free_collision(h);
     h->col_bg = bg_world / hv;                                   // CollisionSource.h:90
     if (nbc == 0) return VARE_OK;                                // size()==0: every query returns background
-    CHK(dalloc(&h->col_slot, (size_t)h->nblk));
-    CHK(dalloc(&h->col_phi, nc));
-    for (int a = 0; a < 3; ++a) { CHK(dalloc(&h->col_v[a], nc)); CHK(dalloc(&h->col_n[a], nc)); }
+    CHK(dalloc(h, &h->col_slot, (size_t)h->nblk));
+    CHK(dalloc(h, &h->col_phi, nc));
+    for (int a = 0; a < 3; ++a) { CHK(dalloc(h, &h->col_v[a], nc)); CHK(dalloc(h, &h->col_n[a], nc)); }
     HIPCHK(hipMemcpy(h->col_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->col_phi, phi.data(), nc * sizeof(float), hipMemcpyHostToDevice));
     std::vector<float> tmp(nc);

@@ -10,8 +10,15 @@
 // Handle pool: a finished tank's handle (device state sized for its grid) is kept, not destroyed, and
 // the next request with the same grid and velocity storage takes it over, so serving does not
 // allocate or free device memory per request (hipFree waits for the whole device, which would stall
-// every other resident tank).  Pooled handles keep their bytes counted against the budget; when a
-// request does not fit, pooled handles are destroyed, least recently used first, until it does.
+// every other resident tank).  Pooled handles keep their live bytes counted against the budget; when
+// the head request does not fit and destroying pooled handles alone makes room, they are destroyed,
+// least recently used first, outside the server lock (their bytes stay counted until they are gone).
+//
+// Accounting: a request is charged the bytes vare_flip_plan_bytes reports for its grid, particle
+// count and parameters (the exact arrays the library allocates for create and the first steps, plus
+// the staging of the particle upload and download).  A pooled handle serves only requests whose plan
+// is within its footprint bound (the largest plan it has served), so it never grows, and such a
+// request is charged that bound.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -31,22 +38,21 @@ struct Request {
     vare_tank_request req;
     std::vector<float> pos, vel;
     int64_t id = 0;
-    size_t bytes = 0;
+    size_t bytes = 0;                 // planned bytes of the request (vare_flip_plan_bytes)
+    size_t charge = 0;                // bytes counted against the budget while it runs
     std::vector<float> out_pos, out_vel;
     std::vector<double> step_ms;
     vare_request_stats st;
     bool done = false;
 };
 
+// Device bytes a request reserves: what vare_flip_create and the first steps of its tank allocate,
+// plus the staging of the particle upload and download (vare_flip_plan_bytes counts exactly the
+// arrays the library allocates).  0 when the request is invalid.
 size_t request_bytes(const vare_tank_request& r) {
-    const int64_t B = r.grid.block_size;
-    if (B <= 0) return 0;
-    const int64_t fx = (r.grid.nx + B - 1) / B, fy = (r.grid.ny + B - 1) / B, fz = (r.grid.nz + B - 1) / B;
-    const int64_t nblk = fx * fy * fz;
-    // particles: 9 float arrays + removal/binning scratch; cells: 14 float fields, FP64 pressure,
-    // descriptors and multigrid levels; frame tables; fixed slack for scan tiles and reductions
-    const int64_t per_particle = 64, per_cell = 84, per_block = 64;
-    return (size_t)(r.n * per_particle + nblk * B * B * B * per_cell + nblk * per_block + (4 << 20));
+    size_t b = 0;
+    if (vare_flip_plan_bytes(&r.grid, &r.params, r.n, 0, &b) != VARE_OK) return 0;
+    return b;
 }
 
 }  // namespace
@@ -55,7 +61,8 @@ struct Pooled {
     vare_flip* h = nullptr;
     vare_grid_desc grid;
     int fp16 = 0;
-    size_t bytes = 0;
+    size_t bytes = 0;                 // live device bytes of the idle handle
+    size_t plan = 0;                  // the largest planned bytes of a request it served: its footprint bound
     uint64_t last_use = 0;
 };
 
@@ -75,7 +82,8 @@ struct vare_server {
     std::vector<std::unique_ptr<Request>> reqs;
     std::mutex mu;
     std::condition_variable cv;
-    size_t used = 0;
+    size_t used = 0;                  // resident charges + pooled handles + handles being destroyed
+    size_t pending = 0;               // bytes of handles being destroyed
     size_t peak = 0;
     int resident = 0;
     int max_resident = 0;
@@ -91,38 +99,16 @@ double since(const Clock::time_point& t0) {
     return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
-// Take the pooled handle for this request's grid out of the pool (h == nullptr if there is none).
-// Called with s->mu held; its bytes leave `used` with it.
-Pooled take_pooled(vare_server* s, Request* q) {
+// Index of a pooled handle that can serve this request without growing, or -1: the same grid and
+// velocity storage, and a request whose planned bytes are within the handle's footprint bound (a
+// handle's device bytes never exceed the plan of the largest request it has served, so such a request
+// allocates nothing new).  Called with s->mu held.
+long find_pooled(const vare_server* s, const Request* q) {
     const int fp16 = q->req.params.velocity_fp16 ? 1 : 0;
-    for (size_t i = 0; i < s->pool.size(); ++i) {
-        if (s->pool[i].fp16 != fp16 || !same_grid(s->pool[i].grid, q->req.grid)) continue;
-        Pooled pd = s->pool[i];
-        s->used -= pd.bytes;
-        s->pool.erase(s->pool.begin() + (long)i);
-        return pd;
-    }
-    return Pooled();
-}
-
-// Put a handle back into the pool.  Called with s->mu held.
-void put_pooled(vare_server* s, const Pooled& pd) {
-    s->pool.push_back(pd);
-    s->used += pd.bytes;
-    s->peak = std::max(s->peak, s->used);
-}
-
-// Destroy pooled handles, least recently used first, until `need` more bytes fit in the budget.
-// Called with s->mu held.
-void evict_for(vare_server* s, size_t need) {
-    while (!s->pool.empty() && s->used + need > s->budget) {
-        size_t lru = 0;
-        for (size_t i = 1; i < s->pool.size(); ++i)
-            if (s->pool[i].last_use < s->pool[lru].last_use) lru = i;
-        vare_flip_destroy(s->pool[lru].h);
-        s->used -= s->pool[lru].bytes;
-        s->pool.erase(s->pool.begin() + (long)lru);
-    }
+    for (size_t i = 0; i < s->pool.size(); ++i)
+        if (s->pool[i].fp16 == fp16 && same_grid(s->pool[i].grid, q->req.grid) && q->bytes <= s->pool[i].plan)
+            return (long)i;
+    return -1;
 }
 
 // Runs the request on h (a pooled handle) or on a new handle; returns the handle to pool afterwards
@@ -169,9 +155,24 @@ void worker(vare_server* s) {
     for (;;) {
         Request* q = nullptr;
         vare_flip* h = nullptr;
+        std::vector<vare_flip*> victims;          // pooled handles to destroy outside the lock
+        size_t freed = 0;                         // their bytes
+        size_t q_plan = 0;                        // footprint bound of the handle the request runs on
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
+                if (!victims.empty()) {
+                    // destroy evicted handles without holding the lock (hipFree waits for the
+                    // device); their bytes stay counted until they are gone
+                    lk.unlock();
+                    for (vare_flip* v : victims) vare_flip_destroy(v);
+                    lk.lock();
+                    s->used -= std::min(freed, s->used);
+                    s->pending -= std::min(freed, s->pending);
+                    victims.clear();
+                    freed = 0;
+                    s->cv.notify_all();
+                }
                 if (s->next >= s->order.size()) return;
                 Request* head = s->reqs[s->order[s->next]].get();
                 const double now = since(s->t0);
@@ -179,19 +180,49 @@ void worker(vare_server* s) {
                     s->cv.wait_for(lk, std::chrono::duration<double>(head->req.arrival_s - now));
                     continue;
                 }
-                const Pooled pd = take_pooled(s, head);
-                evict_for(s, head->bytes);
-                if (s->used + head->bytes > s->budget) {   // FIFO: wait for resident tanks to finish
-                    if (pd.h) put_pooled(s, pd);
-                    s->cv.wait(lk);
+                const long m = find_pooled(s, head);
+                const size_t mine = m >= 0 ? s->pool[(size_t)m].bytes : 0;        // already in `used`
+                const size_t need = m >= 0 ? s->pool[(size_t)m].plan : head->bytes;
+                const size_t base = s->used - mine;
+                if (base + need > s->budget) {
+                    // evict only when destroying pooled handles alone makes room (FIFO: otherwise
+                    // wait for resident tanks to finish)
+                    size_t evictable = 0;
+                    for (size_t i = 0; i < s->pool.size(); ++i)
+                        if ((long)i != m) evictable += s->pool[i].bytes;
+                    if (base - std::min(evictable, base) + need > s->budget) {
+                        s->cv.wait(lk);
+                        continue;
+                    }
+                    size_t room = base;
+                    while (room + need > s->budget) {
+                        size_t lru = s->pool.size();
+                        for (size_t i = 0; i < s->pool.size(); ++i) {
+                            if (find_pooled(s, head) == (long)i) continue;
+                            if (lru == s->pool.size() || s->pool[i].last_use < s->pool[lru].last_use) lru = i;
+                        }
+                        if (lru == s->pool.size()) break;
+                        room -= s->pool[lru].bytes;
+                        freed += s->pool[lru].bytes;
+                        s->pending += s->pool[lru].bytes;
+                        victims.push_back(s->pool[lru].h);
+                        s->pool.erase(s->pool.begin() + (long)lru);
+                    }
                     continue;
                 }
-                h = pd.h;
-                if (h) s->handles_reused++;
-                else s->handles_created++;
+                const long m2 = find_pooled(s, head);
+                if (m2 >= 0) {
+                    h = s->pool[(size_t)m2].h;
+                    q_plan = s->pool[(size_t)m2].plan;
+                    s->pool.erase(s->pool.begin() + m2);
+                    s->handles_reused++;
+                } else {
+                    s->handles_created++;
+                }
                 q = head;
+                q->charge = need;
                 s->next++;
-                s->used += q->bytes;
+                s->used = base + need;
                 s->peak = std::max(s->peak, s->used);
                 s->resident++;
                 s->max_resident = std::max(s->max_resident, s->resident);
@@ -199,22 +230,36 @@ void worker(vare_server* s) {
             }
         }
         h = run_request(s, q, h);
+        vare_flip* drop = nullptr;
+        size_t live = 0;
+        if (h) vare_flip_get_device_bytes(h, &live, nullptr);
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->used -= q->bytes;
-            if (h && (int)s->pool.size() < s->max_slots) {      // the request's bytes pass to the pool
+            s->used -= q->charge;
+            if (h && (int)s->pool.size() < s->max_slots) {      // the handle's live bytes pass to the pool
                 Pooled pd;
                 pd.h = h;
                 pd.grid = q->req.grid;
                 pd.fp16 = q->req.params.velocity_fp16 ? 1 : 0;
-                pd.bytes = q->bytes;
+                pd.bytes = live;
+                pd.plan = std::max(q_plan, q->bytes);
                 pd.last_use = ++s->use_clock;
-                put_pooled(s, pd);
+                s->pool.push_back(pd);
+                s->used += live;
+                s->peak = std::max(s->peak, s->used);
             } else if (h) {
-                vare_flip_destroy(h);
+                drop = h;                                     // destroyed below, its bytes counted until then
+                s->used += live;
+                s->pending += live;
             }
             s->resident--;
             q->done = true;
+        }
+        if (drop) {
+            vare_flip_destroy(drop);
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->used -= std::min(live, s->used);
+            s->pending -= std::min(live, s->pending);
         }
         s->cv.notify_all();
     }
@@ -227,7 +272,7 @@ extern "C" {
 int vare_server_request_bytes(const vare_tank_request* req, size_t* bytes) {
     if (!req || !bytes) return VARE_EINVAL;
     *bytes = request_bytes(*req);
-    return VARE_OK;
+    return *bytes ? VARE_OK : VARE_EINVAL;
 }
 
 int vare_server_create(size_t budget, int max_slots, vare_server** out) {
@@ -255,6 +300,7 @@ int vare_server_submit(vare_server* s, const vare_tank_request* req, int64_t* id
     q->req.pos = nullptr;
     q->req.vel = nullptr;
     q->bytes = request_bytes(*req);
+    if (q->bytes == 0) return VARE_EINVAL;
     std::memset(&q->st, 0, sizeof q->st);
     q->st.arrival_s = req->arrival_s;
     q->st.bytes_reserved = (int64_t)q->bytes;
@@ -276,6 +322,7 @@ int vare_server_run(vare_server* s) {
     s->handles_created = 0;
     s->handles_reused = 0;
     s->used = 0;
+    s->pending = 0;
     for (const Pooled& pd : s->pool) s->used += pd.bytes;
     s->peak = s->used;
     s->resident = 0;
