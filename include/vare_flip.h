@@ -178,6 +178,19 @@ int vare_flip_particles_to_world(vare_flip* h);
 /* FLIPMethod::advanceOneStep(pts, dt) (FLIPMethod.cu:208-298), voxel space, including
  * prepareNextStep (:49-52). */
 int vare_flip_step(vare_flip* h, float dt);
+
+/* vare_flip_step without any device allocation, with bit-identical results.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The step's scratch -- the reductions'
+ * per-workgroup partials and the scans' tile sums -- comes from one static device array of
+ * VARE_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends and shared by every call on that
+ * GPU: calls that overlap on one GPU race.  Nothing in it has to start at zero.  The handle is never
+ * grown: VARE_ENOMEM when the active blocks outgrow the block capacity reserved at create, or when
+ * the step needs more scratch than the array holds (16 bytes per two blocks of block capacity, at
+ * least 32 KiB, plus 4 bytes per 4096 cells or particles); VARE_EINVAL while kernel timing is on.
+ * The multigrid levels are allocated by vare_flip_create / vare_flip_set_params. */
+#define VARE_NOALLOC_SCRATCH_BYTES (4u << 20)
+int vare_flip_step_noalloc(vare_flip* h, float dt);
 /* FLIPMethod::advanceOneFrame (FLIPMethod.cu:177-206) minus postProcess (surfacing, curl,
  * vorticity and curvature are outside the built path): toVoxel, step(params.dt), toWorld. */
 int vare_flip_advance_frame(vare_flip* h);

@@ -142,6 +142,7 @@ SIGNATURES = {
     "vare_flip_particles_to_voxel": (C.c_int, [_P]),
     "vare_flip_particles_to_world": (C.c_int, [_P]),
     "vare_flip_step": (C.c_int, [_P, C.c_float]),
+    "vare_flip_step_noalloc": (C.c_int, [_P, C.c_float]),
     "vare_flip_advance_frame": (C.c_int, [_P]),
     "vare_flip_stage": (C.c_int, [_P, C.c_int, C.c_float]),
     "vare_flip_get_stats": (C.c_int, [_P, C.POINTER(StepStats)]),
@@ -321,6 +322,13 @@ class FlipHandle:
 
     def step_rc(self, dt):
         return self.lib.vare_flip_step(self.h, float(dt))
+
+    def step_noalloc(self, dt):
+        """vare_flip_step_noalloc: the same step with no device allocation (static scratch)."""
+        check(self.lib.vare_flip_step_noalloc(self.h, float(dt)))
+
+    def step_noalloc_rc(self, dt):
+        return self.lib.vare_flip_step_noalloc(self.h, float(dt))
 
     def stage(self, s, dt=0.0):
         check(self.lib.vare_flip_stage(self.h, int(s), float(dt)))

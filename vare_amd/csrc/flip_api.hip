@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <thread>
 #include <atomic>
+#include <mutex>
 #include <unordered_map>
 
 #include "../../include/vare_flip.h"
@@ -73,6 +74,8 @@ struct vare_flip {
     float vox_force[3] = {0, 0, 0};
     float vox_wall = 0.f;
     hipStream_t stream = nullptr;
+    int device = 0;               // the device current at create (its stream's device)
+    int noalloc = 0;              // inside vare_flip_step_noalloc: scratch from the static array, no growth
 
     // particles, SoA: px,py,pz,vx,vy,vz,adj in physical order (bin order after each binning),
     // vort,curv in the reference (logical) order; pid[q] = logical index of physical particle q
@@ -290,6 +293,9 @@ int alloc_cells(vare_flip* h, size_t cap);
 // A failed allocation leaves no cell arrays (capacity 0), so the next call allocates again.
 int ensure_blocks(vare_flip* h, size_t nb) {
     if (nb <= h->bcap && h->mass) return VARE_OK;
+    if (h->noalloc)
+        return fail(VARE_ENOMEM, "%zu active blocks exceed the block capacity %zu (the no-allocation step never grows it)",
+                    nb, h->bcap);
     size_t cap = std::max(nb, h->bcap + h->bcap / 4);
     cap = std::min(cap, (size_t)h->nblk);
     cap = std::max(cap, nb);
@@ -374,6 +380,7 @@ int ensure_particles(vare_flip* h, size_t n) {
 
 int ensure_tiles(vare_flip* h, size_t ntiles) {
     if (ntiles <= h->tiles_cap && h->tiles) return VARE_OK;
+    if (h->noalloc) return fail(VARE_ENOMEM, "a scan of %zu tiles exceeds the no-allocation scratch", ntiles);
     dfree(h, h->tiles);
     CHK(dalloc(h, &h->tiles, ntiles + 16));
     h->tiles_cap = ntiles + 16;
@@ -672,6 +679,7 @@ int alloc_mg(vare_flip* h);
 // A failed allocation leaves no levels, so the next call allocates again.
 int ensure_mg(vare_flip* h) {
     if (!h->mg.empty() && h->mg_bcap == h->bcap) return VARE_OK;
+    if (h->noalloc) return fail(VARE_ENOMEM, "multigrid levels not allocated (the no-allocation step never allocates them)");
     free_mg(h);
     const int rc = alloc_mg(h);
     if (rc != VARE_OK) free_mg(h);
@@ -1319,6 +1327,40 @@ void apply_params(vare_flip* h, const vare_flip_params* p) {
 
 }  // namespace
 
+// Scratch of vare_flip_step_noalloc: the reductions' per-workgroup partials and the scans' tile sums.
+// Host code never uses this name as an address; it uses the per-device address that
+// hipGetSymbolAddress returns (noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[VARE_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// Device address of `scratch` on device `dev`, looked up once per device with that device current.
+int noalloc_scratch(int dev, void** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(VARE_EINVAL, "device %d out of range", dev);
+    std::call_once(g_scratch_once[dev], [dev]() {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != dev) e = hipSetDevice(dev);
+        if (e == hipSuccess) e = hipGetSymbolAddress(&g_scratch_ptr[dev], HIP_SYMBOL(scratch));
+        if (cur >= 0 && cur != dev) (void)hipSetDevice(cur);
+        g_scratch_err[dev] = e;
+    });
+    if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev])
+        return fail(VARE_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                    hipGetErrorString(g_scratch_err[dev]));
+    *out = g_scratch_ptr[dev];
+    return VARE_OK;
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
 extern "C" {
 
 const char* vare_last_error(void) { return g_err.c_str(); }
@@ -1380,6 +1422,7 @@ int vare_flip_create(const vare_grid_desc* g, const vare_flip_params* prm, size_
     if (int rc0 = init_geometry(h, g)) { delete h; return rc0; }
     apply_params(h, p);
     h->vhalf = p->velocity_fp16 ? 1 : 0;
+    if (hipGetDevice(&h->device) != hipSuccess) { delete h; return fail(VARE_EHIP, "hipGetDevice failed"); }
     if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(VARE_EHIP, "hipStreamCreate failed"); }
     int rc = alloc_frame(h);
     if (rc == VARE_OK && hipHostMalloc((void**)&h->st_host, sizeof(CgState), 0) != hipSuccess) rc = fail(VARE_EHIP, "hipHostMalloc failed");
@@ -1388,6 +1431,8 @@ int vare_flip_create(const vare_grid_desc* g, const vare_flip_params* prm, size_
         if (hipEventCreate(&h->step_ev[q]) != hipSuccess) rc = fail(VARE_EHIP, "hipEventCreate failed");
     if (rc == VARE_OK && pcap) rc = ensure_particles(h, pcap);
     if (rc == VARE_OK) rc = ensure_blocks(h, initial_block_capacity(h, bcap));
+    if (rc == VARE_OK && p->solver_mode == VARE_SOLVER_RELATIVE && p->preconditioner == VARE_PRECOND_MG)
+        rc = ensure_mg(h);
     if (rc == VARE_OK) {
         rc = fill_u32(h, h->slot, 0xffffffffu, (size_t)h->nblk);
         if (rc == VARE_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(VARE_EHIP, "init sync failed");
@@ -1466,6 +1511,7 @@ int vare_flip_set_params(vare_flip* h, const vare_flip_params* p) {
     if ((p->velocity_fp16 ? 1 : 0) != h->vhalf)
         return fail(VARE_EINVAL, "velocity_fp16 is fixed when the handle is created");
     apply_params(h, p);
+    if (p->solver_mode == VARE_SOLVER_RELATIVE && p->preconditioner == VARE_PRECOND_MG) CHK(ensure_mg(h));
     return VARE_OK;
 }
 
@@ -1627,6 +1673,41 @@ int vare_flip_step(vare_flip* h, float dt) {
         return fail(VARE_ENOCONV, "PCG stopped at %d iterations, |r|=%g |b|=%g", h->stats.cg_iterations,
                     h->stats.residual_norm, h->stats.b_norm);
     return VARE_OK;
+}
+
+int vare_flip_step_noalloc(vare_flip* h, float dt) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (h->timing) return fail(VARE_EINVAL, "kernel timing creates events on demand: switch it off for the no-allocation step");
+    if (!h->mass) return fail(VARE_ENOMEM, "the handle has no cell arrays");
+    // every reduction of the step writes at most max(kReduceGrid, ceil(blocks / 2)) partials (the
+    // block kernels take two or more blocks per workgroup); the longest scan covers the frame, the
+    // bins or the particles
+    const size_t npart = std::max((size_t)kReduceGrid, (h->bcap + 1) / 2) + 16;
+    const size_t ntile = step_scan_tiles(h, h->np) + 16;
+    const size_t off2 = align256(npart * sizeof(double));
+    const size_t offt = off2 + align256(npart * sizeof(double));
+    const size_t need = offt + ntile * sizeof(int);
+    if (need > (size_t)VARE_NOALLOC_SCRATCH_BYTES)
+        return fail(VARE_ENOMEM, "the step needs %zu bytes of scratch (block capacity %zu, %zu particles); the no-allocation "
+                    "step holds %zu", need, h->bcap, h->np, (size_t)VARE_NOALLOC_SCRATCH_BYTES);
+    void* base = nullptr;
+    CHK(noalloc_scratch(h->device, &base));
+    double* const own_p = h->partials;
+    double* const own_p2 = h->partials2;
+    int* const own_t = h->tiles;
+    const size_t own_tcap = h->tiles_cap;
+    h->partials = (double*)base;
+    h->partials2 = (double*)((char*)base + off2);
+    h->tiles = (int*)((char*)base + offt);
+    h->tiles_cap = ntile;
+    h->noalloc = 1;
+    const int rc = vare_flip_step(h, dt);
+    h->noalloc = 0;
+    h->partials = own_p;
+    h->partials2 = own_p2;
+    h->tiles = own_t;
+    h->tiles_cap = own_tcap;
+    return rc;
 }
 
 int vare_flip_advance_frame(vare_flip* h) {
