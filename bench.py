@@ -19,8 +19,21 @@ Other configurations (--config):
      x<0.25, y<0.3, full z moving +x at 4 m/s, a static box obstacle x in [0.5,0.7], y in [0,0.5],
      z in [0.6,1.4] given as a signed distance field, FP16 velocities, FP64 refinement to 1e-6, 100
      steps at dt = 1/1000 (4 m/s is 8.5 cells per step at 1/240; 1/1000 keeps it near 2).
-For configs 3 and 5 the CPU baseline times one step of the same scene at a quarter of the
-resolution per axis (a bounded sample; the rate is per particle-step).
+The timed region is K consecutive steps of the configuration's run (--steps, default the whole run).
+A shorter region is centred on the run: the simulation first advances untimed to step (run - K) / 2
+(--start-step), so a 20-step sample of the 200-step dam break times steps 90..109.
+
+"vs plain GPU" (configurations 1 and 2, --plain): the same timed steps are run again with the plain
+FP32 GPU step -- FP32 multigrid PCG to the target on its recursive residual, no FP64 refinement --
+and both solvers take one step from the same final particle state, which reports each one's FP64
+relative residual and the divergence ratio D = |div_c v_post| / |div_c v_pre| (the reference's
+central-difference divergence; VARE_FLAG_REPORT_DIVERGENCE).  `vs_plain_gpu` is the ratio of the two
+rates; `vs_baseline` stays null because the reference publishes no number for this metric.
+
+The CPU baseline is the OpenMP build of the oracle (the FP32 restatement of the reference step) on the
+GPU box's host: all threads of the host's share and one thread under the reference's own CG rule, and
+all threads under Jacobi-PCG to 1e-5 relative.  For configs 3 and 5 it times steps of the same scene
+at a quarter of the resolution per axis (a bounded sample; the rate is per particle-step).
 
 Prints one JSON line (driver contract) with `roofline` and `cpu_baseline` objects.
 """
@@ -71,6 +84,15 @@ def parse():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--start-step", type=int, default=None,
+                    help="untimed steps of the simulation before the warm-up and the timed region; default: "
+                         "centre a shorter timed region on the configuration's full run")
+    ap.add_argument("--plain", choices=["on", "off"], default=None,
+                    help="also time the plain FP32 GPU step (FP32 PCG to --tol, no refinement) on the same "
+                         "steps; default on for configurations 1 and 2")
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="threads of the multi-core CPU baseline (default: OMP_NUM_THREADS, else the host's "
+                         "cores up to 16)")
     return ap.parse_args()
 
 
@@ -85,13 +107,36 @@ def solver_kw(a, L, n):
                 mg_sweeps=a.mg_sweeps)
 
 
-def solver_desc(a, cfg=2):
+def solver_desc(a, cfg=2, kw=None):
     if a.solver != "relative":
         return "reference CG rule"
-    if a.no_refine:
+    if (kw is not None and kw["refine_tol"] == 0.0) or (kw is None and a.no_refine):
         return "FP32 %s-PCG to %.0e relative (recursive residual)" % (a.precond, a.tol)
     return ("FP32 %s-PCG (%.0e per solve) + FP64 iterative refinement to %.0e relative FP64 residual"
             % (a.precond, a.inner_tol, a.tol))
+
+
+def plain_kw(a, L, n):
+    """The plain GPU step: FP32 PCG to --tol on its recursive residual, no refinement."""
+    kw = solver_kw(a, L, n)
+    kw.update(cg_rel_tol=a.tol, refine_tol=0.0)
+    return kw
+
+
+def same_state_divergence(a, L, cfg, state, kws):
+    """One step from the same particle state per solver setting: the FP64 relative residual of the
+    pressure and D = |div_c v_post| / |div_c v_pre| (the reference's central-difference divergence)."""
+    sc = scene(cfg)
+    out = {}
+    for name, kw in kws.items():
+        h = scene_handle(L, sc, dict(kw, flags=L.FLAG_REPORT_DIVERGENCE), a.block, state[0].shape[0])
+        h.set_particles(state[0], state[1])
+        step_ok(L, h, sc["dt"])
+        st = h.stats()
+        out[name] = {"fp64_rel_residual": st["true_rel_residual"], "D": st["div_ratio"],
+                     "cg_iterations": st["cg_iterations"], "refine_steps": st["refine_steps"]}
+        h.close()
+    return out
 
 
 def scene(cfg, scale=1):
@@ -142,29 +187,60 @@ def scene_handle(L, sc, kw, B, npart):
     return h
 
 
-def run_scene(a, L, cfg, steps):
-    sc = scene(cfg)
-    kw = solver_kw(a, L, sc["res"])
-    t0 = time.time()
-    pos, vel = seed_scene(L, sc, a.ppc)
-    npart = pos.shape[0]
-    print("bench: seeded %d particles in %.1fs on %s" % (npart, time.time() - t0, L.device_name()),
-          file=sys.stderr, flush=True)
-    B = a.block
-    h = scene_handle(L, sc, kw, B, npart)
+def step_ok(L, h, dt):
+    rc = h.step_rc(dt)
+    if rc not in (L.VARE_OK, L.VARE_ENOCONV):
+        L.check(rc)
+    return rc
 
-    # warm-up: run the first steps of the simulation, then restart from the initial state
+
+def start_step(a, cfg, steps):
+    """First timed step of the simulation: a timed region shorter than the configuration's run is
+    centred on that run, so its steps are representative of the whole run."""
+    if a.start_step is not None:
+        return max(0, a.start_step)
+    full = scene(cfg)["steps"]
+    return max(0, (full - steps) // 2)
+
+
+def run_scene(a, L, cfg, steps, kw=None, timing=True, state=None):
+    """Time `steps` steps of configuration `cfg`.  `state` = (pos, vel, start) reuses a seeded state
+    (pos, vel after `start` untimed steps) instead of seeding and pre-rolling."""
+    sc = scene(cfg)
+    kw = kw or solver_kw(a, L, sc["res"])
+    B = a.block
+    if state is None:
+        t0 = time.time()
+        pos, vel = seed_scene(L, sc, a.ppc)
+        print("bench: seeded %d particles in %.1fs on %s" % (pos.shape[0], time.time() - t0, L.device_name()),
+              file=sys.stderr, flush=True)
+        start = start_step(a, cfg, steps)
+        npart = pos.shape[0]
+        h = scene_handle(L, sc, kw, B, npart)
+        if start:
+            # advance the simulation to the first timed step (untimed), from rest
+            h.set_particles(pos, vel)
+            for _ in range(start):
+                step_ok(L, h, sc["dt"])
+            d = h.get_particles()
+            pos, vel = d["pos"], d["vel"]
+            print("bench: advanced %d untimed steps" % start, file=sys.stderr, flush=True)
+    else:
+        pos, vel, start = state
+        npart = pos.shape[0]
+        h = scene_handle(L, sc, kw, B, npart)
+    state = (pos, vel, start)
+
+    # warm-up: run the next steps, then restart from the first timed step's state
     h.set_particles(pos, vel)
     for _ in range(a.warmup):
-        rc = h.step_rc(sc["dt"])
-        if rc not in (L.VARE_OK, L.VARE_ENOCONV):
-            L.check(rc)
+        step_ok(L, h, sc["dt"])
     h.set_particles(pos, vel)
-    del pos, vel
+    timing = timing and not a.no_kernel_timing
     # HIP events around a launch cost it about 5 us, so inside the timed region only the dominant
     # kernel is timed, on every TIMING_STRIDE-th step (a uniform sample of its launches over the
     # region); the other kernels' lines come from a short run after it
-    if not a.no_kernel_timing:
+    if timing:
         h.kernel_timing(reset=True)
     L.device_synchronize()
 
@@ -173,7 +249,7 @@ def run_scene(a, L, cfg, steps):
     stats = None
     t_start = time.perf_counter()
     for s in range(steps):
-        if not a.no_kernel_timing:
+        if timing:
             h.set_kernel_timing(["update"] if s % TIMING_STRIDE == 0 else [])
         ts = time.perf_counter()
         nps.append(h.num_particles())
@@ -194,7 +270,7 @@ def run_scene(a, L, cfg, steps):
                   file=sys.stderr, flush=True)
     L.device_synchronize()
     total_s = time.perf_counter() - t_start
-    kt = h.kernel_timing(reset=True) if not a.no_kernel_timing else None
+    kt = h.kernel_timing(reset=True) if timing else None
     if kt is not None:
         # untimed: OTHER_TIMING_STEPS more steps with events on the other two reported kernels
         h.set_kernel_timing(["spmv", "mg0"])
@@ -241,10 +317,12 @@ def run_scene(a, L, cfg, steps):
         "ms_per_step": round(total_s * 1e3 / steps, 3),
         "p50_ms": round(float(np.percentile(sm, 50)), 3),
         "p99_ms": round(float(np.percentile(sm, 99)), 3),
-        "config": {"workload": "%s, %d ppc, dt=1/%d, FLIP 0.95, %s%s, %d^3 blocks, %d steps from the initial state"
-                               % (sc["desc"], a.ppc, round(1.0 / sc["dt"]), solver_desc(a, cfg),
-                                  ", FP16 particle velocities" if sc["fp16"] else "", B, steps),
+        "config": {"workload": "%s, %d ppc, dt=1/%d, FLIP 0.95, %s%s, %d^3 blocks, steps %d..%d of the %d-step run"
+                               % (sc["desc"], a.ppc, round(1.0 / sc["dt"]), solver_desc(a, cfg, kw),
+                                  ", FP16 particle velocities" if sc["fp16"] else "", B, start, start + steps - 1,
+                                  max(sc["steps"], start + steps)),
                    "particles": npart, "particles_last": int(nps[-1]), "cells": nx * ny * nz,
+                   "first_timed_step": start,
                    "active_blocks_last": stats["num_blocks"], "frame_blocks": (nx // B) * (ny // B) * (nz // B),
                    "fluid_cells_last": stats["num_fluid"]},
         "cg_iterations_mean": round(float(np.mean(iters)), 1),
@@ -253,18 +331,46 @@ def run_scene(a, L, cfg, steps):
         "fp64_rel_residual_max": float(np.max(truerel)) if truerel else 0.0,
         "refine_steps_mean": round(float(np.mean(refines)), 2) if refines else 0.0,
         "nonconverged_steps": nonconv,
+        "fp64_rel_residual_mean": float(np.mean(truerel)) if truerel else 0.0,
         "roofline": roofline,
     }
-    return h, kw, res
+    return h, kw, res, state
+
+
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_threads(a):
+    if a.cpu_threads:
+        return a.cpu_threads
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, min(16, os.cpu_count() or 1))
 
 
 def cpu_baseline(a, L, cfg, h, kw):
-    """Time the oracle (the FP32 restatement of the reference step, one host thread) with the
-    reference's own solver (unpreconditioned CG, |r| < 1e-3 or 202 products) on a bounded sample of
-    the workload: config 1 all its steps from rest; config 2 one step from the GPU run's final
-    particle state; configs 3 and 5 two steps of the same scene at a quarter of the resolution, after
+    """Time the oracle's OpenMP build (the FP32 restatement of the reference step; loops over
+    particles and cells in parallel, P2G by z-slabs two apart) on the GPU box's host: with every
+    thread of the host's share and with one thread, under the reference's own solver rule
+    (unpreconditioned CG, |r| < 1e-3 or 202 products), and with every thread also under the configured
+    rule (Jacobi-PCG to 1e-5 relative).  A bounded sample of the workload: config 1 all its steps from
+    rest with all threads and its first 10 with one; config 2 one step from the GPU run's final particle
+    state; configs 3 and 5 two steps of the same scene at a quarter of the resolution per axis, after
     four untimed ones."""
     from oracle import oracle as O
+    nproc, model = host_info()
+    nth = cpu_threads(a)
     if cfg == 1:
         sc = scene(1)
         p, v = seed_scene(L, sc, a.ppc)
@@ -278,27 +384,41 @@ def cpu_baseline(a, L, cfg, h, kw):
         sc = scene(cfg, scale=4)
         p, v = seed_scene(L, sc, a.ppc)
         steps, start = a.cpu_steps or 2, "the state after 4 untimed steps"
-    ref_kw = dict(ext_force=kw["ext_force"], wall_level=kw["wall_level"], voxel_size=1.0 / sc["res"])
-    o = O.Oracle(sc["dims"], (0.0, 0.0, 0.0), sc["world"], B=a.block, prm=O.params(**ref_kw))
-    if sc["obstacle"] is not None:
-        o.set_collision_box(sc["obstacle"][0], sc["obstacle"][1], (0.0, 0.0, 0.0), 3.0)
-    o.set_particles(p, v)
-    if cfg in (3, 5):
-        for _ in range(4):          # leave the trivial first steps from rest out of the sample
-            o.step(sc["dt"])
-    npart = p.shape[0]
-    its, counts = [], []
-    t = time.perf_counter()
-    for _ in range(steps):
-        counts.append(o.num_particles())
-        its.append(o.step(sc["dt"]))
-    el = time.perf_counter() - t
-    nfinal = o.num_particles()
-    o.close()
-    return {"value": round(float(np.sum(counts)) / el, 1), "unit": "particle-steps/s", "cores": 1, "kind": "port",
+    base_kw = dict(ext_force=kw["ext_force"], wall_level=kw["wall_level"], voxel_size=1.0 / sc["res"])
+
+    def timed(threads, nsteps, **solver):
+        O.set_threads(threads)
+        o = O.Oracle(sc["dims"], (0.0, 0.0, 0.0), sc["world"], B=a.block, prm=O.params(**base_kw, **solver),
+                     lib_path=O.OMP_LIB_PATH)
+        if sc["obstacle"] is not None:
+            o.set_collision_box(sc["obstacle"][0], sc["obstacle"][1], (0.0, 0.0, 0.0), 3.0)
+        o.set_particles(p, v)
+        if cfg in (3, 5):
+            for _ in range(4):          # leave the trivial first steps from rest out of the sample
+                o.step(sc["dt"])
+        its, counts = [], []
+        t = time.perf_counter()
+        for _ in range(nsteps):
+            counts.append(o.num_particles())
+            its.append(o.step(sc["dt"]))
+        el = time.perf_counter() - t
+        o.close()
+        return {"value": round(float(np.sum(counts)) / el, 1), "steps": nsteps, "seconds": round(el, 2),
+                "cg_iterations_mean": round(float(np.mean(its)), 1)}
+
+    ref_all = timed(nth, steps)
+    ref_one = timed(1, min(steps, 10))
+    rel_all = timed(nth, min(steps, 10), solver_mode=1, cg_rel_tol=1e-5, max_iterations=20000, preconditioner=1)
+    print("bench: CPU baseline %d threads %.3g, 1 thread %.3g, relative 1e-5 %.3g particle-steps/s" % (
+        nth, ref_all["value"], ref_one["value"], rel_all["value"]), file=sys.stderr, flush=True)
+    return {"value": ref_all["value"], "unit": "particle-steps/s", "cores": nth, "kind": "port",
             "sample": "%d step(s) of %s (%d particles, %d ppc) from %s, reference CG rule (mean %.0f iterations), "
-                      "%.1f s, %d particles at the end" % (steps, sc["desc"], npart, a.ppc, start,
-                                                           float(np.mean(its)), el, nfinal)}
+                      "%.1f s on %d threads of the OpenMP build of the oracle"
+                      % (steps, sc["desc"], p.shape[0], a.ppc, start, ref_all["cg_iterations_mean"],
+                         ref_all["seconds"], nth),
+            "host_nproc": nproc, "host_cpu": model,
+            "one_core": dict(ref_one, rule="reference CG rule"),
+            "relative_1e-5": dict(rel_all, cores=nth, rule="Jacobi-PCG to 1e-5 relative (FP32)")}
 
 
 def run_served(a, L):
@@ -374,7 +494,24 @@ def main():
         steps, cpu = res.pop("steps"), None
     else:
         steps = a.steps if a.steps is not None else scene(a.config)["steps"]
-        h, kw, res = run_scene(a, L, a.config, steps)
+        h, kw, res, state = run_scene(a, L, a.config, steps)
+        plain_on = (a.plain or ("on" if a.config in (1, 2) else "off")) == "on"
+        if plain_on and a.solver == "relative" and kw["refine_tol"] > 0.0:
+            # the plain GPU step on the same timed steps: FP32 PCG to the target on its recursive
+            # residual, no FP64 refinement
+            pkw = plain_kw(a, L, scene(a.config)["res"])
+            hp, _, pres, _ = run_scene(a, L, a.config, steps, kw=pkw, timing=False, state=state)
+            hp.close()
+            res["plain_gpu"] = {"value": pres["value"], "ms_per_step": pres["ms_per_step"], "p50_ms": pres["p50_ms"],
+                                "p99_ms": pres["p99_ms"], "cg_iterations_mean": pres["cg_iterations_mean"],
+                                "fp64_rel_residual_max": pres["fp64_rel_residual_max"],
+                                "fp64_rel_residual_mean": pres["fp64_rel_residual_mean"],
+                                "solver": solver_desc(a, a.config, pkw)}
+            res["vs_plain_gpu"] = round(res["value"] / max(pres["value"], 1e-9), 3)
+            # both solvers on one and the same system: the final state of the timed run
+            d = h.get_particles()
+            res["divergence_same_state"] = same_state_divergence(a, L, a.config, (d["pos"], d["vel"]),
+                                                                 {"mixed": kw, "plain": pkw})
         cpu = cpu_baseline(a, L, a.config, h, kw) if a.cpu_baseline == "on" else None
     out = {
         "metric": METRIC,

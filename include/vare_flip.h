@@ -89,8 +89,17 @@ typedef struct vare_flip_params {
     int refine_max;          /* refinement step cap, default 8 */
     int velocity_fp16;       /* 1: particle velocities stored in FP16 (positions stay FP32); fixed at
                                 vare_flip_create */
-    int reserved[1];
+    int flags;               /* VARE_FLAG_* bits, default 0 */
 } vare_flip_params;
+
+/* vare_flip_params.flags */
+#define VARE_FLAG_REPORT_DIVERGENCE 1  /* after the pressure apply, also compute the reference's
+                                          central-difference divergence b of the projected velocity
+                                          (FluidProjection.cu:220-235 on the same cell types) and report
+                                          D = |b(v_post)| / |b(v_pre)| in vare_step_stats.div_ratio */
+#define VARE_FLAG_FP32_APPLY        2  /* with refinement: round the FP64 pressure to FP32 first and apply
+                                          the air extrapolation and gradient in FP32 (default: in FP64
+                                          from the refined pressure) */
 
 /* Frame geometry after SparseFrame::initialize (SparseFrame.cu:72-117): padded dims, block
  * counts, and the voxel<->world transform (Grid.h:431-457). */
@@ -120,6 +129,7 @@ typedef struct vare_step_stats {
     int refine_steps;            /* FP64 iterative-refinement steps (0 when refinement is off) */
     int pad;
     double true_rel_residual;    /* |b - A p|_2 / |b|_2 recomputed in FP64 from the final pressure */
+    double div_ratio;            /* D = |div_c v_post| / |div_c v_pre| (VARE_FLAG_REPORT_DIVERGENCE), else 0 */
 } vare_step_stats;
 
 /* cell fields for vare_flip_get_cell_field (storage order = block-major, x fastest) */

@@ -10,6 +10,12 @@
 //                  -ffp-contract=off), sequential P2G in particle order.
 //   * R = double : the same algorithm in FP64, the error yardstick for the GPU results.
 //
+// Built a second time with -fopenmp (oracle/lib/libvare_oracle_omp.so) as the multi-core CPU baseline
+// that bench.py times: the same stages with the loops over particles and cells run in parallel, and
+// P2G / the count field scattering z-slabs of particles two apart at a time (no two threads write a
+// cell at once, no atomics), so its FP32 sums come in another order.  The checker build has no
+// OpenMP and keeps the sequential order.
+//
 // Documented deviations from the reference (both unavoidable on a CPU, neither changes physics):
 //   * K1 block marking (SparseFrame.cu:149-157) writes out of bounds of _markers when the signed
 //     int index k*fx*fy+j*fx+i is negative; those writes are dropped here.  In-bounds aliasing
@@ -23,6 +29,9 @@
 #include <climits>
 #include <vector>
 #include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace orc {
 
@@ -203,6 +212,7 @@ template <class R> struct State {
     void build_from_points() {
         const int fx = g.fx, fy = g.fy, fz = g.fz, B = g.B;
         markers.assign((size_t)nblk(), 0);
+#pragma omp parallel for
         for (size_t n = 0; n < pos.size(); ++n) {
             const V3<R>& p = pos[n];
             const int ci = Max((int)(p.x) / B, 0);
@@ -213,6 +223,7 @@ template <class R> struct State {
                     for (int i = ci - 1; i <= Min(ci + 1, fx - 1); ++i) {
                         const int64_t t = (int64_t)(k * fx * fy + j * fx + i);   // int arithmetic as in the reference
                         if (t < 0) continue;                                        // reference writes out of bounds here
+#pragma omp atomic
                         markers[(size_t)t] += 1;
                     }
         }
@@ -243,14 +254,44 @@ template <class R> struct State {
     }
 
     // ---- Rasterization::PointsToVelField, sparse (Rasterization.cu:121-190) ----
+    // Run scatter(n) for every particle.  Sequentially in particle order, or in the OpenMP build by
+    // z-slabs of kSlab cell layers of the P2G base cell: a particle writes cells of its own slab and
+    // the first layer of the next, so slabs two apart never share a cell; even slabs run in parallel,
+    // then odd ones, each slab's particles in particle order.
+    template <class F> void for_particles_scatter(F scatter) {
+#ifdef _OPENMP
+        constexpr int kSlab = 4;
+        const int nsl = (int)((g.nz + kSlab - 1) / kSlab);
+        std::vector<int> slab(pos.size());
+        std::vector<size_t> start((size_t)nsl + 1, 0);
+        for (size_t n = 0; n < pos.size(); ++n) {
+            const int ck = Max((int)(pos[n].z - R(0.5)), 0);
+            slab[n] = inside(pos[n]) ? Min(ck / kSlab, nsl - 1) : -1;
+            if (slab[n] >= 0) start[(size_t)slab[n] + 1]++;
+        }
+        for (int q = 0; q < nsl; ++q) start[(size_t)q + 1] += start[(size_t)q];
+        std::vector<size_t> order(start[(size_t)nsl]), fill(start.begin(), start.end() - 1);
+        for (size_t n = 0; n < pos.size(); ++n)
+            if (slab[n] >= 0) order[fill[(size_t)slab[n]]++] = n;
+        for (int parity = 0; parity < 2; ++parity) {
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int q = parity; q < nsl; q += 2)
+                for (size_t t = start[(size_t)q]; t < start[(size_t)q + 1]; ++t) scatter(order[t]);
+        }
+#else
+        for (size_t n = 0; n < pos.size(); ++n) scatter(n);
+#endif
+    }
+
+    // ---- Rasterization::PointsToVelField, sparse (Rasterization.cu:121-190) ----
     void p2g() {
         std::fill(mass.begin(), mass.end(), R(0));
         std::fill(velo.begin(), velo.end(), V3<R>{0, 0, 0});
         const int EI = (int)g.nx - 1, EJ = (int)g.ny - 1, EK = (int)g.nz - 1;
-        for (size_t n = 0; n < pos.size(); ++n) {
+        for_particles_scatter([&](size_t n) {
             const V3<R>& p = pos[n];
             const V3<R>& v = vel[n];
-            if (!inside(p)) continue;
+            if (!inside(p)) return;
             const int ci = Max((int)(p.x - R(0.5)), 0);
             const int cj = Max((int)(p.y - R(0.5)), 0);
             const int ck = Max((int)(p.z - R(0.5)), 0);
@@ -270,7 +311,8 @@ template <class R> struct State {
                         velo[idx].y += w * v.y;
                         velo[idx].z += w * v.z;
                     }
-        }
+        });
+#pragma omp parallel for
         for (size_t n = 0; n < mass.size(); ++n) {    // Rasterization.cu:175-188
             if (mass[n] > R(0)) {
                 velo[n].x /= mass[n];
@@ -287,9 +329,9 @@ template <class R> struct State {
     void count_field() {
         std::fill(surf.begin(), surf.end(), R(0));
         const int EI = (int)g.nx - 1, EJ = (int)g.ny - 1, EK = (int)g.nz - 1;
-        for (size_t n = 0; n < pos.size(); ++n) {
+        for_particles_scatter([&](size_t n) {
             const V3<R>& p = pos[n];
-            if (!inside(p)) continue;
+            if (!inside(p)) return;
             const int ci = Max((int)(p.x - R(0.5)), 0);
             const int cj = Max((int)(p.y - R(0.5)), 0);
             const int ck = Max((int)(p.z - R(0.5)), 0);
@@ -300,7 +342,7 @@ template <class R> struct State {
                         if (idx == NULL_MAX) continue;
                         surf[idx] += R(1);
                     }
-        }
+        });
     }
 
     // ---- cell classification, FLIPMethod.cu:242-280 (no collision source: phi = 5, v = 0,
@@ -308,6 +350,7 @@ template <class R> struct State {
     // ---- FLIPMethod::advanceOneStep mm_kernel (FLIPMethod.cu:242-280), curl disabled ----
     void classify() {
         const R scale = (R)prm.col_vel_scale;
+#pragma omp parallel for
         for (size_t n = 0; n < mark.size(); ++n) {
             int64_t i, j, k;
             find_indices(n, i, j, k);
@@ -442,6 +485,7 @@ template <class R> struct State {
         const float wallLv = (float)vox_wall;
         auto T = [&](int64_t i, int64_t j, int64_t k) { return at(mark, (uint64_t)0, i, j, k); };
         auto Vf = [&](int64_t i, int64_t j, int64_t k) { return at(comp, V3<R>{0, 0, 0}, i, j, k); };
+#pragma omp parallel for
         for (uint64_t n = 0; n < count; ++n) {
             int64_t i, j, k;
             find_indices(fluid[n], i, j, k);
@@ -488,6 +532,7 @@ template <class R> struct State {
     // ---- BLAS (BLAS.cu:12-209) ----
     void spmv(const std::vector<R>& p, std::vector<R>& out) const {     // BLAS::mul, BLAS.cu:12-40
         const size_t rows = p.size();
+#pragma omp parallel for
         for (size_t r = 0; r < rows; ++r) {
             R acc = R(0);
             for (int c = 0; c < 7; ++c) {
@@ -499,6 +544,7 @@ template <class R> struct State {
     }
     static R dot(const std::vector<R>& a, const std::vector<R>& c) {  // BLAS::dot, BLAS.cu:98-104
         double s = 0.0;
+#pragma omp parallel for reduction(+ : s)
         for (size_t n = 0; n < a.size(); ++n) s += (double)a[n] * (double)c[n];
         return (R)s;
     }
@@ -508,6 +554,7 @@ template <class R> struct State {
         const size_t N = b.size();
         std::vector<R> r(N), p(N), Ap(N);
         spmv(x, Ap);
+#pragma omp parallel for
         for (size_t n = 0; n < N; ++n) r[n] = b[n] - Ap[n];      // BLAS::submul(b,A,x,r)
         p = r;                                                    // BLAS::equ
         R rsold = dot(r, r);
@@ -519,12 +566,15 @@ template <class R> struct State {
         for (int i = 0; i < iter + 5; ++i, ++count) {
             spmv(p, Ap);
             const R alpha = rsold / dot(p, Ap);
+#pragma omp parallel for
             for (size_t n = 0; n < N; ++n) x[n] = x[n] + (alpha * p[n]);
+#pragma omp parallel for
             for (size_t n = 0; n < N; ++n) r[n] = r[n] - (alpha * Ap[n]);
             const R rsnew = dot(r, r);
             cg_res = std::sqrt((double)rsnew);
             if (std::sqrt(rsnew) < (R)prm.cg_abs_tol) break;
             const R beta = rsnew / rsold;
+#pragma omp parallel for
             for (size_t n = 0; n < N; ++n) p[n] = r[n] + (beta * p[n]);
             rsold = rsnew;
             if (count > max) break;
@@ -539,12 +589,14 @@ template <class R> struct State {
         std::vector<R> r(N), p(N), z(N), Ap(N), dinv(N);
         for (size_t n = 0; n < N; ++n) dinv[n] = prm.preconditioner == 1 ? R(1) / A_val[n * 7] : R(1);
         spmv(x, Ap);
+#pragma omp parallel for
         for (size_t n = 0; n < N; ++n) r[n] = b[n] - Ap[n];
         b_norm = std::sqrt((double)dot(b, b));
         R rr = dot(r, r);
         cg_res = std::sqrt((double)rr);
         const R tol = (R)prm.cg_rel_tol * (R)b_norm;
         if (std::sqrt(rr) <= tol || b_norm == 0.0) return 0;
+#pragma omp parallel for
         for (size_t n = 0; n < N; ++n) z[n] = dinv[n] * r[n];
         p = z;
         R rz = dot(r, z);
@@ -552,15 +604,19 @@ template <class R> struct State {
         while (it < max) {
             spmv(p, Ap);
             const R alpha = rz / dot(p, Ap);
+#pragma omp parallel for
             for (size_t n = 0; n < N; ++n) x[n] = x[n] + (alpha * p[n]);
+#pragma omp parallel for
             for (size_t n = 0; n < N; ++n) r[n] = r[n] - (alpha * Ap[n]);
             ++it;
             rr = dot(r, r);
             cg_res = std::sqrt((double)rr);
             if (std::sqrt(rr) <= tol) break;
+#pragma omp parallel for
             for (size_t n = 0; n < N; ++n) z[n] = dinv[n] * r[n];
             const R rz_new = dot(r, z);
             const R beta = rz_new / rz;
+#pragma omp parallel for
             for (size_t n = 0; n < N; ++n) p[n] = z[n] + (beta * p[n]);
             rz = rz_new;
         }
@@ -575,6 +631,7 @@ template <class R> struct State {
 
     // K10-K12 on the current solution x (FluidProjection.cu:297-378)
     void apply_pressure() {
+#pragma omp parallel for
         for (size_t n = 0; n < fluid.size(); ++n) {               // K10, :297-306
             const uint64_t idx = fluid[n];
             pres[idx] = x[mark[idx] - kFluid];
@@ -582,6 +639,7 @@ template <class R> struct State {
         const int64_t EI = g.nx - 1, EJ = g.ny - 1, EK = g.nz - 1;
         auto T = [&](int64_t i, int64_t j, int64_t k) { return at(mark, (uint64_t)0, i, j, k); };
         auto P = [&](int64_t i, int64_t j, int64_t k) { return at(pres, R(0), i, j, k); };
+#pragma omp parallel for
         for (size_t n = 0; n < mark.size(); ++n) {                // K11, :314-335
             if (mark[n] != kAir) continue;
             int64_t i, j, k;
@@ -597,6 +655,7 @@ template <class R> struct State {
             if (cnt > 0) pres[n] = press / (R)cnt;
         }
         const float wallLv = (float)vox_wall;
+#pragma omp parallel for
         for (size_t t = 0; t < fluid.size(); ++t) {               // K12, :342-378
             const uint64_t n = fluid[t];
             int64_t i, j, k;
@@ -674,6 +733,7 @@ template <class R> struct State {
         // FLIPMethod.cu:99 uses the literal pair 0.95f / 0.05f
         const R fr = prm.flip_ratio == 0.95f ? R(0.95) : (R)prm.flip_ratio;
         const R pr = prm.flip_ratio == 0.95f ? R(0.05) : R(1) - fr;
+#pragma omp parallel for
         for (size_t ix = 0; ix < pos.size(); ++ix) {
             V3<R>& ps = pos[ix];
             V3<R>& v = vel[ix];
@@ -806,6 +866,17 @@ struct orc_stats {
 };
 
 float orc_rand(uint32_t seed) { return rand01(seed); }
+
+// Threads of the OpenMP build (returns the count in use; 1 in the checker build).
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 void orc_geom(const int64_t dims[3], const float bmin[3], const float bmax[3], int B, int64_t out_dims[3], int out_f[3],
               float out_t[3], float out_s[3]) {
@@ -966,8 +1037,8 @@ size_t orc_get_field(void* hp, int which, double* out) {
 }
 
 // Overwrite a field (testing a stage on given inputs): 0 mass, 1 velo, 2 comp, 3 mark (cell types
-// 0 air / 1 solid / 4 fluid before build_linear_system numbers the fluid cells), 4 pres, 7 x (the
-// solution per fluid rank, for apply_pressure).
+// 0 air / 1 solid / 4 fluid before build_linear_system numbers the fluid cells), 4 pres, 5 surf (the
+// count field), 7 x (the solution per fluid rank, for apply_pressure).
 void orc_set_field(void* hp, int which, const double* in) {
     Handle* h = (Handle*)hp;
     auto run = [&](auto* s) {
@@ -977,6 +1048,7 @@ void orc_set_field(void* hp, int which, const double* in) {
         if (which == 0) for (size_t n = 0; n < s->mass.size(); ++n) s->mass[n] = (R)in[n];
         if (which == 3) for (size_t n = 0; n < s->mark.size(); ++n) s->mark[n] = (uint64_t)in[n];
         if (which == 4) for (size_t n = 0; n < s->pres.size(); ++n) s->pres[n] = (R)in[n];
+        if (which == 5) for (size_t n = 0; n < s->surf.size(); ++n) s->surf[n] = (R)in[n];
         if (which == 7) for (size_t n = 0; n < s->x.size(); ++n) s->x[n] = (R)in[n];
     };
     if (h->fp64) run(h->s64); else run(h->s32);

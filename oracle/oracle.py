@@ -12,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # VARE_ORACLE_LIB points the checker at another build of the same source (the mutation test of
 # tests/test_oracle_mutants.py loads deliberately broken copies this way)
 LIB_PATH = os.environ.get("VARE_ORACLE_LIB") or os.path.join(HERE, "lib", "libvare_oracle.so")
+# the OpenMP build of the same source: the multi-core CPU baseline of bench.py
+OMP_LIB_PATH = os.path.join(HERE, "lib", "libvare_oracle_omp.so")
 
 FIELD_MASS, FIELD_VELO, FIELD_COMP, FIELD_MARK, FIELD_PRES, FIELD_COUNT = 0, 1, 2, 3, 4, 5
 FIELD_B, FIELD_X, FIELD_FLUID, FIELD_AVAL, FIELD_AIND = 6, 7, 8, 9, 10
@@ -33,22 +35,24 @@ class OrcStats(C.Structure):
                 ("cg_residual", C.c_double), ("b_norm", C.c_double)]
 
 
-_lib = None
+_libs = {}
 _V = C.c_void_p
 # sampler(user, world_xyz[3], *sdf, vel[3], grad[3]) -- world units
 SAMPLER = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                       C.POINTER(C.c_float))
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError("oracle not built at %s; run oracle/build.py" % LIB_PATH)
-    lib = C.CDLL(LIB_PATH)
+def load(path=None):
+    """The checker build, or another build of the same source (`path`, e.g. OMP_LIB_PATH)."""
+    path = path or LIB_PATH
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise RuntimeError("oracle not built at %s; run oracle/build.py" % path)
+    lib = C.CDLL(path)
     sig = {
         "orc_rand": (C.c_float, [C.c_uint32]),
+        "orc_set_threads": (C.c_int, [C.c_int]),
         "orc_geom": (None, [_V, _V, _V, C.c_int, _V, _V, _V, _V]),
         "orc_grid_from_voxel_size": (None, [C.c_float, _V, _V, _V]),
         "orc_create": (_V, [_V, _V, _V, C.c_int, C.POINTER(OrcParams), C.c_int]),
@@ -77,8 +81,13 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
+    _libs[path] = lib
     return lib
+
+
+def set_threads(n, path=OMP_LIB_PATH):
+    """Threads of the OpenMP build (returns the count in use)."""
+    return load(path).orc_set_threads(int(n))
 
 
 def _p(a):
@@ -162,8 +171,8 @@ def seed_box(nx, ny, nz, wx, wy, wz, ppc, seed=1):
 class Oracle:
     """One restated FLIPMethod + its particles (voxel space)."""
 
-    def __init__(self, dims, bmin=(0, 0, 0), bmax=(1, 1, 1), B=4, prm: OrcParams = None, fp64=False):
-        self.lib = load()
+    def __init__(self, dims, bmin=(0, 0, 0), bmax=(1, 1, 1), B=4, prm: OrcParams = None, fp64=False, lib_path=None):
+        self.lib = load(lib_path)
         self.dims = np.asarray(dims, np.int64)
         mn = np.asarray(bmin, np.float32)
         mx = np.asarray(bmax, np.float32)

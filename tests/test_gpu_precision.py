@@ -26,7 +26,10 @@ def _div_norm(o, comp):
     return float(np.linalg.norm(o.field(O.FIELD_B)))
 
 
-def test_pool_sphere_fp16_refined_vs_fp64_oracle():
+@pytest.mark.parametrize("apply32", [False, True])
+def test_pool_sphere_fp16_refined_vs_fp64_oracle(apply32):
+    """apply32: the refined FP64 pressure is rounded to FP32 before the air extrapolation and the
+    gradient (VARE_FLAG_FP32_APPLY); otherwise both run in FP64 on the refined pressure."""
     n, B = 64, 8
     pos = L.seed_pool_sphere(n, n, n, 0.3 * n, (0.5 * n, 0.7 * n, 0.5 * n), 0.2 * n, 8, 1)
     kw = dict(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n, solver_mode=L.SOLVER_RELATIVE,
@@ -41,7 +44,8 @@ def test_pool_sphere_fp16_refined_vs_fp64_oracle():
     p0, v0 = st["pos"], st["vel"]                    # v0 holds FP16 values exactly
     print("case pool+sphere 64^3 np=%d" % p0.shape[0], flush=True)
 
-    h = L.FlipHandle(g, L.default_params(velocity_fp16=1, **kw))
+    flags = L.FLAG_REPORT_DIVERGENCE | (L.FLAG_FP32_APPLY if apply32 else 0)
+    h = L.FlipHandle(g, L.default_params(velocity_fp16=1, flags=flags, **kw))
     h.set_particles(p0, v0)
     o64 = O.Oracle((n, n, n), B=B, prm=O.params(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n,
                                                 solver_mode=1, cg_rel_tol=1e-12, max_iterations=20000,
@@ -67,10 +71,12 @@ def test_pool_sphere_fp16_refined_vs_fp64_oracle():
     o64.stage(4, DT)
     e_pv = rel(h.get_particles()["vel"], o64.get_particles()["vel"])
     e_pp = rel(h.get_particles()["pos"], o64.get_particles()["pos"])
-    print("errors vs FP64 oracle: pressure %.2e  grid velocity %.2e  particle velocity %.2e  position %.2e  "
-          "D_mixed %.6e  D_fp64 %.6e  |dD| %.2e  refine %d  FP64 residual %.2e"
-          % (e_p, e_v, e_pv, e_pp, d_g, d_o, abs(d_g - d_o), stats["refine_steps"], stats["true_rel_residual"]),
-          flush=True)
+    print("apply %s: errors vs FP64 oracle: pressure %.2e  grid velocity %.2e  particle velocity %.2e  "
+          "position %.2e  D_mixed %.6e (on the GPU %.6e)  D_fp64 %.6e  |dD| %.2e  refine %d  FP64 residual %.2e"
+          % ("FP32" if apply32 else "FP64", e_p, e_v, e_pv, e_pp, d_g, stats["div_ratio"], d_o, abs(d_g - d_o),
+             stats["refine_steps"], stats["true_rel_residual"]), flush=True)
+    # the step's own report of D (FP32 divergence, FP64 sums) is the oracle's D of the same field
+    assert abs(stats["div_ratio"] - d_g) <= 1e-6 * d_g
     assert e_p < 1e-5
     assert e_v < 1e-5
     assert abs(d_g - d_o) <= 1e-5

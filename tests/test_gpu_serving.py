@@ -122,3 +122,73 @@ def test_pool_eviction_under_budget():
         r0, r2 = srv.result(ids[0]), srv.result(ids[2])
         assert r0[2]["status"] == 0 and r2[2]["status"] == 0
         assert np.array_equal(r0[0], r2[0]) and np.array_equal(r0[1], r2[1])   # same tank, same result
+
+
+def test_served_tank_matches_fp64_oracle():
+    # one tank through the server (MG-PCG to 1e-5 relative) against the FP64 oracle's step on the same
+    # input; the bar is the one of the stage-wise parity test: within max(2 x the FP32 oracle's own
+    # error, 1e-5) of the FP64 oracle, positions within 1e-6
+    from oracle import oracle as O
+    n = 48
+    pos = O.seed_box(n, n, n, 0.5 * n, 0.6 * n, float(n), 6, 51)
+    rng = np.random.default_rng(51)
+    vel = (rng.standard_normal(pos.shape) * 10).astype(np.float32)
+    g = L.grid_desc(n, n, n, block_size=8)
+    srv = L.Server(8 << 30, 2)
+    rid = srv.submit(g, params(n), pos, vel, steps=1, dt=DT)
+    srv.run()
+    ps, vs, st = srv.result(rid)
+    assert st["status"] == 0 and st["steps_done"] == 1
+    kw = dict(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n, solver_mode=1, max_iterations=20000,
+              preconditioner=1)
+    o64 = O.Oracle((n, n, n), B=8, prm=O.params(cg_rel_tol=1e-12, **kw), fp64=True)
+    o32 = O.Oracle((n, n, n), B=8, prm=O.params(cg_rel_tol=1e-5, **kw), fp64=False)
+    for o in (o64, o32):
+        o.set_particles(pos, vel)
+        o.step(DT)
+    r64, r32 = o64.get_particles(), o32.get_particles()
+    assert ps.shape[0] == r64["pos"].shape[0]
+    ev, ev32 = rel(vs, r64["vel"]), rel(r32["vel"], r64["vel"])
+    print("served tank vs FP64 oracle: velocity %.3g (FP32 oracle %.3g), position %.3g" % (
+        ev, ev32, rel(ps, r64["pos"])), flush=True)
+    assert rel(ps, r64["pos"]) < 1e-6
+    assert ev < max(2 * ev32, 1e-5)
+
+
+def test_poisson_stream_256_properties():
+    # the served-batch configuration at its full request count (one step each): 256 tanks with edges
+    # log-uniform in 32..128 (multiples of 8), 4..8 particles per cell, water fractions U(0.2, 0.7),
+    # Poisson arrivals, a 200 GB budget and 16 slots.  Every request succeeds, admission follows
+    # arrival order, the reserved bytes stay within the budget, and sampled tanks give exactly what a
+    # handle of their own gives.
+    rng = np.random.default_rng(7)
+    reqs = []
+    for k in range(256):
+        edge = int(8 * round(np.exp(rng.uniform(np.log(32), np.log(128))) / 8))
+        ppc = int(rng.integers(4, 9))
+        fx, fy = rng.uniform(0.2, 0.7, 2)
+        pos = L.seed_box(edge, edge, edge, fx * edge, fy * edge, float(edge), ppc, 500 + k)
+        reqs.append((edge, pos))
+    arrivals = np.cumsum(rng.exponential(0.002, 256))
+    arrivals -= arrivals[0]
+    budget = 200 << 30
+    srv = L.Server(budget, 16)
+    ids = [srv.submit(L.grid_desc(e, e, e, block_size=8), params(e), p, np.zeros_like(p), steps=1, dt=DT,
+                      arrival_s=float(a)) for (e, p), a in zip(reqs, arrivals)]
+    srv.run()
+    st = srv.stats()
+    print("stream of 256: %s" % st, flush=True)
+    res = [srv.result(i) for i in ids]
+    assert all(r[2]["status"] == 0 and r[2]["steps_done"] == 1 for r in res)
+    assert st["requests"] == 256 and st["peak_bytes"] <= budget and st["max_resident"] <= 16
+    admits = [r[2]["admit_s"] for r in res]                # ids are in arrival order
+    assert admits == sorted(admits)
+    assert all(r[2]["admit_s"] >= r[2]["arrival_s"] for r in res)
+    for k in (0, 37, 101, 180, 255):
+        e, p = reqs[k]
+        h = L.FlipHandle(L.grid_desc(e, e, e, block_size=8), params(e))
+        h.set_particles(p, np.zeros_like(p))
+        h.step(DT)
+        d = h.get_particles()
+        assert np.array_equal(res[k][0], d["pos"]) and np.array_equal(res[k][1], d["vel"]), k
+        h.close()

@@ -29,6 +29,8 @@ SOLVER_RELATIVE = 1
 PRECOND_NONE = 0
 PRECOND_JACOBI = 1
 PRECOND_MG = 2
+FLAG_REPORT_DIVERGENCE = 1
+FLAG_FP32_APPLY = 2
 
 FIELD_MASS = 0
 FIELD_VELOCITY = 1
@@ -73,7 +75,7 @@ class FlipParams(C.Structure):
                 ("max_iterations", C.c_int), ("preconditioner", C.c_int), ("mg_sweeps", C.c_int),
                 ("mg_omega", C.c_float), ("mg_coarse_sweeps", C.c_int), ("particle_path", C.c_int),
                 ("refine_tol", C.c_float), ("refine_max", C.c_int), ("velocity_fp16", C.c_int),
-                ("reserved", C.c_int * 1)]
+                ("flags", C.c_int)]
 
 
 class FrameInfo(C.Structure):
@@ -88,7 +90,7 @@ class StepStats(C.Structure):
                 ("num_cells", C.c_int64), ("num_fluid", C.c_int64), ("cg_iterations", C.c_int),
                 ("converged", C.c_int), ("residual_norm", C.c_double), ("b_norm", C.c_double),
                 ("step_ms", C.c_double), ("solve_ms", C.c_double), ("refine_steps", C.c_int), ("pad", C.c_int),
-                ("true_rel_residual", C.c_double)]
+                ("true_rel_residual", C.c_double), ("div_ratio", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

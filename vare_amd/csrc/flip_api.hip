@@ -1090,7 +1090,10 @@ template <int B> int stage_solve(vare_flip* h) {
     HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     tbegin(h, VARE_KERNEL_APPLY);
-    if (nc && refine) {
+    const bool apply64 = refine && !(h->prm.flags & VARE_FLAG_FP32_APPLY);
+    if (nc && refine && !apply64)       // the refined pressure, rounded once, applied in FP32
+        hipLaunchKernelGGL(k_ir_round, dim3(g), dim3(WG), 0, h->stream, (int)nc, (const double*)h->x64, h->x);
+    if (nc && apply64) {
         // pressure apply in FP64 on the refined pressure; the FP32 field keeps its rounding
         hipLaunchKernelGGL((k_press_extrap_blk<B, double>), dim3(h->nb), dim3(B * B * B), 0, h->stream, h->frame(),
                            (const uint8_t*)h->type, h->x64);
@@ -1110,6 +1113,20 @@ template <int B> int stage_solve(vare_flip* h) {
     tend(h, VARE_KERNEL_APPLY);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(h->step_ev[3], h->stream));
+    h->stats.div_ratio = 0.0;
+    if (nc && (h->prm.flags & VARE_FLAG_REPORT_DIVERGENCE)) {
+        // the reference's divergence of the projected velocity on the same classification
+        hipLaunchKernelGGL((k_divergence<B, true>), dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+                           (const uint8_t*)h->type, (const float*)h->cx, (const float*)h->cy, (const float*)h->cz,
+                           h->vox_wall, vecs(h), h->partials, h->partials2, h->bfluid);
+        hipLaunchKernelGGL(k_fin_divpost, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
+                           (int)g);
+        HIPCHK(hipGetLastError());
+        CgState* hs = h->st_host;
+        HIPCHK(hipMemcpyAsync(&hs->div_bb, &h->st->div_bb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        h->stats.div_ratio = hs->ir_bb > 0.0 ? std::sqrt(std::max(hs->div_bb, 0.0) / hs->ir_bb) : 0.0;
+    }
     const CgState& s = *h->st_host;
     h->stats.cg_iterations = refine ? total_iters : s.count;
     h->stats.converged = refine ? s.ir_done : s.converged;

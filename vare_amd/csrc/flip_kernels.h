@@ -332,6 +332,7 @@ struct CgState {
     double ir_bb;     // |b|^2 of the original system (refinement)
     double ir_rr;     // |b - A x64|^2 of the last refinement step
     double true_rr;   // |b - A x|^2 recomputed in FP64 from the final FP32 pressure
+    double div_bb;    // |b(v_post)|^2: the divergence rule applied to the projected velocity
 };
 
 // Cells visited by the solver's vector kernels: the cells of the listed blocks (those holding a
@@ -351,7 +352,9 @@ struct OpFluidBlocks {
     __device__ void emit(int b, int pre, int v) const { if (v) list[pre] = b; }
 };
 
-template <int B>
+// NormOnly: only |b|^2 of the given velocity (the post-projection divergence report); the solver's
+// vectors and block flags are left alone.
+template <int B, bool NormOnly = false>
 __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type, const float* __restrict__ cx,
                              const float* __restrict__ cy, const float* __restrict__ cz, float wall_level, CgVecs v,
                              double* __restrict__ partials, double* __restrict__ partials2, int* __restrict__ bfluid) {
@@ -407,19 +410,26 @@ __global__ void k_divergence(int nc, FrameK f, const uint8_t* __restrict__ type,
             info = (uint16_t)(kInfoFluid | mask | (diag << 8));
             nfl += 1.0;
         }
+        bb += (double)d * (double)d;
+        if (NormOnly) continue;
         v.b[c] = d;
         v.info[c] = info;
         v.x[c] = 0.f;
         v.r[c] = d;
         v.p[c] = d;
         v.ap[c] = 0.f;
-        bb += (double)d * (double)d;
         if (__ballot(info != 0) && (threadIdx.x & 63) == 0) bfluid[c / Blk<B>::B3] = 1;
     }
     const double s = block_sum(bb, sh);
+    if (NormOnly) {
+        if (threadIdx.x == 0) partials[blockIdx.x] = s;
+        return;
+    }
     const double s2 = block_sum(nfl, sh);
     if (threadIdx.x == 0) { partials[blockIdx.x] = s; partials2[blockIdx.x] = s2; }
 }
+
+__global__ void k_fin_divpost(CgState* __restrict__ st, const double* __restrict__ partials, int n);
 
 // Deterministic sum of n partials by one 256-thread workgroup.  Each thread keeps 8 running sums over
 // strided partials, so 8 loads are in flight per thread, and combines them in a fixed order.
@@ -477,6 +487,13 @@ __global__ void __launch_bounds__(kFinThreads) k_fin_divergence(CgState* __restr
     } else {
         if (tot == 0.0) { st->done = 1; st->converged = 1; }
     }
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_fin_divpost(CgState* __restrict__ st, const double* __restrict__ partials,
+                                                               int n) {
+    __shared__ double sh[kFinWaves];
+    const double tot = fin_sum(partials, n, sh);
+    if (threadIdx.x == 0) st->div_bb = tot;
 }
 
 // Jacobi preconditioner value for a cell (0 off the fluid set).
