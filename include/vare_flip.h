@@ -139,6 +139,7 @@ typedef struct vare_step_stats {
 #define VARE_FIELD_MARK     3   /* uint64  (_sMarkField: 0 air, 1 solid, 4+fluid rank) */
 #define VARE_FIELD_PRESSURE 4   /* float   (_sPresField) */
 #define VARE_FIELD_COUNT    5   /* float   (_sSurfField after PointsToCountField) */
+#define VARE_FIELD_SURFACE  5   /* float   (_sSurfField after vare_flip_post_process: the level set) */
 #define VARE_FIELD_DIV      6   /* float   right-hand side b per cell (0 off the fluid set) */
 
 typedef struct vare_flip vare_flip;
@@ -172,7 +173,13 @@ int vare_flip_get_frame(const vare_flip* h, vare_frame_info* out);
 /* FLIPMethod::newParticle (FLIPMethod.h:75-99), batched: appends n particles with zero
  * vorticity, curvature and adjacency.  pos/vel are xyz-interleaved float[3n]. */
 int vare_flip_add_particles(vare_flip* h, const float* pos, const float* vel, size_t n, int space, int mem);
-/* Replace the whole particle set (checkpoint resume).  vort/curv/adj may be NULL (zeros). */
+/* Replace the whole particle set (checkpoint resume).  vort/curv/adj may be NULL (zeros).
+ * `space` (here, in add_particles and get_particles) names the space of the caller's positions and
+ * velocities relative to voxel-space storage: VARE_SPACE_WORLD converts world -> voxel on the way in
+ * (voxel -> world on the way out), VARE_SPACE_VOXEL copies as is.  The handle does not track which
+ * space its particles are in: particles_to_voxel / particles_to_world and advance_frame convert them
+ * in place, as FLIPMethod does its Particles, so a handle driven by advance_frame takes and returns
+ * world coordinates with VARE_SPACE_VOXEL (no conversion). */
 int vare_flip_set_particles(vare_flip* h, const float* pos, const float* vel, const float* vort,
                             const float* curv, const float* adj, size_t n, int space, int mem);
 /* Read particles back in the reference's order (Array::remove order, Array.h:995-1024).
@@ -201,9 +208,31 @@ int vare_flip_step(vare_flip* h, float dt);
  * The multigrid levels are allocated by vare_flip_create / vare_flip_set_params. */
 #define VARE_NOALLOC_SCRATCH_BYTES (4u << 20)
 int vare_flip_step_noalloc(vare_flip* h, float dt);
-/* FLIPMethod::advanceOneFrame (FLIPMethod.cu:177-206) minus postProcess (surfacing, curl,
- * vorticity and curvature are outside the built path): toVoxel, step(params.dt), toWorld. */
+/* FLIPMethod::advanceOneFrame (FLIPMethod.cu:177-206): toVoxel, step(params.dt), postProcess,
+ * toWorld. */
 int vare_flip_advance_frame(vare_flip* h);
+/* FLIPMethod::postProcess (FLIPMethod.cu:300-405) on the voxel-space particles: rebuild the frame
+ * from them, splat the surface level set and the velocity (Rasterization.cu:311-393), with the water
+ * tank fill the enclosed air below its level (host-side, as the reference), redistance the surface
+ * redist_iteration times (Rasterization.cu:395-490), and update each particle's vorticity (curl of
+ * the splatted velocity) and curvature (divergence of the surface normals).  Afterwards
+ * VARE_FIELD_SURFACE holds the level set, VARE_FIELD_VELOCITY the splatted velocity and
+ * VARE_FIELD_COMP the normals.  The reference's splat sums with float atomics in no fixed order;
+ * this one sums each cell in a fixed order, so the results repeat from run to run. */
+int vare_flip_post_process(vare_flip* h);
+
+/* Sparse -> VDB export (FieldCast.h:14-47, :362-440, as SOP_FlipWater.cpp:251-275 calls it after a
+ * frame): the voxels a caller writes with openvdb accessor setValue, and the grid transform.
+ *   surface:  every active cell, phi = level set * dx (world units); background = 10 * dx; with the
+ *             water tank also the reference's 3-cell shell outside the domain below the level
+ *             (min(world cell centre y - wall_level, value)).
+ *   velocity: every active cell whose world velocity has length >= 1e-30, in world units.
+ * xform[4] = {dx, translate xyz}: openvdb Transform::createLinearTransform(dx) post-translated by
+ * the frame's min corner + dx/2.  With ijk and values NULL only *n is written.  Valid after
+ * vare_flip_post_process (VARE_EINVAL before). */
+int vare_flip_export_surface(vare_flip* h, int32_t* ijk, float* phi, size_t cap, size_t* n, float* background,
+                             float xform[4]);
+int vare_flip_export_velocity(vare_flip* h, int32_t* ijk, float* vel, size_t cap, size_t* n, float xform[4]);
 
 /* Stages of the step, each equal to the named part of FLIPMethod::advanceOneStep:
  *   0 prepareNextStep / SparseFrame::buildFromPoints   (SparseFrame.cu:127-199)

@@ -821,6 +821,254 @@ template <class R> struct State {
         data.resize(size - del);
     }
 
+    // ---- FLIPMethod::postProcess (FLIPMethod.cu:300-405) ----
+    // Scalar lerp with a background (SparseField::lerp, SparseField.h:150-173)
+    R lerp_scalar(const std::vector<R>& f, R bg, const V3<R>& p) const {
+        int64_t i, j, k;
+        R x, y, z;
+        lerp_corner(p, i, j, k, x, y, z);
+        auto F = [&](int64_t a, int64_t b2, int64_t c) { return at(f, bg, a, b2, c); };
+        const R c00 = F(i, j, k) * (R(1) - x) + F(i + 1, j, k) * x;
+        const R c01 = F(i, j, k + 1) * (R(1) - x) + F(i + 1, j, k + 1) * x;
+        const R c10 = F(i, j + 1, k) * (R(1) - x) + F(i + 1, j + 1, k) * x;
+        const R c11 = F(i, j + 1, k + 1) * (R(1) - x) + F(i + 1, j + 1, k + 1) * x;
+        const R c0 = c00 * (R(1) - y) + c10 * y;
+        const R c1 = c01 * (R(1) - y) + c11 * y;
+        return c0 * (R(1) - z) + c1 * z;
+    }
+
+    std::vector<R> splat_w;          // the weight sums of the splat (the reference keeps them in _sPresField)
+
+    // Rasterization::ParticlesToSurfaceAndVelocity (Rasterization.cu:311-393), influence 1.5, droplet
+    // 0.5: surf <- |weighted mean position - cell centre| - r (5 where no weight), velo <- weighted
+    // mean velocity (0 where no weight); the weighted mean position lands in comp.
+    void splat_surface_velocity(R influence, R droplet) {
+        const size_t nc = surf.size();
+        std::vector<R> sw(nc, R(0));
+        std::fill(surf.begin(), surf.end(), R(0));
+        std::fill(comp.begin(), comp.end(), V3<R>{0, 0, 0});
+        std::fill(velo.begin(), velo.end(), V3<R>{0, 0, 0});
+        const int off = (int)influence;                           // neighborCells(p, min, max, (int)h)
+        for (size_t n = 0; n < pos.size(); ++n) {
+            const V3<R>& p = pos[n];
+            const V3<R>& v = vel[n];
+            if (!inside(p)) continue;
+            const int64_t mn[3] = {Clamp((int)p.x - off, 0, (int)g.nx - 1), Clamp((int)p.y - off, 0, (int)g.ny - 1),
+                                   Clamp((int)p.z - off, 0, (int)g.nz - 1)};
+            const int64_t mx[3] = {Clamp((int)p.x + off, 0, (int)g.nx - 1), Clamp((int)p.y + off, 0, (int)g.ny - 1),
+                                   Clamp((int)p.z + off, 0, (int)g.nz - 1)};
+            for (int64_t k = mn[2]; k <= mx[2]; ++k)
+                for (int64_t j = mn[1]; j <= mx[1]; ++j)
+                    for (int64_t i = mn[0]; i <= mx[0]; ++i) {
+                        const R cx = (R)((float)i + 0.5f), cy = (R)((float)j + 0.5f), cz = (R)((float)k + 0.5f);
+                        const R dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+                        const R dist = std::sqrt(dx * dx + dy * dy + dz * dz);     // Vector3::length
+                        const R sr = dist / influence;
+                        const R t = R(1) - sr * sr;
+                        const R w = Max(R(0), t * t * t);                          // Pow3
+                        const uint64_t idx = find_index(i, j, k);
+                        if (idx == NULL_MAX) continue;
+                        sw[idx] += w;
+                        velo[idx].x += w * v.x;
+                        velo[idx].y += w * v.y;
+                        velo[idx].z += w * v.z;
+                        surf[idx] += w;
+                        comp[idx].x += w * p.x;
+                        comp[idx].y += w * p.y;
+                        comp[idx].z += w * p.z;
+                    }
+        }
+        for (size_t n = 0; n < nc; ++n) {                          // kernel2, :375-392
+            if (surf[n] > R(1e-30f)) {
+                int64_t i, j, k;
+                find_indices(n, i, j, k);
+                comp[n].x /= surf[n];
+                comp[n].y /= surf[n];
+                comp[n].z /= surf[n];
+                const R ex = comp[n].x - (R)((float)i + 0.5f), ey = comp[n].y - (R)((float)j + 0.5f),
+                        ez = comp[n].z - (R)((float)k + 0.5f);
+                surf[n] = std::sqrt(ex * ex + ey * ey + ez * ez) - droplet;       // Vector3::distanceTo
+            } else {
+                surf[n] = R(5);
+            }
+            if (sw[n] > R(1e-30f)) {
+                velo[n].x /= sw[n];
+                velo[n].y /= sw[n];
+                velo[n].z /= sw[n];
+            } else {
+                velo[n] = V3<R>{0, 0, 0};
+            }
+        }
+        splat_w = sw;
+    }
+
+    // FLIPMethod.cu:320-363 (water tank): air cells of the surface field below the tank level that
+    // are not connected to the level's layer (flood fill from it, host-serial in the reference as
+    // here) become liquid (-5).
+    void tank_flood_fill() {
+        const int64_t nx = g.nx, ny = g.ny, nz = g.nz;
+        std::vector<char> scan((size_t)(nx * ny * nz), 0);
+        auto S = [&](int64_t i, int64_t j, int64_t k) -> char& { return scan[(size_t)(i + nx * (j + ny * k))]; };
+        auto F = [&](int64_t i, int64_t j, int64_t k) { return at(surf, surf_bg, i, j, k); };
+        std::vector<Idx3> cells;
+        cells.push_back(Idx3{nx - 1, ny - 1, nz - 1});
+        const int64_t wall = (int64_t)(size_t)(float)vox_wall;
+        if (wall >= 0 && wall < ny)                               // (the reference reads out of range otherwise)
+            for (int64_t k = 0; k < nz; ++k)
+                for (int64_t i = 0; i < nx; ++i)
+                    if (F(i, wall, k) > R(0)) cells.push_back(Idx3{i, wall, k});
+        while (!cells.empty()) {
+            const Idx3 c = cells.back();
+            cells.pop_back();
+            S(c.i, c.j, c.k) = 1;
+            if (c.j > wall) continue;
+            const int64_t d[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+            for (int q = 0; q < 6; ++q) {
+                const int64_t i = c.i + d[q][0], j = c.j + d[q][1], k = c.k + d[q][2];
+                if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) continue;
+                if (F(i, j, k) > R(0) && S(i, j, k) == 0) cells.push_back(Idx3{i, j, k});
+            }
+        }
+        for (size_t n = 0; n < surf.size(); ++n) {                // mm_kernel, :351-360
+            int64_t i, j, k;
+            find_indices(n, i, j, k);
+            if (j > wall) continue;
+            if (surf[n] > R(0) && S(i, j, k) == 0) surf[n] = R(-5);
+        }
+    }
+
+    // Rasterization::ReinitializeSurface (Rasterization.cu:395-490): `iter` Godunov steps of
+    // d(phi)/dt = S(phi) (1 - |grad phi|), dt 0.3, both fields with background 10.
+    void reinitialize_surface(int iter, bool tank, R waterlevel) {
+        const int64_t EI = g.nx - 1, EJ = g.ny - 1, EK = g.nz - 1;
+        const R dt = R(0.3f);
+        std::vector<R> tmp(surf.size());
+        for (int it = 0; it < iter; ++it) {
+            auto F = [&](int64_t i, int64_t j, int64_t k) { return at(surf, R(10), i, j, k); };
+            for (size_t n = 0; n < surf.size(); ++n) {
+                int64_t i, j, k;
+                find_indices(n, i, j, k);
+                const R v = surf[n];
+                const R sign = v / std::sqrt(v * v + R(1));
+                R v_i0, v_i1, v_j0, v_j1, v_k0, v_k1;
+                if (tank) {
+                    const R ll = (R)((float)j + 0.5f) - waterlevel;
+                    v_j0 = j == 0 ? ll - R(1) : F(i, j - 1, k);
+                    v_j1 = j == EJ ? ll + R(1) : F(i, j + 1, k);
+                    v_i0 = i == 0 ? ll : F(i - 1, j, k);
+                    v_i1 = i == EI ? ll : F(i + 1, j, k);
+                    v_k0 = k == 0 ? ll : F(i, j, k - 1);
+                    v_k1 = k == EK ? ll : F(i, j, k + 1);
+                } else {
+                    v_i0 = i == 0 ? R(2) * v - F(i + 1, j, k) : F(i - 1, j, k);
+                    v_i1 = i == EI ? R(2) * v - F(i - 1, j, k) : F(i + 1, j, k);
+                    v_j0 = j == 0 ? R(2) * v - F(i, j + 1, k) : F(i, j - 1, k);
+                    v_j1 = j == EJ ? R(2) * v - F(i, j - 1, k) : F(i, j + 1, k);
+                    v_k0 = k == 0 ? R(2) * v - F(i, j, k + 1) : F(i, j, k - 1);
+                    v_k1 = k == EK ? R(2) * v - F(i, j, k - 1) : F(i, j, k + 1);
+                }
+                const R dx0 = v - v_i0, dx1 = v_i1 - v;
+                const R dy0 = v - v_j0, dy1 = v_j1 - v;
+                const R dz0 = v - v_k0, dz1 = v_k1 - v;
+                auto P2 = [](R a) { return a * a; };
+                const R gp = std::sqrt(P2(Max(dx0, R(0))) + P2(Min(dx1, R(0))) + P2(Max(dy0, R(0))) + P2(Min(dy1, R(0))) +
+                                       P2(Max(dz0, R(0))) + P2(Min(dz1, R(0))));
+                const R gm = std::sqrt(P2(Min(dx0, R(0))) + P2(Max(dx1, R(0))) + P2(Min(dy0, R(0))) + P2(Max(dy1, R(0))) +
+                                       P2(Min(dz0, R(0))) + P2(Max(dz1, R(0))));
+                tmp[n] = v - dt * Max(sign, R(0)) * (gp - R(1)) - dt * Min(sign, R(0)) * (gm - R(1));
+            }
+            surf.swap(tmp);
+        }
+    }
+
+    // FieldOP::curl (FieldOP.cu:338-369): central differences, domain faces extrapolated (2v - v_in).
+    void curl_field(const std::vector<V3<R>>& vf, std::vector<V3<R>>& out) const {
+        const int64_t EI = g.nx - 1, EJ = g.ny - 1, EK = g.nz - 1;
+        const V3<R> bg{0, 0, 0};
+        auto F = [&](int64_t i, int64_t j, int64_t k) { return at(vf, bg, i, j, k); };
+        auto ext = [](const V3<R>& v, const V3<R>& o) { return V3<R>{v.x * R(2) - o.x, v.y * R(2) - o.y, v.z * R(2) - o.z}; };
+        out.assign(vf.size(), bg);
+        for (size_t n = 0; n < vf.size(); ++n) {
+            int64_t i, j, k;
+            find_indices(n, i, j, k);
+            const V3<R> v = vf[n];
+            const V3<R> x0 = i == 0 ? ext(v, F(i + 1, j, k)) : F(i - 1, j, k);
+            const V3<R> x1 = i == EI ? ext(v, F(i - 1, j, k)) : F(i + 1, j, k);
+            const V3<R> y0 = j == 0 ? ext(v, F(i, j + 1, k)) : F(i, j - 1, k);
+            const V3<R> y1 = j == EJ ? ext(v, F(i, j - 1, k)) : F(i, j + 1, k);
+            const V3<R> z0 = k == 0 ? ext(v, F(i, j, k + 1)) : F(i, j, k - 1);
+            const V3<R> z1 = k == EK ? ext(v, F(i, j, k - 1)) : F(i, j, k + 1);
+            out[n] = V3<R>{(y1.z - y0.z) * R(0.5) - (z1.y - z0.y) * R(0.5), (z1.x - z0.x) * R(0.5) - (x1.z - x0.z) * R(0.5),
+                           (x1.y - x0.y) * R(0.5) - (y1.x - y0.x) * R(0.5)};
+        }
+    }
+
+    // FieldOP::normalGradient (FieldOP.cu:80-110): normalised central-difference gradient.
+    void normal_gradient(const std::vector<R>& sf, R bg, std::vector<V3<R>>& out) const {
+        const int64_t EI = g.nx - 1, EJ = g.ny - 1, EK = g.nz - 1;
+        auto F = [&](int64_t i, int64_t j, int64_t k) { return at(sf, bg, i, j, k); };
+        out.assign(sf.size(), V3<R>{0, 0, 0});
+        for (size_t n = 0; n < sf.size(); ++n) {
+            int64_t i, j, k;
+            find_indices(n, i, j, k);
+            const R v = sf[n];
+            const R i0 = i == 0 ? R(2) * v - F(i + 1, j, k) : F(i - 1, j, k);
+            const R i1 = i == EI ? R(2) * v - F(i - 1, j, k) : F(i + 1, j, k);
+            const R j0 = j == 0 ? R(2) * v - F(i, j + 1, k) : F(i, j - 1, k);
+            const R j1 = j == EJ ? R(2) * v - F(i, j - 1, k) : F(i, j + 1, k);
+            const R k0 = k == 0 ? R(2) * v - F(i, j, k + 1) : F(i, j, k - 1);
+            const R k1 = k == EK ? R(2) * v - F(i, j, k - 1) : F(i, j, k + 1);
+            out[n] = normalized(V3<R>{(i1 - i0) * R(0.5), (j1 - j0) * R(0.5), (k1 - k0) * R(0.5)});
+        }
+    }
+
+    // FieldOP::curvature (FieldOP.cu:273-301): central-difference divergence of a vector field.
+    void divergence_of(const std::vector<V3<R>>& nf, std::vector<R>& out) const {
+        const int64_t EI = g.nx - 1, EJ = g.ny - 1, EK = g.nz - 1;
+        const V3<R> bg{0, 0, 0};
+        auto F = [&](int64_t i, int64_t j, int64_t k) { return at(nf, bg, i, j, k); };
+        out.assign(nf.size(), R(0));
+        for (size_t n = 0; n < nf.size(); ++n) {
+            int64_t i, j, k;
+            find_indices(n, i, j, k);
+            const V3<R> s = nf[n];
+            const R i0 = i == 0 ? s.x * R(2) - F(i + 1, j, k).x : F(i - 1, j, k).x;
+            const R i1 = i == EI ? s.x * R(2) - F(i - 1, j, k).x : F(i + 1, j, k).x;
+            const R j0 = j == 0 ? s.y * R(2) - F(i, j + 1, k).y : F(i, j - 1, k).y;
+            const R j1 = j == EJ ? s.y * R(2) - F(i, j - 1, k).y : F(i, j + 1, k).y;
+            const R k0 = k == 0 ? s.z * R(2) - F(i, j, k + 1).z : F(i, j, k - 1).z;
+            const R k1 = k == EK ? s.z * R(2) - F(i, j, k - 1).z : F(i, j, k + 1).z;
+            out[n] = (i1 - i0) * R(0.5) + (j1 - j0) * R(0.5) + (k1 - k0) * R(0.5);
+        }
+    }
+
+    // FLIPMethod::postProcess (FLIPMethod.cu:300-405), voxel space; dt = params.dt.
+    void post_process() {
+        build_from_points();                                        // prepareNextStep, :302
+        splat_surface_velocity(R(1.5f), R(0.5f));                   // :318
+        post_after_splat();
+    }
+
+    // FLIPMethod.cu:320-404: everything after the splat, on the current surf and velo fields.
+    void post_after_splat() {
+        if (prm.enable_water_tank) tank_flood_fill();               // :320-363
+        reinitialize_surface(prm.redist_iteration, prm.enable_water_tank != 0, vox_wall);   // :365
+        curl_field(velo, comp);                                     // :367
+        const R dis = (R)std::pow(1.f - prm.vort_pres_rate, prm.dt); // :373
+        for (size_t n = 0; n < pos.size(); ++n) {                   // vort_kernel, :375-385
+            const V3<R> c = lerp_vec(comp, pos[n]);
+            const V3<R> wv{c.x * (R)g.s[0], c.y * (R)g.s[1], c.z * (R)g.s[2]};   // Grid::worldVector
+            R v = std::sqrt(wv.x * wv.x + wv.y * wv.y + wv.z * wv.z);
+            v = Max(R(0), v - R(0.2f));
+            vort[n] -= vort[n] * dis;
+            vort[n] = Max(v, vort[n]);
+        }
+        normal_gradient(surf, surf_bg, comp);                       // :390
+        divergence_of(comp, pres);                                  // :391
+        for (size_t n = 0; n < pos.size(); ++n) curv[n] = lerp_scalar(pres, R(10), pos[n]);   // :393-398, pres bg 10
+        std::fill(pres.begin(), pres.end(), R(0));                  // :402-404
+    }
+
     // ---- FLIPMethod::advanceOneStep (FLIPMethod.cu:208-298) ----
     int step(R dt) {
         build_from_points();                        // prepareNextStep, :49-52 / :210
@@ -1050,6 +1298,32 @@ void orc_set_field(void* hp, int which, const double* in) {
         if (which == 4) for (size_t n = 0; n < s->pres.size(); ++n) s->pres[n] = (R)in[n];
         if (which == 5) for (size_t n = 0; n < s->surf.size(); ++n) s->surf[n] = (R)in[n];
         if (which == 7) for (size_t n = 0; n < s->x.size(); ++n) s->x[n] = (R)in[n];
+    };
+    if (h->fp64) run(h->s64); else run(h->s32);
+}
+
+// FLIPMethod::postProcess (FLIPMethod.cu:300-405) on the current (voxel-space) particles.
+void orc_post_process(void* hp) {
+    Handle* h = (Handle*)hp;
+    if (h->fp64) h->s64->post_process(); else h->s32->post_process();
+}
+
+// Single post-process parts on the current fields (pins): 0 splat, 1 tank flood fill, 2 one
+// redistancing iteration (arg = tank flag), 3 curl velo -> comp, 4 normal gradient surf -> comp,
+// 5 divergence comp -> pres, 6 the whole post-process after the splat (FLIPMethod.cu:320-404).
+void orc_post_part(void* hp, int part, int arg) {
+    Handle* h = (Handle*)hp;
+    auto run = [&](auto* s) {
+        using R = typename std::remove_reference<decltype(s->mass[0])>::type;
+        switch (part) {
+            case 0: s->splat_surface_velocity(R(1.5f), R(0.5f)); break;
+            case 1: s->tank_flood_fill(); break;
+            case 2: s->reinitialize_surface(1, arg != 0, s->vox_wall); break;
+            case 3: { std::vector<V3<R>> o; s->curl_field(s->velo, o); s->comp = o; } break;
+            case 4: { std::vector<V3<R>> o; s->normal_gradient(s->surf, s->surf_bg, o); s->comp = o; } break;
+            case 5: s->divergence_of(s->comp, s->pres); break;
+            case 6: s->post_after_splat(); break;
+        }
     };
     if (h->fp64) run(h->s64); else run(h->s32);
 }

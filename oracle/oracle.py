@@ -67,6 +67,8 @@ def load(path=None):
         "orc_get_field": (C.c_size_t, [_V, C.c_int, _V]),
         "orc_set_field": (None, [_V, C.c_int, _V]),
         "orc_apply_pressure": (None, [_V]),
+        "orc_post_process": (None, [_V]),
+        "orc_post_part": (None, [_V, C.c_int, C.c_int]),
         "orc_get_removed": (C.c_size_t, [_V, _V]),
         "orc_get_stats": (None, [_V, C.POINTER(OrcStats)]),
         "orc_remove_perm": (C.c_size_t, [_V, C.c_size_t, _V]),
@@ -246,6 +248,16 @@ class Oracle:
     def apply_pressure(self):
         """K10-K12 (FluidProjection.cu:297-378) on the x set with set_field(FIELD_X, ...)."""
         self.lib.orc_apply_pressure(self.h)
+
+    def post_process(self):
+        """FLIPMethod::postProcess (FLIPMethod.cu:300-405) on the voxel-space particles."""
+        self.lib.orc_post_process(self.h)
+
+    def post_part(self, part, arg=0):
+        """One part of the post-process on the current fields: 0 splat, 1 tank flood fill, 2 one
+        redistancing iteration (arg: tank), 3 curl velo->comp, 4 normal gradient surf->comp,
+        5 divergence comp->pres, 6 the whole post-process after the splat (FLIPMethod.cu:320-404)."""
+        self.lib.orc_post_part(self.h, int(part), int(arg))
 
     def removed(self):
         out = np.empty(self.lib.orc_get_removed(self.h, None), np.uint8)

@@ -36,9 +36,10 @@ class Frame:
         prm.update(kw)
         self.n = n
         self.o = O.Oracle((n, n, n), B=4, prm=O.params(**prm), fp64=fp64)
-        # one particle in the centre block activates the whole 3x3x3 block neighbourhood
-        c = np.full((1, 3), n / 2 + 0.25, np.float32)
-        self.o.set_particles(c, np.zeros((1, 3), np.float32))
+        # one particle per block activates every block
+        b = (np.arange(n // 4) * 4 + 2.25).astype(np.float32)
+        c = np.stack(np.meshgrid(b, b, b, indexing="ij"), -1).reshape(-1, 3)
+        self.o.set_particles(c, np.zeros_like(c))
         self.o.stage(0)
         self.o.stage(1)
         _, coords = self.o.block_table()

@@ -1,11 +1,12 @@
-"""Mutation check of the oracle's boundary pins (CPU only).
+"""Mutation check of the oracle's pins (CPU only).
 
 Each mutant is a copy of oracle/flip_oracle.cpp with ONE plausible restatement mistake planted in the
 boundary rules of FluidProjection::buildLinearSystem (base/source/FluidProjection.cu:206-244) or
-FluidProjection::solve (:297-378): a flipped sign, a swapped neighbour index, a dropped
-extrapolation, a wrong divisor.  The copy is compiled into a temporary directory and
-tests/test_oracle_boundary.py is run against it (VARE_ORACLE_LIB); every mutant must make that file
-fail, and the unmodified source compiled the same way must pass.  The mistakes live only in the
+FluidProjection::solve (:297-378), or in the frame post-process (FLIPMethod.cu:300-405): a flipped
+sign, a swapped neighbour index or component, a dropped extrapolation or normalisation, a wrong
+divisor or exponent.  The copy is compiled into a temporary directory and the pins
+(tests/test_oracle_boundary.py, tests/test_oracle_post.py) are run against it (VARE_ORACLE_LIB);
+every mutant must make its pin file fail, and the unmodified source compiled the same way must pass.  The mistakes live only in the
 temporary copies: the oracle itself is never changed.
 """
 import os
@@ -48,10 +49,28 @@ MUTANTS = [
      "((float)j + 0.5f <= wallLv + 1.f) ? (uint64_t)kAir : (uint64_t)kSolid"),
 ]
 
+# FLIPMethod::postProcess (FLIPMethod.cu:300-405), pinned by tests/test_oracle_post.py
+POST_PINS = os.path.join(ROOT, "tests", "test_oracle_post.py")
+POST_MUTANTS = [
+    ("splat: droplet added", "surf[n] = std::sqrt(ex * ex + ey * ey + ez * ez) - droplet;",
+     "surf[n] = std::sqrt(ex * ex + ey * ey + ez * ez) + droplet;"),
+    ("splat: kernel power", "const R w = Max(R(0), t * t * t);", "const R w = Max(R(0), t * t);"),
+    ("redistance: upwind sides swapped", "P2(Max(dx0, R(0))) + P2(Min(dx1, R(0)))", "P2(Min(dx0, R(0))) + P2(Max(dx1, R(0)))"),
+    ("redistance: update sign", "tmp[n] = v - dt * Max(sign, R(0)) * (gp - R(1))", "tmp[n] = v + dt * Max(sign, R(0)) * (gp - R(1))"),
+    ("redistance: tank boundary below", "v_j0 = j == 0 ? ll - R(1) : F(i, j - 1, k);", "v_j0 = j == 0 ? ll + R(1) : F(i, j - 1, k);"),
+    ("curl: y component reversed", "(z1.x - z0.x) * R(0.5) - (x1.z - x0.z) * R(0.5)", "(x1.z - x0.z) * R(0.5) - (z1.x - z0.x) * R(0.5)"),
+    ("normal gradient not normalised", "out[n] = normalized(V3<R>{(i1 - i0) * R(0.5), (j1 - j0) * R(0.5), (k1 - k0) * R(0.5)});",
+     "out[n] = V3<R>{(i1 - i0) * R(0.5), (j1 - j0) * R(0.5), (k1 - k0) * R(0.5)};"),
+    ("divergence: wrong component", "const R j1 = j == EJ ? s.y * R(2) - F(i, j - 1, k).y : F(i, j + 1, k).y;",
+     "const R j1 = j == EJ ? s.y * R(2) - F(i, j - 1, k).y : F(i, j + 1, k).x;"),
+    ("vorticity: decay factor", "vort[n] -= vort[n] * dis;", "vort[n] *= dis;"),
+    ("vorticity: threshold sign", "v = Max(R(0), v - R(0.2f));", "v = Max(R(0), v + R(0.2f));"),
+]
+
 FLAGS = ["-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math"]
 
 
-def _build_and_run(src_text, workdir, tag):
+def _build_and_run(src_text, workdir, tag, pins=PINS):
     cpp = os.path.join(workdir, tag + ".cpp")
     so = os.path.join(workdir, tag + ".so")
     with open(cpp, "w") as f:
@@ -60,20 +79,21 @@ def _build_and_run(src_text, workdir, tag):
     if b.returncode != 0:
         return "build failed: " + b.stderr[-400:]
     env = dict(os.environ, VARE_ORACLE_LIB=so)
-    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", PINS], cwd=ROOT,
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", pins], cwd=ROOT,
                        env=env, capture_output=True, text=True, timeout=300)
     return r.returncode
 
 
-def test_every_planted_mistake_fails_a_boundary_pin():
+@pytest.mark.parametrize("mutants,pins", [(MUTANTS, PINS), (POST_MUTANTS, POST_PINS)], ids=["boundary", "post"])
+def test_every_planted_mistake_fails_a_pin(mutants, pins):
     src = open(SRC).read()
-    for name, old, new in MUTANTS:
+    for name, old, new in mutants:
         assert src.count(old) == 1, "mutant %r: text must occur exactly once" % name
     with tempfile.TemporaryDirectory() as d:
-        jobs = [("control", src)] + [("m%02d" % q, src.replace(old, new)) for q, (_, old, new) in enumerate(MUTANTS)]
+        jobs = [("control", src)] + [("m%02d" % q, src.replace(old, new)) for q, (_, old, new) in enumerate(mutants)]
         with ThreadPoolExecutor(max_workers=4) as ex:
-            res = dict(zip([t for t, _ in jobs], ex.map(lambda j: _build_and_run(j[1], d, j[0]), jobs)))
+            res = dict(zip([t for t, _ in jobs], ex.map(lambda j: _build_and_run(j[1], d, j[0], pins), jobs)))
     assert res["control"] == 0, "unmodified oracle must pass the pins: %r" % (res["control"],)
-    survivors = [MUTANTS[q][0] for q in range(len(MUTANTS)) if res["m%02d" % q] != 1]
+    survivors = [mutants[q][0] for q in range(len(mutants)) if res["m%02d" % q] != 1]
     assert not survivors, "mutants not caught (pytest status != 1): %s" % [
-        (n, res["m%02d" % q]) for q, (n, _, _) in enumerate(MUTANTS) if n in survivors]
+        (n, res["m%02d" % q]) for q, (n, _, _) in enumerate(mutants) if n in survivors]

@@ -38,6 +38,7 @@ FIELD_COMP = 2
 FIELD_MARK = 3
 FIELD_PRESSURE = 4
 FIELD_COUNT = 5
+FIELD_SURFACE = 5
 FIELD_DIV = 6
 
 KERNEL_NAMES = ["spmv", "update", "p2g", "g2p", "bin", "div", "apply", "pupdate", "mg0"]
@@ -146,6 +147,9 @@ SIGNATURES = {
     "vare_flip_step": (C.c_int, [_P, C.c_float]),
     "vare_flip_step_noalloc": (C.c_int, [_P, C.c_float]),
     "vare_flip_advance_frame": (C.c_int, [_P]),
+    "vare_flip_post_process": (C.c_int, [_P]),
+    "vare_flip_export_surface": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), _FP, _FP]),
+    "vare_flip_export_velocity": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), _FP]),
     "vare_flip_stage": (C.c_int, [_P, C.c_int, C.c_float]),
     "vare_flip_get_stats": (C.c_int, [_P, C.POINTER(StepStats)]),
     "vare_flip_get_block_table": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -337,6 +341,31 @@ class FlipHandle:
 
     def advance_frame(self):
         check(self.lib.vare_flip_advance_frame(self.h))
+
+    def post_process(self):
+        """vare_flip_post_process: FLIPMethod::postProcess on the voxel-space particles."""
+        check(self.lib.vare_flip_post_process(self.h))
+
+    def export_surface(self):
+        """(ijk[n,3], phi[n], background, xform[4]) of the surface VDB export."""
+        n = C.c_size_t()
+        bg = C.c_float()
+        xf = (C.c_float * 4)()
+        check(self.lib.vare_flip_export_surface(self.h, None, None, 0, C.byref(n), C.byref(bg), xf))
+        ijk = np.empty((n.value, 3), np.int32)
+        phi = np.empty(n.value, np.float32)
+        check(self.lib.vare_flip_export_surface(self.h, ptr(ijk), ptr(phi), n.value, C.byref(n), C.byref(bg), xf))
+        return ijk, phi, bg.value, np.array(list(xf), np.float32)
+
+    def export_velocity(self):
+        """(ijk[n,3], vel[n,3] world units, xform[4]) of the velocity VDB export."""
+        n = C.c_size_t()
+        xf = (C.c_float * 4)()
+        check(self.lib.vare_flip_export_velocity(self.h, None, None, 0, C.byref(n), xf))
+        ijk = np.empty((n.value, 3), np.int32)
+        vel = np.empty((n.value, 3), np.float32)
+        check(self.lib.vare_flip_export_velocity(self.h, ptr(ijk), ptr(vel), n.value, C.byref(n), xf))
+        return ijk, vel, np.array(list(xf), np.float32)
 
     def to_voxel(self):
         check(self.lib.vare_flip_particles_to_voxel(self.h))

@@ -18,12 +18,14 @@
 #include <atomic>
 #include <mutex>
 #include <unordered_map>
+#include <array>
 
 #include "../../include/vare_flip.h"
 #include "flip_kernels.h"
 #include "flip_mg.h"
 #include "flip_particles.h"
 #include "flip_stencil.h"
+#include "flip_post.h"
 
 using namespace vare;
 
@@ -76,6 +78,7 @@ struct vare_flip {
     hipStream_t stream = nullptr;
     int device = 0;               // the device current at create (its stream's device)
     int noalloc = 0;              // inside vare_flip_step_noalloc: scratch from the static array, no growth
+    int post_done = 0;            // the cell fields hold the post-process results (until the next stage 0)
 
     // particles, SoA: px,py,pz,vx,vy,vz,adj in physical order (bin order after each binning),
     // vort,curv in the reference (logical) order; pid[q] = logical index of physical particle q
@@ -1238,6 +1241,116 @@ int stage_remove(vare_flip* h) {
     return VARE_OK;
 }
 
+// FLIPMethod.cu:320-363 (water tank), on the host as in the reference: air cells of the surface field
+// below the tank level that the flood fill from the level's layer does not reach become liquid (-5).
+int tank_flood_fill(vare_flip* h) {
+    const size_t nc = h->ncells();
+    const int B = h->B, B3 = B * B * B, nx = h->nx, ny = h->ny, nz = h->nz;
+    std::vector<float> surf(nc);
+    std::vector<int> slot((size_t)h->nblk), coord((size_t)h->nb);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (nc) HIPCHK(hipMemcpy(surf.data(), h->cnt, nc * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(slot.data(), h->slot, slot.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (h->nb) HIPCHK(hipMemcpy(coord.data(), h->coord, coord.size() * sizeof(int), hipMemcpyDeviceToHost));
+    auto index = [&](int i, int j, int k) -> long long {
+        const int sl = slot[(size_t)(((k / B) * h->fy + j / B) * h->fx + i / B)];
+        return sl < 0 ? -1 : (long long)sl * B3 + ((k % B) * B + (j % B)) * B + (i % B);
+    };
+    auto F = [&](int i, int j, int k) -> float {
+        const long long c = index(i, j, k);
+        return c < 0 ? kSurfBackground : surf[(size_t)c];
+    };
+    std::vector<char> scan((size_t)nx * ny * nz, 0);
+    auto S = [&](int i, int j, int k) -> char& { return scan[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)]; };
+    const int wall = (int)(size_t)h->vox_wall;                   // const size_t wall = _voxelWallLevel
+    std::vector<std::array<int, 3>> cells;
+    cells.push_back({nx - 1, ny - 1, nz - 1});
+    if (wall >= 0 && wall < ny)
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx; ++i)
+                if (F(i, wall, k) > 0.f) cells.push_back({i, wall, k});
+    while (!cells.empty()) {
+        const std::array<int, 3> c = cells.back();
+        cells.pop_back();
+        S(c[0], c[1], c[2]) = 1;
+        if (c[1] > wall) continue;
+        static const int d[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+        for (int q = 0; q < 6; ++q) {
+            const int i = c[0] + d[q][0], j = c[1] + d[q][1], k = c[2] + d[q][2];
+            if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) continue;
+            if (F(i, j, k) > 0.f && S(i, j, k) == 0) cells.push_back({i, j, k});
+        }
+    }
+    bool changed = false;
+    for (size_t c = 0; c < nc; ++c) {
+        const int t = coord[c / B3], l = (int)(c % B3);
+        const int i = (t % h->fx) * B + l % B, j = ((t / h->fx) % h->fy) * B + (l / B) % B,
+                  k = (t / (h->fx * h->fy)) * B + l / (B * B);
+        if (j > wall) continue;
+        if (surf[c] > 0.f && S(i, j, k) == 0) { surf[c] = -5.f; changed = true; }
+    }
+    if (changed) HIPCHK(hipMemcpy(h->cnt, surf.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    return VARE_OK;
+}
+
+// FLIPMethod::postProcess (FLIPMethod.cu:300-405) on the voxel-space particles.  Afterwards the cell
+// fields hold the level set (count field), the splatted velocity (P2G velocity) and the normals
+// (projected velocity); pressure, mass and cell types are 0, as the reference's fields after its frame
+// rebuild.
+template <int B> int run_post_process(vare_flip* h) {
+    h->stage_next = 0;
+    CHK(stage_prepare<B>(h));                                   // prepareNextStep, :302
+    if (h->prm.particle_path != 0) CHK(bin_particles<B>(h));    // the splat reads the bins
+    const size_t nc = h->ncells();
+    const unsigned g = rgrid(nc);
+    const FrameK fk = h->frame();
+    if (nc)
+        hipLaunchKernelGGL(k_splat_gather<B>, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, (const float*)h->pa[0],
+                           (const float*)h->pa[1], (const float*)h->pa[2], h->vel(), (const int*)h->cstart, fk, h->cnt,
+                           h->vx, h->vy, h->vz);
+    HIPCHK(hipGetLastError());
+    const bool tank = h->prm.enable_water_tank != 0;
+    if (tank) CHK(tank_flood_fill(h));                         // :320-363
+    // Rasterization::ReinitializeSurface, the pressure field as the second buffer (:365)
+    float* cur = h->cnt;
+    float* nxt = h->x;
+    for (int it = 0; it < h->prm.redist_iteration && nc; ++it) {
+        if (tank)
+            hipLaunchKernelGGL((k_redistance<B, true>), dim3(g), dim3(WG), 0, h->stream, (int)nc, fk, (const float*)cur,
+                               nxt, h->vox_wall);
+        else
+            hipLaunchKernelGGL((k_redistance<B, false>), dim3(g), dim3(WG), 0, h->stream, (int)nc, fk,
+                               (const float*)cur, nxt, h->vox_wall);
+        std::swap(cur, nxt);
+    }
+    if (cur != h->cnt && nc)
+        HIPCHK(hipMemcpyAsync(h->cnt, cur, nc * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    if (nc) {
+        const Vec3Field velo{h->vx, h->vy, h->vz}, comp{h->cx, h->cy, h->cz};
+        hipLaunchKernelGGL(k_curl<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, fk, velo, h->cx, h->cy, h->cz);   // :367
+        const float dis = powf(1.f - h->prm.vort_pres_rate, h->prm.dt);                                       // :373
+        if (h->np)
+            hipLaunchKernelGGL(k_particle_vorticity<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np,
+                               (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2],
+                               (const int*)h->pid, fk, comp, h->s[0], h->s[1], h->s[2], dis, h->pa[7]);
+        hipLaunchKernelGGL(k_normal_gradient<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, fk, (const float*)h->cnt,
+                           kSurfBackground, h->cx, h->cy, h->cz);                                            // :390
+        hipLaunchKernelGGL(k_divergence_vec<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, fk, comp, h->x);   // :391
+        if (h->np)
+            hipLaunchKernelGGL(k_particle_curvature<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np,
+                               (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2],
+                               (const int*)h->pid, fk, (const float*)h->x, h->pa[8]);                      // :393-398
+        CHK(fill_u32(h, h->x, 0, nc));                                                                       // :402-404
+        CHK(fill_u32(h, h->mass, 0, nc));
+        hipLaunchKernelGGL(k_fill_u32, dim3(nblocks((nc + 3) / 4)), dim3(WG), 0, h->stream, (uint32_t*)h->type, 0u,
+                           (nc + 3) / 4);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->post_done = 1;
+    return VARE_OK;
+}
+
 template <int B> int run_stage(vare_flip* h, int stage, float dt) {
     switch (stage) {
         case 0: return stage_prepare<B>(h);
@@ -1253,6 +1366,7 @@ template <int B> int run_stage(vare_flip* h, int stage, float dt) {
 int dispatch_stage(vare_flip* h, int stage, float dt) {
     if (stage != h->stage_next)
         return fail(VARE_EINVAL, "stage %d called out of order (expected %d)", stage, h->stage_next);
+    if (stage == 0) h->post_done = 0;
     int rc = h->B == 4 ? run_stage<4>(h, stage, dt) : run_stage<8>(h, stage, dt);
     if (rc != VARE_OK) { h->stage_next = 0; return rc; }
     h->stage_next = (stage + 1) % 6;
@@ -1727,13 +1841,120 @@ int vare_flip_step_noalloc(vare_flip* h, float dt) {
     return rc;
 }
 
+int vare_flip_post_process(vare_flip* h) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    const int rc = h->B == 4 ? run_post_process<4>(h) : run_post_process<8>(h);
+    h->stage_next = 0;
+    return rc;
+}
+
 int vare_flip_advance_frame(vare_flip* h) {
     if (!h) return fail(VARE_EINVAL, "null handle");
     if (h->np == 0) return VARE_OK;                         // FLIPMethod.cu:179
-    CHK(convert_all(h, true));
-    int rc = vare_flip_step(h, h->prm.dt);
-    CHK(convert_all(h, false));
+    CHK(convert_all(h, true));                              // :191
+    int rc = vare_flip_step(h, h->prm.dt);                  // :193
+    if (rc == VARE_OK || rc == VARE_ENOCONV) {
+        const int rp = vare_flip_post_process(h);           // :198
+        if (rp != VARE_OK) rc = rp;
+    }
+    CHK(convert_all(h, false));                             // :200
     return rc;
+}
+
+int vare_flip_export_surface(vare_flip* h, int32_t* ijk, float* phi, size_t cap, size_t* n, float* background,
+                             float xform[4]) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (!h->post_done) return fail(VARE_EINVAL, "the surface field is built by vare_flip_post_process");
+    const int B = h->B, B3 = B * B * B;
+    const size_t nc = h->ncells();
+    const float dx = h->s[0];
+    if (background) *background = kSurfBackground * dx;               // FloatGrid::create(background*dx)
+    if (xform) { xform[0] = dx; for (int a = 0; a < 3; ++a) xform[1 + a] = h->t[a] + dx * 0.5f; }
+    std::vector<float> surf(nc);
+    std::vector<int> coord((size_t)h->nb);
+    if (nc) HIPCHK(hipMemcpy(surf.data(), h->cnt, nc * sizeof(float), hipMemcpyDeviceToHost));
+    if (h->nb) HIPCHK(hipMemcpy(coord.data(), h->coord, coord.size() * sizeof(int), hipMemcpyDeviceToHost));
+    // convertSurfaceVDB (FieldCast.h:362-391): every active cell, phi = surf * dx
+    std::vector<std::array<int, 3>> cells(nc);
+    std::vector<float> vals(nc);
+    std::unordered_map<long long, size_t> where;
+    const long long ox = 1 << 20;                                     // key offset: shell cells reach -3
+    auto key = [&](int i, int j, int k) { return (((long long)(k + 8) * ox) + (j + 8)) * ox + (i + 8); };
+    for (size_t c = 0; c < nc; ++c) {
+        const int t = coord[c / B3], l = (int)(c % B3);
+        cells[c] = {(t % h->fx) * B + l % B, ((t / h->fx) % h->fy) * B + (l / B) % B, (t / (h->fx * h->fy)) * B + l / (B * B)};
+        vals[c] = surf[c] * dx;
+    }
+    if (h->prm.enable_water_tank) {
+        // FieldCast.h:392-440: a 3-cell shell outside the domain below the tank level holds
+        // min(world cell centre y - level, value), the faces' cells included
+        for (size_t c = 0; c < nc; ++c) where[key(cells[c][0], cells[c][1], cells[c][2])] = c;
+        const int EI = h->nx - 1, EJ = h->ny - 1, EK = h->nz - 1, bnd = 3;
+        (void)EJ;
+        const float level = h->prm.wall_level;
+        const int lv = (int)((level - h->t[1]) / dx + 1.f);
+        auto put = [&](int i, int j, int k) {
+            const float cy = ((float)j + 0.5f) * h->s[1] + h->t[1];       // Grid::worldPoint
+            const long long kk = key(i, j, k);
+            auto it = where.find(kk);
+            if (it == where.end()) {
+                where[kk] = cells.size();
+                cells.push_back({i, j, k});
+                vals.push_back(std::min(cy - level, kSurfBackground * dx));
+            } else {
+                vals[it->second] = std::min(cy - level, vals[it->second]);
+            }
+        };
+        for (int k = -bnd; k <= EK + bnd; ++k) for (int j = -bnd; j <= 0; ++j) for (int i = -bnd; i <= EI + bnd; ++i) put(i, j, k);
+        for (int k = -bnd; k <= EK + bnd; ++k) for (int j = -bnd; j <= lv + bnd; ++j) for (int i = -bnd; i <= 0; ++i) put(i, j, k);
+        for (int k = -bnd; k <= EK + bnd; ++k) for (int j = -bnd; j <= lv + bnd; ++j) for (int i = EI; i <= EI + bnd; ++i) put(i, j, k);
+        for (int k = -bnd; k <= 0; ++k) for (int j = -bnd; j <= lv + bnd; ++j) for (int i = -bnd; i <= EI + bnd; ++i) put(i, j, k);
+        for (int k = EK; k <= EK + bnd; ++k) for (int j = -bnd; j <= lv + bnd; ++j) for (int i = -bnd; i <= EI + bnd; ++i) put(i, j, k);
+    }
+    if (n) *n = cells.size();
+    if (!ijk && !phi) return VARE_OK;
+    if (cap < cells.size()) return fail(VARE_EINVAL, "capacity %zu < %zu voxels", cap, cells.size());
+    for (size_t q = 0; q < cells.size(); ++q) {
+        if (ijk) for (int a = 0; a < 3; ++a) ijk[3 * q + a] = cells[q][a];
+        if (phi) phi[q] = vals[q];
+    }
+    return VARE_OK;
+}
+
+int vare_flip_export_velocity(vare_flip* h, int32_t* ijk, float* vel, size_t cap, size_t* n, float xform[4]) {
+    if (!h) return fail(VARE_EINVAL, "null handle");
+    if (!h->post_done) return fail(VARE_EINVAL, "the splatted velocity is built by vare_flip_post_process");
+    const int B = h->B, B3 = B * B * B;
+    const size_t nc = h->ncells();
+    const float dx = h->s[0];
+    if (xform) { xform[0] = dx; for (int a = 0; a < 3; ++a) xform[1 + a] = h->t[a] + dx * 0.5f; }
+    std::vector<float> v3[3];
+    std::vector<int> coord((size_t)h->nb);
+    const float* src[3] = {h->vx, h->vy, h->vz};
+    for (int a = 0; a < 3; ++a) {
+        v3[a].resize(nc);
+        if (nc) HIPCHK(hipMemcpy(v3[a].data(), src[a], nc * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    if (h->nb) HIPCHK(hipMemcpy(coord.data(), h->coord, coord.size() * sizeof(int), hipMemcpyDeviceToHost));
+    // convertVelocityVDB (FieldCast.h:14-47): active cells whose world velocity is not below 1e-30
+    size_t cnt = 0;
+    for (size_t c = 0; c < nc; ++c) {
+        const float w[3] = {v3[0][c] * h->s[0], v3[1][c] * h->s[1], v3[2][c] * h->s[2]};   // Grid::worldVector
+        if (std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) < 1e-30f) continue;
+        if (ijk || vel) {
+            if (cnt >= cap) return fail(VARE_EINVAL, "capacity %zu too small", cap);
+            const int t = coord[c / B3], l = (int)(c % B3);
+            if (ijk) {
+                ijk[3 * cnt] = (t % h->fx) * B + l % B;
+                ijk[3 * cnt + 1] = ((t / h->fx) % h->fy) * B + (l / B) % B;
+                ijk[3 * cnt + 2] = (t / (h->fx * h->fy)) * B + l / (B * B);
+            }
+            if (vel) for (int a = 0; a < 3; ++a) vel[3 * cnt + a] = w[a];
+        }
+        cnt++;
+    }
+    if (n) *n = cnt;
+    return VARE_OK;
 }
 
 int vare_flip_get_stats(const vare_flip* h, vare_step_stats* o) {

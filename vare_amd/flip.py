@@ -133,7 +133,7 @@ class FLIPMethod:
         self._download(pts)
         return self._h.stats()
 
-    # FLIPMethod::advanceOneFrame (FLIPMethod.cu:177-206) without postProcess
+    # FLIPMethod::advanceOneFrame (FLIPMethod.cu:177-206): toVoxel, step, postProcess, toWorld
     def advanceOneFrame(self, pts: Particles):
         if pts.size() == 0:
             return None                                  # FLIPMethod.cu:179
@@ -141,6 +141,12 @@ class FLIPMethod:
         self._h.advance_frame()
         self._download(pts)
         return self._h.stats()
+
+    # FLIPMethod::postProcess (FLIPMethod.cu:300-405), voxel space
+    def postProcess(self, pts: Particles):
+        self._upload(pts)
+        self._h.post_process()
+        self._download(pts)
 
     # FLIPMethod::prepareNextStep (FLIPMethod.cu:49-52)
     def prepareNextStep(self, pts: Particles):
@@ -169,7 +175,15 @@ class FLIPMethod:
         return self._h.cell_field(L.FIELD_MASS)
 
     def getSurfaceField(self):
-        return self._h.cell_field(L.FIELD_COUNT)
+        """The level set after a frame (postProcess); the count field after a bare step."""
+        return self._h.cell_field(L.FIELD_SURFACE)
+
+    # the SOP's VDB export (SOP_FlipWater.cpp:251-275 -> FieldCast.h convertSurfaceVDB/convertVelocityVDB)
+    def exportSurfaceVDB(self):
+        return self._h.export_surface()
+
+    def exportVelocityVDB(self):
+        return self._h.export_velocity()
 
     @property
     def handle(self) -> "L.FlipHandle":
