@@ -131,6 +131,7 @@ struct vare_flip {
     int* rank = nullptr;
     double* partials = nullptr;
     double* partials2 = nullptr;
+    float* p2gpart = nullptr;       // P2G: each block's incoming +1-layer partial sums (5 per cell)
 
     // multigrid preconditioner: level 0 aliases r/info/z of the CG; zt0 is its second buffer
     std::vector<MgLevelK> mg;
@@ -287,7 +288,7 @@ void free_cells(vare_flip* h) {
     dfree(h, h->x); dfree(h, h->r); dfree(h, h->p); dfree(h, h->ap); dfree(h, h->b); dfree(h, h->z); dfree(h, h->x64);
     dfree(h, h->r2); dfree(h, h->p2);
     dfree(h, h->info); dfree(h, h->type); dfree(h, h->rank);
-    dfree(h, h->partials); dfree(h, h->partials2);
+    dfree(h, h->partials); dfree(h, h->partials2); dfree(h, h->p2gpart);
     h->bcap = 0;
 }
 
@@ -323,6 +324,8 @@ int alloc_cells(vare_flip* h, size_t cap) {
     CHK(dalloc(h, &h->b, nc)); CHK(dalloc(h, &h->z, nc)); CHK(dalloc(h, &h->x64, nc));
     CHK(dalloc(h, &h->info, nc)); CHK(dalloc(h, &h->type, nc)); CHK(dalloc(h, &h->rank, nc));
     CHK(dalloc(h, &h->partials, npart)); CHK(dalloc(h, &h->partials2, npart));
+    const size_t b1 = (size_t)h->B + 1;
+    CHK(dalloc(h, &h->p2gpart, cap * (b1 * b1 * b1 - (size_t)h->B * h->B * h->B) * 5));
     h->bcap = cap;
     return VARE_OK;
 }
@@ -587,19 +590,23 @@ template <int B> int stage_p2g(vare_flip* h) {
     const size_t nc = h->ncells();
     CellFields cf{h->mass, h->vx, h->vy, h->vz, h->cnt};
     if (h->prm.particle_path == 0) {
-        // gather form: every cell of every active block is written, normalised and classified
+        // two passes over the blocks: own particles into the block and its +1 layer, then the face
+        // cells; every cell of every active block is written, normalised and classified
         tbegin(h, VARE_KERNEL_P2G);
         if (h->nb) {
+            const FrameK fk = h->frame();
             if (h->vhalf)
-                hipLaunchKernelGGL((k_p2g_gather<B, true>), dim3(h->nb), dim3(P2GGather<B>::THREADS), 0, h->stream,
+                hipLaunchKernelGGL((k_p2g_own<B, true>), dim3(h->nb), dim3(P2GOwn<B>::THREADS), 0, h->stream,
                                    (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
-                                   (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type,
-                                   h->colk(), h->prm.col_vel_scale);
+                                   (const int*)h->cstart, fk, cf, h->cx, h->cy, h->cz, h->type, h->colk(),
+                                   h->prm.col_vel_scale, h->p2gpart);
             else
-                hipLaunchKernelGGL((k_p2g_gather<B, false>), dim3(h->nb), dim3(P2GGather<B>::THREADS), 0, h->stream,
+                hipLaunchKernelGGL((k_p2g_own<B, false>), dim3(h->nb), dim3(P2GOwn<B>::THREADS), 0, h->stream,
                                    (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
-                                   (const int*)h->cstart, h->frame(), cf, h->cx, h->cy, h->cz, h->type,
-                                   h->colk(), h->prm.col_vel_scale);
+                                   (const int*)h->cstart, fk, cf, h->cx, h->cy, h->cz, h->type, h->colk(),
+                                   h->prm.col_vel_scale, h->p2gpart);
+            hipLaunchKernelGGL(k_p2g_faces<B>, dim3(h->nb), dim3(P2GOwn<B>::FACE_THREADS), 0, h->stream, fk, cf, h->cx, h->cy,
+                               h->cz, h->type, h->colk(), h->prm.col_vel_scale, (const float*)h->p2gpart);
         }
         tend(h, VARE_KERNEL_P2G);
         HIPCHK(hipGetLastError());

@@ -134,9 +134,10 @@ __global__ void k_pscatter(int np, const int* __restrict__ key, const int* __res
                            unsigned long long* __restrict__ perm) {
     constexpr int J = kBinSlices;
     int idx[J], kk[J], rk[J], pd[J], st[J];
+    const int wg = xcd_swizzle(blockIdx.x, gridDim.x);       // contiguous particle runs per XCD
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        idx[j] = (blockIdx.x * J + j) * WG + threadIdx.x;
+        idx[j] = (wg * J + j) * WG + threadIdx.x;
         const int i = idx[j] < np ? idx[j] : 0;
         kk[j] = key[i]; rk[j] = rank[i]; pd[j] = pid[i];
     }
@@ -348,79 +349,146 @@ __device__ __forceinline__ int tile_cell(const int* nsl, int ti, int tj, int tk)
     return s * Blk<B>::B3 + (lk * B + lj) * B + li;
 }
 
-// Workgroup shape of the gather P2G: one thread per base cell of the (B+1)^3 region whose stencils
-// reach the block (local offsets -1..B-1 per axis).
-template <int B> struct P2GGather {
-    static constexpr int B1 = B + 1;
-    static constexpr int NBASE = B1 * B1 * B1;
-    static constexpr int THREADS = (NBASE + 63) / 64 * 64;
+// ---- P2G in two passes: each block adds only its own particles ----
+//
+// Pass 1 (k_p2g_own, one workgroup per active block, one thread per own base cell): a thread sums the
+// 8 corner terms of its base cell's particles in registers, and the block's cells plus the layer of
+// cells one past its +x/+y/+z faces (the "+1 layer") receive them in 8 LDS passes, one per corner
+// offset.  A wave owns 64 consecutive base cells, whose bins are one contiguous particle range: the
+// wave stages that range through LDS in rounds of C particles with coalesced loads, and each lane
+// then reads its bin's particles from LDS.  Cells whose 8 contributing base cells are all in the
+// block (local x, y, z >= 1) are finished here; the block's face cells (a local coordinate 0) keep
+// their partial sums in the output fields.  A +1-layer cell belongs to one of the 7 blocks above
+// (+x, +y, +z and the edge and corner blocks): its partial goes to that block's "incoming" area, at
+// the same +1-layer index (each entry has exactly one writer), and a block zeroes the entries of its
+// own area whose writer is inactive or outside the frame.  A corner in an inactive block is dropped
+// (isWritable).
+// Pass 2 (k_p2g_faces, one thread per face cell) adds a face cell's incoming partials in the fixed
+// order -x, -y, -x-y, -z, -x-z, -y-z, -x-y-z, then normalises and classifies.  Every sum is in a fixed
+// order, so P2G is bit-reproducible; no particle is read twice.
+template <int B> struct P2GOwn {
+    static constexpr int B3 = B * B * B;
+    static constexpr int THREADS = B3;               // one thread per own base cell
+    static constexpr int NW = B3 / 64;               // waves per workgroup
+    static constexpr int T = B + 1, T3 = T * T * T;  // the block's cells and its +1 layer
+    static constexpr int NPART = T3 - B3;            // +1-layer cells: T^2 (x = B) + B*T (y = B) + B^2 (z = B)
+    static constexpr int C = 256;                    // particles staged per wave and round
+    static constexpr int NFACE = B3 - (B - 1) * (B - 1) * (B - 1);   // cells with a local coordinate 0
+    static constexpr int FACE_THREADS = (NFACE + 63) / 64 * 64;
+    static_assert(B3 % 64 == 0, "a wave owns 64 base cells");
 };
 
-// P2G + count field + normalisation + classification of one active block, as a gather
-// (Rasterization.cu:121-190 and :198-230, FLIPMethod.cu:242-280 with curl disabled).
-// Each thread sums the 8 corner contributions of its base cell's particles (bin order, i.e.
-// ascending reference index) in registers.  The block's cells then receive the corners in 8 passes
-// (corner offset 000, 100, 010, ... in turn): within one pass distinct base cells hit distinct
-// cells, so the LDS updates need no atomics and every cell sums its terms in a fixed order -- the
-// result is bit-reproducible.  A corner past n-1 is skipped (the Min(c+1, n-1) bound); corners in
-// other blocks are gathered by those blocks' workgroups, and corners in inactive blocks are dropped
-// (isWritable).  Every cell of the block is written, so no zero fill is needed.  At 6 waves per SIMD
-// (at most 80 registers) two 768-thread workgroups fit on a CU.
+// Index of a +1-layer cell (x, y, z) in [0,B]^3 with a coordinate equal to B, and back.
+template <int B> __device__ __forceinline__ int p2g_layer_index(int x, int y, int z) {
+    constexpr int T = B + 1;
+    if (x == B) return y + T * z;
+    if (y == B) return T * T + x + B * z;
+    return T * T + B * T + x + B * y;
+}
+template <int B> __device__ __forceinline__ void p2g_layer_cell(int q, int& x, int& y, int& z) {
+    constexpr int T = B + 1;
+    if (q < T * T) { x = B; y = q % T; z = q / T; return; }
+    q -= T * T;
+    if (q < B * T) { y = B; x = q % B; z = q / B; return; }
+    q -= B * T;
+    z = B; x = q % B; y = q / B;
+}
+
+// kernel 2 of Rasterization::ParticlesToVelField and the classification (Rasterization.cu:171-190,
+// FLIPMethod.cu:242-280): normalise a cell's sums and write every P2G output of cell c.
+template <int B>
+__device__ __forceinline__ void p2g_finish(int c, int i, int j, int k, float m, float sx, float sy, float sz, float cn,
+                                           const CellFields& cf, float* __restrict__ cx, float* __restrict__ cy,
+                                           float* __restrict__ cz, uint8_t* __restrict__ type, const ColK& col,
+                                           const FrameK& f, float col_scale) {
+    float vx = 0.f, vy = 0.f, vz = 0.f;
+    uint8_t ty8 = kAir;
+    if (m > 0.f) {
+        vx = sx / m;
+        vy = sy / m;
+        vz = sz / m;
+        ty8 = kFluid;
+    }
+    if (col.any()) ty8 = classify_cell<B>(col, f, col_scale, i, j, k, m, vx, vy, vz);
+    cf.mass[c] = m > 0.f ? m : 0.f;
+    cf.vx[c] = vx; cf.vy[c] = vy; cf.vz[c] = vz;
+    cf.cnt[c] = cn;
+    cx[c] = vx; cy[c] = vy; cz[c] = vz;
+    type[c] = ty8;
+}
+
 template <int B, bool VH>
-__global__ void __launch_bounds__(P2GGather<B>::THREADS) __attribute__((amdgpu_waves_per_eu(6))) k_p2g_gather(
+__global__ void __launch_bounds__(P2GOwn<B>::THREADS) k_p2g_own(
         const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, VelPtr uv,
         const int* __restrict__ cstart, FrameK f, CellFields cf,
         float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz, uint8_t* __restrict__ type, ColK col,
-        float col_scale) {
-    constexpr int B1 = B + 1, NBASE = B1 * B1 * B1, B3 = B * B * B;
+        float col_scale, float* __restrict__ part) {
+    using P = P2GOwn<B>;
+    constexpr int B3 = P::B3, T = P::T, T3 = P::T3, C = P::C, R = C / 64;
     const VelT<VH> u = vel_typed<VH>(uv);
-    __shared__ float tm[B3], tx[B3], ty[B3], tz[B3], tc[B3];
+    __shared__ float st[P::NW][6][C];
+    __shared__ float tm[T3], tx[T3], ty[T3], tz[T3], tc[T3];
     __shared__ int nsl[27];
     const int s = xcd_swizzle(blockIdx.x, gridDim.x);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     load_nbhd(f, s, nsl);
-    for (int q = t; q < B3; q += blockDim.x) { tm[q] = 0.f; tx[q] = 0.f; ty[q] = 0.f; tz[q] = 0.f; tc[q] = 0.f; }
-    __syncthreads();
+    for (int q = t; q < T3; q += P::THREADS) { tm[q] = 0.f; tx[q] = 0.f; ty[q] = 0.f; tz[q] = 0.f; tc[q] = 0.f; }
     const int t0 = f.coord[s];
     const int ox = (t0 % f.fx) * B, oy = ((t0 / f.fx) % f.fy) * B, oz = (t0 / (f.fx * f.fy)) * B;
-    const int li = t % B1 - 1, lj = (t / B1) % B1 - 1, lk = t / (B1 * B1) - 1;     // base cell - origin
+    const int li = t % B, lj = (t / B) % B, lk = t / (B * B);          // base cell - origin
+    const int q0 = cstart[s * B3 + t], q1 = cstart[s * B3 + t + 1];
+    const int w0 = __shfl(q0, 0, 64), w1 = __shfl(q1, 63, 64);        // the wave's particle range
+    const int ci = ox + li, cj = oy + lj, ck = oz + lk;
     float am[8], ax[8], ay[8], az[8];
-    float ac = 0.f;                                  // count term: 1 per particle at every corner
 #pragma unroll
     for (int c = 0; c < 8; ++c) { am[c] = 0.f; ax[c] = 0.f; ay[c] = 0.f; az[c] = 0.f; }
-    if (t < NBASE) {
-        const int dx = li < 0 ? -1 : 0, dy = lj < 0 ? -1 : 0, dz = lk < 0 ? -1 : 0;
-        const int sl = nsl[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-        if (sl >= 0) {
-            const int cell = sl * B3 + ((lk - dz * B) * B + (lj - dy * B)) * B + (li - dx * B);
-            const int q0 = cstart[cell], q1 = cstart[cell + 1];
-            const int ci = ox + li, cj = oy + lj, ck = oz + lk;
-            ac = (float)(q1 - q0);                   // the sum of q1-q0 ones, exact below 2^24
-            for (unsigned n = q0; n < (unsigned)q1; ++n) {     // the bin's particles are contiguous
-                const float x = ld_u32(px, n), y = ld_u32(py, n), z = ld_u32(pz, n);
-                const float vx = u.ldu(0, n), vy = u.ldu(1, n), vz = u.ldu(2, n);
+    for (int c0 = w0; c0 < w1; c0 += C) {
+        float r[R][6];
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int i = ci + (c & 1), j = cj + ((c >> 1) & 1), k = ck + (c >> 2);
-                    const float ddx = x - ((float)i + 0.5f), ddy = y - ((float)j + 0.5f), ddz = z - ((float)k + 0.5f);
-                    const float xw = tent(ddx);
-                    const float yw = tent(ddy);
-                    const float zw = tent(ddz);
-                    const float w = xw * yw * zw + 1e-05f;
-                    am[c] += w;
-                    ax[c] += w * vx;
-                    ay[c] += w * vy;
-                    az[c] += w * vz;
-                }
+        for (int k = 0; k < R; ++k) {
+            const int n = c0 + k * 64 + lane;
+            const unsigned m = (unsigned)(n < w1 ? n : w0);           // lanes past the range re-read the first
+            r[k][0] = ld_u32(px, m); r[k][1] = ld_u32(py, m); r[k][2] = ld_u32(pz, m);
+            r[k][3] = u.ldu(0, m); r[k][4] = u.ldu(1, m); r[k][5] = u.ldu(2, m);
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+#pragma unroll
+            for (int a = 0; a < 6; ++a) st[w][a][k * 64 + lane] = r[k][a];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int lo = q0 > c0 ? q0 : c0, hi = q1 < c0 + C ? q1 : c0 + C;
+        for (int q = lo; q < hi; ++q) {
+            const int e = q - c0;
+            const float x = st[w][0][e], y = st[w][1][e], z = st[w][2][e];
+            const float vx = st[w][3][e], vy = st[w][4][e], vz = st[w][5][e];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int i = ci + (c & 1), j = cj + ((c >> 1) & 1), k = ck + (c >> 2);
+                const float xw = tent(x - ((float)i + 0.5f));
+                const float yw = tent(y - ((float)j + 0.5f));
+                const float zw = tent(z - ((float)k + 0.5f));
+                // w = xw*yw*zw + 1e-5 and the momentum terms with fused multiply-adds
+                const float wt = __builtin_fmaf(xw * yw, zw, 1e-05f);
+                am[c] += wt;
+                ax[c] = __builtin_fmaf(wt, vx, ax[c]);
+                ay[c] = __builtin_fmaf(wt, vy, ay[c]);
+                az[c] = __builtin_fmaf(wt, vz, az[c]);
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    const float ac = (float)(q1 - q0);              // count term: 1 per particle at every corner
+    __syncthreads();
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int ai = li + (c & 1), aj = lj + ((c >> 1) & 1), ak = lk + (c >> 2);
-        if (t < NBASE && ai >= 0 && aj >= 0 && ak >= 0 && ai < B && aj < B && ak < B && ox + ai <= f.nx - 1 &&
-            oy + aj <= f.ny - 1 && oz + ak <= f.nz - 1) {
-            const int q = (ak * B + aj) * B + ai;
+        // a corner past n-1 is skipped (the Min(c+1, n-1) bound)
+        if (ox + ai <= f.nx - 1 && oy + aj <= f.ny - 1 && oz + ak <= f.nz - 1) {
+            const int q = (ak * T + aj) * T + ai;
             tm[q] += am[c];
             tx[q] += ax[c];
             ty[q] += ay[c];
@@ -429,25 +497,65 @@ __global__ void __launch_bounds__(P2GGather<B>::THREADS) __attribute__((amdgpu_w
         }
         __syncthreads();
     }
-    for (int q = t; q < B3; q += blockDim.x) {
-        const int c = s * B3 + q;
-        const float m = tm[q];
-        float vx = 0.f, vy = 0.f, vz = 0.f;
-        uint8_t ty8 = kAir;
-        if (m > 0.f) {
-            vx = tx[q] / m;
-            vy = ty[q] / m;
-            vz = tz[q] / m;
-            ty8 = kFluid;
+    constexpr int NP = P::NPART;
+    for (int q = t; q < NP; q += P::THREADS) {       // own incoming entries whose writer does not exist
+        int x, y, z;
+        p2g_layer_cell<B>(q, x, y, z);
+        const int ex = x == B, ey = y == B, ez = z == B;
+        if (nsl[(1 - ez) * 9 + (1 - ey) * 3 + (1 - ex)] < 0) {
+            float* o = part + (size_t)s * NP * 5 + q;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) o[v * NP] = 0.f;
         }
-        if (col.any())
-            ty8 = classify_cell<B>(col, f, col_scale, ox + q % B, oy + (q / B) % B, oz + q / (B * B), m, vx, vy, vz);
-        cf.mass[c] = m > 0.f ? m : 0.f;
-        cf.vx[c] = vx; cf.vy[c] = vy; cf.vz[c] = vz;
-        cf.cnt[c] = tc[q];
-        cx[c] = vx; cy[c] = vy; cz[c] = vz;
-        type[c] = ty8;
     }
+    for (int q = t; q < T3; q += P::THREADS) {
+        const int x = q % T, y = (q / T) % T, z = q / (T * T);
+        if (x == B || y == B || z == B) {
+            const int ex = x == B, ey = y == B, ez = z == B;
+            const int ts = nsl[(1 + ez) * 9 + (1 + ey) * 3 + (1 + ex)];     // the block the cell lies in
+            if (ts >= 0) {
+                float* o = part + (size_t)ts * NP * 5 + p2g_layer_index<B>(x, y, z);
+                o[0] = tm[q]; o[NP] = tx[q]; o[2 * NP] = ty[q]; o[3 * NP] = tz[q]; o[4 * NP] = tc[q];
+            }
+            continue;
+        }
+        const int c = s * B3 + (z * B + y) * B + x;
+        if (x > 0 && y > 0 && z > 0) {
+            p2g_finish<B>(c, ox + x, oy + y, oz + z, tm[q], tx[q], ty[q], tz[q], tc[q], cf, cx, cy, cz, type, col, f,
+                          col_scale);
+        } else {                                     // face cell: partial sums, finished by k_p2g_faces
+            cf.mass[c] = tm[q]; cf.vx[c] = tx[q]; cf.vy[c] = ty[q]; cf.vz[c] = tz[q]; cf.cnt[c] = tc[q];
+        }
+    }
+}
+
+template <int B>
+__global__ void __launch_bounds__(P2GOwn<B>::FACE_THREADS) k_p2g_faces(FrameK f, CellFields cf, float* __restrict__ cx,
+                                                                       float* __restrict__ cy, float* __restrict__ cz,
+                                                                       uint8_t* __restrict__ type, ColK col,
+                                                                       float col_scale, const float* __restrict__ part) {
+    using P = P2GOwn<B>;
+    constexpr int NP = P::NPART;
+    const int s = blockIdx.x;
+    int fc = threadIdx.x;
+    if (fc >= P::NFACE) return;
+    int x, y, z;                                     // face cells: z = 0, then y = 0, then x = 0
+    if (fc < B * B) { x = fc % B; y = fc / B; z = 0; }
+    else if ((fc -= B * B) < B * (B - 1)) { x = fc % B; y = 0; z = 1 + fc / B; }
+    else { fc -= B * (B - 1); x = 0; y = 1 + fc % (B - 1); z = 1 + fc / (B - 1); }
+    const int c = s * P::B3 + (z * B + y) * B + x;
+    const int t0 = f.coord[s];
+    float m = cf.mass[c], sx = cf.vx[c], sy = cf.vy[c], sz = cf.vz[c], cn = cf.cnt[c];
+    const float* in = part + (size_t)s * NP * 5;
+#pragma unroll
+    for (int e = 1; e < 8; ++e) {
+        const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+        if ((ex && x) || (ey && y) || (ez && z)) continue;
+        const int q = p2g_layer_index<B>(x + ex * B, y + ey * B, z + ez * B);
+        m += in[q]; sx += in[NP + q]; sy += in[2 * NP + q]; sz += in[3 * NP + q]; cn += in[4 * NP + q];
+    }
+    const int ox = (t0 % f.fx) * B, oy = ((t0 / f.fx) % f.fy) * B, oz = (t0 / (f.fx * f.fy)) * B;
+    p2g_finish<B>(c, ox + x, oy + y, oz + z, m, sx, sy, sz, cn, cf, cx, cy, cz, type, col, f, col_scale);
 }
 
 // G2P for one particle given a cell-value fetcher (FLIPMethod.cu:86-171, SparseField.h:150-196).
