@@ -90,6 +90,9 @@ def parse():
     ap.add_argument("--plain", choices=["on", "off"], default=None,
                     help="also time the plain FP32 GPU step (FP32 PCG to --tol, no refinement) on the same "
                          "steps; default on for configurations 1 and 2")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a buffer of MIB mebibytes once on the GPU (torch) and exit: the reference "
+                         "kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--cpu-threads", type=int, default=None,
                     help="threads of the multi-core CPU baseline (default: OMP_NUM_THREADS, else the host's "
                          "cores up to 16)")
@@ -121,6 +124,54 @@ def plain_kw(a, L, n):
     kw = solver_kw(a, L, n)
     kw.update(cg_rel_tol=a.tol, refine_tol=0.0)
     return kw
+
+
+def plain_comparison(a, L, cfg, steps, kw, state, h, value):
+    """The plain GPU step (FP32 PCG, no refinement) on the same timed steps, at two tolerances: the
+    target on its recursive residual, and the loosest recursive tolerance whose pressure meets the
+    target in FP64 on the final state of the timed run (equal post-projection divergence), found by
+    halving from the target.  Also both solvers' FP64 residual and divergence ratio D on that state."""
+    res_n = scene(cfg)["res"]
+    d = h.get_particles()
+    final = (d["pos"], d["vel"])
+    tol = a.tol
+    calib = []
+    equal_kw = None
+    for _ in range(8):
+        pkw = dict(plain_kw(a, L, res_n), cg_rel_tol=tol)
+        r = same_state_divergence(a, L, cfg, final, {"plain": pkw})["plain"]
+        calib.append({"cg_rel_tol": tol, "fp64_rel_residual": r["fp64_rel_residual"], "cg_iterations": r["cg_iterations"]})
+        if r["fp64_rel_residual"] <= a.tol:
+            equal_kw = pkw
+            break
+        tol *= 0.5
+    out = {"plain_calibration": calib}
+    lines = {"plain_gpu": plain_kw(a, L, res_n)}
+    if equal_kw is not None:
+        lines["plain_gpu_equal_divergence"] = equal_kw
+    for name, pkw in lines.items():
+        hp, _, pres, _ = run_scene(a, L, cfg, steps, kw=pkw, timing=False, state=state)
+        hp.close()
+        out[name] = {"value": pres["value"], "ms_per_step": pres["ms_per_step"], "p50_ms": pres["p50_ms"],
+                     "p99_ms": pres["p99_ms"], "cg_iterations_mean": pres["cg_iterations_mean"],
+                     "fp64_rel_residual_max": pres["fp64_rel_residual_max"],
+                     "fp64_rel_residual_mean": pres["fp64_rel_residual_mean"],
+                     "solver": "FP32 %s-PCG to %.2g relative (recursive residual), no refinement"
+                               % (a.precond, pkw["cg_rel_tol"])}
+        print("bench: %s %.3g particle-steps/s, FP64 residual max %.2e" % (
+            name, pres["value"], pres["fp64_rel_residual_max"]), file=sys.stderr, flush=True)
+    if equal_kw is None:
+        out["plain_gpu_equal_divergence"] = None
+        out["note"] = ("no recursive tolerance down to %.2g brought the plain FP32 step's FP64 residual to %.0e"
+                       % (calib[-1]["cg_rel_tol"], a.tol))
+    out["vs_plain_gpu"] = round(value / max(out["plain_gpu"]["value"], 1e-9), 3)
+    if equal_kw is not None:
+        out["vs_plain_gpu_equal_divergence"] = round(value / max(out["plain_gpu_equal_divergence"]["value"], 1e-9), 3)
+    kws = {"mixed": kw, "plain": lines["plain_gpu"]}
+    if equal_kw is not None:
+        kws["plain_equal_divergence"] = equal_kw
+    out["divergence_same_state"] = same_state_divergence(a, L, cfg, final, kws)
+    return out
 
 
 def same_state_divergence(a, L, cfg, state, kws):
@@ -479,8 +530,22 @@ def run_served(a, L):
     }
 
 
+def copy_calibration(mib):
+    import torch
+    n = (mib << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
+
+
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
               "sharding design is described in DESIGN.md)" % a.gpus, file=sys.stderr)
@@ -497,21 +562,7 @@ def main():
         h, kw, res, state = run_scene(a, L, a.config, steps)
         plain_on = (a.plain or ("on" if a.config in (1, 2) else "off")) == "on"
         if plain_on and a.solver == "relative" and kw["refine_tol"] > 0.0:
-            # the plain GPU step on the same timed steps: FP32 PCG to the target on its recursive
-            # residual, no FP64 refinement
-            pkw = plain_kw(a, L, scene(a.config)["res"])
-            hp, _, pres, _ = run_scene(a, L, a.config, steps, kw=pkw, timing=False, state=state)
-            hp.close()
-            res["plain_gpu"] = {"value": pres["value"], "ms_per_step": pres["ms_per_step"], "p50_ms": pres["p50_ms"],
-                                "p99_ms": pres["p99_ms"], "cg_iterations_mean": pres["cg_iterations_mean"],
-                                "fp64_rel_residual_max": pres["fp64_rel_residual_max"],
-                                "fp64_rel_residual_mean": pres["fp64_rel_residual_mean"],
-                                "solver": solver_desc(a, a.config, pkw)}
-            res["vs_plain_gpu"] = round(res["value"] / max(pres["value"], 1e-9), 3)
-            # both solvers on one and the same system: the final state of the timed run
-            d = h.get_particles()
-            res["divergence_same_state"] = same_state_divergence(a, L, a.config, (d["pos"], d["vel"]),
-                                                                 {"mixed": kw, "plain": pkw})
+            res.update(plain_comparison(a, L, a.config, steps, kw, state, h, res["value"]))
         cpu = cpu_baseline(a, L, a.config, h, kw) if a.cpu_baseline == "on" else None
     out = {
         "metric": METRIC,
