@@ -330,7 +330,9 @@ int vare_flip_set_kernel_timing(vare_flip* h, int mask);
 /* Totals per class since the last reset: milliseconds, launches and work items (cells or particles
  * the timed launches processed; filled for VARE_KERNEL_SPMV, VARE_KERNEL_MG0 and, with multigrid and two
  * sweeps, VARE_KERNEL_UPDATE, whose events are
- * recorded on the kernel's own dispatch and so time its execution alone).  Any array may be NULL. */
+ * recorded on the kernel's own dispatch and so time its execution alone).  For VARE_KERNEL_P2G
+ * (particle_path 0: both P2G passes between two events) the units are the algorithmic bytes: 24 per
+ * particle (18 with FP16 velocities) + 33 per active cell.  Any array may be NULL. */
 int vare_flip_get_kernel_timing(vare_flip* h, double ms_total[VARE_KERNEL_COUNT], int64_t launches[VARE_KERNEL_COUNT],
                                 int64_t units[VARE_KERNEL_COUNT], int reset);
 

@@ -447,9 +447,10 @@ void tpair(vare_flip* h, int kid, int64_t units, hipEvent_t& a, hipEvent_t& b) {
     h->ev_kid[h->ev_used] = -2;
     b = h->ev_pool[h->ev_used++];
 }
-void tend(vare_flip* h, int kid) {
+void tend(vare_flip* h, int kid, int64_t units = 0) {
     if (!(h->timing & (1u << kid))) return;
     if (h->ev_used == 0 || h->ev_kid[h->ev_used - 1] != kid) return;
+    h->ev_units[h->ev_used - 1] = units;
     h->ev_kid[h->ev_used] = -2;
     (void)hipEventRecord(h->ev_pool[h->ev_used], h->stream);
     h->ev_used++;
@@ -608,7 +609,9 @@ template <int B> int stage_p2g(vare_flip* h) {
             hipLaunchKernelGGL(k_p2g_faces<B>, dim3(h->nb), dim3(P2GOwn<B>::FACE_THREADS), 0, h->stream, fk, cf, h->cx, h->cy,
                                h->cz, h->type, h->colk(), h->prm.col_vel_scale, (const float*)h->p2gpart);
         }
-        tend(h, VARE_KERNEL_P2G);
+        // algorithmic bytes of both passes: positions and velocities read once (24 B per particle, 18 with
+        // FP16 velocities), mass, velocity, count, projected velocity and type written per cell (33 B)
+        tend(h, VARE_KERNEL_P2G, (int64_t)h->np * (h->vhalf ? 18 : 24) + (int64_t)nc * 33);
         HIPCHK(hipGetLastError());
         return VARE_OK;
     }
