@@ -193,7 +193,9 @@ int vare_flip_particles_to_voxel(vare_flip* h);
 int vare_flip_particles_to_world(vare_flip* h);
 
 /* FLIPMethod::advanceOneStep(pts, dt) (FLIPMethod.cu:208-298), voxel space, including
- * prepareNextStep (:49-52). */
+ * prepareNextStep (:49-52).  Inside the step the adjacency is recomputed for every particle by the
+ * G2P stage and not carried through the binning; after a step that returned an error other than
+ * VARE_ENOCONV the particles' adjacency is undefined until the next step. */
 int vare_flip_step(vare_flip* h, float dt);
 
 /* vare_flip_step without any device allocation, with bit-identical results.  No counterpart in the

@@ -132,6 +132,7 @@ struct vare_flip {
     double* partials = nullptr;
     double* partials2 = nullptr;
     float* p2gpart = nullptr;       // P2G: each block's incoming +1-layer partial sums (5 per cell)
+    int in_step = 0;                // inside vare_flip_step: the binning leaves the adjacency in place
 
     // multigrid preconditioner: level 0 aliases r/info/z of the CG; zt0 is its second buffer
     std::vector<MgLevelK> mg;
@@ -541,12 +542,15 @@ template <int B> int bin_particles(vare_flip* h) {
         hipLaunchKernelGGL(k_sort_long, dim3(kSortLongGrid), dim3(1024), 0, h->stream, (const int*)h->cstart,
                            (const int*)h->longbins, (const int*)(h->dint + 5), h->perm, h->dint + 4);
         // move the particles into bin order (double-buffered) and keep their logical indices
+        // inside vare_flip_step the adjacency is rewritten by G2P for every particle before anything
+        // reads it, so it is not moved there
         ReorderArrays ra;
         for (int a = 0; a < 7; ++a) { ra.src[a] = h->pa[a]; ra.dst[a] = h->pb[a]; }
         ra.vsz = (int)h->vsz();
+        ra.n = h->in_step ? 6 : 7;
         hipLaunchKernelGGL(k_reorder, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np,
                            (const unsigned long long*)h->perm, ra, h->pid2);
-        for (int a = 0; a < 7; ++a) std::swap(h->pa[a], h->pb[a]);
+        for (int a = 0; a < ra.n; ++a) std::swap(h->pa[a], h->pb[a]);
         std::swap(h->pid, h->pid2);
     }
     HIPCHK(hipGetLastError());
@@ -1801,7 +1805,12 @@ int vare_flip_step(vare_flip* h, float dt) {
     h->stage_next = 0;
     h->stats.num_removed = 0;
     HIPCHK(hipEventRecord(h->step_ev[0], h->stream));
-    for (int s = 0; s < 6; ++s) CHK(dispatch_stage(h, s, dt));
+    h->in_step = 1;
+    for (int s = 0; s < 6; ++s) {
+        const int rc = dispatch_stage(h, s, dt);
+        if (rc != VARE_OK) { h->in_step = 0; return rc; }
+    }
+    h->in_step = 0;
     HIPCHK(hipEventRecord(h->step_ev[1], h->stream));
     HIPCHK(hipEventSynchronize(h->step_ev[1]));
     float ms = 0.f, sms = 0.f;

@@ -308,6 +308,7 @@ struct ReorderArrays {
     const void* src[7];     // px, py, pz, vx, vy, vz, adj
     void* dst[7];
     int vsz;
+    int n;                  // arrays moved: 7, or 6 when adj is rewritten before it is read again
 };
 __global__ void k_reorder(int np, const unsigned long long* __restrict__ perm, ReorderArrays ra,
                           int* __restrict__ pid_out) {
@@ -318,6 +319,7 @@ __global__ void k_reorder(int np, const unsigned long long* __restrict__ perm, R
     pid_out[q] = (int)(k >> 32);
 #pragma unroll
     for (int a = 0; a < 7; ++a) {
+        if (a == 6 && ra.n < 7) break;
         if (a >= 3 && a < 6 && ra.vsz == 2) ((_Float16*)ra.dst[a])[q] = ((const _Float16*)ra.src[a])[o];
         else ((float*)ra.dst[a])[q] = ((const float*)ra.src[a])[o];
     }
