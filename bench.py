@@ -79,6 +79,9 @@ def parse():
     ap.add_argument("--no-refine", action="store_true",
                     help="plain FP32 PCG to --tol (recursive residual; its FP64 residual is reported)")
     ap.add_argument("--mg-sweeps", type=int, default=2)
+    ap.add_argument("--mg-omega-pair", type=float, nargs=2, default=None, metavar=("A", "B"),
+                    help="weights of the first and second sweep of each smoothing pair (default: the "
+                         "library's Chebyshev pair; 0 0: weighted Jacobi, 0.8 for both)")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-kernel-timing", action="store_true")
@@ -107,7 +110,8 @@ def solver_kw(a, L, n):
                 cg_rel_tol=a.inner_tol if refine else a.tol, refine_tol=a.tol if refine else 0.0,
                 refine_max=16, max_iterations=20000,
                 preconditioner={"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "mg": L.PRECOND_MG}[a.precond],
-                mg_sweeps=a.mg_sweeps)
+                mg_sweeps=a.mg_sweeps, **({} if a.mg_omega_pair is None else
+                                          dict(mg_omega_a=a.mg_omega_pair[0], mg_omega_b=a.mg_omega_pair[1])))
 
 
 def solver_desc(a, cfg=2, kw=None):
@@ -117,6 +121,16 @@ def solver_desc(a, cfg=2, kw=None):
         return "FP32 %s-PCG to %.0e relative (recursive residual)" % (a.precond, a.tol)
     return ("FP32 %s-PCG (%.0e per solve) + FP64 iterative refinement to %.0e relative FP64 residual"
             % (a.precond, a.inner_tol, a.tol))
+
+
+def smoothing_desc(a):
+    if a.precond != "mg" or a.mg_sweeps != 2:
+        return ""
+    if a.mg_omega_pair is None:
+        return ", Chebyshev-weighted Jacobi smoothing pairs"
+    if a.mg_omega_pair[0] > 0.0 and a.mg_omega_pair[1] > 0.0:
+        return ", smoothing pairs weighted %g and %g" % tuple(a.mg_omega_pair)
+    return ", Jacobi smoothing (0.8)"
 
 
 def plain_kw(a, L, n):
@@ -379,7 +393,7 @@ def run_scene(a, L, cfg, steps, kw=None, timing=True, state=None):
         "p50_ms": round(float(np.percentile(sm, 50)), 3),
         "p99_ms": round(float(np.percentile(sm, 99)), 3),
         "config": {"workload": "%s, %d ppc, dt=1/%d, FLIP 0.95, %s%s, %d^3 blocks, steps %d..%d of the %d-step run"
-                               % (sc["desc"], a.ppc, round(1.0 / sc["dt"]), solver_desc(a, cfg, kw),
+                               % (sc["desc"], a.ppc, round(1.0 / sc["dt"]), solver_desc(a, cfg, kw) + smoothing_desc(a),
                                   ", FP16 particle velocities" if sc["fp16"] else "", B, start, start + steps - 1,
                                   max(sc["steps"], start + steps)),
                    "particles": npart, "particles_last": int(nps[-1]), "cells": nx * ny * nz,

@@ -80,7 +80,8 @@ typedef struct vare_flip_params {
     int max_iterations;      /* relative mode cap, default 2000 */
     int preconditioner;      /* relative mode: VARE_PRECOND_* */
     int mg_sweeps;           /* multigrid: Jacobi sweeps before and after the coarse correction (1|2) */
-    float mg_omega;          /* multigrid: Jacobi weight, default 0.8 */
+    float mg_omega;          /* multigrid: Jacobi weight of the coarsest grid's sweeps (and of the
+                                smoothing sweeps when mg_omega_a/b are 0, or with one sweep), default 0.8 */
     int mg_coarse_sweeps;    /* multigrid: sweeps on the coarsest grid, default 20 */
     int particle_path;       /* 0: block-binned P2G/G2P through LDS tiles (default); 1: one thread per
                                 particle with global atomics / gathers */
@@ -90,6 +91,13 @@ typedef struct vare_flip_params {
     int velocity_fp16;       /* 1: particle velocities stored in FP16 (positions stay FP32); fixed at
                                 vare_flip_create */
     int flags;               /* VARE_FLAG_* bits, default 0 */
+    float mg_omega_a;        /* multigrid, two sweeps: weights of the first and second sweep of each */
+    float mg_omega_b;        /* smoothing pair, each in [0, 2); 0 for either: mg_omega for both.
+                                Default 1.576514 and 0.566352, the roots of the degree-2 Chebyshev
+                                polynomial on [0.4, 2] (Jacobi-scaled 7-point operator: spectrum in
+                                [0, 2], its high-frequency band from 1/3): they damp [1/3, 2] to at most
+                                0.39 against 0.54 for two sweeps of 0.8.  The two sweeps commute, so the
+                                V-cycle stays symmetric.  The coarsest grid keeps mg_omega. */
 } vare_flip_params;
 
 /* vare_flip_params.flags */

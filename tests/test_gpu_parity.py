@@ -476,6 +476,31 @@ def test_multigrid_deep_hierarchy_matches_jacobi():
     assert rel(pm, pj) < 1e-5
 
 
+def test_multigrid_smoothing_pairs():
+    # the two sweeps of each smoothing pair: the default Chebyshev pair (the roots of the degree-2
+    # Chebyshev polynomial on [0.4, 2]) and weighted Jacobi (0.8 twice, mg_omega_a = mg_omega_b = 0)
+    # both precondition the same step to the same pressure, the Chebyshev pair in fewer iterations
+    n = 96
+    pos, vel = make_scene(n, wx=0.5, wy=0.5, seed=12, vel_scale=3.0)
+    g = L.grid_desc(n, n, n, (0, 0, 0), (1, 1, 1), 8)
+    base = dict(ext_force=(0.0, -9.8, 0.0), wall_level=1e30, voxel_size=1.0 / n, solver_mode=1,
+                cg_rel_tol=1e-7, max_iterations=4000, preconditioner=L.PRECOND_MG)
+    assert L.default_params().mg_omega_a == pytest.approx(1.576514) and L.default_params().mg_omega_b == pytest.approx(0.566352)
+    out = {}
+    for name, kw in (("chebyshev", {}), ("jacobi", dict(mg_omega_a=0.0, mg_omega_b=0.0))):
+        h = L.FlipHandle(g, L.default_params(**base, **kw))
+        h.set_particles(pos, vel)
+        h.step(DT)
+        st = h.stats()
+        out[name] = (h.cell_field(L.FIELD_PRESSURE), st)
+        print(name, "iterations", st["cg_iterations"], flush=True)
+        assert st["converged"] == 1
+    assert out["chebyshev"][1]["cg_iterations"] < out["jacobi"][1]["cg_iterations"]
+    assert rel(out["chebyshev"][0], out["jacobi"][0]) < 1e-5
+    with pytest.raises(L.VareError):
+        L.FlipHandle(g, L.default_params(**base, mg_omega_a=2.5))
+
+
 @pytest.mark.parametrize("nu", [1, 2])
 def test_multigrid_solve_independent_of_earlier_solves(nu):
     # 160^3 with 8^3 blocks runs the edge-2 and edge-1 levels through the per-level kernels.  A

@@ -76,7 +76,7 @@ class FlipParams(C.Structure):
                 ("max_iterations", C.c_int), ("preconditioner", C.c_int), ("mg_sweeps", C.c_int),
                 ("mg_omega", C.c_float), ("mg_coarse_sweeps", C.c_int), ("particle_path", C.c_int),
                 ("refine_tol", C.c_float), ("refine_max", C.c_int), ("velocity_fp16", C.c_int),
-                ("flags", C.c_int)]
+                ("flags", C.c_int), ("mg_omega_a", C.c_float), ("mg_omega_b", C.c_float)]
 
 
 class FrameInfo(C.Structure):

@@ -779,8 +779,18 @@ int nwg_k(const vare_flip* h, int K) { return (h->ns + K - 1) / K; }
 // blocks hold every fluid cell of these levels.
 // skip_first: the level-0 pass of two Jacobi sweeps from zero has already written mg_z[0]
 // (k_update_smooth2), so only the restriction runs.
+// Weights of the two sweeps of a smoothing pair: mg_omega_a / mg_omega_b when both are set (two
+// sweeps), else mg_omega for both.
+struct SweepW { float a, b; };
+SweepW sweep_w(const vare_flip* h) {
+    const vare_flip_params& p = h->prm;
+    if (p.mg_sweeps == 2 && p.mg_omega_a > 0.f && p.mg_omega_b > 0.f) return SweepW{p.mg_omega_a, p.mg_omega_b};
+    return SweepW{p.mg_omega, p.mg_omega};
+}
+
 template <int E> void enqueue_block_down(vare_flip* h, int l, int nu, bool skip_first = false) {
     const float w = h->prm.mg_omega;
+    const SweepW sw = sweep_w(h);
     const int* done = done_ptr(h);
     const MgLevelK& L = h->mg[l];
     const MgLevelK& C = h->mg[l + 1];
@@ -793,8 +803,8 @@ template <int E> void enqueue_block_down(vare_flip* h, int l, int nu, bool skip_
         hipLaunchKernelGGL((k_mg0_restrict<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 1, h->mg_zt[l], done);
     } else {
         if (!skip_first)
-            hipLaunchKernelGGL((k_mg0_smooth_first2<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, h->mg_z[l],
-                               done);
+            hipLaunchKernelGGL((k_mg0_smooth_first2<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, sw.a, sw.b,
+                               h->mg_z[l], done);
         hipLaunchKernelGGL((k_mg0_restrict<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 0, h->mg_z[l], done);
     }
 }
@@ -802,6 +812,7 @@ template <int E> void enqueue_block_down(vare_flip* h, int l, int nu, bool skip_
 // rz != nullptr (level 0 only): the last pass also writes per-workgroup partials of r.z.
 template <int E> void enqueue_block_up(vare_flip* h, int l, int nu, double* rz) {
     const float w = h->prm.mg_omega;
+    const SweepW sw = sweep_w(h);
     const int* done = done_ptr(h);
     const MgLevelK& L = h->mg[l];
     constexpr int K = mg_block_k<E>();
@@ -815,10 +826,10 @@ template <int E> void enqueue_block_up(vare_flip* h, int l, int nu, double* rz) 
         hipExtLaunchKernelGGL((k_mg0_prolong_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, w,
                               (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done, rz);
     } else {
-        hipExtLaunchKernelGGL((k_mg0_prolong_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, w,
+        hipExtLaunchKernelGGL((k_mg0_prolong_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, sw.a,
                               (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done,
                               (double*)nullptr);
-        hipLaunchKernelGGL((k_mg0_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, w, (const float*)h->mg_zt[l],
+        hipLaunchKernelGGL((k_mg0_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, sw.b, (const float*)h->mg_zt[l],
                            h->mg_z[l], done, rz);
     }
 }
@@ -839,8 +850,9 @@ template <int KIND> void enqueue_level_down(vare_flip* h, int l, int nu) {
             hipLaunchKernelGGL((k_mg_restrict<KIND, true>), gr, dim3(WG), 0, h->stream, L, C, F, w, h->mg_zt[l], done);
         return;
     }
+    const SweepW sw = sweep_w(h);
     if (L.ncell)
-        hipLaunchKernelGGL((k_mg_smooth_first2<KIND>), dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F, w,
+        hipLaunchKernelGGL((k_mg_smooth_first2<KIND>), dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F, sw.a, sw.b,
                            h->mg_z[l], done);
     if (C.ncell)
         hipLaunchKernelGGL((k_mg_restrict<KIND, false>), gr, dim3(WG), 0, h->stream, L, C, F, w, h->mg_z[l], done);
@@ -858,9 +870,10 @@ template <int KIND> void enqueue_level_up(vare_flip* h, int l, int nu) {
                            (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done);
         return;
     }
-    hipLaunchKernelGGL((k_mg_prolong_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, C, F, w, (const float*)h->mg_z[l],
+    const SweepW sw = sweep_w(h);
+    hipLaunchKernelGGL((k_mg_prolong_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, C, F, sw.a, (const float*)h->mg_z[l],
                        (const float*)h->mg_z[l + 1], h->mg_zt[l], done);
-    hipLaunchKernelGGL((k_mg_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, F, w, (const float*)h->mg_zt[l], h->mg_z[l],
+    hipLaunchKernelGGL((k_mg_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, F, sw.b, (const float*)h->mg_zt[l], h->mg_z[l],
                        done);
 }
 
@@ -925,7 +938,8 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
         const int per = (kMgBottomThreads >> xt.sh3) * kXUpdPasses;     // blocks per x-update workgroup
         nxwg = (unsigned)((xt.nblk + per - 1) / per);
     }
-    hipLaunchKernelGGL(k_mg_bottom, dim3(1 + nxwg), dim3(kMgBottomThreads), 0, h->stream, bt, w, nu,
+    const SweepW sw = sweep_w(h);
+    hipLaunchKernelGGL(k_mg_bottom, dim3(1 + nxwg), dim3(kMgBottomThreads), 0, h->stream, bt, w, sw.a, sw.b, nu,
                        h->prm.mg_coarse_sweeps, done, xt);
     for (int l = lb - 1; l >= 0; --l) {
         const MgLevelK& L = h->mg[l];
@@ -1006,7 +1020,7 @@ template <int B> void enqueue_iteration_k(vare_flip* h, int k) {
         hipEvent_t u0, u1;
         tpair(h, VARE_KERNEL_UPDATE, (int64_t)nv, u0, u1);
         hipExtLaunchKernelGGL((k_update_smooth2<B, kIterK>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, u0, u1, 0,
-                              tab, v, rout, h->prm.mg_omega, h->mg_z[0], h->partials, (const CgState*)h->st);
+                              tab, v, rout, sweep_w(h).a, sweep_w(h).b, h->mg_z[0], h->partials, (const CgState*)h->st);
         hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
                            (const double*)h->partials2, nwg_it);
         // the rest of the V-cycle on the new residual; its last level-0 pass reduces r.z as well
@@ -1402,7 +1416,8 @@ int validate_params(const vare_flip_params* p) {
     if (p->voxel_size <= 0.f) return fail(VARE_EINVAL, "voxel_size must be positive");
     if (p->max_iterations < 1 || p->proj_iteration < 0) return fail(VARE_EINVAL, "bad iteration cap");
     if (p->preconditioner == VARE_PRECOND_MG &&
-        (p->mg_sweeps < 1 || p->mg_sweeps > 2 || !(p->mg_omega > 0.f && p->mg_omega < 1.f) || p->mg_coarse_sweeps < 1))
+        (p->mg_sweeps < 1 || p->mg_sweeps > 2 || !(p->mg_omega > 0.f && p->mg_omega < 1.f) || p->mg_coarse_sweeps < 1 ||
+         !(p->mg_omega_a >= 0.f && p->mg_omega_a < 2.f) || !(p->mg_omega_b >= 0.f && p->mg_omega_b < 2.f)))
         return fail(VARE_EINVAL, "bad multigrid settings");
     return VARE_OK;
 }
@@ -1537,6 +1552,9 @@ int vare_flip_params_default(vare_flip_params* p) {
     p->preconditioner = VARE_PRECOND_NONE;
     p->mg_sweeps = 2;
     p->mg_omega = 0.8f;
+    // the two roots of the degree-2 Chebyshev polynomial on [0.4, 2]: 1 / (1.2 -+ 0.8 / sqrt(2))
+    p->mg_omega_a = 1.576514f;
+    p->mg_omega_b = 0.566352f;
     p->mg_coarse_sweeps = 20;
     p->refine_tol = 0.f;
     p->refine_max = 8;

@@ -256,9 +256,10 @@ __device__ __forceinline__ void mg_nbrs(const MgLevelK& L, int c, const int t[6]
 // descriptor keeps these loads off the descriptor's round trip.
 __device__ __forceinline__ int mg_ldx(int n, int c) { return n >= 0 ? n : c; }
 
-// Two Jacobi sweeps from zero in one pass: zout = z1 + omega (r - A z1)/D with z1 = omega D^-1 r.
+// Two Jacobi sweeps from zero in one pass: zout = z1 + omega2 (r - A z1)/D with z1 = omega D^-1 r.
 template <int KIND>
-__global__ void k_mg_smooth_first2(MgLevelK L, MgFrame F, float omega, float* __restrict__ zout, const int* done) {
+__global__ void k_mg_smooth_first2(MgLevelK L, MgFrame F, float omega, float omega2, float* __restrict__ zout,
+                                   const int* done) {
     int dn = *done;
     const int c = blockIdx.x * WG + threadIdx.x;
     if (c >= L.ncell) return;
@@ -286,7 +287,7 @@ __global__ void k_mg_smooth_first2(MgLevelK L, MgFrame F, float omega, float* __
     float az = d * z1;
 #pragma unroll
     for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
-    const float out = mg_fluid(in) ? z1 + (rc - az) * wd : 0.f;
+    const float out = mg_fluid(in) ? z1 + (rc - az) * (omega2 == omega ? wd : mg_wd(in, omega2)) : 0.f;
     mg_pin(out);
     if (dn) return;
     zout[c] = out;
@@ -583,8 +584,9 @@ __device__ __forceinline__ float bottom_sweep(unsigned in, float r, const float*
 }
 
 // Workgroup 0 runs the bottom V-cycle; workgroups 1.. (when xt.x is set) apply the CG's x += alpha p
-// meanwhile (see XTerm).
-__global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, float omega, int nu, int coarse_sweeps,
+// meanwhile (see XTerm).  omega weighs the coarsest grid's sweeps, wa and wb the first and second
+// sweep of each smoothing pair on the levels above it.
+__global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, float omega, float wa, float wb, int nu, int coarse_sweeps,
                                                                 const int* done, XTerm xt) {
     if (*done) return;
     __shared__ float sr[kMgBottomCells];
@@ -673,7 +675,7 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
         // on the fly (k_mg_smooth_first2)
         for (int c = tid; c < n; c += T) {
             const unsigned ic = in[c];
-            const float wd = mg_wd(ic, omega);
+            const float wd = mg_wd(ic, wa);
             const float z1 = r[c] * wd;
             float out = z1;
             if (nu == 2) {
@@ -681,11 +683,11 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
                 bottom_nbrs(ic, c, sy, szz, nb);
                 float v[6];
 #pragma unroll
-                for (int q = 0; q < 6; ++q) v[q] = r[nb[q]] * mg_wd(in[nb[q]], omega);
+                for (int q = 0; q < 6; ++q) v[q] = r[nb[q]] * mg_wd(in[nb[q]], wa);
                 float az = mg_diag(ic) * z1;
 #pragma unroll
                 for (int q = 0; q < 6; ++q) az -= (ic & (1u << q)) ? v[q] : 0.f;
-                out = z1 + (r[c] - az) * wd;
+                out = z1 + (r[c] - az) * (wb == wa ? wd : mg_wd(ic, wb));
             }
             z[c] = mg_fluid(ic) ? out : 0.f;
         }
@@ -762,12 +764,12 @@ __global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, flo
             float az = mg_diag(ic) * zpc;
 #pragma unroll
             for (int q = 0; q < 6; ++q) az -= (ic & (1u << q)) ? v[q] : 0.f;
-            zt[c] = mg_fluid(ic) ? zpc + (r[c] - az) * mg_wd(ic, omega) : 0.f;
+            zt[c] = mg_fluid(ic) ? zpc + (r[c] - az) * mg_wd(ic, wa) : 0.f;
         }
         __syncthreads();
         zfin = zt;
         if (nu == 2) {                                   // second post-sweep (k_mg_smooth), into z
-            for (int c = tid; c < n; c += T) z[c] = bottom_sweep(in[c], r[c], zt, c, sy, szz, omega);
+            for (int c = tid; c < n; c += T) z[c] = bottom_sweep(in[c], r[c], zt, c, sy, szz, wb);
             __syncthreads();
             zfin = z;
         }

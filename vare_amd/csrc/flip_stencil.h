@@ -463,7 +463,7 @@ __device__ __forceinline__ float mg_first(unsigned in, float r, float omega) {
 // Two Jacobi sweeps from zero: zout = z1 + w (r - A z1)/D, z1 = w D^-1 r.
 template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __restrict__ tab, MgLevelK L,
-                                                                  float omega, float* __restrict__ zout,
+                                                                  float omega, float omega2, float* __restrict__ zout,
                                                                   const int* done) {
     constexpr int B3 = Fine<B>::B3;
     __shared__ float sz[K][B3];
@@ -513,7 +513,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __
 #pragma unroll
             for (int a = 0; a < 3; ++a) hz[a] = mg_first(hi[kb][a], hr[kb][a], omega);
             const float az = apply_row<B>(in[kb], z1[kb], sz[kb], l, side, hz);
-            out = z1[kb] + (rc[kb] - az) * mg_wd(in[kb], omega);
+            out = z1[kb] + (rc[kb] - az) * mg_wd(in[kb], omega2);
         }
         zout[s[kb] * B3 + l] = out;
     }
@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_mg0_smooth_first2(const int* __
 // order, so x is bit-identical.
 template <int B, int K>
 __global__ void __launch_bounds__(Fine<B>::B3) k_update_smooth2(const int* __restrict__ tab, CgVecs v,
-                                                               float* __restrict__ rout, float omega,
+                                                               float* __restrict__ rout, float omega, float omega2,
                                                                float* __restrict__ zout,
                                                                double* __restrict__ partials,
                                                                const CgState* __restrict__ st) {
@@ -592,7 +592,7 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_update_smooth2(const int* __res
 #pragma unroll
             for (int a = 0; a < 3; ++a) hz[a] = mg_first(hi[kb][a], hr[kb][a] - (alpha * ha[kb][a]), omega);
             const float az = apply_row<B>(in[kb], z1[kb], sz[kb], l, side, hz);
-            out = z1[kb] + (rn[kb] - az) * mg_wd(in[kb], omega);
+            out = z1[kb] + (rn[kb] - az) * mg_wd(in[kb], omega2);
         }
         const int c = s[kb] * B3 + l;
         rout[c] = rn[kb];
