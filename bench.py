@@ -79,6 +79,7 @@ def parse():
     ap.add_argument("--no-refine", action="store_true",
                     help="plain FP32 PCG to --tol (recursive residual; its FP64 residual is reported)")
     ap.add_argument("--mg-sweeps", type=int, default=2)
+    ap.add_argument("--mg-coarse-sweeps", type=int, default=20, help="Jacobi sweeps on the coarsest multigrid grid")
     ap.add_argument("--mg-omega-pair", type=float, nargs=2, default=None, metavar=("A", "B"),
                     help="weights of the first and second sweep of each smoothing pair (default: the "
                          "library's Chebyshev pair; 0 0: weighted Jacobi, 0.8 for both)")
@@ -110,7 +111,7 @@ def solver_kw(a, L, n):
                 cg_rel_tol=a.inner_tol if refine else a.tol, refine_tol=a.tol if refine else 0.0,
                 refine_max=16, max_iterations=20000,
                 preconditioner={"none": L.PRECOND_NONE, "jacobi": L.PRECOND_JACOBI, "mg": L.PRECOND_MG}[a.precond],
-                mg_sweeps=a.mg_sweeps, **({} if a.mg_omega_pair is None else
+                mg_sweeps=a.mg_sweeps, mg_coarse_sweeps=a.mg_coarse_sweeps, **({} if a.mg_omega_pair is None else
                                           dict(mg_omega_a=a.mg_omega_pair[0], mg_omega_b=a.mg_omega_pair[1])))
 
 
