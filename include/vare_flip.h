@@ -218,6 +218,26 @@ int vare_flip_step(vare_flip* h, float dt);
  * The multigrid levels are allocated by vare_flip_create / vare_flip_set_params. */
 #define VARE_NOALLOC_SCRATCH_BYTES (4u << 20)
 int vare_flip_step_noalloc(vare_flip* h, float dt);
+/* vare_flip_step on n independent tanks (distinct handles on one device), their kernels launched
+ * together: FLIPMethod::advanceOneStep (FLIPMethod.cu:208-298) per tank, as the reference advances
+ * one solver per call; a caller that runs many solvers (a wedge of tanks) calls this instead of one
+ * vare_flip_step per handle.  Each kernel of the step is launched once for all the tanks that reach
+ * it together, over a list of (tank, workgroup) ranges, so the launch count of a step does not grow
+ * with the number of tanks.  Every tank's result is bit-identical to vare_flip_step on its handle
+ * alone.  dt[i] and status[i] (may be NULL) belong to hs[i]; the return value is the first error.
+ * Kernel timing must be off on every handle (VARE_EINVAL). */
+int vare_flip_step_batched(vare_flip* const* hs, int n, const float* dt, int* status);
+
+typedef struct vare_batch_stats {
+    int64_t steps;            /* vare_flip_step_batched calls */
+    int64_t tanks;            /* tank steps over those calls */
+    int64_t flushes;          /* sync points at which the recorded work was issued */
+    int64_t launches;         /* merged kernel launches */
+    int64_t tank_launches;    /* the kernel launches the tanks would have made on their own */
+} vare_batch_stats;
+/* Counters of the batched steps on `device` since the last reset. */
+int vare_batch_get_stats(int device, vare_batch_stats* out, int reset);
+
 /* FLIPMethod::advanceOneFrame (FLIPMethod.cu:177-206): toVoxel, step(params.dt), postProcess,
  * toWorld. */
 int vare_flip_advance_frame(vare_flip* h);

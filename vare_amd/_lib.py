@@ -121,6 +121,14 @@ class ServerStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("tanks", C.c_int64), ("flushes", C.c_int64), ("launches", C.c_int64),
+                ("tank_launches", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _P = C.c_void_p
 _FP = C.POINTER(C.c_float)
 # vare_collision_sampler(user, wp[3], *sdf, vel[3], grad[3]) -- world units
@@ -146,6 +154,8 @@ SIGNATURES = {
     "vare_flip_particles_to_world": (C.c_int, [_P]),
     "vare_flip_step": (C.c_int, [_P, C.c_float]),
     "vare_flip_step_noalloc": (C.c_int, [_P, C.c_float]),
+    "vare_flip_step_batched": (C.c_int, [C.POINTER(_P), C.c_int, _FP, C.POINTER(C.c_int)]),
+    "vare_batch_get_stats": (C.c_int, [C.c_int, C.POINTER(BatchStats), C.c_int]),
     "vare_flip_advance_frame": (C.c_int, [_P]),
     "vare_flip_post_process": (C.c_int, [_P]),
     "vare_flip_export_surface": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), _FP, _FP]),
@@ -536,6 +546,24 @@ class Server:
         b = C.c_size_t()
         check(load().vare_server_request_bytes(C.byref(r), C.byref(b)))
         return b.value
+
+
+def step_batched(handles, dt):
+    """vare_flip_step_batched: one step of every handle, the kernels of all tanks launched together.
+    dt is one value or one per handle.  Returns the per-tank status codes (raises on the first error)."""
+    n = len(handles)
+    hs = (_P * n)(*[h.h for h in handles])
+    dts = (C.c_float * n)(*([float(dt)] * n if np.isscalar(dt) else [float(x) for x in dt]))
+    st = (C.c_int * n)()
+    check(load().vare_flip_step_batched(hs, n, dts, st))
+    return list(st)
+
+
+def batch_stats(device=0, reset=False):
+    """Counters of the batched steps: calls, tank steps, sync points, merged and per-tank launches."""
+    s = BatchStats()
+    check(load().vare_batch_get_stats(int(device), C.byref(s), 1 if reset else 0))
+    return s.as_dict()
 
 
 def plan_bytes(grid, params, particle_capacity=0, block_capacity=0):

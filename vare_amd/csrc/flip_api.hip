@@ -17,6 +17,7 @@
 #include <thread>
 #include <atomic>
 #include <mutex>
+#include <condition_variable>
 #include <unordered_map>
 #include <array>
 
@@ -26,6 +27,7 @@
 #include "flip_particles.h"
 #include "flip_stencil.h"
 #include "flip_post.h"
+#include "flip_batch.h"
 
 using namespace vare;
 
@@ -181,6 +183,15 @@ struct vare_flip {
     int dry = 0;                  // planning handle (vare_flip_plan_bytes): count, allocate nothing
     uintptr_t dry_next = 0;
 
+    // batched stepping (vare_flip_step_batched): while `rec` is set, launches, readbacks and event
+    // records are recorded here and issued, merged with the other tanks' operations, at the next
+    // sync point; device frees wait for that flush (a recorded launch may still use the array)
+    struct BatchRun* rec = nullptr;
+    std::vector<ROp> rops;
+    std::vector<unsigned char> rpack;
+    std::vector<void*> rfree;
+    int rec_rc = 0;
+
     size_t ncells() const { return (size_t)nb * (size_t)B * B * B; }
     size_t vsz() const { return vhalf ? 2 : 4; }
     void* vptr(int a, size_t off) const { return (char*)pa[3 + a] + off * vsz(); }
@@ -244,7 +255,9 @@ void dfree_bytes(vare_flip* h, void* p) {
         h->dev_bytes -= it->second;
         h->allocs.erase(it);
     }
-    if (!h->dry) (void)hipFree(p);
+    if (h->dry) return;
+    if (h->rec) h->rfree.push_back(p);
+    else (void)hipFree(p);
 }
 template <class T> int dalloc(vare_flip* h, T** p, size_t n) {
     void* v = nullptr;
@@ -255,6 +268,82 @@ template <class T> int dalloc(vare_flip* h, T** p, size_t n) {
 template <class T> void dfree(vare_flip* h, T*& p) {
     dfree_bytes(h, (void*)p);
     p = nullptr;
+}
+
+// ---- launches, readbacks and syncs of the step: direct on the handle's stream, or recorded while
+// the handle steps in a batch ----
+int batch_sync(vare_flip* h);
+
+template <class F, class... P, class... A>
+void record_kernel(vare_flip* h, dim3 grid, dim3 block, size_t shm, A&&... a) {
+    if (grid.x == 0) return;
+    Pack<P...> pk;
+    pack_set(pk, std::index_sequence_for<P...>{}, std::forward<A>(a)...);
+    ROp op;
+    op.kind = ROp::kKernel;
+    op.issue = &batched_issue<F, F::kLB, F::kWPE, P...>;
+    op.grid = grid.x;
+    op.block = block;
+    op.shm = (unsigned)shm;
+    op.pack_off = (unsigned)h->rpack.size();
+    op.pack_len = (unsigned)sizeof pk;
+    op.max_tanks = batch_tanks<P...>();
+    const unsigned char* b = (const unsigned char*)&pk;
+    h->rpack.insert(h->rpack.end(), b, b + sizeof pk);
+    h->rops.push_back(op);
+}
+
+// Launch of kernel `kern` (functor F) with one-dimensional grid `grid`.
+template <class F, class... P, class... A>
+void vlaunch(vare_flip* h, void (*kern)(P...), dim3 grid, dim3 block, size_t shm, A&&... a) {
+    if (!h->rec) {
+        hipLaunchKernelGGL(kern, grid, block, shm, h->stream, std::forward<A>(a)...);
+        return;
+    }
+    record_kernel<F, P...>(h, grid, block, shm, std::forward<A>(a)...);
+}
+// The same with start/stop events on the kernel's own dispatch (kernel timing; never set while
+// the handle steps in a batch).
+template <class F, class... P, class... A>
+void vlaunch_ev(vare_flip* h, void (*kern)(P...), dim3 grid, dim3 block, size_t shm, hipEvent_t e0, hipEvent_t e1,
+                A&&... a) {
+    if (!h->rec) {
+        hipExtLaunchKernelGGL(kern, grid, block, shm, h->stream, e0, e1, 0, std::forward<A>(a)...);
+        return;
+    }
+    record_kernel<F, P...>(h, grid, block, shm, std::forward<A>(a)...);
+}
+
+hipError_t hevent(vare_flip* h, hipEvent_t e) {
+    if (!h->rec) return hipEventRecord(e, h->stream);
+    ROp op;
+    op.kind = ROp::kEvent;
+    op.ev = e;
+    h->rops.push_back(op);
+    return hipSuccess;
+}
+
+// Waits for everything the handle has enqueued.
+int hsync(vare_flip* h) {
+    if (h->rec) return batch_sync(h);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return VARE_OK;
+}
+
+// Device-to-host copy of a few scalars, then a sync.
+int hread(vare_flip* h, void* dst, const void* src, size_t bytes) {
+    if (!h->rec) {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return VARE_OK;
+    }
+    ROp op;
+    op.kind = ROp::kCopy;
+    op.dst = dst;
+    op.src = src;
+    op.bytes = bytes;
+    h->rops.push_back(op);
+    return batch_sync(h);
 }
 
 void free_mg(vare_flip* h) {
@@ -396,7 +485,7 @@ int ensure_tiles(vare_flip* h, size_t ntiles) {
 
 int fill_u32(vare_flip* h, void* p, uint32_t v, size_t n) {
     if (n == 0) return VARE_OK;
-    hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(n)), dim3(WG), 0, h->stream, (uint32_t*)p, v, n);
+    vlaunch<k_fill_u32_f>(h, k_fill_u32, dim3(nblocks(n)), dim3(WG), 0, (uint32_t*)p, v, n);
     HIPCHK(hipGetLastError());
     return VARE_OK;
 }
@@ -406,9 +495,9 @@ template <class Op> int run_scan(vare_flip* h, const Op& op, int n, int total_sl
     if (n <= 0) return fill_u32(h, h->dint + total_slot, 0, 1);
     const size_t ntiles = ((size_t)n + SCAN_TILE - 1) / SCAN_TILE;
     CHK(ensure_tiles(h, ntiles));
-    hipLaunchKernelGGL(k_scan_reduce<Op>, dim3((unsigned)ntiles), dim3(WG), 0, h->stream, op, n, h->tiles);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(WG), 0, h->stream, h->tiles, (int)ntiles, h->dint + total_slot);
-    hipLaunchKernelGGL(k_scan_apply<Op>, dim3((unsigned)ntiles), dim3(WG), 0, h->stream, op, n, (const int*)h->tiles);
+    vlaunch<k_scan_reduce_f<Op>>(h, k_scan_reduce<Op>, dim3((unsigned)ntiles), dim3(WG), 0, op, n, h->tiles);
+    vlaunch<k_scan_tiles_f>(h, k_scan_tiles, dim3(1), dim3(WG), 0, h->tiles, (int)ntiles, h->dint + total_slot);
+    vlaunch<k_scan_apply_f<Op>>(h, k_scan_apply<Op>, dim3((unsigned)ntiles), dim3(WG), 0, op, n, (const int*)h->tiles);
     HIPCHK(hipGetLastError());
     return VARE_OK;
 }
@@ -501,17 +590,24 @@ G2PParams g2p_params(const vare_flip* h, float dt) {
     return gp;
 }
 
+struct k_cg_setup_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, CgState* st, int mode, int max_count, float abs_tol, float rel_tol, int precond) {
+        if (threadIdx.x != 0) return;
+        st->mode = mode;
+        st->max_count = max_count;
+        st->abs_tol = abs_tol;
+        st->rel_tol = rel_tol;
+        st->precond = precond;
+        st->done = 0;
+        st->iter = 0;
+        st->count = 0;
+        st->converged = 0;
+    }
+};
 __global__ void k_cg_setup(CgState* st, int mode, int max_count, float abs_tol, float rel_tol, int precond) {
-    if (threadIdx.x != 0) return;
-    st->mode = mode;
-    st->max_count = max_count;
-    st->abs_tol = abs_tol;
-    st->rel_tol = rel_tol;
-    st->precond = precond;
-    st->done = 0;
-    st->iter = 0;
-    st->count = 0;
-    st->converged = 0;
+    k_cg_setup_f::run(blockIdx.x, gridDim.x, st, mode, max_count, abs_tol, rel_tol, precond);
 }
 
 // Binning: bin every particle by the storage index of its P2G base cell (bin Nc for particles
@@ -525,21 +621,21 @@ template <int B> int bin_particles(vare_flip* h) {
     CHK(fill_u32(h, h->dint + 4, 0, 2));           // [4] sort overflow, [5] long-bin count
     CHK(fill_u32(h, h->dint + 7, 0, 1));           // [7] mid-bin count
     if (h->np)
-        hipLaunchKernelGGL(k_ckey<B>, dim3(nblocks((h->np + kKeySlices - 1) / kKeySlices)), dim3(WG), 0, h->stream,
+        vlaunch<k_ckey_f<B>>(h, k_ckey<B>, dim3(nblocks((h->np + kKeySlices - 1) / kKeySlices)), dim3(WG), 0,
                            (const float*)h->pa[0],
                            (const float*)h->pa[1], (const float*)h->pa[2], (int)h->np, h->frame(), nc, h->pkey, h->prank,
                            h->ccount);
     OpBuckets op{h->ccount, h->cstart};
     CHK(run_scan(h, op, nc + 2, 3));
     if (h->np) {
-        hipLaunchKernelGGL(k_pscatter, dim3(nblocks((h->np + kBinSlices - 1) / kBinSlices)), dim3(WG), 0, h->stream,
+        vlaunch<k_pscatter_f>(h, k_pscatter, dim3(nblocks((h->np + kBinSlices - 1) / kBinSlices)), dim3(WG), 0,
                            (int)h->np, (const int*)h->pkey,
                            (const int*)h->prank, (const int*)h->cstart, (const int*)h->pid, h->perm);
-        hipLaunchKernelGGL(k_sort_block<B>, dim3(h->nb + 1), dim3(WG), 0, h->stream, (const int*)h->cstart, (int)h->nb,
+        vlaunch<k_sort_block_f<B>>(h, k_sort_block<B>, dim3(h->nb + 1), dim3(WG), 0, (const int*)h->cstart, (int)h->nb,
                            nc, h->perm, h->midbins, h->dint + 7, h->longbins, h->dint + 5);
-        hipLaunchKernelGGL(k_sort_wave, dim3(kSortLongGrid * 4), dim3(WG), 0, h->stream, (const int*)h->cstart,
+        vlaunch<k_sort_wave_f>(h, k_sort_wave, dim3(kSortLongGrid * 4), dim3(WG), 0, (const int*)h->cstart,
                            (const int*)h->midbins, (const int*)(h->dint + 7), h->perm);
-        hipLaunchKernelGGL(k_sort_long, dim3(kSortLongGrid), dim3(1024), 0, h->stream, (const int*)h->cstart,
+        vlaunch<k_sort_long_f>(h, k_sort_long, dim3(kSortLongGrid), dim3(1024), 0, (const int*)h->cstart,
                            (const int*)h->longbins, (const int*)(h->dint + 5), h->perm, h->dint + 4);
         // move the particles into bin order (double-buffered) and keep their logical indices
         // inside vare_flip_step the adjacency is rewritten by G2P for every particle before anything
@@ -548,7 +644,7 @@ template <int B> int bin_particles(vare_flip* h) {
         for (int a = 0; a < 7; ++a) { ra.src[a] = h->pa[a]; ra.dst[a] = h->pb[a]; }
         ra.vsz = (int)h->vsz();
         ra.n = h->in_step ? 6 : 7;
-        hipLaunchKernelGGL(k_reorder, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np,
+        vlaunch<k_reorder_f>(h, k_reorder, dim3(nblocks(h->np)), dim3(WG), 0, (int)h->np,
                            (const unsigned long long*)h->perm, ra, h->pid2);
         for (int a = 0; a < ra.n; ++a) std::swap(h->pa[a], h->pb[a]);
         std::swap(h->pid, h->pid2);
@@ -564,27 +660,26 @@ template <int B> int stage_prepare(vare_flip* h) {
     if (h->np) {
         h->occ_stamp = h->occ_stamp == INT_MAX ? 1 : h->occ_stamp + 1;
         if (h->occ_stamp == 1) CHK(fill_u32(h, h->occ, 0, h->nocc()));
-        hipLaunchKernelGGL(k_bin_occ<B>, dim3(nblocks((h->np + kOccSlices - 1) / kOccSlices)), dim3(WG), 0, h->stream,
+        vlaunch<k_bin_occ_f<B>>(h, k_bin_occ<B>, dim3(nblocks((h->np + kOccSlices - 1) / kOccSlices)), dim3(WG), 0,
                            h->pa[0], h->pa[1], h->pa[2],
                            (int)h->np, h->fx, h->fy, h->fz, h->occ_stamp, h->occ);
-        hipLaunchKernelGGL(k_bin_dilate, dim3(nblocks(h->nocc())), dim3(WG), 0, h->stream, h->fx, h->fy, h->fz,
+        vlaunch<k_bin_dilate_f>(h, k_bin_dilate, dim3(nblocks(h->nocc())), dim3(WG), 0, h->fx, h->fy, h->fz,
                            h->occ_stamp, (const int*)h->occ, h->flag);
     }
     tend(h, VARE_KERNEL_BIN);
     if (h->prm.enable_water_tank) {
         const int tall = std::min(std::max((int)(h->vox_wall) / B, 0), h->fy);   // SparseFrame.cu:165
-        hipLaunchKernelGGL(k_bin_tank, dim3(nblocks(h->nblk)), dim3(WG), 0, h->stream, h->fx, h->fy, h->fz, tall, h->flag);
+        vlaunch<k_bin_tank_f>(h, k_bin_tank, dim3(nblocks(h->nblk)), dim3(WG), 0, h->fx, h->fy, h->fz, tall, h->flag);
     }
     HIPCHK(hipGetLastError());
     OpBlocks op{h->flag, h->slot, h->coord};
     CHK(run_scan(h, op, h->nblk, 0));
-    HIPCHK(hipMemcpyAsync(h->dint_host, h->dint, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    CHK(hread(h, h->dint_host, h->dint, sizeof(int)));
     const int nb = h->dint_host[0];
     CHK(ensure_blocks(h, (size_t)nb));
     h->nb = nb;
     if (nb)
-        hipLaunchKernelGGL(k_block_nbr, dim3(nblocks((size_t)nb * 6)), dim3(WG), 0, h->stream, nb, h->fx, h->fy, h->fz,
+        vlaunch<k_block_nbr_f>(h, k_block_nbr, dim3(nblocks((size_t)nb * 6)), dim3(WG), 0, nb, h->fx, h->fy, h->fz,
                            (const int*)h->coord, (const int*)h->slot, h->nbr);
     HIPCHK(hipGetLastError());
     if (h->prm.particle_path == 0) CHK(bin_particles<B>(h));
@@ -601,16 +696,16 @@ template <int B> int stage_p2g(vare_flip* h) {
         if (h->nb) {
             const FrameK fk = h->frame();
             if (h->vhalf)
-                hipLaunchKernelGGL((k_p2g_own<B, true>), dim3(h->nb), dim3(P2GOwn<B>::THREADS), 0, h->stream,
+                vlaunch<k_p2g_own_f<B, true>>(h, k_p2g_own<B, true>, dim3(h->nb), dim3(P2GOwn<B>::THREADS), 0,
                                    (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
                                    (const int*)h->cstart, fk, cf, h->cx, h->cy, h->cz, h->type, h->colk(),
                                    h->prm.col_vel_scale, h->p2gpart);
             else
-                hipLaunchKernelGGL((k_p2g_own<B, false>), dim3(h->nb), dim3(P2GOwn<B>::THREADS), 0, h->stream,
+                vlaunch<k_p2g_own_f<B, false>>(h, k_p2g_own<B, false>, dim3(h->nb), dim3(P2GOwn<B>::THREADS), 0,
                                    (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
                                    (const int*)h->cstart, fk, cf, h->cx, h->cy, h->cz, h->type, h->colk(),
                                    h->prm.col_vel_scale, h->p2gpart);
-            hipLaunchKernelGGL(k_p2g_faces<B>, dim3(h->nb), dim3(P2GOwn<B>::FACE_THREADS), 0, h->stream, fk, cf, h->cx, h->cy,
+            vlaunch<k_p2g_faces_f<B>>(h, k_p2g_faces<B>, dim3(h->nb), dim3(P2GOwn<B>::FACE_THREADS), 0, fk, cf, h->cx, h->cy,
                                h->cz, h->type, h->colk(), h->prm.col_vel_scale, (const float*)h->p2gpart);
         }
         // algorithmic bytes of both passes: positions and velocities read once (24 B per particle, 18 with
@@ -623,11 +718,11 @@ template <int B> int stage_p2g(vare_flip* h) {
     CHK(fill_u32(h, h->vz, 0, nc)); CHK(fill_u32(h, h->cnt, 0, nc));
     tbegin(h, VARE_KERNEL_P2G);
     if (h->np)
-        hipLaunchKernelGGL(k_p2g_atomic<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
+        vlaunch<k_p2g_atomic_f<B>>(h, k_p2g_atomic<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->pa[0], h->pa[1], h->pa[2],
                            h->vel(), (int)h->np, h->frame(), cf);
     tend(h, VARE_KERNEL_P2G);
     if (nc)
-        hipLaunchKernelGGL(k_cell_finalize<B>, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, cf, h->cx, h->cy,
+        vlaunch<k_cell_finalize_f<B>>(h, k_cell_finalize<B>, dim3(nblocks(nc)), dim3(WG), 0, (int)nc, cf, h->cx, h->cy,
                            h->cz, h->type, h->frame(), h->colk(), h->prm.col_vel_scale);
     HIPCHK(hipGetLastError());
     return VARE_OK;
@@ -644,9 +739,9 @@ void enqueue_precond_init(vare_flip* h) {
     const int nv = h->nvirt();
     const unsigned gs = rgrid((size_t)nv);
     if (nv)
-        hipLaunchKernelGGL(k_cg_precond_init, dim3(gs), dim3(WG), 0, h->stream, nv, h->clist(), vecs(h), h->partials,
+        vlaunch<k_cg_precond_init_f>(h, k_cg_precond_init, dim3(gs), dim3(WG), 0, nv, h->clist(), vecs(h), h->partials,
                            (const CgState*)h->st);
-    hipLaunchKernelGGL(k_fin_precond_init, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
+    vlaunch<k_fin_precond_init_f>(h, k_fin_precond_init, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials,
                        nv ? (int)gs : 0);
 }
 
@@ -655,26 +750,25 @@ template <int B> int stage_system(vare_flip* h) {
     const int mode = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? 1 : 0;
     const int precond = mode == 1 ? h->prm.preconditioner : 0;
     const int maxc = mode == 0 ? h->prm.proj_iteration : h->prm.max_iterations;
-    hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(64), 0, h->stream, h->st, mode, maxc, h->prm.cg_abs_tol,
+    vlaunch<k_cg_setup_f>(h, k_cg_setup, dim3(1), dim3(64), 0, h->st, mode, maxc, h->prm.cg_abs_tol,
                        h->prm.cg_rel_tol, precond);
     const unsigned g = rgrid(nc);
     CHK(fill_u32(h, h->bfluid, 0, (size_t)h->nb));
     tbegin(h, VARE_KERNEL_DIV);
-    hipLaunchKernelGGL(k_divergence<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(), (const uint8_t*)h->type,
+    vlaunch<k_divergence_f<B>>(h, k_divergence<B>, dim3(g), dim3(WG), 0, (int)nc, h->frame(), (const uint8_t*)h->type,
                        (const float*)h->cx, (const float*)h->cy, (const float*)h->cz, h->vox_wall, vecs(h), h->partials,
                        h->partials2, h->bfluid);
     tend(h, VARE_KERNEL_DIV);
-    hipLaunchKernelGGL(k_fin_divergence, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
+    vlaunch<k_fin_divergence_f>(h, k_fin_divergence, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials,
                        (const double*)h->partials2, (int)g);
     // the solver's blocks: those holding a fluid cell
     OpFluidBlocks ofb{h->bfluid, h->blist};
     CHK(run_scan(h, ofb, h->nb, 6));
-    HIPCHK(hipMemcpyAsync(h->dint_host + 6, h->dint + 6, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    CHK(hread(h, h->dint_host + 6, h->dint + 6, sizeof(int)));
     h->ns = h->dint_host[6];
     if (h->ns) {
         const int nent = wg_table_entries(h->ns);
-        hipLaunchKernelGGL(k_wg_table, dim3(nblocks((size_t)nent * kTab)), dim3(WG), 0, h->stream,
+        vlaunch<k_wg_table_f>(h, k_wg_table, dim3(nblocks((size_t)nent * kTab)), dim3(WG), 0,
                            (const int*)h->blist, h->ns, (const int*)h->nbr, nent, h->wgtab);
     }
     if (mode == 1) {
@@ -758,8 +852,8 @@ int mg_setup(vare_flip* h) {
         MgLevelK& L = h->mg[l];
         if (!L.dense) L.ncell = h->nb << (3 * L.sh);
         if (L.ncell)
-            hipLaunchKernelGGL(k_mg_flag, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, h->mg[l - 1], L, F);
-        if (L.ncell) hipLaunchKernelGGL(k_mg_info, dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F);
+            vlaunch<k_mg_flag_f>(h, k_mg_flag, dim3(nblocks(L.ncell)), dim3(WG), 0, h->mg[l - 1], L, F);
+        if (L.ncell) vlaunch<k_mg_info_f>(h, k_mg_info, dim3(nblocks(L.ncell)), dim3(WG), 0, L, F);
     }
     HIPCHK(hipGetLastError());
     return VARE_OK;
@@ -800,12 +894,12 @@ template <int E> void enqueue_block_down(vare_flip* h, int l, int nu, bool skip_
     constexpr unsigned E3 = (unsigned)(E * E * E);
     if (!nwg) return;
     if (nu == 1) {
-        hipLaunchKernelGGL((k_mg0_restrict<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 1, h->mg_zt[l], done);
+        vlaunch<k_mg0_restrict_f<E, K>>(h, k_mg0_restrict<E, K>, dim3(nwg), dim3(E3), 0, tab, L, C, w, 1, h->mg_zt[l], done);
     } else {
         if (!skip_first)
-            hipLaunchKernelGGL((k_mg0_smooth_first2<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, sw.a, sw.b,
+            vlaunch<k_mg0_smooth_first2_f<E, K>>(h, k_mg0_smooth_first2<E, K>, dim3(nwg), dim3(E3), 0, tab, L, sw.a, sw.b,
                                h->mg_z[l], done);
-        hipLaunchKernelGGL((k_mg0_restrict<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, C, w, 0, h->mg_z[l], done);
+        vlaunch<k_mg0_restrict_f<E, K>>(h, k_mg0_restrict<E, K>, dim3(nwg), dim3(E3), 0, tab, L, C, w, 0, h->mg_z[l], done);
     }
 }
 
@@ -823,13 +917,13 @@ template <int E> void enqueue_block_up(vare_flip* h, int l, int nu, double* rz) 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (l == 0) tpair(h, VARE_KERNEL_MG0, (int64_t)h->ns * E3, e0, e1);
     if (nu == 1) {
-        hipExtLaunchKernelGGL((k_mg0_prolong_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, w,
+        vlaunch_ev<k_mg0_prolong_smooth_f<E, K>>(h, k_mg0_prolong_smooth<E, K>, dim3(nwg), dim3(E3), 0, e0, e1, tab, L, w,
                               (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done, rz);
     } else {
-        hipExtLaunchKernelGGL((k_mg0_prolong_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, e0, e1, 0, tab, L, sw.a,
+        vlaunch_ev<k_mg0_prolong_smooth_f<E, K>>(h, k_mg0_prolong_smooth<E, K>, dim3(nwg), dim3(E3), 0, e0, e1, tab, L, sw.a,
                               (const float*)h->mg_z[l], (const float*)h->mg_z[l + 1], h->mg_zt[l], done,
                               (double*)nullptr);
-        hipLaunchKernelGGL((k_mg0_smooth<E, K>), dim3(nwg), dim3(E3), 0, h->stream, tab, L, sw.b, (const float*)h->mg_zt[l],
+        vlaunch<k_mg0_smooth_f<E, K>>(h, k_mg0_smooth<E, K>, dim3(nwg), dim3(E3), 0, tab, L, sw.b, (const float*)h->mg_zt[l],
                            h->mg_z[l], done, rz);
     }
 }
@@ -847,15 +941,15 @@ template <int KIND> void enqueue_level_down(vare_flip* h, int l, int nu) {
     const dim3 gr(nblocks((size_t)C.ncell * 8));
     if (nu == 1) {
         if (C.ncell)
-            hipLaunchKernelGGL((k_mg_restrict<KIND, true>), gr, dim3(WG), 0, h->stream, L, C, F, w, h->mg_zt[l], done);
+            vlaunch<k_mg_restrict_f<KIND, true>>(h, k_mg_restrict<KIND, true>, gr, dim3(WG), 0, L, C, F, w, h->mg_zt[l], done);
         return;
     }
     const SweepW sw = sweep_w(h);
     if (L.ncell)
-        hipLaunchKernelGGL((k_mg_smooth_first2<KIND>), dim3(nblocks(L.ncell)), dim3(WG), 0, h->stream, L, F, sw.a, sw.b,
+        vlaunch<k_mg_smooth_first2_f<KIND>>(h, k_mg_smooth_first2<KIND>, dim3(nblocks(L.ncell)), dim3(WG), 0, L, F, sw.a, sw.b,
                            h->mg_z[l], done);
     if (C.ncell)
-        hipLaunchKernelGGL((k_mg_restrict<KIND, false>), gr, dim3(WG), 0, h->stream, L, C, F, w, h->mg_z[l], done);
+        vlaunch<k_mg_restrict_f<KIND, false>>(h, k_mg_restrict<KIND, false>, gr, dim3(WG), 0, L, C, F, w, h->mg_z[l], done);
 }
 
 template <int KIND> void enqueue_level_up(vare_flip* h, int l, int nu) {
@@ -866,14 +960,14 @@ template <int KIND> void enqueue_level_up(vare_flip* h, int l, int nu) {
     const MgLevelK& C = h->mg[l + 1];
     const dim3 gr(nblocks(L.ncell));
     if (nu == 1) {
-        hipLaunchKernelGGL((k_mg_prolong_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, C, F, w,
+        vlaunch<k_mg_prolong_smooth_f<KIND>>(h, k_mg_prolong_smooth<KIND>, gr, dim3(WG), 0, L, C, F, w,
                            (const float*)h->mg_zt[l], (const float*)h->mg_z[l + 1], h->mg_z[l], done);
         return;
     }
     const SweepW sw = sweep_w(h);
-    hipLaunchKernelGGL((k_mg_prolong_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, C, F, sw.a, (const float*)h->mg_z[l],
+    vlaunch<k_mg_prolong_smooth_f<KIND>>(h, k_mg_prolong_smooth<KIND>, gr, dim3(WG), 0, L, C, F, sw.a, (const float*)h->mg_z[l],
                        (const float*)h->mg_z[l + 1], h->mg_zt[l], done);
-    hipLaunchKernelGGL((k_mg_smooth<KIND>), gr, dim3(WG), 0, h->stream, L, F, sw.b, (const float*)h->mg_zt[l], h->mg_z[l],
+    vlaunch<k_mg_smooth_f<KIND>>(h, k_mg_smooth<KIND>, gr, dim3(WG), 0, L, F, sw.b, (const float*)h->mg_zt[l], h->mg_z[l],
                        done);
 }
 
@@ -939,7 +1033,7 @@ template <int B> void enqueue_vcycle_t(vare_flip* h, double* rz, float* r0, bool
         nxwg = (unsigned)((xt.nblk + per - 1) / per);
     }
     const SweepW sw = sweep_w(h);
-    hipLaunchKernelGGL(k_mg_bottom, dim3(1 + nxwg), dim3(kMgBottomThreads), 0, h->stream, bt, w, sw.a, sw.b, nu,
+    vlaunch<k_mg_bottom_f>(h, k_mg_bottom, dim3(1 + nxwg), dim3(kMgBottomThreads), 0, bt, w, sw.a, sw.b, nu,
                        h->prm.mg_coarse_sweeps, done, xt);
     for (int l = lb - 1; l >= 0; --l) {
         const MgLevelK& L = h->mg[l];
@@ -996,7 +1090,7 @@ template <int B> void enqueue_iteration_k(vare_flip* h, int k) {
     v.r = rin;
     if (k == 0) {
         v.p = pk;
-        hipLaunchKernelGGL((k_spmv_blk<B, kSpmvK>), dim3(nwg_sp), dim3(B * B * B), 0, h->stream, tab, v, h->partials,
+        vlaunch<k_spmv_blk_f<B, kSpmvK>>(h, k_spmv_blk<B, kSpmvK>, dim3(nwg_sp), dim3(B * B * B), 0, tab, v, h->partials,
                            (const CgState*)h->st);
     } else {
         v.p = pbuf[(k + 1) & 1];
@@ -1004,44 +1098,41 @@ template <int B> void enqueue_iteration_k(vare_flip* h, int k) {
         tpair(h, VARE_KERNEL_SPMV, (int64_t)nv, e0, e1);
         const int pc = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? h->prm.preconditioner : VARE_PRECOND_NONE;
         if (pc == VARE_PRECOND_MG)
-            hipExtLaunchKernelGGL((k_pupdate_spmv<B, kIterK, 2>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, e0, e1,
-                                  0, tab, v, pk, h->partials, (const CgState*)h->st);
+            vlaunch_ev<k_pupdate_spmv_f<B, kIterK, 2>>(h, k_pupdate_spmv<B, kIterK, 2>, dim3(nwg_it), dim3(B * B * B), 0, e0, e1, tab, v, pk, h->partials, (const CgState*)h->st);
         else if (pc == VARE_PRECOND_JACOBI)
-            hipExtLaunchKernelGGL((k_pupdate_spmv<B, kIterK, 1>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, e0, e1,
-                                  0, tab, v, pk, h->partials, (const CgState*)h->st);
+            vlaunch_ev<k_pupdate_spmv_f<B, kIterK, 1>>(h, k_pupdate_spmv<B, kIterK, 1>, dim3(nwg_it), dim3(B * B * B), 0, e0, e1, tab, v, pk, h->partials, (const CgState*)h->st);
         else
-            hipExtLaunchKernelGGL((k_pupdate_spmv<B, kIterK, 0>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, e0, e1,
-                                  0, tab, v, pk, h->partials, (const CgState*)h->st);
+            vlaunch_ev<k_pupdate_spmv_f<B, kIterK, 0>>(h, k_pupdate_spmv<B, kIterK, 0>, dim3(nwg_it), dim3(B * B * B), 0, e0, e1, tab, v, pk, h->partials, (const CgState*)h->st);
     }
-    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
+    vlaunch<k_fin_alpha_f>(h, k_fin_alpha, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials,
                        k == 0 ? nwg_sp : nwg_it);
     v.p = pk;
     if (fuse) {
         hipEvent_t u0, u1;
         tpair(h, VARE_KERNEL_UPDATE, (int64_t)nv, u0, u1);
-        hipExtLaunchKernelGGL((k_update_smooth2<B, kIterK>), dim3(nwg_it), dim3(B * B * B), 0, h->stream, u0, u1, 0,
+        vlaunch_ev<k_update_smooth2_f<B, kIterK>>(h, k_update_smooth2<B, kIterK>, dim3(nwg_it), dim3(B * B * B), 0, u0, u1,
                               tab, v, rout, sweep_w(h).a, sweep_w(h).b, h->mg_z[0], h->partials, (const CgState*)h->st);
-        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
+        vlaunch<k_fin_update_f>(h, k_fin_update, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials,
                            (const double*)h->partials2, nwg_it);
         // the rest of the V-cycle on the new residual; its last level-0 pass reduces r.z as well
         // the V-cycle's bottom launch also applies this iteration's x += alpha p (p_k, alpha_k)
         const XTerm xt{h->x, pk, (const float*)((const char*)h->st + offsetof(CgState, alpha)), h->blist,
                        B == 8 ? 9 : 6, h->ns};
         enqueue_vcycle(h, h->partials, rout, true, xt);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials, nwg_v0);
+        vlaunch<k_fin_beta_f>(h, k_fin_beta, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials, nwg_v0);
         return;
     }
     const unsigned gs = rgrid((size_t)nv);
     tbegin(h, VARE_KERNEL_UPDATE);
-    hipLaunchKernelGGL(k_cg_update, dim3(gs), dim3(WG), 0, h->stream, nv, h->clist(), v, h->partials, h->partials2,
+    vlaunch<k_cg_update_f>(h, k_cg_update, dim3(gs), dim3(WG), 0, nv, h->clist(), v, h->partials, h->partials2,
                        (const CgState*)h->st);
     tend(h, VARE_KERNEL_UPDATE);
-    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
+    vlaunch<k_fin_update_f>(h, k_fin_update, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials,
                        (const double*)h->partials2, (int)gs);
     if (mg) {
         // the V-cycle's last level-0 pass reduces r.z as well (one partial per workgroup)
         enqueue_vcycle(h, h->partials);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials, nwg_v0);
+        vlaunch<k_fin_beta_f>(h, k_fin_beta, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials, nwg_v0);
     }
 }
 
@@ -1055,8 +1146,7 @@ template <int B> int run_solver(vare_flip* h, int cap, int k, int& total_iters) 
     int launched = 0;
     int chunk = last > 0 ? last : 8;
     for (;;) {
-        HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
+        CHK(hread(h, h->st_host, h->st, sizeof(CgState)));
         if (h->st_host->done) break;
         if (launched > cap + 1) break;                       // defensive: the device flag ends first
         for (int q = 0; q < chunk; ++q) enqueue_iteration<B>(h, launched + q);
@@ -1075,7 +1165,7 @@ template <int B> int stage_solve(vare_flip* h) {
     const unsigned g = rgrid(nc);
     const int mode = h->prm.solver_mode == VARE_SOLVER_RELATIVE ? 1 : 0;
     const int cap = mode == 0 ? h->prm.proj_iteration + 2 : h->prm.max_iterations;
-    HIPCHK(hipEventRecord(h->step_ev[2], h->stream));
+    HIPCHK(hevent(h, h->step_ev[2]));
     const bool refine = mode == 1 && h->prm.refine_tol > 0.f;
     int total_iters = 0, outer = 0;
     const int nv = h->nvirt();
@@ -1086,24 +1176,23 @@ template <int B> int stage_solve(vare_flip* h) {
     const XTail tail{fused ? (const float*)h->p : nullptr, (const float*)h->p2, (const CgState*)h->st};
     CHK(run_solver<B>(h, cap, 0, total_iters));
     if (!refine && fused && nv)
-        hipLaunchKernelGGL(k_cg_xflush, dim3(gv), dim3(WG), 0, h->stream, nv, h->clist(), h->x, tail);
+        vlaunch<k_cg_xflush_f>(h, k_cg_xflush, dim3(gv), dim3(WG), 0, nv, h->clist(), h->x, tail);
     if (refine) {
         // FP64 iterative refinement: x64 += d, r = b - A x64 in FP64, solve A d = r in FP32
         for (;;) {
             // over the solver's blocks only: the other blocks hold no fluid cell, and x and x64 stay 0 there
-            hipLaunchKernelGGL(k_ir_accumulate, dim3(gv), dim3(WG), 0, h->stream, nv, h->clist(), h->x64,
+            vlaunch<k_ir_accumulate_f>(h, k_ir_accumulate, dim3(gv), dim3(WG), 0, nv, h->clist(), h->x64,
                                (const float*)h->x, h->x, 0, tail);
             ++outer;
             const int nwg_ir = nwg_k(h, kIterK);
             if (nwg_ir)
-                hipLaunchKernelGGL((k_ir_residual_blk<B, kIterK>), dim3(nwg_ir), dim3(B * B * B), 0, h->stream,
+                vlaunch<k_ir_residual_blk_f<B, kIterK>>(h, k_ir_residual_blk<B, kIterK>, dim3(nwg_ir), dim3(B * B * B), 0,
                                    (const int*)h->wgtab, (const float*)h->b, (const double*)h->x64,
                                    (const uint16_t*)h->info, h->r, h->x, h->p, h->partials);
-            hipLaunchKernelGGL(k_fin_ir, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
+            vlaunch<k_fin_ir_f>(h, k_fin_ir, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials,
                                nwg_ir, h->prm.refine_tol, h->prm.cg_rel_tol);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
-            HIPCHK(hipStreamSynchronize(h->stream));
+            CHK(hread(h, h->st_host, h->st, sizeof(CgState)));
             if (h->st_host->ir_done || outer >= h->prm.refine_max) break;
             if (h->prm.preconditioner == VARE_PRECOND_MG) enqueue_vcycle(h);
             enqueue_precond_init(h);
@@ -1114,48 +1203,44 @@ template <int B> int stage_solve(vare_flip* h) {
     // every solve.  When refining, the last k_ir_residual_blk computed it from the final FP64 pressure
     // (k_fin_ir stored it); otherwise it is recomputed from the FP32 pressure.
     if (!refine) {
-        hipLaunchKernelGGL((k_true_residual<B, float>), dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+        vlaunch<k_true_residual_f<B, float>>(h, k_true_residual<B, float>, dim3(g), dim3(WG), 0, (int)nc, h->frame(),
                            (const float*)h->b, (const float*)h->x, (const uint16_t*)h->info, h->partials);
-        hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials, (int)g);
+        vlaunch<k_fin_true_f>(h, k_fin_true, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials, (int)g);
     }
-    HIPCHK(hipMemcpyAsync(h->st_host, h->st, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    CHK(hread(h, h->st_host, h->st, sizeof(CgState)));
     tbegin(h, VARE_KERNEL_APPLY);
     const bool apply64 = refine && !(h->prm.flags & VARE_FLAG_FP32_APPLY);
     if (nc && refine && !apply64)       // the refined pressure, rounded once, applied in FP32
-        hipLaunchKernelGGL(k_ir_round, dim3(g), dim3(WG), 0, h->stream, (int)nc, (const double*)h->x64, h->x);
+        vlaunch<k_ir_round_f>(h, k_ir_round, dim3(g), dim3(WG), 0, (int)nc, (const double*)h->x64, h->x);
     if (nc && apply64) {
         // pressure apply in FP64 on the refined pressure; the FP32 field keeps its rounding
-        hipLaunchKernelGGL((k_press_extrap_blk<B, double>), dim3(h->nb), dim3(B * B * B), 0, h->stream, h->frame(),
+        vlaunch<k_press_extrap_blk_f<B, double>>(h, k_press_extrap_blk<B, double>, dim3(h->nb), dim3(B * B * B), 0, h->frame(),
                            (const uint8_t*)h->type, h->x64);
         if (h->ns)
-            hipLaunchKernelGGL((k_press_grad_blk<B, kIterK, double>), dim3(nwg_k(h, kIterK)), dim3(B * B * B), 0,
-                               h->stream, (const int*)h->wgtab, h->frame(), (const uint8_t*)h->type,
+            vlaunch<k_press_grad_blk_f<B, kIterK, double>>(h, k_press_grad_blk<B, kIterK, double>, dim3(nwg_k(h, kIterK)), dim3(B * B * B), 0, (const int*)h->wgtab, h->frame(), (const uint8_t*)h->type,
                                (const double*)h->x64, h->vox_wall, h->cx, h->cy, h->cz);
-        hipLaunchKernelGGL(k_ir_round, dim3(g), dim3(WG), 0, h->stream, (int)nc, (const double*)h->x64, h->x);
+        vlaunch<k_ir_round_f>(h, k_ir_round, dim3(g), dim3(WG), 0, (int)nc, (const double*)h->x64, h->x);
     } else if (nc) {
-        hipLaunchKernelGGL((k_press_extrap_blk<B, float>), dim3(h->nb), dim3(B * B * B), 0, h->stream, h->frame(),
+        vlaunch<k_press_extrap_blk_f<B, float>>(h, k_press_extrap_blk<B, float>, dim3(h->nb), dim3(B * B * B), 0, h->frame(),
                            (const uint8_t*)h->type, h->x);
         if (h->ns)
-            hipLaunchKernelGGL((k_press_grad_blk<B, kIterK, float>), dim3(nwg_k(h, kIterK)), dim3(B * B * B), 0,
-                               h->stream, (const int*)h->wgtab, h->frame(), (const uint8_t*)h->type,
+            vlaunch<k_press_grad_blk_f<B, kIterK, float>>(h, k_press_grad_blk<B, kIterK, float>, dim3(nwg_k(h, kIterK)), dim3(B * B * B), 0, (const int*)h->wgtab, h->frame(), (const uint8_t*)h->type,
                                (const float*)h->x, h->vox_wall, h->cx, h->cy, h->cz);
     }
     tend(h, VARE_KERNEL_APPLY);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(h->step_ev[3], h->stream));
+    HIPCHK(hevent(h, h->step_ev[3]));
     h->stats.div_ratio = 0.0;
     if (nc && (h->prm.flags & VARE_FLAG_REPORT_DIVERGENCE)) {
         // the reference's divergence of the projected velocity on the same classification
-        hipLaunchKernelGGL((k_divergence<B, true>), dim3(g), dim3(WG), 0, h->stream, (int)nc, h->frame(),
+        vlaunch<k_divergence_f<B, true>>(h, k_divergence<B, true>, dim3(g), dim3(WG), 0, (int)nc, h->frame(),
                            (const uint8_t*)h->type, (const float*)h->cx, (const float*)h->cy, (const float*)h->cz,
                            h->vox_wall, vecs(h), h->partials, h->partials2, h->bfluid);
-        hipLaunchKernelGGL(k_fin_divpost, dim3(1), dim3(kFinThreads), 0, h->stream, h->st, (const double*)h->partials,
+        vlaunch<k_fin_divpost_f>(h, k_fin_divpost, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials,
                            (int)g);
         HIPCHK(hipGetLastError());
         CgState* hs = h->st_host;
-        HIPCHK(hipMemcpyAsync(&hs->div_bb, &h->st->div_bb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
+        CHK(hread(h, &hs->div_bb, &h->st->div_bb, sizeof(double)));
         h->stats.div_ratio = hs->ir_bb > 0.0 ? std::sqrt(std::max(hs->div_bb, 0.0) / hs->ir_bb) : 0.0;
     }
     const CgState& s = *h->st_host;
@@ -1178,35 +1263,34 @@ template <int B> int stage_g2p(vare_flip* h, float dt) {
         const bool col = h->colk().any();
         // collision queries and the velocity element type are compile-time choices of the kernel
         if (col && h->vhalf)
-            hipLaunchKernelGGL((k_g2p_tile<B, true, true>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0],
+            vlaunch<k_g2p_tile_f<B, true, true>>(h, k_g2p_tile<B, true, true>, dim3(h->nb + 1), dim3(WG), 0, h->pa[0],
                                h->pa[1], h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
                                g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
         else if (col)
-            hipLaunchKernelGGL((k_g2p_tile<B, true, false>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0],
+            vlaunch<k_g2p_tile_f<B, true, false>>(h, k_g2p_tile<B, true, false>, dim3(h->nb + 1), dim3(WG), 0, h->pa[0],
                                h->pa[1], h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
                                g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
         else if (h->vhalf)
-            hipLaunchKernelGGL((k_g2p_tile<B, false, true>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0],
+            vlaunch<k_g2p_tile_f<B, false, true>>(h, k_g2p_tile<B, false, true>, dim3(h->nb + 1), dim3(WG), 0, h->pa[0],
                                h->pa[1], h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
                                g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
         else
-            hipLaunchKernelGGL((k_g2p_tile<B, false, false>), dim3(h->nb + 1), dim3(WG), 0, h->stream, h->pa[0],
+            vlaunch<k_g2p_tile_f<B, false, false>>(h, k_g2p_tile<B, false, false>, dim3(h->nb + 1), dim3(WG), 0, h->pa[0],
                                h->pa[1], h->pa[2], h->vel(), h->pa[6], (const int*)h->cstart, h->nb, h->frame(), gf,
                                g2p_params(h, dt), h->colk(), h->removed, h->dint + 1);
     } else if (h->np) {
         G2PFields gf{h->cnt, h->cx, h->cy, h->cz, h->vx, h->vy, h->vz};
-        hipLaunchKernelGGL(k_g2p<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, h->pa[0], h->pa[1], h->pa[2],
+        vlaunch<k_g2p_f<B>>(h, k_g2p<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->pa[0], h->pa[1], h->pa[2],
                            h->vel(), h->pa[6], (int)h->np, h->frame(), gf, g2p_params(h, dt), h->colk(), h->removed,
                            h->dint + 1);
     }
     tend(h, VARE_KERNEL_G2P);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(h->dint_host + 1, h->dint + 1, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    CHK(hread(h, h->dint_host + 1, h->dint + 1, sizeof(int)));
     h->nremoved = h->dint_host[1];
     h->last_np_before_remove = h->np;
     if (h->nremoved > 0)
-        hipLaunchKernelGGL(k_removed_logical, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np,
+        vlaunch<k_removed_logical_f>(h, k_removed_logical, dim3(nblocks(h->np)), dim3(WG), 0, (int)h->np,
                            (const uint8_t*)h->removed, (const int*)h->pid, h->removed_log);
     HIPCHK(hipGetLastError());
     return VARE_OK;
@@ -1222,8 +1306,7 @@ int compaction_scans(vare_flip* h, const uint8_t* mask, int n, int newn, int& nm
     CHK(run_scan(h, oh, newn, 0));
     OpSurvivors os{mask, newn, h->src};
     CHK(run_scan(h, os, n - newn, 2));
-    HIPCHK(hipMemcpyAsync(h->dint_host, h->dint, 3 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    CHK(hread(h, h->dint_host, h->dint, 3 * sizeof(int)));
     nmove = h->dint_host[0];
     if (nmove != h->dint_host[2])
         return fail(VARE_EINVAL, "removal compaction mismatch (%d holes, %d survivors)", nmove, h->dint_host[2]);
@@ -1244,9 +1327,9 @@ int stage_remove(vare_flip* h) {
         lg.a[0] = h->pa[7]; lg.a[1] = h->pa[8];
         lg.esz[0] = 4; lg.esz[1] = 4;
         lg.n = 2;
-        hipLaunchKernelGGL(k_remove_move, dim3(nblocks(nmove)), dim3(WG), 0, h->stream, nmove, (const int*)h->hole,
+        vlaunch<k_remove_move_f>(h, k_remove_move, dim3(nblocks(nmove)), dim3(WG), 0, nmove, (const int*)h->hole,
                            (const int*)h->src, lg);
-        hipLaunchKernelGGL(k_remap_build, dim3(nblocks(nmove)), dim3(WG), 0, h->stream, nmove, newn,
+        vlaunch<k_remap_build_f>(h, k_remap_build, dim3(nblocks(nmove)), dim3(WG), 0, nmove, newn,
                            (const int*)h->hole, (const int*)h->src, h->remap);
     }
     // physical order: positions, velocities, adjacency and logical indices
@@ -1258,11 +1341,11 @@ int stage_remove(vare_flip* h) {
         ph.a[7] = h->pid;
         ph.esz[7] = 4;
         ph.n = 8;
-        hipLaunchKernelGGL(k_remove_move, dim3(nblocks(pmove)), dim3(WG), 0, h->stream, pmove, (const int*)h->hole,
+        vlaunch<k_remove_move_f>(h, k_remove_move, dim3(nblocks(pmove)), dim3(WG), 0, pmove, (const int*)h->hole,
                            (const int*)h->src, ph);
     }
     if (nmove && newn)
-        hipLaunchKernelGGL(k_pid_remap, dim3(nblocks(newn)), dim3(WG), 0, h->stream, newn, newn,
+        vlaunch<k_pid_remap_f>(h, k_pid_remap, dim3(nblocks(newn)), dim3(WG), 0, newn, newn,
                            (const int*)h->remap, h->pid);
     HIPCHK(hipGetLastError());
     h->np = (size_t)newn;
@@ -1333,7 +1416,7 @@ template <int B> int run_post_process(vare_flip* h) {
     const unsigned g = rgrid(nc);
     const FrameK fk = h->frame();
     if (nc)
-        hipLaunchKernelGGL(k_splat_gather<B>, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, (const float*)h->pa[0],
+        vlaunch<k_splat_gather_f<B>>(h, k_splat_gather<B>, dim3(nblocks(nc)), dim3(WG), 0, (int)nc, (const float*)h->pa[0],
                            (const float*)h->pa[1], (const float*)h->pa[2], h->vel(), (const int*)h->cstart, fk, h->cnt,
                            h->vx, h->vy, h->vz);
     HIPCHK(hipGetLastError());
@@ -1344,10 +1427,10 @@ template <int B> int run_post_process(vare_flip* h) {
     float* nxt = h->x;
     for (int it = 0; it < h->prm.redist_iteration && nc; ++it) {
         if (tank)
-            hipLaunchKernelGGL((k_redistance<B, true>), dim3(g), dim3(WG), 0, h->stream, (int)nc, fk, (const float*)cur,
+            vlaunch<k_redistance_f<B, true>>(h, k_redistance<B, true>, dim3(g), dim3(WG), 0, (int)nc, fk, (const float*)cur,
                                nxt, h->vox_wall);
         else
-            hipLaunchKernelGGL((k_redistance<B, false>), dim3(g), dim3(WG), 0, h->stream, (int)nc, fk,
+            vlaunch<k_redistance_f<B, false>>(h, k_redistance<B, false>, dim3(g), dim3(WG), 0, (int)nc, fk,
                                (const float*)cur, nxt, h->vox_wall);
         std::swap(cur, nxt);
     }
@@ -1355,22 +1438,22 @@ template <int B> int run_post_process(vare_flip* h) {
         HIPCHK(hipMemcpyAsync(h->cnt, cur, nc * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
     if (nc) {
         const Vec3Field velo{h->vx, h->vy, h->vz}, comp{h->cx, h->cy, h->cz};
-        hipLaunchKernelGGL(k_curl<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, fk, velo, h->cx, h->cy, h->cz);   // :367
+        vlaunch<k_curl_f<B>>(h, k_curl<B>, dim3(g), dim3(WG), 0, (int)nc, fk, velo, h->cx, h->cy, h->cz);   // :367
         const float dis = powf(1.f - h->prm.vort_pres_rate, h->prm.dt);                                       // :373
         if (h->np)
-            hipLaunchKernelGGL(k_particle_vorticity<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np,
+            vlaunch<k_particle_vorticity_f<B>>(h, k_particle_vorticity<B>, dim3(nblocks(h->np)), dim3(WG), 0, (int)h->np,
                                (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2],
                                (const int*)h->pid, fk, comp, h->s[0], h->s[1], h->s[2], dis, h->pa[7]);
-        hipLaunchKernelGGL(k_normal_gradient<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, fk, (const float*)h->cnt,
+        vlaunch<k_normal_gradient_f<B>>(h, k_normal_gradient<B>, dim3(g), dim3(WG), 0, (int)nc, fk, (const float*)h->cnt,
                            kSurfBackground, h->cx, h->cy, h->cz);                                            // :390
-        hipLaunchKernelGGL(k_divergence_vec<B>, dim3(g), dim3(WG), 0, h->stream, (int)nc, fk, comp, h->x);   // :391
+        vlaunch<k_divergence_vec_f<B>>(h, k_divergence_vec<B>, dim3(g), dim3(WG), 0, (int)nc, fk, comp, h->x);   // :391
         if (h->np)
-            hipLaunchKernelGGL(k_particle_curvature<B>, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np,
+            vlaunch<k_particle_curvature_f<B>>(h, k_particle_curvature<B>, dim3(nblocks(h->np)), dim3(WG), 0, (int)h->np,
                                (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2],
                                (const int*)h->pid, fk, (const float*)h->x, h->pa[8]);                      // :393-398
         CHK(fill_u32(h, h->x, 0, nc));                                                                       // :402-404
         CHK(fill_u32(h, h->mass, 0, nc));
-        hipLaunchKernelGGL(k_fill_u32, dim3(nblocks((nc + 3) / 4)), dim3(WG), 0, h->stream, (uint32_t*)h->type, 0u,
+        vlaunch<k_fill_u32_f>(h, k_fill_u32, dim3(nblocks((nc + 3) / 4)), dim3(WG), 0, (uint32_t*)h->type, 0u,
                            (nc + 3) / 4);
     }
     HIPCHK(hipGetLastError());
@@ -1709,7 +1792,7 @@ static int upload_particles(vare_flip* h, size_t off, const float* pos, const fl
             else CHK(fill_u32(h, h->pa[6 + q] + off, 0, n));
         }
     } else {
-        hipLaunchKernelGGL(k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, pos, (void*)(h->pa[0] + off),
+        vlaunch<k_deinterleave_f>(h, k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, (int)n, pos, (void*)(h->pa[0] + off),
                            (void*)(h->pa[1] + off), (void*)(h->pa[2] + off), 0);
         const float* sc[3] = {adj, vort, curv};
         for (int q = 0; q < 3; ++q) {
@@ -1717,12 +1800,12 @@ static int upload_particles(vare_flip* h, size_t off, const float* pos, const fl
             else CHK(fill_u32(h, h->pa[6 + q] + off, 0, n));
         }
     }
-    hipLaunchKernelGGL(k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, mem == VARE_MEM_HOST ? dv : vel,
+    vlaunch<k_deinterleave_f>(h, k_deinterleave, dim3(nblocks(n)), dim3(WG), 0, (int)n, mem == VARE_MEM_HOST ? dv : vel,
                        h->vptr(0, off), h->vptr(1, off), h->vptr(2, off), h->vhalf);
     // new particles are stored at the end in their reference order
-    hipLaunchKernelGGL(k_iota, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, (int)off, h->pid + off);
+    vlaunch<k_iota_f>(h, k_iota, dim3(nblocks(n)), dim3(WG), 0, (int)n, (int)off, h->pid + off);
     if (space == VARE_SPACE_WORLD)
-        hipLaunchKernelGGL(k_to_voxel, dim3(nblocks(n)), dim3(WG), 0, h->stream, (int)n, h->parrays(off), h->t[0],
+        vlaunch<k_to_voxel_f>(h, k_to_voxel, dim3(nblocks(n)), dim3(WG), 0, (int)n, h->parrays(off), h->t[0],
                            h->t[1], h->t[2], h->s[0], h->s[1], h->s[2]);
     const hipError_t e = hipGetLastError();
     const hipError_t e2 = hipStreamSynchronize(h->stream);
@@ -1762,7 +1845,7 @@ int vare_flip_get_particles(vare_flip* h, float* pos, float* vel, float* adj, fl
     if (mem == VARE_MEM_HOST) {
         float* d = nullptr;
         CHK(dalloc(h, &d, 7 * np));
-        hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(),
+        vlaunch<k_export_f>(h, k_export, dim3(nblocks(np)), dim3(WG), 0, (int)np, h->parrays(),
                            (const int*)h->pid, world, h->t[0], h->t[1], h->t[2], h->s[0], h->s[1], h->s[2],
                            pos ? d : nullptr, vel ? d + 3 * np : nullptr, adj ? d + 6 * np : nullptr);
         int rc = hipStreamSynchronize(h->stream) == hipSuccess ? VARE_OK : fail(VARE_EHIP, "particle export failed");
@@ -1778,7 +1861,7 @@ int vare_flip_get_particles(vare_flip* h, float* pos, float* vel, float* adj, fl
         dfree(h, d);
         return rc;
     }
-    hipLaunchKernelGGL(k_export, dim3(nblocks(np)), dim3(WG), 0, h->stream, (int)np, h->parrays(), (const int*)h->pid,
+    vlaunch<k_export_f>(h, k_export, dim3(nblocks(np)), dim3(WG), 0, (int)np, h->parrays(), (const int*)h->pid,
                        world, h->t[0], h->t[1], h->t[2], h->s[0], h->s[1], h->s[2], pos, vel, adj);
     for (int q = 0; q < 2; ++q)
         if (lg[q]) HIPCHK(hipMemcpyAsync(lg[q], h->pa[7 + q], np * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
@@ -1798,10 +1881,10 @@ static int convert_all(vare_flip* h, bool to_voxel) {
     if (!h->np) return VARE_OK;
     const PArrays pa = h->parrays();
     if (to_voxel)
-        hipLaunchKernelGGL(k_to_voxel, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, pa, h->t[0], h->t[1],
+        vlaunch<k_to_voxel_f>(h, k_to_voxel, dim3(nblocks(h->np)), dim3(WG), 0, (int)h->np, pa, h->t[0], h->t[1],
                            h->t[2], h->s[0], h->s[1], h->s[2]);
     else
-        hipLaunchKernelGGL(k_to_world, dim3(nblocks(h->np)), dim3(WG), 0, h->stream, (int)h->np, pa, h->t[0], h->t[1],
+        vlaunch<k_to_world_f>(h, k_to_world, dim3(nblocks(h->np)), dim3(WG), 0, (int)h->np, pa, h->t[0], h->t[1],
                            h->t[2], h->s[0], h->s[1], h->s[2]);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1822,15 +1905,15 @@ int vare_flip_step(vare_flip* h, float dt) {
     if (!h) return fail(VARE_EINVAL, "null handle");
     h->stage_next = 0;
     h->stats.num_removed = 0;
-    HIPCHK(hipEventRecord(h->step_ev[0], h->stream));
+    HIPCHK(hevent(h, h->step_ev[0]));
     h->in_step = 1;
     for (int s = 0; s < 6; ++s) {
         const int rc = dispatch_stage(h, s, dt);
         if (rc != VARE_OK) { h->in_step = 0; return rc; }
     }
     h->in_step = 0;
-    HIPCHK(hipEventRecord(h->step_ev[1], h->stream));
-    HIPCHK(hipEventSynchronize(h->step_ev[1]));
+    HIPCHK(hevent(h, h->step_ev[1]));
+    CHK(hsync(h));
     float ms = 0.f, sms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, h->step_ev[0], h->step_ev[1]));
     HIPCHK(hipEventElapsedTime(&sms, h->step_ev[2], h->step_ev[3]));
@@ -2061,7 +2144,7 @@ int vare_flip_get_cell_field(vare_flip* h, int which, void* out, size_t cap_byte
         CHK(run_scan(h, op, (int)nc, 2));
         unsigned long long* tmp = nullptr;
         CHK(dalloc(h, &tmp, nc));
-        hipLaunchKernelGGL(k_mark_u64, dim3(nblocks(nc)), dim3(WG), 0, h->stream, (int)nc, (const uint8_t*)h->type,
+        vlaunch<k_mark_u64_f>(h, k_mark_u64, dim3(nblocks(nc)), dim3(WG), 0, (int)nc, (const uint8_t*)h->type,
                            (const int*)h->rank, tmp);
         int rc = VARE_OK;
         if (hipStreamSynchronize(h->stream) != hipSuccess || hipMemcpy(out, tmp, nc * 8, hipMemcpyDeviceToHost) != hipSuccess)
@@ -2390,6 +2473,319 @@ int vare_flip_get_kernel_timing(vare_flip* h, double ms[VARE_KERNEL_COUNT], int6
         if (units) units[q] = h->kunits[q];
         if (reset) { h->kms[q] = 0; h->klaunch[q] = 0; h->kunits[q] = 0; }
     }
+    return VARE_OK;
+}
+
+}  // extern "C"
+
+// ---- batched stepping of many tanks (vare_flip_step_batched) ----
+//
+// Each tank's step runs its ordinary host code on a thread of its own, with the handle in record
+// mode: launches, readbacks and event records are appended to the handle's list instead of being
+// issued.  A sync point (a readback of the block count, the solver state, the removal count; the
+// end of the step) is a barrier over the tanks still stepping: when every one of them has reached
+// one, the last to arrive merges all the lists and issues them on one stream.  Consecutive
+// launches of the same kernel by different tanks become one launch of k_batched over all their
+// workgroups; each tank's operations keep their order.  Tanks are independent (no array is shared),
+// so only that order matters, and every tank computes exactly what it computes alone.
+struct BatchRun {
+    std::mutex call;                  // one batched step at a time per device
+    std::mutex m;                     // barrier state below
+    std::condition_variable cv;
+    int active = 0;                   // tanks still inside their step
+    int waiting = 0;                  // of those, waiting at a sync point
+    uint64_t gen = 0;                 // flushes so far
+    std::vector<vare_flip*> members;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // worker threads: one per tank of the largest batch so far
+    std::mutex pm;
+    std::condition_variable pcv, pdone;
+    std::vector<std::thread> threads;
+    uint64_t job = 0;
+    int njob = 0, left = 0;
+    vare_flip* const* jh = nullptr;
+    const float* jdt = nullptr;
+    int* jrc = nullptr;
+    std::vector<std::string> jerr;
+    vare_batch_stats stats{};
+};
+
+namespace {
+
+constexpr size_t kLookahead = 64;     // operations searched ahead when the tanks' next kernels differ
+
+// A merged launch: the operations sel[first, first + n), all of one kernel, or one copy or event.
+struct MergedItem {
+    int kind;
+    size_t first, n;
+};
+
+// Chooses, among the tanks' next operations, the one to issue now: the kernel that the most tanks
+// are at, unless another tank meets it within kLookahead operations (it would then run without that
+// tank); operations no other tank will reach soon go first.  Returns the index of a tank at it.
+size_t pick_head(const std::vector<vare_flip*>& ms, const std::vector<size_t>& head) {
+    size_t first = ms.size();
+    bool uniform = true;
+    for (size_t i = 0; i < ms.size(); ++i) {
+        if (head[i] >= ms[i]->rops.size()) continue;
+        if (first == ms.size()) first = i;
+        else if (!ms[i]->rops[head[i]].same(ms[first]->rops[head[first]])) uniform = false;
+    }
+    if (uniform) return first;
+    size_t best = ms.size();
+    int best_count = -1;
+    bool best_free = false;
+    for (size_t i = 0; i < ms.size(); ++i) {
+        if (head[i] >= ms[i]->rops.size()) continue;
+        const ROp& c = ms[i]->rops[head[i]];
+        bool seen = false;
+        for (size_t j = 0; j < i && !seen; ++j)
+            if (head[j] < ms[j]->rops.size() && ms[j]->rops[head[j]].same(c)) seen = true;
+        if (seen) continue;
+        int count = 0;
+        bool free_ = true;
+        for (size_t j = 0; j < ms.size(); ++j) {
+            if (head[j] >= ms[j]->rops.size()) continue;
+            if (ms[j]->rops[head[j]].same(c)) { ++count; continue; }
+            const size_t end = std::min(ms[j]->rops.size(), head[j] + 1 + kLookahead);
+            for (size_t q = head[j] + 1; q < end && free_; ++q)
+                if (ms[j]->rops[q].same(c)) free_ = false;
+        }
+        if (best == ms.size() || (free_ && !best_free) || (free_ == best_free && count > best_count)) {
+            best = i;
+            best_count = count;
+            best_free = free_;
+        }
+    }
+    return best;
+}
+
+// Merges and issues the recorded operations of the tanks in ms, then waits for them.
+hipError_t run_merged(BatchRun& R, const std::vector<vare_flip*>& ms) {
+    std::vector<size_t> head(ms.size(), 0);
+    std::vector<std::pair<vare_flip*, const ROp*>> sel;
+    std::vector<MergedItem> items;
+    for (;;) {
+        const size_t i0 = pick_head(ms, head);
+        if (i0 >= ms.size()) break;
+        const ROp ref = ms[i0]->rops[head[i0]];
+        if (ref.kind != ROp::kKernel) {
+            items.push_back(MergedItem{ref.kind, sel.size(), 1});
+            sel.emplace_back(ms[i0], &ms[i0]->rops[head[i0]]);
+            ++head[i0];
+            continue;
+        }
+        MergedItem it{ROp::kKernel, sel.size(), 0};
+        for (size_t i = 0; i < ms.size(); ++i) {
+            if (head[i] >= ms[i]->rops.size() || !ms[i]->rops[head[i]].same(ref)) continue;
+            if (it.n == (size_t)ref.max_tanks) {
+                items.push_back(it);
+                it = MergedItem{ROp::kKernel, sel.size(), 0};
+            }
+            sel.emplace_back(ms[i], &ms[i]->rops[head[i]]);
+            ++head[i];
+            ++it.n;
+        }
+        items.push_back(it);
+    }
+    std::vector<unsigned> grid;
+    std::vector<const unsigned char*> packs;
+    for (const MergedItem& it : items) {
+        const ROp& op = *sel[it.first].second;
+        hipError_t e;
+        if (it.kind == ROp::kKernel) {
+            grid.clear();
+            packs.clear();
+            for (size_t q = 0; q < it.n; ++q) {
+                const auto& pr = sel[it.first + q];
+                grid.push_back(pr.second->grid);
+                packs.push_back(pr.first->rpack.data() + pr.second->pack_off);
+            }
+            e = op.issue(op.block, op.shm, R.stream, (int)it.n, grid.data(), packs.data());
+            R.stats.launches++;
+            R.stats.tank_launches += (int64_t)it.n;
+        } else if (it.kind == ROp::kCopy) {
+            e = hipMemcpyAsync(op.dst, op.src, op.bytes, hipMemcpyDeviceToHost, R.stream);
+        } else {
+            e = hipEventRecord(op.ev, R.stream);
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(R.stream);
+}
+
+// With R.m held, when every stepping tank waits (or the last one left with work recorded).
+void flush_locked(BatchRun& R) {
+    std::vector<vare_flip*> ms;
+    for (vare_flip* h : R.members)
+        if (!h->rops.empty()) ms.push_back(h);
+    hipError_t e = hipSuccess;
+    if (!ms.empty()) e = run_merged(R, ms);
+    R.stats.flushes++;
+    for (vare_flip* h : ms) {
+        if (e != hipSuccess) h->rec_rc = VARE_EHIP;
+        h->rops.clear();
+        h->rpack.clear();
+    }
+    for (vare_flip* h : R.members) {
+        for (void* p : h->rfree) (void)hipFree(p);
+        h->rfree.clear();
+    }
+    if (e != hipSuccess) (void)hipGetLastError();
+    R.waiting = 0;
+    R.gen++;
+    R.cv.notify_all();
+}
+
+int batch_sync(vare_flip* h) {
+    BatchRun& R = *h->rec;
+    {
+        std::unique_lock<std::mutex> lk(R.m);
+        R.waiting++;
+        if (R.waiting == R.active) {
+            flush_locked(R);
+        } else {
+            const uint64_t g = R.gen;
+            R.cv.wait(lk, [&] { return R.gen != g; });
+        }
+    }
+    if (h->rec_rc != VARE_OK) return fail(h->rec_rc, "a merged launch of the batched step failed");
+    return VARE_OK;
+}
+
+// The tank's step has returned: the others no longer wait for it.
+void batch_leave(vare_flip* h) {
+    BatchRun& R = *h->rec;
+    std::unique_lock<std::mutex> lk(R.m);
+    R.active--;
+    bool pending = false;
+    for (vare_flip* m : R.members) pending = pending || !m->rops.empty() || !m->rfree.empty();
+    if (R.waiting == R.active && (R.waiting > 0 || pending)) flush_locked(R);
+}
+
+void batch_worker(BatchRun* R, int idx) {
+    uint64_t seen = 0;
+    for (;;) {
+        std::unique_lock<std::mutex> lk(R->pm);
+        R->pcv.wait(lk, [&] { return R->job != seen; });
+        seen = R->job;
+        if (idx >= R->njob) continue;
+        vare_flip* h = R->jh[idx];
+        const float dt = R->jdt[idx];
+        lk.unlock();
+        (void)hipSetDevice(h->device);
+        const int rc = vare_flip_step(h, dt);
+        const char* err = vare_last_error();
+        batch_leave(h);
+        lk.lock();
+        R->jrc[idx] = rc;
+        R->jerr[(size_t)idx] = rc == VARE_OK ? std::string() : std::string(err ? err : "");
+        if (--R->left == 0) R->pdone.notify_all();
+    }
+}
+
+std::mutex g_batch_mu;
+BatchRun* g_batch[64] = {};
+
+// The device's runner (created on first use and kept until the process ends; its threads wait).
+int batch_runner(int dev, BatchRun** out) {
+    if (dev < 0 || dev >= 64) return fail(VARE_EINVAL, "device %d", dev);
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    if (!g_batch[dev]) {
+        BatchRun* R = new BatchRun();
+        R->device = dev;
+        int cur = -1;
+        (void)hipGetDevice(&cur);
+        hipError_t e = hipSetDevice(dev);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking);
+        if (cur >= 0 && cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess) {
+            delete R;
+            return fail(VARE_EHIP, "batch stream: %s", hipGetErrorString(e));
+        }
+        g_batch[dev] = R;
+    }
+    *out = g_batch[dev];
+    return VARE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vare_flip_step_batched(vare_flip* const* hs, int n, const float* dt, int* status) {
+    if (!hs || !dt || n < 1) return fail(VARE_EINVAL, "vare_flip_step_batched: no tanks");
+    for (int i = 0; i < n; ++i) {
+        if (!hs[i]) return fail(VARE_EINVAL, "vare_flip_step_batched: null handle %d", i);
+        if (hs[i]->timing) return fail(VARE_EINVAL, "vare_flip_step_batched: kernel timing is on for tank %d", i);
+        if (hs[i]->rec) return fail(VARE_EINVAL, "vare_flip_step_batched: tank %d is already stepping in a batch", i);
+        if (hs[i]->device != hs[0]->device) return fail(VARE_EINVAL, "vare_flip_step_batched: tanks on different devices");
+        for (int j = 0; j < i; ++j)
+            if (hs[j] == hs[i]) return fail(VARE_EINVAL, "vare_flip_step_batched: tank %d given twice", i);
+    }
+    BatchRun* R = nullptr;
+    CHK(batch_runner(hs[0]->device, &R));
+    std::lock_guard<std::mutex> call(R->call);
+    // the handles' own streams may still hold work (uploads): it comes first
+    for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(hs[i]->stream));
+    std::vector<int> rc((size_t)n, VARE_OK);
+    {
+        std::lock_guard<std::mutex> lk(R->m);
+        R->members.assign(hs, hs + n);
+        R->active = n;
+        R->waiting = 0;
+        for (int i = 0; i < n; ++i) {
+            hs[i]->rec = R;
+            hs[i]->rec_rc = VARE_OK;
+        }
+    }
+    {
+        std::unique_lock<std::mutex> lk(R->pm);
+        while ((int)R->threads.size() < n) {
+            R->threads.emplace_back(batch_worker, R, (int)R->threads.size());
+            R->threads.back().detach();
+        }
+        R->jh = hs;
+        R->jdt = dt;
+        R->jrc = rc.data();
+        R->jerr.assign((size_t)n, std::string());
+        R->njob = n;
+        R->left = n;
+        R->job++;
+        R->pcv.notify_all();
+        R->pdone.wait(lk, [&] { return R->left == 0; });
+        R->njob = 0;
+    }
+    {
+        std::lock_guard<std::mutex> lk(R->m);
+        for (int i = 0; i < n; ++i) {
+            hs[i]->rec = nullptr;
+            for (void* p : hs[i]->rfree) (void)hipFree(p);     // none left after the last flush
+            hs[i]->rfree.clear();
+        }
+        R->members.clear();
+        R->stats.steps++;
+        R->stats.tanks += n;
+    }
+    int first = VARE_OK;
+    for (int i = 0; i < n; ++i) {
+        if (status) status[i] = rc[(size_t)i];
+        if (first == VARE_OK && rc[(size_t)i] != VARE_OK) {
+            first = rc[(size_t)i];
+            g_err = R->jerr[(size_t)i];
+        }
+    }
+    return first;
+}
+
+int vare_batch_get_stats(int device, vare_batch_stats* out, int reset) {
+    if (!out) return fail(VARE_EINVAL, "null output");
+    BatchRun* R = nullptr;
+    CHK(batch_runner(device, &R));
+    std::lock_guard<std::mutex> call(R->call);
+    *out = R->stats;
+    if (reset) std::memset(&R->stats, 0, sizeof R->stats);
     return VARE_OK;
 }
 

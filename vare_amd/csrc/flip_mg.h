@@ -138,39 +138,53 @@ __device__ __forceinline__ bool mg_child_inside(const MgLevelK& L, const MgLevel
 // a Dirichlet boundary.  (Marking a coarse cell fluid when any child is fluid pushes the coarse
 // surface outwards by up to a coarse cell per level and, measured on 256^3 dam-break systems,
 // needs 2.5x the PCG iterations.)
-__global__ void k_mg_flag(MgLevelK L, MgLevelK C, MgFrame F) {
-    const int cc = blockIdx.x * WG + threadIdx.x;
-    if (cc >= C.ncell) return;
-    int nin = 0, nfl = 0;
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        if (!mg_child_inside(L, C, cc, a)) continue;
-        ++nin;
-        const int ch = mg_child(L, C, F, cc, a);
-        if (ch >= 0 && mg_fluid(L.info[ch])) ++nfl;
+struct k_mg_flag_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, MgLevelK L, MgLevelK C, MgFrame F) {
+        const int cc = vbid * WG + threadIdx.x;
+        if (cc >= C.ncell) return;
+        int nin = 0, nfl = 0;
+    #pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            if (!mg_child_inside(L, C, cc, a)) continue;
+            ++nin;
+            const int ch = mg_child(L, C, F, cc, a);
+            if (ch >= 0 && mg_fluid(L.info[ch])) ++nfl;
+        }
+        C.info[cc] = (nfl > 0 && 2 * nfl >= nin) ? kInfoFluid : 0;
     }
-    C.info[cc] = (nfl > 0 && 2 * nfl >= nin) ? kInfoFluid : 0;
+};
+__global__ void k_mg_flag(MgLevelK L, MgLevelK C, MgFrame F) {
+    k_mg_flag_f::run(blockIdx.x, gridDim.x, L, C, F);
 }
 
 // Stencil descriptor of a coarse level from its fluid flags.
-__global__ void k_mg_info(MgLevelK C, MgFrame F) {
-    const int c = blockIdx.x * WG + threadIdx.x;
-    if (c >= C.ncell) return;
-    const uint16_t me = C.info[c];
-    if (!mg_fluid(me)) return;
-    int i, j, k;
-    mg_coords(C, F, c, i, j, k);
-    const bool wall = ((float)(j << C.scale_sh) + 0.5f <= F.wall_level + 1.f);
-    int diag = 6;
-    uint16_t mask = 0;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int n = mg_nbr(C, F, c, q);
-        if (n == -2) diag += wall ? -1 : 1;
-        else if (n >= 0 && mg_fluid(C.info[n])) mask |= (uint16_t)(1u << q);
-        else diag += 1;
+struct k_mg_info_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, MgLevelK C, MgFrame F) {
+        const int c = vbid * WG + threadIdx.x;
+        if (c >= C.ncell) return;
+        const uint16_t me = C.info[c];
+        if (!mg_fluid(me)) return;
+        int i, j, k;
+        mg_coords(C, F, c, i, j, k);
+        const bool wall = ((float)(j << C.scale_sh) + 0.5f <= F.wall_level + 1.f);
+        int diag = 6;
+        uint16_t mask = 0;
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int n = mg_nbr(C, F, c, q);
+            if (n == -2) diag += wall ? -1 : 1;
+            else if (n >= 0 && mg_fluid(C.info[n])) mask |= (uint16_t)(1u << q);
+            else diag += 1;
+        }
+        C.info[c] = (uint16_t)(kInfoFluid | mask | (diag << 8));
     }
-    C.info[c] = (uint16_t)(kInfoFluid | mask | (diag << 8));
+};
+__global__ void k_mg_info(MgLevelK C, MgFrame F) {
+    k_mg_info_f::run(blockIdx.x, gridDim.x, C, F);
 }
 
 // ---- V-cycle kernels of the levels below the block-aligned ones ----
@@ -258,67 +272,85 @@ __device__ __forceinline__ int mg_ldx(int n, int c) { return n >= 0 ? n : c; }
 
 // Two Jacobi sweeps from zero in one pass: zout = z1 + omega2 (r - A z1)/D with z1 = omega D^-1 r.
 template <int KIND>
+struct k_mg_smooth_first2_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, MgLevelK L, MgFrame F, float omega, float omega2, float* __restrict__ zout,
+                                       const int* done) {
+        int dn = *done;
+        const int c = vbid * WG + threadIdx.x;
+        if (c >= L.ncell) return;
+        unsigned in = L.info[c];
+        float rc = L.r[c];
+        int t[6], nb[6];
+        mg_row<KIND>(L, F, c, t);
+        mg_pin7(dn, in, t);
+        mg_pin1(rc);
+        mg_nbrs<KIND>(L, c, t, nb);
+        unsigned ni[6];
+        float nr[6], zn[6];
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int n = mg_ldx(nb[q], c);
+            ni[q] = L.info[n];
+            nr[q] = L.r[n];
+        }
+        mg_pin6(ni);
+        mg_pin6(nr);
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) zn[q] = mg_z0(ni[q], nr[q], omega);
+        const float d = mg_diag(in), wd = mg_wd(in, omega);
+        const float z1 = rc * wd;
+        float az = d * z1;
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
+        const float out = mg_fluid(in) ? z1 + (rc - az) * (omega2 == omega ? wd : mg_wd(in, omega2)) : 0.f;
+        mg_pin(out);
+        if (dn) return;
+        zout[c] = out;
+    }
+};
+template <int KIND>
 __global__ void k_mg_smooth_first2(MgLevelK L, MgFrame F, float omega, float omega2, float* __restrict__ zout,
                                    const int* done) {
-    int dn = *done;
-    const int c = blockIdx.x * WG + threadIdx.x;
-    if (c >= L.ncell) return;
-    unsigned in = L.info[c];
-    float rc = L.r[c];
-    int t[6], nb[6];
-    mg_row<KIND>(L, F, c, t);
-    mg_pin7(dn, in, t);
-    mg_pin1(rc);
-    mg_nbrs<KIND>(L, c, t, nb);
-    unsigned ni[6];
-    float nr[6], zn[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int n = mg_ldx(nb[q], c);
-        ni[q] = L.info[n];
-        nr[q] = L.r[n];
-    }
-    mg_pin6(ni);
-    mg_pin6(nr);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) zn[q] = mg_z0(ni[q], nr[q], omega);
-    const float d = mg_diag(in), wd = mg_wd(in, omega);
-    const float z1 = rc * wd;
-    float az = d * z1;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
-    const float out = mg_fluid(in) ? z1 + (rc - az) * (omega2 == omega ? wd : mg_wd(in, omega2)) : 0.f;
-    mg_pin(out);
-    if (dn) return;
-    zout[c] = out;
+    k_mg_smooth_first2_f<KIND>::run(blockIdx.x, gridDim.x, L, F, omega, omega2, zout, done);
 }
 
 // One Jacobi sweep: zout = z + omega (r - A z)/D.
 template <int KIND>
+struct k_mg_smooth_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, MgLevelK L, MgFrame F, float omega, const float* __restrict__ z, float* __restrict__ zout,
+                                const int* done) {
+        int dn = *done;
+        const int c = vbid * WG + threadIdx.x;
+        if (c >= L.ncell) return;
+        unsigned in = L.info[c];
+        float rc = L.r[c], zc = z[c];
+        int t[6], nb[6];
+        mg_row<KIND>(L, F, c, t);
+        mg_pin7(dn, in, t);
+        mg_pin1(rc);
+        mg_pin1(zc);
+        mg_nbrs<KIND>(L, c, t, nb);
+        float zn[6];
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) zn[q] = z[mg_ldx(nb[q], c)];
+        mg_pin6(zn);
+        float az = mg_diag(in) * zc;
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
+        const float out = mg_fluid(in) ? zc + (rc - az) * mg_wd(in, omega) : 0.f;
+        mg_pin(out);
+        if (dn) return;
+        zout[c] = out;
+    }
+};
+template <int KIND>
 __global__ void k_mg_smooth(MgLevelK L, MgFrame F, float omega, const float* __restrict__ z, float* __restrict__ zout,
                             const int* done) {
-    int dn = *done;
-    const int c = blockIdx.x * WG + threadIdx.x;
-    if (c >= L.ncell) return;
-    unsigned in = L.info[c];
-    float rc = L.r[c], zc = z[c];
-    int t[6], nb[6];
-    mg_row<KIND>(L, F, c, t);
-    mg_pin7(dn, in, t);
-    mg_pin1(rc);
-    mg_pin1(zc);
-    mg_nbrs<KIND>(L, c, t, nb);
-    float zn[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) zn[q] = z[mg_ldx(nb[q], c)];
-    mg_pin6(zn);
-    float az = mg_diag(in) * zc;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
-    const float out = mg_fluid(in) ? zc + (rc - az) * mg_wd(in, omega) : 0.f;
-    mg_pin(out);
-    if (dn) return;
-    zout[c] = out;
+    k_mg_smooth_f<KIND>::run(blockIdx.x, gridDim.x, L, F, omega, z, zout, done);
 }
 
 // Restriction of the residual: rC[cc] = 0.5 * sum over fluid children of (r - A z).  With FIRST the
@@ -331,105 +363,114 @@ __global__ void k_mg_smooth(MgLevelK L, MgFrame F, float omega, const float* __r
 // on one of edge 1 the child's slot and its neighbours' slots are read from the frame's slot table at
 // their coordinates, not through the child's table row, which would add a round trip.
 template <int KIND, bool FIRST>
+struct k_mg_restrict_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, MgLevelK L, MgLevelK C, MgFrame F, float omega, float* __restrict__ zf,
+                                  const int* done) {
+        int dn = *done;
+        const int tid = vbid * WG + threadIdx.x;
+        const int cc = tid >> 3, a = tid & 7;
+        const bool valid = cc < C.ncell;
+        const int ccv = valid ? cc : 0;
+        const unsigned cin = C.info[ccv];
+        int ch, t[6], nb[6];
+        unsigned in0;
+        float rch, zch0;
+        if (KIND == kMgBlk) {                                // mg_child's in-block case
+            const int sh = L.sh, csh = sh - 1, Ec = 1 << csh;
+            const int s = ccv >> (3 * csh);
+            const int l = ccv & ((1 << (3 * csh)) - 1);
+            const int li = l & (Ec - 1), lj = (l >> csh) & (Ec - 1), lk = l >> (2 * csh);
+            ch = (s << (3 * sh)) + ((((2 * lk + (a >> 2)) << sh) + (2 * lj + ((a >> 1) & 1))) << sh) + (2 * li + (a & 1));
+            mg_row<KIND>(L, F, ch, t);
+            in0 = L.info[ch];
+            rch = L.r[ch];
+            zch0 = FIRST ? 0.f : zf[ch];
+            mg_pin7(dn, in0, t);
+            mg_nbrs<KIND>(L, ch, t, nb);
+        } else {
+            const int I = ccv % C.dx, J = (ccv / C.dx) % C.dy, K = ccv / (C.dx * C.dy);
+            const int i = 2 * I + (a & 1), j = 2 * J + ((a >> 1) & 1), k = 2 * K + (a >> 2);
+            const bool inside = i < L.dx && j < L.dy && k < L.dz;
+            if (KIND == kMgDense) {
+                ch = inside ? (k * L.dy + j) * L.dx + i : -1;
+                mg_nbrs<KIND>(L, inside ? ch : 0, t, nb);
+            } else {
+                // slot-table entries at clamped coordinates, all loaded unconditionally, then masked
+                auto slot_raw = [&](int x, int y, int z) -> int {
+                    x = x < 0 ? 0 : (x >= L.dx ? L.dx - 1 : x);
+                    y = y < 0 ? 0 : (y >= L.dy ? L.dy - 1 : y);
+                    z = z < 0 ? 0 : (z >= L.dz ? L.dz - 1 : z);
+                    return F.slot[(z * L.dy + y) * L.dx + x];
+                };
+                auto in_frame = [&](int x, int y, int z) {
+                    return x >= 0 && y >= 0 && z >= 0 && x < L.dx && y < L.dy && z < L.dz;
+                };
+                unsigned cs = (unsigned)slot_raw(i, j, k);
+                t[0] = slot_raw(i - 1, j, k);
+                t[1] = slot_raw(i + 1, j, k);
+                t[2] = slot_raw(i, j - 1, k);
+                t[3] = slot_raw(i, j + 1, k);
+                t[4] = slot_raw(i, j, k - 1);
+                t[5] = slot_raw(i, j, k + 1);
+                mg_pin7(dn, cs, t);
+                nb[0] = in_frame(i - 1, j, k) ? t[0] : -2;        // -2 outside the frame, else slot or -1
+                nb[1] = in_frame(i + 1, j, k) ? t[1] : -2;
+                nb[2] = in_frame(i, j - 1, k) ? t[2] : -2;
+                nb[3] = in_frame(i, j + 1, k) ? t[3] : -2;
+                nb[4] = in_frame(i, j, k - 1) ? t[4] : -2;
+                nb[5] = in_frame(i, j, k + 1) ? t[5] : -2;
+                ch = (inside && (int)cs >= 0) ? (int)cs : -1;
+            }
+            const int cv = ch >= 0 ? ch : 0;
+            in0 = L.info[cv];
+            rch = L.r[cv];
+            zch0 = FIRST ? 0.f : zf[cv];
+        }
+        const int chv = ch >= 0 ? ch : 0;
+        const unsigned in = ch >= 0 ? in0 : 0u;
+        float zch, zn[6];
+        if (FIRST) {
+            unsigned ni[6];
+            float nr[6];
+    #pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int n = mg_ldx(nb[q], chv);
+                ni[q] = L.info[n];
+                nr[q] = L.r[n];
+            }
+            mg_pin6(ni);
+            mg_pin6(nr);
+            zch = rch * mg_wd(in, omega);
+    #pragma unroll
+            for (int q = 0; q < 6; ++q) zn[q] = mg_z0(ni[q], nr[q], omega);
+        } else {
+            zch = zch0;
+    #pragma unroll
+            for (int q = 0; q < 6; ++q) zn[q] = zf[mg_ldx(nb[q], chv)];
+            mg_pin6(zn);
+        }
+        mg_pin1(rch);
+        float az = mg_diag(in) * zch;
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
+        const bool cfl = valid && mg_fluid(cin);
+        const bool fine = valid && mg_fluid(in);
+        float res = (cfl && fine) ? rch - az : 0.f;
+        res += __shfl_xor(res, 1, 64);
+        res += __shfl_xor(res, 2, 64);
+        res += __shfl_xor(res, 4, 64);
+        mg_pin(res);
+        if (dn) return;
+        if (FIRST && fine) zf[ch] = zch;
+        if (valid && a == 0) C.r[cc] = cfl ? 0.5f * res : 0.f;
+    }
+};
+template <int KIND, bool FIRST>
 __global__ void k_mg_restrict(MgLevelK L, MgLevelK C, MgFrame F, float omega, float* __restrict__ zf,
                               const int* done) {
-    int dn = *done;
-    const int tid = blockIdx.x * WG + threadIdx.x;
-    const int cc = tid >> 3, a = tid & 7;
-    const bool valid = cc < C.ncell;
-    const int ccv = valid ? cc : 0;
-    const unsigned cin = C.info[ccv];
-    int ch, t[6], nb[6];
-    unsigned in0;
-    float rch, zch0;
-    if (KIND == kMgBlk) {                                // mg_child's in-block case
-        const int sh = L.sh, csh = sh - 1, Ec = 1 << csh;
-        const int s = ccv >> (3 * csh);
-        const int l = ccv & ((1 << (3 * csh)) - 1);
-        const int li = l & (Ec - 1), lj = (l >> csh) & (Ec - 1), lk = l >> (2 * csh);
-        ch = (s << (3 * sh)) + ((((2 * lk + (a >> 2)) << sh) + (2 * lj + ((a >> 1) & 1))) << sh) + (2 * li + (a & 1));
-        mg_row<KIND>(L, F, ch, t);
-        in0 = L.info[ch];
-        rch = L.r[ch];
-        zch0 = FIRST ? 0.f : zf[ch];
-        mg_pin7(dn, in0, t);
-        mg_nbrs<KIND>(L, ch, t, nb);
-    } else {
-        const int I = ccv % C.dx, J = (ccv / C.dx) % C.dy, K = ccv / (C.dx * C.dy);
-        const int i = 2 * I + (a & 1), j = 2 * J + ((a >> 1) & 1), k = 2 * K + (a >> 2);
-        const bool inside = i < L.dx && j < L.dy && k < L.dz;
-        if (KIND == kMgDense) {
-            ch = inside ? (k * L.dy + j) * L.dx + i : -1;
-            mg_nbrs<KIND>(L, inside ? ch : 0, t, nb);
-        } else {
-            // slot-table entries at clamped coordinates, all loaded unconditionally, then masked
-            auto slot_raw = [&](int x, int y, int z) -> int {
-                x = x < 0 ? 0 : (x >= L.dx ? L.dx - 1 : x);
-                y = y < 0 ? 0 : (y >= L.dy ? L.dy - 1 : y);
-                z = z < 0 ? 0 : (z >= L.dz ? L.dz - 1 : z);
-                return F.slot[(z * L.dy + y) * L.dx + x];
-            };
-            auto in_frame = [&](int x, int y, int z) {
-                return x >= 0 && y >= 0 && z >= 0 && x < L.dx && y < L.dy && z < L.dz;
-            };
-            unsigned cs = (unsigned)slot_raw(i, j, k);
-            t[0] = slot_raw(i - 1, j, k);
-            t[1] = slot_raw(i + 1, j, k);
-            t[2] = slot_raw(i, j - 1, k);
-            t[3] = slot_raw(i, j + 1, k);
-            t[4] = slot_raw(i, j, k - 1);
-            t[5] = slot_raw(i, j, k + 1);
-            mg_pin7(dn, cs, t);
-            nb[0] = in_frame(i - 1, j, k) ? t[0] : -2;        // -2 outside the frame, else slot or -1
-            nb[1] = in_frame(i + 1, j, k) ? t[1] : -2;
-            nb[2] = in_frame(i, j - 1, k) ? t[2] : -2;
-            nb[3] = in_frame(i, j + 1, k) ? t[3] : -2;
-            nb[4] = in_frame(i, j, k - 1) ? t[4] : -2;
-            nb[5] = in_frame(i, j, k + 1) ? t[5] : -2;
-            ch = (inside && (int)cs >= 0) ? (int)cs : -1;
-        }
-        const int cv = ch >= 0 ? ch : 0;
-        in0 = L.info[cv];
-        rch = L.r[cv];
-        zch0 = FIRST ? 0.f : zf[cv];
-    }
-    const int chv = ch >= 0 ? ch : 0;
-    const unsigned in = ch >= 0 ? in0 : 0u;
-    float zch, zn[6];
-    if (FIRST) {
-        unsigned ni[6];
-        float nr[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const int n = mg_ldx(nb[q], chv);
-            ni[q] = L.info[n];
-            nr[q] = L.r[n];
-        }
-        mg_pin6(ni);
-        mg_pin6(nr);
-        zch = rch * mg_wd(in, omega);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) zn[q] = mg_z0(ni[q], nr[q], omega);
-    } else {
-        zch = zch0;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) zn[q] = zf[mg_ldx(nb[q], chv)];
-        mg_pin6(zn);
-    }
-    mg_pin1(rch);
-    float az = mg_diag(in) * zch;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
-    const bool cfl = valid && mg_fluid(cin);
-    const bool fine = valid && mg_fluid(in);
-    float res = (cfl && fine) ? rch - az : 0.f;
-    res += __shfl_xor(res, 1, 64);
-    res += __shfl_xor(res, 2, 64);
-    res += __shfl_xor(res, 4, 64);
-    mg_pin(res);
-    if (dn) return;
-    if (FIRST && fine) zf[ch] = zch;
-    if (valid && a == 0) C.r[cc] = cfl ? 0.5f * res : 0.f;
+    k_mg_restrict_f<KIND, FIRST>::run(blockIdx.x, gridDim.x, L, C, F, omega, zf, done);
 }
 
 // Prolongation (piecewise constant) fused with one Jacobi sweep:
@@ -437,69 +478,78 @@ __global__ void k_mg_restrict(MgLevelK L, MgLevelK C, MgFrame F, float omega, fl
 // parent is arithmetic in its index; otherwise (edge 1, dense) the parents of the cell and of its
 // neighbours follow from the cell's coordinates, so no parent needs its neighbour's index first.
 template <int KIND>
+struct k_mg_prolong_smooth_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, MgLevelK L, MgLevelK C, MgFrame F, float omega, const float* __restrict__ z,
+                                        const float* __restrict__ zc, float* __restrict__ zout, const int* done) {
+        int dn = *done;
+        const int c = vbid * WG + threadIdx.x;
+        if (c >= L.ncell) return;
+        constexpr bool inblk = KIND == kMgBlk;
+        auto parent_ijk = [&](int x, int y, int w) { return ((w >> 1) * C.dy + (y >> 1)) * C.dx + (x >> 1); };
+        auto parent_blk = [&](int x) {                       // mg_parent's in-block case
+            const int sh = L.sh, csh = sh - 1, E = 1 << sh;
+            const int s = x >> (3 * sh);
+            const int l = x & ((1 << (3 * sh)) - 1);
+            const int li = (l & (E - 1)) >> 1, lj = ((l >> sh) & (E - 1)) >> 1, lk = (l >> (2 * sh)) >> 1;
+            return (s << (3 * csh)) + (((lk << csh) + lj) << csh) + li;
+        };
+        unsigned in = L.info[c];
+        float rc = L.r[c], zcc = z[c];
+        int i = 0, j = 0, k = 0;
+        float zpc = 0.f;
+        if (KIND == kMgEdge1) {
+            const int tc = F.coord[c];
+            i = tc % L.dx; j = (tc / L.dx) % L.dy; k = tc / (L.dx * L.dy);
+        } else if (KIND == kMgDense) {
+            i = c % L.dx; j = (c / L.dx) % L.dy; k = c / (L.dx * L.dy);
+            zpc = zc[parent_ijk(i, j, k)];
+        } else {
+            zpc = zc[parent_blk(c)];
+        }
+        int t[6], nb[6];
+        mg_row<KIND>(L, F, c, t);
+        mg_pin7(dn, in, t);
+        mg_pin1(rc);
+        mg_pin1(zcc);
+        if (KIND == kMgEdge1) zpc = zc[parent_ijk(i, j, k)];
+        mg_nbrs<KIND>(L, c, t, nb);
+        float za[6], zb[6], zn[6];
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int n = mg_ldx(nb[q], c);
+            int pn;
+            if (inblk) {
+                pn = parent_blk(n);
+            } else {
+                const int dir = (q & 1) ? 1 : -1;
+                const bool use = nb[q] >= 0;                 // else the cell itself, like n
+                pn = parent_ijk(i + (use && q < 2 ? dir : 0), j + (use && (q >> 1) == 1 ? dir : 0),
+                                k + (use && q >= 4 ? dir : 0));
+            }
+            za[q] = z[n];
+            zb[q] = zc[pn];
+        }
+        mg_pin6(za);
+        mg_pin6(zb);
+        mg_pin1(zpc);
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) zn[q] = za[q] + zb[q];
+        const float zp = zcc + zpc;
+        float az = mg_diag(in) * zp;
+    #pragma unroll
+        for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
+        const float out = mg_fluid(in) ? zp + (rc - az) * mg_wd(in, omega) : 0.f;
+        mg_pin(out);
+        if (dn) return;
+        zout[c] = out;
+    }
+};
+template <int KIND>
 __global__ void k_mg_prolong_smooth(MgLevelK L, MgLevelK C, MgFrame F, float omega, const float* __restrict__ z,
                                     const float* __restrict__ zc, float* __restrict__ zout, const int* done) {
-    int dn = *done;
-    const int c = blockIdx.x * WG + threadIdx.x;
-    if (c >= L.ncell) return;
-    constexpr bool inblk = KIND == kMgBlk;
-    auto parent_ijk = [&](int x, int y, int w) { return ((w >> 1) * C.dy + (y >> 1)) * C.dx + (x >> 1); };
-    auto parent_blk = [&](int x) {                       // mg_parent's in-block case
-        const int sh = L.sh, csh = sh - 1, E = 1 << sh;
-        const int s = x >> (3 * sh);
-        const int l = x & ((1 << (3 * sh)) - 1);
-        const int li = (l & (E - 1)) >> 1, lj = ((l >> sh) & (E - 1)) >> 1, lk = (l >> (2 * sh)) >> 1;
-        return (s << (3 * csh)) + (((lk << csh) + lj) << csh) + li;
-    };
-    unsigned in = L.info[c];
-    float rc = L.r[c], zcc = z[c];
-    int i = 0, j = 0, k = 0;
-    float zpc = 0.f;
-    if (KIND == kMgEdge1) {
-        const int tc = F.coord[c];
-        i = tc % L.dx; j = (tc / L.dx) % L.dy; k = tc / (L.dx * L.dy);
-    } else if (KIND == kMgDense) {
-        i = c % L.dx; j = (c / L.dx) % L.dy; k = c / (L.dx * L.dy);
-        zpc = zc[parent_ijk(i, j, k)];
-    } else {
-        zpc = zc[parent_blk(c)];
-    }
-    int t[6], nb[6];
-    mg_row<KIND>(L, F, c, t);
-    mg_pin7(dn, in, t);
-    mg_pin1(rc);
-    mg_pin1(zcc);
-    if (KIND == kMgEdge1) zpc = zc[parent_ijk(i, j, k)];
-    mg_nbrs<KIND>(L, c, t, nb);
-    float za[6], zb[6], zn[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int n = mg_ldx(nb[q], c);
-        int pn;
-        if (inblk) {
-            pn = parent_blk(n);
-        } else {
-            const int dir = (q & 1) ? 1 : -1;
-            const bool use = nb[q] >= 0;                 // else the cell itself, like n
-            pn = parent_ijk(i + (use && q < 2 ? dir : 0), j + (use && (q >> 1) == 1 ? dir : 0),
-                            k + (use && q >= 4 ? dir : 0));
-        }
-        za[q] = z[n];
-        zb[q] = zc[pn];
-    }
-    mg_pin6(za);
-    mg_pin6(zb);
-    mg_pin1(zpc);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) zn[q] = za[q] + zb[q];
-    const float zp = zcc + zpc;
-    float az = mg_diag(in) * zp;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) az -= (in & (1u << q)) ? zn[q] : 0.f;
-    const float out = mg_fluid(in) ? zp + (rc - az) * mg_wd(in, omega) : 0.f;
-    mg_pin(out);
-    if (dn) return;
-    zout[c] = out;
+    k_mg_prolong_smooth_f<KIND>::run(blockIdx.x, gridDim.x, L, C, F, omega, z, zc, zout, done);
 }
 
 // ---- bottom of the hierarchy: the last levels, at most kMgBottomCells cells, in one workgroup ----
@@ -586,203 +636,211 @@ __device__ __forceinline__ float bottom_sweep(unsigned in, float r, const float*
 // Workgroup 0 runs the bottom V-cycle; workgroups 1.. (when xt.x is set) apply the CG's x += alpha p
 // meanwhile (see XTerm).  omega weighs the coarsest grid's sweeps, wa and wb the first and second
 // sweep of each smoothing pair on the levels above it.
-__global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, float omega, float wa, float wb, int nu, int coarse_sweeps,
-                                                                const int* done, XTerm xt) {
-    if (*done) return;
-    __shared__ float sr[kMgBottomCells];
-    __shared__ float sz[kMgBottomCells];
-    __shared__ float st[kMgBottomCells];
-    __shared__ uint16_t si[kMgBottomCells];
-    const int tid = threadIdx.x;
-    constexpr int T = kMgBottomThreads;
-    if (blockIdx.x > 0) {
-        const float a = *xt.alpha;
-        const int bpp = T >> xt.sh3;                   // blocks per pass
-        const int l = tid & ((1 << xt.sh3) - 1);
-        const int b0 = (blockIdx.x - 1) * bpp * kXUpdPasses + (tid >> xt.sh3);
-        int c[kXUpdPasses];
-        float xv[kXUpdPasses], pv[kXUpdPasses];
-#pragma unroll
-        for (int u = 0; u < kXUpdPasses; ++u) {
-            const int b = __builtin_amdgcn_readfirstlane(b0 + u * bpp);     // uniform: whole waves per block
-            c[u] = b < xt.nblk ? (xt.blk[b] << xt.sh3) | l : -1;
-        }
-#pragma unroll
-        for (int u = 0; u < kXUpdPasses; ++u) {
-            const int cs = c[u] >= 0 ? c[u] : 0;
-            xv[u] = xt.x[cs];
-            pv[u] = xt.p[cs];
-        }
-#pragma unroll
-        for (int u = 0; u < kXUpdPasses; ++u)
-            if (c[u] >= 0) xt.x[c[u]] = xv[u] + (a * pv[u]);
-        return;
-    }
-    constexpr int kPer = (kMgBottomCells + T - 1) / T;      // cells per thread of any one level, at most
-    const int L = bt.nlev;
-    // load: a level has at most kPer cells per thread; their indices, then all of their descriptors
-    // and right-hand sides, are loaded in unrolled rounds (safe index 0 where there is no cell), so a
-    // level costs one round trip to memory, two for a sparse level's slot lookup, not one per cell
-    for (int l = 0; l < L; ++l) {
-        const int n = bt.dx[l] * bt.dy[l] * bt.dz[l];
-        int g[kPer];
-        uint16_t iv[kPer];
-        float rv[kPer];
-        const int sh = bt.sh[l];
-        if (sh < 0) {
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) g[k] = tid + k * T < n ? tid + k * T : -1;
-        } else {                                         // bottom_index with the slot loads batched
-            const int dx = bt.dx[l], dy = bt.dy[l], m = (1 << sh) - 1;
-            int raw[kPer];
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-                const int c = tid + k * T < n ? tid + k * T : 0;
-                const int i = c % dx, j = (c / dx) % dy, kk = c / (dx * dy);
-                raw[k] = bt.slot[((kk >> sh) * bt.fy + (j >> sh)) * bt.fx + (i >> sh)];
+struct k_mg_bottom_f {
+    static constexpr int kLB = kMgBottomThreads;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, MgBottom bt, float omega, float wa, float wb, int nu, int coarse_sweeps,
+                                                                    const int* done, XTerm xt) {
+        if (*done) return;
+        __shared__ float sr[kMgBottomCells];
+        __shared__ float sz[kMgBottomCells];
+        __shared__ float st[kMgBottomCells];
+        __shared__ uint16_t si[kMgBottomCells];
+        const int tid = threadIdx.x;
+        constexpr int T = kMgBottomThreads;
+        if (vbid > 0) {
+            const float a = *xt.alpha;
+            const int bpp = T >> xt.sh3;                   // blocks per pass
+            const int l = tid & ((1 << xt.sh3) - 1);
+            const int b0 = (vbid - 1) * bpp * kXUpdPasses + (tid >> xt.sh3);
+            int c[kXUpdPasses];
+            float xv[kXUpdPasses], pv[kXUpdPasses];
+    #pragma unroll
+            for (int u = 0; u < kXUpdPasses; ++u) {
+                const int b = __builtin_amdgcn_readfirstlane(b0 + u * bpp);     // uniform: whole waves per block
+                c[u] = b < xt.nblk ? (xt.blk[b] << xt.sh3) | l : -1;
             }
-#pragma unroll
+    #pragma unroll
+            for (int u = 0; u < kXUpdPasses; ++u) {
+                const int cs = c[u] >= 0 ? c[u] : 0;
+                xv[u] = xt.x[cs];
+                pv[u] = xt.p[cs];
+            }
+    #pragma unroll
+            for (int u = 0; u < kXUpdPasses; ++u)
+                if (c[u] >= 0) xt.x[c[u]] = xv[u] + (a * pv[u]);
+            return;
+        }
+        constexpr int kPer = (kMgBottomCells + T - 1) / T;      // cells per thread of any one level, at most
+        const int L = bt.nlev;
+        // load: a level has at most kPer cells per thread; their indices, then all of their descriptors
+        // and right-hand sides, are loaded in unrolled rounds (safe index 0 where there is no cell), so a
+        // level costs one round trip to memory, two for a sparse level's slot lookup, not one per cell
+        for (int l = 0; l < L; ++l) {
+            const int n = bt.dx[l] * bt.dy[l] * bt.dz[l];
+            int g[kPer];
+            uint16_t iv[kPer];
+            float rv[kPer];
+            const int sh = bt.sh[l];
+            if (sh < 0) {
+    #pragma unroll
+                for (int k = 0; k < kPer; ++k) g[k] = tid + k * T < n ? tid + k * T : -1;
+            } else {                                         // bottom_index with the slot loads batched
+                const int dx = bt.dx[l], dy = bt.dy[l], m = (1 << sh) - 1;
+                int raw[kPer];
+    #pragma unroll
+                for (int k = 0; k < kPer; ++k) {
+                    const int c = tid + k * T < n ? tid + k * T : 0;
+                    const int i = c % dx, j = (c / dx) % dy, kk = c / (dx * dy);
+                    raw[k] = bt.slot[((kk >> sh) * bt.fy + (j >> sh)) * bt.fx + (i >> sh)];
+                }
+    #pragma unroll
+                for (int k = 0; k < kPer; ++k) {
+                    const int c = tid + k * T;
+                    const int i = c % dx, j = (c / dx) % dy, kk = c / (dx * dy);
+                    g[k] = (c < n && raw[k] >= 0) ? (raw[k] << (3 * sh)) + ((((kk & m) << sh) + (j & m)) << sh) + (i & m) : -1;
+                }
+            }
+    #pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int gs = g[k] >= 0 ? g[k] : 0;
+                iv[k] = bt.info[l][gs];
+                rv[k] = l == 0 ? bt.r0[gs] : 0.f;
+            }
+    #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const int c = tid + k * T;
-                const int i = c % dx, j = (c / dx) % dy, kk = c / (dx * dy);
-                g[k] = (c < n && raw[k] >= 0) ? (raw[k] << (3 * sh)) + ((((kk & m) << sh) + (j & m)) << sh) + (i & m) : -1;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int gs = g[k] >= 0 ? g[k] : 0;
-            iv[k] = bt.info[l][gs];
-            rv[k] = l == 0 ? bt.r0[gs] : 0.f;
-        }
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int c = tid + k * T;
-            if (c < n) {
-                const uint16_t in = g[k] >= 0 ? iv[k] : (uint16_t)0;
-                si[bt.off[l] + c] = in;
-                if (l == 0) sr[c] = mg_fluid(in) ? rv[k] : 0.f;
-            }
-        }
-    }
-    __syncthreads();
-    // down: the pre-smoothed z of level l goes to sz + off[l]
-    for (int l = 0; l + 1 < L; ++l) {
-        const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
-        const float* r = sr + o;
-        float* z = sz + o;
-        const uint16_t* in = si + o;
-        // nu = 1: z = omega r / D; nu = 2: a second sweep from that z, whose neighbour values are formed
-        // on the fly (k_mg_smooth_first2)
-        for (int c = tid; c < n; c += T) {
-            const unsigned ic = in[c];
-            const float wd = mg_wd(ic, wa);
-            const float z1 = r[c] * wd;
-            float out = z1;
-            if (nu == 2) {
-                int nb[6];
-                bottom_nbrs(ic, c, sy, szz, nb);
-                float v[6];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) v[q] = r[nb[q]] * mg_wd(in[nb[q]], wa);
-                float az = mg_diag(ic) * z1;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) az -= (ic & (1u << q)) ? v[q] : 0.f;
-                out = z1 + (r[c] - az) * (wb == wa ? wd : mg_wd(ic, wb));
-            }
-            z[c] = mg_fluid(ic) ? out : 0.f;
-        }
-        __syncthreads();
-        // restriction: rC = 0.5 * sum of the fluid children's residuals r - A z, children summed as
-        // ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) like k_mg_restrict's butterflies
-        const int oc = bt.off[l + 1], cdx = bt.dx[l + 1], cdy = bt.dy[l + 1], nc = cdx * cdy * bt.dz[l + 1];
-        for (int cc = tid; cc < nc; cc += T) {
-            float acc = 0.f;
-            if (mg_fluid(si[oc + cc])) {
-                const int I = cc % cdx, J = (cc / cdx) % cdy, K = cc / (cdx * cdy);
-                float ch[8];
-#pragma unroll
-                for (int a = 0; a < 8; ++a) {
-                    const int i = 2 * I + (a & 1), j = 2 * J + ((a >> 1) & 1), k = 2 * K + (a >> 2);
-                    const bool inside = i < dx && j < dy && k < bt.dz[l];
-                    const int c = inside ? (k * dy + j) * dx + i : 0;
-                    const unsigned ic = inside ? in[c] : 0u;
-                    const float res = r[c] - bottom_az(ic, z, c, sy, szz);
-                    ch[a] = mg_fluid(ic) ? res : 0.f;
+                if (c < n) {
+                    const uint16_t in = g[k] >= 0 ? iv[k] : (uint16_t)0;
+                    si[bt.off[l] + c] = in;
+                    if (l == 0) sr[c] = mg_fluid(in) ? rv[k] : 0.f;
                 }
-                acc = 0.5f * (((ch[0] + ch[1]) + (ch[2] + ch[3])) + ((ch[4] + ch[5]) + (ch[6] + ch[7])));
             }
-            sr[oc + cc] = acc;
         }
         __syncthreads();
-    }
-    // coarsest level: `coarse_sweeps` Jacobi sweeps from zero (the first is omega r / D), alternating
-    // between sz and st; zfin = where the last one is
-    float* zfin;
-    {
-        const int l = L - 1;
-        const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
-        const float* r = sr + o;
-        const uint16_t* in = si + o;
-        float* cur = sz + o;
-        float* nxt = st + o;
-        for (int c = tid; c < n; c += T) cur[c] = (coarse_sweeps > 0 && mg_fluid(in[c])) ? r[c] * mg_wd(in[c], omega) : 0.f;
-        for (int s = 1; s < coarse_sweeps; ++s) {
+        // down: the pre-smoothed z of level l goes to sz + off[l]
+        for (int l = 0; l + 1 < L; ++l) {
+            const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
+            const float* r = sr + o;
+            float* z = sz + o;
+            const uint16_t* in = si + o;
+            // nu = 1: z = omega r / D; nu = 2: a second sweep from that z, whose neighbour values are formed
+            // on the fly (k_mg_smooth_first2)
+            for (int c = tid; c < n; c += T) {
+                const unsigned ic = in[c];
+                const float wd = mg_wd(ic, wa);
+                const float z1 = r[c] * wd;
+                float out = z1;
+                if (nu == 2) {
+                    int nb[6];
+                    bottom_nbrs(ic, c, sy, szz, nb);
+                    float v[6];
+    #pragma unroll
+                    for (int q = 0; q < 6; ++q) v[q] = r[nb[q]] * mg_wd(in[nb[q]], wa);
+                    float az = mg_diag(ic) * z1;
+    #pragma unroll
+                    for (int q = 0; q < 6; ++q) az -= (ic & (1u << q)) ? v[q] : 0.f;
+                    out = z1 + (r[c] - az) * (wb == wa ? wd : mg_wd(ic, wb));
+                }
+                z[c] = mg_fluid(ic) ? out : 0.f;
+            }
             __syncthreads();
-            for (int c = tid; c < n; c += T) nxt[c] = bottom_sweep(in[c], r[c], cur, c, sy, szz, omega);
-            float* tmp = cur; cur = nxt; nxt = tmp;
-        }
-        zfin = ((coarse_sweeps < 1 ? 1 : coarse_sweeps) & 1) ? sz + o : st + o;
-        __syncthreads();
-    }
-    // up: level l's post-smoothed z ends in st (nu = 1) or sz (nu = 2) + off[l]
-    for (int l = L - 2; l >= 0; --l) {
-        const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
-        const int cdx = bt.dx[l + 1], cdy = bt.dy[l + 1];
-        const float* r = sr + o;
-        float* z = sz + o;
-        float* zt = st + o;
-        const uint16_t* in = si + o;
-        const float* zc = zfin;
-        // prolongation fused with the first post-sweep: zp = z + zC[parent] on the cell and its fluid
-        // neighbours, formed on the fly (k_mg_prolong_smooth), into zt
-        for (int c = tid; c < n; c += T) {
-            const unsigned ic = in[c];
-            const int i = c % dx, j = (c / dx) % dy, k = c / (dx * dy);
-            auto par = [&](int x, int y, int w) { return ((w >> 1) * cdy + (y >> 1)) * cdx + (x >> 1); };
-            int nb[6], pn[6];
-            bottom_nbrs(ic, c, sy, szz, nb);
-            pn[0] = par(i - ((ic & 1u) ? 1 : 0), j, k);
-            pn[1] = par(i + ((ic & 2u) ? 1 : 0), j, k);
-            pn[2] = par(i, j - ((ic & 4u) ? 1 : 0), k);
-            pn[3] = par(i, j + ((ic & 8u) ? 1 : 0), k);
-            pn[4] = par(i, j, k - ((ic & 16u) ? 1 : 0));
-            pn[5] = par(i, j, k + ((ic & 32u) ? 1 : 0));
-            float v[6];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) v[q] = z[nb[q]] + zc[pn[q]];
-            const float zpc = z[c] + zc[par(i, j, k)];
-            float az = mg_diag(ic) * zpc;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) az -= (ic & (1u << q)) ? v[q] : 0.f;
-            zt[c] = mg_fluid(ic) ? zpc + (r[c] - az) * mg_wd(ic, wa) : 0.f;
-        }
-        __syncthreads();
-        zfin = zt;
-        if (nu == 2) {                                   // second post-sweep (k_mg_smooth), into z
-            for (int c = tid; c < n; c += T) z[c] = bottom_sweep(in[c], r[c], zt, c, sy, szz, wb);
+            // restriction: rC = 0.5 * sum of the fluid children's residuals r - A z, children summed as
+            // ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) like k_mg_restrict's butterflies
+            const int oc = bt.off[l + 1], cdx = bt.dx[l + 1], cdy = bt.dy[l + 1], nc = cdx * cdy * bt.dz[l + 1];
+            for (int cc = tid; cc < nc; cc += T) {
+                float acc = 0.f;
+                if (mg_fluid(si[oc + cc])) {
+                    const int I = cc % cdx, J = (cc / cdx) % cdy, K = cc / (cdx * cdy);
+                    float ch[8];
+    #pragma unroll
+                    for (int a = 0; a < 8; ++a) {
+                        const int i = 2 * I + (a & 1), j = 2 * J + ((a >> 1) & 1), k = 2 * K + (a >> 2);
+                        const bool inside = i < dx && j < dy && k < bt.dz[l];
+                        const int c = inside ? (k * dy + j) * dx + i : 0;
+                        const unsigned ic = inside ? in[c] : 0u;
+                        const float res = r[c] - bottom_az(ic, z, c, sy, szz);
+                        ch[a] = mg_fluid(ic) ? res : 0.f;
+                    }
+                    acc = 0.5f * (((ch[0] + ch[1]) + (ch[2] + ch[3])) + ((ch[4] + ch[5]) + (ch[6] + ch[7])));
+                }
+                sr[oc + cc] = acc;
+            }
             __syncthreads();
-            zfin = z;
+        }
+        // coarsest level: `coarse_sweeps` Jacobi sweeps from zero (the first is omega r / D), alternating
+        // between sz and st; zfin = where the last one is
+        float* zfin;
+        {
+            const int l = L - 1;
+            const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
+            const float* r = sr + o;
+            const uint16_t* in = si + o;
+            float* cur = sz + o;
+            float* nxt = st + o;
+            for (int c = tid; c < n; c += T) cur[c] = (coarse_sweeps > 0 && mg_fluid(in[c])) ? r[c] * mg_wd(in[c], omega) : 0.f;
+            for (int s = 1; s < coarse_sweeps; ++s) {
+                __syncthreads();
+                for (int c = tid; c < n; c += T) nxt[c] = bottom_sweep(in[c], r[c], cur, c, sy, szz, omega);
+                float* tmp = cur; cur = nxt; nxt = tmp;
+            }
+            zfin = ((coarse_sweeps < 1 ? 1 : coarse_sweeps) & 1) ? sz + o : st + o;
+            __syncthreads();
+        }
+        // up: level l's post-smoothed z ends in st (nu = 1) or sz (nu = 2) + off[l]
+        for (int l = L - 2; l >= 0; --l) {
+            const int o = bt.off[l], dx = bt.dx[l], dy = bt.dy[l], n = dx * dy * bt.dz[l], sy = dx, szz = dx * dy;
+            const int cdx = bt.dx[l + 1], cdy = bt.dy[l + 1];
+            const float* r = sr + o;
+            float* z = sz + o;
+            float* zt = st + o;
+            const uint16_t* in = si + o;
+            const float* zc = zfin;
+            // prolongation fused with the first post-sweep: zp = z + zC[parent] on the cell and its fluid
+            // neighbours, formed on the fly (k_mg_prolong_smooth), into zt
+            for (int c = tid; c < n; c += T) {
+                const unsigned ic = in[c];
+                const int i = c % dx, j = (c / dx) % dy, k = c / (dx * dy);
+                auto par = [&](int x, int y, int w) { return ((w >> 1) * cdy + (y >> 1)) * cdx + (x >> 1); };
+                int nb[6], pn[6];
+                bottom_nbrs(ic, c, sy, szz, nb);
+                pn[0] = par(i - ((ic & 1u) ? 1 : 0), j, k);
+                pn[1] = par(i + ((ic & 2u) ? 1 : 0), j, k);
+                pn[2] = par(i, j - ((ic & 4u) ? 1 : 0), k);
+                pn[3] = par(i, j + ((ic & 8u) ? 1 : 0), k);
+                pn[4] = par(i, j, k - ((ic & 16u) ? 1 : 0));
+                pn[5] = par(i, j, k + ((ic & 32u) ? 1 : 0));
+                float v[6];
+    #pragma unroll
+                for (int q = 0; q < 6; ++q) v[q] = z[nb[q]] + zc[pn[q]];
+                const float zpc = z[c] + zc[par(i, j, k)];
+                float az = mg_diag(ic) * zpc;
+    #pragma unroll
+                for (int q = 0; q < 6; ++q) az -= (ic & (1u << q)) ? v[q] : 0.f;
+                zt[c] = mg_fluid(ic) ? zpc + (r[c] - az) * mg_wd(ic, wa) : 0.f;
+            }
+            __syncthreads();
+            zfin = zt;
+            if (nu == 2) {                                   // second post-sweep (k_mg_smooth), into z
+                for (int c = tid; c < n; c += T) z[c] = bottom_sweep(in[c], r[c], zt, c, sy, szz, wb);
+                __syncthreads();
+                zfin = z;
+            }
+        }
+        {
+            const int n = bt.dx[0] * bt.dy[0] * bt.dz[0];
+    #pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int c = tid + k * T;
+                const int g = c < n ? bottom_index(bt, 0, c) : -1;
+                if (g >= 0) bt.z0[g] = zfin[c];
+            }
         }
     }
-    {
-        const int n = bt.dx[0] * bt.dy[0] * bt.dz[0];
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int c = tid + k * T;
-            const int g = c < n ? bottom_index(bt, 0, c) : -1;
-            if (g >= 0) bt.z0[g] = zfin[c];
-        }
-    }
+};
+__global__ void __launch_bounds__(kMgBottomThreads) k_mg_bottom(MgBottom bt, float omega, float wa, float wb, int nu, int coarse_sweeps,
+                                                                const int* done, XTerm xt) {
+    k_mg_bottom_f::run(blockIdx.x, gridDim.x, bt, omega, wa, wb, nu, coarse_sweeps, done, xt);
 }
 
 }  // namespace vare

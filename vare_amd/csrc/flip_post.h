@@ -29,46 +29,58 @@ constexpr float kSurfBackground = 10.f;   // _sSurfField background after the sp
 // order).  kernel2 (:375-392) follows in the same thread: level set |sum(w p)/sum(w) - centre| - 0.5
 // (5 without weight) and velocity sum(w v)/sum(w) (0 without weight).
 template <int B>
+struct k_splat_gather_f {
+    static constexpr int kLB = WG;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, int nc, const float* __restrict__ px, const float* __restrict__ py,
+                                                         const float* __restrict__ pz, VelPtr u,
+                                                         const int* __restrict__ cstart, FrameK f, float* __restrict__ surf,
+                                                         float* __restrict__ ox, float* __restrict__ oy,
+                                                         float* __restrict__ oz) {
+        for (int c = vbid * WG + threadIdx.x; c < nc; c += vnbid * WG) {
+            int i, j, k;
+            cell_coords<B>(f, c, i, j, k);
+            const float cxf = (float)i + 0.5f, cyf = (float)j + 0.5f, czf = (float)k + 0.5f;
+            float sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f, su = 0.f, sv = 0.f, sr = 0.f;
+            for (int bk = rmax(k - 2, 0); bk <= rmin(k + 1, f.nz - 1); ++bk)
+                for (int bj = rmax(j - 2, 0); bj <= rmin(j + 1, f.ny - 1); ++bj)
+                    for (int bi = rmax(i - 2, 0); bi <= rmin(i + 1, f.nx - 1); ++bi) {
+                        const int bin = cell_index<B>(f, bi, bj, bk);
+                        if (bin < 0) continue;                        // inactive block: no particles
+                        const int q1 = cstart[bin + 1];
+                        for (int q = cstart[bin]; q < q1; ++q) {
+                            const float x = px[q], y = py[q], z = pz[q];
+                            const int di = (int)x - i, dj = (int)y - j, dk = (int)z - k;
+                            if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;
+                            const float dx = x - cxf, dy = y - cyf, dz = z - czf;
+                            const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+                            const float r = dist / kSplatInfluence;
+                            const float tt = 1.f - r * r;
+                            const float w = rmax(0.f, tt * tt * tt);
+                            if (!(w > 0.f)) continue;                 // adding 0 changes no sum
+                            sw += w;
+                            sx += w * x; sy += w * y; sz += w * z;
+                            su += w * u.ld(0, q); sv += w * u.ld(1, q); sr += w * u.ld(2, q);
+                        }
+                    }
+            if (sw > 1e-30f) {
+                const float ex = sx / sw - cxf, ey = sy / sw - cyf, ez = sz / sw - czf;
+                surf[c] = sqrtf(ex * ex + ey * ey + ez * ez) - kSplatDroplet;
+                ox[c] = su / sw; oy[c] = sv / sw; oz[c] = sr / sw;
+            } else {
+                surf[c] = 5.f;
+                ox[c] = 0.f; oy[c] = 0.f; oz[c] = 0.f;
+            }
+        }
+    }
+};
+template <int B>
 __global__ void __launch_bounds__(WG) k_splat_gather(int nc, const float* __restrict__ px, const float* __restrict__ py,
                                                      const float* __restrict__ pz, VelPtr u,
                                                      const int* __restrict__ cstart, FrameK f, float* __restrict__ surf,
                                                      float* __restrict__ ox, float* __restrict__ oy,
                                                      float* __restrict__ oz) {
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
-        int i, j, k;
-        cell_coords<B>(f, c, i, j, k);
-        const float cxf = (float)i + 0.5f, cyf = (float)j + 0.5f, czf = (float)k + 0.5f;
-        float sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f, su = 0.f, sv = 0.f, sr = 0.f;
-        for (int bk = rmax(k - 2, 0); bk <= rmin(k + 1, f.nz - 1); ++bk)
-            for (int bj = rmax(j - 2, 0); bj <= rmin(j + 1, f.ny - 1); ++bj)
-                for (int bi = rmax(i - 2, 0); bi <= rmin(i + 1, f.nx - 1); ++bi) {
-                    const int bin = cell_index<B>(f, bi, bj, bk);
-                    if (bin < 0) continue;                        // inactive block: no particles
-                    const int q1 = cstart[bin + 1];
-                    for (int q = cstart[bin]; q < q1; ++q) {
-                        const float x = px[q], y = py[q], z = pz[q];
-                        const int di = (int)x - i, dj = (int)y - j, dk = (int)z - k;
-                        if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;
-                        const float dx = x - cxf, dy = y - cyf, dz = z - czf;
-                        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-                        const float r = dist / kSplatInfluence;
-                        const float tt = 1.f - r * r;
-                        const float w = rmax(0.f, tt * tt * tt);
-                        if (!(w > 0.f)) continue;                 // adding 0 changes no sum
-                        sw += w;
-                        sx += w * x; sy += w * y; sz += w * z;
-                        su += w * u.ld(0, q); sv += w * u.ld(1, q); sr += w * u.ld(2, q);
-                    }
-                }
-        if (sw > 1e-30f) {
-            const float ex = sx / sw - cxf, ey = sy / sw - cyf, ez = sz / sw - czf;
-            surf[c] = sqrtf(ex * ex + ey * ey + ez * ez) - kSplatDroplet;
-            ox[c] = su / sw; oy[c] = sv / sw; oz[c] = sr / sw;
-        } else {
-            surf[c] = 5.f;
-            ox[c] = 0.f; oy[c] = 0.f; oz[c] = 0.f;
-        }
-    }
+    k_splat_gather_f<B>::run(blockIdx.x, gridDim.x, nc, px, py, pz, u, cstart, f, surf, ox, oy, oz);
 }
 
 // Value of a scalar field at neighbour q of cell c: the cell's value, the background for an inactive
@@ -82,43 +94,51 @@ __device__ __forceinline__ float nbr_val(const float* __restrict__ f, int id, fl
 // differences; inactive neighbours read 10; domain faces extrapolate (2v - v_opp), or with the water
 // tank hold the level's plane (cell.y - level, -1 / +1 below / above).
 template <int B, bool TANK>
-__global__ void k_redistance(int nc, FrameK f, const float* __restrict__ src, float* __restrict__ dst, float level) {
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
-        const float v = src[c];
-        const float sign = v / sqrtf(v * v + 1.f);
-        int nid[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) nid[q] = cell_neighbor<B>(f, c, q);
-        float nv[6];
-        if (TANK) {
-            int i, j, k;
-            cell_coords<B>(f, c, i, j, k);
-            const float ll = ((float)j + 0.5f) - level;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const float e = q == 2 ? ll - 1.f : (q == 3 ? ll + 1.f : ll);
-                nv[q] = nbr_val(src, nid[q], kSurfBackground, e);
+struct k_redistance_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, int nc, FrameK f, const float* __restrict__ src, float* __restrict__ dst, float level) {
+        for (int c = vbid * WG + threadIdx.x; c < nc; c += vnbid * WG) {
+            const float v = src[c];
+            const float sign = v / sqrtf(v * v + 1.f);
+            int nid[6];
+    #pragma unroll
+            for (int q = 0; q < 6; ++q) nid[q] = cell_neighbor<B>(f, c, q);
+            float nv[6];
+            if (TANK) {
+                int i, j, k;
+                cell_coords<B>(f, c, i, j, k);
+                const float ll = ((float)j + 0.5f) - level;
+    #pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const float e = q == 2 ? ll - 1.f : (q == 3 ? ll + 1.f : ll);
+                    nv[q] = nbr_val(src, nid[q], kSurfBackground, e);
+                }
+            } else {
+    #pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float lo = nbr_val(src, nid[2 * a], kSurfBackground, 0.f);
+                    const float hi = nbr_val(src, nid[2 * a + 1], kSurfBackground, 0.f);
+                    nv[2 * a] = nid[2 * a] == -2 ? 2.f * v - hi : lo;
+                    nv[2 * a + 1] = nid[2 * a + 1] == -2 ? 2.f * v - lo : hi;
+                }
             }
-        } else {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const float lo = nbr_val(src, nid[2 * a], kSurfBackground, 0.f);
-                const float hi = nbr_val(src, nid[2 * a + 1], kSurfBackground, 0.f);
-                nv[2 * a] = nid[2 * a] == -2 ? 2.f * v - hi : lo;
-                nv[2 * a + 1] = nid[2 * a + 1] == -2 ? 2.f * v - lo : hi;
-            }
+            const float dx0 = v - nv[0], dx1 = nv[1] - v;
+            const float dy0 = v - nv[2], dy1 = nv[3] - v;
+            const float dz0 = v - nv[4], dz1 = nv[5] - v;
+            auto P2 = [](float a) { return a * a; };
+            const float gp = sqrtf(P2(rmax(dx0, 0.f)) + P2(rmin(dx1, 0.f)) + P2(rmax(dy0, 0.f)) + P2(rmin(dy1, 0.f)) +
+                                   P2(rmax(dz0, 0.f)) + P2(rmin(dz1, 0.f)));
+            const float gm = sqrtf(P2(rmin(dx0, 0.f)) + P2(rmax(dx1, 0.f)) + P2(rmin(dy0, 0.f)) + P2(rmax(dy1, 0.f)) +
+                                   P2(rmin(dz0, 0.f)) + P2(rmax(dz1, 0.f)));
+            constexpr float dt = 0.3f;
+            dst[c] = v - dt * rmax(sign, 0.f) * (gp - 1.f) - dt * rmin(sign, 0.f) * (gm - 1.f);
         }
-        const float dx0 = v - nv[0], dx1 = nv[1] - v;
-        const float dy0 = v - nv[2], dy1 = nv[3] - v;
-        const float dz0 = v - nv[4], dz1 = nv[5] - v;
-        auto P2 = [](float a) { return a * a; };
-        const float gp = sqrtf(P2(rmax(dx0, 0.f)) + P2(rmin(dx1, 0.f)) + P2(rmax(dy0, 0.f)) + P2(rmin(dy1, 0.f)) +
-                               P2(rmax(dz0, 0.f)) + P2(rmin(dz1, 0.f)));
-        const float gm = sqrtf(P2(rmin(dx0, 0.f)) + P2(rmax(dx1, 0.f)) + P2(rmin(dy0, 0.f)) + P2(rmax(dy1, 0.f)) +
-                               P2(rmin(dz0, 0.f)) + P2(rmax(dz1, 0.f)));
-        constexpr float dt = 0.3f;
-        dst[c] = v - dt * rmax(sign, 0.f) * (gp - 1.f) - dt * rmin(sign, 0.f) * (gm - 1.f);
     }
+};
+template <int B, bool TANK>
+__global__ void k_redistance(int nc, FrameK f, const float* __restrict__ src, float* __restrict__ dst, float level) {
+    k_redistance_f<B, TANK>::run(blockIdx.x, gridDim.x, nc, f, src, dst, level);
 }
 
 struct Vec3Field {
@@ -137,60 +157,86 @@ __device__ __forceinline__ void vec_pair(const Vec3Field& v, const int* nid, int
 
 // FieldOP::curl (FieldOP.cu:338-369) of v into out.
 template <int B>
+struct k_curl_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, int nc, FrameK f, Vec3Field v, float* __restrict__ ox, float* __restrict__ oy,
+                           float* __restrict__ oz) {
+        for (int c = vbid * WG + threadIdx.x; c < nc; c += vnbid * WG) {
+            int nid[6];
+    #pragma unroll
+            for (int q = 0; q < 6; ++q) nid[q] = cell_neighbor<B>(f, c, q);
+            const float own[3] = {v.x[c], v.y[c], v.z[c]};
+            float x0y, x1y, x0z, x1z, y0x, y1x, y0z, y1z, z0x, z1x, z0y, z1y;
+            vec_pair<B>(v, nid, 0, 1, own[1], x0y, x1y);
+            vec_pair<B>(v, nid, 0, 2, own[2], x0z, x1z);
+            vec_pair<B>(v, nid, 1, 0, own[0], y0x, y1x);
+            vec_pair<B>(v, nid, 1, 2, own[2], y0z, y1z);
+            vec_pair<B>(v, nid, 2, 0, own[0], z0x, z1x);
+            vec_pair<B>(v, nid, 2, 1, own[1], z0y, z1y);
+            ox[c] = (y1z - y0z) * 0.5f - (z1y - z0y) * 0.5f;
+            oy[c] = (z1x - z0x) * 0.5f - (x1z - x0z) * 0.5f;
+            oz[c] = (x1y - x0y) * 0.5f - (y1x - y0x) * 0.5f;
+        }
+    }
+};
+template <int B>
 __global__ void k_curl(int nc, FrameK f, Vec3Field v, float* __restrict__ ox, float* __restrict__ oy,
                        float* __restrict__ oz) {
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
-        int nid[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) nid[q] = cell_neighbor<B>(f, c, q);
-        const float own[3] = {v.x[c], v.y[c], v.z[c]};
-        float x0y, x1y, x0z, x1z, y0x, y1x, y0z, y1z, z0x, z1x, z0y, z1y;
-        vec_pair<B>(v, nid, 0, 1, own[1], x0y, x1y);
-        vec_pair<B>(v, nid, 0, 2, own[2], x0z, x1z);
-        vec_pair<B>(v, nid, 1, 0, own[0], y0x, y1x);
-        vec_pair<B>(v, nid, 1, 2, own[2], y0z, y1z);
-        vec_pair<B>(v, nid, 2, 0, own[0], z0x, z1x);
-        vec_pair<B>(v, nid, 2, 1, own[1], z0y, z1y);
-        ox[c] = (y1z - y0z) * 0.5f - (z1y - z0y) * 0.5f;
-        oy[c] = (z1x - z0x) * 0.5f - (x1z - x0z) * 0.5f;
-        oz[c] = (x1y - x0y) * 0.5f - (y1x - y0x) * 0.5f;
-    }
+    k_curl_f<B>::run(blockIdx.x, gridDim.x, nc, f, v, ox, oy, oz);
 }
 
 // FieldOP::normalGradient (FieldOP.cu:80-110): normalised central-difference gradient of s (inactive
 // neighbours read the background bg), Vector3::normalized (Vector3.h:350-354).
 template <int B>
+struct k_normal_gradient_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, int nc, FrameK f, const float* __restrict__ s, float bg, float* __restrict__ ox,
+                                      float* __restrict__ oy, float* __restrict__ oz) {
+        for (int c = vbid * WG + threadIdx.x; c < nc; c += vnbid * WG) {
+            const float v = s[c];
+            float g[3];
+    #pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int il = cell_neighbor<B>(f, c, 2 * a), ih = cell_neighbor<B>(f, c, 2 * a + 1);
+                const float l = nbr_val(s, il, bg, 0.f), h = nbr_val(s, ih, bg, 0.f);
+                const float lo = il == -2 ? 2.f * v - h : l;
+                const float hi = ih == -2 ? 2.f * v - l : h;
+                g[a] = (hi - lo) * 0.5f;
+            }
+            const float d = 1.f / sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + 1e-30f);
+            ox[c] = g[0] * d; oy[c] = g[1] * d; oz[c] = g[2] * d;
+        }
+    }
+};
+template <int B>
 __global__ void k_normal_gradient(int nc, FrameK f, const float* __restrict__ s, float bg, float* __restrict__ ox,
                                   float* __restrict__ oy, float* __restrict__ oz) {
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
-        const float v = s[c];
-        float g[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int il = cell_neighbor<B>(f, c, 2 * a), ih = cell_neighbor<B>(f, c, 2 * a + 1);
-            const float l = nbr_val(s, il, bg, 0.f), h = nbr_val(s, ih, bg, 0.f);
-            const float lo = il == -2 ? 2.f * v - h : l;
-            const float hi = ih == -2 ? 2.f * v - l : h;
-            g[a] = (hi - lo) * 0.5f;
-        }
-        const float d = 1.f / sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + 1e-30f);
-        ox[c] = g[0] * d; oy[c] = g[1] * d; oz[c] = g[2] * d;
-    }
+    k_normal_gradient_f<B>::run(blockIdx.x, gridDim.x, nc, f, s, bg, ox, oy, oz);
 }
 
 // FieldOP::curvature (FieldOP.cu:273-301): central-difference divergence of n into out.
 template <int B>
-__global__ void k_divergence_vec(int nc, FrameK f, Vec3Field n, float* __restrict__ out) {
-    for (int c = blockIdx.x * WG + threadIdx.x; c < nc; c += gridDim.x * WG) {
-        int nid[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) nid[q] = cell_neighbor<B>(f, c, q);
-        float lo[3], hi[3];
-        vec_pair<B>(n, nid, 0, 0, n.x[c], lo[0], hi[0]);
-        vec_pair<B>(n, nid, 1, 1, n.y[c], lo[1], hi[1]);
-        vec_pair<B>(n, nid, 2, 2, n.z[c], lo[2], hi[2]);
-        out[c] = (hi[0] - lo[0]) * 0.5f + (hi[1] - lo[1]) * 0.5f + (hi[2] - lo[2]) * 0.5f;
+struct k_divergence_vec_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, int nc, FrameK f, Vec3Field n, float* __restrict__ out) {
+        for (int c = vbid * WG + threadIdx.x; c < nc; c += vnbid * WG) {
+            int nid[6];
+    #pragma unroll
+            for (int q = 0; q < 6; ++q) nid[q] = cell_neighbor<B>(f, c, q);
+            float lo[3], hi[3];
+            vec_pair<B>(n, nid, 0, 0, n.x[c], lo[0], hi[0]);
+            vec_pair<B>(n, nid, 1, 1, n.y[c], lo[1], hi[1]);
+            vec_pair<B>(n, nid, 2, 2, n.z[c], lo[2], hi[2]);
+            out[c] = (hi[0] - lo[0]) * 0.5f + (hi[1] - lo[1]) * 0.5f + (hi[2] - lo[2]) * 0.5f;
+        }
     }
+};
+template <int B>
+__global__ void k_divergence_vec(int nc, FrameK f, Vec3Field n, float* __restrict__ out) {
+    k_divergence_vec_f<B>::run(blockIdx.x, gridDim.x, nc, f, n, out);
 }
 
 // SparseField::lerp (SparseField.h:150-173) of one scalar field at p, background bg.
@@ -221,31 +267,51 @@ __device__ __forceinline__ float lerp_scalar(const FrameK& f, const float* __res
 // vort_kernel (FLIPMethod.cu:375-385): v = max(0, |worldVector(curl lerp)| - 0.2); vort -= vort * dis;
 // vort = max(v, vort).  Particles are in physical order, vorticity in the reference order (pid).
 template <int B>
+struct k_particle_vorticity_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, int np, const float* __restrict__ px, const float* __restrict__ py,
+                                         const float* __restrict__ pz, const int* __restrict__ pid, FrameK f, Vec3Field curl,
+                                         float sx, float sy, float sz, float dis, float* __restrict__ vort) {
+        const int q = vbid * WG + threadIdx.x;
+        if (q >= np) return;
+        const float x = px[q], y = py[q], z = pz[q];
+        const float cx = lerp_scalar<B>(f, curl.x, 0.f, x, y, z) * sx;
+        const float cy = lerp_scalar<B>(f, curl.y, 0.f, x, y, z) * sy;
+        const float cz = lerp_scalar<B>(f, curl.z, 0.f, x, y, z) * sz;
+        float v = sqrtf(cx * cx + cy * cy + cz * cz);
+        v = rmax(0.f, v - 0.2f);
+        const int lg = pid[q];
+        float w = vort[lg];
+        w -= w * dis;
+        vort[lg] = rmax(v, w);
+    }
+};
+template <int B>
 __global__ void k_particle_vorticity(int np, const float* __restrict__ px, const float* __restrict__ py,
                                      const float* __restrict__ pz, const int* __restrict__ pid, FrameK f, Vec3Field curl,
                                      float sx, float sy, float sz, float dis, float* __restrict__ vort) {
-    const int q = blockIdx.x * WG + threadIdx.x;
-    if (q >= np) return;
-    const float x = px[q], y = py[q], z = pz[q];
-    const float cx = lerp_scalar<B>(f, curl.x, 0.f, x, y, z) * sx;
-    const float cy = lerp_scalar<B>(f, curl.y, 0.f, x, y, z) * sy;
-    const float cz = lerp_scalar<B>(f, curl.z, 0.f, x, y, z) * sz;
-    float v = sqrtf(cx * cx + cy * cy + cz * cz);
-    v = rmax(0.f, v - 0.2f);
-    const int lg = pid[q];
-    float w = vort[lg];
-    w -= w * dis;
-    vort[lg] = rmax(v, w);
+    k_particle_vorticity_f<B>::run(blockIdx.x, gridDim.x, np, px, py, pz, pid, f, curl, sx, sy, sz, dis, vort);
 }
 
 // curv_kernel (FLIPMethod.cu:393-398): the curvature field's lerp (background 10) at each particle.
 template <int B>
+struct k_particle_curvature_f {
+    static constexpr int kLB = 1024;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, int np, const float* __restrict__ px, const float* __restrict__ py,
+                                         const float* __restrict__ pz, const int* __restrict__ pid, FrameK f,
+                                         const float* __restrict__ kfield, float* __restrict__ curv) {
+        const int q = vbid * WG + threadIdx.x;
+        if (q >= np) return;
+        curv[pid[q]] = lerp_scalar<B>(f, kfield, kSurfBackground, px[q], py[q], pz[q]);
+    }
+};
+template <int B>
 __global__ void k_particle_curvature(int np, const float* __restrict__ px, const float* __restrict__ py,
                                      const float* __restrict__ pz, const int* __restrict__ pid, FrameK f,
                                      const float* __restrict__ kfield, float* __restrict__ curv) {
-    const int q = blockIdx.x * WG + threadIdx.x;
-    if (q >= np) return;
-    curv[pid[q]] = lerp_scalar<B>(f, kfield, kSurfBackground, px[q], py[q], pz[q]);
+    k_particle_curvature_f<B>::run(blockIdx.x, gridDim.x, np, px, py, pz, pid, f, kfield, curv);
 }
 
 }  // namespace vare
