@@ -14,7 +14,9 @@ Other configurations (--config):
      and FP64 iterative refinement to 1e-8 relative, 10 steps;
   4  served batch: independent tanks (edges log-uniform 32..128, 4..8 particles per cell, water
      fractions 0.2..0.7, 10 steps each) arriving as a Poisson process, continuously batched by the
-     library's server, compared with the same requests solved one at a time ("plain GPU");
+     library's server (every resident tank stepped in one round of shared kernel launches), compared
+     with the same requests solved one at a time ("plain GPU"), with the same arrivals stepped one
+     stream per tank without shared launches, and with all requests waiting at t=0 (capacity);
   5  sparse wave on an obstacle: 1024x512x1024 cells (dx = 1/512, world 2x1x2), a wall of water
      x<0.25, y<0.3, full z moving +x at 4 m/s, a static box obstacle x in [0.5,0.7], y in [0,0.5],
      z in [0.6,1.4] given as a signed distance field, FP16 velocities, FP64 refinement to 1e-6, 100
@@ -515,25 +517,40 @@ def run_served(a, L):
     print("bench: %d requests, %d particles in total" % (nreq, total_particles), file=sys.stderr, flush=True)
     budget = 200 << 30
     # isolated solve time (sequential server, no arrival gaps) sets the Poisson rate
-    warm = L.Server(budget, 1)
-    for g, p, pos, vel in reqs[:8]:
-        warm.submit(g, p, pos, vel, steps=10, dt=DT)
-    t = time.perf_counter()
-    warm.run()
-    iso = (time.perf_counter() - t) / 8
+    # (a first pass loads the kernels and creates the device states; the second one is timed)
+    for timed in (False, True):
+        warm = L.Server(budget, 1, batching=False)
+        for g, p, pos, vel in reqs[:8]:
+            warm.submit(g, p, pos, vel, steps=10, dt=DT)
+        t = time.perf_counter()
+        warm.run()
+        iso = (time.perf_counter() - t) / 8
+        warm.close()
     mean_gap = 0.5 * iso
     arrivals = np.cumsum(rng.exponential(mean_gap, nreq))
     arrivals -= arrivals[0]
 
-    def serve(slots, with_arrivals):
-        srv = L.Server(budget, slots)
+    def serve(slots, with_arrivals, batching=True):
+        srv = L.Server(budget, slots, batching=batching)
         for k, (g, p, pos, vel) in enumerate(reqs):
             srv.submit(g, p, pos, vel, steps=10, dt=DT, arrival_s=float(arrivals[k]) if with_arrivals else 0.0)
         srv.run()
         return srv.stats()
 
-    plain = serve(1, False)                 # one tank at a time, back to back
-    served = serve(a.slots, True)           # Poisson arrivals, continuous batching
+    def brief(st, note):
+        return {"value": round(st["particle_steps_per_s"], 1), "p50_ms": round(st["step_p50_ms"], 3),
+                "p99_ms": round(st["step_p99_ms"], 3), "wall_s": round(st["wall_s"], 3),
+                "max_batch": int(st["max_batch"]), "note": note}
+
+    plain = serve(1, False, batching=False)   # one tank at a time, back to back, vare_flip_step
+    L.batch_stats(reset=True)
+    served = serve(a.slots, True)             # Poisson arrivals, batched rounds
+    bstats = L.batch_stats()
+    streams = serve(a.slots, True, batching=False)   # Poisson arrivals, one stream per resident tank
+    L.batch_stats(reset=True)
+    saturated = serve(a.slots, False)         # every request waiting at t=0: the server's capacity
+    sstats = L.batch_stats()
+    streams_sat = serve(a.slots, False, batching=False)   # all waiting at t=0, one stream per tank
     return {
         "value": round(served["particle_steps_per_s"], 1),
         "ms_per_step": round(served["step_mean_ms"], 3),
@@ -542,15 +559,28 @@ def run_served(a, L):
         "steps": int(served["steps"]),
         "config": {"workload": "served batch: %d tank solves, edges log-uniform 32..128 (multiple of 8), 4..8 ppc, "
                                "water fractions U(0.2,0.7), 10 steps each at dt=1/240, %s, Poisson "
-                               "arrivals (mean gap %.1f ms = 0.5 x isolated solve), %d slots, 200 GB budget"
+                               "arrivals (mean gap %.1f ms = 0.5 x isolated solve), %d slots, 200 GB budget, "
+                               "resident tanks stepped in batched rounds (shared kernel launches)"
                                % (nreq, solver_desc(a, 4), 1e3 * mean_gap, a.slots),
                    "particles_total": total_particles, "max_resident": served["max_resident"],
                    "handles_created": served["handles_created"], "handles_reused": served["handles_reused"],
                    "peak_reserved_gb": round(served["peak_bytes"] / 2 ** 30, 2), "wall_s": round(served["wall_s"], 3)},
-        "plain_gpu": {"value": round(plain["particle_steps_per_s"], 1), "p50_ms": round(plain["step_p50_ms"], 3),
-                      "p99_ms": round(plain["step_p99_ms"], 3), "wall_s": round(plain["wall_s"], 3),
-                      "note": "same requests, one tank at a time"},
+        "batching": {"rounds": int(served["rounds"]), "max_batch": int(served["max_batch"]),
+                     "merged_launches": bstats["launches"], "tank_launches": bstats["tank_launches"],
+                     "sync_points": bstats["flushes"], "flush_host_s": round(bstats["flush_ms"] / 1e3, 3),
+                     "flush_wait_s": round(bstats["wait_ms"] / 1e3, 3),
+                     "saturated": {"rounds": int(saturated["rounds"]), "merged_launches": sstats["launches"],
+                                   "tank_launches": sstats["tank_launches"], "sync_points": sstats["flushes"],
+                                   "flush_host_s": round(sstats["flush_ms"] / 1e3, 3),
+                                   "flush_wait_s": round(sstats["wait_ms"] / 1e3, 3)}},
+        "plain_gpu": brief(plain, "same requests, one tank at a time (vare_flip_step)"),
+        "streams": brief(streams, "same arrivals, each resident tank stepped on its own stream, no shared launches"),
+        "saturated": brief(saturated, "batched rounds with all %d requests waiting at t=0 (capacity)" % nreq),
+        "streams_saturated": brief(streams_sat, "one stream per tank with all %d requests waiting at t=0" % nreq),
         "vs_plain_gpu": round(served["particle_steps_per_s"] / max(plain["particle_steps_per_s"], 1e-9), 3),
+        "vs_streams": round(served["particle_steps_per_s"] / max(streams["particle_steps_per_s"], 1e-9), 3),
+        "saturated_vs_plain_gpu": round(saturated["particle_steps_per_s"] / max(plain["particle_steps_per_s"], 1e-9), 3),
+        "saturated_vs_streams": round(saturated["particle_steps_per_s"] / max(streams_sat["particle_steps_per_s"], 1e-9), 3),
         "roofline": None,
     }
 

@@ -234,6 +234,8 @@ typedef struct vare_batch_stats {
     int64_t flushes;          /* sync points at which the recorded work was issued */
     int64_t launches;         /* merged kernel launches */
     int64_t tank_launches;    /* the kernel launches the tanks would have made on their own */
+    double flush_ms;          /* host wall time of the flushes: merging, issuing and waiting */
+    double wait_ms;           /* of that, waiting for the merged work to finish on the device */
 } vare_batch_stats;
 /* Counters of the batched steps on `device` since the last reset. */
 int vare_batch_get_stats(int device, vare_batch_stats* out, int reset);
@@ -404,9 +406,15 @@ typedef struct vare_server_stats {
     int64_t peak_bytes;                    /* peak reserved HBM (resident tanks + pooled handles) */
     int64_t handles_created;               /* device states allocated by the last run */
     int64_t handles_reused;                /* requests served by a pooled device state of the same grid */
+    int64_t rounds;                        /* batched rounds (one vare_flip_step_batched call each) */
+    int64_t max_batch;                     /* most tanks stepped in one round */
 } vare_server_stats;
 
 int vare_server_create(size_t hbm_budget_bytes, int max_slots, vare_server** out);
+/* Batched stepping (on = 1, the default): every round advances all resident tanks by one step with
+ * vare_flip_step_batched.  Off: each resident tank steps on its own handle's stream (vare_flip_step),
+ * overlapping with the others only as concurrent streams. */
+int vare_server_set_batching(vare_server* s, int on);
 int vare_server_destroy(vare_server* s);
 /* Queue a request (copies the particle arrays); returns its id. */
 int vare_server_submit(vare_server* s, const vare_tank_request* req, int64_t* id);

@@ -115,7 +115,8 @@ class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_int64), ("steps", C.c_int64), ("wall_s", C.c_double),
                 ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double), ("step_mean_ms", C.c_double),
                 ("particle_steps_per_s", C.c_double), ("max_resident", C.c_int), ("pad", C.c_int),
-                ("peak_bytes", C.c_int64), ("handles_created", C.c_int64), ("handles_reused", C.c_int64)]
+                ("peak_bytes", C.c_int64), ("handles_created", C.c_int64), ("handles_reused", C.c_int64),
+                ("rounds", C.c_int64), ("max_batch", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
@@ -123,7 +124,7 @@ class ServerStats(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("tanks", C.c_int64), ("flushes", C.c_int64), ("launches", C.c_int64),
-                ("tank_launches", C.c_int64)]
+                ("tank_launches", C.c_int64), ("flush_ms", C.c_double), ("wait_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -184,6 +185,7 @@ SIGNATURES = {
     "vare_last_error": (C.c_char_p, []),
     "vare_server_create": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(_P)]),
     "vare_server_destroy": (C.c_int, [_P]),
+    "vare_server_set_batching": (C.c_int, [_P, C.c_int]),
     "vare_server_submit": (C.c_int, [_P, C.POINTER(TankRequest), C.POINTER(C.c_int64)]),
     "vare_server_run": (C.c_int, [_P]),
     "vare_server_result": (C.c_int, [_P, C.c_int64, _P, _P, C.c_size_t, C.POINTER(C.c_size_t),
@@ -480,10 +482,11 @@ class FlipHandle:
 class Server:
     """Owning wrapper of a ``vare_server`` (continuous batching of independent tank solves)."""
 
-    def __init__(self, budget_bytes, max_slots):
+    def __init__(self, budget_bytes, max_slots, batching=True):
         self.lib = load()
         self.s = C.c_void_p()
         check(self.lib.vare_server_create(int(budget_bytes), int(max_slots), C.byref(self.s)))
+        check(self.lib.vare_server_set_batching(self.s, 1 if batching else 0))
         self._keep = []
 
     def close(self):

@@ -17,6 +17,7 @@
 #include <thread>
 #include <atomic>
 #include <mutex>
+#include <chrono>
 #include <condition_variable>
 #include <unordered_map>
 #include <array>
@@ -273,6 +274,7 @@ template <class T> void dfree(vare_flip* h, T*& p) {
 // ---- launches, readbacks and syncs of the step: direct on the handle's stream, or recorded while
 // the handle steps in a batch ----
 int batch_sync(vare_flip* h);
+int batch_chunk_hint(const vare_flip* h, int k);
 
 template <class F, class... P, class... A>
 void record_kernel(vare_flip* h, dim3 grid, dim3 block, size_t shm, A&&... a) {
@@ -1145,14 +1147,20 @@ template <int B> int run_solver(vare_flip* h, int cap, int k, int& total_iters) 
     const size_t ev_from = h->ev_used;
     int launched = 0;
     int chunk = last > 0 ? last : 8;
+    // in a batch, every tank enqueues the largest count any of them needed for this solve last step,
+    // so their iterations stay aligned and share launches (iterations past a tank's convergence
+    // return at its done flag)
+    if (h->rec) chunk = std::max(chunk, batch_chunk_hint(h, k));
+    // the first chunk is enqueued without a check: a solve that is done at its set-up returns at
+    // the device flag in every kernel
     for (;;) {
-        CHK(hread(h, h->st_host, h->st, sizeof(CgState)));
-        if (h->st_host->done) break;
-        if (launched > cap + 1) break;                       // defensive: the device flag ends first
         for (int q = 0; q < chunk; ++q) enqueue_iteration<B>(h, launched + q);
         HIPCHK(hipGetLastError());
         launched += chunk;
         chunk = last > 0 ? std::max(2, last / 8) : std::min(chunk * 2, 64);
+        CHK(hread(h, h->st_host, h->st, sizeof(CgState)));
+        if (h->st_host->done) break;
+        if (launched > cap + 1) break;                       // defensive: the device flag ends first
     }
     last = h->st_host->count;
     total_iters += h->st_host->count;
@@ -1207,7 +1215,8 @@ template <int B> int stage_solve(vare_flip* h) {
                            (const float*)h->b, (const float*)h->x, (const uint16_t*)h->info, h->partials);
         vlaunch<k_fin_true_f>(h, k_fin_true, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials, (int)g);
     }
-    CHK(hread(h, h->st_host, h->st, sizeof(CgState)));
+    // (the refinement loop ends with a read of the state after its last kernel)
+    if (!refine) CHK(hread(h, h->st_host, h->st, sizeof(CgState)));
     tbegin(h, VARE_KERNEL_APPLY);
     const bool apply64 = refine && !(h->prm.flags & VARE_FLAG_FP32_APPLY);
     if (nc && refine && !apply64)       // the refined pressure, rounded once, applied in FP32
@@ -2508,6 +2517,7 @@ struct BatchRun {
     const float* jdt = nullptr;
     int* jrc = nullptr;
     std::vector<std::string> jerr;
+    int chunk_hint[20] = {};          // per solve of the step: the most iterations a member took last step
     vare_batch_stats stats{};
 };
 
@@ -2612,7 +2622,10 @@ hipError_t run_merged(BatchRun& R, const std::vector<vare_flip*>& ms) {
         }
         if (e != hipSuccess) return e;
     }
-    return hipStreamSynchronize(R.stream);
+    const auto t0 = std::chrono::steady_clock::now();
+    const hipError_t es = hipStreamSynchronize(R.stream);
+    R.stats.wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return es;
 }
 
 // With R.m held, when every stepping tank waits (or the last one left with work recorded).
@@ -2621,7 +2634,9 @@ void flush_locked(BatchRun& R) {
     for (vare_flip* h : R.members)
         if (!h->rops.empty()) ms.push_back(h);
     hipError_t e = hipSuccess;
+    const auto t0 = std::chrono::steady_clock::now();
     if (!ms.empty()) e = run_merged(R, ms);
+    R.stats.flush_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     R.stats.flushes++;
     for (vare_flip* h : ms) {
         if (e != hipSuccess) h->rec_rc = VARE_EHIP;
@@ -2653,6 +2668,8 @@ int batch_sync(vare_flip* h) {
     if (h->rec_rc != VARE_OK) return fail(h->rec_rc, "a merged launch of the batched step failed");
     return VARE_OK;
 }
+
+int batch_chunk_hint(const vare_flip* h, int k) { return h->rec->chunk_hint[std::min(k, 19)]; }
 
 // The tank's step has returned: the others no longer wait for it.
 void batch_leave(vare_flip* h) {
@@ -2733,6 +2750,10 @@ int vare_flip_step_batched(vare_flip* const* hs, int n, const float* dt, int* st
     {
         std::lock_guard<std::mutex> lk(R->m);
         R->members.assign(hs, hs + n);
+        for (int k = 0; k < 20; ++k) {
+            R->chunk_hint[k] = 0;
+            for (int i = 0; i < n; ++i) R->chunk_hint[k] = std::max(R->chunk_hint[k], hs[i]->last_iters[k]);
+        }
         R->active = n;
         R->waiting = 0;
         for (int i = 0; i < n; ++i) {

@@ -3,9 +3,17 @@
 //
 // Scheduling: requests are admitted strictly in arrival order (FIFO, no overtaking) when (a) the
 // request has arrived, (b) a slot is free and (c) its reserved bytes fit in what is left of the HBM
-// budget.  Each slot is a worker thread that owns one vare_flip handle (its own HIP stream) for the
-// lifetime of the request, so all resident tanks step concurrently on the device.  A finished tank
-// releases its slot immediately and the next arrived request is admitted.
+// budget.  Each slot is a worker thread that owns one vare_flip handle for the lifetime of the request:
+// it creates (or takes from the pool) the handle, uploads the particles, hands the tank to the stepper
+// and, when its steps are done, downloads the result and releases the slot, so the next arrived
+// request is admitted at once.
+//
+// Stepping (batched, the default): one stepper thread advances every resident tank by one step per
+// round with vare_flip_step_batched, so each kernel of the step is launched once for all of them; a
+// tank joins at the round after its upload and leaves after its last step (iteration-level continuous
+// batching).  A tank's step latency is the round's wall time.  With batching off
+// (vare_server_set_batching), each worker steps its own tank on its handle's stream instead, and the
+// tanks overlap only as concurrent streams.
 //
 // Handle pool: a finished tank's handle (device state sized for its grid) is kept, not destroyed, and
 // the next request with the same grid and velocity storage takes it over, so serving does not
@@ -44,6 +52,11 @@ struct Request {
     std::vector<double> step_ms;
     vare_request_stats st;
     bool done = false;
+    // batched stepping: the stepper owns the tank from hand-over until stepped == true
+    vare_flip* h = nullptr;
+    int steps_taken = 0;
+    int step_rc = VARE_OK;
+    bool stepped = false;
 };
 
 // Device bytes a request reserves: what vare_flip_create and the first steps of its tank allocate,
@@ -91,6 +104,13 @@ struct vare_server {
     std::vector<size_t> order;        // request indices sorted by arrival
     Clock::time_point t0;
     double wall_s = 0.0;
+    // batched stepping
+    int batching = 1;
+    std::vector<Request*> ready;      // uploaded tanks waiting to join the next round
+    std::condition_variable cv_step;
+    bool stop_stepper = false;
+    int64_t rounds = 0;
+    int max_batch = 0;
 };
 
 namespace {
@@ -122,7 +142,16 @@ vare_flip* run_request(vare_server* s, Request* q, vare_flip* h) {
     if (rc == VARE_OK)
         rc = vare_flip_set_particles(h, q->pos.data(), q->vel.data(), nullptr, nullptr, nullptr, q->req.n,
                                      VARE_SPACE_VOXEL, VARE_MEM_HOST);
-    for (int k = 0; rc == VARE_OK && k < q->req.steps; ++k) {
+    if (rc == VARE_OK && s->batching && q->req.steps > 0) {
+        // hand the tank to the stepper and wait until it has taken all of its steps
+        std::unique_lock<std::mutex> lk(s->mu);
+        q->h = h;
+        s->ready.push_back(q);
+        s->cv_step.notify_all();
+        s->cv.wait(lk, [&] { return q->stepped; });
+        rc = q->step_rc;
+    }
+    for (int k = 0; rc == VARE_OK && !s->batching && k < q->req.steps; ++k) {
         const auto a = Clock::now();
         rc = vare_flip_step(h, q->req.dt);
         if (rc == VARE_ENOCONV) rc = VARE_OK;      // results are still written; counted as a step
@@ -265,9 +294,62 @@ void worker(vare_server* s) {
     }
 }
 
+// Batched stepping: one round steps every resident tank once with shared launches
+// (vare_flip_step_batched); tanks handed over meanwhile join the next round, finished ones leave.
+void stepper(vare_server* s) {
+    std::vector<Request*> batch;
+    std::vector<vare_flip*> hs;
+    std::vector<float> dts;
+    std::vector<int> st;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_step.wait(lk, [&] { return !s->ready.empty() || !batch.empty() || s->stop_stepper; });
+            if (batch.empty() && s->ready.empty()) return;      // stop_stepper: every request is done
+            batch.insert(batch.end(), s->ready.begin(), s->ready.end());
+            s->ready.clear();
+            s->max_batch = std::max(s->max_batch, (int)batch.size());
+        }
+        hs.clear();
+        dts.clear();
+        for (Request* q : batch) {
+            hs.push_back(q->h);
+            dts.push_back(q->req.dt);
+        }
+        st.assign(batch.size(), VARE_OK);
+        const auto a = Clock::now();
+        (void)vare_flip_step_batched(hs.data(), (int)hs.size(), dts.data(), st.data());
+        const double ms = std::chrono::duration<double, std::milli>(Clock::now() - a).count();
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->rounds++;
+        size_t keep = 0;
+        for (size_t i = 0; i < batch.size(); ++i) {
+            Request* q = batch[i];
+            int rc = st[i];
+            if (rc == VARE_ENOCONV) rc = VARE_OK;      // results are still written; counted as a step
+            q->step_ms.push_back(ms);
+            q->steps_taken++;
+            if (rc != VARE_OK || q->steps_taken >= q->req.steps) {
+                q->step_rc = rc;
+                q->stepped = true;
+            } else {
+                batch[keep++] = q;
+            }
+        }
+        batch.resize(keep);
+        s->cv.notify_all();
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int vare_server_set_batching(vare_server* s, int on) {
+    if (!s) return VARE_EINVAL;
+    s->batching = on ? 1 : 0;
+    return VARE_OK;
+}
 
 int vare_server_request_bytes(const vare_tank_request* req, size_t* bytes) {
     if (!req || !bytes) return VARE_EINVAL;
@@ -328,9 +410,23 @@ int vare_server_run(vare_server* s) {
     s->resident = 0;
     s->max_resident = 0;
     s->t0 = Clock::now();
+    s->ready.clear();
+    s->stop_stepper = false;
+    s->rounds = 0;
+    s->max_batch = 0;
+    std::thread step_thread;
+    if (s->batching) step_thread = std::thread(stepper, s);
     std::vector<std::thread> th;
     for (int k = 0; k < s->max_slots; ++k) th.emplace_back(worker, s);
     for (auto& t : th) t.join();
+    if (s->batching) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop_stepper = true;
+        }
+        s->cv_step.notify_all();
+        step_thread.join();
+    }
     s->wall_s = since(s->t0);
     for (size_t i : s->order)
         if (s->reqs[i]->st.status != VARE_OK) return s->reqs[i]->st.status;
@@ -385,6 +481,8 @@ int vare_server_get_stats(vare_server* s, vare_server_stats* o) {
     o->peak_bytes = (int64_t)s->peak;
     o->handles_created = s->handles_created;
     o->handles_reused = s->handles_reused;
+    o->rounds = s->rounds;
+    o->max_batch = s->max_batch;
     return VARE_OK;
 }
 
