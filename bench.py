@@ -572,7 +572,9 @@ def run_served(a, L):
                      "saturated": {"rounds": int(saturated["rounds"]), "merged_launches": sstats["launches"],
                                    "tank_launches": sstats["tank_launches"], "sync_points": sstats["flushes"],
                                    "flush_host_s": round(sstats["flush_ms"] / 1e3, 3),
-                                   "flush_wait_s": round(sstats["wait_ms"] / 1e3, 3)}},
+                                   "flush_wait_s": round(sstats["wait_ms"] / 1e3, 3),
+                                   "between_flushes_s": round(sstats["gap_ms"] / 1e3, 3),
+                                   "in_batched_steps_s": round(sstats["step_ms"] / 1e3, 3)}},
         "plain_gpu": brief(plain, "same requests, one tank at a time (vare_flip_step)"),
         "streams": brief(streams, "same arrivals, each resident tank stepped on its own stream, no shared launches"),
         "saturated": brief(saturated, "batched rounds with all %d requests waiting at t=0 (capacity)" % nreq),

@@ -236,6 +236,8 @@ typedef struct vare_batch_stats {
     int64_t tank_launches;    /* the kernel launches the tanks would have made on their own */
     double flush_ms;          /* host wall time of the flushes: merging, issuing and waiting */
     double wait_ms;           /* of that, waiting for the merged work to finish on the device */
+    double gap_ms;            /* host time between consecutive flushes of a step (the tanks' host code) */
+    double step_ms;           /* wall time inside vare_flip_step_batched */
 } vare_batch_stats;
 /* Counters of the batched steps on `device` since the last reset. */
 int vare_batch_get_stats(int device, vare_batch_stats* out, int reset);

@@ -124,7 +124,8 @@ class ServerStats(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("tanks", C.c_int64), ("flushes", C.c_int64), ("launches", C.c_int64),
-                ("tank_launches", C.c_int64), ("flush_ms", C.c_double), ("wait_ms", C.c_double)]
+                ("tank_launches", C.c_int64), ("flush_ms", C.c_double), ("wait_ms", C.c_double),
+                ("gap_ms", C.c_double), ("step_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

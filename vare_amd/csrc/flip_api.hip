@@ -2500,10 +2500,9 @@ int vare_flip_get_kernel_timing(vare_flip* h, double ms[VARE_KERNEL_COUNT], int6
 struct BatchRun {
     std::mutex call;                  // one batched step at a time per device
     std::mutex m;                     // barrier state below
-    std::condition_variable cv;
     int active = 0;                   // tanks still inside their step
     int waiting = 0;                  // of those, waiting at a sync point
-    uint64_t gen = 0;                 // flushes so far
+    std::atomic<uint64_t> gen{0};     // flushes so far (waiters spin on it outside the lock)
     std::vector<vare_flip*> members;
     int device = 0;
     hipStream_t stream = nullptr;
@@ -2517,6 +2516,7 @@ struct BatchRun {
     const float* jdt = nullptr;
     int* jrc = nullptr;
     std::vector<std::string> jerr;
+    std::chrono::steady_clock::time_point last_flush_end{};   // within the current batched step
     int chunk_hint[20] = {};          // per solve of the step: the most iterations a member took last step
     vare_batch_stats stats{};
 };
@@ -2635,6 +2635,8 @@ void flush_locked(BatchRun& R) {
         if (!h->rops.empty()) ms.push_back(h);
     hipError_t e = hipSuccess;
     const auto t0 = std::chrono::steady_clock::now();
+    if (R.last_flush_end.time_since_epoch().count())
+        R.stats.gap_ms += std::chrono::duration<double, std::milli>(t0 - R.last_flush_end).count();
     if (!ms.empty()) e = run_merged(R, ms);
     R.stats.flush_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     R.stats.flushes++;
@@ -2649,22 +2651,35 @@ void flush_locked(BatchRun& R) {
     }
     if (e != hipSuccess) (void)hipGetLastError();
     R.waiting = 0;
-    R.gen++;
-    R.cv.notify_all();
+    R.last_flush_end = std::chrono::steady_clock::now();
+    R.gen.fetch_add(1, std::memory_order_release);
+}
+
+// Waits until the flush counter moves past g.  The waiters do not take the lock again on the way
+// out (a condition variable would wake them one after the other through the mutex): they spin,
+// then yield, then sleep briefly.
+void spin_until_past(const std::atomic<uint64_t>& gen, uint64_t g) {
+    for (int i = 0; gen.load(std::memory_order_acquire) == g; ++i) {
+        if (i < 2048) std::this_thread::yield();
+        else std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
 }
 
 int batch_sync(vare_flip* h) {
     BatchRun& R = *h->rec;
+    uint64_t g = 0;
+    bool flushed = false;
     {
-        std::unique_lock<std::mutex> lk(R.m);
+        std::lock_guard<std::mutex> lk(R.m);
         R.waiting++;
         if (R.waiting == R.active) {
             flush_locked(R);
+            flushed = true;
         } else {
-            const uint64_t g = R.gen;
-            R.cv.wait(lk, [&] { return R.gen != g; });
+            g = R.gen.load(std::memory_order_relaxed);
         }
     }
+    if (!flushed) spin_until_past(R.gen, g);
     if (h->rec_rc != VARE_OK) return fail(h->rec_rc, "a merged launch of the batched step failed");
     return VARE_OK;
 }
@@ -2744,6 +2759,8 @@ int vare_flip_step_batched(vare_flip* const* hs, int n, const float* dt, int* st
     BatchRun* R = nullptr;
     CHK(batch_runner(hs[0]->device, &R));
     std::lock_guard<std::mutex> call(R->call);
+    const auto t_call = std::chrono::steady_clock::now();
+    R->last_flush_end = std::chrono::steady_clock::time_point{};
     // the handles' own streams may still hold work (uploads): it comes first
     for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(hs[i]->stream));
     std::vector<int> rc((size_t)n, VARE_OK);
@@ -2788,6 +2805,7 @@ int vare_flip_step_batched(vare_flip* const* hs, int n, const float* dt, int* st
         R->members.clear();
         R->stats.steps++;
         R->stats.tanks += n;
+        R->stats.step_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_call).count();
     }
     int first = VARE_OK;
     for (int i = 0; i < n; ++i) {
