@@ -20,8 +20,10 @@
 //   * K1 block marking (SparseFrame.cu:149-157) writes out of bounds of _markers when the signed
 //     int index k*fx*fy+j*fx+i is negative; those writes are dropped here.  In-bounds aliasing
 //     (a -1 coordinate borrowing from the next axis) is reproduced.
-//   * BLAS::dot (BLAS.cu:98-104) is thrust::inner_product in FP32 with an unspecified tree order;
-//     here it accumulates in double and rounds once to R ("parity unpinned" for the order).
+//   * BLAS::dot (BLAS.cu:98-104) is thrust::inner_product in FP32 with an unspecified tree order,
+//     which cannot be reproduced; here it accumulates in double and rounds once to R, within n ulps
+//     of the exactly rounded sum (pinned by tests/test_oracle_solver.py).  The order changes the
+//     reference-mode CG's iteration count by a few at most.
 #include <cstdint>
 #include <cstddef>
 #include <cstring>
@@ -1114,6 +1116,14 @@ struct orc_stats {
 };
 
 float orc_rand(uint32_t seed) { return rand01(seed); }
+
+// The solver's dot product (BLAS::dot, BLAS.cu:98-104) of the FP32 and FP64 instances, for its pins.
+float orc_dot_f32(const float* a, const float* c, size_t n) {
+    return orc::State<float>::dot(std::vector<float>(a, a + n), std::vector<float>(c, c + n));
+}
+double orc_dot_f64(const double* a, const double* c, size_t n) {
+    return orc::State<double>::dot(std::vector<double>(a, a + n), std::vector<double>(c, c + n));
+}
 
 // Threads of the OpenMP build (returns the count in use; 1 in the checker build).
 int orc_set_threads(int n) {

@@ -52,6 +52,8 @@ def load(path=None):
     lib = C.CDLL(path)
     sig = {
         "orc_rand": (C.c_float, [C.c_uint32]),
+        "orc_dot_f32": (C.c_float, [C.c_void_p, C.c_void_p, C.c_size_t]),
+        "orc_dot_f64": (C.c_double, [C.c_void_p, C.c_void_p, C.c_size_t]),
         "orc_set_threads": (C.c_int, [C.c_int]),
         "orc_geom": (None, [_V, _V, _V, C.c_int, _V, _V, _V, _V]),
         "orc_grid_from_voxel_size": (None, [C.c_float, _V, _V, _V]),
@@ -128,6 +130,15 @@ def params(**kw) -> OrcParams:
         else:
             setattr(p, k, v)
     return p
+
+
+def dot(a, c):
+    """The oracle solver's dot product (BLAS::dot) on float32 or float64 vectors."""
+    if a.dtype == np.float32:
+        a, c = np.ascontiguousarray(a, np.float32), np.ascontiguousarray(c, np.float32)
+        return load().orc_dot_f32(_p(a), _p(c), a.size)
+    a, c = np.ascontiguousarray(a, np.float64), np.ascontiguousarray(c, np.float64)
+    return load().orc_dot_f64(_p(a), _p(c), a.size)
 
 
 def rand(seed: int) -> float:

@@ -2,7 +2,8 @@
 
 Each mutant is a copy of oracle/flip_oracle.cpp with ONE plausible restatement mistake planted in the
 boundary rules of FluidProjection::buildLinearSystem (base/source/FluidProjection.cu:206-244) or
-FluidProjection::solve (:297-378), or in the frame post-process (FLIPMethod.cu:300-405): a flipped
+FluidProjection::solve (:297-378), in the frame post-process (FLIPMethod.cu:300-405) or in the solver's
+dot product (BLAS.cu:98-104): a flipped
 sign, a swapped neighbour index or component, a dropped extrapolation or normalisation, a wrong
 divisor or exponent.  The copy is compiled into a temporary directory and the pins
 (tests/test_oracle_boundary.py, tests/test_oracle_post.py) are run against it (VARE_ORACLE_LIB);
@@ -67,6 +68,17 @@ POST_MUTANTS = [
     ("vorticity: threshold sign", "v = Max(R(0), v - R(0.2f));", "v = Max(R(0), v + R(0.2f));"),
 ]
 
+# BLAS::dot (BLAS.cu:98-104), pinned by tests/test_oracle_solver.py
+DOT_PINS = os.path.join(ROOT, "tests", "test_oracle_solver.py")
+DOT_MUTANTS = [
+    ("dot: a times a", "for (size_t n = 0; n < a.size(); ++n) s += (double)a[n] * (double)c[n];",
+     "for (size_t n = 0; n < a.size(); ++n) s += (double)a[n] * (double)a[n];"),
+    ("dot: last term dropped", "for (size_t n = 0; n < a.size(); ++n) s += (double)a[n] * (double)c[n];",
+     "for (size_t n = 0; n + 1 < a.size(); ++n) s += (double)a[n] * (double)c[n];"),
+    ("dot: accumulated in the solver's precision", "        double s = 0.0;\n#pragma omp parallel for reduction(+ : s)",
+     "        R s = 0;\n#pragma omp parallel for reduction(+ : s)"),
+]
+
 FLAGS = ["-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math"]
 
 
@@ -84,7 +96,8 @@ def _build_and_run(src_text, workdir, tag, pins=PINS):
     return r.returncode
 
 
-@pytest.mark.parametrize("mutants,pins", [(MUTANTS, PINS), (POST_MUTANTS, POST_PINS)], ids=["boundary", "post"])
+@pytest.mark.parametrize("mutants,pins", [(MUTANTS, PINS), (POST_MUTANTS, POST_PINS), (DOT_MUTANTS, DOT_PINS)],
+                         ids=["boundary", "post", "dot"])
 def test_every_planted_mistake_fails_a_pin(mutants, pins):
     src = open(SRC).read()
     for name, old, new in mutants:
