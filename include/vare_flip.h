@@ -241,6 +241,13 @@ typedef struct vare_batch_stats {
 } vare_batch_stats;
 /* Counters of the batched steps on `device` since the last reset. */
 int vare_batch_get_stats(int device, vare_batch_stats* out, int reset);
+/* Test hook (no GPU needed): the merge plan vare_flip_step_batched makes for ntank recorded operation
+ * sequences.  keys holds the sequences one after the other (lens[t] entries for tank t): a kernel as
+ * its id (>= 0; equal ids merge), a readback as -1, an event record as -2.  For the n-th operation
+ * issued, out_tank/out_pos give its tank and its position in that tank's sequence and out_item the
+ * merged launch (or single copy or event) it belongs to.  With the outputs NULL only *nout is set. */
+int vare_debug_batch_plan(const int32_t* keys, const int32_t* lens, int ntank, int32_t* out_tank, int32_t* out_pos,
+                          int32_t* out_item, size_t cap, size_t* nout);
 
 /* FLIPMethod::advanceOneFrame (FLIPMethod.cu:177-206): toVoxel, step(params.dt), postProcess,
  * toWorld. */

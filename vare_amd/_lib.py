@@ -158,6 +158,7 @@ SIGNATURES = {
     "vare_flip_step_noalloc": (C.c_int, [_P, C.c_float]),
     "vare_flip_step_batched": (C.c_int, [C.POINTER(_P), C.c_int, _FP, C.POINTER(C.c_int)]),
     "vare_batch_get_stats": (C.c_int, [C.c_int, C.POINTER(BatchStats), C.c_int]),
+    "vare_debug_batch_plan": (C.c_int, [_P, _P, C.c_int, _P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vare_flip_advance_frame": (C.c_int, [_P]),
     "vare_flip_post_process": (C.c_int, [_P]),
     "vare_flip_export_surface": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t), _FP, _FP]),
@@ -561,6 +562,20 @@ def step_batched(handles, dt):
     st = (C.c_int * n)()
     check(load().vare_flip_step_batched(hs, n, dts, st))
     return list(st)
+
+
+def batch_plan(seqs):
+    """vare_debug_batch_plan: the issue order of recorded operation sequences (kernel ids >= 0, -1 a
+    readback, -2 an event).  Returns (tank, position, merged item) per issued operation."""
+    keys = np.ascontiguousarray(np.concatenate([np.asarray(q, np.int32) for q in seqs]) if seqs else [], np.int32)
+    lens = np.ascontiguousarray([len(q) for q in seqs], np.int32)
+    n = C.c_size_t()
+    check(load().vare_debug_batch_plan(ptr(keys), ptr(lens), len(seqs), None, None, None, 0, C.byref(n)))
+    t = np.empty(n.value, np.int32)
+    p = np.empty(n.value, np.int32)
+    it = np.empty(n.value, np.int32)
+    check(load().vare_debug_batch_plan(ptr(keys), ptr(lens), len(seqs), ptr(t), ptr(p), ptr(it), n.value, C.byref(n)))
+    return t, p, it
 
 
 def batch_stats(device=0, reset=False):

@@ -17,6 +17,7 @@
 #include <thread>
 #include <atomic>
 #include <mutex>
+#include <memory>
 #include <chrono>
 #include <condition_variable>
 #include <unordered_map>
@@ -2572,10 +2573,13 @@ size_t pick_head(const std::vector<vare_flip*>& ms, const std::vector<size_t>& h
 }
 
 // Merges and issues the recorded operations of the tanks in ms, then waits for them.
-hipError_t run_merged(BatchRun& R, const std::vector<vare_flip*>& ms) {
+// The issue order of the tanks' recorded operations: sel lists them, items groups consecutive
+// entries of sel into merged launches (one kernel for several tanks) or single copies and events.
+void plan_merged(const std::vector<vare_flip*>& ms, std::vector<std::pair<vare_flip*, const ROp*>>& sel,
+                 std::vector<MergedItem>& items) {
     std::vector<size_t> head(ms.size(), 0);
-    std::vector<std::pair<vare_flip*, const ROp*>> sel;
-    std::vector<MergedItem> items;
+    sel.clear();
+    items.clear();
     for (;;) {
         const size_t i0 = pick_head(ms, head);
         if (i0 >= ms.size()) break;
@@ -2599,6 +2603,12 @@ hipError_t run_merged(BatchRun& R, const std::vector<vare_flip*>& ms) {
         }
         items.push_back(it);
     }
+}
+
+hipError_t run_merged(BatchRun& R, const std::vector<vare_flip*>& ms) {
+    std::vector<std::pair<vare_flip*, const ROp*>> sel;
+    std::vector<MergedItem> items;
+    plan_merged(ms, sel, items);
     std::vector<unsigned> grid;
     std::vector<const unsigned char*> packs;
     for (const MergedItem& it : items) {
@@ -2816,6 +2826,50 @@ int vare_flip_step_batched(vare_flip* const* hs, int n, const float* dt, int* st
         }
     }
     return first;
+}
+
+int vare_debug_batch_plan(const int32_t* keys, const int32_t* lens, int ntank, int32_t* out_tank, int32_t* out_pos,
+                          int32_t* out_item, size_t cap, size_t* nout) {
+    if (!lens || ntank < 1 || !nout) return fail(VARE_EINVAL, "vare_debug_batch_plan: no sequences");
+    // stand-in tanks whose recorded operations carry only the keys (no device state is touched)
+    std::vector<std::unique_ptr<vare_flip>> tanks;
+    std::vector<vare_flip*> ms;
+    size_t total = 0, off = 0;
+    for (int t = 0; t < ntank; ++t) {
+        if (lens[t] < 0) return fail(VARE_EINVAL, "vare_debug_batch_plan: negative length");
+        tanks.emplace_back(new vare_flip());
+        for (int q = 0; q < lens[t]; ++q) {
+            const int32_t k = keys[off + (size_t)q];
+            ROp op;
+            if (k < 0) {
+                op.kind = k == -1 ? ROp::kCopy : ROp::kEvent;
+            } else {
+                op.kind = ROp::kKernel;
+                op.issue = reinterpret_cast<IssueFn>((uintptr_t)(k + 1) * 16);
+                op.max_tanks = kBatchMaxTanks;
+            }
+            tanks.back()->rops.push_back(op);
+        }
+        off += (size_t)lens[t];
+        total += (size_t)lens[t];
+        ms.push_back(tanks.back().get());
+    }
+    *nout = total;
+    if (!out_tank || !out_pos || !out_item) return VARE_OK;
+    if (cap < total) return fail(VARE_EINVAL, "vare_debug_batch_plan: capacity %zu < %zu", cap, total);
+    std::vector<std::pair<vare_flip*, const ROp*>> sel;
+    std::vector<MergedItem> items;
+    plan_merged(ms, sel, items);
+    for (size_t it = 0; it < items.size(); ++it)
+        for (size_t q = 0; q < items[it].n; ++q) {
+            const auto& pr = sel[items[it].first + q];
+            const size_t e = items[it].first + q;
+            const int t = (int)(std::find(ms.begin(), ms.end(), pr.first) - ms.begin());
+            out_tank[e] = t;
+            out_pos[e] = (int32_t)(pr.second - pr.first->rops.data());
+            out_item[e] = (int32_t)it;
+        }
+    return VARE_OK;
 }
 
 int vare_batch_get_stats(int device, vare_batch_stats* out, int reset) {
