@@ -779,10 +779,25 @@ struct k_mg_bottom_f {
             float* cur = sz + o;
             float* nxt = st + o;
             for (int c = tid; c < n; c += T) cur[c] = (coarse_sweeps > 0 && mg_fluid(in[c])) ? r[c] * mg_wd(in[c], omega) : 0.f;
-            for (int s = 1; s < coarse_sweeps; ++s) {
-                __syncthreads();
-                for (int c = tid; c < n; c += T) nxt[c] = bottom_sweep(in[c], r[c], cur, c, sy, szz, omega);
-                float* tmp = cur; cur = nxt; nxt = tmp;
+            if (n <= 64) {
+                // a coarsest grid of at most 64 cells (one per lane of wave 0, which also wrote its first
+                // sweep above) is swept by that wave alone: LDS reads and writes of one wave complete in
+                // order, so its sweeps need no workgroup barrier; the other waves wait at the barrier below
+                if (tid < 64) {
+                    for (int s = 1; s < coarse_sweeps; ++s) {
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        if (tid < n) nxt[tid] = bottom_sweep(in[tid], r[tid], cur, tid, sy, szz, omega);
+                        float* tmp = cur; cur = nxt; nxt = tmp;
+                    }
+                }
+            } else {
+                for (int s = 1; s < coarse_sweeps; ++s) {
+                    __syncthreads();
+                    for (int c = tid; c < n; c += T) nxt[c] = bottom_sweep(in[c], r[c], cur, c, sy, szz, omega);
+                    float* tmp = cur; cur = nxt; nxt = tmp;
+                }
             }
             zfin = ((coarse_sweeps < 1 ? 1 : coarse_sweeps) & 1) ? sz + o : st + o;
             __syncthreads();
