@@ -639,8 +639,10 @@ __device__ __forceinline__ float bottom_sweep(unsigned in, float r, const float*
 struct k_mg_bottom_f {
     static constexpr int kLB = kMgBottomThreads;
     static constexpr int kWPE = 1;
-    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, MgBottom bt, float omega, float wa, float wb, int nu, int coarse_sweeps,
-                                                                    const int* done, XTerm xt) {
+    // bt by reference: its per-level arrays are indexed by a run-time level, which reads them where they
+    // lie (the kernel-argument segment) instead of copying the struct to private memory
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, const MgBottom& bt, float omega,
+                                               float wa, float wb, int nu, int coarse_sweeps, const int* done, XTerm xt) {
         if (*done) return;
         __shared__ float sr[kMgBottomCells];
         __shared__ float sz[kMgBottomCells];
