@@ -89,7 +89,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--requests", type=int, default=256)
-    ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--start-step", type=int, default=None,
                     help="untimed steps of the simulation before the warm-up and the timed region; default: "
                          "centre a shorter timed region on the configuration's full run")

@@ -1,7 +1,7 @@
 """CPU tests of the merge plan of vare_flip_step_batched (vare_debug_batch_plan): how the operations the
 tanks recorded between two sync points are issued.  Every operation is issued exactly once, each tank's
 operations keep their order, a merged launch holds one kernel and at most one operation per tank (and at
-most 16 tanks), identical sequences merge completely, and a tank with extra or missing operations falls
+most 32 tanks), identical sequences merge completely, and a tank with extra or missing operations falls
 back into step with the others."""
 import numpy as np
 import pytest
@@ -10,7 +10,7 @@ from vare_amd import _lib as L
 
 pytestmark = pytest.mark.skipif(not __import__("os").path.exists(L.LIB_PATH), reason="library not built")
 
-MAX_TANKS = 16
+MAX_TANKS = 32
 
 
 def check_plan(seqs):
@@ -55,7 +55,7 @@ def test_more_tanks_than_one_launch_holds():
     seqs = [[5, 6, 7, -1]] * 40
     _, _, _, items = check_plan(seqs)
     ks = kernel_items(items)
-    assert sorted(len(ops) for ops in ks) == sorted([16, 16, 8] * 3)
+    assert sorted(len(ops) for ops in ks) == sorted([32, 8] * 3)
 
 
 def test_extra_operation_falls_back_in_step():
@@ -81,7 +81,7 @@ def test_uneven_solver_chunks():
 def test_random_sequences_keep_every_guarantee():
     rng = np.random.default_rng(11)
     for trial in range(200):
-        ntank = int(rng.integers(1, 24))
+        ntank = int(rng.integers(1, 48))
         base = list(rng.integers(0, 6, size=int(rng.integers(1, 30))))
         seqs = []
         for _ in range(ntank):

@@ -95,9 +95,10 @@ def test_batched_equals_alone_mixed_tanks():
 
 
 def test_batched_shares_launches():
-    # sixteen tanks of one configuration and uneven sizes: the launches are merged (far fewer merged
-    # launches than the tanks would make alone), and each result equals its tank alone
-    sizes = [32, 40, 48, 56, 64, 32, 40, 48, 56, 64, 32, 48, 64, 40, 56, 72]
+    # twenty-four tanks of one configuration and uneven sizes (more than sixteen, so a merged launch
+    # holds up to 32 tanks): the launches are merged (far fewer merged launches than the tanks would
+    # make alone), and each result equals its tank alone
+    sizes = [32, 40, 48, 56, 64, 32, 40, 48, 56, 64, 32, 48, 64, 40, 56, 72, 32, 40, 48, 56, 64, 40, 48, 32]
     tanks = []
     for k, n in enumerate(sizes):
         p, v = dam(n, n, n, 0.2 + 0.03 * k, 0.7 - 0.02 * k, 4 + k % 5, 100 + k)
@@ -111,7 +112,7 @@ def test_batched_shares_launches():
     bs = L.batch_stats()
     print("batch stats", bs, flush=True)
     assert bs["tank_launches"] > 4 * bs["launches"], bs
-    for i in (0, 7, 15):
+    for i in (0, 7, 15, 23):
         h = make(tanks[i])
         for _ in range(steps):
             h.step(DT)

@@ -15,7 +15,7 @@
 
 namespace vare {
 
-constexpr int kBatchMaxTanks = 16;     // tanks per batched launch (at most 64: one lane of a wave each)
+constexpr int kBatchMaxTanks = 32;     // tanks per batched launch (at most 64: one lane of a wave each)
 
 // The kernel arguments of one launch as a plain aggregate: leaf I holds argument I.  The host
 // (x86-64) and the device (amdgcn) lay it out alike: both follow the Itanium C++ ABI, and every
