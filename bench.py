@@ -33,11 +33,11 @@ central-difference divergence; VARE_FLAG_REPORT_DIVERGENCE).  `vs_plain_gpu` is 
 rates; `vs_baseline` stays null because the reference publishes no number for this metric.
 
 The CPU baseline is the OpenMP build of the oracle (the FP32 restatement of the reference step) on the
-GPU box's host: all threads of the host's share and one thread under the reference's own CG rule, and
+host of the machine the GPU sits in: all threads of the host's share and one thread under the reference's own CG rule, and
 all threads under Jacobi-PCG to 1e-5 relative.  For configs 3 and 5 it times steps of the same scene
 at a quarter of the resolution per axis (a bounded sample; the rate is per particle-step).
 
-Prints one JSON line (driver contract) with `roofline` and `cpu_baseline` objects.
+Prints one JSON line with `roofline` and `cpu_baseline` objects.
 """
 import argparse
 import json
@@ -313,6 +313,7 @@ def run_scene(a, L, cfg, steps, kw=None, timing=True, state=None):
     L.device_synchronize()
 
     step_ms, iters, relres, truerel, refines, nps = [], [], [], [], [], []
+    fluid_launch_cells = 0                  # sum over the timed update launches of the step's fluid cells
     nonconv = 0
     stats = None
     t_start = time.perf_counter()
@@ -329,6 +330,9 @@ def run_scene(a, L, cfg, steps, kw=None, timing=True, state=None):
         step_ms.append((time.perf_counter() - ts) * 1e3)
         stats = h.stats()
         iters.append(stats["cg_iterations"])
+        if timing and s % TIMING_STRIDE == 0:
+            # one update launch ran per PCG iteration of the step
+            fluid_launch_cells += stats["cg_iterations"] * stats["num_fluid"]
         if stats["b_norm"] > 0:
             relres.append(stats["residual_norm"] / stats["b_norm"])
             truerel.append(stats["true_rel_residual"])
@@ -371,6 +375,12 @@ def run_scene(a, L, cfg, steps, kw=None, timing=True, state=None):
         # block face are re-reads.
         roofline = line("k_update_smooth2", "update", 18.0, UPD_TRAFFIC_PER_CELL)
         roofline["timed_on"] = sampled
+        # the same launches counted on fluid cells only (the solver runs over every cell of a block that
+        # holds a fluid cell; the others are no-ops of the stencil descriptor)
+        if kt["update"][2] > 0 and fluid_launch_cells > 0:
+            share = fluid_launch_cells / kt["update"][2]
+            roofline["fluid_cell_share"] = round(share, 4)
+            roofline["frac_on_fluid_cells"] = round(roofline["frac"] * share, 4)
         roofline["others"] = [
             # the level-0 prolongation + Jacobi pass: read z 4 + coarse z 0.5 + r 4 + descriptor 2, write z 4
             line("k_mg0_prolong_smooth", "mg0", 14.5, MG0_TRAFFIC_PER_CELL),
@@ -439,7 +449,7 @@ def cpu_threads(a):
 
 def cpu_baseline(a, L, cfg, h, kw):
     """Time the oracle's OpenMP build (the FP32 restatement of the reference step; loops over
-    particles and cells in parallel, P2G by z-slabs two apart) on the GPU box's host: with every
+    particles and cells in parallel, P2G by z-slabs two apart) on the GPU machine's host: with every
     thread of the host's share and with one thread, under the reference's own solver rule
     (unpreconditioned CG, |r| < 1e-3 or 202 products), and with every thread also under the configured
     rule (Jacobi-PCG to 1e-5 relative).  A bounded sample of the workload: config 1 all its steps from
@@ -604,8 +614,8 @@ def main():
         copy_calibration(a.copy_calibration)
         return
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU "
-              "sharding design is described in DESIGN.md)" % a.gpus, file=sys.stderr)
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the step is not sharded "
+              "across GPUs)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     from vare_amd import _lib as L
 

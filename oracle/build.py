@@ -14,7 +14,7 @@ SRC = os.path.join(HERE, "flip_oracle.cpp")
 OUT_DIR = os.path.join(HERE, "lib")
 OUT = os.path.join(OUT_DIR, "libvare_oracle.so")
 # the multi-core CPU baseline: the same source with OpenMP (parallel loops, slab-coloured P2G).
-# x86-64-v3 (AVX2) rather than -march=native: the library is built here and runs on the GPU box's
+# x86-64-v3 (AVX2) rather than -march=native: the library is built on one machine and runs on another's
 # host, whose CPU model may differ
 OUT_OMP = os.path.join(OUT_DIR, "libvare_oracle_omp.so")
 
