@@ -1071,7 +1071,7 @@ struct k_fin_ir_f {
     static constexpr int kLB = kFinThreads;
     static constexpr int kWPE = 1;
     static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol,
-                             float inner_tol) {
+                             float inner_tol, int cont) {
         __shared__ double sh[kFinWaves];
         const double tot = fin_sum(partials, n, sh);
         if (threadIdx.x != 0) return;
@@ -1082,21 +1082,24 @@ struct k_fin_ir_f {
             st->done = 1;
             return;
         }
-        const double need = 0.5 * (double)outer_tol * sqrt(st->ir_bb / tot);
+        // the next solve's tolerance: what the target still needs, with a margin for the FP32 recursive
+        // residual running ahead of the true one (0.8: measured at 256^3, worst final residual 0.77x target)
+        const double need = 0.8 * (double)outer_tol * sqrt(st->ir_bb / tot);
         st->rel_tol = (float)fmin(0.5, fmax((double)inner_tol, need));
         st->bb = tot;
         st->rr = tot;
+        st->converged = 0;
+        st->done = 0;
+        if (cont) return;             // the solve goes on: iteration count and r.z of its last residual kept
         st->rsold = (float)tot;
         st->rzold = (float)tot;
         st->iter = 0;
         st->count = 0;
-        st->converged = 0;
-        st->done = 0;
     }
 };
 __global__ void __launch_bounds__(kFinThreads) k_fin_ir(CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol,
-                         float inner_tol) {
-    k_fin_ir_f::run(blockIdx.x, gridDim.x, st, partials, n, outer_tol, inner_tol);
+                         float inner_tol, int cont) {
+    k_fin_ir_f::run(blockIdx.x, gridDim.x, st, partials, n, outer_tol, inner_tol, cont);
 }
 
 // |b - A x|^2 in FP64 from the pressure x (FP32, or FP64 when refining); reported with every solve.

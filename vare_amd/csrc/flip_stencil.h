@@ -146,8 +146,8 @@ __device__ __forceinline__ float apply_row(unsigned in, float own, const float* 
 
 // Iterative refinement's FP64 residual r = b - A x64 over the solver's blocks (diag * x first, then
 // the fluid neighbours subtracted in the order -x,+x,-y,+y,-z,+z), handing the FP32 rounding of r to
-// the next correction solve as its right-hand side and first direction (x = 0, p = r), with one
-// partial |r|^2 per workgroup.
+// the next correction solve as its right-hand side and first direction (x = 0, p = r; p == nullptr
+// when the solve continues with its last direction instead), with one partial |r|^2 per workgroup.
 // Not gated by the solver flag: it runs between solves.
 template <int B, int K>
 struct k_ir_residual_blk_f {
@@ -211,7 +211,7 @@ struct k_ir_residual_blk_f {
             const int c = s[kb] * B3 + l;
             r32[c] = rf;
             x[c] = 0.f;
-            p[c] = rf;
+            if (p) p[c] = rf;
             rr += r * r;
         }
         const double tot = fine_block_sum<B>(rr, sh);
