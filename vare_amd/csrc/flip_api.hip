@@ -137,6 +137,7 @@ struct vare_flip {
     double* partials2 = nullptr;
     float* p2gpart = nullptr;       // P2G: each block's incoming +1-layer partial sums (5 per cell)
     int in_step = 0;                // inside vare_flip_step: the binning leaves the adjacency in place
+    int reorder_pending = 0;        // the binning left the move into bin order to P2G's first pass
 
     // multigrid preconditioner: level 0 aliases r/info/z of the CG; zt0 is its second buffer
     std::vector<MgLevelK> mg;
@@ -620,6 +621,7 @@ constexpr int kSortLongGrid = 512;   // workgroups of k_sort_long (strides over 
 
 template <int B> int bin_particles(vare_flip* h) {
     const int nc = (int)h->ncells();
+    h->reorder_pending = 0;
     CHK(fill_u32(h, h->ccount, 0, (size_t)nc + 2));
     CHK(fill_u32(h, h->dint + 4, 0, 2));           // [4] sort overflow, [5] long-bin count
     CHK(fill_u32(h, h->dint + 7, 0, 1));           // [7] mid-bin count
@@ -640,9 +642,14 @@ template <int B> int bin_particles(vare_flip* h) {
                            (const int*)h->midbins, (const int*)(h->dint + 7), h->perm);
         vlaunch<k_sort_long_f>(h, k_sort_long, dim3(kSortLongGrid), dim3(1024), 0, (const int*)h->cstart,
                            (const int*)h->longbins, (const int*)(h->dint + 5), h->perm, h->dint + 4);
-        // move the particles into bin order (double-buffered) and keep their logical indices
-        // inside vare_flip_step the adjacency is rewritten by G2P for every particle before anything
-        // reads it, so it is not moved there
+        // move the particles into bin order (double-buffered) and keep their logical indices.
+        // Inside vare_flip_step P2G's first pass does it as it stages each block's particles (and the
+        // adjacency, which G2P rewrites for every particle before anything reads it, is not moved)
+        if (h->in_step && h->prm.particle_path == 0) {
+            h->reorder_pending = 1;
+            HIPCHK(hipGetLastError());
+            return VARE_OK;
+        }
         ReorderArrays ra;
         for (int a = 0; a < 7; ++a) { ra.src[a] = h->pa[a]; ra.dst[a] = h->pb[a]; }
         ra.vsz = (int)h->vsz();
@@ -695,25 +702,46 @@ template <int B> int stage_p2g(vare_flip* h) {
     if (h->prm.particle_path == 0) {
         // two passes over the blocks: own particles into the block and its +1 layer, then the face
         // cells; every cell of every active block is written, normalised and classified
+        const bool ro = h->reorder_pending != 0;
+        P2GReorder rk{};
+        if (ro) {
+            rk.perm = (const unsigned long long*)h->perm;
+            for (int a = 0; a < 3; ++a) rk.pos[a] = h->pb[a];
+            rk.vel = VelPtr{{h->pb[3], h->pb[4], h->pb[5]}, h->vhalf};
+            rk.pid = h->pid2;
+        }
         tbegin(h, VARE_KERNEL_P2G);
         if (h->nb) {
             const FrameK fk = h->frame();
-            if (h->vhalf)
-                vlaunch<k_p2g_own_f<B, true>>(h, k_p2g_own<B, true>, dim3(h->nb), dim3(P2GOwn<B>::THREADS), 0,
-                                   (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
-                                   (const int*)h->cstart, fk, cf, h->cx, h->cy, h->cz, h->type, h->colk(),
-                                   h->prm.col_vel_scale, h->p2gpart);
-            else
-                vlaunch<k_p2g_own_f<B, false>>(h, k_p2g_own<B, false>, dim3(h->nb), dim3(P2GOwn<B>::THREADS), 0,
-                                   (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(),
-                                   (const int*)h->cstart, fk, cf, h->cx, h->cy, h->cz, h->type, h->colk(),
-                                   h->prm.col_vel_scale, h->p2gpart);
+#define VARE_P2G_OWN(VH, RO)                                                                                    \
+    vlaunch<k_p2g_own_f<B, VH, RO>>(h, k_p2g_own<B, VH, RO>, dim3(h->nb), dim3(P2GOwn<B>::THREADS), 0,            \
+                                    (const float*)h->pa[0], (const float*)h->pa[1], (const float*)h->pa[2], h->vel(), \
+                                    (const int*)h->cstart, fk, cf, h->cx, h->cy, h->cz, h->type, h->colk(),         \
+                                    h->prm.col_vel_scale, h->p2gpart, rk)
+            if (h->vhalf) { if (ro) VARE_P2G_OWN(true, true); else VARE_P2G_OWN(true, false); }
+            else { if (ro) VARE_P2G_OWN(false, true); else VARE_P2G_OWN(false, false); }
+#undef VARE_P2G_OWN
             vlaunch<k_p2g_faces_f<B>>(h, k_p2g_faces<B>, dim3(h->nb), dim3(P2GOwn<B>::FACE_THREADS), 0, fk, cf, h->cx, h->cy,
                                h->cz, h->type, h->colk(), h->prm.col_vel_scale, (const float*)h->p2gpart);
         }
         // algorithmic bytes of both passes: positions and velocities read once (24 B per particle, 18 with
-        // FP16 velocities), mass, velocity, count, projected velocity and type written per cell (33 B)
-        tend(h, VARE_KERNEL_P2G, (int64_t)h->np * (h->vhalf ? 18 : 24) + (int64_t)nc * 33);
+        // FP16 velocities), mass, velocity, count, projected velocity and type written per cell (33 B);
+        // moving the particles into bin order adds the permutation read (8 B) and the positions,
+        // velocities and logical index written (28 B, 22 with FP16 velocities)
+        const int64_t pb = (h->vhalf ? 18 : 24) + (ro ? (h->vhalf ? 30 : 36) : 0);
+        tend(h, VARE_KERNEL_P2G, (int64_t)h->np * pb + (int64_t)nc * 33);
+        if (ro) {
+            // the particles outside the domain (the last bin) are moved by a small strided kernel
+            ReorderArrays ra;
+            for (int a = 0; a < 7; ++a) { ra.src[a] = h->pa[a]; ra.dst[a] = h->pb[a]; }
+            ra.vsz = (int)h->vsz();
+            ra.n = 6;
+            vlaunch<k_reorder_tail_f>(h, k_reorder_tail, dim3(64), dim3(WG), 0, (const int*)h->cstart + nc, (int)h->np,
+                               (const unsigned long long*)h->perm, ra, h->pid2);
+            for (int a = 0; a < 6; ++a) std::swap(h->pa[a], h->pb[a]);
+            std::swap(h->pid, h->pid2);
+            h->reorder_pending = 0;
+        }
         HIPCHK(hipGetLastError());
         return VARE_OK;
     }

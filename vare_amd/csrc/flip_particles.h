@@ -380,6 +380,34 @@ __global__ void k_reorder(int np, const unsigned long long* __restrict__ perm, R
     k_reorder_f::run(blockIdx.x, gridDim.x, np, perm, ra, pid_out);
 }
 
+// k_reorder over the particles from *first (read on the device) to np: the bin of the particles outside
+// the domain, which no block's P2G workgroup covers when P2G moves the particles itself.  A fixed grid
+// strides over the range (usually empty).
+struct k_reorder_tail_f {
+    static constexpr int kLB = WG;
+    static constexpr int kWPE = 1;
+    static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, const int* __restrict__ first, int np,
+                                               const unsigned long long* __restrict__ perm, ReorderArrays ra,
+                                               int* __restrict__ pid_out) {
+        for (int q = *first + (int)(vbid * WG + threadIdx.x); q < np; q += (int)(vnbid * WG)) {
+            const unsigned long long k = perm[q];
+            const int o = (int)(unsigned)(k & 0xffffffffull);
+            pid_out[q] = (int)(k >> 32);
+    #pragma unroll
+            for (int a = 0; a < 7; ++a) {
+                if (a == 6 && ra.n < 7) break;
+                if (a >= 3 && a < 6 && ra.vsz == 2) ((_Float16*)ra.dst[a])[q] = ((const _Float16*)ra.src[a])[o];
+                else ((float*)ra.dst[a])[q] = ((const float*)ra.src[a])[o];
+            }
+        }
+    }
+};
+__global__ void __launch_bounds__(WG) k_reorder_tail(const int* __restrict__ first, int np,
+                                                     const unsigned long long* __restrict__ perm, ReorderArrays ra,
+                                                     int* __restrict__ pid_out) {
+    k_reorder_tail_f::run(blockIdx.x, gridDim.x, first, np, perm, ra, pid_out);
+}
+
 // Slots of the 3x3x3 block neighbourhood of block slot s (index (dz+1)*9+(dy+1)*3+(dx+1)),
 // -1 when inactive or outside the frame.
 __device__ __forceinline__ void load_nbhd(const FrameK& f, int s, int* nsl) {
@@ -474,7 +502,18 @@ __device__ __forceinline__ void p2g_finish(int c, int i, int j, int k, float m, 
     type[c] = ty8;
 }
 
-template <int B, bool VH>
+// The reorder into bin order done by P2G's first pass (RO = true): each staged slot q of the bin-order
+// range takes the particle perm[q] names (its previous physical index below, its logical index above)
+// from the previous arrays, which the pass reads, and writes it, with its logical index, at q of the
+// next arrays.  This is k_reorder's work without its separate pass over the particles.
+struct P2GReorder {
+    const unsigned long long* perm;
+    float* pos[3];
+    VelPtr vel;
+    int* pid;
+};
+
+template <int B, bool VH, bool RO = false>
 struct k_p2g_own_f {
     static constexpr int kLB = P2GOwn<B>::THREADS;
     static constexpr int kWPE = 1;
@@ -482,7 +521,7 @@ struct k_p2g_own_f {
             const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, VelPtr uv,
             const int* __restrict__ cstart, FrameK f, CellFields cf,
             float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz, uint8_t* __restrict__ type, ColK col,
-            float col_scale, float* __restrict__ part) {
+            float col_scale, float* __restrict__ part, P2GReorder ro) {
         using P = P2GOwn<B>;
         constexpr int B3 = P::B3, T = P::T, T3 = P::T3, C = P::C, R = C / 64;
         const VelT<VH> u = vel_typed<VH>(uv);
@@ -504,12 +543,38 @@ struct k_p2g_own_f {
         for (int c = 0; c < 8; ++c) { am[c] = 0.f; ax[c] = 0.f; ay[c] = 0.f; az[c] = 0.f; }
         for (int c0 = w0; c0 < w1; c0 += C) {
             float r[R][6];
+            if constexpr (RO) {
+                unsigned long long kk[R];
     #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const int n = c0 + k * 64 + lane;
-                const unsigned m = (unsigned)(n < w1 ? n : w0);           // lanes past the range re-read the first
-                r[k][0] = ld_u32(px, m); r[k][1] = ld_u32(py, m); r[k][2] = ld_u32(pz, m);
-                r[k][3] = u.ldu(0, m); r[k][4] = u.ldu(1, m); r[k][5] = u.ldu(2, m);
+                for (int k = 0; k < R; ++k) {
+                    const int n = c0 + k * 64 + lane;
+                    kk[k] = ro.perm[n < w1 ? n : w0];                     // lanes past the range re-read the first
+                }
+    #pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const unsigned m = (unsigned)(kk[k] & 0xffffffffull);  // previous physical index
+                    r[k][0] = ld_u32(px, m); r[k][1] = ld_u32(py, m); r[k][2] = ld_u32(pz, m);
+                    r[k][3] = u.ldu(0, m); r[k][4] = u.ldu(1, m); r[k][5] = u.ldu(2, m);
+                }
+                const VelT<VH> ud = vel_typed<VH>(ro.vel);
+    #pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int n = c0 + k * 64 + lane;
+                    if (n < w1) {
+                        const unsigned un = (unsigned)n;
+                        st_u32(ro.pos[0], un, r[k][0]); st_u32(ro.pos[1], un, r[k][1]); st_u32(ro.pos[2], un, r[k][2]);
+                        ud.stu(0, un, r[k][3]); ud.stu(1, un, r[k][4]); ud.stu(2, un, r[k][5]);
+                        st_u32(ro.pid, un, (int)(kk[k] >> 32));
+                    }
+                }
+            } else {
+    #pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int n = c0 + k * 64 + lane;
+                    const unsigned m = (unsigned)(n < w1 ? n : w0);           // lanes past the range re-read the first
+                    r[k][0] = ld_u32(px, m); r[k][1] = ld_u32(py, m); r[k][2] = ld_u32(pz, m);
+                    r[k][3] = u.ldu(0, m); r[k][4] = u.ldu(1, m); r[k][5] = u.ldu(2, m);
+                }
             }
     #pragma unroll
             for (int k = 0; k < R; ++k)
@@ -589,13 +654,14 @@ struct k_p2g_own_f {
         }
     }
 };
-template <int B, bool VH>
+template <int B, bool VH, bool RO = false>
 __global__ void __launch_bounds__(P2GOwn<B>::THREADS) k_p2g_own(
         const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz, VelPtr uv,
         const int* __restrict__ cstart, FrameK f, CellFields cf,
         float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz, uint8_t* __restrict__ type, ColK col,
-        float col_scale, float* __restrict__ part) {
-    k_p2g_own_f<B, VH>::run(blockIdx.x, gridDim.x, px, py, pz, uv, cstart, f, cf, cx, cy, cz, type, col, col_scale, part);
+        float col_scale, float* __restrict__ part, P2GReorder ro) {
+    k_p2g_own_f<B, VH, RO>::run(blockIdx.x, gridDim.x, px, py, pz, uv, cstart, f, cf, cx, cy, cz, type, col, col_scale,
+                                part, ro);
 }
 
 template <int B>
