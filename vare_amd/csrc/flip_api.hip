@@ -608,6 +608,8 @@ struct k_cg_setup_f {
         st->iter = 0;
         st->count = 0;
         st->converged = 0;
+        st->ir_rounds = 0;
+        st->cont = 0;
     }
 };
 __global__ void k_cg_setup(CgState* st, int mode, int max_count, float abs_tol, float rel_tol, int precond) {
@@ -1171,8 +1173,13 @@ template <int B> void enqueue_iteration_k(vare_flip* h, int k) {
 // changes slowly from step to step: the count of the same solve (the k-th of the step) in the
 // previous step is enqueued before the first host check, then small chunks; iterations enqueued
 // past convergence return at once.  k0 > 0 continues a solve after its first k0 iterations (their
-// direction p and r.z kept on the device, the iteration count too).
-template <int B> int run_solver(vare_flip* h, int cap, int k, int& total_iters, int k0 = 0) {
+// direction p and r.z kept on the device, the iteration count too).  With `round` (FP64 refinement)
+// a refinement round is enqueued behind every chunk; it runs only once the solve is done, so the host
+// reads the state once per round, after the round, and the solve's count is the one the round saved.
+struct IrRound;
+template <int B> void enqueue_ir_round(vare_flip* h, const IrRound& ir);
+template <int B> int run_solver(vare_flip* h, int cap, int k, int& total_iters, int k0 = 0,
+                                const IrRound* round = nullptr) {
     int& last = h->last_iters[std::min(k, 19)];
     const size_t ev_from = h->ev_used;
     int launched = 0;
@@ -1183,19 +1190,47 @@ template <int B> int run_solver(vare_flip* h, int cap, int k, int& total_iters, 
     if (h->rec) chunk = std::max(chunk, batch_chunk_hint(h, k));
     // the first chunk is enqueued without a check: a solve that is done at its set-up returns at
     // the device flag in every kernel
+    const int rounds0 = h->st_host->ir_rounds;
     for (;;) {
         for (int q = 0; q < chunk; ++q) enqueue_iteration<B>(h, k0 + launched + q);
+        if (round) enqueue_ir_round<B>(h, *round);
         HIPCHK(hipGetLastError());
         launched += chunk;
         chunk = last > 0 ? std::max(2, last / 8) : std::min(chunk * 2, 64);
         CHK(hread(h, h->st_host, h->st, sizeof(CgState)));
-        if (h->st_host->done) break;
+        if (round ? h->st_host->ir_rounds != rounds0 : h->st_host->done != 0) break;
         if (launched > cap + 1) break;                       // defensive: the device flag ends first
     }
-    last = h->st_host->count - k0;
-    total_iters += h->st_host->count - k0;
-    tdrop_noop(h, ev_from, h->st_host->iter);
+    const int count = round ? h->st_host->last_count : h->st_host->count;
+    last = count - k0;
+    total_iters += count - k0;
+    tdrop_noop(h, ev_from, round ? h->st_host->last_iter : h->st_host->iter);
     return VARE_OK;
+}
+
+// One FP64 refinement round, gated on the solve's done flag: x64 += x (with the last CG term), the FP64
+// residual into both r buffers (x = 0), and k_fin_ir, which stops the refinement at its target or sets
+// up the next FP32 solve: continued (a reliable update) or restarted.
+struct IrRound {
+    XTail tail;
+    int cont_allowed;
+};
+template <int B> void enqueue_ir_round(vare_flip* h, const IrRound& ir) {
+    const int nv = h->nvirt();
+    const unsigned gv = rgrid((size_t)nv);
+    const int* done = done_ptr(h);
+    // over the solver's blocks only: the other blocks hold no fluid cell, and x and x64 stay 0 there
+    vlaunch<k_ir_accumulate_f>(h, k_ir_accumulate, dim3(gv), dim3(WG), 0, nv, h->clist(), h->x64, (const float*)h->x,
+                               h->x, 0, ir.tail, done);
+    const int nwg_ir = nwg_k(h, kIterK);
+    if (nwg_ir)
+        vlaunch<k_ir_residual_blk_f<B, kIterK>>(h, k_ir_residual_blk<B, kIterK>, dim3(nwg_ir), dim3(B * B * B), 0,
+                           (const int*)h->wgtab, (const float*)h->b, (const double*)h->x64,
+                           (const uint16_t*)h->info, h->r, h->x, (float*)nullptr, h->partials, h->r2, done);
+    // the solve continues when at most kContMaxReduction of reduction is left (see stage_solve)
+    constexpr float kContMaxReduction = 1e4f;
+    vlaunch<k_fin_ir_f>(h, k_fin_ir, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials, nwg_ir,
+                        h->prm.refine_tol, h->prm.cg_rel_tol, ir.cont_allowed, kContMaxReduction);
 }
 
 template <int B> int stage_solve(vare_flip* h) {
@@ -1212,53 +1247,35 @@ template <int B> int stage_solve(vare_flip* h) {
     // the fused multigrid iteration leaves the last iteration's x += alpha p to after the solve (XTail)
     const bool fused = mode == 1 && h->prm.preconditioner == VARE_PRECOND_MG && h->prm.mg_sweeps == 2;
     const XTail tail{fused ? (const float*)h->p : nullptr, (const float*)h->p2, (const CgState*)h->st};
-    CHK(run_solver<B>(h, cap, 0, total_iters));
+    const IrRound ir{tail, fused ? 1 : 0};
+    h->st_host->ir_rounds = 0;               // k_cg_setup cleared the device's count; no round has run
+    CHK(run_solver<B>(h, cap, 0, total_iters, 0, refine ? &ir : nullptr));
     if (!refine && fused && nv)
         vlaunch<k_cg_xflush_f>(h, k_cg_xflush, dim3(gv), dim3(WG), 0, nv, h->clist(), h->x, tail);
     if (refine) {
         // FP64 iterative refinement: x64 += d, r = b - A x64 in FP64, then the FP32 solve goes on with r.
         // With the fused multigrid iteration the solve may continue (a reliable update): the true
-        // residual replaces the recursive one in the buffer the next iteration reads, the V-cycle gives
-        // its z and r.z, beta = r.z / (r.z of the last recursive residual), and the next iteration's
-        // direction is z + beta p with the last direction p kept, so the Krylov directions built so far
-        // are not thrown away.  That pays when the next solve has a modest reduction left to make (at
-        // 256^3 with a 1e-5 target: 12.8 -> 12.1 iterations per step); when it must reduce the residual
-        // by more than kContMaxReduction (the 512^3 pool at 1e-8: 14.4 -> 15.3), a fresh start (p = z)
-        // converges faster, so each correction solve A d = r starts afresh.  The reduction is estimated
-        // from the recursive residual the host has just read (the true one is within FP32 drift of it).
-        constexpr double kContMaxReduction = 1e4;
-        int kbase = h->st_host->iter;        // iterations of the solve so far (the next one's index)
+        // residual replaces the recursive one in the r buffers, the V-cycle gives its z and r.z,
+        // beta = r.z / (r.z of the last recursive residual), and the next iteration's direction is
+        // z + beta p with the last direction p kept, so the Krylov directions built so far are not
+        // thrown away.  That pays when the next solve has a modest reduction left to make (at 256^3 with
+        // a 1e-5 target: 12.8 -> 12.1 iterations per step); when it must reduce the residual by more
+        // than 1e4 (the 512^3 pool at 1e-8: 14.4 -> 15.3), a fresh start (p = z) converges faster, so
+        // each correction solve A d = r starts afresh.  k_fin_ir decides, from the solve's last
+        // recursive residual (the true one is within FP32 drift of it).
         for (;;) {
-            // over the solver's blocks only: the other blocks hold no fluid cell, and x and x64 stay 0 there
-            vlaunch<k_ir_accumulate_f>(h, k_ir_accumulate, dim3(gv), dim3(WG), 0, nv, h->clist(), h->x64,
-                               (const float*)h->x, h->x, 0, tail);
-            ++outer;
-            const double rr_rec = h->st_host->rr, bb0 = h->st_host->ir_bb;
-            const bool cont = fused && kbase > 0 && rr_rec > 0.0 &&
-                              (double)h->prm.refine_tol * std::sqrt(bb0 / rr_rec) * kContMaxReduction >= 1.0;
-            float* const rtrue = cont && (kbase & 1) ? h->r2 : h->r;     // the r buffer of iteration kbase
-            const int nwg_ir = nwg_k(h, kIterK);
-            if (nwg_ir)
-                vlaunch<k_ir_residual_blk_f<B, kIterK>>(h, k_ir_residual_blk<B, kIterK>, dim3(nwg_ir), dim3(B * B * B), 0,
-                                   (const int*)h->wgtab, (const float*)h->b, (const double*)h->x64,
-                                   (const uint16_t*)h->info, rtrue, h->x, cont ? (float*)nullptr : h->p,
-                                   h->partials);
-            vlaunch<k_fin_ir_f>(h, k_fin_ir, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials,
-                               nwg_ir, h->prm.refine_tol, h->prm.cg_rel_tol, cont ? 1 : 0);
-            HIPCHK(hipGetLastError());
-            CHK(hread(h, h->st_host, h->st, sizeof(CgState)));
+            ++outer;                          // the round behind the last chunk has run
             if (h->st_host->ir_done || outer >= h->prm.refine_max) break;
-            if (cont) {
-                enqueue_vcycle(h, h->partials, rtrue);
+            const int k0 = h->st_host->iter;  // continuing: the solve's iterations so far; restarting: 0
+            if (h->st_host->cont) {
+                enqueue_vcycle(h, h->partials, (k0 & 1) ? h->r2 : h->r);
                 vlaunch<k_fin_beta_f>(h, k_fin_beta, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials,
                                       nwg_k(h, mg_block_k<B>()));
-                CHK(run_solver<B>(h, cap, outer, total_iters, kbase));
             } else {
                 if (h->prm.preconditioner == VARE_PRECOND_MG) enqueue_vcycle(h);
                 enqueue_precond_init(h);
-                CHK(run_solver<B>(h, cap, outer, total_iters));
             }
-            kbase = h->st_host->iter;
+            CHK(run_solver<B>(h, cap, outer, total_iters, k0, &ir));
         }
     }
     // FP64 recompute of the final residual from the pressure the gradient will use, reported with

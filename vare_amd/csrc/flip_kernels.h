@@ -412,6 +412,10 @@ struct CgState {
     double ir_rr;     // |b - A x64|^2 of the last refinement step
     double true_rr;   // |b - A x|^2 recomputed in FP64 from the final FP32 pressure
     double div_bb;    // |b(v_post)|^2: the divergence rule applied to the projected velocity
+    int ir_rounds;    // refinement rounds run (k_fin_ir) since the solve's set-up
+    int cont;         // the last round continues the FP32 solve (1) or restarts it (0)
+    int last_count;   // count and iter of the FP32 solve the last round followed
+    int last_iter;
 };
 
 // Cells visited by the solver's vector kernels: the cells of the listed blocks (those holding a
@@ -1014,12 +1018,14 @@ struct XTail {
     }
 };
 
-// x64 += d, with d the correction solve's x (its last CG term added, see XTail).
+// x64 += d, with d the correction solve's x (its last CG term added, see XTail).  With done != nullptr
+// it runs only once the solve is done (it is enqueued behind each chunk of iterations).
 struct k_ir_accumulate_f {
     static constexpr int kLB = 1024;
     static constexpr int kWPE = 1;
     static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, int nv, CellList cl, double* __restrict__ x64, const float* __restrict__ d,
-                                    float* __restrict__ x32, int last, XTail tail) {
+                                    float* __restrict__ x32, int last, XTail tail, const int* done) {
+        if (done && !*done) return;
         for (int vi = vbid * WG + threadIdx.x; vi < nv; vi += vnbid * WG) {
             const int c = cl.cell(vi);
             const double v = x64[c] + (double)tail.apply(d[c], c);
@@ -1029,8 +1035,8 @@ struct k_ir_accumulate_f {
     }
 };
 __global__ void k_ir_accumulate(int nv, CellList cl, double* __restrict__ x64, const float* __restrict__ d,
-                                float* __restrict__ x32, int last, XTail tail) {
-    k_ir_accumulate_f::run(blockIdx.x, gridDim.x, nv, cl, x64, d, x32, last, tail);
+                                float* __restrict__ x32, int last, XTail tail, const int* done) {
+    k_ir_accumulate_f::run(blockIdx.x, gridDim.x, nv, cl, x64, d, x32, last, tail, done);
 }
 
 // x = x + alpha_k p_k of the last iteration (XTail) when the solve is not refined.
@@ -1071,10 +1077,14 @@ struct k_fin_ir_f {
     static constexpr int kLB = kFinThreads;
     static constexpr int kWPE = 1;
     static __device__ __forceinline__ void run(const unsigned vbid, const unsigned vnbid, CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol,
-                             float inner_tol, int cont) {
+                             float inner_tol, int cont_allowed, float max_reduction) {
         __shared__ double sh[kFinWaves];
+        if (!st->done) return;                    // the solve is still running: this round comes later
         const double tot = fin_sum(partials, n, sh);
         if (threadIdx.x != 0) return;
+        st->ir_rounds += 1;
+        st->last_count = st->count;
+        st->last_iter = st->iter;
         st->ir_rr = tot;
         st->true_rr = tot;            // |b - A x64|^2 of the current FP64 pressure, in FP64
         if (sqrt(tot) <= (double)outer_tol * sqrt(st->ir_bb) || tot == 0.0) {
@@ -1082,6 +1092,12 @@ struct k_fin_ir_f {
             st->done = 1;
             return;
         }
+        // continue the FP32 solve when at most max_reduction is left to gain, estimated from its last
+        // recursive residual (the true one is within FP32 drift of it)
+        const double rr_rec = st->rr;
+        const int cont = cont_allowed && st->iter > 0 && rr_rec > 0.0 &&
+                         (double)outer_tol * sqrt(st->ir_bb / rr_rec) * (double)max_reduction >= 1.0;
+        st->cont = cont;
         // the next solve's tolerance: what the target still needs, with a margin for the FP32 recursive
         // residual running ahead of the true one (0.8: measured at 256^3, worst final residual 0.77x target)
         const double need = 0.8 * (double)outer_tol * sqrt(st->ir_bb / tot);
@@ -1098,8 +1114,8 @@ struct k_fin_ir_f {
     }
 };
 __global__ void __launch_bounds__(kFinThreads) k_fin_ir(CgState* __restrict__ st, const double* __restrict__ partials, int n, float outer_tol,
-                         float inner_tol, int cont) {
-    k_fin_ir_f::run(blockIdx.x, gridDim.x, st, partials, n, outer_tol, inner_tol, cont);
+                         float inner_tol, int cont_allowed, float max_reduction) {
+    k_fin_ir_f::run(blockIdx.x, gridDim.x, st, partials, n, outer_tol, inner_tol, cont_allowed, max_reduction);
 }
 
 // |b - A x|^2 in FP64 from the pressure x (FP32, or FP64 when refining); reported with every solve.

@@ -146,9 +146,9 @@ __device__ __forceinline__ float apply_row(unsigned in, float own, const float* 
 
 // Iterative refinement's FP64 residual r = b - A x64 over the solver's blocks (diag * x first, then
 // the fluid neighbours subtracted in the order -x,+x,-y,+y,-z,+z), handing the FP32 rounding of r to
-// the next correction solve as its right-hand side and first direction (x = 0, p = r; p == nullptr
-// when the solve continues with its last direction instead), with one partial |r|^2 per workgroup.
-// Not gated by the solver flag: it runs between solves.
+// the next FP32 solve (both r buffers, x = 0; p = r when p != nullptr), with one partial |r|^2 per
+// workgroup.  It runs between solves: with done != nullptr only once the solve is done (it is enqueued
+// behind each chunk of iterations).
 template <int B, int K>
 struct k_ir_residual_blk_f {
     static constexpr int kLB = Fine<B>::B3;
@@ -158,12 +158,15 @@ struct k_ir_residual_blk_f {
                                                                     const double* __restrict__ x64,
                                                                     const uint16_t* __restrict__ info,
                                                                     float* __restrict__ r32, float* __restrict__ x,
-                                                                    float* __restrict__ p, double* __restrict__ partials) {
+                                                                    float* __restrict__ p, double* __restrict__ partials,
+                                                                    float* __restrict__ r32b, const int* done) {
         constexpr int B3 = Fine<B>::B3;
         __shared__ double sx[K][B3];
         __shared__ double sh[Fine<B>::NW];
+        const int dn = done ? *done : 1;
         int s[K], nb[K][6];
         wg_load<K>(vbid, vnbid, tab, s, nb);
+        if (!dn) return;
         const int l = threadIdx.x;
         int side[3], fl[3];
         thread_faces<B>(l, side, fl);
@@ -210,6 +213,7 @@ struct k_ir_residual_blk_f {
             const float rf = (float)r;
             const int c = s[kb] * B3 + l;
             r32[c] = rf;
+            if (r32b) r32b[c] = rf;
             x[c] = 0.f;
             if (p) p[c] = rf;
             rr += r * r;
@@ -224,8 +228,9 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_ir_residual_blk(const int* __re
                                                                 const double* __restrict__ x64,
                                                                 const uint16_t* __restrict__ info,
                                                                 float* __restrict__ r32, float* __restrict__ x,
-                                                                float* __restrict__ p, double* __restrict__ partials) {
-    k_ir_residual_blk_f<B, K>::run(blockIdx.x, gridDim.x, tab, b, x64, info, r32, x, p, partials);
+                                                                float* __restrict__ p, double* __restrict__ partials,
+                                                                float* __restrict__ r32b, const int* done) {
+    k_ir_residual_blk_f<B, K>::run(blockIdx.x, gridDim.x, tab, b, x64, info, r32, x, p, partials, r32b, done);
 }
 
 // Storage indices of the six neighbours of cell l of block s (cell_neighbor's values: -1 inactive, -2
