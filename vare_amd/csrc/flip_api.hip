@@ -1226,7 +1226,7 @@ template <int B> void enqueue_ir_round(vare_flip* h, const IrRound& ir) {
     if (nwg_ir)
         vlaunch<k_ir_residual_blk_f<B, kIterK>>(h, k_ir_residual_blk<B, kIterK>, dim3(nwg_ir), dim3(B * B * B), 0,
                            (const int*)h->wgtab, (const float*)h->b, (const double*)h->x64,
-                           (const uint16_t*)h->info, h->r, h->x, (float*)nullptr, h->partials, h->r2, done);
+                           (const uint16_t*)h->info, h->r, h->x, h->partials, h->r2, done);
     // the solve continues when at most kContMaxReduction of reduction is left (see stage_solve)
     constexpr float kContMaxReduction = 1e4f;
     vlaunch<k_fin_ir_f>(h, k_fin_ir, dim3(1), dim3(kFinThreads), 0, h->st, (const double*)h->partials, nwg_ir,

@@ -998,9 +998,9 @@ namespace vare {
 
 // ------------------------------------------------------------------------------------------------
 // Mixed-precision projection: iterative refinement.  The pressure accumulates in FP64 (x64); each
-// outer step recomputes the residual r = b - A x64 in FP64 (k_ir_residual_blk), hands its FP32
-// rounding to the FP32 PCG as the right-hand side of a correction solve A d = r (d starts at 0), then
-// x64 += d.
+// round adds the FP32 solve's x to it (x64 += x, x then starts again at 0), recomputes the residual
+// r = b - A x64 in FP64 (k_ir_residual_blk) and hands its FP32 rounding to the FP32 PCG, which either
+// continues with it in place of its recursive residual or starts a correction solve A d = r afresh.
 // The fused multigrid CG iteration leaves x += alpha p to the extra workgroups of its V-cycle's bottom
 // launch, which does not run in the iteration that ends the solve; that last term is added here, after
 // the solve: k = st->iter - 1 is the last iteration, alpha_k the state's alpha and p_k the direction
@@ -1069,10 +1069,11 @@ __global__ void k_ir_round(int nc, const double* __restrict__ x64, float* __rest
 
 namespace vare {
 
-// Refinement bookkeeping after k_ir_residual_blk: record the FP64 residual and restart the inner solve
-// on the new right-hand side (its relative tolerance is measured against |r|).
-// The next correction solve only has to close the gap to the target: its relative tolerance is
-// max(inner_tol, 0.5 * outer_tol * |b| / |r|), at most 0.5.
+// Refinement bookkeeping after k_ir_residual_blk (it runs only once the FP32 solve is done): record the
+// FP64 residual, stop at the target, or set up the next FP32 solve, whose relative tolerance is measured
+// against |r| and only has to close the gap to the target: max(inner_tol, 0.8 * outer_tol * |b| / |r|),
+// at most 0.5.  The solve continues (iteration count and the last r.z kept: a reliable update) when
+// cont_allowed and at most max_reduction is left to gain, else it restarts.
 struct k_fin_ir_f {
     static constexpr int kLB = kFinThreads;
     static constexpr int kWPE = 1;

@@ -146,8 +146,8 @@ __device__ __forceinline__ float apply_row(unsigned in, float own, const float* 
 
 // Iterative refinement's FP64 residual r = b - A x64 over the solver's blocks (diag * x first, then
 // the fluid neighbours subtracted in the order -x,+x,-y,+y,-z,+z), handing the FP32 rounding of r to
-// the next FP32 solve (both r buffers, x = 0; p = r when p != nullptr), with one partial |r|^2 per
-// workgroup.  It runs between solves: with done != nullptr only once the solve is done (it is enqueued
+// the next FP32 solve (both r buffers, x = 0; the solve's set-up or its continuation forms the next
+// direction), with one partial |r|^2 per workgroup.  It runs between solves: with done != nullptr only once the solve is done (it is enqueued
 // behind each chunk of iterations).
 template <int B, int K>
 struct k_ir_residual_blk_f {
@@ -158,7 +158,7 @@ struct k_ir_residual_blk_f {
                                                                     const double* __restrict__ x64,
                                                                     const uint16_t* __restrict__ info,
                                                                     float* __restrict__ r32, float* __restrict__ x,
-                                                                    float* __restrict__ p, double* __restrict__ partials,
+                                                                    double* __restrict__ partials,
                                                                     float* __restrict__ r32b, const int* done) {
         constexpr int B3 = Fine<B>::B3;
         __shared__ double sx[K][B3];
@@ -215,7 +215,6 @@ struct k_ir_residual_blk_f {
             r32[c] = rf;
             if (r32b) r32b[c] = rf;
             x[c] = 0.f;
-            if (p) p[c] = rf;
             rr += r * r;
         }
         const double tot = fine_block_sum<B>(rr, sh);
@@ -228,9 +227,9 @@ __global__ void __launch_bounds__(Fine<B>::B3) k_ir_residual_blk(const int* __re
                                                                 const double* __restrict__ x64,
                                                                 const uint16_t* __restrict__ info,
                                                                 float* __restrict__ r32, float* __restrict__ x,
-                                                                float* __restrict__ p, double* __restrict__ partials,
+                                                                double* __restrict__ partials,
                                                                 float* __restrict__ r32b, const int* done) {
-    k_ir_residual_blk_f<B, K>::run(blockIdx.x, gridDim.x, tab, b, x64, info, r32, x, p, partials, r32b, done);
+    k_ir_residual_blk_f<B, K>::run(blockIdx.x, gridDim.x, tab, b, x64, info, r32, x, partials, r32b, done);
 }
 
 // Storage indices of the six neighbours of cell l of block s (cell_neighbor's values: -1 inactive, -2
