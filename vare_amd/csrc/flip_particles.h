@@ -486,14 +486,12 @@ __device__ __forceinline__ void p2g_finish(int c, int i, int j, int k, float m, 
                                            const CellFields& cf, float* __restrict__ cx, float* __restrict__ cy,
                                            float* __restrict__ cz, uint8_t* __restrict__ type, const ColK& col,
                                            const FrameK& f, float col_scale) {
-    float vx = 0.f, vy = 0.f, vz = 0.f;
-    uint8_t ty8 = kAir;
-    if (m > 0.f) {
-        vx = sx / m;
-        vy = sy / m;
-        vz = sz / m;
-        ty8 = kFluid;
-    }
+    // The quotients are formed for every cell and selected (where m = 0 they are not finite and not
+    // kept), so the sums' loads are not moved into a branch taken only by cells with mass.
+    const bool has = m > 0.f;
+    const float qx = sx / m, qy = sy / m, qz = sz / m;
+    float vx = has ? qx : 0.f, vy = has ? qy : 0.f, vz = has ? qz : 0.f;
+    uint8_t ty8 = has ? (uint8_t)kFluid : (uint8_t)kAir;
     if (col.any()) ty8 = classify_cell<B>(col, f, col_scale, i, j, k, m, vx, vy, vz);
     cf.mass[c] = m > 0.f ? m : 0.f;
     cf.vx[c] = vx; cf.vy[c] = vy; cf.vz[c] = vz;
@@ -685,12 +683,26 @@ struct k_p2g_faces_f {
         const int t0 = f.coord[s];
         float m = cf.mass[c], sx = cf.vx[c], sy = cf.vy[c], sz = cf.vz[c], cn = cf.cnt[c];
         const float* in = part + (size_t)s * NP * 5;
+        // The entries of the up to seven blocks below are loaded together, one round trip to memory (an
+        // entry that does not apply to this cell reads entry 0), then added in the order e = 1..7 with
+        // selects, so a skipped entry leaves the sums exactly as they are.
+        float pv[7][5];
+        bool use[7];
     #pragma unroll
         for (int e = 1; e < 8; ++e) {
             const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
-            if ((ex && x) || (ey && y) || (ez && z)) continue;
-            const int q = p2g_layer_index<B>(x + ex * B, y + ey * B, z + ez * B);
-            m += in[q]; sx += in[NP + q]; sy += in[2 * NP + q]; sz += in[3 * NP + q]; cn += in[4 * NP + q];
+            use[e - 1] = !((ex && x) || (ey && y) || (ez && z));
+            const int q = use[e - 1] ? p2g_layer_index<B>(x + ex * B, y + ey * B, z + ez * B) : 0;
+    #pragma unroll
+            for (int v = 0; v < 5; ++v) pv[e - 1][v] = in[v * NP + q];
+        }
+    #pragma unroll
+        for (int e = 0; e < 7; ++e) {
+            m = use[e] ? m + pv[e][0] : m;
+            sx = use[e] ? sx + pv[e][1] : sx;
+            sy = use[e] ? sy + pv[e][2] : sy;
+            sz = use[e] ? sz + pv[e][3] : sz;
+            cn = use[e] ? cn + pv[e][4] : cn;
         }
         const int ox = (t0 % f.fx) * B, oy = ((t0 / f.fx) % f.fy) * B, oz = (t0 / (f.fx * f.fy)) * B;
         p2g_finish<B>(c, ox + x, oy + y, oz + z, m, sx, sy, sz, cn, cf, cx, cy, cz, type, col, f, col_scale);
