@@ -134,8 +134,6 @@ long find_pooled(const vare_server* s, const Request* q) {
 // Runs the request on h (a pooled handle) or on a new handle; returns the handle to pool afterwards
 // (nullptr when it failed and was destroyed).
 vare_flip* run_request(vare_server* s, Request* q, vare_flip* h) {
-    const double admit = since(s->t0);
-    q->st.admit_s = admit;
     int rc = VARE_OK;
     if (h) rc = vare_flip_set_params(h, &q->req.params);
     else rc = vare_flip_create(&q->req.grid, &q->req.params, q->req.n, 0, &h);
@@ -249,6 +247,7 @@ void worker(vare_server* s) {
                     s->handles_created++;
                 }
                 q = head;
+                q->st.admit_s = since(s->t0);     // stamped under the lock, so in admission order
                 q->charge = need;
                 s->next++;
                 s->used = base + need;
