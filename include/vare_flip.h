@@ -380,9 +380,11 @@ int vare_flip_get_kernel_timing(vare_flip* h, double ms_total[VARE_KERNEL_COUNT]
 /* ---- Serving: many independent tank solves, continuously batched on one GPU ----
  * No counterpart in the reference (one Houdini cook drives one FLIPMethod).  A server admits
  * submitted requests in arrival order whenever a slot and enough of the HBM budget are free; each
- * admitted tank gets its own handle and HIP stream, driven by a worker thread, so the steps of all
- * resident tanks overlap on the device; a finished tank releases its slot and memory at once and the
- * next arrived request is admitted (continuous batching). */
+ * admitted tank gets its own handle (a pooled one when one fits), and the resident tanks are stepped
+ * together in rounds of vare_flip_step_batched (every kernel launched once for all of them); with
+ * vare_server_set_batching(s, 0) each tank is stepped on its own stream instead.  A finished tank
+ * releases its slot and memory at once and the next arrived request is admitted (continuous
+ * batching).  A request's admit_s is stamped when it is admitted, so admit_s follows arrival order. */
 typedef struct vare_server vare_server;
 
 typedef struct vare_tank_request {
